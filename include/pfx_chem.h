@@ -1,0 +1,218 @@
+/*
+ * pfx_chem.h -- C-ABI of the MI355X per-cell geochemistry library.
+ *
+ * This is the drop-in boundary for PFLOTRAN's per-cell reaction step.  Every
+ * type is plain C (fixed-width integers, doubles, pointers), so the Fortran
+ * host binds it with ISO_C_BINDING (see pflotran_amd/fortran/pfx_chem_mod.F90).
+ *
+ * Conventions (shared by every entry point):
+ *   - return 0 on success, >0 = number of cells that did not converge
+ *     (details in the per-cell status array), <0 = hard error (PFX_E_*).
+ *     Nothing ever aborts the process (reference: ludcmp calls MPI_Abort,
+ *     src/pflotran/utility.F90:423-432; RReact can `stop`, reaction.F90:3487).
+ *   - species / complex / mineral ids are 0-based (the reference's are
+ *     1-based with count in row 0, reaction_aux.F90:230-231).
+ *   - per-cell vectors are cell-major ("AoS"): value j of cell c lives at
+ *     [c*len + j], exactly the layout of the transport Vec tran_xx
+ *     (reactive_transport.F90:1701-1709).  Jacobian blocks are column-major
+ *     N x N per cell, the layout of the Fortran J(ncomp,ncomp) and of PETSc
+ *     BAIJ blocks with MAT_ROW_ORIENTED=FALSE (discretization.F90:857).
+ *   - all arithmetic is FP64.
+ *   - cell arrays are DEVICE pointers (hipMalloc'd or torch CUDA tensors);
+ *     the call is ordered on `stream` (NULL = default stream) and returns
+ *     after the stream has drained, with the statistics filled in.
+ */
+#ifndef PFX_CHEM_H
+#define PFX_CHEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFX_ABI_VERSION 1
+
+/* largest primary-species count handled by one cell group (row per lane) */
+#define PFX_MAX_PRIMARY 32
+/* largest number of aqueous complexes per table */
+#define PFX_MAX_CPLX 256
+/* largest species count in one complex or mineral reaction (H2O excluded) */
+#define PFX_MAX_RXN_SPEC 8
+#define PFX_MAX_MINERALS 16
+#define PFX_MAX_IONX_RXN 4
+#define PFX_MAX_IONX_CATION 8
+
+/* error codes */
+#define PFX_E_ARG      (-1)   /* bad argument / inconsistent sizes */
+#define PFX_E_DEVICE   (-2)   /* HIP runtime error */
+#define PFX_E_NOMEM    (-3)
+#define PFX_E_UNSUP    (-4)   /* table shape beyond the compiled limits */
+
+/* activity-coefficient update frequency: reaction_aux.F90 ACT_COEF_FREQUENCY_* */
+#define PFX_ACT_OFF          0
+#define PFX_ACT_TIMESTEP     1   /* once per reaction step (LAG), reaction.F90:3407 */
+#define PFX_ACT_NEWTON_ITER  2   /* every Newton iteration, reaction.F90:3415-3418 */
+/* activity-coefficient algorithm: ACT_COEF_ALGORITHM_* */
+#define PFX_ACT_ALG_LAG      0   /* reaction.F90:3992-4051 */
+#define PFX_ACT_ALG_NEWTON   1   /* reaction.F90:3846-3991 */
+
+/* per-cell status */
+#define PFX_CELL_CONVERGED      0
+#define PFX_CELL_MAX_ITERATIONS 1
+#define PFX_CELL_NONFINITE      2
+#define PFX_CELL_SINGULAR       3
+
+/* initial Newton iterate */
+#define PFX_GUESS_PREVIOUS  0   /* previous free ions (RReact's guess, reaction.F90:3370) */
+#define PFX_GUESS_ARRAY     1   /* caller array cells->guess (e.g. learned warm start) */
+#define PFX_GUESS_DEFAULT   2   /* constant opts->default_guess (1e-9 molal, reaction.F90:1481) */
+
+/*
+ * Flattened reaction_type (reaction_aux.F90:142-335, mineral_type
+ * reaction_mineral_aux.F90:77-128).  All pointers are HOST pointers read once
+ * by pfx_tables_create; the library keeps its own device copy.
+ * Row-major 2-D tables with the stated leading dimensions.
+ */
+typedef struct pfx_reaction_desc {
+  int32_t naqcomp;        /* N: primary aqueous species                       */
+  int32_t neqcplx;        /* M: aqueous complexes                             */
+  int32_t nkinmnrl;       /* K: kinetic minerals                              */
+  int32_t neqionxrxn;     /* R: ion-exchange reactions (sites)                */
+  int32_t max_rxn_spec;   /* leading dim of *specid (+1) and *stoich tables   */
+  int32_t max_cation;     /* leading dim of eqionx tables (+1 for specid)     */
+
+  const double *primary_spec_Z;     /* [N]                                    */
+  const double *primary_spec_a0;    /* [N] Debye-Hueckel ion size (Angstrom)  */
+
+  /* aqueous complexes: reaction%eqcplx* */
+  const int32_t *eqcplxspecid;      /* [M][max_rxn_spec+1]; [i][0] = count    */
+  const double *eqcplxstoich;       /* [M][max_rxn_spec]                      */
+  const double *eqcplxh2ostoich;    /* [M] (0 when water absent)              */
+  const double *eqcplx_Z;           /* [M]                                    */
+  const double *eqcplx_a0;          /* [M]                                    */
+  const double *eqcplx_logK;        /* [M] log10 K at the reference temperature */
+  const double *eqcplx_logKcoef;    /* [M][5] Maier-Kelly fit, NULL if unused  */
+
+  /* kinetic minerals: mineral%kinmnrl* */
+  const int32_t *kinmnrlspecid;     /* [K][max_rxn_spec+1]                    */
+  const double *kinmnrlstoich;      /* [K][max_rxn_spec]                      */
+  const double *kinmnrlh2ostoich;   /* [K]                                    */
+  const double *kinmnrl_logK;       /* [K]                                    */
+  const double *kinmnrl_logKcoef;   /* [K][5] or NULL                         */
+  const double *kinmnrl_rate_constant;      /* [K] mol/m^2/s                  */
+  const double *kinmnrl_molar_vol;          /* [K] m^3/mol                    */
+  const double *kinmnrl_activation_energy;  /* [K] J/mol, <=0: none; or NULL  */
+  const double *kinmnrl_affinity_threshold; /* [K] or NULL                    */
+  const double *kinmnrl_rate_limiter;       /* [K] or NULL                    */
+  const double *kinmnrl_affinity_power;     /* [K] or NULL (beta, else 1)     */
+  const double *kinmnrl_Temkin_const;       /* [K] or NULL (sigma, else 1)    */
+
+  /* equilibrium ion exchange: reaction%eqionx_rxn_* */
+  const int32_t *eqionx_rxn_cationid; /* [R][max_cation+1]; [r][0] = count,
+                                         [r][1] = reference cation            */
+  const double *eqionx_rxn_k;         /* [R][max_cation]                      */
+  const double *eqionx_rxn_CEC;       /* [R] eq/m^3 bulk                      */
+  const int32_t *eqionx_rxn_to_surf;  /* [R] kinetic mineral id or -1; NULL = none */
+
+  /* Debye-Hueckel parameters, fixed at the reference temperature
+     (reaction_database.F90:896-990) */
+  double debyeA, debyeB, debyeBdot;
+
+  /* Newton controls (reaction_aux.F90:569-571) */
+  int32_t use_log_formulation;
+  int32_t act_coef_update_frequency;  /* PFX_ACT_*     */
+  int32_t act_coef_update_algorithm;  /* PFX_ACT_ALG_* */
+  int32_t reserved0;
+  double max_dlnC;                    /* 5                         */
+  double max_relative_change_tolerance;  /* 1e-6                   */
+  double max_residual_tolerance;         /* 1e-12                  */
+} pfx_reaction_desc;
+
+/* Per-cell state, device pointers, cell-major.  In = read, Out = written. */
+typedef struct pfx_cells {
+  int64_t ncells;
+  double *total;          /* [ncells*N] In: transported totals tran_xx (mol/L);
+                             Out: totals at the solution                     */
+  double *total_sorb;     /* [ncells*N] In: sorbed totals of the previous state
+                             (mol/m^3 bulk); Out: at the solution. NULL if R==0 */
+  double *pri_molal;      /* [ncells*N] In: previous free ions (molal);
+                             Out: converged free ions                         */
+  double *pri_act_coef;   /* [ncells*N] In/Out; NULL -> 1 (activity off)       */
+  double *sec_act_coef;   /* [ncells*M] In/Out; NULL -> 1                      */
+  double *sec_molal;      /* [ncells*M] Out, may be NULL                       */
+  double *ionx_ref_sorbed;/* [ncells*R] In/Out eqionx_ref_cation_sorbed_conc   */
+  const double *mnrl_volfrac; /* [ncells*K]                                  */
+  const double *mnrl_area;    /* [ncells*K] m^2/m^3 bulk                      */
+  double *mnrl_rate;      /* [ncells*K] Out mol/m^3/s, may be NULL             */
+  const double *temperature;  /* [ncells] deg C; NULL -> table log K (isothermal) */
+  const double *porosity;     /* [ncells] or NULL -> opts default             */
+  const double *saturation;   /* [ncells] or NULL                              */
+  const double *volume;       /* [ncells] m^3 or NULL                          */
+  const double *den_kg;       /* [ncells] kg/m^3 or NULL                       */
+  const double *guess;        /* [ncells*N] used with PFX_GUESS_ARRAY         */
+  int32_t *num_iterations;    /* [ncells] Out, may be NULL                     */
+  int32_t *status;            /* [ncells] Out, may be NULL                     */
+  double *rel_res;            /* [ncells] Out final scaled residual, may be NULL */
+} pfx_cells;
+
+typedef struct pfx_react_opts {
+  double dt;              /* reaction step (s)                                 */
+  double rel_tol;         /* stop when rel_res <= rel_tol (1e-10)             */
+  int32_t max_iterations; /* Newton cap per cell                               */
+  int32_t extra_iterations; /* Newton updates taken after rel_tol is met       */
+  int32_t guess_mode;     /* PFX_GUESS_*                                      */
+  int32_t reference_stop; /* also stop on max|R|<max_residual_tolerance or
+                             max|dm/m|<max_relative_change_tolerance (RReact) */
+  double default_guess;   /* PFX_GUESS_DEFAULT value (molal)                  */
+  double porosity, saturation, volume, den_kg; /* used when arrays are NULL   */
+  void *stream;           /* hipStream_t, NULL = default stream               */
+} pfx_react_opts;
+
+typedef struct pfx_stats {
+  int64_t ncells;
+  int64_t converged;
+  int64_t total_iterations;
+  int32_t max_iterations;
+  int32_t min_iterations;
+  double max_rel_res;
+  double kernel_ms;       /* device time of the batch (HIP events)             */
+  int64_t iteration_histogram[32]; /* [min(it,31)]                             */
+} pfx_stats;
+
+typedef struct pfx_tables pfx_tables;   /* opaque, owns device copies */
+
+/* Library version/ABI check. */
+int pfx_abi_version(void);
+
+/* Flattened reaction_type -> device tables.
+   Replaces the reaction_type consumed by RReact/RTAuxVarCompute
+   (reaction_aux.F90:142-335; setup BasisInit reaction_database.F90:772). */
+int pfx_tables_create(const pfx_reaction_desc *desc, pfx_tables **out);
+int pfx_tables_destroy(pfx_tables *tables);
+
+/* Batched operator-split reaction step: RTReact's cell loop
+   (reactive_transport.F90:1697-1724) around RReact (reaction.F90:3322-3511),
+   solving the dt-consistent per-cell residual of the global-implicit path
+   (reactive_transport.F90:2539-2587, 2735-2758).  One persistent launch. */
+int pfx_react_batch(pfx_tables *tables, pfx_cells *cells,
+                    const pfx_react_opts *opts, pfx_stats *stats);
+
+/* Per-cell residual and analytic Jacobian at the given free ions, without a
+   Newton solve: RTAuxVarCompute + RTAccumulation(Derivative) +
+   RAccumulationSorb(Derivative) + RReaction(derivative=TRUE)
+   (reaction.F90:3419-3441), divided by dt as in RTResidualNonFlux /
+   RTJacobianNonFlux (reactive_transport.F90:2564, 3342-3465).
+   residual: [ncells*N] mol/s; jacobian: [ncells*N*N] column-major blocks
+   (dR/dm, kg water/s).  The activity coefficients in `cells` are used as
+   given (no update).  fixed_accum: [ncells*N] mol, the accumulation of the
+   old state, may be NULL (then computed from cells->total/total_sorb). */
+int pfx_residual_jacobian(pfx_tables *tables, pfx_cells *cells,
+                          const pfx_react_opts *opts,
+                          double *residual, double *jacobian);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFX_CHEM_H */

@@ -1,0 +1,200 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker / CPU baseline, never as
+the product.  Built by oracle/Makefile.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from pflotran_amd import abi
+from pflotran_amd.database import ReactionTables
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+CON_TOTAL, CON_FREE, CON_LOG, CON_PH, CON_CHARGE, CON_TOTAL_SORB = 0, 1, 2, 3, 4, 5
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        lib = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        vp = C.c_void_p
+        lib.or_react_batch.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.ReactOpts), P(abi.Stats), C.c_int]
+        lib.or_residual_jacobian.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.ReactOpts), vp, vp, vp, C.c_int]
+        lib.or_equilibrate_batch.argtypes = [P(abi.ReactionDesc), vp, vp, P(abi.Cells), P(abi.ReactOpts), C.c_int,
+                                             vp, C.c_int]
+        lib.or_auxvar_batch.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.ReactOpts), C.c_int, C.c_int]
+        lib.or_update_kinetic_state.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.ReactOpts), vp, C.c_int]
+        lib.or_rtotal_cell.argtypes = [P(abi.ReactionDesc), vp, vp, vp, vp, C.c_double, vp, vp, vp]
+        lib.or_activity_cell.argtypes = [P(abi.ReactionDesc), vp, vp, vp, vp, vp]
+        lib.or_ionx_cell.argtypes = [P(abi.ReactionDesc), vp, vp, vp, vp, vp, vp]
+        lib.or_ludcmp_status.argtypes = [vp, C.c_int, vp, vp]
+        lib.or_ludcmp_status.restype = C.c_int
+        lib.or_lubksb.argtypes = [vp, C.c_int, vp, vp]
+        lib.or_logk_fit.argtypes = [vp, vp, C.c_int, vp]
+        lib.or_logk_eval.argtypes = [vp, C.c_double]
+        lib.or_logk_eval.restype = C.c_double
+        lib.or_density_ifc67.argtypes = [C.c_double, C.c_double]
+        lib.or_density_ifc67.restype = C.c_double
+        lib.or_debye_params.argtypes = [C.c_double, C.c_int, vp, vp, vp]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+@dataclass
+class HostCells:
+    """numpy arrays for every pfx_cells field (cell-major)."""
+    arrays: Dict[str, Optional[np.ndarray]]
+    ncells: int
+
+    @classmethod
+    def allocate(cls, t: ReactionTables, n: int, sorb=None, temperature=False, per_cell_scalars=False):
+        N, M, K, R = t.N, t.M, t.K, t.R
+        z = lambda *s: np.zeros(s, dtype=np.float64)  # noqa: E731
+        a = {
+            "total": z(n, N), "total_sorb": z(n, N) if (R > 0 if sorb is None else sorb) else None,
+            "pri_molal": np.full((n, N), 1e-9), "pri_act_coef": np.ones((n, N)),
+            "sec_act_coef": np.ones((n, max(M, 1))), "sec_molal": z(n, max(M, 1)),
+            "ionx_ref_sorbed": np.full((n, max(R, 1)), 1e-9), "mnrl_volfrac": z(n, max(K, 1)),
+            "mnrl_area": z(n, max(K, 1)), "mnrl_rate": z(n, max(K, 1)),
+            "temperature": z(n) + 25.0 if temperature else None,
+            "porosity": z(n) if per_cell_scalars else None, "saturation": z(n) if per_cell_scalars else None,
+            "volume": z(n) if per_cell_scalars else None, "den_kg": z(n) if per_cell_scalars else None,
+            "guess": z(n, N), "num_iterations": np.zeros(n, dtype=np.int32),
+            "status": np.zeros(n, dtype=np.int32), "rel_res": z(n),
+        }
+        return cls(a, n)
+
+    def struct(self) -> abi.Cells:
+        c = abi.Cells()
+        c.ncells = self.ncells
+        for f in abi.CELL_FIELDS:
+            setattr(c, f, _ptr(self.arrays.get(f)))
+        return c
+
+    def copy(self) -> "HostCells":
+        return HostCells({k: (None if v is None else v.copy()) for k, v in self.arrays.items()}, self.ncells)
+
+    def __getitem__(self, k):
+        return self.arrays[k]
+
+
+def react_batch(t: ReactionTables, cells: HostCells, opts: abi.ReactOpts, nthreads: int = 0):
+    lib = load()
+    d = t.desc()
+    cs = cells.struct()
+    st = abi.Stats()
+    rc = lib.or_react_batch(C.byref(d), C.byref(cs), C.byref(opts), C.byref(st), nthreads)
+    return rc, st
+
+
+def residual_jacobian(t: ReactionTables, cells: HostCells, opts: abi.ReactOpts, fixed_accum=None, nthreads=0):
+    lib = load()
+    d = t.desc()
+    cs = cells.struct()
+    n, N = cells.ncells, t.N
+    res = np.zeros((n, N))
+    jac = np.zeros((n, N, N))  # [cell][col][row] memory = column-major block
+    rc = lib.or_residual_jacobian(C.byref(d), C.byref(cs), C.byref(opts), _ptr(fixed_accum), _ptr(res),
+                                  _ptr(jac), nthreads)
+    assert rc == 0
+    return res, np.ascontiguousarray(jac.transpose(0, 2, 1))  # -> [cell][row][col]
+
+
+def equilibrate(t: ReactionTables, ctype: Sequence[int], conc: np.ndarray, cells: HostCells,
+                opts: abi.ReactOpts, molal=False, nthreads=0):
+    lib = load()
+    d = t.desc()
+    ct = np.ascontiguousarray(ctype, dtype=np.int32)
+    cc = np.ascontiguousarray(conc, dtype=np.float64).reshape(cells.ncells, t.N)
+    its = np.zeros(cells.ncells, dtype=np.int32)
+    cs = cells.struct()
+    rc = lib.or_equilibrate_batch(C.byref(d), _ptr(ct), _ptr(cc), C.byref(cs), C.byref(opts), int(molal),
+                                  _ptr(its), nthreads)
+    return rc, its
+
+
+def auxvar(t: ReactionTables, cells: HostCells, opts: abi.ReactOpts, update_activity=False, nthreads=0):
+    lib = load()
+    d = t.desc()
+    cs = cells.struct()
+    lib.or_auxvar_batch(C.byref(d), C.byref(cs), C.byref(opts), int(update_activity), nthreads)
+
+
+def update_kinetic_state(t: ReactionTables, cells: HostCells, opts: abi.ReactOpts, nthreads=0):
+    lib = load()
+    d = t.desc()
+    cs = cells.struct()
+    vf = np.ascontiguousarray(cells["mnrl_volfrac"])
+    lib.or_update_kinetic_state(C.byref(d), C.byref(cs), C.byref(opts), _ptr(vf), nthreads)
+    cells.arrays["mnrl_volfrac"] = vf
+
+
+def batch_steps(t: ReactionTables, cells: HostCells, opts: abi.ReactOpts, dts: Sequence[float], nthreads=1):
+    """Global-implicit batch run without transport: per step the reference
+    refreshes the fixed accumulation from the current state
+    (RTUpdateFixedAccumulation, reactive_transport.F90:726-848), then updates
+    activity coefficients (PMRTInitializeTimestep, pm_rt.F90:410-413), solves
+    the per-cell residual, and updates mineral volume fractions
+    (RUpdateKineticState, reaction.F90:5340-5366).  The per-cell solve is the
+    same system as the operator-split RReact with dt-consistent units."""
+    its = []
+    for dt in dts:
+        opts.dt = dt
+        auxvar(t, cells, opts, update_activity=False, nthreads=nthreads)   # totals at the old state
+        rc, st = react_batch(t, cells, opts, nthreads=nthreads)
+        if rc != 0:
+            raise RuntimeError(f"oracle react failed rc={rc} stats={st.as_dict()}")
+        its.append(st.total_iterations)
+        update_kinetic_state(t, cells, opts, nthreads=nthreads)
+    auxvar(t, cells, opts, update_activity=False, nthreads=nthreads)
+    return its
+
+
+def lu_solve(A: np.ndarray, b: np.ndarray):
+    """ludcmp/lubksb on a copy (column-major as the reference)."""
+    lib = load()
+    n = A.shape[0]
+    a = np.asfortranarray(A.astype(np.float64)).copy(order="F")
+    indx = np.zeros(n, dtype=np.int32)
+    dd = np.zeros(1, dtype=np.int32)
+    st = lib.or_ludcmp_status(a.ctypes.data, n, indx.ctypes.data, dd.ctypes.data)
+    x = np.ascontiguousarray(b, dtype=np.float64).copy()
+    if st == 0:
+        lib.or_lubksb(a.ctypes.data, n, indx.ctypes.data, x.ctypes.data)
+    return st, x, indx, a
+
+
+def density_ifc67(t=25.0, p=101325.0) -> float:
+    return load().or_density_ifc67(t, p)
+
+
+def debye(tref=25.0, use_bdot=True):
+    A, B, D = (np.zeros(1) for _ in range(3))
+    load().or_debye_params(tref, int(use_bdot), A.ctypes.data, B.ctypes.data, D.ctypes.data)
+    return float(A[0]), float(B[0]), float(D[0])
+
+
+def logk_fit(temps, logK):
+    t = np.ascontiguousarray(temps, dtype=np.float64)
+    k = np.ascontiguousarray(logK, dtype=np.float64)
+    c = np.zeros(5)
+    load().or_logk_fit(t.ctypes.data, k.ctypes.data, len(t), c.ctypes.data)
+    return c

@@ -1,0 +1,1128 @@
+/*
+ * pfx_oracle.c -- CPU FP64 oracle for the per-cell reaction step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or as the timed CPU baseline).  The product (pflotran_amd/libpfx_chem.so)
+ * never links, loads or calls it.
+ *
+ * A plain-C restatement of the reference Fortran, function by function, in
+ * the reference's operation order (citations are to src/pflotran/ of the
+ * reference tree).  The reference itself cannot be built here (it needs PETSc
+ * headers/libraries and MPI, which the image lacks), so this restatement is
+ * pinned instead by the reference's regression golds (tests/golden/), by
+ * closed forms and invariants, and by brute force on small systems; see the
+ * tests/test_oracle_*.py files for which check pins which function.
+ *
+ * Pinning status per function (checks live in tests/):
+ *   or_rtotal            golds (carbonate/ca-carbonate speciation), complex-step dT/dm
+ *   or_activity_lag      golds (gamma H+, HCO3-, Ca++), closed form
+ *   or_activity_newton   closed-form fixed point of I (self-consistency)
+ *   or_ionx              gold (ion-exchange-valocchi sorbed totals), charge invariant,
+ *                        equal-valence closed form
+ *   or_kinetic_mineral   golds (calcite-kinetics rate, pH, VF), finite-difference Jacobian
+ *   or_ludcmp/or_lubksb  long-double Gaussian elimination, pivot-order tie test
+ *   or_react_cell        1-cell GI golds (the same per-cell system), brute force
+ *   or_equilibrate_cell  equilibrium golds (pH, totals, gamma, sorbed)
+ *   or_logk_fit          reproduces tabulated log K; 25 C special case
+ *   or_density_ifc67     steam-table value at 25 C (parity unpinned beyond the
+ *                        golds that depend on it)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/pfx_chem.h"
+
+/* pflotran_constants.F90:33,40-53 (truncated conversion kept on purpose) */
+#define OR_LOG_TO_LN 2.30258509299
+#define OR_IDEAL_GAS 8.31446
+#define OR_FMWH2O 18.01534
+#define OR_H2O_TC 647.3
+#define OR_H2O_PC 22.064e6
+
+#define NMAX PFX_MAX_PRIMARY
+#define MMAX PFX_MAX_CPLX
+#define KMAX PFX_MAX_MINERALS
+#define RMAX PFX_MAX_IONX_RXN
+#define CMAX PFX_MAX_IONX_CATION
+
+/* ------------------------------------------------------------------ */
+/* per-cell working state (mirrors reactive_transport_auxvar_type,
+   reactive_transport_aux.F90:18-73, for the fields on the path)        */
+typedef struct {
+  int N, M, K, R;
+  double m[NMAX];               /* pri_molal */
+  double gam[NMAX];             /* pri_act_coef */
+  double sec[MMAX];             /* sec_molal */
+  double gsec[MMAX];            /* sec_act_coef */
+  double total[NMAX];           /* total(:,1) mol/L */
+  double dtotal[NMAX][NMAX];    /* aqueous%dtotal(i,j,1) */
+  double sorb[NMAX];            /* total_sorb_eq */
+  double dsorb[NMAX][NMAX];     /* dtotal_sorb_eq */
+  double ionx_ref[RMAX];        /* eqionx_ref_cation_sorbed_conc */
+  double vf[KMAX], area[KMAX], rate[KMAX];
+  double ln_act_h2o;
+  double temp, porosity, sat, volume, den;
+  double logK[MMAX], logKm[KMAX];   /* per-cell (temperature dependent) */
+  int status;
+} ocell;
+
+/* ------------------------------------------------------------------ */
+/* utility.F90:677-701 Interpolate */
+static double or_interp(double xh, double xl, double x, double yh, double yl) {
+  if (fabs(xh - xl) < 1e-40) return yh;
+  return yl + (x - xl) / (xh - xl) * (yh - yl);
+}
+
+/* reaction_database.F90:896-990: Debye-Hueckel A, B, Bdot at the reference
+   temperature by piecewise-linear interpolation of fixed tables. */
+void or_debye_params(double tref, int use_bdot, double *A, double *B, double *Bdot) {
+  static const double T[] = {0, 25, 60, 100, 150, 200, 250, 300, 350};
+  static const double a[] = {0.4939, 0.5114, 0.5465, 0.5995, 0.6855, 0.7994, 0.9593, 1.2180, 1.2180};
+  static const double b[] = {0.3253, 0.3288, 0.3346, 0.3421, 0.3525, 0.3639, 0.3766, 0.3925, 0.3925};
+  static const double d[] = {0.0374, 0.0410, 0.0440, 0.0460, 0.0470, 0.0470, 0.0340, 0.0000, 0.0000};
+  if (tref <= 0.01) { *A = a[0]; *B = b[0]; *Bdot = d[0]; }
+  else if (tref > 350.0) { *A = a[8]; *B = b[8]; *Bdot = d[8]; }
+  else {
+    int k = 1;
+    while (k < 8 && tref > T[k]) k++;
+    *A = or_interp(T[k], T[k - 1], tref, a[k], a[k - 1]);
+    *B = or_interp(T[k], T[k - 1], tref, b[k], b[k - 1]);
+    *Bdot = or_interp(T[k], T[k - 1], tref, d[k], d[k - 1]);
+  }
+  if (!use_bdot) *Bdot = 0.0;
+}
+
+/* eos_water.F90:876-940 EOSWaterDensityIFC67, density only (kg/m^3). */
+double or_density_ifc67(double t, double p) {
+  static const double aa[23] = {
+      6.824687741e03, -5.422063673e02, -2.096666205e04, 3.941286787e04,
+      -6.733277739e04, 9.902381028e04, -1.093911774e05, 8.590841667e04,
+      -4.511168742e04, 1.418138926e04, -2.017271113e03, 7.982692717e00,
+      -2.616571843e-2, 1.522411790e-3, 2.284279054e-2, 2.421647003e02,
+      1.269716088e-10, 2.074838328e-7, 2.174020350e-8, 1.105710498e-9,
+      1.293441934e01, 1.308119072e-5, 6.047626338e-14};
+  const double a1 = 8.438375405e-1, a2 = 5.362162162e-4, a3 = 1.720000000e00,
+               a4 = 7.342278489e-2, a5 = 4.975858870e-2, a6 = 6.537154300e-1,
+               a7 = 1.150000000e-6, a8 = 1.510800000e-5, a9 = 1.418800000e-1,
+               a10 = 7.002753165e00, a11 = 2.995284926e-4, a12 = 2.040000000e-1;
+  const double vc1 = 0.00317;
+  double theta = (t + 273.15) * (1.0 / OR_H2O_TC);
+  double theta2x = theta * theta;
+  double theta18 = pow(theta, 18.0);
+  double theta20 = theta18 * theta2x;
+  double beta = p * (1.0 / OR_H2O_PC);
+  double beta2x = beta * beta;
+  double yy = 1.0 - a1 * theta2x - a2 * pow(theta, -6.0);
+  double xx = a3 * yy * yy - 2.0 * (a4 * theta - a5 * beta);
+  xx = xx > 0.0 ? sqrt(xx) : 1e-6;
+  double zz = yy + xx;
+  double u0 = -5.0 / 17.0;
+  double u1 = aa[11] * a5 * pow(zz, u0);
+  double u2 = 1.0 / (a8 + pow(theta, 11.0));
+  double u3 = aa[17] + (2.0 * aa[18] + 3.0 * aa[19] * beta) * beta;
+  double u4 = 1.0 / (a7 + theta18 * theta);
+  double u5 = pow(a10 + beta, -4.0);
+  double u6 = a11 - 3.0 * u5;
+  double u7 = aa[20] * theta18 * (a9 + theta2x);
+  double u8 = aa[15] * pow(a6 - theta, 9.0);
+  double vr = u1 + aa[12] + theta * (aa[13] + aa[14] * theta) + u8 * (a6 - theta) +
+              aa[16] * u4 - u2 * u3 - u6 * u7 +
+              (3.0 * aa[21] * (a12 - theta) + 4.0 * aa[22] * beta / theta20) * beta2x;
+  return 1.0 / (vr * vc1);
+}
+
+/* reaction_aux.F90:1336-1408 ReactionFitLogKCoef: least-squares fit of the
+   5-term Maier-Kelly form over the database temperatures, entries equal to
+   500 are missing; normal equations solved with ludcmp/lubksb. */
+void or_ludcmp(double *A, int n, int *indx, int *d); /* column-major */
+void or_lubksb(const double *A, int n, const int *indx, double *b);
+
+void or_logk_fit(const double *temps, const double *logK, int ntemp, double coefs[5]) {
+  double vec[5][64];
+  int use[64];
+  for (int i = 0; i < ntemp; i++) {
+    double tk = temps[i] + 273.15;
+    vec[0][i] = log(tk);
+    vec[1][i] = 1.0;
+    vec[2][i] = tk;
+    vec[3][i] = 1.0 / tk;
+    vec[4][i] = 1.0 / (tk * tk);
+  }
+  for (int j = 0; j < 5; j++) {
+    coefs[j] = 0.0;
+    for (int i = 0; i < ntemp; i++) {
+      if (fabs(logK[i] - 500.0) < 1e-10) {
+        use[i] = 0;
+      } else {
+        coefs[j] += vec[j][i] * logK[i];
+        use[i] = 1;
+      }
+    }
+  }
+  double a[25]; /* column-major a(j,k) */
+  for (int j = 0; j < 5; j++)
+    for (int k = j; k < 5; k++) {
+      double s = 0.0;
+      for (int i = 0; i < ntemp; i++)
+        if (use[i]) s += vec[j][i] * vec[k][i];
+      a[j + 5 * k] = s;
+      a[k + 5 * j] = s;
+    }
+  int indx[5], dd;
+  or_ludcmp(a, 5, indx, &dd);
+  or_lubksb(a, 5, indx, coefs);
+}
+
+/* reaction_aux.F90:1461-1488 ReactionInterpolateLogK */
+double or_logk_eval(const double c[5], double temp) {
+  double tk = temp + 273.15;
+  return c[0] * log(tk) + c[1] + c[2] * tk + c[3] / tk + c[4] / (tk * tk);
+}
+
+/* ------------------------------------------------------------------ */
+/* utility.F90:393-476 ludcmp: Crout LU with implicit partial pivoting,
+   column-major A(n,n).  Returns 0, or the 1-based singular row. */
+static int or_ludcmp_impl(double *A, int n, int *indx, int *d) {
+#define AA(i, j) A[(i) + (size_t)(j) * n]
+  double vv[64];
+  int imax = 0;
+  *d = 1;
+  for (int i = 0; i < n; i++) {
+    double aamax = 0.0;
+    for (int j = 0; j < n; j++)
+      if (fabs(AA(i, j)) > aamax) aamax = fabs(AA(i, j));
+    if (aamax <= 0.0) return i + 1; /* reference: MPI_Abort */
+    vv[i] = 1.0 / aamax;
+  }
+  for (int j = 0; j < n; j++) {
+    for (int i = 0; i < j; i++) {
+      double sum = AA(i, j);
+      for (int k = 0; k < i; k++) sum -= AA(i, k) * AA(k, j);
+      AA(i, j) = sum;
+    }
+    double aamax = 0.0;
+    for (int i = j; i < n; i++) {
+      double sum = AA(i, j);
+      for (int k = 0; k < j; k++) sum -= AA(i, k) * AA(k, j);
+      AA(i, j) = sum;
+      double dum = vv[i] * fabs(sum);
+      if (dum >= aamax) { /* ties: the later row wins (utility.F90:451) */
+        imax = i;
+        aamax = dum;
+      }
+    }
+    if (j != imax) {
+      for (int k = 0; k < n; k++) {
+        double dum = AA(imax, k);
+        AA(imax, k) = AA(j, k);
+        AA(j, k) = dum;
+      }
+      *d = -*d;
+      vv[imax] = vv[j];
+    }
+    indx[j] = imax;
+    if (AA(j, j) == 0.0) AA(j, j) = 1.0e-20;
+    if (j != n - 1) {
+      double dum = 1.0 / AA(j, j);
+      for (int i = j + 1; i < n; i++) AA(i, j) *= dum;
+    }
+  }
+  return 0;
+#undef AA
+}
+
+void or_ludcmp(double *A, int n, int *indx, int *d) { (void)or_ludcmp_impl(A, n, indx, d); }
+int or_ludcmp_status(double *A, int n, int *indx, int *d) { return or_ludcmp_impl(A, n, indx, d); }
+
+/* utility.F90:480-523 lubksb (forward substitution skips leading zeros) */
+void or_lubksb(const double *A, int n, const int *indx, double *b) {
+#define AA(i, j) A[(i) + (size_t)(j) * n]
+  int ii = -1;
+  for (int i = 0; i < n; i++) {
+    int ll = indx[i];
+    double sum = b[ll];
+    b[ll] = b[i];
+    if (ii >= 0) {
+      for (int j = ii; j < i; j++) sum -= AA(i, j) * b[j];
+    } else if (sum != 0.0) {
+      ii = i;
+    }
+    b[i] = sum;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double sum = b[i];
+    for (int j = i + 1; j < n; j++) sum -= AA(i, j) * b[j];
+    b[i] = sum / AA(i, i);
+  }
+#undef AA
+}
+
+/* ------------------------------------------------------------------ */
+/* table access helpers */
+#define CPLX_N(d, i) ((d)->eqcplxspecid[(size_t)(i) * ((d)->max_rxn_spec + 1)])
+#define CPLX_ID(d, i, s) ((d)->eqcplxspecid[(size_t)(i) * ((d)->max_rxn_spec + 1) + 1 + (s)])
+#define CPLX_NU(d, i, s) ((d)->eqcplxstoich[(size_t)(i) * (d)->max_rxn_spec + (s)])
+#define MNRL_N(d, i) ((d)->kinmnrlspecid[(size_t)(i) * ((d)->max_rxn_spec + 1)])
+#define MNRL_ID(d, i, s) ((d)->kinmnrlspecid[(size_t)(i) * ((d)->max_rxn_spec + 1) + 1 + (s)])
+#define MNRL_NU(d, i, s) ((d)->kinmnrlstoich[(size_t)(i) * (d)->max_rxn_spec + (s)])
+#define IONX_N(d, r) ((d)->eqionx_rxn_cationid[(size_t)(r) * ((d)->max_cation + 1)])
+#define IONX_ID(d, r, j) ((d)->eqionx_rxn_cationid[(size_t)(r) * ((d)->max_cation + 1) + 1 + (j)])
+#define IONX_K(d, r, j) ((d)->eqionx_rxn_k[(size_t)(r) * (d)->max_cation + (j)])
+
+/* reaction.F90:3812-4053 RActivityCoefficients. Water activity is not
+   modelled (ln_act_h2o stays 0, the reference default use_activity_h2o=F). */
+void or_activity(const pfx_reaction_desc *d, ocell *c) {
+  const int N = d->naqcomp, M = d->neqcplx;
+  const double A = d->debyeA, B = d->debyeB, Bd = d->debyeBdot;
+  if (d->act_coef_update_algorithm == PFX_ACT_ALG_NEWTON) {
+    /* reaction.F90:3846-3991 */
+    double ln_act[NMAX];
+    for (int j = 0; j < N; j++) ln_act[j] = log(c->m[j]) + log(c->gam[j]);
+    double fpri = 0.0;
+    for (int j = 0; j < N; j++) fpri += c->m[j] * d->primary_spec_Z[j] * d->primary_spec_Z[j];
+    int it = 0;
+    double II = 0.0;
+    for (;;) {
+      it++;
+      if (it > 50) { c->status = PFX_CELL_NONFINITE; return; } /* reference: NaN poisoning */
+      double I = fpri;
+      for (int i = 0; i < M; i++) I += c->sec[i] * d->eqcplx_Z[i] * d->eqcplx_Z[i];
+      I = 0.5 * I;
+      double f = I;
+      if (fabs(I - II) < 1.0e-6 * I) break;
+      if (M > 0) {
+        double didi = 0.0, sqrt_I = sqrt(I);
+        for (int i = 0; i < M; i++) {
+          double zi = d->eqcplx_Z[i];
+          if (fabs(zi) > 0.0) {
+            double t = 1.0 + B * d->eqcplx_a0[i] * sqrt_I;
+            double sum = 0.5 * A * zi * zi / (sqrt_I * t * t) - Bd;
+            for (int s = 0; s < CPLX_N(d, i); s++) {
+              int j = CPLX_ID(d, i, s);
+              double zj = d->primary_spec_Z[j];
+              if (fabs(zj) > 0.0) {
+                double tj = 1.0 + B * d->primary_spec_a0[j] * sqrt_I;
+                double dgamdi = -0.5 * A * zj * zj / (sqrt_I * tj * tj) + Bd;
+                sum += CPLX_NU(d, i, s) * dgamdi;
+              }
+            }
+            double dcdi = c->sec[i] * OR_LOG_TO_LN * sum;
+            didi += 0.5 * zi * zi * dcdi;
+          }
+        }
+        double den = 1.0 - didi;
+        II = fabs(den) > 0.0 ? (f - I * didi) / den : f;
+      } else {
+        II = f;
+      }
+      I = II;
+      double sqrt_I = sqrt(I);
+      for (int j = 0; j < N; j++) {
+        double z = d->primary_spec_Z[j];
+        c->gam[j] = fabs(z) > 0.0
+                        ? exp((-z * z * sqrt_I * A / (1.0 + d->primary_spec_a0[j] * B * sqrt_I) + Bd * I) * OR_LOG_TO_LN)
+                        : 1.0;
+      }
+      for (int i = 0; i < M; i++) {
+        double z = d->eqcplx_Z[i];
+        c->gsec[i] = fabs(z) > 0.0
+                         ? exp((-z * z * sqrt_I * A / (1.0 + d->eqcplx_a0[i] * B * sqrt_I) + Bd * I) * OR_LOG_TO_LN)
+                         : 1.0;
+        double lnQK = -c->logK[i] * OR_LOG_TO_LN;
+        lnQK += d->eqcplxh2ostoich[i] * c->ln_act_h2o;
+        for (int s = 0; s < CPLX_N(d, i); s++) lnQK += CPLX_NU(d, i, s) * ln_act[CPLX_ID(d, i, s)];
+        c->sec[i] = exp(lnQK) / c->gsec[i];
+      }
+    }
+    return;
+  }
+  /* LAG: reaction.F90:3992-4051 */
+  double I = 0.0;
+  for (int j = 0; j < N; j++) I += c->m[j] * d->primary_spec_Z[j] * d->primary_spec_Z[j];
+  for (int i = 0; i < M; i++) I += c->sec[i] * d->eqcplx_Z[i] * d->eqcplx_Z[i];
+  I = 0.5 * I;
+  double sqrt_I = sqrt(I);
+  for (int j = 0; j < N; j++) {
+    double z = d->primary_spec_Z[j];
+    c->gam[j] = fabs(z) > 1e-10
+                    ? exp((-z * z * sqrt_I * A / (1.0 + d->primary_spec_a0[j] * B * sqrt_I) + Bd * I) * OR_LOG_TO_LN)
+                    : 1.0;
+  }
+  for (int i = 0; i < M; i++) {
+    double z = d->eqcplx_Z[i];
+    c->gsec[i] = fabs(z) > 1e-10
+                     ? exp((-z * z * sqrt_I * A / (1.0 + d->eqcplx_a0[i] * B * sqrt_I) + Bd * I) * OR_LOG_TO_LN)
+                     : 1.0;
+  }
+}
+
+/* reaction.F90:4057-4158 RTotal: mass action, totals (mol/L) and dT/dm. */
+void or_rtotal(const pfx_reaction_desc *d, ocell *c) {
+  const int N = d->naqcomp, M = d->neqcplx;
+  double ln_conc[NMAX], ln_act[NMAX];
+  double den_kg_per_L = c->den * 1.0e-3;
+  for (int j = 0; j < N; j++) {
+    ln_conc[j] = log(c->m[j]);
+    ln_act[j] = ln_conc[j] + log(c->gam[j]);
+    c->total[j] = c->m[j];
+    for (int k = 0; k < N; k++) c->dtotal[j][k] = 0.0;
+    c->dtotal[j][j] = 1.0;
+  }
+  for (int i = 0; i < M; i++) {
+    double lnQK = -c->logK[i] * OR_LOG_TO_LN;
+    if (d->eqcplxh2ostoich[i] != 0.0) lnQK += d->eqcplxh2ostoich[i] * c->ln_act_h2o;
+    const int ns = CPLX_N(d, i);
+    for (int s = 0; s < ns; s++) lnQK += CPLX_NU(d, i, s) * ln_act[CPLX_ID(d, i, s)];
+    c->sec[i] = exp(lnQK) / c->gsec[i];
+    for (int s = 0; s < ns; s++) c->total[CPLX_ID(d, i, s)] += CPLX_NU(d, i, s) * c->sec[i];
+    for (int t = 0; t < ns; t++) {
+      int jc = CPLX_ID(d, i, t);
+      double tmp = CPLX_NU(d, i, t) * exp(lnQK - ln_conc[jc]) / c->gsec[i];
+      for (int s = 0; s < ns; s++) c->dtotal[CPLX_ID(d, i, s)][jc] += CPLX_NU(d, i, s) * tmp;
+    }
+  }
+  for (int j = 0; j < N; j++) {
+    c->total[j] *= den_kg_per_L;
+    for (int k = 0; k < N; k++) c->dtotal[j][k] *= den_kg_per_L;
+  }
+}
+
+/* reaction.F90:4305-4535 RTotalSorbEqIonx (the `#if 1` KDj Newton branch). */
+void or_ionx(const pfx_reaction_desc *d, ocell *c) {
+  const int N = d->naqcomp;
+  for (int j = 0; j < N; j++) {
+    c->sorb[j] = 0.0;
+    for (int k = 0; k < N; k++) c->dsorb[j][k] = 0.0;
+  }
+  for (int r = 0; r < d->neqionxrxn; r++) {
+    const int nc = IONX_N(d, r);
+    double X[CMAX];
+    double omega = d->eqionx_rxn_CEC[r];
+    if (d->eqionx_rxn_to_surf && d->eqionx_rxn_to_surf[r] >= 0)
+      omega = fmax(d->eqionx_rxn_CEC[r] * c->vf[d->eqionx_rxn_to_surf[r]], 1e-40);
+    int zflag = 0;
+    for (int i = 0; i < nc && !zflag; i++)
+      for (int j = 0; j < nc; j++)
+        if (fabs(d->primary_spec_Z[IONX_ID(d, r, i)] - d->primary_spec_Z[IONX_ID(d, r, j)]) > 0.1) {
+          zflag = 1;
+          break;
+        }
+    if (zflag) {
+      int ic = IONX_ID(d, r, 0);
+      double ref_conc = c->m[ic] * c->gam[ic];
+      double ref_Z = d->primary_spec_Z[ic];
+      double ref_k = IONX_K(d, r, 0);
+      double ref_X = ref_Z * c->ionx_ref[r] / omega;
+      int one_more = 0, it = 0;
+      double KDj = ref_X / (ref_k * ref_conc);
+      for (;;) {
+        it++;
+        if (it > 20000) { c->status = PFX_CELL_MAX_ITERATIONS; break; }
+        ref_X = KDj * (ref_k * ref_conc);
+        X[0] = ref_X;
+        double total = ref_X, dres = 0.0;
+        for (int j = 1; j < nc; j++) {
+          int jc = IONX_ID(d, r, j);
+          X[j] = IONX_K(d, r, j) * c->m[jc] * c->gam[jc] * pow(KDj, d->primary_spec_Z[jc] / ref_Z);
+          total += X[j];
+          dres += X[j] / KDj * d->primary_spec_Z[jc];
+        }
+        dres = dres / ref_Z + (ref_k * ref_conc);
+        double res = 1.0 - total;
+        if (one_more) break;
+        double delta = res / dres;
+        KDj = KDj + delta;
+        KDj = fmax(KDj, 1e-40);
+        if (fabs(delta / KDj) < 1e-12) one_more = 1;
+      }
+      c->ionx_ref[r] = ref_X * omega / ref_Z;
+    } else {
+      double sumkm = 0.0;
+      for (int j = 0; j < nc; j++) {
+        int jc = IONX_ID(d, r, j);
+        X[j] = c->m[jc] * c->gam[jc] * IONX_K(d, r, j);
+        sumkm += X[j];
+      }
+      for (int j = 0; j < nc; j++) X[j] /= sumkm;
+    }
+    double sumZX = 0.0;
+    for (int i = 0; i < nc; i++) sumZX += d->primary_spec_Z[IONX_ID(d, r, i)] * X[i];
+    for (int i = 0; i < nc; i++) {
+      int ic = IONX_ID(d, r, i);
+      double t1 = X[i] * omega / d->primary_spec_Z[ic];
+      c->sorb[ic] += t1;
+      double t2 = d->primary_spec_Z[ic] / sumZX;
+      for (int j = 0; j < nc; j++) {
+        int jc = IONX_ID(d, r, j);
+        if (i == j)
+          c->dsorb[ic][jc] += t1 * (1.0 - (t2 * X[j])) / c->m[jc];
+        else
+          c->dsorb[ic][jc] += (-t1) * t2 * X[j] / c->m[jc];
+      }
+    }
+  }
+}
+
+/* reaction_mineral.F90:564-1000 RKineticMineral (no prefactor species).
+   Res (mol/s) and Jac (kg water/s) accumulate; rates stored per m^3 bulk. */
+void or_kinetic_mineral(const pfx_reaction_desc *d, ocell *c, double *Res, double Jac[NMAX][NMAX], int deriv) {
+  const int N = d->naqcomp;
+  double ln_conc[NMAX], ln_act[NMAX];
+  for (int j = 0; j < N; j++) {
+    ln_conc[j] = log(c->m[j]);
+    ln_act[j] = ln_conc[j] + log(c->gam[j]);
+  }
+  for (int k = 0; k < d->nkinmnrl; k++) c->rate[k] = 0.0;
+  for (int k = 0; k < d->nkinmnrl; k++) {
+    double lnQK = -c->logKm[k] * OR_LOG_TO_LN;
+    if (d->kinmnrlh2ostoich[k] != 0.0) lnQK += d->kinmnrlh2ostoich[k] * c->ln_act_h2o;
+    const int ns = MNRL_N(d, k);
+    for (int s = 0; s < ns; s++) lnQK += MNRL_NU(d, k, s) * ln_act[MNRL_ID(d, k, s)];
+    double QK = lnQK <= 6.90776 ? exp(lnQK) : 1.0e3;
+    double temkin = d->kinmnrl_Temkin_const ? d->kinmnrl_Temkin_const[k] : 0.0;
+    double aff = d->kinmnrl_Temkin_const ? 1.0 - pow(QK, 1.0 / temkin) : 1.0 - QK;
+    double sgn = copysign(1.0, aff);
+    if (!(c->vf[k] > 0.0 || sgn < 0.0)) continue;
+    double thr = d->kinmnrl_affinity_threshold ? d->kinmnrl_affinity_threshold[k] : 0.0;
+    if (thr > 0.0 && sgn < 0.0 && QK < thr) continue;
+    double lim = d->kinmnrl_rate_limiter ? d->kinmnrl_rate_limiter[k] : 0.0;
+    if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
+    double arr = 1.0;
+    double Ea = d->kinmnrl_activation_energy ? d->kinmnrl_activation_energy[k] : 0.0;
+    if (Ea > 0.0) arr = exp(Ea / OR_IDEAL_GAS * (1.0 / (25.0 + 273.15) - 1.0 / (c->temp + 273.15)));
+    double sum_pref = d->kinmnrl_rate_constant[k] * arr;
+    double Im_const = -c->area[k];
+    double Im;
+    if (d->kinmnrl_affinity_power)
+      Im = Im_const * sgn * pow(fabs(aff), d->kinmnrl_affinity_power[k]) * sum_pref;
+    else
+      Im = Im_const * sgn * fabs(aff) * sum_pref;
+    c->rate[k] = Im;
+    Im_const *= c->volume;
+    Im *= c->volume;
+    for (int s = 0; s < ns; s++) Res[MNRL_ID(d, k, s)] += MNRL_NU(d, k, s) * Im;
+    if (!deriv) continue;
+    double dIm_dQK;
+    if (d->kinmnrl_affinity_power)
+      dIm_dQK = -Im * d->kinmnrl_affinity_power[k] / fabs(aff);
+    else
+      dIm_dQK = -Im_const * sum_pref;
+    if (d->kinmnrl_Temkin_const) dIm_dQK = dIm_dQK * (1.0 / temkin) / QK * (1.0 - aff);
+    if (lim <= 0.0) {
+      for (int t = 0; t < ns; t++) {
+        int jc = MNRL_ID(d, k, t);
+        double dQK_dCj = MNRL_NU(d, k, t) * QK * exp(-ln_conc[jc]);
+        double dQK_dmj = dQK_dCj * c->den * 1.0e-3;
+        for (int s = 0; s < ns; s++) Jac[MNRL_ID(d, k, s)][jc] += MNRL_NU(d, k, s) * dIm_dQK * dQK_dmj;
+      }
+    } else {
+      double den = 1.0 + (1.0 - aff) / lim;
+      for (int t = 0; t < ns; t++) {
+        int jc = MNRL_ID(d, k, t);
+        double dQK_dCj = MNRL_NU(d, k, t) * QK * exp(-ln_conc[jc]);
+        double dQK_dmj = dQK_dCj * c->den * 1.0e-3;
+        for (int s = 0; s < ns; s++)
+          Jac[MNRL_ID(d, k, s)][jc] += MNRL_NU(d, k, s) * dIm_dQK * (1.0 + QK / lim / den) * dQK_dmj / den;
+      }
+    }
+  }
+}
+
+/* RTAuxVarCompute (reaction.F90:4969-5006): RTotal then RTotalSorb */
+static void or_auxvar_compute(const pfx_reaction_desc *d, ocell *c) {
+  or_rtotal(d, c);
+  if (d->neqionxrxn > 0) or_ionx(d, c);
+}
+
+/* Accumulation (mol), RTAccumulation reaction.F90:5072-5148 +
+   RAccumulationSorb :4539-4566 */
+static void or_accumulation(const pfx_reaction_desc *d, const ocell *c, double *acc) {
+  const int N = d->naqcomp;
+  double psv_t = c->porosity * c->sat * 1000.0 * c->volume;
+  for (int j = 0; j < N; j++) acc[j] = psv_t * c->total[j];
+  if (d->neqionxrxn > 0)
+    for (int j = 0; j < N; j++) acc[j] += c->sorb[j] * c->volume;
+}
+
+/* dt-consistent residual (mol/s) and Jacobian (kg/s) of one cell, the
+   per-cell part of RTResidualNonFlux/RTJacobianNonFlux
+   (reactive_transport.F90:2539-2587, 2735-2758, 3342-3465). */
+static void or_res_jac(const pfx_reaction_desc *d, ocell *c, const double *fixed_accum, double dt,
+                       double *R, double J[NMAX][NMAX], double *scale) {
+  const int N = d->naqcomp;
+  or_auxvar_compute(d, c);
+  double acc[NMAX];
+  or_accumulation(d, c, acc);
+  double psvd_t = c->porosity * c->sat * 1000.0 * c->volume / dt;
+  double v_t = c->volume / dt;
+  for (int j = 0; j < N; j++) {
+    R[j] = acc[j] / dt - fixed_accum[j] / dt;
+    for (int k = 0; k < N; k++) {
+      J[j][k] = c->dtotal[j][k] * psvd_t;
+      if (d->neqionxrxn > 0) J[j][k] += c->dsorb[j][k] * v_t;
+    }
+  }
+  if (d->nkinmnrl > 0) or_kinetic_mineral(d, c, R, J, 1);
+  if (scale) {
+    /* magnitude of every term in component j (mol/s): the denominator of the
+       scaled residual rel_res = max_j |R_j| / scale_j */
+    double psv = c->porosity * c->sat * c->den * c->volume; /* m^3 water * kg/m^3 */
+    double sc[NMAX];
+    for (int j = 0; j < N; j++) sc[j] = c->m[j];
+    for (int i = 0; i < d->neqcplx; i++)
+      for (int s = 0; s < CPLX_N(d, i); s++) sc[CPLX_ID(d, i, s)] += fabs(CPLX_NU(d, i, s)) * c->sec[i];
+    for (int j = 0; j < N; j++) {
+      double s = psv * sc[j];
+      if (d->neqionxrxn > 0) s += fabs(c->sorb[j]) * c->volume;
+      scale[j] = s / dt + fabs(fixed_accum[j]) / dt;
+    }
+  }
+}
+
+static double or_rel_res(const double *R, const double *scale, int N) {
+  double r = 0.0;
+  for (int j = 0; j < N; j++) {
+    double q = fabs(R[j]) / fmax(scale[j], 1e-300);
+    if (!(q <= r)) r = q; /* propagates NaN */
+  }
+  return r;
+}
+
+/* reaction.F90:4835-4880 RSolve: row equilibration, log-column scaling,
+   ludcmp/lubksb.  J is consumed. */
+static int or_rsolve(const double *R, double J[NMAX][NMAX], const double *conc, double *update, int N, int uselog) {
+  double A[NMAX * NMAX];
+  double rhs[NMAX];
+  for (int i = 0; i < N; i++) {
+    double norm = 0.0;
+    for (int k = 0; k < N; k++) norm = fmax(norm, fabs(J[i][k]));
+    norm = 1.0 / fmax(1.0, norm);
+    rhs[i] = R[i] * norm;
+    for (int k = 0; k < N; k++) J[i][k] *= norm;
+  }
+  if (uselog)
+    for (int k = 0; k < N; k++)
+      for (int i = 0; i < N; i++) J[i][k] *= conc[k];
+  for (int i = 0; i < N; i++)
+    for (int k = 0; k < N; k++) A[i + k * N] = J[i][k];
+  int indx[NMAX], dd;
+  int st = or_ludcmp_impl(A, N, indx, &dd);
+  if (st) return st;
+  or_lubksb(A, N, indx, rhs);
+  for (int i = 0; i < N; i++) update[i] = rhs[i];
+  return 0;
+}
+
+/* per-cell temperature-dependent log K (RUpdateTempDependentCoefs,
+   reaction.F90:5433-5524, computed into per-cell storage instead of the
+   shared reaction_type arrays) */
+static void or_cell_logk(const pfx_reaction_desc *d, const double *temp, int64_t cell, ocell *c) {
+  for (int i = 0; i < d->neqcplx; i++) c->logK[i] = d->eqcplx_logK[i];
+  for (int k = 0; k < d->nkinmnrl; k++) c->logKm[k] = d->kinmnrl_logK[k];
+  if (temp) {
+    if (d->eqcplx_logKcoef)
+      for (int i = 0; i < d->neqcplx; i++) c->logK[i] = or_logk_eval(d->eqcplx_logKcoef + 5 * i, temp[cell]);
+    if (d->kinmnrl_logKcoef)
+      for (int k = 0; k < d->nkinmnrl; k++) c->logKm[k] = or_logk_eval(d->kinmnrl_logKcoef + 5 * k, temp[cell]);
+  }
+}
+
+static void or_load_cell(const pfx_reaction_desc *d, const pfx_cells *cl, const pfx_react_opts *o, int64_t ci, ocell *c) {
+  const int N = d->naqcomp, M = d->neqcplx, K = d->nkinmnrl, R = d->neqionxrxn;
+  memset(c, 0, sizeof(*c));
+  c->N = N; c->M = M; c->K = K; c->R = R;
+  for (int j = 0; j < N; j++) {
+    c->m[j] = cl->pri_molal[ci * N + j];
+    c->gam[j] = cl->pri_act_coef ? cl->pri_act_coef[ci * N + j] : 1.0;
+  }
+  for (int i = 0; i < M; i++) c->gsec[i] = cl->sec_act_coef ? cl->sec_act_coef[ci * M + i] : 1.0;
+  for (int r = 0; r < R; r++) c->ionx_ref[r] = cl->ionx_ref_sorbed ? cl->ionx_ref_sorbed[ci * R + r] : 1e-9;
+  for (int k = 0; k < K; k++) {
+    c->vf[k] = cl->mnrl_volfrac ? cl->mnrl_volfrac[ci * K + k] : 0.0;
+    c->area[k] = cl->mnrl_area ? cl->mnrl_area[ci * K + k] : 0.0;
+  }
+  c->temp = cl->temperature ? cl->temperature[ci] : 25.0;
+  c->porosity = cl->porosity ? cl->porosity[ci] : o->porosity;
+  c->sat = cl->saturation ? cl->saturation[ci] : o->saturation;
+  c->volume = cl->volume ? cl->volume[ci] : o->volume;
+  c->den = cl->den_kg ? cl->den_kg[ci] : o->den_kg;
+  or_cell_logk(d, cl->temperature, ci, c);
+}
+
+/* ------------------------------------------------------------------ */
+/* One operator-split reaction step for one cell (RReact,
+   reaction.F90:3322-3511), solving the dt-consistent residual. */
+static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_opts *o, int64_t ci,
+                         int *iters_out, double *relres_out) {
+  const int N = d->naqcomp, M = d->neqcplx, K = d->nkinmnrl, R = d->neqionxrxn;
+  ocell c;
+  or_load_cell(d, cl, o, ci, &c);
+  const double dt = o->dt;
+
+  /* fixed accumulation from the transported totals and the old sorbed
+     state (reaction.F90:3370, 3399-3404) */
+  double fixed[NMAX];
+  double psv_t = c.porosity * c.sat * 1000.0 * c.volume;
+  for (int j = 0; j < N; j++) {
+    fixed[j] = psv_t * cl->total[ci * N + j];
+    if (R > 0) fixed[j] += (cl->total_sorb ? cl->total_sorb[ci * N + j] : 0.0) * c.volume;
+  }
+  /* activity coefficients once per step from the previous state
+     (reaction.F90:3407-3409); sec_molal of the previous state is
+     recomputed from the previous free ions and coefficients */
+  if (d->act_coef_update_frequency != PFX_ACT_OFF) {
+    or_rtotal(d, &c);
+    or_activity(d, &c);
+  }
+  /* initial iterate */
+  for (int j = 0; j < N; j++) {
+    if (o->guess_mode == PFX_GUESS_ARRAY && cl->guess) c.m[j] = cl->guess[ci * N + j];
+    else if (o->guess_mode == PFX_GUESS_DEFAULT) c.m[j] = o->default_guess;
+  }
+  int it = 0, status = PFX_CELL_MAX_ITERATIONS, extra = 0, done = 0;
+  double rr = INFINITY;
+  double Rv[NMAX], J[NMAX][NMAX], sc[NMAX], upd[NMAX];
+  while (!done) {
+    if (d->act_coef_update_frequency == PFX_ACT_NEWTON_ITER) {
+      or_rtotal(&*d, &c);
+      or_activity(d, &c);
+    }
+    or_res_jac(d, &c, fixed, dt, Rv, J, sc);
+    rr = or_rel_res(Rv, sc, N);
+    if (!isfinite(rr)) { status = PFX_CELL_NONFINITE; break; }
+    if (rr <= o->rel_tol) {
+      if (extra >= o->extra_iterations) { status = PFX_CELL_CONVERGED; break; }
+      extra++;
+    }
+    if (o->reference_stop) {
+      double mx = 0.0;
+      for (int j = 0; j < N; j++) mx = fmax(mx, fabs(Rv[j] * dt)); /* reference units: mol */
+      if (mx < d->max_residual_tolerance) { status = PFX_CELL_CONVERGED; break; }
+    }
+    if (it >= o->max_iterations) { status = PFX_CELL_MAX_ITERATIONS; break; }
+    it++;
+    if (or_rsolve(Rv, J, c.m, upd, N, d->use_log_formulation)) { status = PFX_CELL_SINGULAR; break; }
+    double prev[NMAX], maxrel = 0.0;
+    for (int j = 0; j < N; j++) prev[j] = c.m[j];
+    if (d->use_log_formulation) {
+      for (int j = 0; j < N; j++) {
+        double u = copysign(1.0, upd[j]) * fmin(fabs(upd[j]), d->max_dlnC);
+        c.m[j] = prev[j] * exp(-u);
+      }
+    } else {
+      double min_ratio = 1e20;
+      for (int j = 0; j < N; j++)
+        if (prev[j] <= upd[j]) {
+          double ratio = fabs(prev[j] / upd[j]);
+          if (ratio < min_ratio) min_ratio = ratio;
+        }
+      if (min_ratio < 1.0)
+        for (int j = 0; j < N; j++) upd[j] *= min_ratio * 0.99;
+      for (int j = 0; j < N; j++) c.m[j] = prev[j] - upd[j];
+    }
+    for (int j = 0; j < N; j++) maxrel = fmax(maxrel, fabs((c.m[j] - prev[j]) / prev[j]));
+    if (o->reference_stop && maxrel < d->max_relative_change_tolerance) {
+      or_res_jac(d, &c, fixed, dt, Rv, J, sc);
+      rr = or_rel_res(Rv, sc, N);
+      status = PFX_CELL_CONVERGED;
+      done = 1;
+    }
+  }
+  /* "one last update" (reaction.F90:3507): outputs at the returned iterate
+     (or_res_jac above already left totals, sorbed totals and rates there) */
+  for (int j = 0; j < N; j++) {
+    cl->pri_molal[ci * N + j] = c.m[j];
+    cl->total[ci * N + j] = c.total[j];
+    if (cl->pri_act_coef) cl->pri_act_coef[ci * N + j] = c.gam[j];
+    if (cl->total_sorb && R > 0) cl->total_sorb[ci * N + j] = c.sorb[j];
+  }
+  for (int i = 0; i < M; i++) {
+    if (cl->sec_act_coef) cl->sec_act_coef[ci * M + i] = c.gsec[i];
+    if (cl->sec_molal) cl->sec_molal[ci * M + i] = c.sec[i];
+  }
+  for (int r = 0; r < R; r++)
+    if (cl->ionx_ref_sorbed) cl->ionx_ref_sorbed[ci * R + r] = c.ionx_ref[r];
+  for (int k = 0; k < K; k++)
+    if (cl->mnrl_rate) cl->mnrl_rate[ci * K + k] = c.rate[k];
+  if (c.status) status = c.status;
+  *iters_out = it;
+  *relres_out = rr;
+  return status;
+}
+
+static void or_stats_init(pfx_stats *s, int64_t n) {
+  memset(s, 0, sizeof(*s));
+  s->ncells = n;
+  s->min_iterations = 1 << 30;
+}
+
+int or_react_batch(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_opts *o, pfx_stats *stats,
+                   int nthreads) {
+  if (!d || !cl || !o) return PFX_E_ARG;
+  if (d->naqcomp > NMAX || d->neqcplx > MMAX || d->nkinmnrl > KMAX || d->neqionxrxn > RMAX) return PFX_E_UNSUP;
+  const int64_t n = cl->ncells;
+  int64_t conv = 0, tot_it = 0, hist[32] = {0};
+  int maxit = 0, minit = 1 << 30;
+  double maxrr = 0.0;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) reduction(+ : conv, tot_it) \
+    reduction(max : maxit, maxrr) reduction(min : minit)
+#endif
+  for (int64_t ci = 0; ci < n; ci++) {
+    int it;
+    double rr;
+    int st = or_react_cell(d, cl, o, ci, &it, &rr);
+    if (cl->num_iterations) cl->num_iterations[ci] = it;
+    if (cl->status) cl->status[ci] = st;
+    if (cl->rel_res) cl->rel_res[ci] = rr;
+    conv += (st == PFX_CELL_CONVERGED);
+    tot_it += it;
+    if (it > maxit) maxit = it;
+    if (it < minit) minit = it;
+    if (rr > maxrr) maxrr = rr;
+  }
+  if (stats) {
+    or_stats_init(stats, n);
+    stats->converged = conv;
+    stats->total_iterations = tot_it;
+    stats->max_iterations = maxit;
+    stats->min_iterations = minit;
+    stats->max_rel_res = maxrr;
+    if (cl->num_iterations)
+      for (int64_t ci = 0; ci < n; ci++) hist[cl->num_iterations[ci] < 31 ? cl->num_iterations[ci] : 31]++;
+    memcpy(stats->iteration_histogram, hist, sizeof(hist));
+  }
+  (void)nthreads;
+  return (int)(n - conv);
+}
+
+/* Residual/Jacobian at the given state, no Newton (pfx_residual_jacobian). */
+int or_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_opts *o,
+                         const double *fixed_accum, double *residual, double *jacobian, int nthreads) {
+  if (!d || !cl || !o) return PFX_E_ARG;
+  if (d->naqcomp > NMAX || d->neqcplx > MMAX) return PFX_E_UNSUP;
+  const int N = d->naqcomp;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+  for (int64_t ci = 0; ci < cl->ncells; ci++) {
+    ocell c;
+    or_load_cell(d, cl, o, ci, &c);
+    double fixed[NMAX], Rv[NMAX], J[NMAX][NMAX];
+    if (fixed_accum) {
+      for (int j = 0; j < N; j++) fixed[j] = fixed_accum[ci * N + j];
+    } else {
+      double psv_t = c.porosity * c.sat * 1000.0 * c.volume;
+      for (int j = 0; j < N; j++) {
+        fixed[j] = psv_t * cl->total[ci * N + j];
+        if (d->neqionxrxn > 0 && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
+      }
+    }
+    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL);
+    for (int j = 0; j < N; j++) {
+      residual[ci * N + j] = Rv[j];
+      for (int k = 0; k < N; k++) jacobian[ci * N * N + j + (size_t)k * N] = J[j][k];
+    }
+    if (cl->sec_molal)
+      for (int i = 0; i < d->neqcplx; i++) cl->sec_molal[ci * d->neqcplx + i] = c.sec[i];
+  }
+  (void)nthreads;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Constraint speciation, ReactionEquilibrateConstraint
+   (reaction.F90:1308-2046) for TOTAL, FREE, LOG, PH (H+ primary) and
+   CHARGE_BAL constraints.  Types: 0 TOTAL, 1 FREE, 2 LOG, 3 PH, 4 CHARGE_BAL,
+   5 TOTAL_SORB (transport_constraint.F90 CONSTRAINT_* in that meaning). */
+#define OR_CON_TOTAL 0
+#define OR_CON_FREE 1
+#define OR_CON_LOG 2
+#define OR_CON_PH 3
+#define OR_CON_CHARGE 4
+#define OR_CON_TOTAL_SORB 5
+
+static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype, const double *conc_in,
+                               ocell *c, int use_prev, int molal, int *iters) {
+  const int N = d->naqcomp;
+  double conc[NMAX], total_conc[NMAX], free_conc[NMAX];
+  double molal_to_molar = molal ? c->den / 1000.0 : 1.0;
+  double molar_to_molal = molal ? 1.0 : 1000.0 / c->den;
+  for (int j = 0; j < N; j++) conc[j] = conc_in[j];
+  for (int j = 0; j < N; j++) free_conc[j] = use_prev ? c->m[j] : 1.0e-9;
+  for (int j = 0; j < N; j++) {
+    total_conc[j] = 0.0;
+    switch (ctype[j]) {
+      case OR_CON_TOTAL: total_conc[j] = conc[j] * molal_to_molar; break;
+      case OR_CON_TOTAL_SORB: total_conc[j] = conc[j]; break;
+      case OR_CON_FREE: free_conc[j] = conc[j] * molar_to_molal; break;
+      case OR_CON_LOG: free_conc[j] = pow(10.0, conc[j]) * molar_to_molal; break;
+      case OR_CON_CHARGE: if (!use_prev) free_conc[j] = conc[j] * molar_to_molal; break;
+      case OR_CON_PH: free_conc[j] = pow(10.0, -conc[j]); break;
+      default: return -1;
+    }
+  }
+  for (int j = 0; j < N; j++) c->m[j] = free_conc[j];
+  int nit = 0, nit_act_on = 0;
+  int compute_act = use_prev;
+  double Res[NMAX], J[NMAX][NMAX], upd[NMAX], prev[NMAX];
+  for (;;) {
+    for (int j = 0; j < N; j++)
+      if (ctype[j] == OR_CON_FREE || ctype[j] == OR_CON_LOG) c->m[j] = free_conc[j];
+    if (d->act_coef_update_frequency != PFX_ACT_OFF && compute_act) or_activity(d, c);
+    or_rtotal(d, c);
+    if (d->neqionxrxn > 0) or_ionx(d, c);
+    for (int j = 0; j < N; j++) {
+      for (int k = 0; k < N; k++) J[j][k] = 0.0;
+      switch (ctype[j]) {
+        case OR_CON_TOTAL:
+          Res[j] = c->total[j] - total_conc[j];
+          for (int k = 0; k < N; k++) J[j][k] = c->dtotal[j][k];
+          break;
+        case OR_CON_TOTAL_SORB:
+          Res[j] = c->sorb[j] - total_conc[j];
+          for (int k = 0; k < N; k++) J[j][k] = c->dsorb[j][k];
+          break;
+        case OR_CON_FREE:
+        case OR_CON_LOG:
+          Res[j] = 0.0;
+          J[j][j] = 1.0;
+          break;
+        case OR_CON_CHARGE:
+          Res[j] = 0.0;
+          for (int jj = 0; jj < N; jj++) {
+            Res[j] += d->primary_spec_Z[jj] * c->total[jj];
+            for (int kk = 0; kk < N; kk++) J[j][jj] += d->primary_spec_Z[kk] * c->dtotal[kk][jj];
+          }
+          if (c->m[j] < 1e-20) {
+            double z = d->primary_spec_Z[j];
+            if ((Res[j] > 0.0 && z > 0.0) || (Res[j] < 0.0 && z < 0.0)) c->m[j] = 1.0e-3;
+          }
+          break;
+        case OR_CON_PH:
+          Res[j] = 0.0;
+          c->m[j] = pow(10.0, -conc[j]) / c->gam[j];
+          J[j][j] = 1.0;
+          break;
+      }
+    }
+    double max_res = 0.0;
+    for (int j = 0; j < N; j++) max_res = fmax(max_res, fabs(Res[j]));
+    int uselog = d->use_log_formulation;
+    if (uselog && nit > 3 && nit < 9) uselog = (nit % 2 == 0);
+    if (or_rsolve(Res, J, c->m, upd, N, uselog)) return PFX_CELL_SINGULAR;
+    for (int j = 0; j < N; j++) prev[j] = c->m[j];
+    if (uselog) {
+      for (int j = 0; j < N; j++) {
+        double u = copysign(1.0, upd[j]) * fmin(fabs(upd[j]), d->max_dlnC);
+        c->m[j] = c->m[j] * exp(-u);
+      }
+    } else {
+      double min_ratio = 1e20;
+      for (int j = 0; j < N; j++)
+        if (prev[j] <= upd[j]) {
+          double ratio = fabs(prev[j] / upd[j]);
+          if (ratio < min_ratio) min_ratio = ratio;
+        }
+      if (min_ratio <= 1.0)
+        for (int j = 0; j < N; j++) upd[j] *= min_ratio * 0.99;
+      for (int j = 0; j < N; j++) c->m[j] = prev[j] - upd[j];
+    }
+    double mn = c->m[0];
+    for (int j = 1; j < N; j++) mn = fmin(mn, c->m[j]);
+    if (!(mn > 0.0)) return PFX_CELL_NONFINITE;
+    double maxrel = 0.0;
+    for (int j = 0; j < N; j++) maxrel = fmax(maxrel, fabs((c->m[j] - prev[j]) / prev[j]));
+    nit++;
+    if (nit >= 10000) return PFX_CELL_MAX_ITERATIONS;
+    if (max_res < d->max_residual_tolerance && maxrel < d->max_relative_change_tolerance) {
+      if (compute_act && nit - nit_act_on > 1) break;
+      if (!compute_act) nit_act_on = nit;
+      compute_act = 1;
+    }
+  }
+  if (d->neqionxrxn > 0) or_ionx(d, c);
+  *iters = nit;
+  return 0;
+}
+
+/* Batched constraint speciation followed by the reference's initial
+   auxiliary-variable refresh (condition_control.F90:770-775, 929-935):
+   free ions round-trip molal->molar->molal, totals recomputed, and, with
+   activity on, two passes of (activity update, totals).
+   ctype[N] shared by all cells, conc[ncells*N] per cell.  Outputs written to
+   cells->pri_molal/pri_act_coef/sec_act_coef/sec_molal/total/total_sorb/
+   ionx_ref_sorbed. */
+int or_equilibrate_batch(const pfx_reaction_desc *d, const int32_t *ctype, const double *conc, pfx_cells *cl,
+                         const pfx_react_opts *o, int molal, int32_t *iters, int nthreads) {
+  if (!d || !cl || !o) return PFX_E_ARG;
+  const int N = d->naqcomp, M = d->neqcplx, R = d->neqionxrxn;
+  int64_t bad = 0;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads) reduction(+ : bad)
+#endif
+  for (int64_t ci = 0; ci < cl->ncells; ci++) {
+    ocell c;
+    or_load_cell(d, cl, o, ci, &c);
+    for (int j = 0; j < N; j++) c.gam[j] = 1.0;
+    for (int i = 0; i < M; i++) c.gsec[i] = 1.0;
+    for (int r = 0; r < R; r++) c.ionx_ref[r] = 1e-9; /* reactive_transport_aux.F90:304 */
+    int it = 0;
+    int st = or_equilibrate_cell(d, ctype + 0, conc + ci * N, &c, 0, molal, &it);
+    if (st) bad++;
+    /* initial-condition refresh of the cell auxvar, which (isothermal deck,
+       no datasets) is not the constraint auxvar: it starts from
+       RTAuxVarInit's unit activity coefficients and 1e-9 exchange state
+       (reactive_transport_aux.F90:213-304, condition_control.F90:694-720) */
+    for (int j = 0; j < N; j++) c.gam[j] = 1.0;
+    for (int i = 0; i < M; i++) c.gsec[i] = 1.0;
+    for (int r = 0; r < R; r++) c.ionx_ref[r] = 1e-9;
+    for (int j = 0; j < N; j++) {
+      double molarity = c.m[j] * c.den / 1000.0;
+      c.m[j] = molarity / c.den * 1000.0;
+    }
+    or_rtotal(d, &c);
+    if (R > 0) or_ionx(d, &c);
+    if (d->act_coef_update_frequency != PFX_ACT_OFF) {
+      for (int pass = 0; pass < 2; pass++) {
+        or_activity(d, &c);
+        or_rtotal(d, &c);
+        if (R > 0) or_ionx(d, &c);
+      }
+    }
+    for (int j = 0; j < N; j++) {
+      cl->pri_molal[ci * N + j] = c.m[j];
+      if (cl->pri_act_coef) cl->pri_act_coef[ci * N + j] = c.gam[j];
+      if (cl->total) cl->total[ci * N + j] = c.total[j];
+      if (cl->total_sorb) cl->total_sorb[ci * N + j] = R > 0 ? c.sorb[j] : 0.0;
+    }
+    for (int i = 0; i < M; i++) {
+      if (cl->sec_act_coef) cl->sec_act_coef[ci * M + i] = c.gsec[i];
+      if (cl->sec_molal) cl->sec_molal[ci * M + i] = c.sec[i];
+    }
+    for (int r = 0; r < R; r++)
+      if (cl->ionx_ref_sorbed) cl->ionx_ref_sorbed[ci * R + r] = c.ionx_ref[r];
+    if (iters) iters[ci] = st ? -st : it;
+  }
+  (void)nthreads;
+  return (int)bad;
+}
+
+/* Per-cell totals (and sorbed totals, sec molal) at the given free ions and
+   activity coefficients: RTAuxVarCompute for a batch. */
+int or_auxvar_batch(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_opts *o, int update_activity,
+                    int nthreads) {
+  const int N = d->naqcomp, M = d->neqcplx, R = d->neqionxrxn;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+  for (int64_t ci = 0; ci < cl->ncells; ci++) {
+    ocell c;
+    or_load_cell(d, cl, o, ci, &c);
+    if (update_activity) {
+      or_rtotal(d, &c);
+      or_activity(d, &c);
+    }
+    or_rtotal(d, &c);
+    if (R > 0) or_ionx(d, &c);
+    for (int j = 0; j < N; j++) {
+      if (cl->total) cl->total[ci * N + j] = c.total[j];
+      if (cl->total_sorb) cl->total_sorb[ci * N + j] = R > 0 ? c.sorb[j] : 0.0;
+      if (cl->pri_act_coef) cl->pri_act_coef[ci * N + j] = c.gam[j];
+    }
+    for (int i = 0; i < M; i++) {
+      if (cl->sec_molal) cl->sec_molal[ci * M + i] = c.sec[i];
+      if (cl->sec_act_coef) cl->sec_act_coef[ci * M + i] = c.gsec[i];
+    }
+    for (int r = 0; r < R; r++)
+      if (cl->ionx_ref_sorbed) cl->ionx_ref_sorbed[ci * R + r] = c.ionx_ref[r];
+  }
+  (void)nthreads;
+  return 0;
+}
+
+/* Kinetic state update after a converged step (RUpdateKineticState,
+   reaction.F90:5340-5366): vf += rate * molar volume * dt, clamped >= 0,
+   with the rate re-evaluated at the solution. */
+int or_update_kinetic_state(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_opts *o, double *volfrac,
+                            int nthreads) {
+  const int N = d->naqcomp, K = d->nkinmnrl;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+  for (int64_t ci = 0; ci < cl->ncells; ci++) {
+    ocell c;
+    or_load_cell(d, cl, o, ci, &c);
+    for (int k = 0; k < K; k++) c.vf[k] = volfrac[ci * K + k];
+    double Rv[NMAX], J[NMAX][NMAX];
+    memset(Rv, 0, sizeof(Rv));
+    or_kinetic_mineral(d, &c, Rv, J, 0);
+    for (int k = 0; k < K; k++) {
+      double v = c.vf[k] + c.rate[k] * d->kinmnrl_molar_vol[k] * o->dt;
+      volfrac[ci * K + k] = v < 0.0 ? 0.0 : v;
+      if (cl->mnrl_rate) cl->mnrl_rate[ci * K + k] = c.rate[k];
+    }
+    (void)N;
+  }
+  (void)nthreads;
+  return 0;
+}
+
+/* building blocks exposed for the unit tests (one cell, host arrays) */
+int or_rtotal_cell(const pfx_reaction_desc *d, const double *m, const double *gam, const double *gsec,
+                   const double *logK, double den, double *total, double *dtotal_colmajor, double *sec) {
+  ocell c;
+  memset(&c, 0, sizeof(c));
+  const int N = d->naqcomp, M = d->neqcplx;
+  for (int j = 0; j < N; j++) { c.m[j] = m[j]; c.gam[j] = gam ? gam[j] : 1.0; }
+  for (int i = 0; i < M; i++) { c.gsec[i] = gsec ? gsec[i] : 1.0; c.logK[i] = logK ? logK[i] : d->eqcplx_logK[i]; }
+  c.den = den;
+  or_rtotal(d, &c);
+  for (int j = 0; j < N; j++) {
+    total[j] = c.total[j];
+    for (int k = 0; k < N; k++) dtotal_colmajor[j + k * N] = c.dtotal[j][k];
+  }
+  for (int i = 0; i < M; i++) sec[i] = c.sec[i];
+  return 0;
+}
+
+int or_activity_cell(const pfx_reaction_desc *d, const double *m, const double *sec, double *gam, double *gsec,
+                     double *sec_out) {
+  ocell c;
+  memset(&c, 0, sizeof(c));
+  const int N = d->naqcomp, M = d->neqcplx;
+  for (int j = 0; j < N; j++) { c.m[j] = m[j]; c.gam[j] = gam[j]; }
+  for (int i = 0; i < M; i++) { c.sec[i] = sec[i]; c.gsec[i] = gsec[i]; c.logK[i] = d->eqcplx_logK[i]; }
+  or_activity(d, &c);
+  for (int j = 0; j < N; j++) gam[j] = c.gam[j];
+  for (int i = 0; i < M; i++) { gsec[i] = c.gsec[i]; if (sec_out) sec_out[i] = c.sec[i]; }
+  return c.status;
+}
+
+int or_ionx_cell(const pfx_reaction_desc *d, const double *m, const double *gam, const double *vf, double *ionx_ref,
+                 double *sorb, double *dsorb_colmajor) {
+  ocell c;
+  memset(&c, 0, sizeof(c));
+  const int N = d->naqcomp;
+  for (int j = 0; j < N; j++) { c.m[j] = m[j]; c.gam[j] = gam ? gam[j] : 1.0; }
+  for (int k = 0; k < d->nkinmnrl; k++) c.vf[k] = vf ? vf[k] : 0.0;
+  for (int r = 0; r < d->neqionxrxn; r++) c.ionx_ref[r] = ionx_ref[r];
+  or_ionx(d, &c);
+  for (int r = 0; r < d->neqionxrxn; r++) ionx_ref[r] = c.ionx_ref[r];
+  for (int j = 0; j < N; j++) {
+    sorb[j] = c.sorb[j];
+    for (int k = 0; k < N; k++) dsorb_colmajor[j + k * N] = c.dsorb[j][k];
+  }
+  return c.status;
+}

@@ -1,0 +1,104 @@
+"""ctypes mirror of include/pfx_chem.h (the C-ABI of libpfx_chem.so).
+
+The structures here are byte-for-byte the C structs; tests/test_abi.py checks
+the field offsets against the header compiled with the host compiler.
+"""
+import ctypes as C
+
+PFX_ABI_VERSION = 1
+PFX_MAX_PRIMARY = 32
+PFX_MAX_CPLX = 256
+PFX_MAX_RXN_SPEC = 8
+PFX_MAX_MINERALS = 16
+PFX_MAX_IONX_RXN = 4
+PFX_MAX_IONX_CATION = 8
+
+PFX_E_ARG, PFX_E_DEVICE, PFX_E_NOMEM, PFX_E_UNSUP = -1, -2, -3, -4
+
+PFX_ACT_OFF, PFX_ACT_TIMESTEP, PFX_ACT_NEWTON_ITER = 0, 1, 2
+PFX_ACT_ALG_LAG, PFX_ACT_ALG_NEWTON = 0, 1
+
+PFX_CELL_CONVERGED, PFX_CELL_MAX_ITERATIONS, PFX_CELL_NONFINITE, PFX_CELL_SINGULAR = 0, 1, 2, 3
+PFX_GUESS_PREVIOUS, PFX_GUESS_ARRAY, PFX_GUESS_DEFAULT = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class ReactionDesc(C.Structure):
+    _fields_ = [
+        ("naqcomp", C.c_int32), ("neqcplx", C.c_int32), ("nkinmnrl", C.c_int32),
+        ("neqionxrxn", C.c_int32), ("max_rxn_spec", C.c_int32), ("max_cation", C.c_int32),
+        ("primary_spec_Z", _dp), ("primary_spec_a0", _dp),
+        ("eqcplxspecid", _ip), ("eqcplxstoich", _dp), ("eqcplxh2ostoich", _dp),
+        ("eqcplx_Z", _dp), ("eqcplx_a0", _dp), ("eqcplx_logK", _dp), ("eqcplx_logKcoef", _dp),
+        ("kinmnrlspecid", _ip), ("kinmnrlstoich", _dp), ("kinmnrlh2ostoich", _dp),
+        ("kinmnrl_logK", _dp), ("kinmnrl_logKcoef", _dp), ("kinmnrl_rate_constant", _dp),
+        ("kinmnrl_molar_vol", _dp), ("kinmnrl_activation_energy", _dp),
+        ("kinmnrl_affinity_threshold", _dp), ("kinmnrl_rate_limiter", _dp),
+        ("kinmnrl_affinity_power", _dp), ("kinmnrl_Temkin_const", _dp),
+        ("eqionx_rxn_cationid", _ip), ("eqionx_rxn_k", _dp), ("eqionx_rxn_CEC", _dp),
+        ("eqionx_rxn_to_surf", _ip),
+        ("debyeA", C.c_double), ("debyeB", C.c_double), ("debyeBdot", C.c_double),
+        ("use_log_formulation", C.c_int32), ("act_coef_update_frequency", C.c_int32),
+        ("act_coef_update_algorithm", C.c_int32), ("reserved0", C.c_int32),
+        ("max_dlnC", C.c_double), ("max_relative_change_tolerance", C.c_double),
+        ("max_residual_tolerance", C.c_double),
+    ]
+
+
+class Cells(C.Structure):
+    _fields_ = [
+        ("ncells", C.c_int64),
+        ("total", C.c_void_p), ("total_sorb", C.c_void_p), ("pri_molal", C.c_void_p),
+        ("pri_act_coef", C.c_void_p), ("sec_act_coef", C.c_void_p), ("sec_molal", C.c_void_p),
+        ("ionx_ref_sorbed", C.c_void_p), ("mnrl_volfrac", C.c_void_p), ("mnrl_area", C.c_void_p),
+        ("mnrl_rate", C.c_void_p), ("temperature", C.c_void_p), ("porosity", C.c_void_p),
+        ("saturation", C.c_void_p), ("volume", C.c_void_p), ("den_kg", C.c_void_p),
+        ("guess", C.c_void_p), ("num_iterations", C.c_void_p), ("status", C.c_void_p),
+        ("rel_res", C.c_void_p),
+    ]
+
+
+class ReactOpts(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("rel_tol", C.c_double), ("max_iterations", C.c_int32),
+        ("extra_iterations", C.c_int32), ("guess_mode", C.c_int32), ("reference_stop", C.c_int32),
+        ("default_guess", C.c_double), ("porosity", C.c_double), ("saturation", C.c_double),
+        ("volume", C.c_double), ("den_kg", C.c_double), ("stream", C.c_void_p),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("ncells", C.c_int64), ("converged", C.c_int64), ("total_iterations", C.c_int64),
+        ("max_iterations", C.c_int32), ("min_iterations", C.c_int32), ("max_rel_res", C.c_double),
+        ("kernel_ms", C.c_double), ("iteration_histogram", C.c_int64 * 32),
+    ]
+
+    def as_dict(self):
+        return {
+            "ncells": self.ncells, "converged": self.converged,
+            "total_iterations": self.total_iterations, "max_iterations": self.max_iterations,
+            "min_iterations": self.min_iterations, "max_rel_res": self.max_rel_res,
+            "kernel_ms": self.kernel_ms, "iteration_histogram": list(self.iteration_histogram),
+        }
+
+
+CELL_FIELDS = [f for f, _ in Cells._fields_ if f != "ncells"]
+
+
+def default_opts(dt=1.0, rel_tol=1e-10, max_iterations=100, extra_iterations=0,
+                 guess_mode=PFX_GUESS_PREVIOUS, reference_stop=0, default_guess=1e-9,
+                 porosity=0.25, saturation=1.0, volume=1.0, den_kg=997.0, stream=None):
+    o = ReactOpts()
+    o.dt = dt
+    o.rel_tol = rel_tol
+    o.max_iterations = max_iterations
+    o.extra_iterations = extra_iterations
+    o.guess_mode = guess_mode
+    o.reference_stop = reference_stop
+    o.default_guess = default_guess
+    o.porosity, o.saturation, o.volume, o.den_kg = porosity, saturation, volume, den_kg
+    o.stream = stream
+    return o

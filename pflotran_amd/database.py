@@ -1,0 +1,408 @@
+"""Reaction tables: the flattened ``reaction_type`` handed to the C-ABI.
+
+Host-side setup that a PFLOTRAN Fortran host would do in its own
+``DatabaseRead``/``BasisInit`` (src/pflotran/reaction_database.F90:25, 772).
+This module reads PFLOTRAN's ASCII thermodynamic database format for decks
+whose primary species are the database basis (no basis swap), and builds the
+numpy arrays of ``pfx_reaction_desc`` (include/pfx_chem.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import shlex
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import abi
+
+# pflotran_constants.F90:48 (truncated on purpose; the kernels use the same)
+LOG_TO_LN = 2.30258509299
+H2O_NAMES = ("H2O",)
+
+
+# --------------------------------------------------------------------------
+# database file
+
+
+@dataclass
+class DbSpecies:
+    name: str
+    a0: float
+    Z: float
+    mw: float
+
+
+@dataclass
+class DbReaction:
+    name: str
+    species: List[Tuple[float, str]]
+    logK: List[float]
+    a0: float = 0.0
+    Z: float = 0.0
+    mw: float = 0.0
+    molar_volume: float = 0.0  # minerals: cm^3/mol as written
+
+
+@dataclass
+class Database:
+    temperatures: List[float]
+    primary: Dict[str, DbSpecies] = field(default_factory=dict)
+    complexes: Dict[str, DbReaction] = field(default_factory=dict)
+    gases: Dict[str, DbReaction] = field(default_factory=dict)
+    minerals: Dict[str, DbReaction] = field(default_factory=dict)
+
+
+def _num(tok: str) -> float:
+    return float(tok.lower().replace("d", "e"))
+
+
+def read_database(path: str) -> Database:
+    """Parse the block layout of reaction_database.F90:113-420: temperatures,
+    then primary species | complexes | gases | minerals | surface complexes,
+    each section closed by a 'null' entry."""
+    with open(path) as f:
+        lines = [ln for ln in f.read().splitlines() if ln.strip()]
+    head = shlex.split(lines[0])
+    ntemp = int(head[1])
+    db = Database(temperatures=[_num(t) for t in head[2:2 + ntemp]])
+    section = 0
+    for ln in lines[1:]:
+        toks = shlex.split(ln)
+        if not toks:
+            continue
+        name = toks[0]
+        if name == "null":
+            section += 1
+            if section >= 5:
+                break
+            continue
+        try:
+            if section == 0:
+                db.primary[name] = DbSpecies(name, _num(toks[1]), _num(toks[2]), _num(toks[3]))
+            elif section == 1:
+                n = int(toks[1])
+                sp = [(_num(toks[2 + 2 * i]), toks[3 + 2 * i]) for i in range(n)]
+                p = 2 + 2 * n
+                lk = [_num(t) for t in toks[p:p + ntemp]]
+                a0, Z, mw = (_num(t) for t in toks[p + ntemp:p + ntemp + 3])
+                db.complexes[name] = DbReaction(name, sp, lk, a0, Z, mw)
+            elif section in (2, 3):
+                mv = _num(toks[1])
+                n = int(toks[2])
+                sp = [(_num(toks[3 + 2 * i]), toks[4 + 2 * i]) for i in range(n)]
+                p = 3 + 2 * n
+                lk = [_num(t) for t in toks[p:p + ntemp]]
+                mw = _num(toks[p + ntemp])
+                tgt = db.gases if section == 2 else db.minerals
+                tgt[name] = DbReaction(name, sp, lk, mw=mw, molar_volume=mv)
+        except (IndexError, ValueError):
+            # entries this build never selects (e.g. malformed test species)
+            continue
+    return db
+
+
+# --------------------------------------------------------------------------
+# closed-form setup helpers
+
+
+def debye_params(tref: float = 25.0, use_bdot: bool = True) -> Tuple[float, float, float]:
+    """Debye-Hueckel A, B, Bdot at the reference temperature
+    (reaction_database.F90:896-990): piecewise-linear in fixed tables."""
+    T = [0, 25, 60, 100, 150, 200, 250, 300, 350]
+    a = [0.4939, 0.5114, 0.5465, 0.5995, 0.6855, 0.7994, 0.9593, 1.2180, 1.2180]
+    b = [0.3253, 0.3288, 0.3346, 0.3421, 0.3525, 0.3639, 0.3766, 0.3925, 0.3925]
+    d = [0.0374, 0.0410, 0.0440, 0.0460, 0.0470, 0.0470, 0.0340, 0.0000, 0.0000]
+    if tref <= 0.01:
+        A, B, Bd = a[0], b[0], d[0]
+    elif tref > 350.0:
+        A, B, Bd = a[8], b[8], d[8]
+    else:
+        k = 1
+        while k < 8 and tref > T[k]:
+            k += 1
+        w = (tref - T[k - 1]) / (T[k] - T[k - 1])
+        A = a[k - 1] + w * (a[k] - a[k - 1])
+        B = b[k - 1] + w * (b[k] - b[k - 1])
+        Bd = d[k - 1] + w * (d[k] - d[k - 1])
+    return A, B, (Bd if use_bdot else 0.0)
+
+
+def logk_fit(temps: Sequence[float], logK: Sequence[float]) -> np.ndarray:
+    """5-term Maier-Kelly least-squares fit of log K(T) over the database
+    temperatures, entries of 500 are missing (reaction_aux.F90:1336-1408)."""
+    tk = np.asarray(temps, dtype=np.float64) + 273.15
+    lk = np.asarray(logK, dtype=np.float64)
+    use = np.abs(lk - 500.0) >= 1e-10
+    V = np.stack([np.log(tk), np.ones_like(tk), tk, 1.0 / tk, 1.0 / (tk * tk)])[:, use]
+    if use.sum() < 5:
+        # under-determined (e.g. one tabulated temperature): minimum-norm fit;
+        # only used for non-isothermal runs, which need >= 5 temperatures
+        return np.linalg.lstsq(V.T, lk[use], rcond=None)[0]
+    return np.linalg.solve(V @ V.T, V @ lk[use])
+
+
+def logk_eval(coefs: np.ndarray, temp_c) -> np.ndarray:
+    """reaction_aux.F90:1461-1488 ReactionInterpolateLogK"""
+    tk = np.asarray(temp_c, dtype=np.float64) + 273.15
+    c = np.asarray(coefs)
+    return c[..., 0] * np.log(tk) + c[..., 1] + c[..., 2] * tk + c[..., 3] / tk + c[..., 4] / (tk * tk)
+
+
+def water_density_ifc67(t: float = 25.0, p: float = 101325.0) -> float:
+    """IFC-67 liquid water density in kg/m^3 (eos_water.F90:876-940), the
+    reference water density of a transport-only run (factory_subsurface.F90:995)."""
+    aa = [6.824687741e03, -5.422063673e02, -2.096666205e04, 3.941286787e04,
+          -6.733277739e04, 9.902381028e04, -1.093911774e05, 8.590841667e04,
+          -4.511168742e04, 1.418138926e04, -2.017271113e03, 7.982692717e00,
+          -2.616571843e-2, 1.522411790e-3, 2.284279054e-2, 2.421647003e02,
+          1.269716088e-10, 2.074838328e-7, 2.174020350e-8, 1.105710498e-9,
+          1.293441934e01, 1.308119072e-5, 6.047626338e-14]
+    a1, a2, a3, a4 = 8.438375405e-1, 5.362162162e-4, 1.72, 7.342278489e-2
+    a5, a6, a7, a8 = 4.975858870e-2, 6.537154300e-1, 1.15e-6, 1.5108e-5
+    a9, a10, a11, a12 = 1.4188e-1, 7.002753165e00, 2.995284926e-4, 2.04e-1
+    theta = (t + 273.15) * (1.0 / 647.3)
+    th2 = theta * theta
+    th18 = theta ** 18.0
+    th20 = th18 * th2
+    beta = p * (1.0 / 22.064e6)
+    b2 = beta * beta
+    yy = 1.0 - a1 * th2 - a2 * theta ** (-6.0)
+    xx = a3 * yy * yy - 2.0 * (a4 * theta - a5 * beta)
+    xx = math.sqrt(xx) if xx > 0 else 1e-6
+    zz = yy + xx
+    u1 = aa[11] * a5 * zz ** (-5.0 / 17.0)
+    u2 = 1.0 / (a8 + theta ** 11.0)
+    u3 = aa[17] + (2.0 * aa[18] + 3.0 * aa[19] * beta) * beta
+    u4 = 1.0 / (a7 + th18 * theta)
+    u5 = (a10 + beta) ** (-4.0)
+    u6 = a11 - 3.0 * u5
+    u7 = aa[20] * th18 * (a9 + th2)
+    u8 = aa[15] * (a6 - theta) ** 9.0
+    vr = (u1 + aa[12] + theta * (aa[13] + aa[14] * theta) + u8 * (a6 - theta) + aa[16] * u4
+          - u2 * u3 - u6 * u7 + (3.0 * aa[21] * (a12 - theta) + 4.0 * aa[22] * beta / th20) * b2)
+    return 1.0 / (vr * 0.00317)
+
+
+# --------------------------------------------------------------------------
+# flattened tables
+
+
+@dataclass
+class MineralKinetics:
+    rate_constant: float           # mol/m^2/s
+    activation_energy: float = 0.0  # J/mol
+    affinity_threshold: float = 0.0
+    rate_limiter: float = 0.0
+
+
+@dataclass
+class IonExchangeRxn:
+    cations: List[Tuple[str, float]]   # reference cation first, k=1
+    CEC: float                         # eq/m^3 bulk
+    mineral: Optional[str] = None
+
+
+@dataclass
+class ReactionTables:
+    """numpy form of pfx_reaction_desc; `desc()` returns the ctypes struct
+    (the arrays must outlive it: they are kept on this object)."""
+    primary_names: List[str]
+    complex_names: List[str]
+    mineral_names: List[str]
+    Z: np.ndarray
+    a0: np.ndarray
+    cplx_specid: np.ndarray      # [M][S+1] int32
+    cplx_stoich: np.ndarray      # [M][S]
+    cplx_h2ostoich: np.ndarray
+    cplx_Z: np.ndarray
+    cplx_a0: np.ndarray
+    cplx_logK: np.ndarray
+    cplx_logKcoef: Optional[np.ndarray]
+    mnrl_specid: np.ndarray
+    mnrl_stoich: np.ndarray
+    mnrl_h2ostoich: np.ndarray
+    mnrl_logK: np.ndarray
+    mnrl_logKcoef: Optional[np.ndarray]
+    mnrl_rate_constant: np.ndarray
+    mnrl_molar_vol: np.ndarray
+    mnrl_activation_energy: np.ndarray
+    mnrl_affinity_threshold: np.ndarray
+    mnrl_rate_limiter: np.ndarray
+    ionx_cationid: np.ndarray    # [R][C+1]
+    ionx_k: np.ndarray           # [R][C]
+    ionx_CEC: np.ndarray
+    ionx_to_surf: np.ndarray
+    debye: Tuple[float, float, float] = (0.5114, 0.3288, 0.0410)
+    use_log_formulation: int = 1
+    act_freq: int = abi.PFX_ACT_OFF
+    act_alg: int = abi.PFX_ACT_ALG_LAG
+    max_dlnC: float = 5.0
+    max_rel_change_tol: float = 1e-6
+    max_residual_tol: float = 1e-12
+    _desc: Optional[abi.ReactionDesc] = None
+
+    @property
+    def N(self) -> int:
+        return len(self.primary_names)
+
+    @property
+    def M(self) -> int:
+        return len(self.complex_names)
+
+    @property
+    def K(self) -> int:
+        return len(self.mineral_names)
+
+    @property
+    def R(self) -> int:
+        return self.ionx_CEC.shape[0]
+
+    def desc(self) -> abi.ReactionDesc:
+        d = abi.ReactionDesc()
+        d.naqcomp, d.neqcplx, d.nkinmnrl, d.neqionxrxn = self.N, self.M, self.K, self.R
+        d.max_rxn_spec = self.cplx_stoich.shape[1]
+        d.max_cation = self.ionx_k.shape[1]
+
+        def dp(a):
+            if a is None:
+                return C.cast(None, C.POINTER(C.c_double))
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+            return a.ctypes.data_as(C.POINTER(C.c_double))
+
+        def ip(a):
+            if a is None:
+                return C.cast(None, C.POINTER(C.c_int32))
+            assert a.dtype == np.int32 and a.flags.c_contiguous
+            return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+        d.primary_spec_Z, d.primary_spec_a0 = dp(self.Z), dp(self.a0)
+        d.eqcplxspecid, d.eqcplxstoich = ip(self.cplx_specid), dp(self.cplx_stoich)
+        d.eqcplxh2ostoich, d.eqcplx_Z, d.eqcplx_a0 = dp(self.cplx_h2ostoich), dp(self.cplx_Z), dp(self.cplx_a0)
+        d.eqcplx_logK, d.eqcplx_logKcoef = dp(self.cplx_logK), dp(self.cplx_logKcoef)
+        d.kinmnrlspecid, d.kinmnrlstoich = ip(self.mnrl_specid), dp(self.mnrl_stoich)
+        d.kinmnrlh2ostoich, d.kinmnrl_logK = dp(self.mnrl_h2ostoich), dp(self.mnrl_logK)
+        d.kinmnrl_logKcoef = dp(self.mnrl_logKcoef)
+        d.kinmnrl_rate_constant, d.kinmnrl_molar_vol = dp(self.mnrl_rate_constant), dp(self.mnrl_molar_vol)
+        d.kinmnrl_activation_energy = dp(self.mnrl_activation_energy)
+        d.kinmnrl_affinity_threshold = dp(self.mnrl_affinity_threshold)
+        d.kinmnrl_rate_limiter = dp(self.mnrl_rate_limiter)
+        d.kinmnrl_affinity_power = dp(None)
+        d.kinmnrl_Temkin_const = dp(None)
+        d.eqionx_rxn_cationid, d.eqionx_rxn_k = ip(self.ionx_cationid), dp(self.ionx_k)
+        d.eqionx_rxn_CEC, d.eqionx_rxn_to_surf = dp(self.ionx_CEC), ip(self.ionx_to_surf)
+        d.debyeA, d.debyeB, d.debyeBdot = self.debye
+        d.use_log_formulation = self.use_log_formulation
+        d.act_coef_update_frequency = self.act_freq
+        d.act_coef_update_algorithm = self.act_alg
+        d.max_dlnC = self.max_dlnC
+        d.max_relative_change_tolerance = self.max_rel_change_tol
+        d.max_residual_tolerance = self.max_residual_tol
+        self._desc = d
+        return d
+
+    def flops_per_iteration(self) -> Dict[str, float]:
+        """Algorithmic work of one Newton iteration of one cell, counted from
+        the realised tables (FMA = 2 flops; exp/log/pow counted separately)."""
+        N, M, K = self.N, self.M, self.K
+        s = self.cplx_specid[:, 0].astype(np.float64) if M else np.zeros(0)
+        t = self.mnrl_specid[:, 0].astype(np.float64) if K else np.zeros(0)
+        rtotal = float(np.sum(4 * s + 2 + 4 * s + 2 * s * s)) + N * N + 2 * N
+        accum = N + N * N
+        ionx = 0.0
+        for r in range(self.R):
+            c = float(self.ionx_cationid[r, 0])
+            ionx += 10 * (c * 6) + 3 * c * c
+        mnrl = float(np.sum(2 * t + 12 + 2 * t * t + 3 * t)) if K else 0.0
+        rsolve = 2 * N * N + N + N * N + (2.0 / 3.0) * N ** 3 + N * N + 2 * N * N
+        update = 3 * N
+        flops = rtotal + accum + ionx + mnrl + rsolve + update
+        transc = 2 * N + float(np.sum(1 + s)) + K + N
+        return {"flops": flops, "transcendentals": transc}
+
+
+def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[str] = (),
+                 minerals: Dict[str, MineralKinetics] | None = None,
+                 ion_exchange: Sequence[IonExchangeRxn] = (), *, temperature: float = 25.0,
+                 log_formulation: bool = True, act_freq: int = abi.PFX_ACT_OFF,
+                 act_alg: int = abi.PFX_ACT_ALG_LAG, use_bdot: bool = True,
+                 fit_logk: bool = False) -> ReactionTables:
+    """Flatten a deck's CHEMISTRY block against a database (no basis swap:
+    every complex/mineral species must be a primary or H2O).  log K at the
+    reference temperature is the tabulated value when the temperature is a
+    database temperature (ReactionInitializeLogK, reaction_aux.F90:1438-1450),
+    otherwise the Maier-Kelly fit."""
+    minerals = dict(minerals or {})
+    pidx = {n: i for i, n in enumerate(primaries)}
+    N, M, K, R = len(primaries), len(secondaries), len(minerals), len(ion_exchange)
+    S = 1
+    for n in secondaries:
+        S = max(S, sum(1 for _, sp in db.complexes[n].species if sp not in H2O_NAMES))
+    for n in minerals:
+        S = max(S, sum(1 for _, sp in db.minerals[n].species if sp not in H2O_NAMES))
+    temps = db.temperatures
+    itemp = next((i for i, t in enumerate(temps) if abs(t - temperature) < 1e-10), None)
+
+    def logk_at(lk):
+        if itemp is not None and abs(lk[itemp] - 500.0) > 1e-10:
+            return lk[itemp]
+        return float(logk_eval(logk_fit(temps, lk), temperature))
+
+    Z = np.array([db.primary[n].Z for n in primaries], dtype=np.float64)
+    a0 = np.array([db.primary[n].a0 for n in primaries], dtype=np.float64)
+
+    def rxn_arrays(names, src):
+        spec = np.zeros((len(names), S + 1), dtype=np.int32)
+        nu = np.zeros((len(names), S), dtype=np.float64)
+        h2o = np.zeros(len(names), dtype=np.float64)
+        logK = np.zeros(len(names), dtype=np.float64)
+        coef = np.zeros((len(names), 5), dtype=np.float64)
+        for i, n in enumerate(names):
+            r = src[n]
+            k = 0
+            for st, sp in r.species:
+                if sp in H2O_NAMES:
+                    h2o[i] = st
+                    continue
+                if sp not in pidx:
+                    raise ValueError(f"{n}: species {sp} is not a primary (basis swap unsupported)")
+                spec[i, 1 + k] = pidx[sp]
+                nu[i, k] = st
+                k += 1
+            spec[i, 0] = k
+            logK[i] = logk_at(r.logK)
+            coef[i] = logk_fit(temps, r.logK)
+        return spec, nu, h2o, logK, coef
+
+    cs, cn, ch, cl, cc = rxn_arrays(list(secondaries), db.complexes)
+    mnames = list(minerals)
+    ms, mn, mh, ml, mc = rxn_arrays(mnames, db.minerals)
+    Cmax = max([len(r.cations) for r in ion_exchange] + [1])
+    ionx_id = np.zeros((R, Cmax + 1), dtype=np.int32)
+    ionx_k = np.zeros((R, Cmax), dtype=np.float64)
+    for r, rx in enumerate(ion_exchange):
+        ionx_id[r, 0] = len(rx.cations)
+        for j, (nm, k) in enumerate(rx.cations):
+            ionx_id[r, 1 + j] = pidx[nm]
+            ionx_k[r, j] = k
+    return ReactionTables(
+        primary_names=list(primaries), complex_names=list(secondaries), mineral_names=mnames,
+        Z=Z, a0=a0, cplx_specid=cs, cplx_stoich=cn, cplx_h2ostoich=ch,
+        cplx_Z=np.array([db.complexes[n].Z for n in secondaries], dtype=np.float64),
+        cplx_a0=np.array([db.complexes[n].a0 for n in secondaries], dtype=np.float64),
+        cplx_logK=cl, cplx_logKcoef=cc if fit_logk else None,
+        mnrl_specid=ms, mnrl_stoich=mn, mnrl_h2ostoich=mh, mnrl_logK=ml,
+        mnrl_logKcoef=mc if fit_logk else None,
+        mnrl_rate_constant=np.array([minerals[n].rate_constant for n in mnames], dtype=np.float64),
+        mnrl_molar_vol=np.array([db.minerals[n].molar_volume * 1e-6 for n in mnames], dtype=np.float64),
+        mnrl_activation_energy=np.array([minerals[n].activation_energy for n in mnames], dtype=np.float64),
+        mnrl_affinity_threshold=np.array([minerals[n].affinity_threshold for n in mnames], dtype=np.float64),
+        mnrl_rate_limiter=np.array([minerals[n].rate_limiter for n in mnames], dtype=np.float64),
+        ionx_cationid=ionx_id, ionx_k=ionx_k,
+        ionx_CEC=np.array([rx.CEC for rx in ion_exchange], dtype=np.float64),
+        ionx_to_surf=np.array([mnames.index(rx.mineral) if rx.mineral else -1 for rx in ion_exchange],
+                              dtype=np.int32),
+        debye=debye_params(temperature, use_bdot), use_log_formulation=int(log_formulation),
+        act_freq=act_freq, act_alg=act_alg)

@@ -1,0 +1,64 @@
+"""Regenerate the committed fixtures under tests/golden/ from the reference
+tree (run in the build container only; the GPU box never reads the
+reference).  Fixtures are data: database excerpts and the regression-gold
+values of the reference's own batch chemistry tests.
+
+    python tests/golden/make_fixtures.py [/root/reference]
+"""
+import json
+import os
+import re
+import shutil
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+BATCH = os.path.join(REF, "regression_tests", "ascem", "batch")
+
+# database files held by the reference's batch tests, copied verbatim
+for name in ("carbonate.dat", "ca-carbonate.dat", "calcite.dat"):
+    shutil.copy(os.path.join(BATCH, name), os.path.join(HERE, "db", name))
+
+# excerpt of database/hanford.dat: the species of the 1-D calcite column
+# (5 primaries, 8 complexes, calcite) and of the Valocchi ion-exchange test
+KEEP = {"H2O", "H+", "HCO3-", "Ca++", "Mg++", "Cl-", "Na+",
+        "OH-", "CO2(aq)", "CO3--", "CaHCO3+", "CaCO3(aq)", "MgHCO3+", "MgCO3(aq)", "CaCl+",
+        "Calcite", "Halite"}
+out = []
+with open(os.path.join(REF, "database", "hanford.dat")) as f:
+    for i, ln in enumerate(f):
+        m = re.match(r"^'([^']*)'", ln)
+        if i == 0 or (m and (m.group(1) in KEEP or m.group(1) == "null")):
+            out.append(ln if ln.endswith("\n") else ln + "\n")
+with open(os.path.join(HERE, "db", "hanford_subset.dat"), "w") as f:
+    f.writelines(out)
+
+
+def parse_gold(path):
+    vals, sec = {}, None
+    for ln in open(path):
+        m = re.match(r"^-- (\w+): (.*) --$", ln.strip())
+        if m:
+            sec = f"{m.group(1)}: {m.group(2)}"
+            vals[sec] = {}
+            continue
+        m = re.match(r"^\s*([\w\s\-\(\)]+):\s+(\S+)$", ln.rstrip())
+        if m and sec:
+            key = m.group(1).strip()
+            try:
+                vals[sec][key] = float(m.group(2))
+            except ValueError:
+                pass
+    return vals
+
+
+golds = {}
+for test in ("carbonate-unit-activity", "carbonate-debye-huckel-activity",
+             "ca-carbonate-unit-activity", "ca-carbonate-debye-huckel-activity",
+             "calcite-kinetics", "calcite-kinetics-volume-fractions",
+             "ion-exchange-valocchi"):
+    golds[test] = parse_gold(os.path.join(BATCH, f"{test}.regression.gold"))
+golds["_source"] = "regression_tests/ascem/batch/*.regression.gold (tolerance 1e-12 absolute, batch.cfg)"
+with open(os.path.join(HERE, "batch_golds.json"), "w") as f:
+    json.dump(golds, f, indent=1, sort_keys=True)
+print("fixtures written to", HERE)
