@@ -63,6 +63,13 @@ extern "C" {
 #define PFX_CELL_NONFINITE      2
 #define PFX_CELL_SINGULAR       3
 
+/* constraint types of pfx_equilibrate_batch (transport_constraint.F90
+   CONSTRAINT_* meanings) */
+#define PFX_CONSTRAINT_TOTAL 0   /* total component (mol/L or mol/kg)      */
+#define PFX_CONSTRAINT_FREE  1   /* free ion concentration                 */
+#define PFX_CONSTRAINT_LOG   2   /* log10 free ion concentration           */
+#define PFX_CONSTRAINT_PH    3   /* pH (component must be H+)              */
+
 /* initial Newton iterate */
 #define PFX_GUESS_PREVIOUS  0   /* previous free ions (RReact's guess, reaction.F90:3370) */
 #define PFX_GUESS_ARRAY     1   /* caller array cells->guess (e.g. learned warm start) */
@@ -210,6 +217,30 @@ int pfx_react_batch(pfx_tables *tables, pfx_cells *cells,
 int pfx_residual_jacobian(pfx_tables *tables, pfx_cells *cells,
                           const pfx_react_opts *opts,
                           double *residual, double *jacobian);
+
+/* Batched constraint speciation: ReactionEquilibrateConstraint
+   (reaction.F90:1308-2046) for every cell, as called per cell by
+   CondControlAssignTranInitCond (condition_control.F90:725-750).
+   constraint_type[N] (PFX_CONSTRAINT_*) is shared by all cells;
+   constraint_conc[ncells*N] (device) holds each cell's values.  Starts from
+   1e-9 molal with unit activity coefficients, switches activity on after the
+   first convergence (LAG, every iteration), and stops on the reference
+   criteria max|R| < max_residual_tolerance and max|dm/m| <
+   max_relative_change_tolerance.  Out: pri_molal, pri_act_coef,
+   sec_act_coef, sec_molal, total (at the last evaluation), total_sorb and
+   ionx_ref_sorbed at the solution, num_iterations, status.
+   molal != 0: concentrations in mol/kg water (CHEMISTRY MOLAL). */
+int pfx_equilibrate_batch(pfx_tables *tables, pfx_cells *cells, const int32_t *constraint_type,
+                          const double *constraint_conc, int32_t molal,
+                          const pfx_react_opts *opts, pfx_stats *stats);
+
+/* Auxiliary-variable refresh at the given free ions: RTUpdateAuxVars's cell
+   loop (reactive_transport.F90:3790-3846): RTAuxVarCompute (totals, sorbed
+   totals, complex molalities), preceded `activity_passes` times by
+   RActivityCoefficients + RTAuxVarCompute (the initial-condition refresh
+   of condition_control.F90:929-935 uses 2). */
+int pfx_auxvar_batch(pfx_tables *tables, pfx_cells *cells, int32_t activity_passes,
+                     const pfx_react_opts *opts);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,591 @@
+// Host side of libpfx_chem.so: the C-ABI of include/pfx_chem.h.
+//
+// pfx_tables_create flattens the reaction network into the device table
+// blob (row-per-lane lists, see pfx_device.h); pfx_react_batch launches the
+// persistent Newton kernel over a batch of cells and reduces the per-cell
+// statistics on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pfx_device.h"
+
+template <int G, int MODE>
+__global__ void pfx_react_kernel(const pfx_kernel_params P);
+extern "C" int pfx_scr_doubles(int M, int G, int K);
+
+struct pfx_tables {
+  pfx_dev_layout L;
+  unsigned char *d_blob = nullptr;
+  unsigned int *d_queue = nullptr;
+  // per-call scratch (grown on demand)
+  int32_t *d_iters = nullptr, *d_status = nullptr;
+  double *d_relres = nullptr;
+  int64_t scratch_cells = 0;
+  unsigned long long *d_stats = nullptr;  // reduction buffer
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int device = 0;
+  int num_cu = 0;
+};
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int64_t kMaxLds = 64 * 1024;
+
+int g_last_hip_error = 0;
+
+#define HIPCHK(x)                                                   \
+  do {                                                              \
+    hipError_t e_ = (x);                                            \
+    if (e_ != hipSuccess) {                                         \
+      g_last_hip_error = (int)e_;                                   \
+      fprintf(stderr, "pfx_chem: %s failed: %s (%s:%d)\n", #x,     \
+              hipGetErrorString(e_), __FILE__, __LINE__);           \
+      return PFX_E_DEVICE;                                          \
+    }                                                               \
+  } while (0)
+
+int group_size(int N) {
+  if (N <= 8) return 8;
+  if (N <= 16) return 16;
+  if (N <= 32) return 32;
+  return 0;
+}
+
+struct BlobBuilder {
+  std::vector<unsigned char> bytes;
+  template <typename T>
+  int64_t add(const std::vector<T> &v) {
+    size_t off = (bytes.size() + 15) & ~size_t(15);
+    bytes.resize(off + v.size() * sizeof(T) + 16);  // never zero-sized
+    if (!v.empty()) memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+    bytes.resize(off + std::max<size_t>(v.size() * sizeof(T), 8));
+    return (int64_t)off;
+  }
+};
+
+// ---------------------------------------------------------------------------
+__global__ void k_zero_u32(unsigned int *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// stats[0]=converged, [1]=sum iterations, [2]=max it, [3]=min it (inverted),
+// [4]=max rel_res bits, [5..36]=histogram
+__global__ void k_stats(const int32_t *it, const int32_t *st, const double *rr, int64_t n,
+                        unsigned long long *out) {
+  __shared__ unsigned long long s_conv, s_sum, s_max, s_min, s_rr;
+  __shared__ unsigned long long s_hist[32];
+  if (threadIdx.x == 0) { s_conv = 0; s_sum = 0; s_max = 0; s_min = 0; s_rr = 0; }
+  if (threadIdx.x < 32) s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long conv = 0, sum = 0, mx = 0, mn = 0, rmax = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = it[i];
+    conv += (st[i] == PFX_CELL_CONVERGED);
+    sum += (unsigned long long)k;
+    mx = max(mx, (unsigned long long)k);
+    mn = max(mn, (unsigned long long)(0x7fffffff - k));
+    double v = rr[i];
+    unsigned long long b = (v == v) ? (unsigned long long)__double_as_longlong(fabs(v)) : 0x7ff8000000000000ull;
+    rmax = max(rmax, b);
+    atomicAdd(&s_hist[k < 31 ? (k < 0 ? 0 : k) : 31], 1ull);
+  }
+  atomicAdd(&s_conv, conv);
+  atomicAdd(&s_sum, sum);
+  atomicMax(&s_max, mx);
+  atomicMax(&s_min, mn);
+  atomicMax(&s_rr, rmax);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], s_conv);
+    atomicAdd(&out[1], s_sum);
+    atomicMax(&out[2], s_max);
+    atomicMax(&out[3], s_min);
+    atomicMax(&out[4], s_rr);
+  }
+  if (threadIdx.x < 32) atomicAdd(&out[5 + threadIdx.x], s_hist[threadIdx.x]);
+}
+
+int ensure_scratch(pfx_tables *t, int64_t n) {
+  if (n <= t->scratch_cells) return 0;
+  if (t->d_iters) HIPCHK(hipFree(t->d_iters));
+  if (t->d_status) HIPCHK(hipFree(t->d_status));
+  if (t->d_relres) HIPCHK(hipFree(t->d_relres));
+  t->d_iters = nullptr;
+  t->d_status = nullptr;
+  t->d_relres = nullptr;
+  HIPCHK(hipMalloc(&t->d_iters, n * sizeof(int32_t)));
+  HIPCHK(hipMalloc(&t->d_status, n * sizeof(int32_t)));
+  HIPCHK(hipMalloc(&t->d_relres, n * sizeof(double)));
+  t->scratch_cells = n;
+  return 0;
+}
+
+template <int MODE>
+const void *kernel_for(int G) {
+  return G == 8 ? (const void *)pfx_react_kernel<8, MODE>
+       : G == 16 ? (const void *)pfx_react_kernel<16, MODE>
+                 : (const void *)pfx_react_kernel<32, MODE>;
+}
+
+const void *kernel_ptr(int G, int mode) {
+  switch (mode) {
+    case PFX_MODE_REACT: return kernel_for<PFX_MODE_REACT>(G);
+    case PFX_MODE_EVAL: return kernel_for<PFX_MODE_EVAL>(G);
+    case PFX_MODE_EQUIL: return kernel_for<PFX_MODE_EQUIL>(G);
+    default: return kernel_for<PFX_MODE_AUX>(G);
+  }
+}
+
+int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, double *ms) {
+  const int G = t->L.G;
+  const int groups = BLOCK / G;
+  const int64_t lds = t->L.lds_table_bytes + (int64_t)groups * pfx_scr_doubles(t->L.M, G, t->L.K) * 8;
+  if (lds > kMaxLds) return PFX_E_UNSUP;
+  int occ = 0;
+  const void *fn = kernel_ptr(G, mode);
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BLOCK, (size_t)lds));
+  if (occ < 1) occ = 1;
+  int64_t need_blocks = (P.ncells + groups - 1) / groups;
+  int64_t blocks = std::min<int64_t>(need_blocks, (int64_t)t->num_cu * occ);
+  if (blocks < 1) blocks = 1;
+  // chunk: enough cells per fetch to keep queue atomics rare, small enough
+  // to balance the tail
+  int64_t per_wave = std::max<int64_t>(1, P.ncells / (blocks * (BLOCK / 64)));
+  P.chunk = (int32_t)std::max<int64_t>(64 / G, std::min<int64_t>(64, per_wave / 8 + 1));
+  P.queue = t->d_queue;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, t->d_queue, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(t->ev0, stream));
+  void *args[] = {(void *)&P};
+  HIPCHK(hipLaunchKernel(fn, dim3((unsigned)blocks), dim3(BLOCK), args, (size_t)lds, stream));
+  HIPCHK(hipEventRecord(t->ev1, stream));
+  HIPCHK(hipEventSynchronize(t->ev1));
+  float f = 0.f;
+  HIPCHK(hipEventElapsedTime(&f, t->ev0, t->ev1));
+  if (ms) *ms = f;
+  return 0;
+}
+
+void fill_params(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, pfx_kernel_params &P) {
+  memset(&P, 0, sizeof(P));
+  P.L = t->L;
+  P.tables = t->d_blob;
+  P.ncells = cells->ncells;
+  P.total = cells->total;
+  P.total_sorb = cells->total_sorb;
+  P.pri_molal = cells->pri_molal;
+  P.pri_act_coef = cells->pri_act_coef;
+  P.sec_act_coef = cells->sec_act_coef;
+  P.sec_molal = cells->sec_molal;
+  P.ionx_ref = cells->ionx_ref_sorbed;
+  P.mnrl_volfrac = cells->mnrl_volfrac;
+  P.mnrl_area = cells->mnrl_area;
+  P.mnrl_rate = cells->mnrl_rate;
+  P.temperature = cells->temperature;
+  P.porosity = cells->porosity;
+  P.saturation = cells->saturation;
+  P.volume = cells->volume;
+  P.den_kg = cells->den_kg;
+  P.guess = cells->guess;
+  P.num_iterations = cells->num_iterations;
+  P.status = cells->status;
+  P.rel_res = cells->rel_res;
+  P.dt = o->dt;
+  P.rel_tol = o->rel_tol;
+  P.default_guess = o->default_guess;
+  P.def_porosity = o->porosity;
+  P.def_saturation = o->saturation;
+  P.def_volume = o->volume;
+  P.def_den = o->den_kg;
+  P.max_iterations = o->max_iterations;
+  P.extra_iterations = o->extra_iterations;
+  P.guess_mode = o->guess_mode;
+  P.reference_stop = o->reference_stop;
+}
+
+int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o) {
+  if (!c || !o) return PFX_E_ARG;
+  if (c->ncells < 0 || c->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
+  if (!c->total || !c->pri_molal) return PFX_E_ARG;
+  if (t->L.R > 0 && !c->total_sorb) return PFX_E_ARG;
+  if (t->L.K > 0 && (!c->mnrl_volfrac || !c->mnrl_area)) return PFX_E_ARG;
+  if (o->guess_mode == PFX_GUESS_ARRAY && !c->guess) return PFX_E_ARG;
+  if (!(o->dt > 0.0)) return PFX_E_ARG;
+  return 0;
+}
+
+int finish_stats(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream_t stream, double ms,
+                 pfx_stats *stats) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, t->d_stats, 64);
+  int sblocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(k_stats, dim3(sblocks), dim3(256), 0, stream, P.num_iterations, P.status, P.rel_res, n,
+                     t->d_stats);
+  HIPCHK(hipGetLastError());
+  unsigned long long h[64];
+  HIPCHK(hipMemcpyAsync(h, t->d_stats, sizeof(h), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  const int64_t conv = (int64_t)h[0];
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->ncells = n;
+    stats->converged = conv;
+    stats->total_iterations = (int64_t)h[1];
+    stats->max_iterations = (int32_t)h[2];
+    stats->min_iterations = (int32_t)(0x7fffffff - (long long)h[3]);
+    double rr;
+    memcpy(&rr, &h[4], sizeof(rr));
+    stats->max_rel_res = rr;
+    stats->kernel_ms = ms;
+    for (int i = 0; i < 32; i++) stats->iteration_histogram[i] = (int64_t)h[5 + i];
+  }
+  return (int)(n - conv);
+}
+
+int use_scratch(pfx_tables *t, pfx_kernel_params &P, int64_t n) {
+  if (!P.num_iterations || !P.status || !P.rel_res) {
+    int rc = ensure_scratch(t, n);
+    if (rc) return rc;
+    if (!P.num_iterations) P.num_iterations = t->d_iters;
+    if (!P.status) P.status = t->d_status;
+    if (!P.rel_res) P.rel_res = t->d_relres;
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" int pfx_abi_version(void) { return PFX_ABI_VERSION; }
+
+extern "C" int pfx_last_hip_error(void) { return g_last_hip_error; }
+
+extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
+  if (!d || !out) return PFX_E_ARG;
+  *out = nullptr;
+  const int N = d->naqcomp, M = d->neqcplx, K = d->nkinmnrl, R = d->neqionxrxn;
+  const int S0 = d->max_rxn_spec, C0 = d->max_cation;
+  if (N < 1 || M < 0 || K < 0 || R < 0 || S0 < 1 || (R > 0 && C0 < 1)) return PFX_E_ARG;
+  if (N > PFX_MAX_PRIMARY || M > PFX_MAX_CPLX || K > PFX_MAX_MINERALS || R > PFX_MAX_IONX_RXN ||
+      S0 > PFX_MAX_RXN_SPEC || C0 > PFX_MAX_IONX_CATION)
+    return PFX_E_UNSUP;
+  if (!d->primary_spec_Z || !d->primary_spec_a0) return PFX_E_ARG;
+  if (M > 0 && (!d->eqcplxspecid || !d->eqcplxstoich || !d->eqcplx_Z || !d->eqcplx_a0 || !d->eqcplx_logK))
+    return PFX_E_ARG;
+  if (K > 0 && (!d->kinmnrlspecid || !d->kinmnrlstoich || !d->kinmnrl_logK || !d->kinmnrl_rate_constant))
+    return PFX_E_ARG;
+  if (R > 0 && (!d->eqionx_rxn_cationid || !d->eqionx_rxn_k || !d->eqionx_rxn_CEC)) return PFX_E_ARG;
+  if (d->act_coef_update_frequency == PFX_ACT_NEWTON_ITER ||
+      (d->act_coef_update_frequency != PFX_ACT_OFF && d->act_coef_update_algorithm != PFX_ACT_ALG_LAG))
+    return PFX_E_UNSUP;  // activity updates inside the Newton loop: not in this kernel
+  for (int i = 0; i < M; i++)
+    if (d->eqcplxh2ostoich && d->eqcplxh2ostoich[i] != 0.0) {
+      // water activity is 1 (ln a_w = 0, use_activity_h2o = F): the term vanishes
+    }
+  const int G = group_size(N);
+  pfx_dev_layout L;
+  memset(&L, 0, sizeof(L));
+  L.N = N; L.M = M; L.K = K; L.R = R; L.G = G; L.S = S0;
+  L.Q = (M + G - 1) / G;
+
+  auto cid = [&](int i, int s) { return d->eqcplxspecid[(size_t)i * (S0 + 1) + 1 + s]; };
+  auto cnt = [&](int i) { return d->eqcplxspecid[(size_t)i * (S0 + 1)]; };
+  auto cnu = [&](int i, int s) { return d->eqcplxstoich[(size_t)i * S0 + s]; };
+  for (int i = 0; i < M; i++) {
+    if (cnt(i) < 0 || cnt(i) > S0) return PFX_E_ARG;
+    for (int s = 0; s < cnt(i); s++)
+      if (cid(i, s) < 0 || cid(i, s) >= N) return PFX_E_ARG;
+  }
+
+  BlobBuilder B;
+  std::vector<double> Z(G, 0.0), a0(G, 0.0);
+  for (int j = 0; j < N; j++) { Z[j] = d->primary_spec_Z[j]; a0[j] = d->primary_spec_a0[j]; }
+  L.o_Z = B.add(Z);
+  L.o_a0 = B.add(a0);
+  std::vector<int32_t> cspec((size_t)M * S0, -1);
+  std::vector<double> cnuv((size_t)M * S0, 0.0), cZ(M), ca0(M), clogK(M), ccoef((size_t)M * 5, 0.0);
+  for (int i = 0; i < M; i++) {
+    for (int s = 0; s < cnt(i); s++) { cspec[i * S0 + s] = cid(i, s); cnuv[i * S0 + s] = cnu(i, s); }
+    cZ[i] = d->eqcplx_Z[i];
+    ca0[i] = d->eqcplx_a0[i];
+    clogK[i] = d->eqcplx_logK[i];
+    if (d->eqcplx_logKcoef) for (int q = 0; q < 5; q++) ccoef[i * 5 + q] = d->eqcplx_logKcoef[i * 5 + q];
+  }
+  L.has_logk_coef = d->eqcplx_logKcoef != nullptr;
+  L.o_cspec = B.add(cspec);
+  L.o_cnu = B.add(cnuv);
+  L.o_cZ = B.add(cZ);
+  L.o_ca0 = B.add(ca0);
+  L.o_clogK = B.add(clogK);
+
+  // adjacency of each primary (its complexes) and the off-diagonal
+  // co-occurrence lists of dT_r/dm_k, per column k, per lane r
+  std::vector<std::vector<std::pair<int, double>>> adj(G);
+  std::vector<std::vector<std::vector<std::pair<int, double>>>> off(G, std::vector<std::vector<std::pair<int, double>>>(G));
+  for (int i = 0; i < M; i++)
+    for (int s = 0; s < cnt(i); s++) {
+      const int rr = cid(i, s);
+      adj[rr].push_back({i, cnu(i, s)});
+      for (int t2 = 0; t2 < cnt(i); t2++) {
+        const int kk = cid(i, t2);
+        if (kk == rr) continue;
+        off[kk][rr].push_back({i, cnu(i, s) * cnu(i, t2)});
+      }
+    }
+  // merge duplicate species within a complex would double-count: the
+  // reference sums stoichiometry slots independently, so lists keep every slot
+  int maxdeg = 0;
+  for (int r = 0; r < G; r++) maxdeg = std::max<int>(maxdeg, (int)adj[r].size());
+  L.maxdeg = maxdeg;
+  std::vector<int32_t> adj_c((size_t)std::max(maxdeg, 1) * G, M);
+  std::vector<double> adj_nu((size_t)std::max(maxdeg, 1) * G, 0.0);
+  for (int r = 0; r < G; r++)
+    for (size_t e = 0; e < adj[r].size(); e++) {
+      adj_c[e * G + r] = adj[r][e].first;
+      adj_nu[e * G + r] = adj[r][e].second;
+    }
+  L.o_adj_c = B.add(adj_c);
+  L.o_adj_nu = B.add(adj_nu);
+  int total_off = 0;
+  for (int k = 0; k < G; k++) {
+    L.off_start[k] = total_off;
+    int mx = 0;
+    for (int r = 0; r < G; r++) mx = std::max<int>(mx, (int)off[k][r].size());
+    total_off += mx;
+  }
+  L.off_start[G] = total_off;
+  for (int k = G + 1; k <= PFX_MAX_PRIMARY; k++) L.off_start[k] = total_off;
+  std::vector<int32_t> off_c((size_t)std::max(total_off, 1) * G, M);
+  std::vector<double> off_coef((size_t)std::max(total_off, 1) * G, 0.0);
+  for (int k = 0; k < G; k++)
+    for (int r = 0; r < G; r++)
+      for (size_t e = 0; e < off[k][r].size(); e++) {
+        const size_t row = L.off_start[k] + e;
+        off_c[row * G + r] = off[k][r][e].first;
+        off_coef[row * G + r] = off[k][r][e].second;
+      }
+  L.o_off_c = B.add(off_c);
+  L.o_off_coef = B.add(off_coef);
+
+  // minerals
+  auto mid = [&](int k, int s) { return d->kinmnrlspecid[(size_t)k * (S0 + 1) + 1 + s]; };
+  auto mcnt = [&](int k) { return d->kinmnrlspecid[(size_t)k * (S0 + 1)]; };
+  std::vector<int32_t> mspec((size_t)std::max(K, 1) * S0, -1);
+  std::vector<double> mnu((size_t)std::max(K, 1) * S0, 0.0), mnuG((size_t)std::max(K, 1) * G, 0.0);
+  std::vector<double> mlogK(std::max(K, 1)), mcoef((size_t)std::max(K, 1) * 5, 0.0), mk(std::max(K, 1)),
+      mmv(std::max(K, 1)), mEa(std::max(K, 1), 0.0), mthr(std::max(K, 1), 0.0), mlim(std::max(K, 1), 0.0),
+      mpow(std::max(K, 1), 1.0), mtk(std::max(K, 1), 1.0);
+  for (int k = 0; k < K; k++) {
+    if (mcnt(k) < 0 || mcnt(k) > S0) return PFX_E_ARG;
+    for (int s = 0; s < mcnt(k); s++) {
+      const int sp = mid(k, s);
+      if (sp < 0 || sp >= N) return PFX_E_ARG;
+      mspec[k * S0 + s] = sp;
+      mnu[k * S0 + s] = d->kinmnrlstoich[(size_t)k * S0 + s];
+      mnuG[k * G + sp] += d->kinmnrlstoich[(size_t)k * S0 + s];
+    }
+    mlogK[k] = d->kinmnrl_logK[k];
+    if (d->kinmnrl_logKcoef) for (int q = 0; q < 5; q++) mcoef[k * 5 + q] = d->kinmnrl_logKcoef[k * 5 + q];
+    mk[k] = d->kinmnrl_rate_constant[k];
+    mmv[k] = d->kinmnrl_molar_vol ? d->kinmnrl_molar_vol[k] : 0.0;
+    if (d->kinmnrl_activation_energy) mEa[k] = d->kinmnrl_activation_energy[k];
+    if (d->kinmnrl_affinity_threshold) mthr[k] = d->kinmnrl_affinity_threshold[k];
+    if (d->kinmnrl_rate_limiter) mlim[k] = d->kinmnrl_rate_limiter[k];
+    if (d->kinmnrl_affinity_power) mpow[k] = d->kinmnrl_affinity_power[k];
+    if (d->kinmnrl_Temkin_const) mtk[k] = d->kinmnrl_Temkin_const[k];
+  }
+  L.has_mnrl_coef = d->kinmnrl_logKcoef != nullptr;
+  L.mnrl_flags = (d->kinmnrl_affinity_threshold ? 1 : 0) | (d->kinmnrl_rate_limiter ? 2 : 0) |
+                 (d->kinmnrl_affinity_power ? 4 : 0) | (d->kinmnrl_Temkin_const ? 8 : 0);
+  L.o_mspec = B.add(mspec);
+  L.o_mnu = B.add(mnu);
+  L.o_mnuG = B.add(mnuG);
+  L.o_mlogK = B.add(mlogK);
+  L.o_mk = B.add(mk);
+  L.o_mEa = B.add(mEa);
+  L.o_mthr = B.add(mthr);
+  L.o_mlim = B.add(mlim);
+  L.o_mpow = B.add(mpow);
+  L.o_mtemkin = B.add(mtk);
+  // ion exchange
+  std::vector<double> ionx_k((size_t)std::max(R, 1) * G, 0.0);
+  std::vector<int32_t> ionx_role((size_t)std::max(R, 1) * G, -1);
+  for (int q = 0; q < R; q++) {
+    const int nc = d->eqionx_rxn_cationid[(size_t)q * (C0 + 1)];
+    if (nc < 1 || nc > C0) return PFX_E_ARG;
+    L.ionx_ncat[q] = nc;
+    L.ionx_CEC[q] = d->eqionx_rxn_CEC[q];
+    L.ionx_to_surf[q] = d->eqionx_rxn_to_surf ? d->eqionx_rxn_to_surf[q] : -1;
+    if (L.ionx_to_surf[q] >= K) return PFX_E_ARG;
+    for (int j = 0; j < nc; j++) {
+      const int sp = d->eqionx_rxn_cationid[(size_t)q * (C0 + 1) + 1 + j];
+      if (sp < 0 || sp >= N) return PFX_E_ARG;
+      if (ionx_role[q * G + sp] >= 0) return PFX_E_ARG;  // cation listed twice
+      ionx_role[q * G + sp] = j;
+      ionx_k[q * G + sp] = d->eqionx_rxn_k[(size_t)q * C0 + j];
+    }
+    const int ref = d->eqionx_rxn_cationid[(size_t)q * (C0 + 1) + 1];
+    L.ionx_ref_lane[q] = ref;
+    L.ionx_ref_k[q] = d->eqionx_rxn_k[(size_t)q * C0];
+    int zflag = 0;
+    for (int i = 0; i < nc && !zflag; i++)
+      for (int j = 0; j < nc; j++) {
+        const int a = d->eqionx_rxn_cationid[(size_t)q * (C0 + 1) + 1 + i];
+        const int b = d->eqionx_rxn_cationid[(size_t)q * (C0 + 1) + 1 + j];
+        if (std::fabs(d->primary_spec_Z[a] - d->primary_spec_Z[b]) > 0.1) { zflag = 1; break; }
+      }
+    L.ionx_zflag[q] = zflag;
+  }
+  L.o_ionx_k = B.add(ionx_k);
+  L.o_ionx_role = B.add(ionx_role);
+  // not needed inside the Newton loop: kept past the LDS prefix
+  L.lds_table_bytes = ((int64_t)B.bytes.size() + 15) & ~int64_t(15);
+  L.o_clogKcoef = B.add(ccoef);
+  L.o_mlogKcoef = B.add(mcoef);
+  L.o_mmolvol = B.add(mmv);
+  L.o_ch2o = B.add(std::vector<double>(std::max(M, 1), 0.0));
+  L.o_mh2o = B.add(std::vector<double>(std::max(K, 1), 0.0));
+  // the coefficient tables are read per cell at load time, from the LDS copy
+  // when they fit: stage the whole blob
+  B.bytes.resize((B.bytes.size() + 15) & ~size_t(15));
+  L.blob_bytes = (int64_t)B.bytes.size();
+  L.lds_table_bytes = L.blob_bytes;
+  L.debyeA = d->debyeA;
+  L.debyeB = d->debyeB;
+  L.debyeBdot = d->debyeBdot;
+  L.use_log = d->use_log_formulation;
+  L.act_freq = d->act_coef_update_frequency;
+  L.act_alg = d->act_coef_update_algorithm;
+  L.max_dlnC = d->max_dlnC;
+  L.max_rel_change_tol = d->max_relative_change_tolerance;
+  L.max_residual_tol = d->max_residual_tolerance;
+  const int64_t lds = L.lds_table_bytes + (int64_t)(BLOCK / G) * pfx_scr_doubles(M, G, K) * 8;
+  if (lds > kMaxLds) return PFX_E_UNSUP;
+
+  pfx_tables *t = new pfx_tables();
+  t->L = L;
+  HIPCHK(hipGetDevice(&t->device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, t->device));
+  t->num_cu = prop.multiProcessorCount;
+  HIPCHK(hipMalloc(&t->d_blob, (size_t)L.blob_bytes));
+  HIPCHK(hipMemcpy(t->d_blob, B.bytes.data(), (size_t)L.blob_bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&t->d_queue, 256));
+  HIPCHK(hipMalloc(&t->d_stats, 64 * sizeof(unsigned long long)));
+  HIPCHK(hipEventCreate(&t->ev0));
+  HIPCHK(hipEventCreate(&t->ev1));
+  *out = t;
+  return 0;
+}
+
+extern "C" int pfx_tables_destroy(pfx_tables *t) {
+  if (!t) return PFX_E_ARG;
+  (void)hipFree(t->d_blob);
+  (void)hipFree(t->d_queue);
+  (void)hipFree(t->d_stats);
+  if (t->d_iters) (void)hipFree(t->d_iters);
+  if (t->d_status) (void)hipFree(t->d_status);
+  if (t->d_relres) (void)hipFree(t->d_relres);
+  if (t->ev0) (void)hipEventDestroy(t->ev0);
+  if (t->ev1) (void)hipEventDestroy(t->ev1);
+  delete t;
+  return 0;
+}
+
+extern "C" int pfx_react_batch(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, pfx_stats *stats) {
+  if (!t) return PFX_E_ARG;
+  int rc = check_cells(t, cells, o);
+  if (rc) return rc;
+  if (o->max_iterations < 0 || o->extra_iterations < 0) return PFX_E_ARG;
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->ncells = cells->ncells;
+  }
+  if (cells->ncells == 0) return 0;
+  HIPCHK(hipSetDevice(t->device));
+  hipStream_t stream = (hipStream_t)o->stream;
+  pfx_kernel_params P;
+  fill_params(t, cells, o, P);
+  if ((rc = use_scratch(t, P, cells->ncells))) return rc;
+  double ms = 0.0;
+  if ((rc = launch(t, P, PFX_MODE_REACT, stream, &ms))) return rc;
+  return finish_stats(t, P, cells->ncells, stream, ms, stats);
+}
+
+extern "C" int pfx_residual_jacobian(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, double *residual,
+                                     double *jacobian) {
+  if (!t || !residual || !jacobian) return PFX_E_ARG;
+  int rc = check_cells(t, cells, o);
+  if (rc) return rc;
+  if (cells->ncells == 0) return 0;
+  HIPCHK(hipSetDevice(t->device));
+  pfx_kernel_params P;
+  fill_params(t, cells, o, P);
+  P.res_out = residual;
+  P.jac_out = jacobian;
+  P.num_iterations = nullptr;
+  P.status = nullptr;
+  P.rel_res = nullptr;
+  double ms = 0.0;
+  return launch(t, P, PFX_MODE_EVAL, (hipStream_t)o->stream, &ms);
+}
+
+extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int32_t *constraint_type,
+                                     const double *constraint_conc, int32_t molal, const pfx_react_opts *o,
+                                     pfx_stats *stats) {
+  if (!t || !cells || !o || !constraint_type || !constraint_conc || !cells->pri_molal || !cells->total)
+    return PFX_E_ARG;
+  if (cells->ncells < 0 || cells->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
+  if (t->L.R > 0 && !cells->total_sorb) return PFX_E_ARG;
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->ncells = cells->ncells;
+  }
+  int nph = 0;
+  for (int j = 0; j < t->L.N; j++) {
+    const int c = constraint_type[j];
+    if (c < PFX_CONSTRAINT_TOTAL || c > PFX_CONSTRAINT_PH) return PFX_E_UNSUP;
+    nph += c == PFX_CONSTRAINT_PH;
+  }
+  if (nph > 1) return PFX_E_ARG;
+  if (cells->ncells == 0) return 0;
+  HIPCHK(hipSetDevice(t->device));
+  hipStream_t stream = (hipStream_t)o->stream;
+  pfx_kernel_params P;
+  fill_params(t, cells, o, P);
+  P.constraint_conc = constraint_conc;
+  for (int j = 0; j < t->L.N; j++) P.ctype[j] = constraint_type[j];
+  P.molal = molal;
+  if (P.max_iterations <= 0) P.max_iterations = 10000;
+  int rc = use_scratch(t, P, cells->ncells);
+  if (rc) return rc;
+  double ms = 0.0;
+  if ((rc = launch(t, P, PFX_MODE_EQUIL, stream, &ms))) return rc;
+  return finish_stats(t, P, cells->ncells, stream, ms, stats);
+}
+
+extern "C" int pfx_auxvar_batch(pfx_tables *t, pfx_cells *cells, int32_t activity_passes, const pfx_react_opts *o) {
+  if (!t || !cells || !o || !cells->pri_molal || !cells->total || activity_passes < 0) return PFX_E_ARG;
+  if (cells->ncells < 0 || cells->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
+  if (t->L.R > 0 && !cells->total_sorb) return PFX_E_ARG;
+  if (cells->ncells == 0) return 0;
+  HIPCHK(hipSetDevice(t->device));
+  pfx_kernel_params P;
+  fill_params(t, cells, o, P);
+  P.aux_passes = activity_passes;
+  P.num_iterations = nullptr;
+  P.status = nullptr;
+  P.rel_res = nullptr;
+  double ms = 0.0;
+  return launch(t, P, PFX_MODE_AUX, (hipStream_t)o->stream, &ms);
+}

@@ -1,0 +1,83 @@
+// Device-side table layout and kernel parameters shared by the HIP kernels
+// and the host code of libpfx_chem.so.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/pfx_chem.h"
+
+#define PFX_LOG_TO_LN 2.30258509299       /* pflotran_constants.F90:48 */
+#define PFX_IDEAL_GAS 8.31446             /* pflotran_constants.F90:53 */
+
+// Tables derived once per reaction network by pfx_tables_create.  All arrays
+// live in one device allocation; offsets are in bytes from its base.
+//
+// Row-per-lane layout: a cell occupies a group of G lanes, lane r owns
+// primary species r (row r of the local Jacobian).  Per-lane tables are
+// stored [entry][G] so the G lanes of a group read consecutive words.
+struct pfx_dev_layout {
+  int32_t N, M, K, R, G, S;     // sizes; S = species slots per complex/mineral
+  int32_t Q;                    // complex rounds per lane = ceil(M/G)
+  int32_t maxdeg;               // max complexes touching one primary
+  int32_t off_start[PFX_MAX_PRIMARY + 1];  // per-column off-diagonal list offsets
+  int32_t has_logk_coef, has_mnrl_coef;
+  int32_t ionx_ncat[PFX_MAX_IONX_RXN];
+  int32_t ionx_zflag[PFX_MAX_IONX_RXN];
+  int32_t ionx_to_surf[PFX_MAX_IONX_RXN];
+  double ionx_CEC[PFX_MAX_IONX_RXN];
+  double ionx_ref_k[PFX_MAX_IONX_RXN];
+  int32_t ionx_ref_lane[PFX_MAX_IONX_RXN];
+  double debyeA, debyeB, debyeBdot;
+  int32_t use_log, act_freq, act_alg;
+  double max_dlnC, max_rel_change_tol, max_residual_tol;
+  int32_t mnrl_flags;           // bit0 threshold, bit1 limiter, bit2 power, bit3 Temkin
+
+  // byte offsets into the table blob (doubles unless noted)
+  int64_t o_Z, o_a0;                 // [G]
+  int64_t o_cspec;                   // int32 [M][S]  (-1 = empty slot)
+  int64_t o_cnu;                     // [M][S]
+  int64_t o_cZ, o_ca0;               // [M]
+  int64_t o_clogK;                   // [M] log10 K at the reference temperature
+  int64_t o_clogKcoef;               // [M][5]
+  int64_t o_ch2o;                    // [M]
+  int64_t o_adj_c;                   // int32 [maxdeg][G], -> M = zero slot
+  int64_t o_adj_nu;                  // [maxdeg][G]
+  int64_t o_off_c;                   // int32 [off_total][G]
+  int64_t o_off_coef;                // [off_total][G]
+  int64_t o_mspec;                   // int32 [K][S]
+  int64_t o_mnu;                     // [K][S]
+  int64_t o_mnuG;                    // [K][G] dense stoichiometry per lane
+  int64_t o_mlogK, o_mlogKcoef, o_mh2o;
+  int64_t o_mk, o_mmolvol, o_mEa, o_mthr, o_mlim, o_mpow, o_mtemkin;  // [K]
+  int64_t o_ionx_k;                  // [R][G] selectivity of the lane's cation (0 if none)
+  int64_t o_ionx_role;               // int32 [R][G] cation slot (0 = reference) or -1
+  int64_t blob_bytes;
+  int64_t lds_table_bytes;           // prefix of the blob staged into LDS
+};
+
+enum { PFX_MODE_REACT = 0, PFX_MODE_EVAL = 1, PFX_MODE_EQUIL = 2, PFX_MODE_AUX = 3 };
+
+struct pfx_kernel_params {
+  pfx_dev_layout L;
+  const unsigned char *tables;    // device blob
+  // cells
+  int64_t ncells;
+  double *total, *total_sorb, *pri_molal, *pri_act_coef, *sec_act_coef, *sec_molal, *ionx_ref;
+  const double *mnrl_volfrac, *mnrl_area;
+  double *mnrl_rate;
+  const double *temperature, *porosity, *saturation, *volume, *den_kg, *guess;
+  int32_t *num_iterations, *status;
+  double *rel_res;
+  // options
+  double dt, rel_tol, default_guess;
+  double def_porosity, def_saturation, def_volume, def_den;
+  int32_t max_iterations, extra_iterations, guess_mode, reference_stop;
+  double *res_out, *jac_out;    // residual/Jacobian outputs
+  // constraint speciation
+  const double *constraint_conc;   // [ncells*N]
+  int32_t ctype[PFX_MAX_PRIMARY];  // PFX_CONSTRAINT_* per component
+  int32_t molal;                   // constraint concentrations in mol/kg
+  int32_t aux_passes;              // activity passes of the aux-var refresh
+  // work queue
+  unsigned int *queue;          // global cell cursor
+  int32_t chunk;                // cells taken per fetch
+};

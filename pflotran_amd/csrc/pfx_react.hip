@@ -1,0 +1,757 @@
+// Batched per-cell reaction step for MI355X (gfx950), FP64.
+//
+// One cell per group of G lanes (G = 8, 16 or 32 >= N primaries); lane r
+// owns primary species r: its free ion, its total, row r of the local
+// Jacobian and row r of the LU factors.  Reaction tables are staged in LDS
+// once per workgroup; per-cell complex data (concentrations, activity
+// coefficients, log K) live in a per-group LDS slice.  A persistent grid
+// pulls cells from a device work queue in chunks, and a group whose cell
+// converged refills from the queue while the other groups of its wavefront
+// keep iterating, so cells with 2 or 25 Newton iterations share one launch.
+//
+// Reference path (src/pflotran/):
+//   RReact                reaction.F90:3322-3511   (Newton loop, log update)
+//   RTAuxVarCompute       reaction.F90:4969-5006
+//   RTotal                reaction.F90:4057-4158   (phases A/B below)
+//   RActivityCoefficients reaction.F90:3992-4051   (LAG)
+//   RTotalSorbEqIonx      reaction.F90:4305-4535
+//   RKineticMineral       reaction_mineral.F90:564-1000 (no prefactor species)
+//   RSolve/ludcmp/lubksb  reaction.F90:4835-4880, utility.F90:393-523
+//   RTAccumulation(Derivative), RAccumulationSorb(Derivative)
+//                         reaction.F90:5072-5232, 4539-4603
+// The residual is the dt-consistent per-cell residual of the global-implicit
+// path (reactive_transport.F90:2539-2587, 2735-2758): (accum - accum_old)/dt
+// + mineral rates, in mol/s.
+#include <hip/hip_runtime.h>
+
+#include "pfx_device.h"
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ double bperm_d(double v, int src_lane) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, lo);
+  hi = __builtin_amdgcn_ds_bpermute(src_lane << 2, hi);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ int bperm_i(int v, int src_lane) {
+  return __builtin_amdgcn_ds_bpermute(src_lane << 2, v);
+}
+
+// group-wide reductions by xor butterfly: every lane of the group ends with
+// the bitwise-same value (each step combines a commutative pair).
+template <int G>
+__device__ __forceinline__ double gsum(double v, int lane) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += bperm_d(v, lane ^ o);
+  return v;
+}
+template <int G>
+__device__ __forceinline__ double gmax(double v, int lane) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    double w = bperm_d(v, lane ^ o);
+    v = (w > v || w != w) ? w : v;  // NaN wins: a broken cell is seen as broken
+  }
+  return v;
+}
+template <int G>
+__device__ __forceinline__ double gmin(double v, int lane) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v = fmin(v, bperm_d(v, lane ^ o));
+  return v;
+}
+// argmax with ties resolved to the larger row index
+template <int G>
+__device__ __forceinline__ void gargmax(double &v, int &idx, int lane) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    double w = bperm_d(v, lane ^ o);
+    int j = bperm_i(idx, lane ^ o);
+    if (w > v || (w == v && j > idx)) {
+      v = w;
+      idx = j;
+    }
+  }
+}
+
+__device__ __forceinline__ void wave_sync() {
+  // LDS written by other lanes of this wavefront becomes visible to the
+  // whole wavefront (lanes of one group never span wavefronts).
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS accessors: every table and scratch array is addressed by a 32-bit
+// byte offset into the dynamic LDS segment, so all accesses stay ds_read /
+// ds_write and no 64-bit pointers are held in registers.
+extern __shared__ __align__(16) unsigned char smem[];
+#define LD(off, i) (((const double *)(smem + (off)))[(i)])
+#define LI(off, i) (((const int32_t *)(smem + (off)))[(i)])
+#define SD(off, i) (((double *)(smem + (off)))[(i)])
+
+// per-group scratch slice (doubles), as byte offsets from the slice base:
+//   C[M+1] complex molalities (C[M] = 0, padding slot), gsec[M] complex
+//   activity coefficients, lnK[M] = -log10 K * LOG_TO_LN of this cell,
+//   la[G] ln activities, mv[G] free ions, im[G] inverse free ions,
+//   xb[G] ion-exchange fractions, mlnK/mkeff/marea/mvf/mrate[K] per mineral
+struct Scr {
+  int C, gsec, lnK, la, mv, im, xb, mlnK, mkeff, marea, mvf, mrate;
+};
+
+__host__ __device__ inline int scr_doubles(int M, int G, int K) {
+  int n = (M + 1) + 2 * M + 4 * G + 5 * K;
+  return (n + 1) & ~1;
+}
+
+__device__ __forceinline__ Scr make_scr(int base, int M, int G, int K) {
+  Scr s;
+  int p = base;
+  s.C = p; p += (M + 1) * 8;
+  s.gsec = p; p += M * 8;
+  s.lnK = p; p += M * 8;
+  s.la = p; p += G * 8;
+  s.mv = p; p += G * 8;
+  s.im = p; p += G * 8;
+  s.xb = p; p += G * 8;
+  s.mlnK = p; p += K * 8;
+  s.mkeff = p; p += K * 8;
+  s.marea = p; p += K * 8;
+  s.mvf = p; p += K * 8;
+  s.mrate = p;
+  return s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Kernel modes (template MODE):
+//   PFX_MODE_REACT  reaction step, RReact (reaction.F90:3322-3511)
+//   PFX_MODE_EVAL   residual + Jacobian at given free ions (reaction.F90:3419-3441)
+//   PFX_MODE_EQUIL  constraint speciation, ReactionEquilibrateConstraint
+//                   (reaction.F90:1308-2046), TOTAL/FREE/LOG/PH rows
+//   PFX_MODE_AUX    aux-var refresh, RTUpdateAuxVars (reactive_transport.F90:3790-3846)
+//
+// Phases of a cell in REACT: 0 = RTotal at the previous state, 1 = activity
+// update pending, 2 = Newton.  Other modes start in phase 2.
+template <int G, int MODE>
+__global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_params Pk) {
+  // Parameters are read through a pointer that is made opaque once per pass
+  // of the persistent loop, so the ~100 uniform parameters are re-read from
+  // the kernarg segment (scalar cache) where they are used instead of all
+  // being held in SGPRs across the whole loop.
+  typedef const __attribute__((address_space(4))) pfx_kernel_params *KP;
+  uint64_t praw = (uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr();
+  const int N = Pk.L.N, M = Pk.L.M, K = Pk.L.K, R = Pk.L.R;
+  const int lds_tab = (int)Pk.L.lds_table_bytes;
+
+  // stage the reaction tables into LDS (16-byte words)
+  {
+    const int4 *src = (const int4 *)Pk.tables;
+    int4 *dst = (int4 *)smem;
+    const int nw = lds_tab >> 4;
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int r = threadIdx.x & (G - 1);
+  const int gbase = lane - r;
+  const bool live = r < N;  // padding lanes carry an identity row
+  const int sdoubles = scr_doubles(M, G, K);
+  const Scr S = make_scr(lds_tab + (int)(threadIdx.x / G) * sdoubles * 8, M, G, K);
+  if (r == 0) SD(S.C, M) = 0.0;
+
+  const double Zr = live ? LD(Pk.L.o_Z, r) : 0.0;
+  const double a0r = live ? LD(Pk.L.o_a0, r) : 0.0;
+
+  // wave-level work queue state (identical in all lanes of the wave)
+  long long q_next = 0, q_end = 0;
+  bool exhausted = false;
+
+  // per-group cell state
+  long long cell = -1;
+  int phase = 0, it = 0, extra = 0;
+  bool stop_next = false;   // reference relative-change stop: finish at the next evaluation
+  int cell_flag = PFX_CELL_CONVERGED;
+  double m = 1.0, gam = 1.0, lng = 0.0, fixed = 0.0, tot_out = 0.0, sorb_out = 0.0;
+  double porosity = 0, sat = 0, vol = 0, den = 0;
+  // constraint speciation state
+  int ctype = PFX_CONSTRAINT_FREE, nit_on = 0;
+  bool compute_act = false;
+  double tcon = 0.0, free_conc = 1e-9;
+  double ionx_state[PFX_MAX_IONX_RXN];
+#pragma unroll
+  for (int q = 0; q < PFX_MAX_IONX_RXN; q++) ionx_state[q] = 1e-9;
+
+  for (;;) {
+    asm volatile("" : "+s"(praw));
+    KP Pp = (KP)praw;
+#define P (*Pp)
+#define L (Pp->L)
+    // ---------------------------------------------------------------
+    // refill: groups without a cell take the next ones from the queue
+    {
+      const bool need = (cell < 0) && !exhausted;
+      unsigned long long mask = __ballot(need && r == 0);
+      int nneed = __popcll(mask);
+      int rank = __popcll(mask & ((1ull << lane) - 1ull));
+      long long mycell = -1;
+      while (nneed > 0) {
+        long long avail = q_end - q_next;
+        if (avail <= 0) {
+          unsigned int base = 0;
+          if (lane == 0) base = atomicAdd(P.queue, (unsigned int)P.chunk);
+          base = __shfl(base, 0);
+          q_next = base;
+          q_end = (long long)base + P.chunk;
+          if (q_end > P.ncells) q_end = P.ncells;
+          if (q_next >= P.ncells) {
+            exhausted = true;
+            break;
+          }
+          continue;
+        }
+        long long take = nneed < avail ? nneed : avail;
+        if (need && r == 0 && rank >= 0 && rank < take) mycell = q_next + rank;
+        rank -= (int)take;
+        nneed -= (int)take;
+        q_next += take;
+      }
+      if (need) {
+        int lo = bperm_i((int)(mycell & 0xffffffff), gbase);
+        int hi = bperm_i((int)(mycell >> 32), gbase);
+        long long c = ((long long)hi << 32) | (unsigned int)lo;
+        if (c >= 0) {
+          cell = c;
+          phase = (MODE == PFX_MODE_REACT) ? 0 : 2;
+          it = 0;
+          extra = 0;
+          stop_next = false;
+          cell_flag = PFX_CELL_CONVERGED;
+          compute_act = false;
+          nit_on = 0;
+          // ---- load the cell
+          porosity = P.porosity ? P.porosity[c] : P.def_porosity;
+          sat = P.saturation ? P.saturation[c] : P.def_saturation;
+          vol = P.volume ? P.volume[c] : P.def_volume;
+          den = P.den_kg ? P.den_kg[c] : P.def_den;
+          const double tempc = P.temperature ? P.temperature[c] : 25.0;
+          const double tk = tempc + 273.15;
+          if (MODE == PFX_MODE_EQUIL) {
+            gam = 1.0;
+            for (int cc = r; cc < M; cc += G) SD(S.gsec, cc) = 1.0;
+            const double molal_to_molar = P.molal ? den / 1000.0 : 1.0;
+            const double molar_to_molal = P.molal ? 1.0 : 1000.0 / den;
+            ctype = live ? P.ctype[r] : PFX_CONSTRAINT_FREE;
+            const double conc = live ? P.constraint_conc[c * N + r] : 0.0;
+            free_conc = 1.0e-9;
+            tcon = 0.0;
+            if (ctype == PFX_CONSTRAINT_TOTAL) tcon = conc * molal_to_molar;
+            else if (ctype == PFX_CONSTRAINT_FREE) free_conc = conc * molar_to_molal;
+            else if (ctype == PFX_CONSTRAINT_LOG) free_conc = pow(10.0, conc) * molar_to_molal;
+            else if (ctype == PFX_CONSTRAINT_PH) { free_conc = pow(10.0, -conc); tcon = conc; }
+            m = live ? free_conc : 1.0;
+          } else {
+            if (live) {
+              m = P.pri_molal[c * N + r];
+              gam = P.pri_act_coef ? P.pri_act_coef[c * N + r] : 1.0;
+              if (MODE != PFX_MODE_AUX) {
+                const double tot_in = P.total[c * N + r];
+                const double sorb_old = (R > 0 && P.total_sorb) ? P.total_sorb[c * N + r] : 0.0;
+                fixed = porosity * sat * 1000.0 * vol * tot_in + (R > 0 ? sorb_old * vol : 0.0);
+              }
+            } else {
+              m = 1.0;
+              gam = 1.0;
+              fixed = 0.0;
+            }
+            for (int cc = r; cc < M; cc += G) SD(S.gsec, cc) = P.sec_act_coef ? P.sec_act_coef[c * M + cc] : 1.0;
+          }
+          for (int cc = r; cc < M; cc += G) {
+            double lk = LD(L.o_clogK, cc);
+            if (P.temperature && L.has_logk_coef) {
+              const int cfo = (int)L.o_clogKcoef + 5 * cc * 8;
+              lk = LD(cfo, 0) * log(tk) + LD(cfo, 1) + LD(cfo, 2) * tk + LD(cfo, 3) / tk + LD(cfo, 4) / (tk * tk);
+            }
+            SD(S.lnK, cc) = -lk * PFX_LOG_TO_LN;
+          }
+          for (int k = r; k < K; k += G) {
+            double lk = LD(L.o_mlogK, k);
+            if (P.temperature && L.has_mnrl_coef) {
+              const int cfo = (int)L.o_mlogKcoef + 5 * k * 8;
+              lk = LD(cfo, 0) * log(tk) + LD(cfo, 1) + LD(cfo, 2) * tk + LD(cfo, 3) / tk + LD(cfo, 4) / (tk * tk);
+            }
+            SD(S.mlnK, k) = -lk * PFX_LOG_TO_LN;
+            double arr = 1.0;
+            const double Ea = LD(L.o_mEa, k);
+            if (Ea > 0.0) arr = exp(Ea / PFX_IDEAL_GAS * (1.0 / (25.0 + 273.15) - 1.0 / (tempc + 273.15)));
+            SD(S.mkeff, k) = LD(L.o_mk, k) * arr;
+            SD(S.marea, k) = P.mnrl_area ? P.mnrl_area[c * K + k] : 0.0;
+            SD(S.mvf, k) = P.mnrl_volfrac ? P.mnrl_volfrac[c * K + k] : 0.0;
+            SD(S.mrate, k) = 0.0;
+          }
+#pragma unroll
+          for (int q = 0; q < PFX_MAX_IONX_RXN; q++)
+            ionx_state[q] = (MODE != PFX_MODE_EQUIL && q < R && P.ionx_ref) ? P.ionx_ref[c * R + q] : 1e-9;
+          lng = log(gam);
+          if (MODE == PFX_MODE_REACT && L.act_freq == PFX_ACT_OFF) {
+            phase = 2;
+            if (live) {
+              if (P.guess_mode == PFX_GUESS_ARRAY) m = P.guess[c * N + r];
+              else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
+            }
+          }
+        }
+      }
+    }
+    if (__ballot(cell >= 0) == 0ull) break;
+    wave_sync();
+    const bool active = cell >= 0;
+
+    // ---------------------------------------------------------------
+    // activity coefficients from the current free ions and the complex
+    // molalities of the previous RTotal (LAG form, reaction.F90:3992-4051)
+    bool act_now = false;
+    if (MODE == PFX_MODE_REACT)
+      act_now = phase == 1 || (phase == 2 && L.act_freq == PFX_ACT_NEWTON_ITER);
+    else if (MODE == PFX_MODE_EQUIL)
+      act_now = compute_act;
+    else if (MODE == PFX_MODE_AUX)
+      act_now = it >= 1;
+    if (active && act_now) {
+      double part = live ? m * Zr * Zr : 0.0;
+      for (int cc = r; cc < M; cc += G) part += SD(S.C, cc) * LD(L.o_cZ, cc) * LD(L.o_cZ, cc);
+      const double I = 0.5 * gsum<G>(part, lane);
+      const double sqI = sqrt(I);
+      const double A = L.debyeA, B = L.debyeB, Bd = L.debyeBdot;
+      if (live) {
+        gam = fabs(Zr) > 1e-10 ? exp((-Zr * Zr * sqI * A / (1.0 + a0r * B * sqI) + Bd * I) * PFX_LOG_TO_LN)
+                               : 1.0;
+        lng = log(gam);
+      }
+      for (int cc = r; cc < M; cc += G) {
+        const double z = LD(L.o_cZ, cc);
+        SD(S.gsec, cc) = fabs(z) > 1e-10
+                             ? exp((-z * z * sqI * A / (1.0 + LD(L.o_ca0, cc) * B * sqI) + Bd * I) * PFX_LOG_TO_LN)
+                             : 1.0;
+      }
+      if (MODE == PFX_MODE_REACT && phase == 1) {
+        phase = 2;
+        if (live) {
+          if (P.guess_mode == PFX_GUESS_ARRAY) m = P.guess[cell * N + r];
+          else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
+        }
+      }
+    }
+    if (MODE == PFX_MODE_EQUIL && active && live &&
+        (ctype == PFX_CONSTRAINT_FREE || ctype == PFX_CONSTRAINT_LOG))
+      m = free_conc;  // reaction.F90:1552-1558
+
+    // ---------------------------------------------------------------
+    // RTotal, phase A: complex molalities, complex c computed by lane c % G
+    if (active) {
+      SD(S.la, r) = live ? log(m) + lng : 0.0;
+      SD(S.mv, r) = m;
+      SD(S.im, r) = 1.0 / m;
+    }
+    wave_sync();
+    if (active) {
+      for (int cc = r; cc < M; cc += G) {
+        double lnQK = SD(S.lnK, cc);
+        for (int s = 0; s < L.S; s++) {
+          const int sp = LI(L.o_cspec, cc * L.S + s);
+          if (sp >= 0) lnQK += LD(L.o_cnu, cc * L.S + s) * SD(S.la, sp);
+        }
+        SD(S.C, cc) = exp(lnQK) / SD(S.gsec, cc);
+      }
+    }
+    wave_sync();
+
+    // phase B: totals, |contributions|, and row r of dT/dm (kg/L)
+    double row[G];
+    double Tsum = 0.0, absum = 0.0;
+    const double den_kg_per_L = den * 1.0e-3;
+    if (active) {
+      double diag = 0.0;
+      const int maxdeg = L.maxdeg;
+      for (int e = 0; e < maxdeg; e++) {
+        const int cc = LI(L.o_adj_c, e * G + r);
+        const double nu = LD(L.o_adj_nu, e * G + r);
+        const double Cc = SD(S.C, cc);
+        Tsum += nu * Cc;
+        absum += fabs(nu) * Cc;
+        diag += nu * nu * Cc;
+      }
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+        double acc = 0.0;
+        const int e0 = L.off_start[k], e1 = L.off_start[k + 1];
+        for (int e = e0; e < e1; e++)
+          acc += LD(L.o_off_coef, e * G + r) * SD(S.C, LI(L.o_off_c, e * G + r));
+        row[k] = (k == r) ? (live ? 1.0 + diag * SD(S.im, k) : 1.0) : acc * SD(S.im, k);
+      }
+#pragma unroll
+      for (int k = 0; k < G; k++) row[k] = live ? row[k] * den_kg_per_L : row[k];
+      tot_out = (m + Tsum) * den_kg_per_L;
+    }
+    if (MODE == PFX_MODE_REACT && active && phase == 0) {
+      phase = 1;  // complexes of the previous state are in LDS: update activity next pass
+      wave_sync();
+      continue;
+    }
+    if (MODE == PFX_MODE_AUX && active && it < P.aux_passes) {
+      it++;
+      wave_sync();
+      continue;
+    }
+
+    // ---------------------------------------------------------------
+    // equilibrium ion exchange (RTotalSorbEqIonx): sorbed total of row r and
+    // d(sorbed)/dm added to row[] with weight ionx_w
+    const bool need_ionx = (MODE == PFX_MODE_REACT || MODE == PFX_MODE_EVAL) ? true : false;
+    const bool ionx_final = (MODE == PFX_MODE_AUX || MODE == PFX_MODE_EQUIL);
+    double Sr = 0.0;
+    const double dt = P.dt;
+    const double psv_t = porosity * sat * 1000.0 * vol;
+    if (active && R > 0 && (need_ionx || ionx_final)) {
+      const double ionx_w = need_ionx ? vol / dt : 0.0;
+      for (int q = 0; q < R; q++) {
+        const int role = live ? LI(L.o_ionx_role, q * G + r) : -1;
+        const double kq = live ? LD(L.o_ionx_k, q * G + r) : 0.0;
+        double omega = L.ionx_CEC[q];
+        if (L.ionx_to_surf[q] >= 0) omega = fmax(L.ionx_CEC[q] * SD(S.mvf, L.ionx_to_surf[q]), 1e-40);
+        double X = 0.0;
+        if (L.ionx_zflag[q]) {
+          const int rl = L.ionx_ref_lane[q];
+          const double ref_conc = bperm_d(m * gam, gbase + rl);
+          const double ref_Z = LD(L.o_Z, rl);
+          const double ref_k = L.ionx_ref_k[q];
+          double ref_X = ref_Z * ionx_state[q] / omega;
+          double KDj = ref_X / (ref_k * ref_conc);
+          bool one_more = false;
+          const double zratio = Zr / ref_Z;
+          int iit = 0;
+          for (;;) {
+            if (++iit > 20000) {  // reference: error stop (reaction.F90:4382-4385)
+              cell_flag = PFX_CELL_MAX_ITERATIONS;
+              break;
+            }
+            ref_X = KDj * (ref_k * ref_conc);
+            X = role == 0 ? ref_X : (role > 0 ? kq * m * gam * pow(KDj, zratio) : 0.0);
+            const double total = gsum<G>(X, lane);
+            double dres = gsum<G>(role > 0 ? X / KDj * Zr : 0.0, lane);
+            dres = dres / ref_Z + (ref_k * ref_conc);
+            const double res = 1.0 - total;
+            if (one_more) break;
+            const double delta = res / dres;
+            KDj = KDj + delta;
+            KDj = fmax(KDj, 1e-40);
+            if (fabs(delta / KDj) < 1e-12) one_more = true;
+          }
+          ionx_state[q] = ref_X * omega / ref_Z;
+        } else {
+          X = role >= 0 ? m * gam * kq : 0.0;
+          const double sumkm = gsum<G>(X, lane);
+          X = X / sumkm;
+        }
+        const double sumZX = gsum<G>(Zr * X, lane);
+        const double t1 = role >= 0 ? X * omega / Zr : 0.0;
+        const double t2 = Zr / sumZX;
+        Sr += t1;
+        if (need_ionx) {
+          SD(S.xb, r) = X;
+          wave_sync();
+#pragma unroll
+          for (int k = 0; k < G; k++) {
+            const double Xk = SD(S.xb, k);
+            const double d = (k == r) ? t1 * (1.0 - (t2 * Xk)) : (-t1) * t2 * Xk;
+            row[k] += d * SD(S.im, k) * ionx_w / (live ? psv_t / dt : 1.0);
+          }
+          wave_sync();
+        }
+      }
+    }
+    sorb_out = Sr;
+
+    // ---------------------------------------------------------------
+    // residual row, Jacobian row, convergence test
+    double Rr = 0.0, scale = 1.0;
+    bool finish = false;
+    int status = PFX_CELL_CONVERGED;
+    double rr = 0.0;
+    int use_log = L.use_log;
+    if (MODE == PFX_MODE_AUX && active) finish = true;
+    if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EVAL) && active) {
+      // accumulation (mol/s) and its derivative: the dt-consistent form
+#pragma unroll
+      for (int k = 0; k < G; k++) row[k] = live ? row[k] * (psv_t / dt) : row[k];
+      if (live) {
+        Rr = (psv_t * tot_out + (R > 0 ? Sr * vol : 0.0)) / dt - fixed / dt;
+        const double psv = porosity * sat * den * vol;
+        scale = psv * (m + absum) / dt + fabs(Sr) * vol / dt + fabs(fixed) / dt;
+      }
+      // kinetic minerals (group-uniform rates, row-local residual/Jacobian)
+      for (int k = 0; k < K; k++) {
+        double lnQK = SD(S.mlnK, k);
+        for (int s = 0; s < L.S; s++) {
+          const int sp = LI(L.o_mspec, k * L.S + s);
+          if (sp >= 0) lnQK += LD(L.o_mnu, k * L.S + s) * SD(S.la, sp);
+        }
+        const double QK = lnQK <= 6.90776 ? exp(lnQK) : 1.0e3;
+        const bool temkin = L.mnrl_flags & 8;
+        const double tk_ = temkin ? LD(L.o_mtemkin, k) : 1.0;
+        double aff = temkin ? 1.0 - pow(QK, 1.0 / tk_) : 1.0 - QK;
+        const double sgn = copysign(1.0, aff);
+        double Im = 0.0;
+        if (SD(S.mvf, k) > 0.0 || sgn < 0.0) {
+          const double thr = (L.mnrl_flags & 1) ? LD(L.o_mthr, k) : 0.0;
+          if (!(thr > 0.0 && sgn < 0.0 && QK < thr)) {
+            const double lim = (L.mnrl_flags & 2) ? LD(L.o_mlim, k) : 0.0;
+            if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
+            const double rate = SD(S.mkeff, k);
+            double Im_const = -SD(S.marea, k);
+            const bool pw = L.mnrl_flags & 4;
+            Im = pw ? Im_const * sgn * pow(fabs(aff), LD(L.o_mpow, k)) * rate : Im_const * sgn * fabs(aff) * rate;
+            if (r == 0) SD(S.mrate, k) = Im;
+            Im_const *= vol;
+            const double ImV = Im * vol;
+            const double nur = live ? LD(L.o_mnuG, k * G + r) : 0.0;
+            Rr += nur * ImV;
+            double dIm_dQK = pw ? -ImV * LD(L.o_mpow, k) / fabs(aff) : -Im_const * rate;
+            if (temkin) dIm_dQK = dIm_dQK * (1.0 / tk_) / QK * (1.0 - aff);
+            double f = nur * dIm_dQK * QK * den * 1.0e-3;
+            if (lim > 0.0) {
+              const double dd = 1.0 + (1.0 - aff) / lim;
+              f = f * (1.0 + QK / lim / dd) / dd;
+            }
+#pragma unroll
+            for (int kk = 0; kk < G; kk++) row[kk] += f * LD(L.o_mnuG, k * G + kk) * SD(S.im, kk);
+          } else if (r == 0) {
+            SD(S.mrate, k) = 0.0;
+          }
+        } else if (r == 0) {
+          SD(S.mrate, k) = 0.0;
+        }
+      }
+      rr = gmax<G>(live ? fabs(Rr) / fmax(scale, 1e-300) : 0.0, lane);
+      if (MODE == PFX_MODE_EVAL) {
+        finish = true;
+      } else if (cell_flag != PFX_CELL_CONVERGED) {
+        finish = true;
+        status = cell_flag;
+      } else if (!(rr == rr) || rr == INFINITY) {
+        finish = true;
+        status = PFX_CELL_NONFINITE;
+      } else if (stop_next) {
+        finish = true;
+      } else {
+        if (rr <= P.rel_tol) {
+          if (extra >= P.extra_iterations) finish = true;
+          else extra++;
+        }
+        if (!finish && P.reference_stop) {
+          const double mx = gmax<G>(live ? fabs(Rr * dt) : 0.0, lane);
+          if (mx < L.max_residual_tol) finish = true;
+        }
+        if (!finish && it >= P.max_iterations) {
+          finish = true;
+          status = PFX_CELL_MAX_ITERATIONS;
+        }
+      }
+    }
+    if (MODE == PFX_MODE_EQUIL && active) {
+      // constraint rows (reaction.F90:1597-1660)
+      if (ctype == PFX_CONSTRAINT_TOTAL) {
+        Rr = tot_out - tcon;
+      } else {
+        Rr = 0.0;
+        if (ctype == PFX_CONSTRAINT_PH && live) m = pow(10.0, -tcon) / gam;
+#pragma unroll
+        for (int k = 0; k < G; k++) row[k] = (k == r) ? 1.0 : 0.0;
+      }
+      if (ctype == PFX_CONSTRAINT_PH) SD(S.mv, r) = m;
+      if (it > 3 && it < 9) use_log = use_log ? (it % 2 == 0) : 0;  // reaction.F90:1797-1804
+      rr = gmax<G>(live ? fabs(Rr) : 0.0, lane);  // max |Res| (mol/L)
+      if (!(rr == rr) || rr == INFINITY) {
+        finish = true;
+        status = PFX_CELL_NONFINITE;
+      } else if (stop_next) {
+        finish = true;
+      }
+    }
+    wave_sync();
+
+    // ---------------------------------------------------------------
+    // RSolve: row equilibration, ln-column scaling, pivoted LU, solve
+    double upd = 0.0;
+    if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
+      double b = 0.0;
+      if (live) {
+        double mx = 0.0;
+#pragma unroll
+        for (int k = 0; k < G; k++) mx = fmax(mx, fabs(row[k]));
+        const double norm = 1.0 / fmax(1.0, mx);
+        b = Rr * norm;
+#pragma unroll
+        for (int k = 0; k < G; k++) row[k] *= norm;
+        if (use_log) {
+#pragma unroll
+          for (int k = 0; k < G; k++) row[k] *= SD(S.mv, k);
+        }
+      }
+      double vv = 0.0;
+      {
+        double mx = 0.0;
+#pragma unroll
+        for (int k = 0; k < G; k++) mx = fmax(mx, fabs(row[k]));
+        vv = mx > 0.0 ? 1.0 / mx : 0.0;
+        if (live && !(mx > 0.0)) status = PFX_CELL_SINGULAR;
+      }
+      bool pivoted = !live;
+      int plane[G];
+#pragma unroll
+      for (int j = 0; j < G; j++) {
+        plane[j] = 0;
+        if (j < N) {
+          double cand = pivoted ? -1.0 : vv * fabs(row[j]);
+          int p = r;
+          gargmax<G>(cand, p, lane);
+          plane[j] = p;
+          const int src = gbase + p;
+          // stream the pivot row one element at a time
+          double piv = bperm_d(row[j], src);
+          if (piv == 0.0) piv = 1.0e-20;
+          const bool elim = (r != p) && !pivoted;
+          const double f = elim ? row[j] / piv : 0.0;
+          if (r == p) {
+            pivoted = true;
+            row[j] = piv;
+          } else if (elim) {
+            row[j] = f;
+          }
+#pragma unroll
+          for (int k = j + 1; k < G; k++) {
+            const double pk = bperm_d(row[k], src);
+            if (elim) row[k] -= f * pk;
+          }
+          const double pb = bperm_d(b, src);
+          if (elim) b -= f * pb;
+        }
+      }
+      // back substitution: the pivot row of step j is held by lane plane[j];
+      // rows pivoted before step j still need x_j, later ones are finished
+#pragma unroll
+      for (int j = G - 1; j >= 0; j--) {
+        if (j < N) {
+          const int pl = plane[j];
+          double xj = (r == pl) ? b / row[j] : 0.0;
+          xj = bperm_d(xj, gbase + pl);
+          if (r == j) upd = xj;
+          b -= row[j] * xj;
+        }
+      }
+      const double bad = gmax<G>(status == PFX_CELL_SINGULAR ? 1.0 : 0.0, lane);
+      if (bad > 0.0) {
+        finish = true;  // singular Jacobian row (reference: MPI_Abort)
+        status = PFX_CELL_SINGULAR;
+      }
+    }
+
+    // ---------------------------------------------------------------
+    // Newton update (reaction.F90:3454-3470; constraint form :1806-1826)
+    if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
+      const double mprev = m;
+      if (use_log) {
+        const double u = copysign(1.0, upd) * fmin(fabs(upd), L.max_dlnC);
+        if (live) m = m * exp(-u);
+      } else {
+        double ratio = (live && m <= upd) ? fabs(m / upd) : 1e20;
+        const double min_ratio = gmin<G>(ratio, lane);
+        if (MODE == PFX_MODE_EQUIL ? (min_ratio <= 1.0) : (min_ratio < 1.0)) upd = upd * min_ratio * 0.99;
+        if (live) m = m - upd;
+      }
+      const double rel = gmax<G>(live ? fabs((m - mprev) / mprev) : 0.0, lane);
+      it++;
+      if (MODE == PFX_MODE_REACT) {
+        if (P.reference_stop && rel < L.max_rel_change_tol) stop_next = true;
+      } else {
+        // constraint convergence (reaction.F90:1952-1966)
+        const double mn = gmin<G>(live ? m : 1.0, lane);
+        if (!(mn > 0.0)) {
+          finish = true;
+          status = PFX_CELL_NONFINITE;
+        } else if (it >= 10000 || it >= P.max_iterations) {
+          finish = true;
+          status = PFX_CELL_MAX_ITERATIONS;
+        } else if (rr < L.max_residual_tol && rel < L.max_rel_change_tol) {
+          if (compute_act && it - nit_on > 1) {
+            finish = true;
+          } else {
+            if (!compute_act) nit_on = it;
+            compute_act = L.act_freq != PFX_ACT_OFF;
+            if (L.act_freq == PFX_ACT_OFF && it - nit_on > 1) finish = true;
+          }
+        }
+      }
+    }
+
+    // ---------------------------------------------------------------
+    // retire the cell: write the state at this iterate
+    if (active && finish) {
+      const long long c = cell;
+      if (MODE == PFX_MODE_EVAL) {
+        if (live) {
+          P.res_out[c * N + r] = Rr;
+#pragma unroll
+          for (int k = 0; k < G; k++)
+            if (k < N) P.jac_out[c * N * N + (long long)k * N + r] = row[k];
+        }
+      } else {
+        if (MODE == PFX_MODE_EQUIL && R > 0) {
+          // sorbed totals at the final free ions (reaction.F90:1970-1975)
+          // are produced by the aux-var refresh; keep the exchange state
+        }
+        if (live) {
+          P.pri_molal[c * N + r] = m;
+          P.total[c * N + r] = tot_out;
+          if (P.pri_act_coef) P.pri_act_coef[c * N + r] = gam;
+          if (R > 0 && P.total_sorb) P.total_sorb[c * N + r] = sorb_out;
+        }
+        for (int cc = r; cc < M; cc += G) {
+          if (P.sec_act_coef) P.sec_act_coef[c * M + cc] = SD(S.gsec, cc);
+          if (P.sec_molal) P.sec_molal[c * M + cc] = SD(S.C, cc);
+        }
+        if (MODE == PFX_MODE_REACT && P.mnrl_rate)
+          for (int k = r; k < K; k += G) P.mnrl_rate[c * K + k] = SD(S.mrate, k);
+        if (r == 0) {
+          for (int q = 0; q < R; q++)
+            if (P.ionx_ref) P.ionx_ref[c * R + q] = ionx_state[q];
+          if (MODE != PFX_MODE_AUX) {
+            if (P.num_iterations) P.num_iterations[c] = it;
+            if (P.status) P.status[c] = status;
+            if (P.rel_res) P.rel_res[c] = rr;
+          }
+        }
+      }
+      cell = -1;
+    }
+    wave_sync();
+#undef P
+#undef L
+  }
+}
+
+#define PFX_INST(G)                                                                 \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_REACT>(const pfx_kernel_params); \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_EVAL>(const pfx_kernel_params);  \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_EQUIL>(const pfx_kernel_params); \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_AUX>(const pfx_kernel_params);
+PFX_INST(8)
+PFX_INST(16)
+PFX_INST(32)
+
+extern "C" int pfx_scr_doubles(int M, int G, int K) { return scr_doubles(M, G, K); }

@@ -1,0 +1,181 @@
+"""Python mirror of the C-ABI (libpfx_chem.so): device state and batch calls.
+
+PyTorch is used only for device memory and streams; all chemistry runs in the
+HIP kernels of libpfx_chem.so.  There is no CPU fallback: if the library is
+missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import abi
+from .database import ReactionTables
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpfx_chem.so")
+
+EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_react_batch",
+            "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch"]
+
+_lib = None
+
+
+class PfxError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libpfx_chem.so and declare the C signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PfxError(f"libpfx_chem.so not built: {path} (run `python -m pflotran_amd.build`)")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    lib.pfx_abi_version.restype = C.c_int
+    lib.pfx_tables_create.argtypes = [P(abi.ReactionDesc), P(C.c_void_p)]
+    lib.pfx_tables_destroy.argtypes = [C.c_void_p]
+    lib.pfx_react_batch.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), P(abi.Stats)]
+    lib.pfx_residual_jacobian.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), C.c_void_p, C.c_void_p]
+    lib.pfx_equilibrate_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_void_p, C.c_void_p, C.c_int32,
+                                          P(abi.ReactOpts), P(abi.Stats)]
+    lib.pfx_auxvar_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_int32, P(abi.ReactOpts)]
+    if lib.pfx_abi_version() != abi.PFX_ABI_VERSION:
+        raise PfxError("ABI version mismatch between libpfx_chem.so and pflotran_amd.abi")
+    _lib = lib
+    return lib
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise PfxError("no GPU visible: the pfx kernels run on the device only")
+    return torch
+
+
+class Tables:
+    """Device copy of a reaction network (pfx_tables_create)."""
+
+    def __init__(self, tables: ReactionTables):
+        lib = load_library()
+        _torch()
+        self.host = tables
+        self._desc = tables.desc()
+        h = C.c_void_p()
+        rc = lib.pfx_tables_create(C.byref(self._desc), C.byref(h))
+        if rc != 0:
+            raise PfxError(f"pfx_tables_create failed: {rc}")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().pfx_tables_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class DeviceCells:
+    """torch CUDA tensors for the pfx_cells fields (cell-major, FP64)."""
+    t: Dict[str, object]
+    ncells: int
+
+    @classmethod
+    def from_host(cls, arrays: Dict[str, Optional[np.ndarray]], ncells: int, device="cuda"):
+        torch = _torch()
+        t = {}
+        for k, v in arrays.items():
+            if v is None:
+                t[k] = None
+            else:
+                t[k] = torch.from_numpy(np.ascontiguousarray(v)).to(device)
+        return cls(t, ncells)
+
+    def struct(self) -> abi.Cells:
+        c = abi.Cells()
+        c.ncells = self.ncells
+        for f in abi.CELL_FIELDS:
+            v = self.t.get(f)
+            setattr(c, f, None if v is None else v.data_ptr())
+        return c
+
+    def to_host(self) -> Dict[str, Optional[np.ndarray]]:
+        return {k: (None if v is None else v.cpu().numpy()) for k, v in self.t.items()}
+
+    def __getitem__(self, k):
+        return self.t[k]
+
+
+def react_batch(tables: Tables, cells: DeviceCells, opts: abi.ReactOpts):
+    """pfx_react_batch: returns (rc, stats)."""
+    lib = load_library()
+    torch = _torch()
+    cs = cells.struct()
+    st = abi.Stats()
+    if opts.stream is None:
+        opts.stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.pfx_react_batch(tables.handle, C.byref(cs), C.byref(opts), C.byref(st))
+    if rc < 0:
+        raise PfxError(f"pfx_react_batch failed: {rc}")
+    return rc, st
+
+
+def residual_jacobian(tables: Tables, cells: DeviceCells, opts: abi.ReactOpts):
+    """pfx_residual_jacobian: residual [n,N] and Jacobian [n,N,N] (row, col)."""
+    lib = load_library()
+    torch = _torch()
+    n, N = cells.ncells, tables.host.N
+    res = torch.zeros((n, N), dtype=torch.float64, device="cuda")
+    jac = torch.zeros((n, N, N), dtype=torch.float64, device="cuda")  # [cell][col][row]
+    cs = cells.struct()
+    if opts.stream is None:
+        opts.stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.pfx_residual_jacobian(tables.handle, C.byref(cs), C.byref(opts), res.data_ptr(), jac.data_ptr())
+    if rc != 0:
+        raise PfxError(f"pfx_residual_jacobian failed: {rc}")
+    torch.cuda.synchronize()
+    return res.cpu().numpy(), jac.transpose(1, 2).contiguous().cpu().numpy()
+
+
+def equilibrate(tables: Tables, cells: DeviceCells, constraint_type, constraint_conc, opts: abi.ReactOpts,
+                molal: bool = False):
+    """pfx_equilibrate_batch; constraint_conc is a [n, N] array (host or device)."""
+    lib = load_library()
+    torch = _torch()
+    ct = np.ascontiguousarray(constraint_type, dtype=np.int32)
+    cc = constraint_conc
+    if not isinstance(cc, torch.Tensor):
+        cc = torch.from_numpy(np.ascontiguousarray(cc, dtype=np.float64)).cuda()
+    cs = cells.struct()
+    st = abi.Stats()
+    if opts.stream is None:
+        opts.stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.pfx_equilibrate_batch(tables.handle, C.byref(cs), ct.ctypes.data, cc.data_ptr(), int(molal),
+                                   C.byref(opts), C.byref(st))
+    if rc < 0:
+        raise PfxError(f"pfx_equilibrate_batch failed: {rc}")
+    return rc, st
+
+
+def auxvar(tables: Tables, cells: DeviceCells, activity_passes: int, opts: abi.ReactOpts):
+    """pfx_auxvar_batch: totals / sorbed totals / complexes at the given free ions."""
+    lib = load_library()
+    torch = _torch()
+    cs = cells.struct()
+    if opts.stream is None:
+        opts.stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.pfx_auxvar_batch(tables.handle, C.byref(cs), int(activity_passes), C.byref(opts))
+    if rc != 0:
+        raise PfxError(f"pfx_auxvar_batch failed: {rc}")
+    torch.cuda.synchronize()

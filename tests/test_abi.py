@@ -1,0 +1,54 @@
+"""The C-ABI library loads and exports every entry point declared in
+include/pfx_chem.h, and the ctypes mirror matches the C struct layouts."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from pflotran_amd import abi, runtime
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pfx_chem.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*int\s+(pfx_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_expected_entry_points():
+    assert declared_functions() == sorted(runtime.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = runtime.load_library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.pfx_abi_version() == abi.PFX_ABI_VERSION
+
+
+def test_null_arguments_rejected_without_gpu_work():
+    lib = runtime.load_library()
+    assert lib.pfx_tables_create(None, None) == abi.PFX_E_ARG
+    assert lib.pfx_react_batch(None, None, None, None) == abi.PFX_E_ARG
+    assert lib.pfx_tables_destroy(None) == abi.PFX_E_ARG
+
+
+@pytest.mark.parametrize("struct,cname", [(abi.ReactionDesc, "pfx_reaction_desc"), (abi.Cells, "pfx_cells"),
+                                          (abi.ReactOpts, "pfx_react_opts"), (abi.Stats, "pfx_stats")])
+def test_struct_layout_matches_header(tmp_path, struct, cname):
+    fields = [f for f, _ in struct._fields_]
+    src = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', 'int main(void){',
+           f'printf("%zu\\n", sizeof({cname}));']
+    src += [f'printf("%zu\\n", offsetof({cname}, {f}));' for f in fields]
+    src.append("return 0;}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(c)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == C.sizeof(struct)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(struct, f).offset == off, f

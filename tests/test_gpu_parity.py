@@ -1,0 +1,168 @@
+"""HIP kernels (through the C-ABI) against the CPU oracle on the same seeded
+inputs.  Tolerances: free ions 1e-10 relative, Jacobian entries 1e-12
+relative to the largest entry of their row, residual 1e-12 relative to the
+accumulation magnitude."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from chemdefs import calcite_kinetics, ca_carbonate, carbonate, carbonate_column, valocchi
+from conftest import gold
+from gpu_cases import host_arrays, speciated_state
+from pflotran_amd import abi, runtime, synth
+from pflotran_amd.database import water_density_ifc67
+
+pytestmark = pytest.mark.gpu
+
+DEN = water_density_ifc67()
+
+
+def rel_err(a, b):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def net15():
+    return synth.random_network(15, 50, 3, ionx=True)
+
+
+def test_residual_jacobian_parity(net15):
+    t = net15
+    n = 333
+    cells, _, _, ok = speciated_state(t, n)
+    assert ok.all()
+    o = synth.react_opts()
+    # evaluate at a perturbed iterate so kinetics and exchange are far from balance
+    cells["pri_molal"][:] *= np.exp(0.3 * np.sin(np.arange(n * t.N)).reshape(n, t.N))
+    res_o, jac_o = orc.residual_jacobian(t, cells, o)
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    res_g, jac_g = runtime.residual_jacobian(tab, dc, synth.react_opts())
+    rowmax = np.max(np.abs(jac_o), axis=2, keepdims=True)
+    assert np.max(np.abs(jac_g - jac_o) / rowmax) <= 1e-12
+    psv = 0.25 * 1.0 * 1000.0 * 1.0
+    acc = psv * np.abs(cells["total"]) / synth.STEP_DT
+    assert np.max(np.abs(res_g - res_o) / (acc + np.abs(res_o) + 1e-300)) <= 1e-12
+
+
+def test_react_parity_default_guess(net15):
+    t = net15
+    n = 512
+    cells, _, _, ok = speciated_state(t, n)
+    assert ok.all()
+    o = synth.react_opts(extra_iterations=1)
+    ref = cells.copy()
+    rc_o, st_o = orc.react_batch(t, ref, o)
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    rc_g, st_g = runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1))
+    out = dc.to_host()
+    assert rc_g == rc_o == 0, (st_g.as_dict(), st_o.as_dict())
+    assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
+    assert rel_err(out["total"], ref["total"]) <= 1e-10
+    assert rel_err(out["pri_act_coef"], ref["pri_act_coef"]) <= 1e-12
+    assert rel_err(out["total_sorb"], ref["total_sorb"]) <= 1e-10
+    assert np.max(np.abs(out["mnrl_rate"] - ref["mnrl_rate"]) / np.maximum(np.abs(ref["mnrl_rate"]), 1e-30)) <= 1e-9
+    assert np.all(out["status"] == 0)
+    # iteration counts agree closely (rounding can move a cell across the tolerance)
+    assert np.mean(np.abs(out["num_iterations"] - ref["num_iterations"])) < 0.5
+
+
+@pytest.mark.parametrize("n", [1, 7, 33, 1000])
+def test_react_ragged_and_previous_guess(net15, n):
+    t = net15
+    cells, _, _, ok = speciated_state(t, n, seed=7)
+    o = synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS, extra_iterations=1)
+    ref = cells.copy()
+    orc.react_batch(t, ref, o)
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS, extra_iterations=1))
+    out = dc.to_host()
+    assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
+
+
+def test_equilibrate_parity(net15):
+    t = net15
+    n = 256
+    cells, ctype, conc, ok = speciated_state(t, n, seed=11)
+    tab = runtime.Tables(t)
+    arr = host_arrays(cells)
+    arr["pri_molal"][:] = 1e-9
+    dc = runtime.DeviceCells.from_host(arr, n)
+    o = synth.react_opts()
+    rc, st = runtime.equilibrate(tab, dc, ctype, conc, o)
+    assert rc == 0, st.as_dict()
+    # the initial-condition refresh: unit coefficients, two activity passes
+    dc.t["pri_act_coef"].fill_(1.0)
+    dc.t["sec_act_coef"].fill_(1.0)
+    dc.t["ionx_ref_sorbed"].fill_(1e-9)
+    runtime.auxvar(tab, dc, 2, synth.react_opts())
+    out = dc.to_host()
+    assert rel_err(out["pri_molal"], cells["pri_molal"]) <= 1e-10
+    assert rel_err(out["pri_act_coef"], cells["pri_act_coef"]) <= 1e-10
+    assert rel_err(out["total"][:, 1:], cells["total"][:, 1:]) <= 1e-10
+
+
+def _gpu_speciate(t, setup, ncells=1):
+    tab = runtime.Tables(t)
+    arr = synth.host_state_arrays(t, ncells, with_temperature=False)
+    if "vf" in setup:
+        arr["mnrl_volfrac"][:, 0] = setup["vf"]
+        arr["mnrl_area"][:, 0] = setup["area"]
+    dc = runtime.DeviceCells.from_host(arr, ncells)
+    o = synth.react_opts(porosity=setup["porosity"])
+    conc = np.tile(np.array(setup["conc"], dtype=np.float64), (ncells, 1))
+    rc, st = runtime.equilibrate(tab, dc, setup["ctype"], conc, o)
+    assert rc == 0, st.as_dict()
+    dc.t["pri_act_coef"].fill_(1.0)
+    dc.t["sec_act_coef"].fill_(1.0)
+    if dc.t["ionx_ref_sorbed"] is not None:
+        dc.t["ionx_ref_sorbed"].fill_(1e-9)
+    passes = 2 if t.act_freq != abi.PFX_ACT_OFF else 0
+    runtime.auxvar(tab, dc, passes, synth.react_opts(porosity=setup["porosity"]))
+    return tab, dc
+
+
+@pytest.mark.parametrize("act", [False, True])
+def test_gpu_golds_speciation(golds, act):
+    for name, (t, setup), ph_col, species in (
+            ("carbonate-debye-huckel-activity" if act else "carbonate-unit-activity", carbonate(act), 0,
+             ["H+", "HCO3-"]),
+            ("ca-carbonate-debye-huckel-activity" if act else "ca-carbonate-unit-activity", ca_carbonate(act), 2,
+             ["HCO3-", "Ca++", "H+"])):
+        _, dc = _gpu_speciate(t, setup)
+        out = dc.to_host()
+        m, g = out["pri_molal"][0], out["pri_act_coef"][0]
+        assert abs(-np.log10(g[ph_col] * m[ph_col]) - gold(golds, name, "GENERIC: pH")) < 1e-12
+        for j, sp in enumerate(species):
+            assert abs(out["total"][0][j] - gold(golds, name, f"CONCENTRATION: Total {sp}")) < 1e-12
+            if act:
+                assert abs(g[j] - gold(golds, name, f"GENERIC: Gamma {sp}")) < 1e-12
+
+
+def test_gpu_gold_calcite_kinetics(golds):
+    """calcite-kinetics: 500 reaction steps of 0.5 s on the GPU, each with the
+    fixed accumulation refreshed from the current state."""
+    name = "calcite-kinetics"
+    t, setup = calcite_kinetics()
+    tab, dc = _gpu_speciate(t, setup)
+    o = synth.react_opts(porosity=0.5, guess_mode=abi.PFX_GUESS_PREVIOUS, rel_tol=1e-14, extra_iterations=1,
+                         max_iterations=60, dt=0.5)
+    molar_vol = t.mnrl_molar_vol[0]
+    for _ in range(500):
+        runtime.auxvar(tab, dc, 0, synth.react_opts(porosity=0.5))  # accumulation of the old state
+        rc, st = runtime.react_batch(tab, dc, synth.react_opts(porosity=0.5, guess_mode=abi.PFX_GUESS_PREVIOUS,
+                                                               rel_tol=1e-14, extra_iterations=1,
+                                                               max_iterations=60, dt=0.5))
+        assert rc == 0
+        # RUpdateKineticState: vf += rate * molar volume * dt (rate at the solution)
+        dc.t["mnrl_volfrac"].add_(dc.t["mnrl_rate"] * (molar_vol * 0.5)).clamp_(min=0.0)
+    runtime.auxvar(tab, dc, 0, synth.react_opts(porosity=0.5))
+    out = dc.to_host()
+    m, g = out["pri_molal"][0], out["pri_act_coef"][0]
+    assert abs(-np.log10(g[0] * m[0]) - gold(golds, name, "GENERIC: pH")) < 1e-12
+    for j, sp in enumerate(["H+", "HCO3-", "Ca++"]):
+        assert abs(out["total"][0][j] - gold(golds, name, f"CONCENTRATION: Total {sp}")) < 1e-12
+    assert abs(out["mnrl_volfrac"][0, 0] - gold(golds, name, "VOLUME_FRACTION: Calcite VF")) < 1e-12
+    assert abs(out["mnrl_rate"][0, 0] - gold(golds, name, "RATE: Calcite Rate")) < 1e-12

@@ -1,0 +1,81 @@
+"""The oracle against the reference's own regression golds
+(regression_tests/ascem/batch/*.regression.gold, 1e-12 absolute per batch.cfg).
+A one-cell global-implicit run without transport solves exactly the per-cell
+reaction system of RReact, so these pin the oracle's chemistry end to end."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from chemdefs import calcite_kinetics, ca_carbonate, carbonate, valocchi
+from conftest import gold
+from pflotran_amd import abi
+from pflotran_amd.database import water_density_ifc67
+
+TOL = 1e-12  # batch.cfg default-test-criteria
+DEN = water_density_ifc67()
+
+
+def speciate(t, setup):
+    cells = orc.HostCells.allocate(t, 1)
+    if "vf" in setup:
+        cells["mnrl_volfrac"][0, 0] = setup["vf"]
+        cells["mnrl_area"][0, 0] = setup["area"]
+    o = abi.default_opts(porosity=setup["porosity"], den_kg=DEN)
+    rc, its = orc.equilibrate(t, setup["ctype"], np.array(setup["conc"]), cells, o)
+    assert rc == 0 and its[0] > 0
+    return cells, o
+
+
+@pytest.mark.parametrize("act", [False, True])
+def test_carbonate_speciation(golds, act):
+    name = "carbonate-debye-huckel-activity" if act else "carbonate-unit-activity"
+    t, setup = carbonate(act)
+    cells, _ = speciate(t, setup)
+    m, g = cells["pri_molal"][0], cells["pri_act_coef"][0]
+    assert abs(-np.log10(g[0] * m[0]) - gold(golds, name, "GENERIC: pH")) < TOL
+    assert abs(cells["total"][0][0] - gold(golds, name, "CONCENTRATION: Total H+")) < TOL
+    assert abs(cells["total"][0][1] - gold(golds, name, "CONCENTRATION: Total HCO3-")) < TOL
+    if act:
+        assert abs(g[0] - gold(golds, name, "GENERIC: Gamma H+")) < TOL
+        assert abs(g[1] - gold(golds, name, "GENERIC: Gamma HCO3-")) < TOL
+    else:
+        assert np.all(g == 1.0)
+
+
+@pytest.mark.parametrize("act", [False, True])
+def test_ca_carbonate_speciation(golds, act):
+    name = "ca-carbonate-debye-huckel-activity" if act else "ca-carbonate-unit-activity"
+    t, setup = ca_carbonate(act)
+    cells, _ = speciate(t, setup)
+    m, g = cells["pri_molal"][0], cells["pri_act_coef"][0]
+    assert abs(-np.log10(g[2] * m[2]) - gold(golds, name, "GENERIC: pH")) < TOL
+    for j, sp in enumerate(["HCO3-", "Ca++", "H+"]):
+        assert abs(cells["total"][0][j] - gold(golds, name, f"CONCENTRATION: Total {sp}")) < TOL
+        if act:
+            assert abs(g[j] - gold(golds, name, f"GENERIC: Gamma {sp}")) < TOL
+
+
+def test_valocchi_ion_exchange(golds):
+    name = "ion-exchange-valocchi"
+    t, setup = valocchi()
+    cells, _ = speciate(t, setup)
+    for j, sp in enumerate(["Na+", "Ca++", "Mg++", "Cl-"]):
+        assert abs(cells["total"][0][j] - gold(golds, name, f"CONCENTRATION: Total {sp}")) < TOL
+        assert abs(cells["total_sorb"][0][j] - gold(golds, name, f"CONCENTRATION: Total Sorbed {sp}")) < TOL
+    # charge invariant of the exchanger: sum z_i S_i = CEC
+    Z = t.Z
+    assert abs(np.dot(Z, cells["total_sorb"][0]) - 750.0) < 1e-10
+
+
+def test_calcite_kinetics(golds):
+    name = "calcite-kinetics"
+    t, setup = calcite_kinetics()
+    cells, o = speciate(t, setup)
+    o.rel_tol, o.extra_iterations, o.max_iterations = 1e-14, 1, 60
+    orc.batch_steps(t, cells, o, [0.5] * 500)
+    m, g = cells["pri_molal"][0], cells["pri_act_coef"][0]
+    assert abs(-np.log10(g[0] * m[0]) - gold(golds, name, "GENERIC: pH")) < TOL
+    for j, sp in enumerate(["H+", "HCO3-", "Ca++"]):
+        assert abs(cells["total"][0][j] - gold(golds, name, f"CONCENTRATION: Total {sp}")) < TOL
+    assert abs(cells["mnrl_volfrac"][0, 0] - gold(golds, name, "VOLUME_FRACTION: Calcite VF")) < TOL
+    assert abs(cells["mnrl_rate"][0, 0] - gold(golds, name, "RATE: Calcite Rate")) < TOL
