@@ -156,9 +156,7 @@ def cpu_baseline(t, sample):
 
     threads = min(os.cpu_count() or 1, 16)
     n = sample or 1500 * threads
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from gpu_cases import speciated_state  # noqa: E402  (oracle speciation of the same draws)
-    cells, _, _, _ = speciated_state(t, n, nthreads=threads)
+    cells, _, _, _ = orc.speciated_state(t, n, nthreads=threads)
     work = cells.copy()
     t0 = time.perf_counter()
     rc, st = orc.react_batch(t, work, synth.react_opts(), nthreads=threads)

@@ -198,3 +198,32 @@ def logk_fit(temps, logK):
     c = np.zeros(5)
     load().or_logk_fit(t.ctypes.data, k.ctypes.data, len(t), c.ctypes.data)
     return c
+
+
+def speciated_state(t, n, seed=None, workload_id=2, temperature=True, nthreads=0, only_ok=True):
+    """Constraint speciation (totals of every primary but H+, pH) of seeded
+    synthetic cells by the oracle: the previous state of each cell before
+    its reaction step.  With only_ok, cells whose speciation fails are
+    replaced by further draws."""
+    from pflotran_amd import synth
+    if seed is None:
+        seed = synth.SEED
+    if only_ok:
+        cells, ctype, conc, ok = speciated_state(t, int(n * 1.2) + 8, seed, workload_id, temperature, nthreads,
+                                                 only_ok=False)
+        idx = np.where(ok)[0][:n]
+        assert len(idx) == n, "not enough speciated cells"
+        sub = HostCells({k: (None if v is None else v[idx].copy()) for k, v in cells.arrays.items()}, n)
+        return sub, ctype, conc[idx], np.ones(n, dtype=bool)
+    ci = synth.random_cells(t, n, seed=seed, workload_id=workload_id)
+    cells = HostCells.allocate(t, n, temperature=temperature)
+    if temperature:
+        cells["temperature"][:] = ci.temp
+    cells["mnrl_volfrac"][:] = ci.vf[:, :cells["mnrl_volfrac"].shape[1]]
+    cells["mnrl_area"][:] = ci.area[:, :cells["mnrl_area"].shape[1]]
+    conc = ci.totals.copy()
+    conc[:, 0] = ci.ph
+    ctype = [CON_PH] + [CON_TOTAL] * (t.N - 1)
+    rc, its = equilibrate(t, ctype, conc, cells, synth.react_opts(), nthreads=nthreads)
+    ok = (its > 0) & np.all(np.isfinite(cells["pri_molal"]), axis=1) & np.all(cells["pri_molal"] > 0, axis=1)
+    return cells, ctype, conc, ok

@@ -130,18 +130,91 @@ def debye_params(tref: float = 25.0, use_bdot: bool = True) -> Tuple[float, floa
     return A, B, (Bd if use_bdot else 0.0)
 
 
+def _crout_solve(a: List[List[float]], b: List[float]) -> List[float]:
+    """Crout LU with implicit partial pivoting and substitution, in the
+    reference's operation order (utility.F90:393-523), for the small normal
+    equations of the log K fit; the fit is ill-conditioned enough that a
+    different elimination order moves log K by ~1e-8."""
+    n = len(b)
+    A = [row[:] for row in a]
+    vv = [1.0 / max(abs(x) for x in row) for row in A]
+    indx = [0] * n
+    for j in range(n):
+        for i in range(j):
+            s = A[i][j]
+            for k in range(i):
+                s -= A[i][k] * A[k][j]
+            A[i][j] = s
+        aamax, imax = 0.0, j
+        for i in range(j, n):
+            s = A[i][j]
+            for k in range(j):
+                s -= A[i][k] * A[k][j]
+            A[i][j] = s
+            dum = vv[i] * abs(s)
+            if dum >= aamax:
+                imax, aamax = i, dum
+        if j != imax:
+            A[imax], A[j] = A[j], A[imax]
+            vv[imax] = vv[j]
+        indx[j] = imax
+        if A[j][j] == 0.0:
+            A[j][j] = 1e-20
+        if j != n - 1:
+            dum = 1.0 / A[j][j]
+            for i in range(j + 1, n):
+                A[i][j] *= dum
+    x = list(b)
+    ii = -1
+    for i in range(n):
+        ll = indx[i]
+        s = x[ll]
+        x[ll] = x[i]
+        if ii >= 0:
+            for j in range(ii, i):
+                s -= A[i][j] * x[j]
+        elif s != 0.0:
+            ii = i
+        x[i] = s
+    for i in range(n - 1, -1, -1):
+        s = x[i]
+        for j in range(i + 1, n):
+            s -= A[i][j] * x[j]
+        x[i] = s / A[i][i]
+    return x
+
+
 def logk_fit(temps: Sequence[float], logK: Sequence[float]) -> np.ndarray:
     """5-term Maier-Kelly least-squares fit of log K(T) over the database
-    temperatures, entries of 500 are missing (reaction_aux.F90:1336-1408)."""
-    tk = np.asarray(temps, dtype=np.float64) + 273.15
-    lk = np.asarray(logK, dtype=np.float64)
-    use = np.abs(lk - 500.0) >= 1e-10
-    V = np.stack([np.log(tk), np.ones_like(tk), tk, 1.0 / tk, 1.0 / (tk * tk)])[:, use]
-    if use.sum() < 5:
+    temperatures, entries of 500 are missing (reaction_aux.F90:1336-1408):
+    normal equations accumulated and solved in the reference's order."""
+    vec = []
+    for t in temps:
+        tk = float(t) + 273.15
+        vec.append([math.log(tk), 1.0, tk, 1.0 / tk, 1.0 / (tk * tk)])
+    use = [abs(float(k) - 500.0) >= 1e-10 for k in logK]
+    if sum(use) < 5:
         # under-determined (e.g. one tabulated temperature): minimum-norm fit;
         # only used for non-isothermal runs, which need >= 5 temperatures
-        return np.linalg.lstsq(V.T, lk[use], rcond=None)[0]
-    return np.linalg.solve(V @ V.T, V @ lk[use])
+        V = np.array([v for v, u in zip(vec, use) if u])
+        return np.linalg.lstsq(V, np.asarray(logK, dtype=np.float64)[np.array(use)], rcond=None)[0]
+    coefs = []
+    for j in range(5):
+        s = 0.0
+        for i in range(len(temps)):
+            if use[i]:
+                s += vec[i][j] * float(logK[i])
+        coefs.append(s)
+    a = [[0.0] * 5 for _ in range(5)]
+    for j in range(5):
+        for k in range(j, 5):
+            s = 0.0
+            for i in range(len(temps)):
+                if use[i]:
+                    s += vec[i][j] * vec[i][k]
+            a[j][k] = s
+            a[k][j] = s
+    return np.array(_crout_solve(a, coefs))
 
 
 def logk_eval(coefs: np.ndarray, temp_c) -> np.ndarray:

@@ -1,0 +1,25 @@
+! Host-side probe of the Fortran binding: ABI version and argument checks
+! (no device work), plus the byte sizes of the bound types for comparison
+! with the C structs.
+program pfx_abi_probe
+  use, intrinsic :: iso_c_binding
+  use pfx_chem_module
+  implicit none
+  type(pfx_reaction_desc) :: desc
+  type(pfx_cells) :: cells
+  type(pfx_react_opts) :: opts
+  type(pfx_stats) :: stats
+  type(c_ptr) :: tables
+  integer(c_int) :: rc
+
+  print '(a,i0)', 'abi ', pfx_abi_version()
+  desc%naqcomp = 0
+  rc = pfx_tables_create(desc, tables)
+  print '(a,i0)', 'create_bad ', rc
+  rc = pfx_react_batch(c_null_ptr, cells, opts, stats)
+  print '(a,i0)', 'react_null ', rc
+  print '(a,i0)', 'sizeof_desc ', c_sizeof(desc)
+  print '(a,i0)', 'sizeof_cells ', c_sizeof(cells)
+  print '(a,i0)', 'sizeof_opts ', c_sizeof(opts)
+  print '(a,i0)', 'sizeof_stats ', c_sizeof(stats)
+end program pfx_abi_probe

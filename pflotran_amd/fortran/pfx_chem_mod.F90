@@ -1,0 +1,144 @@
+! ISO_C_BINDING interface of libpfx_chem.so (include/pfx_chem.h) for the
+! PFLOTRAN Fortran host.  The derived types mirror the C structs field by
+! field; reaction_type and reactive_transport_auxvar_type stay unchanged and
+! are packed into these contiguous arrays by the caller (see the
+! pfx_react_cells routine sketched in INTEGRATION.md).
+module pfx_chem_module
+
+  use, intrinsic :: iso_c_binding
+
+  implicit none
+
+  private
+
+  integer(c_int), parameter, public :: PFX_ABI_EXPECTED = 1
+  integer(c_int), parameter, public :: PFX_E_ARG = -1, PFX_E_DEVICE = -2, &
+                                       PFX_E_NOMEM = -3, PFX_E_UNSUP = -4
+  integer(c_int), parameter, public :: PFX_ACT_OFF = 0, PFX_ACT_TIMESTEP = 1, &
+                                       PFX_ACT_NEWTON_ITER = 2
+  integer(c_int), parameter, public :: PFX_ACT_ALG_LAG = 0, PFX_ACT_ALG_NEWTON = 1
+  integer(c_int), parameter, public :: PFX_CELL_CONVERGED = 0, &
+                                       PFX_CELL_MAX_ITERATIONS = 1, &
+                                       PFX_CELL_NONFINITE = 2, PFX_CELL_SINGULAR = 3
+  integer(c_int), parameter, public :: PFX_CONSTRAINT_TOTAL = 0, &
+                                       PFX_CONSTRAINT_FREE = 1, &
+                                       PFX_CONSTRAINT_LOG = 2, PFX_CONSTRAINT_PH = 3
+  integer(c_int), parameter, public :: PFX_GUESS_PREVIOUS = 0, PFX_GUESS_ARRAY = 1, &
+                                       PFX_GUESS_DEFAULT = 2
+
+  type, bind(c), public :: pfx_reaction_desc
+    integer(c_int32_t) :: naqcomp, neqcplx, nkinmnrl, neqionxrxn
+    integer(c_int32_t) :: max_rxn_spec, max_cation
+    type(c_ptr) :: primary_spec_Z, primary_spec_a0
+    type(c_ptr) :: eqcplxspecid, eqcplxstoich, eqcplxh2ostoich
+    type(c_ptr) :: eqcplx_Z, eqcplx_a0, eqcplx_logK, eqcplx_logKcoef
+    type(c_ptr) :: kinmnrlspecid, kinmnrlstoich, kinmnrlh2ostoich
+    type(c_ptr) :: kinmnrl_logK, kinmnrl_logKcoef, kinmnrl_rate_constant
+    type(c_ptr) :: kinmnrl_molar_vol, kinmnrl_activation_energy
+    type(c_ptr) :: kinmnrl_affinity_threshold, kinmnrl_rate_limiter
+    type(c_ptr) :: kinmnrl_affinity_power, kinmnrl_Temkin_const
+    type(c_ptr) :: eqionx_rxn_cationid, eqionx_rxn_k, eqionx_rxn_CEC
+    type(c_ptr) :: eqionx_rxn_to_surf
+    real(c_double) :: debyeA, debyeB, debyeBdot
+    integer(c_int32_t) :: use_log_formulation
+    integer(c_int32_t) :: act_coef_update_frequency
+    integer(c_int32_t) :: act_coef_update_algorithm
+    integer(c_int32_t) :: reserved0
+    real(c_double) :: max_dlnC
+    real(c_double) :: max_relative_change_tolerance
+    real(c_double) :: max_residual_tolerance
+  end type pfx_reaction_desc
+
+  type, bind(c), public :: pfx_cells
+    integer(c_int64_t) :: ncells
+    type(c_ptr) :: total, total_sorb, pri_molal, pri_act_coef, sec_act_coef
+    type(c_ptr) :: sec_molal, ionx_ref_sorbed, mnrl_volfrac, mnrl_area
+    type(c_ptr) :: mnrl_rate, temperature, porosity, saturation, volume
+    type(c_ptr) :: den_kg, guess, num_iterations, status, rel_res
+  end type pfx_cells
+
+  type, bind(c), public :: pfx_react_opts
+    real(c_double) :: dt, rel_tol
+    integer(c_int32_t) :: max_iterations, extra_iterations
+    integer(c_int32_t) :: guess_mode, reference_stop
+    real(c_double) :: default_guess
+    real(c_double) :: porosity, saturation, volume, den_kg
+    type(c_ptr) :: stream
+  end type pfx_react_opts
+
+  type, bind(c), public :: pfx_stats
+    integer(c_int64_t) :: ncells, converged, total_iterations
+    integer(c_int32_t) :: max_iterations, min_iterations
+    real(c_double) :: max_rel_res, kernel_ms
+    integer(c_int64_t) :: iteration_histogram(32)
+  end type pfx_stats
+
+  public :: pfx_abi_version, pfx_tables_create, pfx_tables_destroy, &
+            pfx_react_batch, pfx_residual_jacobian, pfx_equilibrate_batch, &
+            pfx_auxvar_batch
+
+  interface
+    function pfx_abi_version() bind(c, name='pfx_abi_version')
+      import :: c_int
+      integer(c_int) :: pfx_abi_version
+    end function pfx_abi_version
+
+    function pfx_tables_create(desc, tables) bind(c, name='pfx_tables_create')
+      import :: c_int, c_ptr, pfx_reaction_desc
+      type(pfx_reaction_desc), intent(in) :: desc
+      type(c_ptr), intent(out) :: tables
+      integer(c_int) :: pfx_tables_create
+    end function pfx_tables_create
+
+    function pfx_tables_destroy(tables) bind(c, name='pfx_tables_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: tables
+      integer(c_int) :: pfx_tables_destroy
+    end function pfx_tables_destroy
+
+    function pfx_react_batch(tables, cells, opts, stats) &
+        bind(c, name='pfx_react_batch')
+      import :: c_int, c_ptr, pfx_cells, pfx_react_opts, pfx_stats
+      type(c_ptr), value :: tables
+      type(pfx_cells), intent(inout) :: cells
+      type(pfx_react_opts), intent(in) :: opts
+      type(pfx_stats), intent(out) :: stats
+      integer(c_int) :: pfx_react_batch
+    end function pfx_react_batch
+
+    function pfx_residual_jacobian(tables, cells, opts, residual, jacobian) &
+        bind(c, name='pfx_residual_jacobian')
+      import :: c_int, c_ptr, pfx_cells, pfx_react_opts
+      type(c_ptr), value :: tables
+      type(pfx_cells), intent(inout) :: cells
+      type(pfx_react_opts), intent(in) :: opts
+      type(c_ptr), value :: residual, jacobian
+      integer(c_int) :: pfx_residual_jacobian
+    end function pfx_residual_jacobian
+
+    function pfx_equilibrate_batch(tables, cells, constraint_type, &
+                                   constraint_conc, molal, opts, stats) &
+        bind(c, name='pfx_equilibrate_batch')
+      import :: c_int, c_int32_t, c_ptr, pfx_cells, pfx_react_opts, pfx_stats
+      type(c_ptr), value :: tables
+      type(pfx_cells), intent(inout) :: cells
+      integer(c_int32_t), intent(in) :: constraint_type(*)
+      type(c_ptr), value :: constraint_conc
+      integer(c_int32_t), value :: molal
+      type(pfx_react_opts), intent(in) :: opts
+      type(pfx_stats), intent(out) :: stats
+      integer(c_int) :: pfx_equilibrate_batch
+    end function pfx_equilibrate_batch
+
+    function pfx_auxvar_batch(tables, cells, activity_passes, opts) &
+        bind(c, name='pfx_auxvar_batch')
+      import :: c_int, c_int32_t, c_ptr, pfx_cells, pfx_react_opts
+      type(c_ptr), value :: tables
+      type(pfx_cells), intent(inout) :: cells
+      integer(c_int32_t), value :: activity_passes
+      type(pfx_react_opts), intent(in) :: opts
+      integer(c_int) :: pfx_auxvar_batch
+    end function pfx_auxvar_batch
+  end interface
+
+end module pfx_chem_module

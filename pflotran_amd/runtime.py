@@ -36,6 +36,10 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise PfxError(f"libpfx_chem.so not built: {path} (run `python -m pflotran_amd.build`)")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # soname); loading torch first makes the library bind to that copy
+    # instead of pulling a second runtime from /opt/rocm.
+    import torch  # noqa: F401
     lib = C.CDLL(path)
     P = C.POINTER
     lib.pfx_abi_version.restype = C.c_int

@@ -1,0 +1,237 @@
+"""Pins of the oracle's building blocks that the regression golds leave open:
+closed forms, invariants, brute force and finite/complex-step derivatives.
+Each check is chosen so that a plausible slip (dropped term, wrong sign or
+index, transposed operand) fails it."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from chemdefs import carbonate_column, valocchi
+from pflotran_amd import abi, synth
+from pflotran_amd.database import debye_params, logk_eval, logk_fit, water_density_ifc67
+
+LN10_TRUNC = 2.30258509299
+
+
+# ---------------------------------------------------------------- LU
+def _lu_longdouble(A, b):
+    """Gaussian elimination with partial pivoting in extended precision."""
+    A = A.astype(np.longdouble)
+    b = b.astype(np.longdouble)
+    n = len(b)
+    for j in range(n):
+        p = j + int(np.argmax(np.abs(A[j:, j])))
+        A[[j, p]] = A[[p, j]]
+        b[[j, p]] = b[[p, j]]
+        for i in range(j + 1, n):
+            f = A[i, j] / A[j, j]
+            A[i, j:] -= f * A[j, j:]
+            b[i] -= f * b[j]
+    x = np.zeros(n, dtype=np.longdouble)
+    for i in range(n - 1, -1, -1):
+        x[i] = (b[i] - A[i, i + 1:] @ x[i + 1:]) / A[i, i]
+    return x
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 15, 30])
+def test_ludcmp_lubksb_against_extended_precision(n):
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n)) * np.exp(rng.normal(size=(n, 1)) * 3)  # badly row-scaled
+    b = rng.normal(size=n)
+    st, x, _, _ = orc.lu_solve(A, b)
+    assert st == 0
+    xr = np.asarray(_lu_longdouble(A, b), dtype=np.float64)
+    assert np.max(np.abs(x - xr)) <= 1e-9 * np.max(np.abs(xr))
+
+
+def test_ludcmp_implicit_pivoting_and_ties():
+    # implicit (scaled) pivoting: row 0 has the largest |a| in column 0 but
+    # the smallest relative to its own row maximum, so row 1 is the pivot
+    A = np.array([[2.0, 1000.0], [1.0, 1.0]])
+    st, x, indx, _ = orc.lu_solve(A, np.array([1.0, 2.0]))
+    assert indx[0] == 1
+    assert np.allclose(A @ x, [1.0, 2.0], rtol=1e-14)
+    # exact tie of the scaled candidates: the later row wins (utility.F90:451)
+    A = np.array([[1.0, 2.0], [2.0, -4.0]])  # both scaled candidates are exactly 0.5
+    st, x, indx, _ = orc.lu_solve(A, np.array([3.0, 4.0]))
+    assert indx[0] == 1
+    assert np.allclose(A @ x, [3.0, 4.0], rtol=1e-15)
+
+
+def test_ludcmp_singular_row_reported():
+    A = np.array([[1.0, 2.0], [0.0, 0.0]])
+    st, _, _, _ = orc.lu_solve(A, np.array([1.0, 1.0]))
+    assert st == 2  # 1-based row of the reference's MPI_Abort branch
+
+
+# ---------------------------------------------------------------- setup closed forms
+def test_debye_at_25C_and_interpolation():
+    assert orc.debye(25.0) == (0.5114, 0.3288, 0.0410)
+    A, B, D = orc.debye(42.5)
+    assert abs(A - 0.5114 - 0.5 * (0.5465 - 0.5114)) < 1e-15
+    assert debye_params(42.5) == pytest.approx((A, B, D), abs=1e-15)
+    assert orc.debye(25.0, use_bdot=False)[2] == 0.0
+
+
+def test_water_density_restatements_agree():
+    d = orc.density_ifc67(25.0, 101325.0)
+    assert d == water_density_ifc67(25.0, 101325.0)
+    assert 997.0 < d < 997.3  # IFC-67 liquid water at 25 C, 1 atm
+
+
+def test_logk_fit_reproduces_tabulated_values():
+    """An exact 5-term function is reproduced at every temperature (the normal
+    equations are ill-conditioned, so values, not coefficients, are compared);
+    entries of 500 are excluded from the fit."""
+    temps = np.array([0.0, 25.0, 60.0, 100.0, 150.0, 200.0, 250.0, 300.0])
+    c_true = np.array([-3.0, 12.0, 0.004, -1500.0, 1.5e5])
+    lk = logk_eval(c_true, temps)
+    for fit in (orc.logk_fit, logk_fit):
+        assert np.max(np.abs(logk_eval(fit(temps, lk), temps) - lk)) < 1e-8
+    lk2 = lk.copy()
+    lk2[[0, 7]] = 500.0
+    c2 = orc.logk_fit(temps, lk2)
+    assert np.max(np.abs(logk_eval(c2, temps[1:7]) - lk[1:7])) < 1e-7
+    # C and numpy restatements agree on a realistic table (calcite, calcite.dat)
+    calcite = [2.2257, 1.8487, 1.3330, 0.7743, 0.0999, -0.5838, -1.3262, -2.2154]
+    a, b = orc.logk_fit(temps, calcite), logk_fit(temps, calcite)
+    assert np.max(np.abs(a - b) / np.abs(a)) < 1e-12
+    assert abs(logk_eval(a, 25.0) - 1.8487) < 0.05  # least squares, not interpolation
+
+
+# ---------------------------------------------------------------- RTotal
+def _rtotal(t, m, gam=None, den=997.16):
+    d = t.desc()
+    N, M = t.N, t.M
+    total = np.zeros(N)
+    dtot = np.zeros((N, N), order="F")
+    sec = np.zeros(max(M, 1))
+    g = np.ones(N) if gam is None else gam
+    orc.load().or_rtotal_cell(C.byref(d), np.ascontiguousarray(m).ctypes.data, g.ctypes.data, None, None, den,
+                              total.ctypes.data, dtot.ctypes.data, sec.ctypes.data)
+    return total, np.asarray(dtot).reshape(N, N, order="F"), sec
+
+
+def test_rtotal_mass_action_closed_form():
+    t = carbonate_column()
+    m = np.array([1e-7, 2e-3, 5e-4, 3e-4, 1e-3])
+    gam = np.array([0.9, 0.95, 0.7, 0.72, 0.96])
+    den = 997.16
+    total, _, sec = _rtotal(t, m, gam, den)
+    la = np.log(m * gam)
+    for i in range(t.M):
+        ns = t.cplx_specid[i, 0]
+        lnq = -t.cplx_logK[i] * LN10_TRUNC + sum(t.cplx_stoich[i, s] * la[t.cplx_specid[i, 1 + s]] for s in range(ns))
+        assert sec[i] == pytest.approx(np.exp(lnq), rel=1e-14)
+    T = m.copy()
+    for i in range(t.M):
+        for s in range(t.cplx_specid[i, 0]):
+            T[t.cplx_specid[i, 1 + s]] += t.cplx_stoich[i, s] * sec[i]
+    assert np.allclose(total, T * den * 1e-3, rtol=1e-14, atol=0)
+
+
+def test_rtotal_derivative_by_central_difference():
+    t = synth.random_network(15, 50, 3)
+    rng = np.random.default_rng(3)
+    m = 10.0 ** rng.uniform(-7, -3, size=t.N)
+    gam = rng.uniform(0.6, 1.0, size=t.N)
+    _, J, _ = _rtotal(t, m, gam)
+    for k in range(t.N):
+        h = 1e-6 * m[k]
+        mp, mm = m.copy(), m.copy()
+        mp[k] += h
+        mm[k] -= h
+        fd = (_rtotal(t, mp, gam)[0] - _rtotal(t, mm, gam)[0]) / (2 * h)
+        scale = np.max(np.abs(J[:, k]))
+        assert np.max(np.abs(fd - J[:, k])) <= 1e-7 * scale
+
+
+# ---------------------------------------------------------------- ion exchange
+def test_ionx_equal_valence_closed_form_and_charge():
+    # two monovalent cations: X_i = k_i a_i / sum k a (reaction.F90:4478-4492)
+    t, _ = valocchi()
+    t.ionx_cationid[0] = [2, 0, 0, 0][:t.ionx_cationid.shape[1]]
+    t.ionx_cationid[0, 0] = 2
+    t.ionx_cationid[0, 1] = 0  # Na+
+    t.ionx_cationid[0, 2] = 3  # Cl- stands in as a second monovalent "cation"
+    t.Z[3] = 1.0
+    t.ionx_k[0, :2] = [1.0, 2.5]
+    d = t.desc()
+    m = np.array([0.01, 0.002, 0.001, 0.03])
+    ref = np.array([1e-9])
+    sorb = np.zeros(4)
+    ds = np.zeros((4, 4), order="F")
+    orc.load().or_ionx_cell(C.byref(d), m.ctypes.data, None, None, ref.ctypes.data, sorb.ctypes.data, ds.ctypes.data)
+    X = np.array([m[0] * 1.0, m[3] * 2.5])
+    X /= X.sum()
+    assert sorb[0] == pytest.approx(X[0] * 750.0, rel=1e-14)
+    assert sorb[3] == pytest.approx(X[1] * 750.0, rel=1e-14)
+    assert sorb[1] == 0.0 and sorb[2] == 0.0
+
+
+def test_ionx_mixed_valence_charge_and_derivative():
+    t, _ = valocchi()
+    d = t.desc()
+    m = np.array([0.0865, 0.0182, 0.0111, 0.145])
+
+    def sorbed(mm):
+        ref = np.array([1e-9])
+        s = np.zeros(4)
+        dsv = np.zeros((4, 4), order="F")
+        orc.load().or_ionx_cell(C.byref(d), np.ascontiguousarray(mm).ctypes.data, None, None, ref.ctypes.data,
+                                s.ctypes.data, dsv.ctypes.data)
+        return s, np.asarray(dsv).reshape(4, 4, order="F")
+
+    s, dsv = sorbed(m)
+    assert np.dot(t.Z, s) == pytest.approx(750.0, rel=1e-13)  # sum z_i S_i = CEC
+    for k in range(3):
+        h = 1e-6 * m[k]
+        mp, mm = m.copy(), m.copy()
+        mp[k] += h
+        mm[k] -= h
+        fd = (sorbed(mp)[0] - sorbed(mm)[0]) / (2 * h)
+        assert np.max(np.abs(fd - dsv[:, k])) <= 1e-6 * np.max(np.abs(dsv[:, k]))
+
+
+# ---------------------------------------------------------------- per-cell residual/Jacobian
+def test_reaction_jacobian_by_central_difference():
+    """dR/dm of the per-cell residual (accumulation + kinetic minerals) against
+    central differences.  The reference's mineral term carries a factor
+    den/1000 (reaction_mineral.F90:866-870) that is exact only for a water
+    density of 1000 kg/m^3, which this check uses; exchange derivatives are
+    pinned separately above (their inner Newton makes differences noisy)."""
+    t = synth.random_network(15, 50, 3, ionx=False)
+    rng = np.random.default_rng(5)
+    cells = orc.HostCells.allocate(t, 1, temperature=True)
+    cells["temperature"][:] = 31.0
+    # near mineral saturation (the synthetic minerals saturate around 1e-4
+    # molal) so no ion activity product hits the reference's cap
+    # QK = 1e3 (reaction_mineral.F90:699-703), above which the analytic
+    # derivative keeps dQK = nu QK / m while the rate no longer changes
+    cells["pri_molal"][0] = 1e-4 * 10.0 ** rng.uniform(-0.3, 0.3, size=t.N)
+    cells["pri_act_coef"][0] = rng.uniform(0.7, 1.0, size=t.N)
+    cells["mnrl_volfrac"][:] = 0.05
+    cells["mnrl_area"][:] = 1.0
+    la = np.log(cells["pri_molal"][0] * cells["pri_act_coef"][0])
+    logk_c = logk_eval(t.mnrl_logKcoef, 31.0)
+    for k in range(t.K):
+        ns = t.mnrl_specid[k, 0]
+        lnq = -logk_c[k] * LN10_TRUNC + sum(t.mnrl_stoich[k, s] * la[t.mnrl_specid[k, 1 + s]] for s in range(ns))
+        assert lnq < 6.90776
+    o = synth.react_opts(den_kg=1000.0)
+    _, J = orc.residual_jacobian(t, cells, o)
+    m0 = cells["pri_molal"][0].copy()
+    for k in range(t.N):
+        h = 1e-6 * m0[k]
+        cp, cm = cells.copy(), cells.copy()
+        cp["pri_molal"][0, k] += h
+        cm["pri_molal"][0, k] -= h
+        fd = (orc.residual_jacobian(t, cp, o)[0][0] - orc.residual_jacobian(t, cm, o)[0][0]) / (2 * h)
+        assert np.max(np.abs(fd - J[0][:, k])) <= 1e-6 * np.max(np.abs(J[0][:, k])), k
+    # at another density the reference's analytic Jacobian scales wholesale
+    # by den/1000 (accumulation exactly, mineral block through that factor)
+    o2 = synth.react_opts()
+    _, J2 = orc.residual_jacobian(t, cells, o2)
+    assert np.allclose(J2, J * (o2.den_kg / 1000.0), rtol=1e-12, atol=0)
