@@ -93,6 +93,19 @@ extern __shared__ __align__(16) unsigned char smem[];
 #define LI(off, i) (((const int32_t *)(smem + (off)))[(i)])
 #define SD(off, i) (((double *)(smem + (off)))[(i)])
 
+// 5-term log K(T) (reaction_aux.F90:1461-1488) evaluated without fused
+// multiply-add, in the reference's order: the fitted coefficients cancel to
+// ~1e-3 of their terms, so contraction alone would move K by ~1e-12.
+__device__ __forceinline__ double logk_5term(int cfo, double tk) {
+#pragma clang fp contract(off)
+  double v = LD(cfo, 0) * log(tk);
+  v = v + LD(cfo, 1);
+  v = v + LD(cfo, 2) * tk;
+  v = v + LD(cfo, 3) / tk;
+  v = v + LD(cfo, 4) / (tk * tk);
+  return v;
+}
+
 // per-group scratch slice (doubles), as byte offsets from the slice base:
 //   C[M+1] complex molalities (C[M] = 0, padding slot), gsec[M] complex
 //   activity coefficients, lnK[M] = -log10 K * LOG_TO_LN of this cell,
@@ -275,7 +288,7 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
             double lk = LD(L.o_clogK, cc);
             if (P.temperature && L.has_logk_coef) {
               const int cfo = (int)L.o_clogKcoef + 5 * cc * 8;
-              lk = LD(cfo, 0) * log(tk) + LD(cfo, 1) + LD(cfo, 2) * tk + LD(cfo, 3) / tk + LD(cfo, 4) / (tk * tk);
+              lk = logk_5term(cfo, tk);
             }
             SD(S.lnK, cc) = -lk * PFX_LOG_TO_LN;
           }
@@ -283,7 +296,7 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
             double lk = LD(L.o_mlogK, k);
             if (P.temperature && L.has_mnrl_coef) {
               const int cfo = (int)L.o_mlogKcoef + 5 * k * 8;
-              lk = LD(cfo, 0) * log(tk) + LD(cfo, 1) + LD(cfo, 2) * tk + LD(cfo, 3) / tk + LD(cfo, 4) / (tk * tk);
+              lk = logk_5term(cfo, tk);
             }
             SD(S.mlnK, k) = -lk * PFX_LOG_TO_LN;
             double arr = 1.0;
@@ -319,7 +332,7 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
     if (MODE == PFX_MODE_REACT)
       act_now = phase == 1 || (phase == 2 && L.act_freq == PFX_ACT_NEWTON_ITER);
     else if (MODE == PFX_MODE_EQUIL)
-      act_now = compute_act;
+      act_now = compute_act && L.act_freq != PFX_ACT_OFF;
     else if (MODE == PFX_MODE_AUX)
       act_now = it >= 1;
     if (active && act_now) {
@@ -688,12 +701,14 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
           finish = true;
           status = PFX_CELL_MAX_ITERATIONS;
         } else if (rr < L.max_residual_tol && rel < L.max_rel_change_tol) {
+          // activity switches on after the first convergence; at least two
+          // more iterations follow (reaction.F90:1952-1966).  With activity
+          // off the switch is still taken, only the update is skipped.
           if (compute_act && it - nit_on > 1) {
             finish = true;
           } else {
             if (!compute_act) nit_on = it;
-            compute_act = L.act_freq != PFX_ACT_OFF;
-            if (L.act_freq == PFX_ACT_OFF && it - nit_on > 1) finish = true;
+            compute_act = true;
           }
         }
       }
