@@ -155,7 +155,7 @@ def cpu_baseline(t, sample):
     from pflotran_amd import synth
 
     threads = min(os.cpu_count() or 1, 16)
-    n = sample or 1500 * threads
+    n = sample or 12000 * threads
     cells, _, _, _ = orc.speciated_state(t, n, nthreads=threads)
     work = cells.copy()
     t0 = time.perf_counter()
