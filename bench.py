@@ -33,6 +33,8 @@ def main():
     ap.add_argument("--ncells", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0, help="cells in the CPU baseline sample (0 = auto)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain 256 MiB device copy (HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the per-cell reaction step shards by "
@@ -140,6 +142,15 @@ def main():
                      "algorithmic_bytes_per_cell": bytes_per_cell,
                      "achieved_GBs": bytes_per_cell * n / (kern_avg_ms * 1e-3) / 1e9},
     }
+    if args.calibrate_copy:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
+        # calibrate the byte counters (256 MiB read + 256 MiB written)
+        x = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda").fill_(1.0)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        line["copy_calibration_bytes"] = x.numel() * 8
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(t, args.cpu_sample)
     print(json.dumps(line), flush=True)

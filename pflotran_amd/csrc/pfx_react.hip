@@ -40,40 +40,81 @@ __device__ __forceinline__ int bperm_i(int v, int src_lane) {
   return __builtin_amdgcn_ds_bpermute(src_lane << 2, v);
 }
 
-// group-wide reductions by xor butterfly: every lane of the group ends with
-// the bitwise-same value (each step combines a commutative pair).
-template <int G>
-__device__ __forceinline__ double gsum(double v, int lane) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += bperm_d(v, lane ^ o);
-  return v;
+// Group-wide reductions.  Within a DPP row (16 lanes) the exchanges are
+// DPP moves (VALU, a few cycles each) instead of LDS-crossbar permutes:
+//   G = 16: row rotations by 8, 4, 2, 1;
+//   G = 8 : half-row mirror, then quad permutations xor 2 and xor 1;
+//   G = 32: the G = 16 steps, then one permute across the two rows.
+// Every step combines a commutative pair, so all lanes of the group end
+// with the bitwise-same value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
-template <int G>
-__device__ __forceinline__ double gmax(double v, int lane) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) {
-    double w = bperm_d(v, lane ^ o);
-    v = (w > v || w != w) ? w : v;  // NaN wins: a broken cell is seen as broken
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+constexpr int DPP_ROR8 = 0x128, DPP_ROR4 = 0x124, DPP_ROR2 = 0x122, DPP_ROR1 = 0x121;
+constexpr int DPP_HALF_MIRROR = 0x141, DPP_XOR2 = 0x4E, DPP_XOR1 = 0xB1;
+
+struct OpSum {
+  __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
+};
+struct OpMax {  // NaN wins: a broken cell is seen as broken
+  __device__ __forceinline__ double operator()(double a, double b) const {
+    return (b > a || b != b) ? b : a;
+  }
+};
+struct OpMin {
+  __device__ __forceinline__ double operator()(double a, double b) const { return fmin(a, b); }
+};
+
+template <int G, class Op>
+__device__ __forceinline__ double greduce(double v, int lane, Op op) {
+  if constexpr (G == 8) {
+    v = op(v, dpp_d<DPP_HALF_MIRROR>(v));
+    v = op(v, dpp_d<DPP_XOR2>(v));
+    v = op(v, dpp_d<DPP_XOR1>(v));
+  } else {
+    v = op(v, dpp_d<DPP_ROR8>(v));
+    v = op(v, dpp_d<DPP_ROR4>(v));
+    v = op(v, dpp_d<DPP_ROR2>(v));
+    v = op(v, dpp_d<DPP_ROR1>(v));
+    if constexpr (G == 32) v = op(v, bperm_d(v, lane ^ 16));
   }
   return v;
 }
 template <int G>
-__device__ __forceinline__ double gmin(double v, int lane) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v = fmin(v, bperm_d(v, lane ^ o));
-  return v;
-}
+__device__ __forceinline__ double gsum(double v, int lane) { return greduce<G>(v, lane, OpSum()); }
+template <int G>
+__device__ __forceinline__ double gmax(double v, int lane) { return greduce<G>(v, lane, OpMax()); }
+template <int G>
+__device__ __forceinline__ double gmin(double v, int lane) { return greduce<G>(v, lane, OpMin()); }
+
 // argmax with ties resolved to the larger row index
+__device__ __forceinline__ void amax_step(double &v, int &idx, double w, int j) {
+  if (w > v || (w == v && j > idx)) {
+    v = w;
+    idx = j;
+  }
+}
 template <int G>
 __device__ __forceinline__ void gargmax(double &v, int &idx, int lane) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) {
-    double w = bperm_d(v, lane ^ o);
-    int j = bperm_i(idx, lane ^ o);
-    if (w > v || (w == v && j > idx)) {
-      v = w;
-      idx = j;
-    }
+  if constexpr (G == 8) {
+    amax_step(v, idx, dpp_d<DPP_HALF_MIRROR>(v), dpp_i<DPP_HALF_MIRROR>(idx));
+    amax_step(v, idx, dpp_d<DPP_XOR2>(v), dpp_i<DPP_XOR2>(idx));
+    amax_step(v, idx, dpp_d<DPP_XOR1>(v), dpp_i<DPP_XOR1>(idx));
+  } else {
+    amax_step(v, idx, dpp_d<DPP_ROR8>(v), dpp_i<DPP_ROR8>(idx));
+    amax_step(v, idx, dpp_d<DPP_ROR4>(v), dpp_i<DPP_ROR4>(idx));
+    amax_step(v, idx, dpp_d<DPP_ROR2>(v), dpp_i<DPP_ROR2>(idx));
+    amax_step(v, idx, dpp_d<DPP_ROR1>(v), dpp_i<DPP_ROR1>(idx));
+    if constexpr (G == 32) amax_step(v, idx, bperm_d(v, lane ^ 16), bperm_i(idx, lane ^ 16));
   }
 }
 
@@ -375,9 +416,13 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
     if (active) {
       for (int cc = r; cc < M; cc += G) {
         double lnQK = SD(S.lnK, cc);
-        for (int s = 0; s < L.S; s++) {
-          const int sp = LI(L.o_cspec, cc * L.S + s);
-          if (sp >= 0) lnQK += LD(L.o_cnu, cc * L.S + s) * SD(S.la, sp);
+        const int nS = L.S;
+#pragma unroll
+        for (int s = 0; s < PFX_MAX_RXN_SPEC; s++) {
+          if (s < nS) {
+            const int sp = LI(L.o_cspec, cc * nS + s);
+            if (sp >= 0) lnQK += LD(L.o_cnu, cc * nS + s) * SD(S.la, sp);
+          }
         }
         SD(S.C, cc) = exp(lnQK) / SD(S.gsec, cc);
       }
@@ -391,6 +436,7 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
     if (active) {
       double diag = 0.0;
       const int maxdeg = L.maxdeg;
+#pragma unroll 2
       for (int e = 0; e < maxdeg; e++) {
         const int cc = LI(L.o_adj_c, e * G + r);
         const double nu = LD(L.o_adj_nu, e * G + r);
@@ -401,10 +447,18 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
       }
 #pragma unroll
       for (int k = 0; k < G; k++) {
-        double acc = 0.0;
+        // two independent partial sums keep two LDS gathers in flight
+        double acc0 = 0.0, acc1 = 0.0;
         const int e0 = L.off_start[k], e1 = L.off_start[k + 1];
-        for (int e = e0; e < e1; e++)
-          acc += LD(L.o_off_coef, e * G + r) * SD(S.C, LI(L.o_off_c, e * G + r));
+        int e = e0;
+        for (; e + 1 < e1; e += 2) {
+          const int c0 = LI(L.o_off_c, e * G + r), c1 = LI(L.o_off_c, (e + 1) * G + r);
+          const double f0 = LD(L.o_off_coef, e * G + r), f1 = LD(L.o_off_coef, (e + 1) * G + r);
+          acc0 += f0 * SD(S.C, c0);
+          acc1 += f1 * SD(S.C, c1);
+        }
+        if (e < e1) acc0 += LD(L.o_off_coef, e * G + r) * SD(S.C, LI(L.o_off_c, e * G + r));
+        const double acc = acc0 + acc1;
         row[k] = (k == r) ? (live ? 1.0 + diag * SD(S.im, k) : 1.0) : acc * SD(S.im, k);
       }
 #pragma unroll
