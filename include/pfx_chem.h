@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PFX_ABI_VERSION 1
+#define PFX_ABI_VERSION 2
 
 /* largest primary-species count handled by one cell group (row per lane) */
 #define PFX_MAX_PRIMARY 32
@@ -74,6 +74,10 @@ extern "C" {
 #define PFX_GUESS_PREVIOUS  0   /* previous free ions (RReact's guess, reaction.F90:3370) */
 #define PFX_GUESS_ARRAY     1   /* caller array cells->guess (e.g. learned warm start) */
 #define PFX_GUESS_DEFAULT   2   /* constant opts->default_guess (1e-9 molal, reaction.F90:1481) */
+#define PFX_GUESS_ARRAY_FALLBACK 3 /* cells->guess; a cell not converged after
+                                     opts->fallback_after iterations (or that
+                                     fails) restarts from opts->default_guess
+                                     with the full max_iterations budget */
 
 /*
  * Flattened reaction_type (reaction_aux.F90:142-335, mineral_type
@@ -174,6 +178,9 @@ typedef struct pfx_react_opts {
   double default_guess;   /* PFX_GUESS_DEFAULT value (molal)                  */
   double porosity, saturation, volume, den_kg; /* used when arrays are NULL   */
   void *stream;           /* hipStream_t, NULL = default stream               */
+  int32_t fallback_after; /* PFX_GUESS_ARRAY_FALLBACK: iterations of the warm
+                             attempt (0 = max_iterations)                     */
+  int32_t reserved1;
 } pfx_react_opts;
 
 typedef struct pfx_stats {
@@ -185,6 +192,7 @@ typedef struct pfx_stats {
   double max_rel_res;
   double kernel_ms;       /* device time of the batch (HIP events)             */
   int64_t iteration_histogram[32]; /* [min(it,31)]                             */
+  int64_t fallbacks;      /* cells restarted from the default guess            */
 } pfx_stats;
 
 typedef struct pfx_tables pfx_tables;   /* opaque, owns device copies */

@@ -1,0 +1,93 @@
+/*
+ * pfx_surrogate.h -- learned warm start of the batched reaction step.
+ *
+ * No counterpart in the reference: RReact starts its Newton iteration from
+ * the previous free-ion molalities (reaction.F90:3370), or from 1e-9 molal
+ * after constraint speciation (reaction.F90:1481).  These entry points add
+ * a multilayer perceptron that maps a cell's transported totals, sorbed
+ * totals, temperature and mineral volume fractions to the log free-ion
+ * molalities after the step, used as the initial iterate of
+ * pfx_react_batch with PFX_GUESS_ARRAY_FALLBACK.  The FP64 Newton from the
+ * default guess stays the fallback, and every returned state passes the
+ * same FP64 convergence test.
+ *
+ * Matrix products run in BF16 on the matrix cores with FP32 accumulation;
+ * master weights, gradients and Adam moments are FP32.  All array arguments
+ * are DEVICE pointers unless stated; `stream` is a hipStream_t (NULL =
+ * default stream).  Return 0 on success or a PFX_E_* code.
+ *
+ * Network layout: dims[0] inputs, dims[1..nlayers-1] hidden widths (GELU),
+ * dims[nlayers] outputs (linear).  Weight matrix of layer l is
+ * [dims[l+1]][dims[l]] row-major.
+ *
+ * Features of a cell (pfx_surrogate_features, pfx_surrogate_guess), N
+ * primaries, K minerals, slots:
+ *   [0, N)        slog(total[j])         slog(v) = sign(v) log10(1 + |v| 1e12)
+ *   [N, 2N)       slog(total_sorb[j])    (0 without ion exchange)
+ *   2N            temperature (C)        (25 if the array is NULL)
+ *   [2N+1, 2N+1+K) mineral volume fraction
+ *   rest          0
+ * each normalised as (x - in_shift) * in_scale.  Outputs are normalised
+ * ln molality: ln m_j = y_j / out_scale_j + out_shift_j.
+ */
+#ifndef PFX_SURROGATE_H
+#define PFX_SURROGATE_H
+
+#include <stdint.h>
+
+#include "pfx_chem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pfx_mlp pfx_mlp;   /* opaque, owns device weights and buffers */
+
+/* Network with random (seeded) initial weights and activation buffers for
+   up to batch_max training rows.  nlayers <= 8, every width <= 4096. */
+int pfx_mlp_create(int32_t nlayers, const int32_t *dims, int32_t batch_max, uint64_t seed, pfx_mlp **out);
+int pfx_mlp_destroy(pfx_mlp *mlp);
+
+/* HOST <-> device copy of one parameter array: which = 0 weights of
+   `layer`, 1 biases, 2 weight gradients, 3 bias gradients, 4 normalisation
+   (in_shift[dims0], in_scale[dims0], out_shift[out], out_scale[out]
+   concatenated, layer ignored).  to_device != 0 uploads. */
+int pfx_mlp_params(pfx_mlp *mlp, int32_t which, int32_t layer, float *host, int32_t to_device);
+
+/* Inference Y[B][out] = f(X[B][dims0]) with the BF16 product kernels. */
+int pfx_mlp_forward(pfx_mlp *mlp, const float *X, int64_t B, float *Y, void *stream);
+
+/* Gradients of the mean squared error mean((f(X) - T)^2) over B <=
+   batch_max rows into the gradient arrays; *loss_out (HOST, may be NULL)
+   receives the loss (the call then waits for the stream). */
+int pfx_mlp_gradients(pfx_mlp *mlp, const float *X, const float *T, int32_t B, float *loss_out, void *stream);
+
+/* One Adam step (bias-corrected) on the master weights from the current
+   gradients. */
+int pfx_mlp_adam(pfx_mlp *mlp, float lr, float beta1, float beta2, float eps, void *stream);
+
+/* pfx_mlp_gradients followed by pfx_mlp_adam(lr, 0.9, 0.999, 1e-8). */
+int pfx_mlp_train_step(pfx_mlp *mlp, const float *X, const float *T, int32_t B, float lr, float *loss_out,
+                       void *stream);
+
+/* Normalised features X[ncells][dims0] of a batch of cells (uses total,
+   total_sorb, temperature, mnrl_volfrac). */
+int pfx_surrogate_features(pfx_tables *tables, pfx_mlp *mlp, const pfx_cells *cells, float *X, void *stream);
+
+/* Fused warm start: features, every layer and the guess in one launch,
+   cells->guess[ncells*N] = exp(ln m) in FP64.  Needs dims0 <= 256, hidden
+   widths multiples of 64 up to 256, 1 <= outputs <= 16 and N <= outputs
+   (else PFX_E_UNSUP; pfx_surrogate_features + pfx_mlp_forward +
+   pfx_surrogate_guess_from cover other shapes).  Waits for nothing. */
+int pfx_surrogate_guess(pfx_tables *tables, pfx_mlp *mlp, pfx_cells *cells, void *stream);
+
+/* Guess from network outputs Y[ncells][out]: guess[c*N+j] =
+   exp(Y[c][j] / out_scale_j + out_shift_j). */
+int pfx_surrogate_guess_from(pfx_mlp *mlp, const float *Y, int64_t ncells, int32_t naqcomp, double *guess,
+                             void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFX_SURROGATE_H */

@@ -659,7 +659,7 @@ static void or_load_cell(const pfx_reaction_desc *d, const pfx_cells *cl, const 
 /* One operator-split reaction step for one cell (RReact,
    reaction.F90:3322-3511), solving the dt-consistent residual. */
 static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_opts *o, int64_t ci,
-                         int *iters_out, double *relres_out) {
+                         int *iters_out, double *relres_out, int *fallbacks_out) {
   const int N = d->naqcomp, M = d->neqcplx, K = d->nkinmnrl, R = d->neqionxrxn;
   ocell c;
   or_load_cell(d, cl, o, ci, &c);
@@ -681,13 +681,20 @@ static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
     or_activity(d, &c);
   }
   /* initial iterate */
+  const int warm = o->guess_mode == PFX_GUESS_ARRAY || o->guess_mode == PFX_GUESS_ARRAY_FALLBACK;
   for (int j = 0; j < N; j++) {
-    if (o->guess_mode == PFX_GUESS_ARRAY && cl->guess) c.m[j] = cl->guess[ci * N + j];
+    if (warm && cl->guess) c.m[j] = cl->guess[ci * N + j];
     else if (o->guess_mode == PFX_GUESS_DEFAULT) c.m[j] = o->default_guess;
   }
   int it = 0, status = PFX_CELL_MAX_ITERATIONS, extra = 0, done = 0;
+  /* learned warm start with fallback: the warm attempt is capped at
+     fallback_after iterations; a failed attempt restarts from the default
+     guess with the full max_iterations budget (no reference counterpart) */
+  int fallback = o->guess_mode == PFX_GUESS_ARRAY_FALLBACK;
+  int cap = fallback && o->fallback_after > 0 ? o->fallback_after : o->max_iterations;
   double rr = INFINITY;
   double Rv[NMAX], J[NMAX][NMAX], sc[NMAX], upd[NMAX];
+restart:
   while (!done) {
     if (d->act_coef_update_frequency == PFX_ACT_NEWTON_ITER) {
       or_rtotal(&*d, &c);
@@ -705,7 +712,7 @@ static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
       for (int j = 0; j < N; j++) mx = fmax(mx, fabs(Rv[j] * dt)); /* reference units: mol */
       if (mx < d->max_residual_tolerance) { status = PFX_CELL_CONVERGED; break; }
     }
-    if (it >= o->max_iterations) { status = PFX_CELL_MAX_ITERATIONS; break; }
+    if (it >= cap) { status = PFX_CELL_MAX_ITERATIONS; break; }
     it++;
     if (or_rsolve(Rv, J, c.m, upd, N, d->use_log_formulation)) { status = PFX_CELL_SINGULAR; break; }
     double prev[NMAX], maxrel = 0.0;
@@ -733,6 +740,17 @@ static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
       status = PFX_CELL_CONVERGED;
       done = 1;
     }
+  }
+  if (c.status) status = c.status;
+  if (fallback && status != PFX_CELL_CONVERGED) {
+    fallback = 0;
+    c.status = 0;
+    cap = it + o->max_iterations;
+    extra = 0;
+    for (int j = 0; j < N; j++) c.m[j] = o->default_guess;
+    for (int r = 0; r < R; r++) c.ionx_ref[r] = cl->ionx_ref_sorbed ? cl->ionx_ref_sorbed[ci * R + r] : 1e-9;
+    *fallbacks_out = 1;
+    goto restart;
   }
   /* "one last update" (reaction.F90:3507): outputs at the returned iterate
      (or_res_jac above already left totals, sorbed totals and rates there) */
@@ -767,18 +785,19 @@ int or_react_batch(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_op
   if (!d || !cl || !o) return PFX_E_ARG;
   if (d->naqcomp > NMAX || d->neqcplx > MMAX || d->nkinmnrl > KMAX || d->neqionxrxn > RMAX) return PFX_E_UNSUP;
   const int64_t n = cl->ncells;
-  int64_t conv = 0, tot_it = 0, hist[32] = {0};
+  int64_t conv = 0, tot_it = 0, nfb = 0, hist[32] = {0};
   int maxit = 0, minit = 1 << 30;
   double maxrr = 0.0;
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) reduction(+ : conv, tot_it) \
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) reduction(+ : conv, tot_it, nfb) \
     reduction(max : maxit, maxrr) reduction(min : minit)
 #endif
   for (int64_t ci = 0; ci < n; ci++) {
-    int it;
+    int it, fb = 0;
     double rr;
-    int st = or_react_cell(d, cl, o, ci, &it, &rr);
+    int st = or_react_cell(d, cl, o, ci, &it, &rr, &fb);
+    nfb += fb;
     if (cl->num_iterations) cl->num_iterations[ci] = it;
     if (cl->status) cl->status[ci] = st;
     if (cl->rel_res) cl->rel_res[ci] = rr;
@@ -795,6 +814,7 @@ int or_react_batch(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_op
     stats->max_iterations = maxit;
     stats->min_iterations = minit;
     stats->max_rel_res = maxrr;
+    stats->fallbacks = nfb;
     if (cl->num_iterations)
       for (int64_t ci = 0; ci < n; ci++) hist[cl->num_iterations[ci] < 31 ? cl->num_iterations[ci] : 31]++;
     memcpy(stats->iteration_histogram, hist, sizeof(hist));

@@ -5,7 +5,7 @@ the field offsets against the header compiled with the host compiler.
 """
 import ctypes as C
 
-PFX_ABI_VERSION = 1
+PFX_ABI_VERSION = 2
 PFX_MAX_PRIMARY = 32
 PFX_MAX_CPLX = 256
 PFX_MAX_RXN_SPEC = 8
@@ -19,7 +19,7 @@ PFX_ACT_OFF, PFX_ACT_TIMESTEP, PFX_ACT_NEWTON_ITER = 0, 1, 2
 PFX_ACT_ALG_LAG, PFX_ACT_ALG_NEWTON = 0, 1
 
 PFX_CELL_CONVERGED, PFX_CELL_MAX_ITERATIONS, PFX_CELL_NONFINITE, PFX_CELL_SINGULAR = 0, 1, 2, 3
-PFX_GUESS_PREVIOUS, PFX_GUESS_ARRAY, PFX_GUESS_DEFAULT = 0, 1, 2
+PFX_GUESS_PREVIOUS, PFX_GUESS_ARRAY, PFX_GUESS_DEFAULT, PFX_GUESS_ARRAY_FALLBACK = 0, 1, 2, 3
 PFX_CONSTRAINT_TOTAL, PFX_CONSTRAINT_FREE, PFX_CONSTRAINT_LOG, PFX_CONSTRAINT_PH = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
@@ -67,6 +67,7 @@ class ReactOpts(C.Structure):
         ("extra_iterations", C.c_int32), ("guess_mode", C.c_int32), ("reference_stop", C.c_int32),
         ("default_guess", C.c_double), ("porosity", C.c_double), ("saturation", C.c_double),
         ("volume", C.c_double), ("den_kg", C.c_double), ("stream", C.c_void_p),
+        ("fallback_after", C.c_int32), ("reserved1", C.c_int32),
     ]
 
 
@@ -75,6 +76,7 @@ class Stats(C.Structure):
         ("ncells", C.c_int64), ("converged", C.c_int64), ("total_iterations", C.c_int64),
         ("max_iterations", C.c_int32), ("min_iterations", C.c_int32), ("max_rel_res", C.c_double),
         ("kernel_ms", C.c_double), ("iteration_histogram", C.c_int64 * 32),
+        ("fallbacks", C.c_int64),
     ]
 
     def as_dict(self):
@@ -83,6 +85,7 @@ class Stats(C.Structure):
             "total_iterations": self.total_iterations, "max_iterations": self.max_iterations,
             "min_iterations": self.min_iterations, "max_rel_res": self.max_rel_res,
             "kernel_ms": self.kernel_ms, "iteration_histogram": list(self.iteration_histogram),
+            "fallbacks": self.fallbacks,
         }
 
 
@@ -91,7 +94,7 @@ CELL_FIELDS = [f for f, _ in Cells._fields_ if f != "ncells"]
 
 def default_opts(dt=1.0, rel_tol=1e-10, max_iterations=100, extra_iterations=0,
                  guess_mode=PFX_GUESS_PREVIOUS, reference_stop=0, default_guess=1e-9,
-                 porosity=0.25, saturation=1.0, volume=1.0, den_kg=997.0, stream=None):
+                 porosity=0.25, saturation=1.0, volume=1.0, den_kg=997.0, stream=None, fallback_after=0):
     o = ReactOpts()
     o.dt = dt
     o.rel_tol = rel_tol
@@ -102,4 +105,5 @@ def default_opts(dt=1.0, rel_tol=1e-10, max_iterations=100, extra_iterations=0,
     o.default_guess = default_guess
     o.porosity, o.saturation, o.volume, o.den_kg = porosity, saturation, volume, den_kg
     o.stream = stream
+    o.fallback_after = fallback_after
     return o

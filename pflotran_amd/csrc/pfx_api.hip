@@ -164,7 +164,7 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   int64_t per_wave = std::max<int64_t>(1, P.ncells / (blocks * (BLOCK / 64)));
   P.chunk = (int32_t)std::max<int64_t>(64 / G, std::min<int64_t>(64, per_wave / 8 + 1));
   P.queue = t->d_queue;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, t->d_queue, 1);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, t->d_queue, 2);  // cursor, fallbacks
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(t->ev0, stream));
   void *args[] = {(void *)&P};
@@ -212,6 +212,8 @@ void fill_params(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, pfx_k
   P.extra_iterations = o->extra_iterations;
   P.guess_mode = o->guess_mode;
   P.reference_stop = o->reference_stop;
+  P.fallback_after = o->fallback_after > 0 ? o->fallback_after : o->max_iterations;
+  P.fallback_count = t->d_queue + 1;
 }
 
 int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o) {
@@ -220,7 +222,10 @@ int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o
   if (!c->total || !c->pri_molal) return PFX_E_ARG;
   if (t->L.R > 0 && !c->total_sorb) return PFX_E_ARG;
   if (t->L.K > 0 && (!c->mnrl_volfrac || !c->mnrl_area)) return PFX_E_ARG;
-  if (o->guess_mode == PFX_GUESS_ARRAY && !c->guess) return PFX_E_ARG;
+  if ((o->guess_mode == PFX_GUESS_ARRAY || o->guess_mode == PFX_GUESS_ARRAY_FALLBACK) && !c->guess)
+    return PFX_E_ARG;
+  if (o->guess_mode < PFX_GUESS_PREVIOUS || o->guess_mode > PFX_GUESS_ARRAY_FALLBACK) return PFX_E_ARG;
+  if (o->fallback_after < 0) return PFX_E_ARG;
   if (!(o->dt > 0.0)) return PFX_E_ARG;
   return 0;
 }
@@ -233,7 +238,9 @@ int finish_stats(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream
                      t->d_stats);
   HIPCHK(hipGetLastError());
   unsigned long long h[64];
+  unsigned int nfb = 0;
   HIPCHK(hipMemcpyAsync(h, t->d_stats, sizeof(h), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(&nfb, t->d_queue + 1, sizeof(nfb), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
   const int64_t conv = (int64_t)h[0];
   if (stats) {
@@ -248,6 +255,7 @@ int finish_stats(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream
     stats->max_rel_res = rr;
     stats->kernel_ms = ms;
     for (int i = 0; i < 32; i++) stats->iteration_histogram[i] = (int64_t)h[5 + i];
+    stats->fallbacks = P.guess_mode == PFX_GUESS_ARRAY_FALLBACK ? (int64_t)nfb : 0;
   }
   return (int)(n - conv);
 }
@@ -264,6 +272,13 @@ int use_scratch(pfx_tables *t, pfx_kernel_params &P, int64_t n) {
 }
 
 }  // namespace
+
+void pfx_internal::table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R) {
+  *N = t->L.N;
+  *M = t->L.M;
+  *K = t->L.K;
+  *R = t->L.R;
+}
 
 // ---------------------------------------------------------------------------
 extern "C" int pfx_abi_version(void) { return PFX_ABI_VERSION; }

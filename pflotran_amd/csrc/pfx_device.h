@@ -71,6 +71,8 @@ struct pfx_kernel_params {
   double dt, rel_tol, default_guess;
   double def_porosity, def_saturation, def_volume, def_den;
   int32_t max_iterations, extra_iterations, guess_mode, reference_stop;
+  int32_t fallback_after;       // warm-attempt iteration cap (PFX_GUESS_ARRAY_FALLBACK)
+  unsigned int *fallback_count; // cells restarted from the default guess
   double *res_out, *jac_out;    // residual/Jacobian outputs
   // constraint speciation
   const double *constraint_conc;   // [ncells*N]
@@ -81,3 +83,9 @@ struct pfx_kernel_params {
   unsigned int *queue;          // global cell cursor
   int32_t chunk;                // cells taken per fetch
 };
+
+// Sizes of a network held by an opaque pfx_tables (defined in pfx_api.hip)
+// for the other translation units of the library.
+namespace pfx_internal {
+void table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R);
+}

@@ -1,0 +1,731 @@
+// Learned warm start of the batched reaction step (include/pfx_surrogate.h):
+// a multilayer perceptron whose matrix products run on the gfx950 matrix
+// cores (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation).
+//
+//  * training: a tiled BF16 GEMM (64x64 block tile, 2x2 MFMA tiles per
+//    wavefront) with fused bias/GELU epilogue for the forward pass, the same
+//    GEMM on transposed operands (split-K, atomic FP32 accumulation) for the
+//    weight gradients, elementwise GELU backward, and Adam on FP32 master
+//    weights;
+//  * inference: one fused kernel per 64 cells that computes the features,
+//    keeps every activation in LDS (BF16, two ping-pong buffers), streams
+//    the weights from L2 in the exact MFMA fragment order (pre-swizzled,
+//    one 16-byte load per lane per fragment), and writes the FP64 Newton
+//    guess.  Inference computes D = W x X^T: rows are output features and
+//    columns are cells, so each lane ends with 4 consecutive features of
+//    one cell and stores them as one 8-byte LDS write.
+//
+// MFMA 16x16x32 operand layout used throughout (lane l of 64):
+//   A (16 x 32): row l%16, k = 8*(l/16) + 0..7
+//   B (32 x 16): col l%16, k = 8*(l/16) + 0..7
+//   D (16 x 16): col l%16, rows 4*(l/16) + 0..3
+// (the k order inside a fragment only has to agree between A and B).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/pfx_surrogate.h"
+#include "pfx_device.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define MLP_MAX_LAYERS 8
+
+struct pfx_mlp {
+  int nlayers;
+  int dims[MLP_MAX_LAYERS + 1];
+  float *W[MLP_MAX_LAYERS], *b[MLP_MAX_LAYERS];  // FP32 master, W[l]: [dims[l+1]][dims[l]]
+  float *gW[MLP_MAX_LAYERS], *gb[MLP_MAX_LAYERS];
+  float *mW[MLP_MAX_LAYERS], *vW[MLP_MAX_LAYERS], *mb[MLP_MAX_LAYERS], *vb[MLP_MAX_LAYERS];
+  uint4 *Wsw[MLP_MAX_LAYERS];  // BF16 weights in MFMA fragment order (inference)
+  bool sw_dirty;
+  float *in_shift, *in_scale, *out_shift, *out_scale;
+  int bmax;
+  float *Z[MLP_MAX_LAYERS];      // pre-activations [bmax][dims[l+1]]
+  float *A[MLP_MAX_LAYERS + 1];  // activations; A[0] is the caller's input
+  float *dA, *dZ;                // [bmax][max width]
+  float *loss;
+  int adam_step;
+};
+
+namespace {
+
+#define MCHK(x)                                                               \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess) {                                                   \
+      fprintf(stderr, "pfx_mlp: %s failed: %s (%s:%d)\n", #x,                 \
+              hipGetErrorString(e_), __FILE__, __LINE__);                     \
+      return PFX_E_DEVICE;                                                    \
+    }                                                                         \
+  } while (0)
+
+__host__ __device__ inline int roundup(int v, int m) { return (v + m - 1) / m * m; }
+
+// FP32 -> BF16, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+__device__ __forceinline__ bf16x8 as_frag(uint4 u) { return __builtin_bit_cast(bf16x8, u); }
+
+__device__ __forceinline__ float slog(double v) {
+  const float s = log10f(1.0f + (float)fabs(v) * 1e12f);
+  return v < 0.0 ? -s : s;
+}
+
+// raw feature f of cell c (slots documented in pfx_surrogate.h)
+__device__ __forceinline__ float feature_raw(int f, int64_t c, int N, int K, int R, const double *total,
+                                             const double *sorb, const double *temp, const double *vf) {
+  if (f < N) return slog(total[c * N + f]);
+  if (f < 2 * N) return (R > 0 && sorb) ? slog(sorb[c * N + (f - N)]) : 0.0f;
+  if (f == 2 * N) return temp ? (float)temp[c] : 25.0f;
+  if (f < 2 * N + 1 + K) return vf ? (float)vf[c * K + (f - 2 * N - 1)] : 0.0f;
+  return 0.0f;
+}
+
+enum { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_ATOMIC = 3 };
+
+constexpr int BM = 64, BN = 64, BK = 32, LDSK = BK + 8;
+
+// C[m][n] (op)= sum_k A(m,k) B(n,k) with A(m,k) = A[m*sam + k*sak] and
+// B(n,k) = B[n*sbn + k*sbk] (FP32 in memory, rounded to BF16 when staged
+// in LDS).  256 threads per 64x64 tile, wavefront w owns the 32x32 quarter
+// (w/2, w%2) as 2x2 MFMA tiles.  blockIdx.z splits K (EPI_ATOMIC only).
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const float *__restrict__ A, long sam,
+                                                   long sak, const float *__restrict__ B, long sbn, long sbk,
+                                                   float *__restrict__ C, long ldc, const float *__restrict__ bias,
+                                                   float *__restrict__ Zout, int ksplit) {
+  __shared__ __attribute__((aligned(16))) unsigned short As[BM][LDSK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[BN][LDSK];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int kchunk = roundup((K + ksplit - 1) / ksplit, BK);
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const int lr = t >> 2, lk = (t & 3) * 8;  // staging: row, first of 8 k
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    {
+      const int m = m0 + lr, n = n0 + lr;
+      unsigned short va[8], vb[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int k = k0 + lk + q;
+        va[q] = f2bf((m < M && k < kend) ? A[(long)m * sam + (long)k * sak] : 0.0f);
+        vb[q] = f2bf((n < N && k < kend) ? B[(long)n * sbn + (long)k * sbk] : 0.0f);
+      }
+      uint4 ua, ub;
+      ua.x = va[0] | ((unsigned)va[1] << 16);
+      ua.y = va[2] | ((unsigned)va[3] << 16);
+      ua.z = va[4] | ((unsigned)va[5] << 16);
+      ua.w = va[6] | ((unsigned)va[7] << 16);
+      ub.x = vb[0] | ((unsigned)vb[1] << 16);
+      ub.y = vb[2] | ((unsigned)vb[3] << 16);
+      ub.z = vb[4] | ((unsigned)vb[5] << 16);
+      ub.w = vb[6] | ((unsigned)vb[7] << 16);
+      *(uint4 *)&As[lr][lk] = ua;
+      *(uint4 *)&Bs[lr][lk] = ub;
+    }
+    __syncthreads();
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    bf16x8 af[2], bfr[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      af[i] = as_frag(*(const uint4 *)&As[wm + 16 * i + fr][fk]);
+      bfr[i] = as_frag(*(const uint4 *)&Bs[wn + 16 * i + fr][fk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + q;
+        const int n = n0 + wn + 16 * j + (lane & 15);
+        if (m < M && n < N) {
+          float v = acc[i][j][q];
+          if (EPI == EPI_ATOMIC) {
+            atomicAdd(&C[(long)m * ldc + n], v);
+          } else if (EPI == EPI_STORE) {
+            C[(long)m * ldc + n] = v;
+          } else {
+            v += bias[n];
+            if (EPI == EPI_BIAS_GELU) {
+              Zout[(long)m * ldc + n] = v;
+              v = gelu(v);
+            }
+            C[(long)m * ldc + n] = v;
+          }
+        }
+      }
+}
+
+__global__ void k_fill(float *p, long n, float v) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_gelu_back(const float *dA, const float *Z, float *dZ, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) dZ[i] = dA[i] * gelu_grad(Z[i]);
+}
+
+// dY = 2 (Y - T) / n and the loss sum((Y - T)^2) / n
+__global__ void k_mse_grad(const float *Y, const float *T, float *dY, float *loss, long n, float inv) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (i < n) {
+    const float d = Y[i] - T[i];
+    dY[i] = 2.0f * d * inv;
+    l = d * d * inv;
+  }
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+}
+
+// db[o] = sum_b dZ[b][o]; one block per 64 columns, rows split over waves
+__global__ __launch_bounds__(256) void k_colsum(const float *dZ, int B, int N, float *db) {
+  __shared__ float red[4][64];
+  const int o = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int w = threadIdx.x >> 6;
+  float s = 0.f;
+  if (o < N)
+    for (int b = w; b < B; b += 4) s += dZ[(long)b * N + o];
+  red[w][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (w == 0 && o < N) db[o] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+__global__ void k_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1, float b2, float c1,
+                       float c2, float eps) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// W[N][K] FP32 -> BF16 fragments: uint4 index ((nt * Kp/32 + kt) * 64 + lane)
+// holds W[nt*16 + lane%16][kt*32 + 8*(lane/16) + 0..7] (zero outside)
+__global__ void k_swizzle(const float *W, int N, int K, int Np, int Kp, uint4 *out) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int kts = Kp / 32;
+  const long total = (long)(Np / 16) * kts * 64;
+  if (i >= total) return;
+  const int lane = (int)(i & 63);
+  const long f = i >> 6;
+  const int kt = (int)(f % kts), nt = (int)(f / kts);
+  const int n = nt * 16 + (lane & 15);
+  const int k0 = kt * 32 + 8 * (lane >> 4);
+  unsigned short v[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) v[q] = f2bf((n < N && k0 + q < K) ? W[(long)n * K + k0 + q] : 0.0f);
+  uint4 u;
+  u.x = v[0] | ((unsigned)v[1] << 16);
+  u.y = v[2] | ((unsigned)v[3] << 16);
+  u.z = v[4] | ((unsigned)v[5] << 16);
+  u.w = v[6] | ((unsigned)v[7] << 16);
+  out[i] = u;
+}
+
+__global__ void k_features(int64_t ncells, int N, int K, int R, const double *total, const double *sorb,
+                           const double *temp, const double *vf, int nfeat, const float *shift, const float *scale,
+                           float *X) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= ncells * nfeat) return;
+  const int64_t c = i / nfeat;
+  const int f = (int)(i % nfeat);
+  X[i] = (feature_raw(f, c, N, K, R, total, sorb, temp, vf) - shift[f]) * scale[f];
+}
+
+__device__ __forceinline__ double guess_of(float y, int j, const float *shift, const float *scale) {
+  const double lnm = (double)y / (double)scale[j] + (double)shift[j];
+  return exp(fmin(fmax(lnm, -700.0), 30.0));
+}
+
+__global__ void k_guess(const float *Y, int nout, const float *shift, const float *scale, double *guess, int64_t n,
+                        int N) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n * N) return;
+  const long c = i / N;
+  const int j = (int)(i % N);
+  guess[i] = guess_of(Y[c * nout + j], j, shift, scale);
+}
+
+// ---------------------------------------------------------------------------
+// fused inference
+struct mlp_dev {
+  int nlayers;
+  int dims[MLP_MAX_LAYERS + 1];
+  int kpad[MLP_MAX_LAYERS];  // input width of layer l rounded up to 32
+  const uint4 *Wsw[MLP_MAX_LAYERS];
+  const float *b[MLP_MAX_LAYERS];
+  const float *in_shift, *in_scale, *out_shift, *out_scale;
+};
+
+constexpr int SC = 64;      // cells per block
+constexpr int ASTR = 264;   // LDS activation row stride (BF16 elements, 528 B)
+
+__global__ __launch_bounds__(256) void k_surrogate_guess(const mlp_dev D, int64_t ncells, int N, int K, int R,
+                                                         const double *__restrict__ total,
+                                                         const double *__restrict__ sorb,
+                                                         const double *__restrict__ temp,
+                                                         const double *__restrict__ vf, double *__restrict__ guess) {
+  __shared__ __attribute__((aligned(16))) unsigned short act[2][SC * ASTR];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t c0 = (int64_t)blockIdx.x * SC;
+  // features of the block's 64 cells, four threads per cell
+  {
+    const int cl = t >> 2, part = t & 3;
+    const int64_t c = c0 + cl;
+    const int nf = D.dims[0], per = D.kpad[0] / 4;
+    for (int f = part * per; f < (part + 1) * per; f++) {
+      float x = 0.0f;
+      if (c < ncells && f < nf)
+        x = (feature_raw(f, c, N, K, R, total, sorb, temp, vf) - D.in_shift[f]) * D.in_scale[f];
+      act[0][cl * ASTR + f] = f2bf(x);
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  const int col = lane & 15, kq = 8 * (lane >> 4), rq = 4 * (lane >> 4);
+  for (int l = 0; l < D.nlayers - 1; l++) {
+    const int kts = D.kpad[l] / 32;
+    const int ntw = D.dims[l + 1] / 64;  // 16-row tiles per wavefront (1..4)
+    const uint4 *Wl = D.Wsw[l];
+    const unsigned short *in = act[cur];
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < kts; kt++) {
+      bf16x8 a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        a[i] = i < ntw ? as_frag(Wl[((long)(w * ntw + i) * kts + kt) * 64 + lane]) : as_frag(make_uint4(0, 0, 0, 0));
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = as_frag(*(const uint4 *)&in[(j * 16 + col) * ASTR + kt * 32 + kq]);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        if (i < ntw)
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    unsigned short *out = act[cur ^ 1];
+    const float *bl = D.b[l];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (i < ntw) {
+        const int n0 = (w * ntw + i) * 16 + rq;
+        const float4 bv = *(const float4 *)&bl[n0];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int cl = j * 16 + col;
+          const unsigned short h0 = f2bf(gelu(acc[i][j][0] + bv.x));
+          const unsigned short h1 = f2bf(gelu(acc[i][j][1] + bv.y));
+          const unsigned short h2 = f2bf(gelu(acc[i][j][2] + bv.z));
+          const unsigned short h3 = f2bf(gelu(acc[i][j][3] + bv.w));
+          uint2 u;
+          u.x = h0 | ((unsigned)h1 << 16);
+          u.y = h2 | ((unsigned)h3 << 16);
+          *(uint2 *)&out[cl * ASTR + n0] = u;
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  // output layer: one 16-row tile, wavefront w takes cells 16w..16w+15
+  {
+    const int l = D.nlayers - 1;
+    const int kts = D.kpad[l] / 32;
+    const uint4 *Wl = D.Wsw[l];
+    const unsigned short *in = act[cur];
+    f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < kts; kt++) {
+      const bf16x8 a = as_frag(Wl[(long)kt * 64 + lane]);
+      const bf16x8 b = as_frag(*(const uint4 *)&in[(w * 16 + col) * ASTR + kt * 32 + kq]);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int64_t c = c0 + w * 16 + col;
+    const int nout = D.dims[D.nlayers];
+    if (c < ncells) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = rq + q;
+        if (j < N && j < nout) guess[c * N + j] = guess_of(acc[q] + D.b[l][j], j, D.out_shift, D.out_scale);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+int launch_gemm(int epi, int M, int N, int K, const float *A, long sam, long sak, const float *B, long sbn,
+                long sbk, float *C, long ldc, const float *bias, float *Zout, int ksplit, hipStream_t s) {
+  if (M <= 0 || N <= 0) return 0;
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, ksplit);
+  switch (epi) {
+    case EPI_STORE:
+      hipLaunchKernelGGL(k_gemm_bf16<EPI_STORE>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbn, sbk, C, ldc,
+                         bias, Zout, ksplit);
+      break;
+    case EPI_BIAS_GELU:
+      hipLaunchKernelGGL(k_gemm_bf16<EPI_BIAS_GELU>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbn, sbk, C,
+                         ldc, bias, Zout, ksplit);
+      break;
+    case EPI_BIAS:
+      hipLaunchKernelGGL(k_gemm_bf16<EPI_BIAS>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbn, sbk, C, ldc,
+                         bias, Zout, ksplit);
+      break;
+    default:
+      hipLaunchKernelGGL(k_gemm_bf16<EPI_ATOMIC>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbn, sbk, C,
+                         ldc, bias, Zout, ksplit);
+  }
+  MCHK(hipGetLastError());
+  return 0;
+}
+
+int fill(float *p, long n, float v, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  MCHK(hipGetLastError());
+  return 0;
+}
+
+int max_width(const pfx_mlp *m) {
+  int d = 0;
+  for (int l = 0; l <= m->nlayers; l++) d = d > m->dims[l] ? d : m->dims[l];
+  return d;
+}
+
+// forward pass of B <= bmax rows; the pre-activations go to Z (training) or
+// to the dZ scratch (inference), the output to Y
+int forward(pfx_mlp *m, const float *X, int B, float *Y, bool keep, hipStream_t s) {
+  const float *in = X;
+  for (int l = 0; l < m->nlayers; l++) {
+    const int K = m->dims[l], N = m->dims[l + 1];
+    const bool last = l == m->nlayers - 1;
+    float *out = last ? Y : m->A[l + 1];
+    float *z = keep ? m->Z[l] : m->dZ;
+    int rc = launch_gemm(last ? EPI_BIAS : EPI_BIAS_GELU, B, N, K, in, K, 1, m->W[l], K, 1, out, N, m->b[l], z, 1,
+                         s);
+    if (rc) return rc;
+    in = out;
+  }
+  return 0;
+}
+
+int refresh_swizzled(pfx_mlp *m, hipStream_t s) {
+  if (!m->sw_dirty) return 0;
+  for (int l = 0; l < m->nlayers; l++) {
+    const int N = m->dims[l + 1], K = m->dims[l];
+    const int Np = roundup(N, 16), Kp = roundup(K, 32);
+    const long total = (long)(Np / 16) * (Kp / 32) * 64;
+    hipLaunchKernelGGL(k_swizzle, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, m->W[l], N, K, Np, Kp,
+                       m->Wsw[l]);
+  }
+  MCHK(hipGetLastError());
+  m->sw_dirty = false;
+  return 0;
+}
+
+bool fused_shape_ok(const pfx_mlp *m, int N) {
+  if (m->nlayers < 2 || m->dims[0] > 256) return false;
+  for (int l = 1; l < m->nlayers; l++)
+    if (m->dims[l] % 64 != 0 || m->dims[l] > 256) return false;
+  const int nout = m->dims[m->nlayers];
+  return nout >= 1 && nout <= 16 && N <= nout;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" int pfx_mlp_create(int32_t nlayers, const int32_t *dims, int32_t batch_max, uint64_t seed,
+                              pfx_mlp **out) {
+  if (!out) return PFX_E_ARG;
+  *out = nullptr;
+  if (!dims || nlayers < 1 || nlayers > MLP_MAX_LAYERS || batch_max < 1) return PFX_E_ARG;
+  for (int l = 0; l <= nlayers; l++)
+    if (dims[l] < 1 || dims[l] > 4096) return PFX_E_ARG;
+  pfx_mlp *m = new pfx_mlp();
+  memset(m, 0, sizeof(*m));
+  m->nlayers = nlayers;
+  for (int l = 0; l <= nlayers; l++) m->dims[l] = dims[l];
+  m->bmax = batch_max;
+  // uniform(-a, a) initial weights, a = sqrt(3 / fan_in), from a seeded
+  // splitmix64 stream on the host; zero biases
+  uint64_t st = seed;
+  auto next = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  int rc = 0;
+  for (int l = 0; l < nlayers && !rc; l++) {
+    const long nw = (long)dims[l + 1] * dims[l];
+    const int nb = dims[l + 1];
+    std::vector<float> w(nw), bz(nb, 0.0f);
+    const double a = std::sqrt(3.0 / dims[l]);
+    for (long i = 0; i < nw; i++) w[i] = (float)(((double)(next() >> 11) * 0x1.0p-53 * 2.0 - 1.0) * a);
+    const long nsw = (long)(roundup(dims[l + 1], 16) / 16) * (roundup(dims[l], 32) / 32) * 64;
+    if (hipMalloc(&m->W[l], nw * 4) != hipSuccess || hipMalloc(&m->b[l], nb * 4) != hipSuccess ||
+        hipMalloc(&m->gW[l], nw * 4) != hipSuccess || hipMalloc(&m->gb[l], nb * 4) != hipSuccess ||
+        hipMalloc(&m->mW[l], nw * 4) != hipSuccess || hipMalloc(&m->vW[l], nw * 4) != hipSuccess ||
+        hipMalloc(&m->mb[l], nb * 4) != hipSuccess || hipMalloc(&m->vb[l], nb * 4) != hipSuccess ||
+        hipMalloc(&m->Wsw[l], nsw * sizeof(uint4)) != hipSuccess ||
+        hipMalloc(&m->Z[l], (long)batch_max * nb * 4) != hipSuccess ||
+        hipMalloc(&m->A[l + 1], (long)batch_max * nb * 4) != hipSuccess) {
+      rc = PFX_E_NOMEM;
+      break;
+    }
+    if (hipMemcpy(m->W[l], w.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->b[l], bz.data(), nb * 4, hipMemcpyHostToDevice) != hipSuccess)
+      rc = PFX_E_DEVICE;
+    if (!rc) rc = fill(m->mW[l], nw, 0.f, 0);
+    if (!rc) rc = fill(m->vW[l], nw, 0.f, 0);
+    if (!rc) rc = fill(m->mb[l], nb, 0.f, 0);
+    if (!rc) rc = fill(m->vb[l], nb, 0.f, 0);
+    if (!rc) rc = fill(m->gW[l], nw, 0.f, 0);
+    if (!rc) rc = fill(m->gb[l], nb, 0.f, 0);
+  }
+  const int md = max_width(m);
+  if (!rc && (hipMalloc(&m->dA, (long)batch_max * md * 4) != hipSuccess ||
+              hipMalloc(&m->dZ, (long)batch_max * md * 4) != hipSuccess || hipMalloc(&m->loss, 4) != hipSuccess ||
+              hipMalloc(&m->in_shift, dims[0] * 4) != hipSuccess ||
+              hipMalloc(&m->in_scale, dims[0] * 4) != hipSuccess ||
+              hipMalloc(&m->out_shift, dims[nlayers] * 4) != hipSuccess ||
+              hipMalloc(&m->out_scale, dims[nlayers] * 4) != hipSuccess))
+    rc = PFX_E_NOMEM;
+  if (!rc) rc = fill(m->in_shift, dims[0], 0.f, 0);
+  if (!rc) rc = fill(m->in_scale, dims[0], 1.f, 0);
+  if (!rc) rc = fill(m->out_shift, dims[nlayers], 0.f, 0);
+  if (!rc) rc = fill(m->out_scale, dims[nlayers], 1.f, 0);
+  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = PFX_E_DEVICE;
+  m->sw_dirty = true;
+  if (rc) {
+    pfx_mlp_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" int pfx_mlp_destroy(pfx_mlp *m) {
+  if (!m) return PFX_E_ARG;
+  for (int l = 0; l < m->nlayers; l++) {
+    (void)hipFree(m->W[l]);
+    (void)hipFree(m->b[l]);
+    (void)hipFree(m->gW[l]);
+    (void)hipFree(m->gb[l]);
+    (void)hipFree(m->mW[l]);
+    (void)hipFree(m->vW[l]);
+    (void)hipFree(m->mb[l]);
+    (void)hipFree(m->vb[l]);
+    (void)hipFree(m->Wsw[l]);
+    (void)hipFree(m->Z[l]);
+    (void)hipFree(m->A[l + 1]);
+  }
+  (void)hipFree(m->dA);
+  (void)hipFree(m->dZ);
+  (void)hipFree(m->loss);
+  (void)hipFree(m->in_shift);
+  (void)hipFree(m->in_scale);
+  (void)hipFree(m->out_shift);
+  (void)hipFree(m->out_scale);
+  delete m;
+  return 0;
+}
+
+extern "C" int pfx_mlp_params(pfx_mlp *m, int32_t which, int32_t layer, float *host, int32_t to_device) {
+  if (!m || !host) return PFX_E_ARG;
+  if (which == 4) {
+    const int ni = m->dims[0], no = m->dims[m->nlayers];
+    float *dst[4] = {m->in_shift, m->in_scale, m->out_shift, m->out_scale};
+    const int len[4] = {ni, ni, no, no};
+    long off = 0;
+    for (int q = 0; q < 4; q++) {
+      if (to_device) MCHK(hipMemcpy(dst[q], host + off, len[q] * 4, hipMemcpyHostToDevice));
+      else MCHK(hipMemcpy(host + off, dst[q], len[q] * 4, hipMemcpyDeviceToHost));
+      off += len[q];
+    }
+    return 0;
+  }
+  if (layer < 0 || layer >= m->nlayers || which < 0 || which > 3) return PFX_E_ARG;
+  float *p = which == 0 ? m->W[layer] : which == 1 ? m->b[layer] : which == 2 ? m->gW[layer] : m->gb[layer];
+  const long n = (which == 0 || which == 2) ? (long)m->dims[layer + 1] * m->dims[layer] : m->dims[layer + 1];
+  if (to_device) {
+    MCHK(hipMemcpy(p, host, n * 4, hipMemcpyHostToDevice));
+    if (which <= 1) m->sw_dirty = true;
+  } else {
+    MCHK(hipMemcpy(host, p, n * 4, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+extern "C" int pfx_mlp_forward(pfx_mlp *m, const float *X, int64_t B, float *Y, void *stream) {
+  if (!m || B < 0 || (B > 0 && (!X || !Y))) return PFX_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  for (int64_t b0 = 0; b0 < B; b0 += m->bmax) {
+    const int nb = (int)((B - b0) < m->bmax ? (B - b0) : m->bmax);
+    int rc = forward(m, X + b0 * m->dims[0], nb, Y + b0 * m->dims[m->nlayers], false, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int pfx_mlp_gradients(pfx_mlp *m, const float *X, const float *T, int32_t B, float *loss_out,
+                                 void *stream) {
+  if (!m || !X || !T || B < 1 || B > m->bmax) return PFX_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int L = m->nlayers;
+  m->A[0] = (float *)X;
+  int rc = forward(m, X, B, m->A[L], true, s);
+  if (rc) return rc;
+  const int no = m->dims[L];
+  if ((rc = fill(m->loss, 1, 0.f, s))) return rc;
+  const long nout = (long)B * no;
+  hipLaunchKernelGGL(k_mse_grad, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s, m->A[L], T, m->dZ, m->loss,
+                     nout, 1.0f / (float)nout);
+  MCHK(hipGetLastError());
+  for (int l = L - 1; l >= 0; l--) {
+    const int K = m->dims[l], N = m->dims[l + 1];
+    // dW[n][k] = sum_b dZ[b][n] A[b][k], split over the batch
+    if ((rc = fill(m->gW[l], (long)N * K, 0.f, s))) return rc;
+    const int ksplit = B >= 4096 ? 16 : (B >= 512 ? 4 : 1);
+    if ((rc = launch_gemm(EPI_ATOMIC, N, K, B, m->dZ, 1, N, m->A[l], 1, K, m->gW[l], K, nullptr, nullptr, ksplit,
+                          s)))
+      return rc;
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, m->dZ, B, N, m->gb[l]);
+    MCHK(hipGetLastError());
+    if (l > 0) {
+      // dA[b][k] = sum_n dZ[b][n] W[n][k]; dZ of layer l-1 = dA * gelu'(Z)
+      if ((rc = launch_gemm(EPI_STORE, B, K, N, m->dZ, N, 1, m->W[l], 1, K, m->dA, K, nullptr, nullptr, 1, s)))
+        return rc;
+      const long n = (long)B * K;
+      hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->dA, m->Z[l - 1], m->dZ,
+                         n);
+      MCHK(hipGetLastError());
+    }
+  }
+  if (loss_out) {
+    MCHK(hipMemcpyAsync(loss_out, m->loss, 4, hipMemcpyDeviceToHost, s));
+    MCHK(hipStreamSynchronize(s));
+  }
+  return 0;
+}
+
+extern "C" int pfx_mlp_adam(pfx_mlp *m, float lr, float beta1, float beta2, float eps, void *stream) {
+  if (!m) return PFX_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  m->adam_step++;
+  const float c1 = 1.0f - powf(beta1, (float)m->adam_step), c2 = 1.0f - powf(beta2, (float)m->adam_step);
+  for (int l = 0; l < m->nlayers; l++) {
+    const long nw = (long)m->dims[l + 1] * m->dims[l], nb = m->dims[l + 1];
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, m->W[l], m->gW[l], m->mW[l],
+                       m->vW[l], nw, lr, beta1, beta2, c1, c2, eps);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, m->b[l], m->gb[l], m->mb[l],
+                       m->vb[l], nb, lr, beta1, beta2, c1, c2, eps);
+  }
+  MCHK(hipGetLastError());
+  m->sw_dirty = true;
+  return 0;
+}
+
+extern "C" int pfx_mlp_train_step(pfx_mlp *m, const float *X, const float *T, int32_t B, float lr, float *loss_out,
+                                  void *stream) {
+  int rc = pfx_mlp_gradients(m, X, T, B, loss_out, stream);
+  if (rc) return rc;
+  return pfx_mlp_adam(m, lr, 0.9f, 0.999f, 1e-8f, stream);
+}
+
+extern "C" int pfx_surrogate_features(pfx_tables *tables, pfx_mlp *m, const pfx_cells *cells, float *X,
+                                      void *stream) {
+  if (!tables || !m || !cells || cells->ncells < 0) return PFX_E_ARG;
+  int N, M, K, R;
+  pfx_internal::table_sizes(tables, &N, &M, &K, &R);
+  if (2 * N + 1 + K > m->dims[0]) return PFX_E_ARG;
+  if (cells->ncells == 0) return 0;
+  if (!X || !cells->total || (R > 0 && !cells->total_sorb)) return PFX_E_ARG;
+  const long n = cells->ncells * (long)m->dims[0];
+  hipLaunchKernelGGL(k_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     cells->ncells, N, K, R, cells->total, cells->total_sorb, cells->temperature,
+                     cells->mnrl_volfrac, m->dims[0], m->in_shift, m->in_scale, X);
+  MCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pfx_surrogate_guess(pfx_tables *tables, pfx_mlp *m, pfx_cells *cells, void *stream) {
+  if (!tables || !m || !cells || cells->ncells < 0) return PFX_E_ARG;
+  int N, M, K, R;
+  pfx_internal::table_sizes(tables, &N, &M, &K, &R);
+  if (2 * N + 1 + K > m->dims[0]) return PFX_E_ARG;
+  if (!fused_shape_ok(m, N)) return PFX_E_UNSUP;
+  if (cells->ncells == 0) return 0;
+  if (!cells->guess || !cells->total || (R > 0 && !cells->total_sorb)) return PFX_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = refresh_swizzled(m, s);
+  if (rc) return rc;
+  mlp_dev D;
+  memset(&D, 0, sizeof(D));
+  D.nlayers = m->nlayers;
+  for (int l = 0; l <= m->nlayers; l++) D.dims[l] = m->dims[l];
+  for (int l = 0; l < m->nlayers; l++) {
+    D.kpad[l] = roundup(m->dims[l], 32);
+    D.Wsw[l] = m->Wsw[l];
+    D.b[l] = m->b[l];
+  }
+  D.in_shift = m->in_shift;
+  D.in_scale = m->in_scale;
+  D.out_shift = m->out_shift;
+  D.out_scale = m->out_scale;
+  const unsigned blocks = (unsigned)((cells->ncells + SC - 1) / SC);
+  hipLaunchKernelGGL(k_surrogate_guess, dim3(blocks), dim3(256), 0, s, D, cells->ncells, N, K, R, cells->total,
+                     cells->total_sorb, cells->temperature, cells->mnrl_volfrac, (double *)cells->guess);
+  MCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pfx_surrogate_guess_from(pfx_mlp *m, const float *Y, int64_t ncells, int32_t naqcomp, double *guess,
+                                        void *stream) {
+  if (!m || ncells < 0 || naqcomp < 1 || naqcomp > m->dims[m->nlayers]) return PFX_E_ARG;
+  if (ncells == 0) return 0;
+  if (!Y || !guess) return PFX_E_ARG;
+  const long n = ncells * naqcomp;
+  hipLaunchKernelGGL(k_guess, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, Y,
+                     m->dims[m->nlayers], m->out_shift, m->out_scale, guess, ncells, naqcomp);
+  MCHK(hipGetLastError());
+  return 0;
+}

@@ -229,6 +229,8 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
   // per-group cell state
   long long cell = -1;
   int phase = 0, it = 0, extra = 0;
+  int it_cap = 0;           // iteration at which the current attempt gives up
+  bool fell_back = false;   // warm attempt abandoned, restarted from the default guess
   bool stop_next = false;   // reference relative-change stop: finish at the next evaluation
   int cell_flag = PFX_CELL_CONVERGED;
   double m = 1.0, gam = 1.0, lng = 0.0, fixed = 0.0, tot_out = 0.0, sorb_out = 0.0;
@@ -284,6 +286,9 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
           phase = (MODE == PFX_MODE_REACT) ? 0 : 2;
           it = 0;
           extra = 0;
+          fell_back = false;
+          it_cap = (MODE == PFX_MODE_REACT && P.guess_mode == PFX_GUESS_ARRAY_FALLBACK) ? P.fallback_after
+                                                                                     : P.max_iterations;
           stop_next = false;
           cell_flag = PFX_CELL_CONVERGED;
           compute_act = false;
@@ -355,7 +360,8 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
           if (MODE == PFX_MODE_REACT && L.act_freq == PFX_ACT_OFF) {
             phase = 2;
             if (live) {
-              if (P.guess_mode == PFX_GUESS_ARRAY) m = P.guess[c * N + r];
+              if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
+                m = P.guess[c * N + r];
               else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
             }
           }
@@ -396,7 +402,8 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
       if (MODE == PFX_MODE_REACT && phase == 1) {
         phase = 2;
         if (live) {
-          if (P.guess_mode == PFX_GUESS_ARRAY) m = P.guess[cell * N + r];
+          if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
+            m = P.guess[cell * N + r];
           else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
         }
       }
@@ -624,7 +631,7 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
           const double mx = gmax<G>(live ? fabs(Rr * dt) : 0.0, lane);
           if (mx < L.max_residual_tol) finish = true;
         }
-        if (!finish && it >= P.max_iterations) {
+        if (!finish && it >= it_cap) {
           finish = true;
           status = PFX_CELL_MAX_ITERATIONS;
         }
@@ -766,6 +773,24 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
           }
         }
       }
+    }
+
+    // ---------------------------------------------------------------
+    // warm start abandoned: restart from the default guess with the
+    // activity coefficients of the step's start and the full budget
+    if (MODE == PFX_MODE_REACT && active && finish && status != PFX_CELL_CONVERGED &&
+        P.guess_mode == PFX_GUESS_ARRAY_FALLBACK && !fell_back) {
+      fell_back = true;
+      finish = false;
+      it_cap = it + P.max_iterations;
+      extra = 0;
+      stop_next = false;
+      cell_flag = PFX_CELL_CONVERGED;
+      if (live) m = P.default_guess;
+#pragma unroll
+      for (int q = 0; q < PFX_MAX_IONX_RXN; q++)
+        ionx_state[q] = (q < R && P.ionx_ref) ? P.ionx_ref[cell * R + q] : 1e-9;
+      if (r == 0) atomicAdd(P.fallback_count, 1u);
     }
 
     // ---------------------------------------------------------------

@@ -18,6 +18,8 @@ program pfx_abi_probe
   print '(a,i0)', 'create_bad ', rc
   rc = pfx_react_batch(c_null_ptr, cells, opts, stats)
   print '(a,i0)', 'react_null ', rc
+  rc = pfx_mlp_destroy(c_null_ptr)
+  print '(a,i0)', 'mlp_null ', rc
   print '(a,i0)', 'sizeof_desc ', c_sizeof(desc)
   print '(a,i0)', 'sizeof_cells ', c_sizeof(cells)
   print '(a,i0)', 'sizeof_opts ', c_sizeof(opts)

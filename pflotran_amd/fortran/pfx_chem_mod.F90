@@ -1,8 +1,7 @@
 ! ISO_C_BINDING interface of libpfx_chem.so (include/pfx_chem.h) for the
 ! PFLOTRAN Fortran host.  The derived types mirror the C structs field by
 ! field; reaction_type and reactive_transport_auxvar_type stay unchanged and
-! are packed into these contiguous arrays by the caller (see the
-! pfx_react_cells routine sketched in INTEGRATION.md).
+! are packed into these contiguous arrays by the caller.
 module pfx_chem_module
 
   use, intrinsic :: iso_c_binding
@@ -11,7 +10,7 @@ module pfx_chem_module
 
   private
 
-  integer(c_int), parameter, public :: PFX_ABI_EXPECTED = 1
+  integer(c_int), parameter, public :: PFX_ABI_EXPECTED = 2
   integer(c_int), parameter, public :: PFX_E_ARG = -1, PFX_E_DEVICE = -2, &
                                        PFX_E_NOMEM = -3, PFX_E_UNSUP = -4
   integer(c_int), parameter, public :: PFX_ACT_OFF = 0, PFX_ACT_TIMESTEP = 1, &
@@ -24,7 +23,7 @@ module pfx_chem_module
                                        PFX_CONSTRAINT_FREE = 1, &
                                        PFX_CONSTRAINT_LOG = 2, PFX_CONSTRAINT_PH = 3
   integer(c_int), parameter, public :: PFX_GUESS_PREVIOUS = 0, PFX_GUESS_ARRAY = 1, &
-                                       PFX_GUESS_DEFAULT = 2
+                                       PFX_GUESS_DEFAULT = 2, PFX_GUESS_ARRAY_FALLBACK = 3
 
   type, bind(c), public :: pfx_reaction_desc
     integer(c_int32_t) :: naqcomp, neqcplx, nkinmnrl, neqionxrxn
@@ -64,6 +63,7 @@ module pfx_chem_module
     real(c_double) :: default_guess
     real(c_double) :: porosity, saturation, volume, den_kg
     type(c_ptr) :: stream
+    integer(c_int32_t) :: fallback_after, reserved1
   end type pfx_react_opts
 
   type, bind(c), public :: pfx_stats
@@ -71,11 +71,16 @@ module pfx_chem_module
     integer(c_int32_t) :: max_iterations, min_iterations
     real(c_double) :: max_rel_res, kernel_ms
     integer(c_int64_t) :: iteration_histogram(32)
+    integer(c_int64_t) :: fallbacks
   end type pfx_stats
 
   public :: pfx_abi_version, pfx_tables_create, pfx_tables_destroy, &
             pfx_react_batch, pfx_residual_jacobian, pfx_equilibrate_batch, &
             pfx_auxvar_batch
+  ! learned warm start (include/pfx_surrogate.h)
+  public :: pfx_mlp_create, pfx_mlp_destroy, pfx_mlp_params, pfx_mlp_forward, &
+            pfx_mlp_gradients, pfx_mlp_adam, pfx_mlp_train_step, &
+            pfx_surrogate_features, pfx_surrogate_guess, pfx_surrogate_guess_from
 
   interface
     function pfx_abi_version() bind(c, name='pfx_abi_version')
@@ -139,6 +144,88 @@ module pfx_chem_module
       type(pfx_react_opts), intent(in) :: opts
       integer(c_int) :: pfx_auxvar_batch
     end function pfx_auxvar_batch
+
+    function pfx_mlp_create(nlayers, dims, batch_max, seed, mlp) &
+        bind(c, name='pfx_mlp_create')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      integer(c_int32_t), value :: nlayers
+      integer(c_int32_t), intent(in) :: dims(*)
+      integer(c_int32_t), value :: batch_max
+      integer(c_int64_t), value :: seed
+      type(c_ptr), intent(out) :: mlp
+      integer(c_int) :: pfx_mlp_create
+    end function pfx_mlp_create
+
+    function pfx_mlp_destroy(mlp) bind(c, name='pfx_mlp_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mlp
+      integer(c_int) :: pfx_mlp_destroy
+    end function pfx_mlp_destroy
+
+    function pfx_mlp_params(mlp, which, layer, host, to_device) &
+        bind(c, name='pfx_mlp_params')
+      import :: c_int, c_int32_t, c_float, c_ptr
+      type(c_ptr), value :: mlp
+      integer(c_int32_t), value :: which, layer, to_device
+      real(c_float), intent(inout) :: host(*)
+      integer(c_int) :: pfx_mlp_params
+    end function pfx_mlp_params
+
+    function pfx_mlp_forward(mlp, x, b, y, stream) bind(c, name='pfx_mlp_forward')
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: mlp, x, y, stream
+      integer(c_int64_t), value :: b
+      integer(c_int) :: pfx_mlp_forward
+    end function pfx_mlp_forward
+
+    function pfx_mlp_gradients(mlp, x, t, b, loss, stream) &
+        bind(c, name='pfx_mlp_gradients')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: mlp, x, t, loss, stream
+      integer(c_int32_t), value :: b
+      integer(c_int) :: pfx_mlp_gradients
+    end function pfx_mlp_gradients
+
+    function pfx_mlp_adam(mlp, lr, beta1, beta2, eps, stream) bind(c, name='pfx_mlp_adam')
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: mlp, stream
+      real(c_float), value :: lr, beta1, beta2, eps
+      integer(c_int) :: pfx_mlp_adam
+    end function pfx_mlp_adam
+
+    function pfx_mlp_train_step(mlp, x, t, b, lr, loss, stream) &
+        bind(c, name='pfx_mlp_train_step')
+      import :: c_int, c_int32_t, c_float, c_ptr
+      type(c_ptr), value :: mlp, x, t, loss, stream
+      integer(c_int32_t), value :: b
+      real(c_float), value :: lr
+      integer(c_int) :: pfx_mlp_train_step
+    end function pfx_mlp_train_step
+
+    function pfx_surrogate_features(tables, mlp, cells, x, stream) &
+        bind(c, name='pfx_surrogate_features')
+      import :: c_int, c_ptr, pfx_cells
+      type(c_ptr), value :: tables, mlp, x, stream
+      type(pfx_cells), intent(in) :: cells
+      integer(c_int) :: pfx_surrogate_features
+    end function pfx_surrogate_features
+
+    function pfx_surrogate_guess(tables, mlp, cells, stream) &
+        bind(c, name='pfx_surrogate_guess')
+      import :: c_int, c_ptr, pfx_cells
+      type(c_ptr), value :: tables, mlp, stream
+      type(pfx_cells), intent(inout) :: cells
+      integer(c_int) :: pfx_surrogate_guess
+    end function pfx_surrogate_guess
+
+    function pfx_surrogate_guess_from(mlp, y, ncells, naqcomp, guess, stream) &
+        bind(c, name='pfx_surrogate_guess_from')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: mlp, y, guess, stream
+      integer(c_int64_t), value :: ncells
+      integer(c_int32_t), value :: naqcomp
+      integer(c_int) :: pfx_surrogate_guess_from
+    end function pfx_surrogate_guess_from
   end interface
 
 end module pfx_chem_module

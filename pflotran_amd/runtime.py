@@ -20,7 +20,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpfx_chem.so")
 
 EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_react_batch",
-            "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch"]
+            "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch",
+            # learned warm start (include/pfx_surrogate.h)
+            "pfx_mlp_create", "pfx_mlp_destroy", "pfx_mlp_params", "pfx_mlp_forward", "pfx_mlp_gradients",
+            "pfx_mlp_adam", "pfx_mlp_train_step", "pfx_surrogate_features", "pfx_surrogate_guess",
+            "pfx_surrogate_guess_from"]
 
 _lib = None
 
@@ -50,6 +54,17 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_equilibrate_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_void_p, C.c_void_p, C.c_int32,
                                           P(abi.ReactOpts), P(abi.Stats)]
     lib.pfx_auxvar_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_int32, P(abi.ReactOpts)]
+    vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+    lib.pfx_mlp_create.argtypes = [i32, vp, i32, C.c_uint64, P(vp)]
+    lib.pfx_mlp_destroy.argtypes = [vp]
+    lib.pfx_mlp_params.argtypes = [vp, i32, i32, vp, i32]
+    lib.pfx_mlp_forward.argtypes = [vp, vp, i64, vp, vp]
+    lib.pfx_mlp_gradients.argtypes = [vp, vp, vp, i32, vp, vp]
+    lib.pfx_mlp_adam.argtypes = [vp, f32, f32, f32, f32, vp]
+    lib.pfx_mlp_train_step.argtypes = [vp, vp, vp, i32, f32, vp, vp]
+    lib.pfx_surrogate_features.argtypes = [vp, vp, P(abi.Cells), vp, vp]
+    lib.pfx_surrogate_guess.argtypes = [vp, vp, P(abi.Cells), vp]
+    lib.pfx_surrogate_guess_from.argtypes = [vp, vp, i64, i32, vp, vp]
     if lib.pfx_abi_version() != abi.PFX_ABI_VERSION:
         raise PfxError("ABI version mismatch between libpfx_chem.so and pflotran_amd.abi")
     _lib = lib

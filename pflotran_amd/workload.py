@@ -1,0 +1,66 @@
+"""Device-resident batches of synthetic cells at the start of a reaction step.
+
+A cell's previous state is its constraint speciation (pH for H+, total
+concentration for every other primary, pfx_equilibrate_batch) followed by the
+initial aux-var refresh of the reference (unit activity coefficients, then two
+activity passes: condition_control.F90:929-935).  Cells whose speciation fails
+are replaced by further draws, so a batch holds exactly the cells asked for.
+"""
+from __future__ import annotations
+
+from . import abi, runtime, synth
+from .database import ReactionTables
+
+# fields a reaction step overwrites: restore them to repeat the same step
+STEP_INPUTS = ("total", "total_sorb", "pri_molal", "pri_act_coef", "sec_act_coef", "ionx_ref_sorbed")
+
+
+def speciated_cells(t: ReactionTables, tab: runtime.Tables, n: int, seed: int = synth.SEED,
+                    workload_id: int = 2, oversample: float = 1.08):
+    """n speciated cells on the device (runtime.DeviceCells) and the number
+    of failed draws."""
+    import torch
+    n_draw = int(n * oversample) + 64
+    ci = synth.random_cells(t, n_draw, seed=seed, workload_id=workload_id)
+    arr = synth.host_state_arrays(t, n_draw, with_temperature=True)
+    arr["temperature"][:] = ci.temp
+    arr["mnrl_volfrac"][:] = ci.vf
+    arr["mnrl_area"][:] = ci.area
+    conc = ci.totals.copy()
+    conc[:, 0] = ci.ph
+    ctype = [abi.PFX_CONSTRAINT_PH] + [abi.PFX_CONSTRAINT_TOTAL] * (t.N - 1)
+    dc = runtime.DeviceCells.from_host(arr, n_draw)
+    runtime.equilibrate(tab, dc, ctype, conc, synth.react_opts(max_iterations=300))
+    m = dc.t["pri_molal"]
+    ok = (dc.t["status"] == 0) & torch.isfinite(m).all(dim=1) & (m > 0).all(dim=1)
+    idx = torch.nonzero(ok).flatten()[:n]
+    if idx.numel() < n:
+        raise runtime.PfxError(f"only {idx.numel()} of {n} cells speciated")
+    dc = runtime.DeviceCells({k: (None if v is None else v.index_select(0, idx).contiguous())
+                              for k, v in dc.t.items()}, n)
+    dc.t["pri_act_coef"].fill_(1.0)
+    dc.t["sec_act_coef"].fill_(1.0)
+    dc.t["ionx_ref_sorbed"].fill_(1e-9)
+    runtime.auxvar(tab, dc, 2, synth.react_opts())
+    return dc, n_draw - int(ok.sum())
+
+
+def snapshot(dc: runtime.DeviceCells):
+    return {k: dc.t[k].clone() for k in STEP_INPUTS if dc.t.get(k) is not None}
+
+
+def restore(dc: runtime.DeviceCells, snap):
+    for k, v in snap.items():
+        dc.t[k].copy_(v)
+
+
+def host_arrays(dc: runtime.DeviceCells):
+    return {k: (None if v is None else v.cpu().numpy()) for k, v in dc.t.items()}
+
+
+def subset(dc: runtime.DeviceCells, idx) -> runtime.DeviceCells:
+    return runtime.DeviceCells({k: (None if v is None else v.index_select(0, idx).contiguous())
+                                for k, v in dc.t.items()}, int(idx.numel()))
+
+
+__all__ = ["speciated_cells", "snapshot", "restore", "host_arrays", "subset", "STEP_INPUTS"]

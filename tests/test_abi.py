@@ -1,6 +1,7 @@
 """The C-ABI library loads and exports every entry point declared in
 include/pfx_chem.h, and the ctypes mirror matches the C struct layouts."""
 import ctypes as C
+import glob
 import os
 import re
 import subprocess
@@ -11,11 +12,14 @@ from pflotran_amd import abi, runtime
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "pfx_chem.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*int\s+(pfx_\w+)\s*\(", text, flags=re.M)))
+    names = set()
+    for h in HEADERS:
+        names |= set(re.findall(r"^\s*int\s+(pfx_\w+)\s*\(", open(h).read(), flags=re.M))
+    return sorted(names)
 
 
 def test_header_declares_expected_entry_points():
@@ -34,6 +38,16 @@ def test_null_arguments_rejected_without_gpu_work():
     assert lib.pfx_tables_create(None, None) == abi.PFX_E_ARG
     assert lib.pfx_react_batch(None, None, None, None) == abi.PFX_E_ARG
     assert lib.pfx_tables_destroy(None) == abi.PFX_E_ARG
+    assert lib.pfx_mlp_create(0, None, 0, 0, None) == abi.PFX_E_ARG
+    assert lib.pfx_mlp_destroy(None) == abi.PFX_E_ARG
+    assert lib.pfx_mlp_forward(None, None, 0, None, None) == abi.PFX_E_ARG
+    assert lib.pfx_surrogate_guess(None, None, None, None) == abi.PFX_E_ARG
+
+
+def test_every_header_compiles_as_c(tmp_path):
+    src = tmp_path / "inc.c"
+    src.write_text("".join(f'#include "{h}"\n' for h in HEADERS) + "int main(void){return 0;}\n")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-o", str(tmp_path / "inc"), str(src)], check=True)
 
 
 @pytest.mark.parametrize("struct,cname", [(abi.ReactionDesc, "pfx_reaction_desc"), (abi.Cells, "pfx_cells"),

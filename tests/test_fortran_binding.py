@@ -26,6 +26,7 @@ def test_fortran_module_binds_the_c_abi(tmp_path):
     assert int(vals["abi"]) == abi.PFX_ABI_VERSION
     assert int(vals["create_bad"]) == abi.PFX_E_ARG
     assert int(vals["react_null"]) == abi.PFX_E_ARG
+    assert int(vals["mlp_null"]) == abi.PFX_E_ARG
     assert int(vals["sizeof_desc"]) == C.sizeof(abi.ReactionDesc)
     assert int(vals["sizeof_cells"]) == C.sizeof(abi.Cells)
     assert int(vals["sizeof_opts"]) == C.sizeof(abi.ReactOpts)

@@ -235,3 +235,20 @@ def test_reaction_jacobian_by_central_difference():
     o2 = synth.react_opts()
     _, J2 = orc.residual_jacobian(t, cells, o2)
     assert np.allclose(J2, J * (o2.den_kg / 1000.0), rtol=1e-12, atol=0)
+
+
+def test_oracle_fallback_restarts_from_the_default_guess():
+    # every warm attempt from 1e3 molal fails within 3 iterations: the
+    # fallback run must be the plain run plus the abandoned iterations
+    from pflotran_amd import abi, synth
+    t = synth.random_network(15, 50, 3, ionx=True)
+    cells, _, _, _ = orc.speciated_state(t, 40)
+    a = cells.copy()
+    a["guess"][:] = 1e3
+    _, st_a = orc.react_batch(t, a, synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK, fallback_after=3))
+    b = cells.copy()
+    _, st_b = orc.react_batch(t, b, synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT))
+    assert st_a.fallbacks == 40 and st_b.fallbacks == 0
+    assert np.array_equal(a["pri_molal"], b["pri_molal"])
+    assert np.array_equal(a["status"], b["status"])
+    assert np.array_equal(a["num_iterations"], b["num_iterations"] + 3)
