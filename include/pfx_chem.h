@@ -250,6 +250,58 @@ int pfx_equilibrate_batch(pfx_tables *tables, pfx_cells *cells, const int32_t *c
 int pfx_auxvar_batch(pfx_tables *tables, pfx_cells *cells, int32_t activity_passes,
                      const pfx_react_opts *opts);
 
+/*
+ * Structured grid of the global-implicit assembly (a DMDA box, natural
+ * ordering: cell c = i + nx*(j + ny*k)).  Internal faces connect a cell
+ * ("up") with its +x/+y/+z neighbour ("dn"); their Darcy fluxes are
+ * positive from up to dn.  Face arrays are device pointers:
+ *   x faces [nz][ny][nx-1], y faces [nz][ny-1][nx], z faces [nz-1][ny][nx].
+ * dd is the harmonic mean of saturation * porosity * (mechanical dispersion
+ * + tortuosity * molecular diffusion) over the distance (TDispersion,
+ * transport.F90:65-234), m/s, the same for every component.  The x-min and
+ * x-max faces of the box may carry boundary connections (the boundary is
+ * "up", the cell "dn", flux positive into the domain, [nz][ny] arrays);
+ * their totals are bc_total (mol/L, [N], device) or, when NULL, the cell's
+ * own (zero gradient).  Other boundary faces are closed.
+ */
+typedef struct pfx_grid {
+  int32_t nx, ny, nz;
+  int32_t log_formulation;   /* 1: Jacobian columns scaled by m (unknown ln m) */
+  double area[3];            /* face area per direction (m^2)                */
+  const double *q[3];        /* internal-face Darcy flux (m/s)                */
+  const double *dd[3];       /* internal-face dispersion / distance (m/s)     */
+  const double *bc_q[2];     /* x-min, x-max boundary flux (NULL = closed)    */
+  const double *bc_dd[2];    /* x-min, x-max boundary dispersion / distance   */
+  const double *bc_total[2]; /* x-min, x-max boundary totals (NULL = zero gradient) */
+} pfx_grid;
+
+/* Block sparsity of the 7-point Jacobian: row_ptr[ncells+1] and
+   col_idx[nnzb] (DEVICE, int32), columns sorted within a row, the layout of
+   PETSc BAIJ from DMCreateMatrix on a star stencil
+   (discretization.F90:849-857).  With row_ptr == NULL only *nnzb is set. */
+int pfx_gi_structure(const pfx_grid *grid, int32_t *row_ptr, int32_t *col_idx, int64_t *nnzb);
+
+/* Global-implicit residual and Jacobian of reactive transport at the free
+   ions in cells->pri_molal: RTResidual = RTResidualFlux + RTResidualNonFlux
+   (reactive_transport.F90:2057-2782) and RTJacobian = RTJacobianFlux +
+   RTJacobianNonFlux (:2903-3525), upwind fluxes (TFluxCoef,
+   transport.F90:756-819; TFlux :368-439; TFluxDerivative :529-622).
+   cells->total / total_sorb hold the accumulation of the OLD state.
+   residual: [ncells*N] mol/s; bsr_values: [nnzb][N][N] column-major blocks
+   in the row_ptr order of pfx_gi_structure (kg water/s, or per unit ln m
+   with log_formulation).  Activity coefficients are used as given. */
+int pfx_gi_residual_jacobian(pfx_tables *tables, pfx_cells *cells, const pfx_grid *grid,
+                             const pfx_react_opts *opts, const int32_t *row_ptr,
+                             double *residual, double *bsr_values);
+
+/* Mineral volume-fraction update after a converged step: RUpdateKineticState
+   (reaction.F90:5340-5366), called per cell by RTUpdateKineticState
+   (reactive_transport.F90:642).  vf_k += mnrl_rate_k * molar_vol_k * dt,
+   clamped at 0, with the rates that pfx_react_batch returned at the
+   solution (the RKineticMineral evaluation the reference repeats there).
+   Writes cells->mnrl_volfrac (device). */
+int pfx_update_kinetic_state(pfx_tables *tables, pfx_cells *cells, const pfx_react_opts *opts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,9 @@ def load():
         lib.or_density_ifc67.argtypes = [C.c_double, C.c_double]
         lib.or_density_ifc67.restype = C.c_double
         lib.or_debye_params.argtypes = [C.c_double, C.c_int, vp, vp, vp]
+        lib.or_gi_structure.argtypes = [P(abi.Grid), vp, vp, vp]
+        lib.or_gi_residual_jacobian.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.Grid), P(abi.ReactOpts),
+                                                vp, vp, vp, vp, C.c_int]
         _lib = lib
     return _lib
 
@@ -227,3 +230,79 @@ def speciated_state(t, n, seed=None, workload_id=2, temperature=True, nthreads=0
     rc, its = equilibrate(t, ctype, conc, cells, synth.react_opts(), nthreads=nthreads)
     ok = (its > 0) & np.all(np.isfinite(cells["pri_molal"]), axis=1) & np.all(cells["pri_molal"] > 0, axis=1)
     return cells, ctype, conc, ok
+
+
+class HostGrid:
+    """numpy arrays of a pfx_grid (structured box, natural ordering)."""
+
+    def __init__(self, nx, ny, nz, q, dd, area=(1.0, 1.0, 1.0), bc_q=(None, None), bc_dd=(None, None),
+                 bc_total=(None, None), log_formulation=False):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.q = [np.ascontiguousarray(a, dtype=np.float64) for a in q]
+        self.dd = [np.ascontiguousarray(a, dtype=np.float64) for a in dd]
+        self.area = tuple(float(a) for a in area)
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        self.bc_q = [conv(a) for a in bc_q]
+        self.bc_dd = [conv(a) for a in bc_dd]
+        self.bc_total = [conv(a) for a in bc_total]
+        self.log_formulation = bool(log_formulation)
+
+    @property
+    def ncells(self):
+        return self.nx * self.ny * self.nz
+
+    def struct(self) -> abi.Grid:
+        g = abi.Grid()
+        g.nx, g.ny, g.nz = self.nx, self.ny, self.nz
+        g.log_formulation = int(self.log_formulation)
+        for d in range(3):
+            g.area[d] = self.area[d]
+            g.q[d] = _ptr(self.q[d])
+            g.dd[d] = _ptr(self.dd[d])
+        for s_ in range(2):
+            g.bc_q[s_] = _ptr(self.bc_q[s_])
+            g.bc_dd[s_] = _ptr(self.bc_dd[s_])
+            g.bc_total[s_] = _ptr(self.bc_total[s_])
+        return g
+
+
+def gi_structure(grid: HostGrid):
+    lib = load()
+    g = grid.struct()
+    nnzb = C.c_int64(0)
+    lib.or_gi_structure(C.byref(g), None, None, C.byref(nnzb))
+    row_ptr = np.zeros(grid.ncells + 1, dtype=np.int32)
+    col_idx = np.zeros(nnzb.value, dtype=np.int32)
+    lib.or_gi_structure(C.byref(g), row_ptr.ctypes.data, col_idx.ctypes.data, C.byref(nnzb))
+    return row_ptr, col_idx
+
+
+def gi_residual_jacobian(t: ReactionTables, cells: HostCells, grid: HostGrid, opts: abi.ReactOpts, nthreads=0):
+    """Global residual [n, N] and BSR values [nnzb, N, N] (row, col) with
+    (row_ptr, col_idx)."""
+    lib = load()
+    row_ptr, col_idx = gi_structure(grid)
+    n, N = grid.ncells, t.N
+    r = np.zeros((n, N))
+    vals = np.zeros((len(col_idx), N, N))   # [block][col][row] memory = column-major blocks
+    d = t.desc()
+    cs = cells.struct()
+    g = grid.struct()
+    rc = lib.or_gi_residual_jacobian(C.byref(d), C.byref(cs), C.byref(g), C.byref(opts), row_ptr.ctypes.data,
+                                     col_idx.ctypes.data, r.ctypes.data, vals.ctypes.data, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"or_gi_residual_jacobian failed: {rc}")
+    return r, np.ascontiguousarray(vals.transpose(0, 2, 1)), row_ptr, col_idx
+
+
+def bsr_to_dense(vals, row_ptr, col_idx):
+    """Dense [n*N, n*N] matrix of BSR blocks given as [block][row][col]."""
+    nb, N, _ = vals.shape
+    n = len(row_ptr) - 1
+    A = np.zeros((n * N, n * N))
+    for row in range(n):
+        for b in range(row_ptr[row], row_ptr[row + 1]):
+            col = col_idx[b]
+            A[row * N:(row + 1) * N, col * N:(col + 1) * N] = vals[b]
+    return A
+

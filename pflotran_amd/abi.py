@@ -89,6 +89,14 @@ class Stats(C.Structure):
         }
 
 
+class Grid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("log_formulation", C.c_int32),
+        ("area", C.c_double * 3), ("q", C.c_void_p * 3), ("dd", C.c_void_p * 3),
+        ("bc_q", C.c_void_p * 2), ("bc_dd", C.c_void_p * 2), ("bc_total", C.c_void_p * 2),
+    ]
+
+
 CELL_FIELDS = [f for f, _ in Cells._fields_ if f != "ncells"]
 
 

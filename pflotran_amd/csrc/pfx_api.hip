@@ -27,6 +27,8 @@ struct pfx_tables {
   double *d_relres = nullptr;
   int64_t scratch_cells = 0;
   unsigned long long *d_stats = nullptr;  // reduction buffer
+  double *d_gi = nullptr;                 // global-implicit per-cell scratch
+  int64_t gi_cells = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int device = 0;
   int num_cu = 0;
@@ -114,6 +116,17 @@ __global__ void k_stats(const int32_t *it, const int32_t *st, const double *rr, 
     atomicMax(&out[4], s_rr);
   }
   if (threadIdx.x < 32) atomicAdd(&out[5 + threadIdx.x], s_hist[threadIdx.x]);
+}
+
+// RUpdateKineticState (reaction.F90:5340-5366): vf += rate * molar volume
+// * dt, clamped at zero; one thread per (cell, mineral)
+__global__ void k_kinetic_state(const unsigned char *blob, int64_t o_molvol, int K, int64_t n, const double *rate,
+                                double *volfrac, double dt) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n * K) return;
+  const double mv = ((const double *)(blob + o_molvol))[i % K];
+  const double v = volfrac[i] + rate[i] * mv * dt;
+  volfrac[i] = v < 0.0 ? 0.0 : v;
 }
 
 int ensure_scratch(pfx_tables *t, int64_t n) {
@@ -272,6 +285,42 @@ int use_scratch(pfx_tables *t, pfx_kernel_params &P, int64_t n) {
 }
 
 }  // namespace
+
+// per-cell scratch of the global-implicit assembly: non-flux residual [N],
+// non-flux Jacobian [N*N], totals [N], dT/dm [N*N]
+int pfx_internal::gi_scratch(pfx_tables *t, int64_t ncells, double **rnf, double **jnf, double **tot,
+                             double **dtot) {
+  const int64_t N = t->L.N, per = 2 * N + 2 * N * N;
+  if (ncells > t->gi_cells) {
+    HIPCHK(hipSetDevice(t->device));
+    if (t->d_gi) HIPCHK(hipFree(t->d_gi));
+    t->d_gi = nullptr;
+    t->gi_cells = 0;
+    HIPCHK(hipMalloc(&t->d_gi, (size_t)(ncells * per) * sizeof(double)));
+    t->gi_cells = ncells;
+  }
+  *rnf = t->d_gi;
+  *jnf = *rnf + ncells * N;
+  *tot = *jnf + ncells * N * N;
+  *dtot = *tot + ncells * N;
+  return 0;
+}
+
+int pfx_internal::gi_evaluate_cells(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, int col_scale,
+                                    double *rnf, double *jnf, double *tot, double *dtot) {
+  int rc = check_cells(t, cells, o);
+  if (rc) return rc;
+  if (cells->ncells == 0) return 0;
+  HIPCHK(hipSetDevice(t->device));
+  pfx_kernel_params P;
+  fill_params(t, cells, o, P);
+  P.res_out = rnf;
+  P.jac_out = jnf;
+  P.tot_eval_out = tot;
+  P.dtot_out = dtot;
+  P.col_scale = col_scale ? 1 : 0;
+  return launch(t, P, PFX_MODE_EVAL, (hipStream_t)o->stream, nullptr);
+}
 
 void pfx_internal::table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R) {
   *N = t->L.N;
@@ -508,6 +557,7 @@ extern "C" int pfx_tables_destroy(pfx_tables *t) {
   (void)hipFree(t->d_blob);
   (void)hipFree(t->d_queue);
   (void)hipFree(t->d_stats);
+  if (t->d_gi) (void)hipFree(t->d_gi);
   if (t->d_iters) (void)hipFree(t->d_iters);
   if (t->d_status) (void)hipFree(t->d_status);
   if (t->d_relres) (void)hipFree(t->d_relres);
@@ -603,4 +653,19 @@ extern "C" int pfx_auxvar_batch(pfx_tables *t, pfx_cells *cells, int32_t activit
   P.rel_res = nullptr;
   double ms = 0.0;
   return launch(t, P, PFX_MODE_AUX, (hipStream_t)o->stream, &ms);
+}
+
+extern "C" int pfx_update_kinetic_state(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o) {
+  if (!t || !cells || !o || cells->ncells < 0) return PFX_E_ARG;
+  if (!(o->dt > 0.0)) return PFX_E_ARG;
+  if (cells->ncells == 0 || t->L.K == 0) return 0;
+  if (!cells->mnrl_rate || !cells->mnrl_volfrac) return PFX_E_ARG;
+  HIPCHK(hipSetDevice(t->device));
+  hipStream_t stream = (hipStream_t)o->stream;
+  const int64_t n = cells->ncells * (int64_t)t->L.K;
+  hipLaunchKernelGGL(k_kinetic_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, t->d_blob,
+                     t->L.o_mmolvol, t->L.K, cells->ncells, cells->mnrl_rate, (double *)cells->mnrl_volfrac, o->dt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+  return 0;
 }

@@ -74,6 +74,9 @@ struct pfx_kernel_params {
   int32_t fallback_after;       // warm-attempt iteration cap (PFX_GUESS_ARRAY_FALLBACK)
   unsigned int *fallback_count; // cells restarted from the default guess
   double *res_out, *jac_out;    // residual/Jacobian outputs
+  double *dtot_out;             // EVAL: dT/dm blocks [ncells][N][N] column-major (kg/L), may be NULL
+  double *tot_eval_out;         // EVAL: totals at the free ions [ncells][N] (mol/L), may be NULL
+  int32_t col_scale;            // EVAL: scale Jacobian and dT/dm columns by m (log unknowns)
   // constraint speciation
   const double *constraint_conc;   // [ncells*N]
   int32_t ctype[PFX_MAX_PRIMARY];  // PFX_CONSTRAINT_* per component
@@ -88,4 +91,7 @@ struct pfx_kernel_params {
 // for the other translation units of the library.
 namespace pfx_internal {
 void table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R);
+int gi_scratch(pfx_tables *t, int64_t ncells, double **rnf, double **jnf, double **tot, double **dtot);
+int gi_evaluate_cells(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, int col_scale, double *rnf,
+                      double *jnf, double *tot, double *dtot);
 }

@@ -1,0 +1,230 @@
+// Global-implicit assembly on a structured grid (include/pfx_chem.h,
+// pfx_gi_structure / pfx_gi_residual_jacobian).
+//
+// Two launches per evaluation:
+//  1. the persistent per-cell kernel in EVAL mode writes, per cell, the
+//     non-flux residual (accumulation/dt - old accumulation/dt + reaction),
+//     the non-flux Jacobian block, the aqueous totals and dT/dm (column-
+//     major blocks, columns already scaled by m for log unknowns);
+//  2. k_gi_assemble, one wavefront per block row, adds the upwind flux
+//     terms of the row's faces (TFluxCoef, TFlux, TFluxDerivative) and
+//     writes the row's residual and its 7 (or fewer) BSR blocks.  With a
+//     species-independent dispersion every flux block is a scalar times the
+//     neighbour's dT/dm block, so the row is a handful of scaled block copies:
+//     the kernel is HBM-bound (≈7 * N^2 * 8 B written per cell).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pfx_device.h"
+
+namespace {
+
+#define GCHK(x)                                                               \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess) {                                                   \
+      fprintf(stderr, "pfx_gi: %s failed: %s (%s:%d)\n", #x,                  \
+              hipGetErrorString(e_), __FILE__, __LINE__);                     \
+      return PFX_E_DEVICE;                                                    \
+    }                                                                         \
+  } while (0)
+
+struct gi_params {
+  int32_t nx, ny, nz, N;
+  double area[3];
+  const double *q[3], *dd[3];
+  const double *bc_q[2], *bc_dd[2], *bc_total[2];
+  const int32_t *row_ptr;
+  const double *rnf, *jnf, *tot, *dtot;
+  double *res, *vals;
+};
+
+// TFluxCoef (transport.F90:756-819), upwinding, one liquid phase: L/s
+__device__ __forceinline__ void flux_coef(double area, double q, double dd, double &Tup, double &Tdn) {
+  double cup, cdn;
+  if (q > 0.0) {
+    cup = dd + q;
+    cdn = -dd;
+  } else {
+    cup = dd;
+    cdn = -dd + q;
+  }
+  Tup = cup * area * 1000.0;
+  Tdn = cdn * area * 1000.0;
+}
+
+__global__ __launch_bounds__(256) void k_gi_assemble(const gi_params P) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nx = P.nx, ny = P.ny, nz = P.nz, n = nx * ny * nz;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= n) return;  // whole wavefront
+  const int N = P.N, NN = N * N;
+  const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+  const bool comp = lane < N;
+  const double tc = comp ? P.tot[c * N + lane] : 0.0;
+  double res = comp ? P.rnf[c * N + lane] : 0.0;
+  double s_diag = 0.0;  // scalar of dT_c in the diagonal block
+
+  // neighbours in column order: z-, y-, x-, (self), x+, y+, z+
+  int64_t nb[6];
+  double tnb[6];
+  bool has[6];
+  const int64_t stride[3] = {1, nx, nx * ny};
+  const int64_t idx[3] = {i, j, k};
+  const int64_t ext[3] = {nx, ny, nz};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int64_t ex = nx - (d == 0), ey = ny - (d == 1);
+    // lower face: (c - stride) is up, c is dn
+    const int lo = 2 - d;  // slot: z- = 0, y- = 1, x- = 2
+    has[lo] = idx[d] > 0;
+    if (has[lo]) {
+      const int64_t up = c - stride[d];
+      const int64_t ui = up % nx, uj = (up / nx) % ny, uk = up / (nx * ny);
+      const int64_t f = ui + ex * (uj + ey * uk);
+      double Tup, Tdn;
+      flux_coef(P.area[d], P.q[d][f], P.dd[d][f], Tup, Tdn);
+      const double tn = comp ? P.tot[up * N + lane] : 0.0;
+      res -= Tup * tn + Tdn * tc;
+      s_diag -= Tdn;
+      nb[lo] = up;
+      tnb[lo] = -Tup;
+    }
+    // upper face: c is up, (c + stride) is dn
+    const int hi = 3 + d;  // x+ = 3, y+ = 4, z+ = 5
+    has[hi] = idx[d] < ext[d] - 1;
+    if (has[hi]) {
+      const int64_t dn = c + stride[d];
+      const int64_t f = i + ex * (j + ey * k);
+      double Tup, Tdn;
+      flux_coef(P.area[d], P.q[d][f], P.dd[d][f], Tup, Tdn);
+      const double tn = comp ? P.tot[dn * N + lane] : 0.0;
+      res += Tup * tc + Tdn * tn;
+      s_diag += Tup;
+      nb[hi] = dn;
+      tnb[hi] = Tdn;
+    }
+  }
+  // boundary connections on the x faces (boundary = up, cell = dn)
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (P.bc_q[side] && i == (side == 0 ? 0 : nx - 1)) {
+      const int64_t f = j + ny * k;
+      double Tup, Tdn;
+      flux_coef(P.area[0], P.bc_q[side][f], P.bc_dd[side][f], Tup, Tdn);
+      const double tb = P.bc_total[side] ? (comp ? P.bc_total[side][lane] : 0.0) : tc;
+      res -= Tup * tb + Tdn * tc;
+      s_diag -= Tdn;
+    }
+  }
+  if (comp) P.res[c * N + lane] = res;
+
+  // blocks of the row, in column order
+  int64_t b = P.row_ptr[c];
+  const int order[7] = {0, 1, 2, -1, 3, 4, 5};
+#pragma unroll
+  for (int s = 0; s < 7; s++) {
+    const int o = order[s];
+    if (o < 0) {
+      const double *Jn = P.jnf + c * NN;
+      const double *Dc = P.dtot + c * NN;
+      double *out = P.vals + b * NN;
+      for (int e = lane; e < NN; e += 64) out[e] = Jn[e] + s_diag * Dc[e];
+      b++;
+    } else if (has[o]) {
+      const double *Dn = P.dtot + nb[o] * NN;
+      double *out = P.vals + b * NN;
+      const double t = tnb[o];
+      for (int e = lane; e < NN; e += 64) out[e] = t * Dn[e];
+      b++;
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+static bool grid_ok(const pfx_grid *g) {
+  if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1) return false;
+  if ((int64_t)g->nx * g->ny * g->nz > (int64_t)0x7fffffff / 8) return false;
+  for (int d = 0; d < 3; d++) {
+    const int64_t ext = d == 0 ? g->nx : (d == 1 ? g->ny : g->nz);
+    if (ext > 1 && (!g->q[d] || !g->dd[d])) return false;
+  }
+  for (int s = 0; s < 2; s++)
+    if (g->bc_q[s] && !g->bc_dd[s]) return false;
+  return true;
+}
+
+extern "C" int pfx_gi_structure(const pfx_grid *g, int32_t *row_ptr, int32_t *col_idx, int64_t *nnzb) {
+  if (!g || !nnzb || g->nx < 1 || g->ny < 1 || g->nz < 1) return PFX_E_ARG;
+  const int64_t nx = g->nx, ny = g->ny, nz = g->nz, n = nx * ny * nz;
+  // 7 blocks per interior row, one fewer per box face the cell touches
+  const int64_t total = 7 * n - 2 * (ny * nz + nx * nz + nx * ny);
+  *nnzb = total;
+  if (!row_ptr) return 0;
+  if (!col_idx || total > (int64_t)0x7fffffff) return PFX_E_ARG;
+  std::vector<int32_t> rp(n + 1), ci(total);
+  int64_t p = 0;
+  for (int64_t c = 0; c < n; c++) {
+    const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    rp[c] = (int32_t)p;
+    if (k > 0) ci[p++] = (int32_t)(c - nx * ny);
+    if (j > 0) ci[p++] = (int32_t)(c - nx);
+    if (i > 0) ci[p++] = (int32_t)(c - 1);
+    ci[p++] = (int32_t)c;
+    if (i < nx - 1) ci[p++] = (int32_t)(c + 1);
+    if (j < ny - 1) ci[p++] = (int32_t)(c + nx);
+    if (k < nz - 1) ci[p++] = (int32_t)(c + nx * ny);
+  }
+  rp[n] = (int32_t)p;
+  if (p != total) return PFX_E_DEVICE;
+  GCHK(hipMemcpy(row_ptr, rp.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  GCHK(hipMemcpy(col_idx, ci.data(), total * sizeof(int32_t), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int pfx_gi_residual_jacobian(pfx_tables *t, pfx_cells *cells, const pfx_grid *g, const pfx_react_opts *o,
+                                        const int32_t *row_ptr, double *residual, double *bsr_values) {
+  if (!t || !cells || !o || !row_ptr || !residual || !bsr_values || !grid_ok(g)) return PFX_E_ARG;
+  const int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  if (cells->ncells != n) return PFX_E_ARG;
+  int N, M, K, R;
+  pfx_internal::table_sizes(t, &N, &M, &K, &R);
+  double *rnf, *jnf, *tot, *dtot;
+  int rc = pfx_internal::gi_scratch(t, n, &rnf, &jnf, &tot, &dtot);
+  if (rc) return rc;
+  rc = pfx_internal::gi_evaluate_cells(t, cells, o, g->log_formulation, rnf, jnf, tot, dtot);
+  if (rc) return rc;
+  gi_params P;
+  memset(&P, 0, sizeof(P));
+  P.nx = g->nx;
+  P.ny = g->ny;
+  P.nz = g->nz;
+  P.N = N;
+  for (int d = 0; d < 3; d++) {
+    P.area[d] = g->area[d];
+    P.q[d] = g->q[d];
+    P.dd[d] = g->dd[d];
+  }
+  for (int s = 0; s < 2; s++) {
+    P.bc_q[s] = g->bc_q[s];
+    P.bc_dd[s] = g->bc_dd[s];
+    P.bc_total[s] = g->bc_total[s];
+  }
+  P.row_ptr = row_ptr;
+  P.rnf = rnf;
+  P.jnf = jnf;
+  P.tot = tot;
+  P.dtot = dtot;
+  P.res = residual;
+  P.vals = bsr_values;
+  hipStream_t s = (hipStream_t)o->stream;
+  hipLaunchKernelGGL(k_gi_assemble, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, P);
+  GCHK(hipGetLastError());
+  GCHK(hipStreamSynchronize(s));
+  return 0;
+}

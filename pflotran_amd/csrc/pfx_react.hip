@@ -472,6 +472,16 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
       for (int k = 0; k < G; k++) row[k] = live ? row[k] * den_kg_per_L : row[k];
       tot_out = (m + Tsum) * den_kg_per_L;
     }
+    if (MODE == PFX_MODE_EVAL && active && live) {
+      // aqueous totals and dT/dm of the cell for the flux terms of the
+      // global-implicit assembly (RTAuxVarCompute's total and dtotal)
+      if (P.tot_eval_out) P.tot_eval_out[cell * N + r] = tot_out;
+      if (P.dtot_out) {
+#pragma unroll
+        for (int k = 0; k < G; k++)
+          if (k < N) P.dtot_out[cell * N * N + (long long)k * N + r] = P.col_scale ? row[k] * SD(S.mv, k) : row[k];
+      }
+    }
     if (MODE == PFX_MODE_REACT && active && phase == 0) {
       phase = 1;  // complexes of the previous state are in LDS: update activity next pass
       wave_sync();
@@ -802,7 +812,7 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
           P.res_out[c * N + r] = Rr;
 #pragma unroll
           for (int k = 0; k < G; k++)
-            if (k < N) P.jac_out[c * N * N + (long long)k * N + r] = row[k];
+            if (k < N) P.jac_out[c * N * N + (long long)k * N + r] = P.col_scale ? row[k] * SD(S.mv, k) : row[k];
         }
       } else {
         if (MODE == PFX_MODE_EQUIL && R > 0) {

@@ -76,7 +76,7 @@ module pfx_chem_module
 
   public :: pfx_abi_version, pfx_tables_create, pfx_tables_destroy, &
             pfx_react_batch, pfx_residual_jacobian, pfx_equilibrate_batch, &
-            pfx_auxvar_batch
+            pfx_auxvar_batch, pfx_update_kinetic_state
   ! learned warm start (include/pfx_surrogate.h)
   public :: pfx_mlp_create, pfx_mlp_destroy, pfx_mlp_params, pfx_mlp_forward, &
             pfx_mlp_gradients, pfx_mlp_adam, pfx_mlp_train_step, &
@@ -144,6 +144,15 @@ module pfx_chem_module
       type(pfx_react_opts), intent(in) :: opts
       integer(c_int) :: pfx_auxvar_batch
     end function pfx_auxvar_batch
+
+    function pfx_update_kinetic_state(tables, cells, opts) &
+        bind(c, name='pfx_update_kinetic_state')
+      import :: c_int, c_ptr, pfx_cells, pfx_react_opts
+      type(c_ptr), value :: tables
+      type(pfx_cells), intent(inout) :: cells
+      type(pfx_react_opts), intent(in) :: opts
+      integer(c_int) :: pfx_update_kinetic_state
+    end function pfx_update_kinetic_state
 
     function pfx_mlp_create(nlayers, dims, batch_max, seed, mlp) &
         bind(c, name='pfx_mlp_create')

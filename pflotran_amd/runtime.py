@@ -20,7 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpfx_chem.so")
 
 EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_react_batch",
-            "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch",
+            "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch", "pfx_update_kinetic_state",
+            "pfx_gi_structure", "pfx_gi_residual_jacobian",
             # learned warm start (include/pfx_surrogate.h)
             "pfx_mlp_create", "pfx_mlp_destroy", "pfx_mlp_params", "pfx_mlp_forward", "pfx_mlp_gradients",
             "pfx_mlp_adam", "pfx_mlp_train_step", "pfx_surrogate_features", "pfx_surrogate_guess",
@@ -54,6 +55,10 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_equilibrate_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_void_p, C.c_void_p, C.c_int32,
                                           P(abi.ReactOpts), P(abi.Stats)]
     lib.pfx_auxvar_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_int32, P(abi.ReactOpts)]
+    lib.pfx_update_kinetic_state.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts)]
+    lib.pfx_gi_structure.argtypes = [P(abi.Grid), C.c_void_p, C.c_void_p, P(C.c_int64)]
+    lib.pfx_gi_residual_jacobian.argtypes = [C.c_void_p, P(abi.Cells), P(abi.Grid), P(abi.ReactOpts), C.c_void_p,
+                                             C.c_void_p, C.c_void_p]
     vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
     lib.pfx_mlp_create.argtypes = [i32, vp, i32, C.c_uint64, P(vp)]
     lib.pfx_mlp_destroy.argtypes = [vp]
@@ -198,3 +203,95 @@ def auxvar(tables: Tables, cells: DeviceCells, activity_passes: int, opts: abi.R
     if rc != 0:
         raise PfxError(f"pfx_auxvar_batch failed: {rc}")
     torch.cuda.synchronize()
+
+
+def update_kinetic_state(tables: Tables, cells: DeviceCells, opts: abi.ReactOpts):
+    """pfx_update_kinetic_state: mineral volume fractions after a converged step."""
+    lib = load_library()
+    torch = _torch()
+    cs = cells.struct()
+    if opts.stream is None:
+        opts.stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.pfx_update_kinetic_state(tables.handle, C.byref(cs), C.byref(opts))
+    if rc != 0:
+        raise PfxError(f"pfx_update_kinetic_state failed: {rc}")
+
+
+class DeviceGrid:
+    """Device copy of a structured grid (pfx_grid): face fluxes, dispersion
+    coefficients and boundary data as torch CUDA tensors."""
+
+    def __init__(self, nx, ny, nz, q, dd, area=(1.0, 1.0, 1.0), bc_q=(None, None), bc_dd=(None, None),
+                 bc_total=(None, None), log_formulation=False):
+        torch = _torch()
+        dev = lambda a: None if a is None else torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)  # noqa
+                                                               if not isinstance(a, torch.Tensor) else a,
+                                                               dtype=torch.float64, device="cuda").contiguous()
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.q = [dev(a) for a in q]
+        self.dd = [dev(a) for a in dd]
+        self.area = tuple(float(a) for a in area)
+        self.bc_q = [dev(a) for a in bc_q]
+        self.bc_dd = [dev(a) for a in bc_dd]
+        self.bc_total = [dev(a) for a in bc_total]
+        self.log_formulation = bool(log_formulation)
+
+    @property
+    def ncells(self):
+        return self.nx * self.ny * self.nz
+
+    def struct(self) -> abi.Grid:
+        g = abi.Grid()
+        g.nx, g.ny, g.nz = self.nx, self.ny, self.nz
+        g.log_formulation = int(self.log_formulation)
+        ptr = lambda a: None if (a is None or a.numel() == 0) else a.data_ptr()  # noqa: E731
+        for d in range(3):
+            g.area[d] = self.area[d]
+            g.q[d] = ptr(self.q[d])
+            g.dd[d] = ptr(self.dd[d])
+        for s_ in range(2):
+            g.bc_q[s_] = ptr(self.bc_q[s_])
+            g.bc_dd[s_] = ptr(self.bc_dd[s_])
+            g.bc_total[s_] = ptr(self.bc_total[s_])
+        return g
+
+
+def gi_structure(grid: DeviceGrid):
+    """pfx_gi_structure: (row_ptr, col_idx) int32 device tensors."""
+    lib = load_library()
+    torch = _torch()
+    g = grid.struct()
+    nnzb = C.c_int64(0)
+    rc = lib.pfx_gi_structure(C.byref(g), None, None, C.byref(nnzb))
+    if rc != 0:
+        raise PfxError(f"pfx_gi_structure failed: {rc}")
+    row_ptr = torch.empty(grid.ncells + 1, dtype=torch.int32, device="cuda")
+    col_idx = torch.empty(nnzb.value, dtype=torch.int32, device="cuda")
+    rc = lib.pfx_gi_structure(C.byref(g), row_ptr.data_ptr(), col_idx.data_ptr(), C.byref(nnzb))
+    if rc != 0:
+        raise PfxError(f"pfx_gi_structure failed: {rc}")
+    return row_ptr, col_idx
+
+
+def gi_residual_jacobian(tables: Tables, cells: DeviceCells, grid: DeviceGrid, opts: abi.ReactOpts, row_ptr,
+                         residual=None, values=None):
+    """pfx_gi_residual_jacobian: residual [n, N] and BSR values [nnzb, N, N]
+    in memory order [block][col][row] (column-major blocks), device tensors."""
+    lib = load_library()
+    torch = _torch()
+    N = tables.host.N
+    nnzb = int(row_ptr[-1].item())
+    if residual is None:
+        residual = torch.empty((grid.ncells, N), dtype=torch.float64, device="cuda")
+    if values is None:
+        values = torch.empty((nnzb, N, N), dtype=torch.float64, device="cuda")
+    cs = cells.struct()
+    g = grid.struct()
+    if opts.stream is None:
+        opts.stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.pfx_gi_residual_jacobian(tables.handle, C.byref(cs), C.byref(g), C.byref(opts), row_ptr.data_ptr(),
+                                      residual.data_ptr(), values.data_ptr())
+    if rc != 0:
+        raise PfxError(f"pfx_gi_residual_jacobian failed: {rc}")
+    return residual, values
+

@@ -1,0 +1,69 @@
+"""Global-implicit assembly on the GPU (pfx_gi_structure,
+pfx_gi_residual_jacobian) against the oracle's restatement of RTResidual /
+RTJacobian on the same seeded inputs.  Tolerances: residual entries 1e-12 of
+the accumulation magnitude of their cell; Jacobian entries 2e-12 of the
+largest entry of their global row (the exchanger rows carry the inner
+Newton's 1e-12 tolerance, as in the per-cell parity test)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from gpu_cases import host_arrays, speciated_state
+from pflotran_amd import runtime, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def net15():
+    return synth.random_network(15, 50, 3, ionx=True)
+
+
+def faces(rng, nx, ny, nz):
+    shapes = [(nz, ny, nx - 1), (nz, ny - 1, nx), (nz - 1, ny, nx)]
+    q = [rng.normal(0, 1e-6, size=s) for s in shapes]
+    dd = [rng.uniform(0.2, 1.0, size=s) * 1e-7 for s in shapes]
+    return q, dd
+
+
+@pytest.mark.parametrize("shape,log_form,dirichlet", [((3, 2, 2), False, True), ((8, 6, 5), True, False),
+                                                      ((1, 1, 7), False, False), ((5, 4, 1), True, True)])
+def test_gi_assembly_parity(net15, shape, log_form, dirichlet):
+    t = net15
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    rng = np.random.default_rng(n)
+    q, dd = faces(rng, nx, ny, nz)
+    bc_q = (rng.normal(0, 1e-6, size=(nz, ny)), rng.normal(0, 1e-6, size=(nz, ny)))
+    bc_dd = (np.full((nz, ny), 2e-7), np.full((nz, ny), 3e-7))
+    bc_total = (np.full(t.N, 1e-3), np.full(t.N, 2e-3) if dirichlet else None)
+    hg = orc.HostGrid(nx, ny, nz, q, dd, area=(0.5, 0.7, 0.9), bc_q=bc_q, bc_dd=bc_dd, bc_total=bc_total,
+                      log_formulation=log_form)
+    cells, _, _, _ = speciated_state(t, n, seed=17)
+    cells["pri_molal"][:] *= np.exp(0.2 * np.sin(np.arange(n * t.N)).reshape(n, t.N))
+    opts = synth.react_opts()
+    r_o, v_o, rp_o, ci_o = orc.gi_residual_jacobian(t, cells, hg, opts)
+
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    dg = runtime.DeviceGrid(nx, ny, nz, q, dd, area=(0.5, 0.7, 0.9), bc_q=bc_q, bc_dd=bc_dd, bc_total=bc_total,
+                            log_formulation=log_form)
+    rp, ci = runtime.gi_structure(dg)
+    assert np.array_equal(rp.cpu().numpy(), rp_o) and np.array_equal(ci.cpu().numpy(), ci_o)
+    r_g, v_g = runtime.gi_residual_jacobian(tab, dc, dg, synth.react_opts(), rp)
+    r_g = r_g.cpu().numpy()
+    v_g = v_g.transpose(1, 2).contiguous().cpu().numpy()   # -> [block][row][col]
+
+    acc = 0.25 * 1000.0 * np.abs(cells["total"]) / synth.STEP_DT
+    assert np.max(np.abs(r_g - r_o) / (acc + np.abs(r_o) + 1e-300)) <= 1e-12
+    # Jacobian: per global row, relative to the row's largest entry
+    N = t.N
+    rowmax = np.zeros((n, N))
+    for row in range(n):
+        for b in range(rp_o[row], rp_o[row + 1]):
+            rowmax[row] = np.maximum(rowmax[row], np.max(np.abs(v_o[b]), axis=1))
+    err = 0.0
+    for row in range(n):
+        for b in range(rp_o[row], rp_o[row + 1]):
+            err = max(err, np.max(np.abs(v_g[b] - v_o[b]) / rowmax[row][:, None]))
+    assert err <= 2e-12

@@ -167,15 +167,13 @@ def test_gpu_gold_calcite_kinetics(golds):
     tab, dc = _gpu_speciate(t, setup)
     o = synth.react_opts(porosity=0.5, guess_mode=abi.PFX_GUESS_PREVIOUS, rel_tol=1e-14, extra_iterations=1,
                          max_iterations=60, dt=0.5)
-    molar_vol = t.mnrl_molar_vol[0]
     for _ in range(500):
         runtime.auxvar(tab, dc, 0, synth.react_opts(porosity=0.5))  # accumulation of the old state
         rc, st = runtime.react_batch(tab, dc, synth.react_opts(porosity=0.5, guess_mode=abi.PFX_GUESS_PREVIOUS,
                                                                rel_tol=1e-14, extra_iterations=1,
                                                                max_iterations=60, dt=0.5))
         assert rc == 0
-        # RUpdateKineticState: vf += rate * molar volume * dt (rate at the solution)
-        dc.t["mnrl_volfrac"].add_(dc.t["mnrl_rate"] * (molar_vol * 0.5)).clamp_(min=0.0)
+        runtime.update_kinetic_state(tab, dc, synth.react_opts(porosity=0.5, dt=0.5))
     runtime.auxvar(tab, dc, 0, synth.react_opts(porosity=0.5))
     out = dc.to_host()
     m, g = out["pri_molal"][0], out["pri_act_coef"][0]
@@ -184,3 +182,25 @@ def test_gpu_gold_calcite_kinetics(golds):
         assert abs(out["total"][0][j] - gold(golds, name, f"CONCENTRATION: Total {sp}")) < 1e-12
     assert abs(out["mnrl_volfrac"][0, 0] - gold(golds, name, "VOLUME_FRACTION: Calcite VF")) < 1e-12
     assert abs(out["mnrl_rate"][0, 0] - gold(golds, name, "RATE: Calcite Rate")) < 1e-12
+
+
+def test_update_kinetic_state_parity(net15):
+    # RUpdateKineticState after a converged step: the GPU update from the
+    # step's returned rates against the oracle, which recomputes the rates
+    # at the returned state
+    t = net15
+    n = 200
+    cells, _, _, _ = speciated_state(t, n, seed=5)
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1))
+    state = orc.HostCells({k: (None if v is None else v.copy()) for k, v in dc.to_host().items()}, n)
+    runtime.update_kinetic_state(tab, dc, synth.react_opts())
+    orc.update_kinetic_state(t, state, synth.react_opts())
+    vf0 = cells["mnrl_volfrac"]
+    d_g = dc.t["mnrl_volfrac"].cpu().numpy() - vf0
+    d_o = state["mnrl_volfrac"] - vf0
+    assert np.count_nonzero(d_o) > 0
+    # the change itself (rate * molar volume * dt) to 1e-9 of its size, plus
+    # the rounding of vf0 + change
+    assert np.all(np.abs(d_g - d_o) <= 1e-9 * np.abs(d_o) + 4e-16 * vf0)

@@ -1,0 +1,124 @@
+"""Pins of the oracle's global-implicit assembly (or_gi_residual_jacobian):
+finite-difference columns of the global residual (linear and log unknowns),
+flux conservation over a closed box, and the upwind closed form of a
+pure-advection column."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from pflotran_amd import synth
+
+
+@pytest.fixture(scope="module")
+def net8():
+    # no minerals: their capped lnQK (reaction_mineral.F90:699-703) makes the
+    # analytic Jacobian differ from the derivative far from saturation; the
+    # mineral terms are pinned per cell in test_oracle_math
+    return synth.random_network(8, 20, 0, ionx=False)
+
+
+def faces(rng, nx, ny, nz, qscale=1e-6, ddscale=1e-7):
+    shapes = [(nz, ny, nx - 1), (nz, ny - 1, nx), (nz - 1, ny, nx)]
+    q = [rng.normal(0, qscale, size=s) for s in shapes]
+    dd = [rng.uniform(0.2, 1.0, size=s) * ddscale for s in shapes]
+    return q, dd
+
+
+def state(t, nx, ny, nz, seed=3):
+    n = nx * ny * nz
+    cells, _, _, _ = orc.speciated_state(t, n, seed=seed)
+    # an iterate away from the old state so every term is active
+    cells["pri_molal"][:] *= np.exp(0.2 * np.cos(np.arange(n * t.N)).reshape(n, t.N))
+    return cells
+
+
+def residual(t, cells, grid, opts):
+    return orc.gi_residual_jacobian(t, cells, grid, opts)[0].ravel()
+
+
+@pytest.mark.parametrize("log_form", [False, True])
+def test_gi_jacobian_matches_central_differences(net8, log_form):
+    t = net8
+    nx, ny, nz = 3, 2, 2
+    rng = np.random.default_rng(21)
+    q, dd = faces(rng, nx, ny, nz)
+    bc = (rng.normal(0, 1e-6, size=(nz, ny)), rng.normal(0, 1e-6, size=(nz, ny)))
+    # Dirichlet on both boundary faces: with a zero-gradient face the boundary
+    # totals follow the cell, which the reference Jacobian ignores (its
+    # boundary loop adds only Jdn, reactive_transport.F90:3218)
+    grid = orc.HostGrid(nx, ny, nz, q, dd, area=(0.5, 0.7, 0.9), bc_q=bc,
+                        bc_dd=(np.full((nz, ny), 2e-7), np.full((nz, ny), 3e-7)),
+                        bc_total=(np.full(t.N, 1e-3), np.full(t.N, 2e-3)), log_formulation=log_form)
+    cells = state(t, nx, ny, nz)
+    # at 1000 kg/m3 the reference's mineral Jacobian factor (den * 1e-3 where
+    # the exact derivative has none) is exact, so the analytic Jacobian is
+    # the true derivative
+    opts = synth.react_opts(den_kg=1000.0)
+    r, vals, rp, ci = orc.gi_residual_jacobian(t, cells, grid, opts)
+    J = orc.bsr_to_dense(vals, rp, ci)
+    n, N = grid.ncells, t.N
+    m0 = cells["pri_molal"].copy()
+    h = 1e-6
+    # size of the terms summed into each residual entry (accumulation and
+    # fluxes): the rounding noise of a central difference is ~eps * terms / step
+    tmax = max(np.max(np.abs(a)) for a in q + dd) * 1000.0 + 0.25 * 1000.0 / opts.dt
+    terms = (tmax * 8 * np.maximum(np.abs(cells["total"]), 1e-3)).ravel()
+    for col in range(n * N):
+        c, k = divmod(col, N)
+        if log_form:
+            mp, mm = m0.copy(), m0.copy()
+            mp[c, k] *= np.exp(h)
+            mm[c, k] *= np.exp(-h)
+            step = 2 * h
+        else:
+            mp, mm = m0.copy(), m0.copy()
+            mp[c, k] *= 1 + h
+            mm[c, k] *= 1 - h
+            step = 2 * h * m0[c, k]
+        cells.arrays["pri_molal"] = mp
+        rp_ = residual(t, cells, grid, opts)
+        cells.arrays["pri_molal"] = mm
+        rm_ = residual(t, cells, grid, opts)
+        cells.arrays["pri_molal"] = m0
+        fd = (rp_ - rm_) / step
+        scale = np.max(np.abs(J), axis=1) + 1e-300
+        noise = 16 * np.finfo(float).eps * terms / step
+        assert np.all(np.abs(fd - J[:, col]) <= 1e-6 * scale + noise), col
+
+
+def test_gi_flux_conserves_mass_in_a_closed_box(net8):
+    t = net8
+    nx, ny, nz = 4, 3, 2
+    rng = np.random.default_rng(5)
+    q, dd = faces(rng, nx, ny, nz)
+    grid = orc.HostGrid(nx, ny, nz, q, dd)
+    cells = state(t, nx, ny, nz, seed=9)
+    opts = synth.react_opts()
+    r, _, _, _ = orc.gi_residual_jacobian(t, cells, grid, opts)
+    rnf, _ = orc.residual_jacobian(t, cells, opts)
+    flux = r - rnf
+    # every internal flux leaves one cell and enters its neighbour
+    scale = np.max(np.abs(flux), axis=0)
+    assert np.all(np.abs(flux.sum(axis=0)) <= 1e-12 * scale * grid.ncells)
+
+
+def test_gi_upwind_column_closed_form(net8):
+    t = net8
+    nx = 6
+    qv, A = 3.1688e-8, 2.0
+    grid = orc.HostGrid(nx, 1, 1, [np.full((1, 1, nx - 1), qv), np.zeros((1, 0, nx)), np.zeros((0, 1, nx))],
+                        [np.zeros((1, 1, nx - 1)), np.zeros((1, 0, nx)), np.zeros((0, 1, nx))], area=(A, 1, 1),
+                        bc_q=(np.full((1, 1), qv), np.full((1, 1), -qv)), bc_dd=(np.zeros((1, 1)), np.zeros((1, 1))),
+                        bc_total=(np.full(t.N, 2e-3), None))
+    cells = state(t, nx, 1, 1, seed=13)
+    opts = synth.react_opts()
+    r, _, _, _ = orc.gi_residual_jacobian(t, cells, grid, opts)
+    rnf, _ = orc.residual_jacobian(t, cells, opts)
+    # totals at the current iterate (mol/L)
+    work = cells.copy()
+    orc.auxvar(t, work, opts)
+    T = work["total"]
+    flux = r - rnf
+    upstream = np.vstack([np.full(t.N, 2e-3), T[:-1]])
+    expect = qv * A * 1000.0 * (T - upstream)
+    assert np.allclose(flux, expect, rtol=1e-12, atol=1e-12 * np.max(np.abs(expect)))
