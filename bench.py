@@ -68,8 +68,10 @@ def main():
     ap.add_argument("--epochs", type=float, default=3.0)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--fallback-after", type=int, default=30)
+    ap.add_argument("--fallback-after", type=int, default=50)
     ap.add_argument("--no-surrogate", action="store_true", help="plain method only")
+    ap.add_argument("--no-gi", action="store_true", help="skip the global-implicit assembly leg")
+    ap.add_argument("--gi-side", type=int, default=128, help="cells per side of the global-implicit grid")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -226,6 +228,8 @@ def main():
                      "algorithmic_bytes_per_cell": bytes_per_cell,
                      "achieved_GBs": bytes_per_cell * n / (kern_ms * 1e-3) / 1e9},
     })
+    if not args.no_gi:
+        line["gi"] = gi_leg(t, tab, args, log)
     if args.calibrate_copy:
         # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
         # calibrate the byte counters (256 MiB read + 256 MiB written)
@@ -238,6 +242,59 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(t, dc, snap, gpu_m, gpu_status, args.cpu_sample)
     print(json.dumps(line), flush=True)
+
+
+def gi_leg(t, tab, args, log):
+    """Global-implicit assembly of a side^3 grid (7-point, upwind): the
+    first residual/Jacobian evaluation of a time step (x = old state), the
+    BSR Jacobian with column-major 15x15 blocks.  HBM-bound."""
+    import numpy as np
+    import torch
+
+    from pflotran_amd import runtime, synth, workload
+
+    side = args.gi_side
+    n = side ** 3
+    torch.cuda.empty_cache()
+    t0 = time.time()
+    dc, _ = workload.speciated_cells(t, tab, n, workload_id=5)
+    bc_total = dc.t["total"].mean(dim=0).cpu().numpy()
+    g = runtime.DeviceGrid(**workload.gi_grid(side, side, side, bc_total=bc_total))
+    row_ptr, col_idx = runtime.gi_structure(g)
+    nnzb = int(col_idx.numel())
+    N = t.N
+    res = torch.empty((n, N), dtype=torch.float64, device="cuda")
+    vals = torch.empty((nnzb, N, N), dtype=torch.float64, device="cuda")
+    log(f"gi set-up: {n} cells, {nnzb} blocks ({vals.numel() * 8 / 1e9:.1f} GB), {time.time() - t0:.1f} s")
+    opts = synth.react_opts()
+    runtime.gi_residual_jacobian(tab, dc, g, opts, row_ptr, res, vals)   # warm-up
+    ev, asm, wall = [], [], []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        runtime.gi_residual_jacobian(tab, dc, g, synth.react_opts(), row_ptr, res, vals)
+        torch.cuda.synchronize()
+        wall.append((time.perf_counter() - t1) * 1e3)
+        e_ms, a_ms = tab.timing()
+        ev.append(e_ms)
+        asm.append(a_ms)
+    a_ms = float(np.mean(asm))
+    # algorithmic bytes of the assembly kernel: per cell read the non-flux
+    # residual, totals, non-flux block and dT/dm once; write the residual and
+    # every BSR block once
+    bytes_asm = 8 * (n * (2 * N + 2 * N * N) + n * N + nnzb * N * N)
+    ok = bool(torch.isfinite(res).all()) and bool(torch.isfinite(vals).all())
+    out = {"workload": f"{side}^3 grid, N={N}, 7-point upwind, ln k ~ N(ln 1e-12, 1), log unknowns",
+           "ncells": n, "nnzb": nnzb, "ms_per_evaluation": float(np.mean(wall)),
+           "ms_cell_kernel": float(np.mean(ev)), "ms_assembly_kernel": a_ms,
+           "assembly_bytes": bytes_asm, "assembly_GBs": bytes_asm / (a_ms * 1e-3) / 1e9,
+           "assembly_frac_hbm": bytes_asm / (a_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+           "cells_per_s": n / (float(np.mean(wall)) * 1e-3), "finite": ok}
+    del vals, res, dc
+    torch.cuda.empty_cache()
+    log(f"gi: {out['ms_per_evaluation']:.2f} ms/evaluation, assembly {a_ms:.2f} ms "
+        f"({out['assembly_GBs']:.0f} GB/s)")
+    return out
 
 
 def cpu_baseline(t, dc, snap, gpu_m, gpu_status, sample):

@@ -206,6 +206,12 @@ int pfx_abi_version(void);
 int pfx_tables_create(const pfx_reaction_desc *desc, pfx_tables **out);
 int pfx_tables_destroy(pfx_tables *tables);
 
+/* Device time (ms, HIP events) of the launches of the last call made with
+   these tables: ms[0] the per-cell kernel, ms[1] the global-implicit
+   assembly kernel (0 if the call had none).  Profiling aid; no counterpart
+   in the reference (its PetscLogEvent timers play this role). */
+int pfx_tables_timing(pfx_tables *tables, double *ms);
+
 /* Batched operator-split reaction step: RTReact's cell loop
    (reactive_transport.F90:1697-1724) around RReact (reaction.F90:3322-3511),
    solving the dt-consistent per-cell residual of the global-implicit path
@@ -293,6 +299,21 @@ int pfx_gi_structure(const pfx_grid *grid, int32_t *row_ptr, int32_t *col_idx, i
 int pfx_gi_residual_jacobian(pfx_tables *tables, pfx_cells *cells, const pfx_grid *grid,
                              const pfx_react_opts *opts, const int32_t *row_ptr,
                              double *residual, double *bsr_values);
+
+/* Transport coefficients of the grid's faces: RTUpdateTransportCoefs
+   (reactive_transport.F90:852-1020) -> TDispersion / TDispersionBC
+   (transport.F90:65-379), liquid phase, longitudinal dispersion only
+   (no transverse tensor), uniform spacing dx[3] (host):
+     D = max(alpha_L |q| + s phi tau D_m, 1e-40) per side,
+     dd = D_up D_dn / (D_up dist_dn + D_dn dist_up), dist = dx/2 each side;
+   boundary faces: bc_type 1 (Dirichlet) dd = D_cell / (dx/2), 2 (zero
+   gradient or closed) dd = 0.  Cell arrays [ncells] (device; saturation and
+   tortuosity may be NULL = 1) and face fluxes grid->q / grid->bc_q in;
+   dd_out[3] (internal faces) and bc_dd_out[2] (device, may be NULL) out. */
+int pfx_gi_dispersion(const pfx_grid *grid, const double *porosity, const double *saturation,
+                      const double *tortuosity, double dispersivity, double diffusion,
+                      const double *dx, const int32_t *bc_type, double **dd_out,
+                      double **bc_dd_out, void *stream);
 
 /* Mineral volume-fraction update after a converged step: RUpdateKineticState
    (reaction.F90:5340-5366), called per cell by RTUpdateKineticState

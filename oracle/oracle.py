@@ -51,6 +51,7 @@ def load():
         lib.or_density_ifc67.restype = C.c_double
         lib.or_debye_params.argtypes = [C.c_double, C.c_int, vp, vp, vp]
         lib.or_gi_structure.argtypes = [P(abi.Grid), vp, vp, vp]
+        lib.or_gi_dispersion.argtypes = [P(abi.Grid), vp, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp]
         lib.or_gi_residual_jacobian.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.Grid), P(abi.ReactOpts),
                                                 vp, vp, vp, vp, C.c_int]
         _lib = lib
@@ -305,4 +306,22 @@ def bsr_to_dense(vals, row_ptr, col_idx):
             col = col_idx[b]
             A[row * N:(row + 1) * N, col * N:(col + 1) * N] = vals[b]
     return A
+
+
+def gi_dispersion(grid: HostGrid, porosity, saturation=None, tortuosity=None, dispersivity=0.0, diffusion=1e-9,
+                  dx=(1.0, 1.0, 1.0), bc_type=(0, 0)):
+    """Face dispersion coefficients (internal [3] arrays, boundary [2])."""
+    lib = load()
+    g = grid.struct()
+    dd = [np.zeros_like(a) for a in grid.q]
+    bc = [np.zeros((grid.nz, grid.ny)) for _ in range(2)]
+    ddp = (C.c_void_p * 3)(*[a.ctypes.data for a in dd])
+    bcp = (C.c_void_p * 2)(*[a.ctypes.data for a in bc])
+    conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    phi, sat, tau = conv(porosity), conv(saturation), conv(tortuosity)
+    dxa = np.ascontiguousarray(dx, dtype=np.float64)
+    bt = np.ascontiguousarray(bc_type, dtype=np.int32)
+    lib.or_gi_dispersion(C.byref(g), _ptr(phi), _ptr(sat), _ptr(tau), dispersivity, diffusion, dxa.ctypes.data,
+                         bt.ctypes.data, ddp, bcp)
+    return dd, bc
 

@@ -1315,3 +1315,51 @@ int or_gi_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx
   free(tot); free(dtot); free(rnf); free(jnf);
   return 0;
 }
+
+/* TDispersion / TDispersionBC (transport.F90:65-379) on the structured
+   grid, liquid phase, longitudinal dispersion, uniform spacing: the face
+   loop of RTUpdateTransportCoefs (reactive_transport.F90:852-1020). */
+static double or_hydro_disp(const double *phi, const double *sat, const double *tau, int64_t c, double alpha,
+                            double dm, double q) {
+  const double s = sat ? sat[c] : 1.0, t = tau ? tau[c] : 1.0;
+  const double v = alpha * fabs(q) + s * phi[c] * t * dm;
+  return v > 1e-40 ? v : 1e-40;
+}
+
+int or_gi_dispersion(const pfx_grid *g, const double *phi, const double *sat, const double *tau, double alpha,
+                     double dm, const double *dx, const int32_t *bc_type, double **dd, double **bc_dd) {
+  const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+  for (int d = 0; d < 3; d++) {
+    const int64_t ex = nx - (d == 0), ey = ny - (d == 1), ez = nz - (d == 2);
+    const int64_t stride = d == 0 ? 1 : (d == 1 ? nx : nx * ny);
+    for (int64_t k = 0; k < ez; k++)
+      for (int64_t j = 0; j < ey; j++)
+        for (int64_t i = 0; i < ex; i++) {
+          const int64_t f = i + ex * (j + ey * k);
+          const int64_t up = i + nx * (j + ny * k), dn = up + stride;
+          const double q = g->q[d][f];
+          const double su = sat ? sat[up] : 1.0, sd = sat ? sat[dn] : 1.0;
+          if (su < 1e-40 || sd < 1e-40) {
+            dd[d][f] = 0.0;
+            continue;
+          }
+          const double Du = or_hydro_disp(phi, sat, tau, up, alpha, dm, q);
+          const double Dd = or_hydro_disp(phi, sat, tau, dn, alpha, dm, q);
+          const double dist_up = 0.5 * dx[d], dist_dn = 0.5 * dx[d];
+          dd[d][f] = (Du * Dd) / (Du * dist_dn + Dd * dist_up);
+        }
+  }
+  for (int side = 0; side < 2; side++) {
+    if (!bc_dd || !bc_dd[side]) continue;
+    for (int64_t jk = 0; jk < ny * nz; jk++) {
+      const int64_t c = (side == 0 ? 0 : nx - 1) + nx * jk;
+      double v = 0.0;
+      if (bc_type && bc_type[side] == 1) {
+        const double q = g->bc_q[side] ? g->bc_q[side][jk] : 0.0;
+        v = or_hydro_disp(phi, sat, tau, c, alpha, dm, q) / (0.5 * dx[0]);
+      }
+      bc_dd[side][jk] = v;
+    }
+  }
+  return 0;
+}

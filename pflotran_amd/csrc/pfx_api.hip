@@ -29,6 +29,7 @@ struct pfx_tables {
   unsigned long long *d_stats = nullptr;  // reduction buffer
   double *d_gi = nullptr;                 // global-implicit per-cell scratch
   int64_t gi_cells = 0;
+  double last_ms[2] = {0.0, 0.0};         // device time of the last call's launches
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int device = 0;
   int num_cu = 0;
@@ -187,6 +188,8 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   float f = 0.f;
   HIPCHK(hipEventElapsedTime(&f, t->ev0, t->ev1));
   if (ms) *ms = f;
+  t->last_ms[0] = f;
+  t->last_ms[1] = 0.0;
   return 0;
 }
 
@@ -320,6 +323,20 @@ int pfx_internal::gi_evaluate_cells(pfx_tables *t, pfx_cells *cells, const pfx_r
   P.dtot_out = dtot;
   P.col_scale = col_scale ? 1 : 0;
   return launch(t, P, PFX_MODE_EVAL, (hipStream_t)o->stream, nullptr);
+}
+
+int pfx_internal::timed_launch_begin(pfx_tables *t, hipStream_t s) {
+  HIPCHK(hipEventRecord(t->ev0, s));
+  return 0;
+}
+
+int pfx_internal::timed_launch_end(pfx_tables *t, hipStream_t s, int slot) {
+  HIPCHK(hipEventRecord(t->ev1, s));
+  HIPCHK(hipEventSynchronize(t->ev1));
+  float f = 0.f;
+  HIPCHK(hipEventElapsedTime(&f, t->ev0, t->ev1));
+  t->last_ms[slot] = f;
+  return 0;
 }
 
 void pfx_internal::table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R) {
@@ -667,5 +684,12 @@ extern "C" int pfx_update_kinetic_state(pfx_tables *t, pfx_cells *cells, const p
                      t->L.o_mmolvol, t->L.K, cells->ncells, cells->mnrl_rate, (double *)cells->mnrl_volfrac, o->dt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
+  return 0;
+}
+
+extern "C" int pfx_tables_timing(pfx_tables *t, double *ms) {
+  if (!t || !ms) return PFX_E_ARG;
+  ms[0] = t->last_ms[0];
+  ms[1] = t->last_ms[1];
   return 0;
 }

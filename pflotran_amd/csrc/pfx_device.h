@@ -94,4 +94,7 @@ void table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R);
 int gi_scratch(pfx_tables *t, int64_t ncells, double **rnf, double **jnf, double **tot, double **dtot);
 int gi_evaluate_cells(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, int col_scale, double *rnf,
                       double *jnf, double *tot, double *dtot);
+// event-timed launch bracket; the time goes to the tables' last_ms[slot]
+int timed_launch_begin(pfx_tables *t, hipStream_t s);
+int timed_launch_end(pfx_tables *t, hipStream_t s, int slot);
 }

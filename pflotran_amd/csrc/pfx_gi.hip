@@ -144,6 +144,65 @@ __global__ __launch_bounds__(256) void k_gi_assemble(const gi_params P) {
   }
 }
 
+struct disp_params {
+  int32_t nx, ny, nz;
+  const double *q[3], *bc_q[2];
+  const double *phi, *sat, *tau;
+  double alpha, dm, dx[3];
+  int32_t bc_type[2];
+  double *dd[3], *bc_dd[2];
+};
+
+__device__ __forceinline__ double hydro_disp(const disp_params &P, int64_t c, double q) {
+  const double s = P.sat ? P.sat[c] : 1.0;
+  const double t = P.tau ? P.tau[c] : 1.0;
+  return fmax(P.alpha * fabs(q) + s * P.phi[c] * t * P.dm, 1e-40);
+}
+
+// one thread per face: x faces, then y faces, then z faces, then the
+// x-min / x-max boundary faces
+__global__ void k_gi_dispersion(const disp_params P) {
+  const int64_t nx = P.nx, ny = P.ny, nz = P.nz;
+  const int64_t nfx = (nx - 1) * ny * nz, nfy = nx * (ny - 1) * nz, nfz = nx * ny * (nz - 1), nb = ny * nz;
+  int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int d = 0;
+  if (f >= nfx) {
+    f -= nfx;
+    d = 1;
+    if (f >= nfy) {
+      f -= nfy;
+      d = 2;
+      if (f >= nfz) {
+        f -= nfz;
+        d = 3;
+        if (f >= 2 * nb) return;
+      }
+    }
+  }
+  if (d < 3) {
+    const int64_t ex = nx - (d == 0), ey = ny - (d == 1);
+    const int64_t i = f % ex, j = (f / ex) % ey, k = f / (ex * ey);
+    const int64_t up = i + nx * (j + ny * k);
+    const int64_t dn = up + (d == 0 ? 1 : (d == 1 ? nx : nx * ny));
+    const double q = P.q[d][f];
+    const double Du = hydro_disp(P, up, q), Dd = hydro_disp(P, dn, q);
+    const double half = 0.5 * P.dx[d];
+    const double su = P.sat ? P.sat[up] : 1.0, sd = P.sat ? P.sat[dn] : 1.0;
+    P.dd[d][f] = (su < 1e-40 || sd < 1e-40) ? 0.0 : (Du * Dd) / (Du * half + Dd * half);
+  } else {
+    const int side = (int)(f / nb);
+    const int64_t jk = f % nb;
+    if (!P.bc_dd[side]) return;
+    double v = 0.0;
+    if (P.bc_type[side] == 1) {
+      const int64_t c = (side == 0 ? 0 : nx - 1) + nx * jk;
+      const double q = P.bc_q[side] ? P.bc_q[side][jk] : 0.0;
+      v = hydro_disp(P, c, q) / (0.5 * P.dx[0]);
+    }
+    P.bc_dd[side][jk] = v;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -223,7 +282,43 @@ extern "C" int pfx_gi_residual_jacobian(pfx_tables *t, pfx_cells *cells, const p
   P.res = residual;
   P.vals = bsr_values;
   hipStream_t s = (hipStream_t)o->stream;
+  if ((rc = pfx_internal::timed_launch_begin(t, s))) return rc;
   hipLaunchKernelGGL(k_gi_assemble, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, P);
+  GCHK(hipGetLastError());
+  return pfx_internal::timed_launch_end(t, s, 1);
+}
+
+extern "C" int pfx_gi_dispersion(const pfx_grid *g, const double *porosity, const double *saturation,
+                                 const double *tortuosity, double dispersivity, double diffusion, const double *dx,
+                                 const int32_t *bc_type, double **dd_out, double **bc_dd_out, void *stream) {
+  if (!g || !porosity || !dx || !dd_out || g->nx < 1 || g->ny < 1 || g->nz < 1) return PFX_E_ARG;
+  disp_params P;
+  memset(&P, 0, sizeof(P));
+  P.nx = g->nx;
+  P.ny = g->ny;
+  P.nz = g->nz;
+  const int64_t ext[3] = {g->nx, g->ny, g->nz};
+  for (int d = 0; d < 3; d++) {
+    P.q[d] = g->q[d];
+    P.dd[d] = dd_out[d];
+    P.dx[d] = dx[d];
+    if (ext[d] > 1 && (!P.q[d] || !P.dd[d])) return PFX_E_ARG;
+    if (!(dx[d] > 0.0)) return PFX_E_ARG;
+  }
+  for (int s = 0; s < 2; s++) {
+    P.bc_q[s] = g->bc_q[s];
+    P.bc_dd[s] = bc_dd_out ? bc_dd_out[s] : nullptr;
+    P.bc_type[s] = bc_type ? bc_type[s] : 0;
+  }
+  P.phi = porosity;
+  P.sat = saturation;
+  P.tau = tortuosity;
+  P.alpha = dispersivity;
+  P.dm = diffusion;
+  const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+  const int64_t nf = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1) + 2 * ny * nz;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_gi_dispersion, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, P);
   GCHK(hipGetLastError());
   GCHK(hipStreamSynchronize(s));
   return 0;

@@ -182,6 +182,17 @@ __device__ __forceinline__ Scr make_scr(int base, int M, int G, int K) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// x^z for the valence ratios of ion exchange: the common ratios 1, 2, 1/2
+// and 3 without the log/exp of a general power (same value to the last bit
+// or within one rounding)
+__device__ __forceinline__ double pow_ratio(double x, double z) {
+  if (z == 1.0) return x;
+  if (z == 2.0) return x * x;
+  if (z == 0.5) return sqrt(x);
+  if (z == 3.0) return x * x * x;
+  return pow(x, z);
+}
+
 // Kernel modes (template MODE):
 //   PFX_MODE_REACT  reaction step, RReact (reaction.F90:3322-3511)
 //   PFX_MODE_EVAL   residual + Jacobian at given free ions (reaction.F90:3419-3441)
@@ -525,7 +536,8 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
               break;
             }
             ref_X = KDj * (ref_k * ref_conc);
-            X = role == 0 ? ref_X : (role > 0 ? kq * m * gam * pow(KDj, zratio) : 0.0);
+            if (role > 0) X = kq * m * gam * pow_ratio(KDj, zratio);
+            else X = role == 0 ? ref_X : 0.0;
             const double total = gsum<G>(X, lane);
             double dres = gsum<G>(role > 0 ? X / KDj * Zr : 0.0, lane);
             dres = dres / ref_Z + (ref_k * ref_conc);
@@ -547,13 +559,14 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
         const double t2 = Zr / sumZX;
         Sr += t1;
         if (need_ionx) {
+          const double wscale = ionx_w / (live ? psv_t / dt : 1.0);
           SD(S.xb, r) = X;
           wave_sync();
 #pragma unroll
           for (int k = 0; k < G; k++) {
             const double Xk = SD(S.xb, k);
             const double d = (k == r) ? t1 * (1.0 - (t2 * Xk)) : (-t1) * t2 * Xk;
-            row[k] += d * SD(S.im, k) * ionx_w / (live ? psv_t / dt : 1.0);
+            row[k] += d * SD(S.im, k) * wscale;
           }
           wave_sync();
         }
@@ -710,7 +723,7 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
           double piv = bperm_d(row[j], src);
           if (piv == 0.0) piv = 1.0e-20;
           const bool elim = (r != p) && !pivoted;
-          const double f = elim ? row[j] / piv : 0.0;
+          const double f = elim ? row[j] / piv : 0.0;  // 0 leaves the row unchanged below
           if (r == p) {
             pivoted = true;
             row[j] = piv;
@@ -720,10 +733,10 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
 #pragma unroll
           for (int k = j + 1; k < G; k++) {
             const double pk = bperm_d(row[k], src);
-            if (elim) row[k] -= f * pk;
+            row[k] -= f * pk;
           }
           const double pb = bperm_d(b, src);
-          if (elim) b -= f * pb;
+          b -= f * pb;
         }
       }
       // back substitution: the pivot row of step j is held by lane plane[j];

@@ -9,6 +9,7 @@ program pfx_abi_probe
   type(pfx_cells) :: cells
   type(pfx_react_opts) :: opts
   type(pfx_stats) :: stats
+  type(pfx_grid) :: grid
   type(c_ptr) :: tables
   integer(c_int) :: rc
 
@@ -24,4 +25,5 @@ program pfx_abi_probe
   print '(a,i0)', 'sizeof_cells ', c_sizeof(cells)
   print '(a,i0)', 'sizeof_opts ', c_sizeof(opts)
   print '(a,i0)', 'sizeof_stats ', c_sizeof(stats)
+  print '(a,i0)', 'sizeof_grid ', c_sizeof(grid)
 end program pfx_abi_probe

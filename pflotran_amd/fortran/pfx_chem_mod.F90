@@ -66,6 +66,13 @@ module pfx_chem_module
     integer(c_int32_t) :: fallback_after, reserved1
   end type pfx_react_opts
 
+  type, bind(c), public :: pfx_grid
+    integer(c_int32_t) :: nx, ny, nz, log_formulation
+    real(c_double) :: area(3)
+    type(c_ptr) :: q(3), dd(3)
+    type(c_ptr) :: bc_q(2), bc_dd(2), bc_total(2)
+  end type pfx_grid
+
   type, bind(c), public :: pfx_stats
     integer(c_int64_t) :: ncells, converged, total_iterations
     integer(c_int32_t) :: max_iterations, min_iterations
@@ -76,7 +83,8 @@ module pfx_chem_module
 
   public :: pfx_abi_version, pfx_tables_create, pfx_tables_destroy, &
             pfx_react_batch, pfx_residual_jacobian, pfx_equilibrate_batch, &
-            pfx_auxvar_batch, pfx_update_kinetic_state
+            pfx_auxvar_batch, pfx_update_kinetic_state, pfx_tables_timing, &
+            pfx_gi_structure, pfx_gi_residual_jacobian
   ! learned warm start (include/pfx_surrogate.h)
   public :: pfx_mlp_create, pfx_mlp_destroy, pfx_mlp_params, pfx_mlp_forward, &
             pfx_mlp_gradients, pfx_mlp_adam, pfx_mlp_train_step, &
@@ -144,6 +152,32 @@ module pfx_chem_module
       type(pfx_react_opts), intent(in) :: opts
       integer(c_int) :: pfx_auxvar_batch
     end function pfx_auxvar_batch
+
+    function pfx_tables_timing(tables, ms) bind(c, name='pfx_tables_timing')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: tables
+      real(c_double), intent(out) :: ms(2)
+      integer(c_int) :: pfx_tables_timing
+    end function pfx_tables_timing
+
+    function pfx_gi_structure(grid, row_ptr, col_idx, nnzb) bind(c, name='pfx_gi_structure')
+      import :: c_int, c_int64_t, c_ptr, pfx_grid
+      type(pfx_grid), intent(in) :: grid
+      type(c_ptr), value :: row_ptr, col_idx
+      integer(c_int64_t), intent(out) :: nnzb
+      integer(c_int) :: pfx_gi_structure
+    end function pfx_gi_structure
+
+    function pfx_gi_residual_jacobian(tables, cells, grid, opts, row_ptr, residual, values) &
+        bind(c, name='pfx_gi_residual_jacobian')
+      import :: c_int, c_ptr, pfx_cells, pfx_grid, pfx_react_opts
+      type(c_ptr), value :: tables
+      type(pfx_cells), intent(inout) :: cells
+      type(pfx_grid), intent(in) :: grid
+      type(pfx_react_opts), intent(in) :: opts
+      type(c_ptr), value :: row_ptr, residual, values
+      integer(c_int) :: pfx_gi_residual_jacobian
+    end function pfx_gi_residual_jacobian
 
     function pfx_update_kinetic_state(tables, cells, opts) &
         bind(c, name='pfx_update_kinetic_state')

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "libpfx_chem.so")
 
 EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_react_batch",
             "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch", "pfx_update_kinetic_state",
-            "pfx_gi_structure", "pfx_gi_residual_jacobian",
+            "pfx_gi_structure", "pfx_gi_residual_jacobian", "pfx_tables_timing", "pfx_gi_dispersion",
             # learned warm start (include/pfx_surrogate.h)
             "pfx_mlp_create", "pfx_mlp_destroy", "pfx_mlp_params", "pfx_mlp_forward", "pfx_mlp_gradients",
             "pfx_mlp_adam", "pfx_mlp_train_step", "pfx_surrogate_features", "pfx_surrogate_guess",
@@ -56,7 +56,10 @@ def load_library(path: str = LIB_PATH):
                                           P(abi.ReactOpts), P(abi.Stats)]
     lib.pfx_auxvar_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_int32, P(abi.ReactOpts)]
     lib.pfx_update_kinetic_state.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts)]
+    lib.pfx_tables_timing.argtypes = [C.c_void_p, C.c_void_p]
     lib.pfx_gi_structure.argtypes = [P(abi.Grid), C.c_void_p, C.c_void_p, P(C.c_int64)]
+    lib.pfx_gi_dispersion.argtypes = [P(abi.Grid), C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.pfx_gi_residual_jacobian.argtypes = [C.c_void_p, P(abi.Cells), P(abi.Grid), P(abi.ReactOpts), C.c_void_p,
                                              C.c_void_p, C.c_void_p]
     vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
@@ -96,6 +99,12 @@ class Tables:
         if rc != 0:
             raise PfxError(f"pfx_tables_create failed: {rc}")
         self.handle = h
+
+    def timing(self):
+        """(per-cell kernel ms, assembly kernel ms) of the last call."""
+        ms = np.zeros(2)
+        load_library().pfx_tables_timing(self.handle, ms.ctypes.data)
+        return float(ms[0]), float(ms[1])
 
     def close(self):
         if getattr(self, "handle", None):
@@ -294,4 +303,29 @@ def gi_residual_jacobian(tables: Tables, cells: DeviceCells, grid: DeviceGrid, o
     if rc != 0:
         raise PfxError(f"pfx_gi_residual_jacobian failed: {rc}")
     return residual, values
+
+
+def gi_dispersion(grid: DeviceGrid, porosity, saturation=None, tortuosity=None, dispersivity=0.0, diffusion=1e-9,
+                  dx=(1.0, 1.0, 1.0), bc_type=(0, 0)):
+    """pfx_gi_dispersion: fills grid.dd and grid.bc_dd (device) from cell
+    properties (device tensors [ncells]) and the grid's face fluxes."""
+    lib = load_library()
+    torch = _torch()
+    for d in range(3):
+        if grid.dd[d] is None or grid.dd[d].shape != grid.q[d].shape:
+            grid.dd[d] = torch.zeros_like(grid.q[d])
+    for s_ in range(2):
+        if grid.bc_dd[s_] is None:
+            grid.bc_dd[s_] = torch.zeros((grid.nz, grid.ny), dtype=torch.float64, device="cuda")
+    g = grid.struct()
+    ptr = lambda a: None if a is None else a.data_ptr()  # noqa: E731
+    ddp = (C.c_void_p * 3)(*[ptr(a) if a.numel() else None for a in grid.dd])
+    bcp = (C.c_void_p * 2)(*[ptr(a) for a in grid.bc_dd])
+    dxa = np.ascontiguousarray(dx, dtype=np.float64)
+    bt = np.ascontiguousarray(bc_type, dtype=np.int32)
+    rc = lib.pfx_gi_dispersion(C.byref(g), ptr(porosity), ptr(saturation), ptr(tortuosity), dispersivity, diffusion,
+                               dxa.ctypes.data, bt.ctypes.data, ddp, bcp,
+                               C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    if rc != 0:
+        raise PfxError(f"pfx_gi_dispersion failed: {rc}")
 

@@ -27,7 +27,7 @@ from .database import (ReactionTables, debye_params, logk_eval, logk_fit, water_
 SEED = 20261202
 DB_TEMPS = [0.0, 25.0, 60.0, 100.0, 150.0, 200.0, 250.0, 300.0]
 
-FIELDS = {"table": 1, "totals": 2, "ph": 3, "temp": 4, "vf": 5, "guess": 6, "perturb": 7}
+FIELDS = {"table": 1, "totals": 2, "ph": 3, "temp": 4, "vf": 5, "guess": 6, "perturb": 7, "perm": 8}
 
 
 def rng(workload_id: int, field: str, seed: int = SEED) -> np.random.Generator:

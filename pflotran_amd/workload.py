@@ -64,3 +64,35 @@ def subset(dc: runtime.DeviceCells, idx) -> runtime.DeviceCells:
 
 
 __all__ = ["speciated_cells", "snapshot", "restore", "host_arrays", "subset", "STEP_INPUTS"]
+
+
+# ---------------------------------------------------------------------------
+# structured grid of the global-implicit configuration
+D_MOLECULAR = 1e-9      # m^2/s
+TORTUOSITY = 1.0
+POROSITY = 0.25
+Q_MEAN = 3.1688e-8      # 1 m/yr
+
+
+def gi_grid(nx: int, ny: int, nz: int, seed: int = synth.SEED, workload_id: int = 5, dx: float = 1.0,
+            log_formulation: bool = True, bc_total=None):
+    """Host arrays of a 7-point grid with a seeded log-normal permeability
+    (ln k ~ N(ln 1e-12, 1)) and a steady flow field: each x row carries the
+    1-D Darcy flux of its harmonic-mean permeability under one head drop
+    for the whole box, scaled to a mean of 1 m/yr (divergence-free; no
+    cross flow).  Dispersion: saturation * porosity * tortuosity * D_m over
+    the cell distance (alpha_L = 0, TDispersion); at the Dirichlet x-min
+    inflow over dx/2, none at the zero-gradient x-max outflow
+    (TDispersionBC).  Returns a dict of numpy arrays for runtime.DeviceGrid."""
+    import numpy as np
+    lnk = synth.rng(workload_id, "perm").normal(np.log(1e-12), 1.0, size=(nz, ny, nx))
+    k = np.exp(lnk)
+    k_h = nx / np.sum(1.0 / k, axis=2)                 # [nz][ny]
+    q_row = Q_MEAN * k_h / np.exp(np.mean(np.log(k_h)))
+    dd = POROSITY * 1.0 * TORTUOSITY * D_MOLECULAR / dx
+    q = [np.repeat(q_row[:, :, None], max(nx - 1, 0), axis=2), np.zeros((nz, max(ny - 1, 0), nx)),
+         np.zeros((max(nz - 1, 0), ny, nx))]
+    dda = [np.full(a.shape, dd) for a in q]
+    return dict(nx=nx, ny=ny, nz=nz, q=q, dd=dda, area=(dx * dx,) * 3,
+                bc_q=(q_row.copy(), -q_row.copy()), bc_dd=(np.full((nz, ny), 2 * dd), np.zeros((nz, ny))),
+                bc_total=(bc_total, None), log_formulation=log_formulation)

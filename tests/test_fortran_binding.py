@@ -31,3 +31,4 @@ def test_fortran_module_binds_the_c_abi(tmp_path):
     assert int(vals["sizeof_cells"]) == C.sizeof(abi.Cells)
     assert int(vals["sizeof_opts"]) == C.sizeof(abi.ReactOpts)
     assert int(vals["sizeof_stats"]) == C.sizeof(abi.Stats)
+    assert int(vals["sizeof_grid"]) == C.sizeof(abi.Grid)

@@ -67,3 +67,24 @@ def test_gi_assembly_parity(net15, shape, log_form, dirichlet):
         for b in range(rp_o[row], rp_o[row + 1]):
             err = max(err, np.max(np.abs(v_g[b] - v_o[b]) / rowmax[row][:, None]))
     assert err <= 2e-12
+
+
+def test_gi_dispersion_parity():
+    import torch
+    nx, ny, nz = 7, 5, 4
+    n = nx * ny * nz
+    rng = np.random.default_rng(12)
+    q = [rng.normal(0, 1e-6, size=s) for s in [(nz, ny, nx - 1), (nz, ny - 1, nx), (nz - 1, ny, nx)]]
+    bc_q = (np.full((nz, ny), 2e-6), -np.full((nz, ny), 1e-6))
+    phi = rng.uniform(0.1, 0.4, size=n)
+    sat = rng.uniform(0.5, 1.0, size=n)
+    tau = rng.uniform(0.5, 1.0, size=n)
+    hg = orc.HostGrid(nx, ny, nz, q, [np.zeros_like(a) for a in q], bc_q=bc_q)
+    dd_o, bc_o = orc.gi_dispersion(hg, phi, sat, tau, 0.2, 1e-9, (1.0, 2.0, 0.5), bc_type=(1, 2))
+    dg = runtime.DeviceGrid(nx, ny, nz, q, [None, None, None], bc_q=bc_q)
+    dev = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
+    runtime.gi_dispersion(dg, dev(phi), dev(sat), dev(tau), 0.2, 1e-9, (1.0, 2.0, 0.5), bc_type=(1, 2))
+    for d in range(3):
+        assert np.allclose(dg.dd[d].cpu().numpy(), dd_o[d], rtol=4e-15, atol=0)
+    for s_ in range(2):
+        assert np.allclose(dg.bc_dd[s_].cpu().numpy(), bc_o[s_], rtol=4e-15, atol=0)

@@ -122,3 +122,31 @@ def test_gi_upwind_column_closed_form(net8):
     upstream = np.vstack([np.full(t.N, 2e-3), T[:-1]])
     expect = qv * A * 1000.0 * (T - upstream)
     assert np.allclose(flux, expect, rtol=1e-12, atol=1e-12 * np.max(np.abs(expect)))
+
+
+def test_dispersion_closed_forms():
+    nx, ny, nz = 4, 3, 2
+    n = nx * ny * nz
+    rng = np.random.default_rng(8)
+    q = [rng.normal(0, 1e-6, size=s) for s in [(nz, ny, nx - 1), (nz, ny - 1, nx), (nz - 1, ny, nx)]]
+    grid = orc.HostGrid(nx, ny, nz, q, [np.zeros_like(a) for a in q], bc_q=(np.full((nz, ny), 2e-6), None))
+    phi = rng.uniform(0.1, 0.4, size=n)
+    sat = rng.uniform(0.5, 1.0, size=n)
+    alpha, dm, dx = 0.3, 1e-9, (1.0, 2.0, 0.5)
+    dd, bc = orc.gi_dispersion(grid, phi, sat, None, alpha, dm, dx, bc_type=(1, 2))
+    # x face between cells 0 and 1: harmonic mean of the two sides over dx
+    Du = alpha * abs(q[0][0, 0, 0]) + sat[0] * phi[0] * dm
+    Dd = alpha * abs(q[0][0, 0, 0]) + sat[1] * phi[1] * dm
+    assert dd[0][0, 0, 0] == pytest.approx(2 * Du * Dd / (dx[0] * (Du + Dd)), rel=1e-15)
+    # z face between cells 0 and nx*ny
+    c1 = nx * ny
+    Du = alpha * abs(q[2][0, 0, 0]) + sat[0] * phi[0] * dm
+    Dd = alpha * abs(q[2][0, 0, 0]) + sat[c1] * phi[c1] * dm
+    assert dd[2][0, 0, 0] == pytest.approx(2 * Du * Dd / (dx[2] * (Du + Dd)), rel=1e-15)
+    # Dirichlet boundary: the cell's dispersion over half a cell; zero-gradient: 0
+    assert bc[0][0, 0] == pytest.approx((alpha * 2e-6 + sat[0] * phi[0] * dm) / (0.5 * dx[0]), rel=1e-15)
+    assert np.all(bc[1] == 0.0)
+    # uniform properties: dd = D / dx
+    dd_u, _ = orc.gi_dispersion(grid, np.full(n, 0.25), None, None, 0.0, dm, (1.0, 1.0, 1.0))
+    for a in dd_u:
+        assert np.allclose(a, 0.25 * dm, rtol=1e-15, atol=0)
