@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--fallback-after", type=int, default=50)
     ap.add_argument("--no-surrogate", action="store_true", help="plain method only")
     ap.add_argument("--no-gi", action="store_true", help="skip the global-implicit assembly leg")
+    ap.add_argument("--no-zones", action="store_true", help="skip the chemistry-zones leg")
+    ap.add_argument("--zones-scale", type=float, default=1.0, help="fraction of the 2M zoned cells")
     ap.add_argument("--gi-side", type=int, default=128, help="cells per side of the global-implicit grid")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (HBM byte-counter calibration)")
@@ -230,6 +232,8 @@ def main():
     })
     if not args.no_gi:
         line["gi"] = gi_leg(t, tab, args, log)
+    if not args.no_zones:
+        line["zones"] = zones_leg(args, log)
     if args.calibrate_copy:
         # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
         # calibrate the byte counters (256 MiB read + 256 MiB written)
@@ -294,6 +298,89 @@ def gi_leg(t, tab, args, log):
     torch.cuda.empty_cache()
     log(f"gi: {out['ms_per_evaluation']:.2f} ms/evaluation, assembly {a_ms:.2f} ms "
         f"({out['assembly_GBs']:.0f} GB/s)")
+    return out
+
+
+def zones_leg(args, log):
+    """Uneven batch in one persistent launch: 4 chemistry zones (N = 8, 12,
+    20, 30; 3N complexes, 3 minerals) of 1M/500k/300k/200k cells, half of
+    each zone within 0.05 of mineral saturation; one reaction step from the
+    previous free ions.  Timed: the zoned launch with the split the
+    autotuner (rebalance from measured zone times) picked, replayed as a
+    HIP graph; beside it the same cells as 4 separate launches."""
+    import numpy as np
+    import torch
+
+    from pflotran_amd import abi, runtime, synth, workload
+
+    nets = workload.zone_networks()
+    tabs = [runtime.Tables(t) for t in nets]
+    sizes = [max(1000, int(n * args.zones_scale)) for n in workload.ZONE_CELLS]
+    t0 = time.time()
+    dcs, inside = [], []
+    for z, (t, tab, n) in enumerate(zip(nets, tabs, sizes)):
+        dc, lo, ins = workload.near_saturation_cells(t, tab, n, seed=synth.SEED + z, workload_id=3)
+        dcs.append(dc)
+        inside.append(float(ins.mean()))
+    snaps = [workload.snapshot(dc) for dc in dcs]
+    log(f"zones set-up: {sum(sizes)} cells in {time.time() - t0:.1f} s, near saturation {inside}")
+
+    def restore():
+        for dc, sn in zip(dcs, snaps):
+            workload.restore(dc, sn)
+
+    opts = synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS)
+    opts.stream = torch.cuda.current_stream().cuda_stream
+    # separate launches, one per zone
+    restore()
+    runtime.react_batch(tabs[0], dcs[0], opts)
+    sep = []
+    for _ in range(3):
+        restore()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for tab, dc in zip(tabs, dcs):
+            runtime.react_batch(tab, dc, opts)
+        torch.cuda.synchronize()
+        sep.append((time.perf_counter() - t1) * 1e3)
+    # zoned launch: automatic split, then rebalanced from the zone times
+    plan = None
+    history = []
+    for _ in range(4):
+        restore()
+        rc, st, plan = runtime.react_zones(tabs, dcs, opts, plan)
+        history.append({"blocks": list(plan.blocks[:4]), "zone_ms": list(plan.zone_ms[:4]),
+                        "kernel_ms": plan.kernel_ms})
+        runtime.rebalance(plan)
+    best = min(history, key=lambda h: h["kernel_ms"])
+    plan.blocks[:4] = best["blocks"]
+    graph = runtime.ZonesGraph(tabs, dcs, opts, plan)
+    g_ms, k_ms = [], []
+    for _ in range(5):
+        restore()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        graph.launch()
+        rc, st, gp = graph.stats()
+        g_ms.append((time.perf_counter() - t1) * 1e3)
+        k_ms.append(gp.kernel_ms)
+    graph.close()
+    n = sum(sizes)
+    conv = sum(s.converged for s in st)
+    its = [s.total_iterations / s.ncells for s in st]
+    hist = np.sum([np.array(list(s.iteration_histogram)) for s in st], axis=0)
+    out = {"workload": "4 zones N=8/12/20/30 (M=3N, K=3), 1M/500k/300k/200k cells x scale, half within 0.05 "
+                       "of mineral-0 saturation, previous-state guess",
+           "ncells": n, "zone_cells": sizes, "near_saturation_fraction": inside,
+           "graph_ms_per_step": float(np.mean(g_ms)), "graph_kernel_ms": float(np.mean(k_ms)),
+           "cells_per_s": conv / (float(np.mean(g_ms)) * 1e-3),
+           "separate_launches_ms": float(np.mean(sep)), "autotune_history": history, "blocks": best["blocks"],
+           "mean_iterations_per_zone": its, "iteration_histogram": hist.tolist(),
+           "converged": int(conv)}
+    log(f"zones: graph {out['graph_ms_per_step']:.2f} ms/step (kernel {out['graph_kernel_ms']:.2f}), "
+        f"separate {out['separate_launches_ms']:.2f} ms, its/zone {[round(x, 2) for x in its]}")
+    del dcs, snaps
+    torch.cuda.empty_cache()
     return out
 
 

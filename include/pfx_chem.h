@@ -42,6 +42,7 @@ extern "C" {
 #define PFX_MAX_MINERALS 16
 #define PFX_MAX_IONX_RXN 4
 #define PFX_MAX_IONX_CATION 8
+#define PFX_MAX_ZONES 8
 
 /* error codes */
 #define PFX_E_ARG      (-1)   /* bad argument / inconsistent sizes */
@@ -69,6 +70,8 @@ extern "C" {
 #define PFX_CONSTRAINT_FREE  1   /* free ion concentration                 */
 #define PFX_CONSTRAINT_LOG   2   /* log10 free ion concentration           */
 #define PFX_CONSTRAINT_PH    3   /* pH (component must be H+)              */
+#define PFX_CONSTRAINT_CHARGE 4  /* charge balance: sum z_j T_j = 0; the value
+                                    is the starting free concentration      */
 
 /* initial Newton iterate */
 #define PFX_GUESS_PREVIOUS  0   /* previous free ions (RReact's guess, reaction.F90:3370) */
@@ -314,6 +317,45 @@ int pfx_gi_dispersion(const pfx_grid *grid, const double *porosity, const double
                       const double *tortuosity, double dispersivity, double diffusion,
                       const double *dx, const int32_t *bc_type, double **dd_out,
                       double **bc_dd_out, void *stream);
+
+/*
+ * Chemistry zones in one persistent launch.  No counterpart in the
+ * reference, where RTReact loops over every cell with one reaction_type:
+ * here each zone has its own tables (its own primaries, complexes and
+ * minerals) and its own batch of cells, and one launch reacts them all.
+ * The workgroups are split between the zones; each zone drains its own
+ * work queue, and every workgroup stamps the wall clock when it retires, so
+ * the split can follow the measured work (pfx_zone_plan_rebalance).
+ */
+typedef struct pfx_zone_plan {
+  int32_t nzones;
+  int32_t blocks[PFX_MAX_ZONES];   /* In: workgroups per zone (all 0 = split by
+                                      cells * N^2); Out: the split used       */
+  double zone_ms[PFX_MAX_ZONES];   /* Out: when the zone's last workgroup retired */
+  double kernel_ms;                /* Out: the launch (HIP events)            */
+} pfx_zone_plan;
+
+/* pfx_react_batch for nzones (tables[z], cells[z]) pairs in one launch;
+   opts are shared (dt, tolerances, guess mode).  plan may be NULL (split
+   by cells * N^2).  stats: [nzones], may be NULL.  Returns the number of
+   cells that did not converge, or a PFX_E_* code. */
+int pfx_react_zones(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, const pfx_react_opts *opts,
+                    pfx_zone_plan *plan, pfx_stats *stats);
+
+/* New split from the last launch's zone times: blocks_z proportional to
+   blocks_z * zone_ms_z (the work each zone took), same total. */
+int pfx_zone_plan_rebalance(pfx_zone_plan *plan);
+
+/* The same zoned step captured as a HIP graph (queue resets, the zoned
+   kernel and the statistics reductions as kernel nodes, no copies), for
+   time-step loops that replay it on the same buffers. */
+typedef struct pfx_graph pfx_graph;
+int pfx_zones_graph_create(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells,
+                           const pfx_react_opts *opts, const pfx_zone_plan *plan, pfx_graph **out);
+int pfx_graph_launch(pfx_graph *graph, void *stream);
+/* statistics of the last replay (waits for the stream it was launched on) */
+int pfx_graph_stats(pfx_graph *graph, pfx_stats *stats, pfx_zone_plan *plan);
+int pfx_graph_destroy(pfx_graph *graph);
 
 /* Mineral volume-fraction update after a converged step: RUpdateKineticState
    (reaction.F90:5340-5366), called per cell by RTUpdateKineticState

@@ -12,6 +12,7 @@ PFX_MAX_RXN_SPEC = 8
 PFX_MAX_MINERALS = 16
 PFX_MAX_IONX_RXN = 4
 PFX_MAX_IONX_CATION = 8
+PFX_MAX_ZONES = 8
 
 PFX_E_ARG, PFX_E_DEVICE, PFX_E_NOMEM, PFX_E_UNSUP = -1, -2, -3, -4
 
@@ -20,7 +21,7 @@ PFX_ACT_ALG_LAG, PFX_ACT_ALG_NEWTON = 0, 1
 
 PFX_CELL_CONVERGED, PFX_CELL_MAX_ITERATIONS, PFX_CELL_NONFINITE, PFX_CELL_SINGULAR = 0, 1, 2, 3
 PFX_GUESS_PREVIOUS, PFX_GUESS_ARRAY, PFX_GUESS_DEFAULT, PFX_GUESS_ARRAY_FALLBACK = 0, 1, 2, 3
-PFX_CONSTRAINT_TOTAL, PFX_CONSTRAINT_FREE, PFX_CONSTRAINT_LOG, PFX_CONSTRAINT_PH = 0, 1, 2, 3
+PFX_CONSTRAINT_TOTAL, PFX_CONSTRAINT_FREE, PFX_CONSTRAINT_LOG, PFX_CONSTRAINT_PH, PFX_CONSTRAINT_CHARGE = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -94,6 +95,13 @@ class Grid(C.Structure):
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("log_formulation", C.c_int32),
         ("area", C.c_double * 3), ("q", C.c_void_p * 3), ("dd", C.c_void_p * 3),
         ("bc_q", C.c_void_p * 2), ("bc_dd", C.c_void_p * 2), ("bc_total", C.c_void_p * 2),
+    ]
+
+
+class ZonePlan(C.Structure):
+    _fields_ = [
+        ("nzones", C.c_int32), ("blocks", C.c_int32 * PFX_MAX_ZONES), ("zone_ms", C.c_double * PFX_MAX_ZONES),
+        ("kernel_ms", C.c_double),
     ]
 
 

@@ -16,6 +16,8 @@
 
 template <int G, int MODE>
 __global__ void pfx_react_kernel(const pfx_kernel_params P);
+__global__ void pfx_react_zones_kernel(const pfx_kernel_params *zone_params, const pfx_zone_launch Z,
+                                       uint64_t *finish);
 extern "C" int pfx_scr_doubles(int M, int G, int K);
 
 struct pfx_tables {
@@ -30,6 +32,10 @@ struct pfx_tables {
   double *d_gi = nullptr;                 // global-implicit per-cell scratch
   int64_t gi_cells = 0;
   double last_ms[2] = {0.0, 0.0};         // device time of the last call's launches
+  // zoned launches led by these tables: parameter buffer and retire stamps
+  pfx_kernel_params *d_zone_params = nullptr;
+  uint64_t *d_finish = nullptr;
+  int finish_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int device = 0;
   int num_cu = 0;
@@ -38,7 +44,7 @@ struct pfx_tables {
 namespace {
 
 constexpr int BLOCK = 256;
-constexpr int64_t kMaxLds = 64 * 1024;
+constexpr int64_t kMaxLds = 160 * 1024;   // gfx950 LDS per workgroup
 
 int g_last_hip_error = 0;
 
@@ -168,6 +174,7 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   if (lds > kMaxLds) return PFX_E_UNSUP;
   int occ = 0;
   const void *fn = kernel_ptr(G, mode);
+  if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BLOCK, (size_t)lds));
   if (occ < 1) occ = 1;
   int64_t need_blocks = (P.ncells + groups - 1) / groups;
@@ -246,13 +253,18 @@ int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o
   return 0;
 }
 
-int finish_stats(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream_t stream, double ms,
-                 pfx_stats *stats) {
+// statistics reduction into t->d_stats (kernels only: capturable)
+int stats_launch(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream_t stream) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, t->d_stats, 64);
   int sblocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
   hipLaunchKernelGGL(k_stats, dim3(sblocks), dim3(256), 0, stream, P.num_iterations, P.status, P.rel_res, n,
                      t->d_stats);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int stats_read(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream_t stream, double ms,
+               pfx_stats *stats) {
   unsigned long long h[64];
   unsigned int nfb = 0;
   HIPCHK(hipMemcpyAsync(h, t->d_stats, sizeof(h), hipMemcpyDeviceToHost, stream));
@@ -274,6 +286,155 @@ int finish_stats(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream
     stats->fallbacks = P.guess_mode == PFX_GUESS_ARRAY_FALLBACK ? (int64_t)nfb : 0;
   }
   return (int)(n - conv);
+}
+
+int finish_stats(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream_t stream, double ms,
+                 pfx_stats *stats) {
+  int rc = stats_launch(t, P, n, stream);
+  if (rc) return rc;
+  return stats_read(t, P, n, stream, ms, stats);
+}
+
+__global__ void k_stamp(uint64_t *p) {
+  if (threadIdx.x == 0) *p = wall_clock64();
+}
+
+int use_scratch(pfx_tables *t, pfx_kernel_params &P, int64_t n);
+
+// ---------------------------------------------------------------------------
+// zoned launches
+struct zone_setup {
+  int nzones = 0;
+  pfx_kernel_params P[PFX_MAX_ZONES];
+  int64_t n[PFX_MAX_ZONES];
+  pfx_zone_launch Z;
+  int64_t lds = 0;
+  int total_blocks = 0;
+};
+
+int zone_lds(const pfx_tables *t) {
+  const int G = t->L.G;
+  return (int)(t->L.lds_table_bytes + (int64_t)(BLOCK / G) * pfx_scr_doubles(t->L.M, G, t->L.K) * 8);
+}
+
+int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, const pfx_react_opts *o,
+                  const pfx_zone_plan *plan, zone_setup &zs) {
+  if (nzones < 1 || nzones > PFX_MAX_ZONES || !tables || !cells || !o) return PFX_E_ARG;
+  if (o->max_iterations < 0 || o->extra_iterations < 0) return PFX_E_ARG;
+  const int dev = tables[0] ? tables[0]->device : 0;
+  for (int z = 0; z < nzones; z++) {
+    if (!tables[z] || tables[z]->device != dev) return PFX_E_ARG;
+    for (int y = 0; y < z; y++)
+      if (tables[y] == tables[z]) return PFX_E_ARG;  // queues and statistics are per tables
+    int rc = check_cells(tables[z], &cells[z], o);
+    if (rc) return rc;
+  }
+  HIPCHK(hipSetDevice(dev));
+  zs.nzones = nzones;
+  memset(&zs.Z, 0, sizeof(zs.Z));
+  zs.Z.nzones = nzones;
+  zs.lds = 0;
+  double w[PFX_MAX_ZONES], wsum = 0.0;
+  for (int z = 0; z < nzones; z++) {
+    pfx_tables *t = tables[z];
+    fill_params(t, &cells[z], o, zs.P[z]);
+    int rc = use_scratch(t, zs.P[z], cells[z].ncells);
+    if (rc) return rc;
+    zs.P[z].queue = t->d_queue;
+    zs.n[z] = cells[z].ncells;
+    zs.Z.G[z] = t->L.G;
+    zs.lds = std::max<int64_t>(zs.lds, zone_lds(t));
+    w[z] = (double)cells[z].ncells * t->L.N * t->L.N;
+    wsum += w[z];
+  }
+  if (zs.lds > kMaxLds) return PFX_E_UNSUP;
+  if (zs.lds > 64 * 1024)
+    HIPCHK(hipFuncSetAttribute((const void *)pfx_react_zones_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)zs.lds));
+  int occ = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)pfx_react_zones_kernel, BLOCK,
+                                                        (size_t)zs.lds));
+  if (occ < 1) occ = 1;
+  const int full = tables[0]->num_cu * occ;
+  int blocks[PFX_MAX_ZONES];
+  bool given = plan && plan->nzones == nzones;
+  for (int z = 0; given && z < nzones; z++) given = plan->blocks[z] > 0;
+  int total = 0;
+  for (int z = 0; z < nzones; z++) {
+    int b = given ? plan->blocks[z] : (int)(full * (wsum > 0 ? w[z] / wsum : 1.0 / nzones) + 0.5);
+    const int64_t need = (zs.n[z] + (BLOCK / zs.Z.G[z]) - 1) / (BLOCK / zs.Z.G[z]);
+    b = (int)std::max<int64_t>(1, std::min<int64_t>(b, std::max<int64_t>(need, 1)));
+    blocks[z] = b;
+    total += b;
+  }
+  int acc = 0;
+  for (int z = 0; z < nzones; z++) {
+    zs.Z.block_start[z] = acc;
+    acc += blocks[z];
+    // chunk: as in the single-batch launch, from the zone's own share
+    const int64_t per_wave = std::max<int64_t>(1, zs.n[z] / ((int64_t)blocks[z] * (BLOCK / 64)));
+    zs.P[z].chunk = (int32_t)std::max<int64_t>(64 / zs.Z.G[z], std::min<int64_t>(64, per_wave / 8 + 1));
+  }
+  zs.Z.block_start[nzones] = acc;
+  zs.total_blocks = total;
+  return 0;
+}
+
+// queue resets, start stamp, zoned kernel, statistics (all kernels)
+int zones_enqueue(pfx_tables *const *tables, zone_setup &zs, const pfx_kernel_params *d_params, uint64_t *d_finish,
+                  hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+  for (int z = 0; z < zs.nzones; z++)
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, tables[z]->d_queue, 2);
+  hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, d_finish + zs.total_blocks);
+  HIPCHK(hipGetLastError());
+  if (ev0) HIPCHK(hipEventRecord(ev0, s));
+  void *args[] = {(void *)&d_params, (void *)&zs.Z, (void *)&d_finish};
+  HIPCHK(hipLaunchKernel((const void *)pfx_react_zones_kernel, dim3((unsigned)zs.total_blocks), dim3(BLOCK), args,
+                         (size_t)zs.lds, s));
+  if (ev1) HIPCHK(hipEventRecord(ev1, s));
+  for (int z = 0; z < zs.nzones; z++) {
+    int rc = stats_launch(tables[z], zs.P[z], zs.n[z], s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int zones_finish(pfx_tables *const *tables, const zone_setup &zs, const uint64_t *d_finish, hipStream_t s,
+                 double ms, pfx_zone_plan *plan, pfx_stats *stats) {
+  int notconv = 0;
+  for (int z = 0; z < zs.nzones; z++) {
+    int rc = stats_read(tables[z], zs.P[z], zs.n[z], s, ms, stats ? &stats[z] : nullptr);
+    if (rc < 0) return rc;
+    notconv += rc;
+  }
+  if (plan) {
+    std::vector<uint64_t> fin(zs.total_blocks + 1);
+    HIPCHK(hipMemcpy(fin.data(), d_finish, fin.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    int rate_khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, tables[0]->device));
+    const uint64_t t0 = fin[zs.total_blocks];
+    plan->nzones = zs.nzones;
+    for (int z = 0; z < zs.nzones; z++) {
+      uint64_t last = t0;
+      for (int b = zs.Z.block_start[z]; b < zs.Z.block_start[z + 1]; b++) last = std::max(last, fin[b]);
+      plan->blocks[z] = zs.Z.block_start[z + 1] - zs.Z.block_start[z];
+      plan->zone_ms[z] = rate_khz > 0 ? (double)(last - t0) / rate_khz : 0.0;
+    }
+    plan->kernel_ms = ms;
+  }
+  return notconv;
+}
+
+int ensure_zone_buffers(pfx_tables *t, int blocks) {
+  if (!t->d_zone_params) HIPCHK(hipMalloc(&t->d_zone_params, PFX_MAX_ZONES * sizeof(pfx_kernel_params)));
+  if (blocks + 1 > t->finish_cap) {
+    if (t->d_finish) HIPCHK(hipFree(t->d_finish));
+    t->d_finish = nullptr;
+    t->finish_cap = 0;
+    HIPCHK(hipMalloc(&t->d_finish, (blocks + 1) * sizeof(uint64_t)));
+    t->finish_cap = blocks + 1;
+  }
+  return 0;
 }
 
 int use_scratch(pfx_tables *t, pfx_kernel_params &P, int64_t n) {
@@ -559,6 +720,10 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, t->device));
   t->num_cu = prop.multiProcessorCount;
+  if (lds > (int64_t)prop.sharedMemPerBlock) {
+    delete t;
+    return PFX_E_UNSUP;
+  }
   HIPCHK(hipMalloc(&t->d_blob, (size_t)L.blob_bytes));
   HIPCHK(hipMemcpy(t->d_blob, B.bytes.data(), (size_t)L.blob_bytes, hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&t->d_queue, 256));
@@ -575,6 +740,8 @@ extern "C" int pfx_tables_destroy(pfx_tables *t) {
   (void)hipFree(t->d_queue);
   (void)hipFree(t->d_stats);
   if (t->d_gi) (void)hipFree(t->d_gi);
+  if (t->d_zone_params) (void)hipFree(t->d_zone_params);
+  if (t->d_finish) (void)hipFree(t->d_finish);
   if (t->d_iters) (void)hipFree(t->d_iters);
   if (t->d_status) (void)hipFree(t->d_status);
   if (t->d_relres) (void)hipFree(t->d_relres);
@@ -636,7 +803,7 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
   int nph = 0;
   for (int j = 0; j < t->L.N; j++) {
     const int c = constraint_type[j];
-    if (c < PFX_CONSTRAINT_TOTAL || c > PFX_CONSTRAINT_PH) return PFX_E_UNSUP;
+    if (c < PFX_CONSTRAINT_TOTAL || c > PFX_CONSTRAINT_CHARGE) return PFX_E_UNSUP;
     nph += c == PFX_CONSTRAINT_PH;
   }
   if (nph > 1) return PFX_E_ARG;
@@ -647,6 +814,8 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
   fill_params(t, cells, o, P);
   P.constraint_conc = constraint_conc;
   for (int j = 0; j < t->L.N; j++) P.ctype[j] = constraint_type[j];
+  P.has_charge = 0;
+  for (int j = 0; j < t->L.N; j++) P.has_charge |= constraint_type[j] == PFX_CONSTRAINT_CHARGE;
   P.molal = molal;
   if (P.max_iterations <= 0) P.max_iterations = 10000;
   int rc = use_scratch(t, P, cells->ncells);
@@ -692,4 +861,129 @@ extern "C" int pfx_tables_timing(pfx_tables *t, double *ms) {
   ms[0] = t->last_ms[0];
   ms[1] = t->last_ms[1];
   return 0;
+}
+
+extern "C" int pfx_react_zones(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, const pfx_react_opts *o,
+                               pfx_zone_plan *plan, pfx_stats *stats) {
+  zone_setup zs;
+  int rc = zones_prepare(nzones, tables, cells, o, plan, zs);
+  if (rc) return rc;
+  if (stats)
+    for (int z = 0; z < nzones; z++) {
+      memset(&stats[z], 0, sizeof(pfx_stats));
+      stats[z].ncells = cells[z].ncells;
+    }
+  pfx_tables *lead = tables[0];
+  if ((rc = ensure_zone_buffers(lead, zs.total_blocks))) return rc;
+  hipStream_t s = (hipStream_t)o->stream;
+  HIPCHK(hipMemcpy(lead->d_zone_params, zs.P, nzones * sizeof(pfx_kernel_params), hipMemcpyHostToDevice));
+  if ((rc = zones_enqueue(tables, zs, lead->d_zone_params, lead->d_finish, s, lead->ev0, lead->ev1))) return rc;
+  HIPCHK(hipEventSynchronize(lead->ev1));
+  float f = 0.f;
+  HIPCHK(hipEventElapsedTime(&f, lead->ev0, lead->ev1));
+  lead->last_ms[0] = f;
+  lead->last_ms[1] = 0.0;
+  return zones_finish(tables, zs, lead->d_finish, s, f, plan, stats);
+}
+
+extern "C" int pfx_zone_plan_rebalance(pfx_zone_plan *plan) {
+  if (!plan || plan->nzones < 1 || plan->nzones > PFX_MAX_ZONES) return PFX_E_ARG;
+  double work[PFX_MAX_ZONES], sum = 0.0;
+  int total = 0;
+  for (int z = 0; z < plan->nzones; z++) {
+    if (plan->blocks[z] < 1) return PFX_E_ARG;
+    work[z] = plan->blocks[z] * std::max(plan->zone_ms[z], 1e-6);
+    sum += work[z];
+    total += plan->blocks[z];
+  }
+  int acc = 0;
+  for (int z = 0; z < plan->nzones; z++) {
+    int b = (int)(total * work[z] / sum + 0.5);
+    plan->blocks[z] = std::max(1, b);
+    acc += plan->blocks[z];
+  }
+  // keep the total: adjust the largest share
+  int zmax = 0;
+  for (int z = 1; z < plan->nzones; z++)
+    if (plan->blocks[z] > plan->blocks[zmax]) zmax = z;
+  plan->blocks[zmax] = std::max(1, plan->blocks[zmax] + total - acc);
+  return 0;
+}
+
+struct pfx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int nzones = 0;
+  pfx_tables *tables[PFX_MAX_ZONES];
+  zone_setup zs;
+  pfx_kernel_params *d_params = nullptr;
+  uint64_t *d_finish = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t last = nullptr;
+};
+
+extern "C" int pfx_graph_destroy(pfx_graph *g) {
+  if (!g) return PFX_E_ARG;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->d_params) (void)hipFree(g->d_params);
+  if (g->d_finish) (void)hipFree(g->d_finish);
+  if (g->ev0) (void)hipEventDestroy(g->ev0);
+  if (g->ev1) (void)hipEventDestroy(g->ev1);
+  delete g;
+  return 0;
+}
+
+extern "C" int pfx_zones_graph_create(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells,
+                                      const pfx_react_opts *o, const pfx_zone_plan *plan, pfx_graph **out) {
+  if (!out) return PFX_E_ARG;
+  *out = nullptr;
+  pfx_graph *g = new pfx_graph();
+  int rc = zones_prepare(nzones, tables, cells, o, plan, g->zs);
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  g->nzones = nzones;
+  for (int z = 0; z < nzones; z++) g->tables[z] = tables[z];
+  hipStream_t cs = nullptr;
+  bool ok = hipMalloc(&g->d_params, nzones * sizeof(pfx_kernel_params)) == hipSuccess &&
+            hipMalloc(&g->d_finish, (g->zs.total_blocks + 1) * sizeof(uint64_t)) == hipSuccess &&
+            hipEventCreate(&g->ev0) == hipSuccess && hipEventCreate(&g->ev1) == hipSuccess &&
+            hipMemcpy(g->d_params, g->zs.P, nzones * sizeof(pfx_kernel_params), hipMemcpyHostToDevice) ==
+                hipSuccess &&
+            hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) == hipSuccess;
+  if (ok) {
+    rc = zones_enqueue(tables, g->zs, g->d_params, g->d_finish, cs, nullptr, nullptr);
+    hipGraph_t graph = nullptr;
+    const bool ended = hipStreamEndCapture(cs, &graph) == hipSuccess;
+    g->graph = graph;
+    ok = rc == 0 && ended && graph && hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  }
+  if (cs) (void)hipStreamDestroy(cs);
+  if (!ok) {
+    pfx_graph_destroy(g);
+    return rc ? rc : PFX_E_DEVICE;
+  }
+  *out = g;
+  return 0;
+}
+
+extern "C" int pfx_graph_launch(pfx_graph *g, void *stream) {
+  if (!g || !g->exec) return PFX_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(hipEventRecord(g->ev0, s));
+  HIPCHK(hipGraphLaunch(g->exec, s));
+  HIPCHK(hipEventRecord(g->ev1, s));
+  g->last = s;
+  return 0;
+}
+
+extern "C" int pfx_graph_stats(pfx_graph *g, pfx_stats *stats, pfx_zone_plan *plan) {
+  if (!g) return PFX_E_ARG;
+  HIPCHK(hipEventSynchronize(g->ev1));
+  float f = 0.f;
+  HIPCHK(hipEventElapsedTime(&f, g->ev0, g->ev1));
+  return zones_finish(g->tables, g->zs, g->d_finish, g->last, f, plan, stats);
 }

@@ -56,6 +56,13 @@ struct pfx_dev_layout {
 
 enum { PFX_MODE_REACT = 0, PFX_MODE_EVAL = 1, PFX_MODE_EQUIL = 2, PFX_MODE_AUX = 3 };
 
+// block split of a zoned launch (pfx_react_zones)
+struct pfx_zone_launch {
+  int32_t nzones;
+  int32_t block_start[PFX_MAX_ZONES + 1];
+  int32_t G[PFX_MAX_ZONES];
+};
+
 struct pfx_kernel_params {
   pfx_dev_layout L;
   const unsigned char *tables;    // device blob
@@ -80,6 +87,7 @@ struct pfx_kernel_params {
   // constraint speciation
   const double *constraint_conc;   // [ncells*N]
   int32_t ctype[PFX_MAX_PRIMARY];  // PFX_CONSTRAINT_* per component
+  int32_t has_charge;              // some component carries the charge balance
   int32_t molal;                   // constraint concentrations in mol/kg
   int32_t aux_passes;              // activity passes of the aux-var refresh
   // work queue

@@ -202,20 +202,23 @@ __device__ __forceinline__ double pow_ratio(double x, double z) {
 //
 // Phases of a cell in REACT: 0 = RTotal at the previous state, 1 = activity
 // update pending, 2 = Newton.  Other modes start in phase 2.
+//
+// The parameters live in constant memory (the kernarg segment for a single
+// batch, a device buffer for a zoned launch) and are read through a pointer
+// that is made opaque once per pass of the persistent loop, so the ~100
+// uniform parameters are re-read through the scalar cache where they are
+// used instead of all being held in SGPRs across the whole loop.
+typedef const __attribute__((address_space(4))) pfx_kernel_params *KP;
+
 template <int G, int MODE>
-__global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_params Pk) {
-  // Parameters are read through a pointer that is made opaque once per pass
-  // of the persistent loop, so the ~100 uniform parameters are re-read from
-  // the kernarg segment (scalar cache) where they are used instead of all
-  // being held in SGPRs across the whole loop.
-  typedef const __attribute__((address_space(4))) pfx_kernel_params *KP;
-  uint64_t praw = (uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr();
-  const int N = Pk.L.N, M = Pk.L.M, K = Pk.L.K, R = Pk.L.R;
-  const int lds_tab = (int)Pk.L.lds_table_bytes;
+__device__ __forceinline__ void react_body(uint64_t praw) {
+  const KP P0 = (KP)praw;
+  const int N = P0->L.N, M = P0->L.M, K = P0->L.K, R = P0->L.R;
+  const int lds_tab = (int)P0->L.lds_table_bytes;
 
   // stage the reaction tables into LDS (16-byte words)
   {
-    const int4 *src = (const int4 *)Pk.tables;
+    const int4 *src = (const int4 *)P0->tables;
     int4 *dst = (int4 *)smem;
     const int nw = lds_tab >> 4;
     for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
@@ -230,8 +233,8 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
   const Scr S = make_scr(lds_tab + (int)(threadIdx.x / G) * sdoubles * 8, M, G, K);
   if (r == 0) SD(S.C, M) = 0.0;
 
-  const double Zr = live ? LD(Pk.L.o_Z, r) : 0.0;
-  const double a0r = live ? LD(Pk.L.o_a0, r) : 0.0;
+  const double Zr = live ? LD(P0->L.o_Z, r) : 0.0;
+  const double a0r = live ? LD(P0->L.o_a0, r) : 0.0;
 
   // wave-level work queue state (identical in all lanes of the wave)
   long long q_next = 0, q_end = 0;
@@ -324,6 +327,7 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
             else if (ctype == PFX_CONSTRAINT_FREE) free_conc = conc * molar_to_molal;
             else if (ctype == PFX_CONSTRAINT_LOG) free_conc = pow(10.0, conc) * molar_to_molal;
             else if (ctype == PFX_CONSTRAINT_PH) { free_conc = pow(10.0, -conc); tcon = conc; }
+            else if (ctype == PFX_CONSTRAINT_CHARGE) free_conc = conc * molar_to_molal;
             m = live ? free_conc : 1.0;
           } else {
             if (live) {
@@ -662,15 +666,35 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
     }
     if (MODE == PFX_MODE_EQUIL && active) {
       // constraint rows (reaction.F90:1597-1660)
+      if (P.has_charge) {
+        // charge balance (reaction.F90:1624-1651): R = sum_j z_j T_j and
+        // row sum_j z_j dT_j/dm_k, reduced over the group from the rows
+        // of dT/dm before any row is replaced
+        const double zc = live ? Zr : 0.0;
+        const double Rc = gsum<G>(zc * tot_out, lane);
+        double crow[G];
+#pragma unroll
+        for (int k = 0; k < G; k++) crow[k] = gsum<G>(zc * row[k], lane);
+        if (ctype == PFX_CONSTRAINT_CHARGE && live) {
+          Rr = Rc;
+#pragma unroll
+          for (int k = 0; k < G; k++) row[k] = crow[k];
+          // a vanishing counter-ion restarts at 1e-3 when the imbalance
+          // needs more of it (reaction.F90:1635-1650)
+          if (m < 1e-20 && ((Rr > 0.0 && Zr > 0.0) || (Rr < 0.0 && Zr < 0.0))) m = 1.0e-3;
+        }
+      }
       if (ctype == PFX_CONSTRAINT_TOTAL) {
         Rr = tot_out - tcon;
+      } else if (ctype == PFX_CONSTRAINT_CHARGE) {
+        // row set above
       } else {
         Rr = 0.0;
         if (ctype == PFX_CONSTRAINT_PH && live) m = pow(10.0, -tcon) / gam;
 #pragma unroll
         for (int k = 0; k < G; k++) row[k] = (k == r) ? 1.0 : 0.0;
       }
-      if (ctype == PFX_CONSTRAINT_PH) SD(S.mv, r) = m;
+      if (ctype == PFX_CONSTRAINT_PH || ctype == PFX_CONSTRAINT_CHARGE) SD(S.mv, r) = m;
       if (it > 3 && it < 9) use_log = use_log ? (it % 2 == 0) : 0;  // reaction.F90:1797-1804
       rr = gmax<G>(live ? fabs(Rr) : 0.0, lane);  // max |Res| (mol/L)
       if (!(rr == rr) || rr == INFINITY) {
@@ -860,6 +884,30 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
 #undef P
 #undef L
   }
+}
+
+template <int G, int MODE>
+__global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_params Pk) {
+  (void)Pk;
+  react_body<G, MODE>((uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr());
+}
+
+// Several chemistry zones in one persistent launch: workgroups
+// [block_start[z], block_start[z+1]) serve zone z, whose parameters (its own
+// tables, cells and queue) are zone_params[z]; each workgroup stamps the
+// wall clock when it retires, so the host can rebalance the split.
+__global__ __launch_bounds__(BLOCK) void pfx_react_zones_kernel(const pfx_kernel_params *zone_params,
+                                                                const pfx_zone_launch Z, uint64_t *finish) {
+  int z = 0;
+  while (z + 1 < Z.nzones && (int)blockIdx.x >= Z.block_start[z + 1]) z++;
+  const uint64_t praw = (uint64_t)(uintptr_t)(zone_params + z);
+  switch (Z.G[z]) {
+    case 8: react_body<8, PFX_MODE_REACT>(praw); break;
+    case 16: react_body<16, PFX_MODE_REACT>(praw); break;
+    default: react_body<32, PFX_MODE_REACT>(praw); break;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && finish) finish[blockIdx.x] = wall_clock64();
 }
 
 #define PFX_INST(G)                                                                 \

@@ -22,6 +22,8 @@ LIB_PATH = os.path.join(HERE, "libpfx_chem.so")
 EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_react_batch",
             "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch", "pfx_update_kinetic_state",
             "pfx_gi_structure", "pfx_gi_residual_jacobian", "pfx_tables_timing", "pfx_gi_dispersion",
+            "pfx_react_zones", "pfx_zone_plan_rebalance", "pfx_zones_graph_create", "pfx_graph_launch",
+            "pfx_graph_stats", "pfx_graph_destroy",
             # learned warm start (include/pfx_surrogate.h)
             "pfx_mlp_create", "pfx_mlp_destroy", "pfx_mlp_params", "pfx_mlp_forward", "pfx_mlp_gradients",
             "pfx_mlp_adam", "pfx_mlp_train_step", "pfx_surrogate_features", "pfx_surrogate_guess",
@@ -57,6 +59,14 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_auxvar_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_int32, P(abi.ReactOpts)]
     lib.pfx_update_kinetic_state.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts)]
     lib.pfx_tables_timing.argtypes = [C.c_void_p, C.c_void_p]
+    lib.pfx_react_zones.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, P(abi.ReactOpts), P(abi.ZonePlan),
+                                    C.c_void_p]
+    lib.pfx_zone_plan_rebalance.argtypes = [P(abi.ZonePlan)]
+    lib.pfx_zones_graph_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, P(abi.ReactOpts), P(abi.ZonePlan),
+                                           P(C.c_void_p)]
+    lib.pfx_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+    lib.pfx_graph_stats.argtypes = [C.c_void_p, C.c_void_p, P(abi.ZonePlan)]
+    lib.pfx_graph_destroy.argtypes = [C.c_void_p]
     lib.pfx_gi_structure.argtypes = [P(abi.Grid), C.c_void_p, C.c_void_p, P(C.c_int64)]
     lib.pfx_gi_dispersion.argtypes = [P(abi.Grid), C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -328,4 +338,80 @@ def gi_dispersion(grid: DeviceGrid, porosity, saturation=None, tortuosity=None, 
                                C.c_void_p(torch.cuda.current_stream().cuda_stream))
     if rc != 0:
         raise PfxError(f"pfx_gi_dispersion failed: {rc}")
+
+
+def _zone_arrays(tables, cells):
+    nz = len(tables)
+    if nz != len(cells) or not 1 <= nz <= abi.PFX_MAX_ZONES:
+        raise PfxError("one cell batch per zone, at most PFX_MAX_ZONES zones")
+    handles = (C.c_void_p * nz)(*[t.handle.value for t in tables])
+    structs = (abi.Cells * nz)(*[c.struct() for c in cells])
+    return nz, handles, structs
+
+
+def react_zones(tables, cells, opts: abi.ReactOpts, plan: Optional[abi.ZonePlan] = None):
+    """pfx_react_zones: one persistent launch over several chemistry zones.
+    Returns (rc, [stats per zone], plan)."""
+    lib = load_library()
+    torch = _torch()
+    nz, handles, structs = _zone_arrays(tables, cells)
+    stats = (abi.Stats * nz)()
+    if plan is None:
+        plan = abi.ZonePlan()
+        plan.nzones = nz
+    if opts.stream is None:
+        opts.stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.pfx_react_zones(nz, handles, structs, C.byref(opts), C.byref(plan), stats)
+    if rc < 0:
+        raise PfxError(f"pfx_react_zones failed: {rc}")
+    return rc, list(stats), plan
+
+
+def rebalance(plan: abi.ZonePlan):
+    rc = load_library().pfx_zone_plan_rebalance(C.byref(plan))
+    if rc != 0:
+        raise PfxError(f"pfx_zone_plan_rebalance failed: {rc}")
+    return plan
+
+
+class ZonesGraph:
+    """pfx_zones_graph_create / launch / stats: a captured zoned step."""
+
+    def __init__(self, tables, cells, opts: abi.ReactOpts, plan: Optional[abi.ZonePlan] = None):
+        lib = load_library()
+        _torch()
+        nz, handles, structs = _zone_arrays(tables, cells)
+        self.nz = nz
+        self._keep = (tables, cells, structs)
+        h = C.c_void_p()
+        rc = lib.pfx_zones_graph_create(nz, handles, structs, C.byref(opts), C.byref(plan) if plan else None,
+                                        C.byref(h))
+        if rc != 0:
+            raise PfxError(f"pfx_zones_graph_create failed: {rc}")
+        self.handle = h
+
+    def launch(self):
+        import torch
+        rc = load_library().pfx_graph_launch(self.handle, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if rc != 0:
+            raise PfxError(f"pfx_graph_launch failed: {rc}")
+
+    def stats(self):
+        stats = (abi.Stats * self.nz)()
+        plan = abi.ZonePlan()
+        rc = load_library().pfx_graph_stats(self.handle, stats, C.byref(plan))
+        if rc < 0:
+            raise PfxError(f"pfx_graph_stats failed: {rc}")
+        return rc, list(stats), plan
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load_library().pfx_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 

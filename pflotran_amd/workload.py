@@ -96,3 +96,84 @@ def gi_grid(nx: int, ny: int, nz: int, seed: int = synth.SEED, workload_id: int 
     return dict(nx=nx, ny=ny, nz=nz, q=q, dd=dda, area=(dx * dx,) * 3,
                 bc_q=(q_row.copy(), -q_row.copy()), bc_dd=(np.full((nz, ny), 2 * dd), np.zeros((nz, ny))),
                 bc_total=(bc_total, None), log_formulation=log_formulation)
+
+
+# ---------------------------------------------------------------------------
+# uneven batch: chemistry zones, half the cells near a mineral's saturation
+ZONE_N = (8, 12, 20, 30)
+ZONE_CELLS = (1_000_000, 500_000, 300_000, 200_000)
+
+
+def zone_networks(seed: int = synth.SEED):
+    """One network per zone: N primaries, 3N complexes, 3 kinetic minerals."""
+    return [synth.random_network(N, 3 * N, 3, ionx=False, seed=seed + 101 * N) for N in ZONE_N]
+
+
+def mineral_log_omega(t: ReactionTables, dc: runtime.DeviceCells, k: int = 0):
+    """log10 saturation index of mineral k at the cells' state (host)."""
+    import numpy as np
+
+    from .database import logk_eval
+    m = dc.t["pri_molal"].cpu().numpy()
+    g = dc.t["pri_act_coef"].cpu().numpy()
+    temp = dc.t["temperature"].cpu().numpy() if dc.t.get("temperature") is not None else np.full(len(m), 25.0)
+    la = np.log10(m * g)
+    ns = int(t.mnrl_specid[k, 0])
+    lq = np.zeros(len(m))
+    for s_ in range(ns):
+        lq += t.mnrl_stoich[k, s_] * la[:, t.mnrl_specid[k, 1 + s_]]
+    if t.mnrl_logKcoef is not None:
+        lk = logk_eval(t.mnrl_logKcoef[k][None, :], temp)
+    else:
+        lk = np.full(len(m), t.mnrl_logK[k])
+    return lq - lk
+
+
+def near_saturation_cells(t: ReactionTables, tab: runtime.Tables, n: int, seed: int, workload_id: int,
+                          fraction: float = 0.5, window: float = 0.05, passes: int = 3, oversample: float = 1.5):
+    """n speciated cells; the first `fraction` of them moved toward
+    |log Omega| <= window for mineral 0 by rescaling the total of its
+    dominant non-H+ primary and re-speciating (a few passes).  Returns the
+    cells, the final log Omega and the mask of cells inside the window."""
+    import numpy as np
+    import torch
+    n_draw = int(n * oversample) + 64
+    ci = synth.random_cells(t, n_draw, seed=seed, workload_id=workload_id)
+    arr = synth.host_state_arrays(t, n_draw, with_temperature=True)
+    arr["temperature"][:] = ci.temp
+    arr["mnrl_volfrac"][:] = ci.vf
+    arr["mnrl_area"][:] = ci.area
+    conc = ci.totals.copy()
+    conc[:, 0] = ci.ph
+    ctype = [abi.PFX_CONSTRAINT_PH] + [abi.PFX_CONSTRAINT_TOTAL] * (t.N - 1)
+    # dominant non-H+ primary of mineral 0 and its coefficient
+    ns = int(t.mnrl_specid[0, 0])
+    cand = [(abs(t.mnrl_stoich[0, s_]), int(t.mnrl_specid[0, 1 + s_]), t.mnrl_stoich[0, s_]) for s_ in range(ns)
+            if int(t.mnrl_specid[0, 1 + s_]) != 0]
+    _, jp, nu = max(cand)
+    near = np.zeros(n_draw, dtype=bool)
+    near[:int(fraction * n_draw)] = True
+    target = synth.rng(workload_id, "perturb", seed).uniform(-window, window, size=n_draw)
+    dc = None
+    for it in range(passes + 1):
+        dc = runtime.DeviceCells.from_host(arr, n_draw)
+        runtime.equilibrate(tab, dc, ctype, conc, synth.react_opts(max_iterations=300))
+        if it == passes:
+            break
+        lo = mineral_log_omega(t, dc)
+        shift = np.where(near & np.isfinite(lo), (target - lo) / nu, 0.0)
+        conc[:, jp] *= 10.0 ** np.clip(shift, -2.0, 2.0)
+        conc[:, jp] = np.clip(conc[:, jp], 1e-12, 1.0)
+    m = dc.t["pri_molal"]
+    ok = (dc.t["status"] == 0) & torch.isfinite(m).all(dim=1) & (m > 0).all(dim=1)
+    idx = torch.nonzero(ok).flatten()[:n]
+    if idx.numel() < n:
+        raise runtime.PfxError(f"only {idx.numel()} of {n} cells speciated")
+    dc = subset(dc, idx)
+    dc.t["pri_act_coef"].fill_(1.0)
+    dc.t["sec_act_coef"].fill_(1.0)
+    dc.t["ionx_ref_sorbed"].fill_(1e-9)
+    runtime.auxvar(tab, dc, 2, synth.react_opts())
+    lo = mineral_log_omega(t, dc)
+    inside = near[idx.cpu().numpy()] & (np.abs(lo) <= window)
+    return dc, lo, inside

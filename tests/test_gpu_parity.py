@@ -204,3 +204,20 @@ def test_update_kinetic_state_parity(net15):
     # the change itself (rate * molar volume * dt) to 1e-9 of its size, plus
     # the rounding of vf0 + change
     assert np.all(np.abs(d_g - d_o) <= 1e-9 * np.abs(d_o) + 4e-16 * vf0)
+
+
+def test_gpu_gold_valocchi_charge_balance(golds):
+    """ion-exchange-valocchi: Na+/Ca++/Mg++ totals with Cl- by charge balance
+    (constraint speciation on the GPU), then the sorbed totals of the
+    exchanger at the solution."""
+    from chemdefs import valocchi
+    name = "ion-exchange-valocchi"
+    t, setup = valocchi()
+    tab, dc = _gpu_speciate(t, setup)
+    out = dc.to_host()
+    for j, sp in enumerate(["Na+", "Ca++", "Mg++", "Cl-"]):
+        assert abs(out["total"][0][j] - gold(golds, name, f"CONCENTRATION: Total {sp}")) < 1e-12
+        g_s = gold(golds, name, f"CONCENTRATION: Total Sorbed {sp}")
+        assert abs(out["total_sorb"][0][j] - g_s) <= 1e-12 * max(abs(g_s), 1.0)
+    # charge balance holds at the solution: sum z_j T_j = 0
+    assert abs(np.dot(t.Z, out["total"][0])) < 1e-11
