@@ -114,6 +114,10 @@ def main():
         plain_step()
     plain_ms = timed_steps(torch, args.steps, plain_step)
     st_plain = box["st"]
+    # free ions of the plain method on the cells the CPU baseline will solve
+    n_cpu = min(args.cpu_sample or 12000 * min(os.cpu_count() or 1, 16), n)
+    plain_m = dc.t["pri_molal"][:n_cpu].cpu().numpy().copy()
+    plain_status = dc.t["status"][:n_cpu].cpu().numpy().copy()
     plain_kernel_ms = st_plain.kernel_ms
     plain_ms_step = float(np.mean(plain_ms))
     plain_value = st_plain.converged / (plain_ms_step * 1e-3)
@@ -210,7 +214,19 @@ def main():
     iters = st_main.total_iterations
     flops = (iters + n) * (fl["flops"] + fl["transcendentals"])
     achieved = flops / (kern_ms * 1e-3) / 1e12
-    bytes_per_cell = 8 * (4 * t.N + 2 * t.M + 3 * t.K + t.R + 2) + 4 * 2
+    # algorithmic bytes per cell: read totals, sorbed totals, free ions,
+    # guess, both activity-coefficient arrays, vf, area, T and the exchange
+    # state once; write free ions, totals, sorbed totals, coefficients,
+    # complexes, rates, exchange state, rel_res, iterations and status once
+    bytes_per_cell = 8 * ((5 * t.N + t.M + 2 * t.K + 1 + t.R) + (4 * t.N + 2 * t.M + t.K + t.R + 1)) + 8
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic_pfx_react_kernel_1M.json")) as f:
+            prof = json.load(f)["pfx_react_kernel<16,0>"]
+        if n == 1_000_000 and warm is not None:
+            traffic = prof["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        traffic = None
     line.update({
         "value": value,
         "ms_per_step": ms_per_step,
@@ -223,7 +239,9 @@ def main():
         "plain": plain,
         "warm": warm,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_source": "profiles/pmc_traffic_pfx_react_kernel_1M.json (FETCH_SIZE x2.0, "
+                                       "WRITE_SIZE x1.0 from a 256 MiB copy)" if traffic else None,
                      "kernel": "pfx_react_kernel<16,0>", "kernel_ms": kern_ms,
                      "flops_per_iteration": fl["flops"], "transcendentals_per_iteration": fl["transcendentals"],
                      "note": "FP64 VALU-bound Newton; MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s",
@@ -244,7 +262,7 @@ def main():
         torch.cuda.synchronize()
         line["copy_calibration_bytes"] = x.numel() * 8
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(t, dc, snap, gpu_m, gpu_status, args.cpu_sample)
+        line["cpu_baseline"] = cpu_baseline(t, dc, snap, gpu_m, gpu_status, plain_m, plain_status, n_cpu)
     print(json.dumps(line), flush=True)
 
 
@@ -343,16 +361,24 @@ def zones_leg(args, log):
             runtime.react_batch(tab, dc, opts)
         torch.cuda.synchronize()
         sep.append((time.perf_counter() - t1) * 1e3)
-    # zoned launch: automatic split, then rebalanced from the zone times
-    plan = None
+    # zoned launch, autotuned: for each kernel variant (registers as needed /
+    # capped at two wavefronts per SIMD) the split starts from cells * N^2
+    # and is rebalanced from the measured zone times; the fastest is kept
     history = []
-    for _ in range(4):
-        restore()
-        rc, st, plan = runtime.react_zones(tabs, dcs, opts, plan)
-        history.append({"blocks": list(plan.blocks[:4]), "zone_ms": list(plan.zone_ms[:4]),
-                        "kernel_ms": plan.kernel_ms})
-        runtime.rebalance(plan)
+    for variant in (0, 1):
+        plan = abi.ZonePlan()
+        plan.nzones = 4
+        plan.variant = variant
+        for _ in range(3):
+            restore()
+            rc, st, plan = runtime.react_zones(tabs, dcs, opts, plan)
+            history.append({"variant": variant, "blocks": list(plan.blocks[:4]),
+                            "zone_ms": list(plan.zone_ms[:4]), "kernel_ms": plan.kernel_ms})
+            runtime.rebalance(plan)
     best = min(history, key=lambda h: h["kernel_ms"])
+    plan = abi.ZonePlan()
+    plan.nzones = 4
+    plan.variant = best["variant"]
     plan.blocks[:4] = best["blocks"]
     graph = runtime.ZonesGraph(tabs, dcs, opts, plan)
     g_ms, k_ms = [], []
@@ -375,6 +401,7 @@ def zones_leg(args, log):
            "graph_ms_per_step": float(np.mean(g_ms)), "graph_kernel_ms": float(np.mean(k_ms)),
            "cells_per_s": conv / (float(np.mean(g_ms)) * 1e-3),
            "separate_launches_ms": float(np.mean(sep)), "autotune_history": history, "blocks": best["blocks"],
+           "variant": best["variant"],
            "mean_iterations_per_zone": its, "iteration_histogram": hist.tolist(),
            "converged": int(conv)}
     log(f"zones: graph {out['graph_ms_per_step']:.2f} ms/step (kernel {out['graph_kernel_ms']:.2f}), "
@@ -384,11 +411,13 @@ def zones_leg(args, log):
     return out
 
 
-def cpu_baseline(t, dc, snap, gpu_m, gpu_status, sample):
+def cpu_baseline(t, dc, snap, gpu_m, gpu_status, plain_m, plain_status, n):
     """The oracle (C restatement of RReact, OpenMP over cells) on the first
-    cells of the same held-out batch (their step inputs copied to the host),
-    from the reference's default guess, timed on the host cores; its free
-    ions are compared with the GPU's final ones."""
+    n cells of the same held-out batch (their step inputs copied to the
+    host), from the reference's default guess, timed on the host cores.  Its
+    free ions are compared with the GPU plain method's (the same algorithm
+    from the same guess: parity) and with the GPU warm-start method's (a
+    different path to the same 1e-10 tolerance)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
@@ -396,7 +425,6 @@ def cpu_baseline(t, dc, snap, gpu_m, gpu_status, sample):
     from pflotran_amd import synth
 
     threads = min(os.cpu_count() or 1, 16)
-    n = min(sample or 12000 * threads, dc.ncells)
     cells = orc.HostCells.allocate(t, n, temperature=True)
     for k, v in cells.arrays.items():
         src = snap.get(k, dc.t.get(k))
@@ -405,13 +433,19 @@ def cpu_baseline(t, dc, snap, gpu_m, gpu_status, sample):
     t0 = time.perf_counter()
     rc, st = orc.react_batch(t, cells, synth.react_opts(), nthreads=threads)
     dt = time.perf_counter() - t0
-    m_gpu = gpu_m[:n].cpu().numpy()
-    ok = (cells["status"] == 0) & (gpu_status[:n].cpu().numpy() == 0)
-    err = float(np.max(np.abs(m_gpu[ok] - cells["pri_molal"][ok]) / cells["pri_molal"][ok])) if ok.any() else None
+    def compare(m_gpu, s_gpu):
+        ok = (cells["status"] == 0) & (s_gpu == 0)
+        if not ok.any():
+            return None
+        rel = np.max(np.abs(m_gpu[ok] - cells["pri_molal"][ok]) / cells["pri_molal"][ok], axis=1)
+        return {"max": float(rel.max()), "p99": float(np.quantile(rel, 0.99)), "median": float(np.median(rel)),
+                "cells": int(ok.sum())}
+
     return {"value": st.converged / dt, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"first {n} cells of the held-out batch, same tables/tolerance, default guess; "
                       f"{dt:.2f} s, {st.total_iterations} Newton iterations",
-            "gpu_vs_oracle_max_rel_err_free_ions": err, "compared_cells": int(ok.sum())}
+            "free_ion_rel_err_plain_gpu_vs_oracle": compare(plain_m, plain_status),
+            "free_ion_rel_err_warm_gpu_vs_oracle": compare(gpu_m[:n].cpu().numpy(), gpu_status[:n].cpu().numpy())}
 
 
 if __name__ == "__main__":

@@ -333,6 +333,10 @@ typedef struct pfx_zone_plan {
                                       cells * N^2); Out: the split used       */
   double zone_ms[PFX_MAX_ZONES];   /* Out: when the zone's last workgroup retired */
   double kernel_ms;                /* Out: the launch (HIP events)            */
+  int32_t variant;                 /* In/Out: 0 registers as needed (1 wavefront
+                                      per SIMD with 32-lane zones), 1 capped at
+                                      2 wavefronts per SIMD (spills)          */
+  int32_t reserved;
 } pfx_zone_plan;
 
 /* pfx_react_batch for nzones (tables[z], cells[z]) pairs in one launch;

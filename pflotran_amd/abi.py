@@ -101,7 +101,7 @@ class Grid(C.Structure):
 class ZonePlan(C.Structure):
     _fields_ = [
         ("nzones", C.c_int32), ("blocks", C.c_int32 * PFX_MAX_ZONES), ("zone_ms", C.c_double * PFX_MAX_ZONES),
-        ("kernel_ms", C.c_double),
+        ("kernel_ms", C.c_double), ("variant", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

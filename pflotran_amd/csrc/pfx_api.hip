@@ -18,6 +18,8 @@ template <int G, int MODE>
 __global__ void pfx_react_kernel(const pfx_kernel_params P);
 __global__ void pfx_react_zones_kernel(const pfx_kernel_params *zone_params, const pfx_zone_launch Z,
                                        uint64_t *finish);
+__global__ void pfx_react_zones_kernel_w2(const pfx_kernel_params *zone_params, const pfx_zone_launch Z,
+                                          uint64_t *finish);
 extern "C" int pfx_scr_doubles(int M, int G, int K);
 
 struct pfx_tables {
@@ -305,6 +307,8 @@ int use_scratch(pfx_tables *t, pfx_kernel_params &P, int64_t n);
 // zoned launches
 struct zone_setup {
   int nzones = 0;
+  int variant = 0;
+  const void *fn = nullptr;
   pfx_kernel_params P[PFX_MAX_ZONES];
   int64_t n[PFX_MAX_ZONES];
   pfx_zone_launch Z;
@@ -348,12 +352,11 @@ int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, c
     wsum += w[z];
   }
   if (zs.lds > kMaxLds) return PFX_E_UNSUP;
-  if (zs.lds > 64 * 1024)
-    HIPCHK(hipFuncSetAttribute((const void *)pfx_react_zones_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)zs.lds));
+  zs.variant = plan && plan->variant == 1 ? 1 : 0;
+  zs.fn = zs.variant ? (const void *)pfx_react_zones_kernel_w2 : (const void *)pfx_react_zones_kernel;
+  if (zs.lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(zs.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)zs.lds));
   int occ = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)pfx_react_zones_kernel, BLOCK,
-                                                        (size_t)zs.lds));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, zs.fn, BLOCK, (size_t)zs.lds));
   if (occ < 1) occ = 1;
   const int full = tables[0]->num_cu * occ;
   int blocks[PFX_MAX_ZONES];
@@ -389,8 +392,7 @@ int zones_enqueue(pfx_tables *const *tables, zone_setup &zs, const pfx_kernel_pa
   HIPCHK(hipGetLastError());
   if (ev0) HIPCHK(hipEventRecord(ev0, s));
   void *args[] = {(void *)&d_params, (void *)&zs.Z, (void *)&d_finish};
-  HIPCHK(hipLaunchKernel((const void *)pfx_react_zones_kernel, dim3((unsigned)zs.total_blocks), dim3(BLOCK), args,
-                         (size_t)zs.lds, s));
+  HIPCHK(hipLaunchKernel(zs.fn, dim3((unsigned)zs.total_blocks), dim3(BLOCK), args, (size_t)zs.lds, s));
   if (ev1) HIPCHK(hipEventRecord(ev1, s));
   for (int z = 0; z < zs.nzones; z++) {
     int rc = stats_launch(tables[z], zs.P[z], zs.n[z], s);
@@ -421,6 +423,7 @@ int zones_finish(pfx_tables *const *tables, const zone_setup &zs, const uint64_t
       plan->zone_ms[z] = rate_khz > 0 ? (double)(last - t0) / rate_khz : 0.0;
     }
     plan->kernel_ms = ms;
+    plan->variant = zs.variant;
   }
   return notconv;
 }

@@ -209,17 +209,21 @@ __global__ void k_mse_grad(const float *Y, const float *T, float *dY, float *los
   if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
 }
 
-// db[o] = sum_b dZ[b][o]; one block per 64 columns, rows split over waves
+// db[o] += sum over a slice of rows of dZ[b][o]: blockIdx.x covers 64
+// columns, blockIdx.y a slice of rows (db zeroed beforehand)
+constexpr int COLSUM_ROWS = 128;
 __global__ __launch_bounds__(256) void k_colsum(const float *dZ, int B, int N, float *db) {
   __shared__ float red[4][64];
   const int o = blockIdx.x * 64 + (threadIdx.x & 63);
   const int w = threadIdx.x >> 6;
+  const int b0 = blockIdx.y * COLSUM_ROWS, b1 = min(B, b0 + COLSUM_ROWS);
   float s = 0.f;
   if (o < N)
-    for (int b = w; b < B; b += 4) s += dZ[(long)b * N + o];
+    for (int b = b0 + w; b < b1; b += 4) s += dZ[(long)b * N + o];
   red[w][threadIdx.x & 63] = s;
   __syncthreads();
-  if (w == 0 && o < N) db[o] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+  if (w == 0 && o < N)
+    atomicAdd(&db[o], red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 __global__ void k_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1, float b2, float c1,
@@ -628,7 +632,9 @@ extern "C" int pfx_mlp_gradients(pfx_mlp *m, const float *X, const float *T, int
     if ((rc = launch_gemm(EPI_ATOMIC, N, K, B, m->dZ, 1, N, m->A[l], 1, K, m->gW[l], K, nullptr, nullptr, ksplit,
                           s)))
       return rc;
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, m->dZ, B, N, m->gb[l]);
+    if ((rc = fill(m->gb[l], N, 0.f, s))) return rc;
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((N + 63) / 64), (unsigned)((B + COLSUM_ROWS - 1) / COLSUM_ROWS)),
+                       dim3(256), 0, s, m->dZ, B, N, m->gb[l]);
     MCHK(hipGetLastError());
     if (l > 0) {
       // dA[b][k] = sum_n dZ[b][n] W[n][k]; dZ of layer l-1 = dA * gelu'(Z)

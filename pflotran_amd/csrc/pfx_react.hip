@@ -733,15 +733,13 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         if (live && !(mx > 0.0)) status = PFX_CELL_SINGULAR;
       }
       bool pivoted = !live;
-      int plane[G];
+      int mystep = -1;  // elimination step at which this lane's row was the pivot
 #pragma unroll
       for (int j = 0; j < G; j++) {
-        plane[j] = 0;
         if (j < N) {
           double cand = pivoted ? -1.0 : vv * fabs(row[j]);
           int p = r;
           gargmax<G>(cand, p, lane);
-          plane[j] = p;
           const int src = gbase + p;
           // stream the pivot row one element at a time
           double piv = bperm_d(row[j], src);
@@ -750,6 +748,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           const double f = elim ? row[j] / piv : 0.0;  // 0 leaves the row unchanged below
           if (r == p) {
             pivoted = true;
+            mystep = j;
             row[j] = piv;
           } else if (elim) {
             row[j] = f;
@@ -763,14 +762,13 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           b -= f * pb;
         }
       }
-      // back substitution: the pivot row of step j is held by lane plane[j];
-      // rows pivoted before step j still need x_j, later ones are finished
+      // back substitution: the pivot row of step j is held by the lane whose
+      // mystep is j, the only non-zero term of the group sum that spreads
+      // x_j; rows pivoted before step j still need x_j, later ones are done
 #pragma unroll
       for (int j = G - 1; j >= 0; j--) {
         if (j < N) {
-          const int pl = plane[j];
-          double xj = (r == pl) ? b / row[j] : 0.0;
-          xj = bperm_d(xj, gbase + pl);
+          const double xj = gsum<G>(mystep == j ? b / row[j] : 0.0, lane);
           if (r == j) upd = xj;
           b -= row[j] * xj;
         }
@@ -896,8 +894,8 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_param
 // [block_start[z], block_start[z+1]) serve zone z, whose parameters (its own
 // tables, cells and queue) are zone_params[z]; each workgroup stamps the
 // wall clock when it retires, so the host can rebalance the split.
-__global__ __launch_bounds__(BLOCK) void pfx_react_zones_kernel(const pfx_kernel_params *zone_params,
-                                                                const pfx_zone_launch Z, uint64_t *finish) {
+__device__ __forceinline__ void zones_body(const pfx_kernel_params *zone_params, const pfx_zone_launch &Z,
+                                           uint64_t *finish) {
   int z = 0;
   while (z + 1 < Z.nzones && (int)blockIdx.x >= Z.block_start[z + 1]) z++;
   const uint64_t praw = (uint64_t)(uintptr_t)(zone_params + z);
@@ -908,6 +906,19 @@ __global__ __launch_bounds__(BLOCK) void pfx_react_zones_kernel(const pfx_kernel
   }
   __syncthreads();
   if (threadIdx.x == 0 && finish) finish[blockIdx.x] = wall_clock64();
+}
+
+// variant 0: registers as the 32-lane path needs them (one wavefront per
+// SIMD when such a zone is present); variant 1: capped at two wavefronts per
+// SIMD, the 32-lane path spilling what does not fit
+__global__ __launch_bounds__(BLOCK) void pfx_react_zones_kernel(const pfx_kernel_params *zone_params,
+                                                                const pfx_zone_launch Z, uint64_t *finish) {
+  zones_body(zone_params, Z, finish);
+}
+
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void pfx_react_zones_kernel_w2(
+    const pfx_kernel_params *zone_params, const pfx_zone_launch Z, uint64_t *finish) {
+  zones_body(zone_params, Z, finish);
 }
 
 #define PFX_INST(G)                                                                 \

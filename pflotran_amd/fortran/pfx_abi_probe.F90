@@ -10,6 +10,7 @@ program pfx_abi_probe
   type(pfx_react_opts) :: opts
   type(pfx_stats) :: stats
   type(pfx_grid) :: grid
+  type(pfx_zone_plan) :: plan
   type(c_ptr) :: tables
   integer(c_int) :: rc
 
@@ -26,4 +27,5 @@ program pfx_abi_probe
   print '(a,i0)', 'sizeof_opts ', c_sizeof(opts)
   print '(a,i0)', 'sizeof_stats ', c_sizeof(stats)
   print '(a,i0)', 'sizeof_grid ', c_sizeof(grid)
+  print '(a,i0)', 'sizeof_plan ', c_sizeof(plan)
 end program pfx_abi_probe

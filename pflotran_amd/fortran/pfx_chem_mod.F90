@@ -74,6 +74,14 @@ module pfx_chem_module
     type(c_ptr) :: bc_q(2), bc_dd(2), bc_total(2)
   end type pfx_grid
 
+  type, bind(c), public :: pfx_zone_plan
+    integer(c_int32_t) :: nzones
+    integer(c_int32_t) :: blocks(8)
+    real(c_double) :: zone_ms(8)
+    real(c_double) :: kernel_ms
+    integer(c_int32_t) :: variant, reserved
+  end type pfx_zone_plan
+
   type, bind(c), public :: pfx_stats
     integer(c_int64_t) :: ncells, converged, total_iterations
     integer(c_int32_t) :: max_iterations, min_iterations
@@ -85,7 +93,9 @@ module pfx_chem_module
   public :: pfx_abi_version, pfx_tables_create, pfx_tables_destroy, &
             pfx_react_batch, pfx_residual_jacobian, pfx_equilibrate_batch, &
             pfx_auxvar_batch, pfx_update_kinetic_state, pfx_tables_timing, &
-            pfx_gi_structure, pfx_gi_residual_jacobian
+            pfx_gi_structure, pfx_gi_residual_jacobian, pfx_gi_dispersion, &
+            pfx_react_zones, pfx_zone_plan_rebalance, pfx_zones_graph_create, &
+            pfx_graph_launch, pfx_graph_stats, pfx_graph_destroy
   ! learned warm start (include/pfx_surrogate.h)
   public :: pfx_mlp_create, pfx_mlp_destroy, pfx_mlp_params, pfx_mlp_forward, &
             pfx_mlp_gradients, pfx_mlp_adam, pfx_mlp_train_step, &
@@ -179,6 +189,69 @@ module pfx_chem_module
       type(c_ptr), value :: row_ptr, residual, values
       integer(c_int) :: pfx_gi_residual_jacobian
     end function pfx_gi_residual_jacobian
+
+    function pfx_gi_dispersion(grid, porosity, saturation, tortuosity, dispersivity, &
+                               diffusion, dx, bc_type, dd_out, bc_dd_out, stream) &
+        bind(c, name='pfx_gi_dispersion')
+      import :: c_int, c_int32_t, c_double, c_ptr, pfx_grid
+      type(pfx_grid), intent(in) :: grid
+      type(c_ptr), value :: porosity, saturation, tortuosity
+      real(c_double), value :: dispersivity, diffusion
+      real(c_double), intent(in) :: dx(3)
+      integer(c_int32_t), intent(in) :: bc_type(2)
+      type(c_ptr), intent(in) :: dd_out(3), bc_dd_out(2)
+      type(c_ptr), value :: stream
+      integer(c_int) :: pfx_gi_dispersion
+    end function pfx_gi_dispersion
+
+    function pfx_react_zones(nzones, tables, cells, opts, plan, stats) bind(c, name='pfx_react_zones')
+      import :: c_int, c_int32_t, c_ptr, pfx_cells, pfx_react_opts, pfx_zone_plan, pfx_stats
+      integer(c_int32_t), value :: nzones
+      type(c_ptr), intent(in) :: tables(*)
+      type(pfx_cells), intent(inout) :: cells(*)
+      type(pfx_react_opts), intent(in) :: opts
+      type(pfx_zone_plan), intent(inout) :: plan
+      type(pfx_stats), intent(out) :: stats(*)
+      integer(c_int) :: pfx_react_zones
+    end function pfx_react_zones
+
+    function pfx_zone_plan_rebalance(plan) bind(c, name='pfx_zone_plan_rebalance')
+      import :: c_int, pfx_zone_plan
+      type(pfx_zone_plan), intent(inout) :: plan
+      integer(c_int) :: pfx_zone_plan_rebalance
+    end function pfx_zone_plan_rebalance
+
+    function pfx_zones_graph_create(nzones, tables, cells, opts, plan, graph) &
+        bind(c, name='pfx_zones_graph_create')
+      import :: c_int, c_int32_t, c_ptr, pfx_cells, pfx_react_opts, pfx_zone_plan
+      integer(c_int32_t), value :: nzones
+      type(c_ptr), intent(in) :: tables(*)
+      type(pfx_cells), intent(inout) :: cells(*)
+      type(pfx_react_opts), intent(in) :: opts
+      type(pfx_zone_plan), intent(in) :: plan
+      type(c_ptr), intent(out) :: graph
+      integer(c_int) :: pfx_zones_graph_create
+    end function pfx_zones_graph_create
+
+    function pfx_graph_launch(graph, stream) bind(c, name='pfx_graph_launch')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: graph, stream
+      integer(c_int) :: pfx_graph_launch
+    end function pfx_graph_launch
+
+    function pfx_graph_stats(graph, stats, plan) bind(c, name='pfx_graph_stats')
+      import :: c_int, c_ptr, pfx_stats, pfx_zone_plan
+      type(c_ptr), value :: graph
+      type(pfx_stats), intent(out) :: stats(*)
+      type(pfx_zone_plan), intent(out) :: plan
+      integer(c_int) :: pfx_graph_stats
+    end function pfx_graph_stats
+
+    function pfx_graph_destroy(graph) bind(c, name='pfx_graph_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: graph
+      integer(c_int) :: pfx_graph_destroy
+    end function pfx_graph_destroy
 
     function pfx_update_kinetic_state(tables, cells, opts) &
         bind(c, name='pfx_update_kinetic_state')

@@ -52,7 +52,7 @@ def test_every_header_compiles_as_c(tmp_path):
 
 @pytest.mark.parametrize("struct,cname", [(abi.ReactionDesc, "pfx_reaction_desc"), (abi.Cells, "pfx_cells"),
                                           (abi.ReactOpts, "pfx_react_opts"), (abi.Stats, "pfx_stats"),
-                                          (abi.Grid, "pfx_grid")])
+                                          (abi.Grid, "pfx_grid"), (abi.ZonePlan, "pfx_zone_plan")])
 def test_struct_layout_matches_header(tmp_path, struct, cname):
     fields = [f for f, _ in struct._fields_]
     src = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', 'int main(void){',

@@ -32,3 +32,4 @@ def test_fortran_module_binds_the_c_abi(tmp_path):
     assert int(vals["sizeof_opts"]) == C.sizeof(abi.ReactOpts)
     assert int(vals["sizeof_stats"]) == C.sizeof(abi.Stats)
     assert int(vals["sizeof_grid"]) == C.sizeof(abi.Grid)
+    assert int(vals["sizeof_plan"]) == C.sizeof(abi.ZonePlan)
