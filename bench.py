@@ -85,7 +85,7 @@ def main():
     import numpy as np
     import torch
 
-    from pflotran_amd import abi, runtime, surrogate, synth, workload
+    from pflotran_amd import abi, autotune, runtime, surrogate, synth, workload
 
     torch.cuda.set_device(0)
     t = synth.random_network(15, 50, 3, ionx=True)
@@ -110,6 +110,9 @@ def main():
         restore()
         box["rc"], box["st"] = runtime.react_batch(tab, dc, plain_opts)
 
+    # untimed: pick the kernel variant (register cap) by measurement
+    plain_tune = autotune.tune_kernel(tab, dc, lambda: runtime.react_batch(tab, dc, plain_opts))
+    log(f"plain kernel variants (ms): {plain_tune}")
     for _ in range(args.warmup):
         plain_step()
     plain_ms = timed_steps(torch, args.steps, plain_step)
@@ -139,7 +142,7 @@ def main():
     plain = {"cells_per_s": plain_value, "ms_per_step": plain_ms_step, "kernel_ms": plain_kernel_ms,
              "converged": int(st_plain.converged), "mean_iterations": st_plain.total_iterations / n,
              "max_iterations": st_plain.max_iterations, "max_rel_res": st_plain.max_rel_res,
-             "histogram": list(st_plain.iteration_histogram)}
+             "histogram": list(st_plain.iteration_histogram), "kernel_variant_ms": plain_tune}
 
     fl = t.flops_per_iteration()
     warm = None
@@ -172,6 +175,9 @@ def main():
             box["rc"], box["st"] = runtime.react_batch(tab, dc, warm_opts)
             mlp_ms.append(ev[0].elapsed_time(ev[1]))
 
+        surrogate.guess(tab, mlp, dc)
+        warm_tune = autotune.tune_kernel(tab, dc, lambda: runtime.react_batch(tab, dc, warm_opts))
+        log(f"warm kernel variants (ms): {warm_tune}")
         for _ in range(args.warmup):
             warm_step()
         mlp_ms.clear()
@@ -191,6 +197,7 @@ def main():
                 "histogram": list(st_warm.iteration_histogram),
                 "speedup_vs_plain": warm_value / plain_value,
                 "iteration_reduction": (st_plain.total_iterations / max(st_warm.total_iterations, 1)),
+                "kernel_variant_ms": warm_tune, "kernel_variant": tab.tune(abi.PFX_KERNEL_REACT),
                 "prediction_error_ln": pred,
                 "training": {"pairs": int(args.train_pairs), "epochs": args.epochs, "batch": args.batch,
                              "steps": tr["steps"], "seconds": tr["seconds"], "samples_per_s": tr["samples_per_s"],
@@ -329,7 +336,7 @@ def zones_leg(args, log):
     import numpy as np
     import torch
 
-    from pflotran_amd import abi, runtime, synth, workload
+    from pflotran_amd import abi, autotune, runtime, synth, workload
 
     nets = workload.zone_networks()
     tabs = [runtime.Tables(t) for t in nets]
@@ -361,25 +368,10 @@ def zones_leg(args, log):
             runtime.react_batch(tab, dc, opts)
         torch.cuda.synchronize()
         sep.append((time.perf_counter() - t1) * 1e3)
-    # zoned launch, autotuned: for each kernel variant (registers as needed /
-    # capped at two wavefronts per SIMD) the split starts from cells * N^2
-    # and is rebalanced from the measured zone times; the fastest is kept
-    history = []
-    for variant in (0, 1):
-        plan = abi.ZonePlan()
-        plan.nzones = 4
-        plan.variant = variant
-        for _ in range(3):
-            restore()
-            rc, st, plan = runtime.react_zones(tabs, dcs, opts, plan)
-            history.append({"variant": variant, "blocks": list(plan.blocks[:4]),
-                            "zone_ms": list(plan.zone_ms[:4]), "kernel_ms": plan.kernel_ms})
-            runtime.rebalance(plan)
-    best = min(history, key=lambda h: h["kernel_ms"])
-    plan = abi.ZonePlan()
-    plan.nzones = 4
-    plan.variant = best["variant"]
-    plan.blocks[:4] = best["blocks"]
+    # zoned launch, autotuned: per kernel variant (register cap) the split
+    # starts from cells * N^2 and is rebalanced from the measured zone times
+    plan, history = autotune.tune_zones(tabs, dcs, opts)
+    best = {"variant": plan.variant, "blocks": list(plan.blocks[:4])}
     graph = runtime.ZonesGraph(tabs, dcs, opts, plan)
     g_ms, k_ms = [], []
     for _ in range(5):
@@ -401,7 +393,8 @@ def zones_leg(args, log):
            "graph_ms_per_step": float(np.mean(g_ms)), "graph_kernel_ms": float(np.mean(k_ms)),
            "cells_per_s": conv / (float(np.mean(g_ms)) * 1e-3),
            "separate_launches_ms": float(np.mean(sep)), "autotune_history": history, "blocks": best["blocks"],
-           "variant": best["variant"],
+           "variant": best["variant"], "graph_plan": {"variant": gp.variant, "blocks": list(gp.blocks[:4]),
+                                                      "zone_ms": list(gp.zone_ms[:4])},
            "mean_iterations_per_zone": its, "iteration_histogram": hist.tolist(),
            "converged": int(conv)}
     log(f"zones: graph {out['graph_ms_per_step']:.2f} ms/step (kernel {out['graph_kernel_ms']:.2f}), "

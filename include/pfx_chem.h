@@ -209,6 +209,19 @@ int pfx_abi_version(void);
 int pfx_tables_create(const pfx_reaction_desc *desc, pfx_tables **out);
 int pfx_tables_destroy(pfx_tables *tables);
 
+/* Register-cap variant of the per-cell kernels (no counterpart in the
+   reference; an autotuning knob): kernel = PFX_KERNEL_*; variant 0 gives
+   the kernel the registers its code wants, 1 and 2 cap them at 2 and 3
+   wavefronts per SIMD (the compiler spills the rest).  Only the Newton
+   kernels (REACT, EQUIL) have capped variants.  variant < 0 queries.
+   Returns the variant (query) or 0, or PFX_E_ARG.  Defaults, from
+   measurement on MI355X: 2 for groups of up to 16 lanes, 1 for 32. */
+#define PFX_KERNEL_REACT 0   /* pfx_react_batch        */
+#define PFX_KERNEL_EVAL  1   /* pfx_residual_jacobian  */
+#define PFX_KERNEL_EQUIL 2   /* pfx_equilibrate_batch  */
+#define PFX_KERNEL_AUX   3   /* pfx_auxvar_batch       */
+int pfx_tables_tune(pfx_tables *tables, int32_t kernel, int32_t variant);
+
 /* Device time (ms, HIP events) of the launches of the last call made with
    these tables: ms[0] the per-cell kernel, ms[1] the global-implicit
    assembly kernel (0 if the call had none).  Profiling aid; no counterpart
@@ -334,8 +347,8 @@ typedef struct pfx_zone_plan {
   double zone_ms[PFX_MAX_ZONES];   /* Out: when the zone's last workgroup retired */
   double kernel_ms;                /* Out: the launch (HIP events)            */
   int32_t variant;                 /* In/Out: 0 registers as needed (1 wavefront
-                                      per SIMD with 32-lane zones), 1 capped at
-                                      2 wavefronts per SIMD (spills)          */
+                                      per SIMD with 32-lane zones), 1 and 2
+                                      capped at 2 and 3 wavefronts per SIMD   */
   int32_t reserved;
 } pfx_zone_plan;
 

@@ -17,6 +17,7 @@ PFX_MAX_ZONES = 8
 PFX_E_ARG, PFX_E_DEVICE, PFX_E_NOMEM, PFX_E_UNSUP = -1, -2, -3, -4
 
 PFX_ACT_OFF, PFX_ACT_TIMESTEP, PFX_ACT_NEWTON_ITER = 0, 1, 2
+PFX_KERNEL_REACT, PFX_KERNEL_EVAL, PFX_KERNEL_EQUIL, PFX_KERNEL_AUX = 0, 1, 2, 3
 PFX_ACT_ALG_LAG, PFX_ACT_ALG_NEWTON = 0, 1
 
 PFX_CELL_CONVERGED, PFX_CELL_MAX_ITERATIONS, PFX_CELL_NONFINITE, PFX_CELL_SINGULAR = 0, 1, 2, 3

@@ -14,12 +14,11 @@
 
 #include "pfx_device.h"
 
-template <int G, int MODE>
+template <int G, int MODE, int W>
 __global__ void pfx_react_kernel(const pfx_kernel_params P);
+template <int W>
 __global__ void pfx_react_zones_kernel(const pfx_kernel_params *zone_params, const pfx_zone_launch Z,
                                        uint64_t *finish);
-__global__ void pfx_react_zones_kernel_w2(const pfx_kernel_params *zone_params, const pfx_zone_launch Z,
-                                          uint64_t *finish);
 extern "C" int pfx_scr_doubles(int M, int G, int K);
 
 struct pfx_tables {
@@ -34,6 +33,7 @@ struct pfx_tables {
   double *d_gi = nullptr;                 // global-implicit per-cell scratch
   int64_t gi_cells = 0;
   double last_ms[2] = {0.0, 0.0};         // device time of the last call's launches
+  int variant[4] = {0, 0, 0, 0};          // register-cap variant per kernel mode (pfx_tables_tune)
   // zoned launches led by these tables: parameter buffer and retire stamps
   pfx_kernel_params *d_zone_params = nullptr;
   uint64_t *d_finish = nullptr;
@@ -153,20 +153,31 @@ int ensure_scratch(pfx_tables *t, int64_t n) {
   return 0;
 }
 
-template <int MODE>
+template <int MODE, int W>
 const void *kernel_for(int G) {
-  return G == 8 ? (const void *)pfx_react_kernel<8, MODE>
-       : G == 16 ? (const void *)pfx_react_kernel<16, MODE>
-                 : (const void *)pfx_react_kernel<32, MODE>;
+  return G == 8 ? (const void *)pfx_react_kernel<8, MODE, W>
+       : G == 16 ? (const void *)pfx_react_kernel<16, MODE, W>
+                 : (const void *)pfx_react_kernel<32, MODE, W>;
 }
 
-const void *kernel_ptr(int G, int mode) {
+// variant v caps the registers at v + 1 wavefronts per SIMD (v = 0: none);
+// only the Newton modes have the capped variants
+const void *kernel_ptr(int G, int mode, int v) {
   switch (mode) {
-    case PFX_MODE_REACT: return kernel_for<PFX_MODE_REACT>(G);
-    case PFX_MODE_EVAL: return kernel_for<PFX_MODE_EVAL>(G);
-    case PFX_MODE_EQUIL: return kernel_for<PFX_MODE_EQUIL>(G);
-    default: return kernel_for<PFX_MODE_AUX>(G);
+    case PFX_MODE_REACT:
+      return v == 2 ? kernel_for<PFX_MODE_REACT, 3>(G)
+           : v == 1 ? kernel_for<PFX_MODE_REACT, 2>(G) : kernel_for<PFX_MODE_REACT, 1>(G);
+    case PFX_MODE_EVAL: return kernel_for<PFX_MODE_EVAL, 1>(G);
+    case PFX_MODE_EQUIL:
+      return v == 2 ? kernel_for<PFX_MODE_EQUIL, 3>(G)
+           : v == 1 ? kernel_for<PFX_MODE_EQUIL, 2>(G) : kernel_for<PFX_MODE_EQUIL, 1>(G);
+    default: return kernel_for<PFX_MODE_AUX, 1>(G);
   }
+}
+
+const void *zones_kernel_ptr(int v) {
+  return v == 2 ? (const void *)pfx_react_zones_kernel<3>
+       : v == 1 ? (const void *)pfx_react_zones_kernel<2> : (const void *)pfx_react_zones_kernel<1>;
 }
 
 int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, double *ms) {
@@ -175,7 +186,7 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   const int64_t lds = t->L.lds_table_bytes + (int64_t)groups * pfx_scr_doubles(t->L.M, G, t->L.K) * 8;
   if (lds > kMaxLds) return PFX_E_UNSUP;
   int occ = 0;
-  const void *fn = kernel_ptr(G, mode);
+  const void *fn = kernel_ptr(G, mode, t->variant[mode & 3]);
   if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BLOCK, (size_t)lds));
   if (occ < 1) occ = 1;
@@ -352,8 +363,8 @@ int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, c
     wsum += w[z];
   }
   if (zs.lds > kMaxLds) return PFX_E_UNSUP;
-  zs.variant = plan && plan->variant == 1 ? 1 : 0;
-  zs.fn = zs.variant ? (const void *)pfx_react_zones_kernel_w2 : (const void *)pfx_react_zones_kernel;
+  zs.variant = plan && plan->variant >= 0 && plan->variant <= 2 ? plan->variant : 0;
+  zs.fn = zones_kernel_ptr(zs.variant);
   if (zs.lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(zs.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)zs.lds));
   int occ = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, zs.fn, BLOCK, (size_t)zs.lds));
@@ -723,6 +734,7 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, t->device));
   t->num_cu = prop.multiProcessorCount;
+  t->variant[PFX_MODE_REACT] = t->variant[PFX_MODE_EQUIL] = G <= 16 ? 2 : 1;
   if (lds > (int64_t)prop.sharedMemPerBlock) {
     delete t;
     return PFX_E_UNSUP;
@@ -989,4 +1001,12 @@ extern "C" int pfx_graph_stats(pfx_graph *g, pfx_stats *stats, pfx_zone_plan *pl
   float f = 0.f;
   HIPCHK(hipEventElapsedTime(&f, g->ev0, g->ev1));
   return zones_finish(g->tables, g->zs, g->d_finish, g->last, f, plan, stats);
+}
+
+extern "C" int pfx_tables_tune(pfx_tables *t, int32_t mode, int32_t variant) {
+  if (!t || mode < PFX_MODE_REACT || mode > PFX_MODE_AUX) return PFX_E_ARG;
+  if (variant < 0) return t->variant[mode];
+  if (variant > 2 || ((mode == PFX_MODE_EVAL || mode == PFX_MODE_AUX) && variant != 0)) return PFX_E_ARG;
+  t->variant[mode] = variant;
+  return 0;
 }

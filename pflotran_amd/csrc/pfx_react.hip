@@ -884,8 +884,13 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   }
 }
 
-template <int G, int MODE>
-__global__ __launch_bounds__(BLOCK) void pfx_react_kernel(const pfx_kernel_params Pk) {
+// W: minimum wavefronts per SIMD the register allocation must allow (1 =
+// as many registers as the code wants; 2, 3 spill what does not fit).  The
+// fastest choice depends on the group size and mode and is picked by
+// measurement (pfx_tables_tune).
+template <int G, int MODE, int W>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W))) void pfx_react_kernel(
+    const pfx_kernel_params Pk) {
   (void)Pk;
   react_body<G, MODE>((uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr());
 }
@@ -908,24 +913,26 @@ __device__ __forceinline__ void zones_body(const pfx_kernel_params *zone_params,
   if (threadIdx.x == 0 && finish) finish[blockIdx.x] = wall_clock64();
 }
 
-// variant 0: registers as the 32-lane path needs them (one wavefront per
-// SIMD when such a zone is present); variant 1: capped at two wavefronts per
-// SIMD, the 32-lane path spilling what does not fit
-__global__ __launch_bounds__(BLOCK) void pfx_react_zones_kernel(const pfx_kernel_params *zone_params,
-                                                                const pfx_zone_launch Z, uint64_t *finish) {
-  zones_body(zone_params, Z, finish);
-}
-
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void pfx_react_zones_kernel_w2(
+// W as for pfx_react_kernel: 1 leaves the registers to the 32-lane path (one
+// wavefront per SIMD when such a zone is present), 2 and 3 cap them
+template <int W>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W))) void pfx_react_zones_kernel(
     const pfx_kernel_params *zone_params, const pfx_zone_launch Z, uint64_t *finish) {
   zones_body(zone_params, Z, finish);
 }
+template __global__ void pfx_react_zones_kernel<1>(const pfx_kernel_params *, const pfx_zone_launch, uint64_t *);
+template __global__ void pfx_react_zones_kernel<2>(const pfx_kernel_params *, const pfx_zone_launch, uint64_t *);
+template __global__ void pfx_react_zones_kernel<3>(const pfx_kernel_params *, const pfx_zone_launch, uint64_t *);
 
-#define PFX_INST(G)                                                                 \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_REACT>(const pfx_kernel_params); \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_EVAL>(const pfx_kernel_params);  \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_EQUIL>(const pfx_kernel_params); \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_AUX>(const pfx_kernel_params);
+#define PFX_INST(G)                                                                    \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_REACT, 1>(const pfx_kernel_params); \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_REACT, 2>(const pfx_kernel_params); \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_REACT, 3>(const pfx_kernel_params); \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_EVAL, 1>(const pfx_kernel_params);  \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_EQUIL, 1>(const pfx_kernel_params); \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_EQUIL, 2>(const pfx_kernel_params); \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_EQUIL, 3>(const pfx_kernel_params); \
+  template __global__ void pfx_react_kernel<G, PFX_MODE_AUX, 1>(const pfx_kernel_params);
 PFX_INST(8)
 PFX_INST(16)
 PFX_INST(32)

@@ -95,7 +95,7 @@ module pfx_chem_module
             pfx_auxvar_batch, pfx_update_kinetic_state, pfx_tables_timing, &
             pfx_gi_structure, pfx_gi_residual_jacobian, pfx_gi_dispersion, &
             pfx_react_zones, pfx_zone_plan_rebalance, pfx_zones_graph_create, &
-            pfx_graph_launch, pfx_graph_stats, pfx_graph_destroy
+            pfx_graph_launch, pfx_graph_stats, pfx_graph_destroy, pfx_tables_tune
   ! learned warm start (include/pfx_surrogate.h)
   public :: pfx_mlp_create, pfx_mlp_destroy, pfx_mlp_params, pfx_mlp_forward, &
             pfx_mlp_gradients, pfx_mlp_adam, pfx_mlp_train_step, &
@@ -163,6 +163,13 @@ module pfx_chem_module
       type(pfx_react_opts), intent(in) :: opts
       integer(c_int) :: pfx_auxvar_batch
     end function pfx_auxvar_batch
+
+    function pfx_tables_tune(tables, kernel, variant) bind(c, name='pfx_tables_tune')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: tables
+      integer(c_int32_t), value :: kernel, variant
+      integer(c_int) :: pfx_tables_tune
+    end function pfx_tables_tune
 
     function pfx_tables_timing(tables, ms) bind(c, name='pfx_tables_timing')
       import :: c_int, c_double, c_ptr

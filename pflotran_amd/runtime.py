@@ -23,7 +23,7 @@ EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_r
             "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch", "pfx_update_kinetic_state",
             "pfx_gi_structure", "pfx_gi_residual_jacobian", "pfx_tables_timing", "pfx_gi_dispersion",
             "pfx_react_zones", "pfx_zone_plan_rebalance", "pfx_zones_graph_create", "pfx_graph_launch",
-            "pfx_graph_stats", "pfx_graph_destroy",
+            "pfx_graph_stats", "pfx_graph_destroy", "pfx_tables_tune",
             # learned warm start (include/pfx_surrogate.h)
             "pfx_mlp_create", "pfx_mlp_destroy", "pfx_mlp_params", "pfx_mlp_forward", "pfx_mlp_gradients",
             "pfx_mlp_adam", "pfx_mlp_train_step", "pfx_surrogate_features", "pfx_surrogate_guess",
@@ -59,6 +59,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_auxvar_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_int32, P(abi.ReactOpts)]
     lib.pfx_update_kinetic_state.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts)]
     lib.pfx_tables_timing.argtypes = [C.c_void_p, C.c_void_p]
+    lib.pfx_tables_tune.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.pfx_react_zones.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, P(abi.ReactOpts), P(abi.ZonePlan),
                                     C.c_void_p]
     lib.pfx_zone_plan_rebalance.argtypes = [P(abi.ZonePlan)]
@@ -109,6 +110,13 @@ class Tables:
         if rc != 0:
             raise PfxError(f"pfx_tables_create failed: {rc}")
         self.handle = h
+
+    def tune(self, kernel: int, variant: int = -1) -> int:
+        """pfx_tables_tune: set (variant >= 0) or query the register-cap variant."""
+        rc = load_library().pfx_tables_tune(self.handle, kernel, variant)
+        if rc < 0:
+            raise PfxError(f"pfx_tables_tune failed: {rc}")
+        return rc if variant < 0 else variant
 
     def timing(self):
         """(per-cell kernel ms, assembly kernel ms) of the last call."""

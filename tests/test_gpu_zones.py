@@ -110,3 +110,23 @@ def test_near_saturation_workload():
     # most of the first half sits inside the window, the rest is unconstrained
     assert inside.sum() >= 0.6 * 1000
     assert np.all(np.abs(lo[inside]) <= 0.05)
+
+
+def test_kernel_variants_give_identical_bits(zones):
+    # the register cap changes register allocation and spills, not arithmetic
+    nets, cells = zones
+    for t, c in ((nets[1], cells[1]), (nets[3], cells[3])):
+        tab = runtime.Tables(t)
+        outs = []
+        for v in (0, 1, 2):
+            assert tab.tune(abi.PFX_KERNEL_REACT, v) == v
+            assert tab.tune(abi.PFX_KERNEL_REACT) == v
+            dc = device(c, c.ncells)
+            runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT))
+            outs.append(outputs(dc))
+        for o in outs[1:]:
+            for k in FIELDS:
+                if k in o:
+                    assert np.array_equal(o[k], outs[0][k]), k
+    with pytest.raises(runtime.PfxError):
+        tab.tune(abi.PFX_KERNEL_EVAL, 1)
