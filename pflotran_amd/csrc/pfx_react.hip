@@ -741,7 +741,9 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           int p = r;
           gargmax<G>(cand, p, lane);
           const int src = gbase + p;
-          // stream the pivot row one element at a time
+          // stream the pivot row one element at a time (lane to lane; an LDS
+          // broadcast of the row measured slower: it adds a store-to-load
+          // wait to every step)
           double piv = bperm_d(row[j], src);
           if (piv == 0.0) piv = 1.0e-20;
           const bool elim = (r != p) && !pivoted;
