@@ -699,7 +699,8 @@ static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
 restart:
   while (!done) {
     if (d->act_coef_update_frequency == PFX_ACT_NEWTON_ITER) {
-      or_rtotal(&*d, &c);
+      /* reaction.F90:3415-3418: the complexes are those of the previous
+         RTAuxVarCompute (the previous iterate) */
       or_activity(d, &c);
     }
     or_res_jac(d, &c, fixed, dt, Rv, J, sc);
@@ -902,6 +903,8 @@ static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype,
     if (d->act_coef_update_frequency != PFX_ACT_OFF && compute_act) or_activity(d, c);
     or_rtotal(d, c);
     if (d->neqionxrxn > 0) or_ionx(d, c);
+    /* the reference stops on these failures (reaction.F90:3866-3877, 4382-4385) */
+    if (c->status) return c->status;
     for (int j = 0; j < N; j++) {
       for (int k = 0; k < N; k++) J[j][k] = 0.0;
       switch (ctype[j]) {

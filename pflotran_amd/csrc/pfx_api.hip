@@ -541,9 +541,9 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   if (K > 0 && (!d->kinmnrlspecid || !d->kinmnrlstoich || !d->kinmnrl_logK || !d->kinmnrl_rate_constant))
     return PFX_E_ARG;
   if (R > 0 && (!d->eqionx_rxn_cationid || !d->eqionx_rxn_k || !d->eqionx_rxn_CEC)) return PFX_E_ARG;
-  if (d->act_coef_update_frequency == PFX_ACT_NEWTON_ITER ||
-      (d->act_coef_update_frequency != PFX_ACT_OFF && d->act_coef_update_algorithm != PFX_ACT_ALG_LAG))
-    return PFX_E_UNSUP;  // activity updates inside the Newton loop: not in this kernel
+  if (d->act_coef_update_frequency < PFX_ACT_OFF || d->act_coef_update_frequency > PFX_ACT_NEWTON_ITER ||
+      (d->act_coef_update_algorithm != PFX_ACT_ALG_LAG && d->act_coef_update_algorithm != PFX_ACT_ALG_NEWTON))
+    return PFX_E_ARG;
   for (int i = 0; i < M; i++)
     if (d->eqcplxh2ostoich && d->eqcplxh2ostoich[i] != 0.0) {
       // water activity is 1 (ln a_w = 0, use_activity_h2o = F): the term vanishes

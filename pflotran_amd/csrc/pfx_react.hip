@@ -13,7 +13,7 @@
 //   RReact                reaction.F90:3322-3511   (Newton loop, log update)
 //   RTAuxVarCompute       reaction.F90:4969-5006
 //   RTotal                reaction.F90:4057-4158   (phases A/B below)
-//   RActivityCoefficients reaction.F90:3992-4051   (LAG)
+//   RActivityCoefficients reaction.F90:3812-4053   (LAG and NEWTON)
 //   RTotalSorbEqIonx      reaction.F90:4305-4535
 //   RKineticMineral       reaction_mineral.F90:564-1000 (no prefactor species)
 //   RSolve/ludcmp/lubksb  reaction.F90:4835-4880, utility.F90:393-523
@@ -389,30 +389,118 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 
     // ---------------------------------------------------------------
     // activity coefficients from the current free ions and the complex
-    // molalities of the previous RTotal (LAG form, reaction.F90:3992-4051)
+    // molalities of the previous RTotal (RActivityCoefficients,
+    // reaction.F90:3812-4053)
     bool act_now = false;
     if (MODE == PFX_MODE_REACT)
-      act_now = phase == 1 || (phase == 2 && L.act_freq == PFX_ACT_NEWTON_ITER);
+      act_now = phase == 1 || (phase == 2 && L.act_freq == PFX_ACT_NEWTON_ITER && !stop_next);
     else if (MODE == PFX_MODE_EQUIL)
       act_now = compute_act && L.act_freq != PFX_ACT_OFF;
     else if (MODE == PFX_MODE_AUX)
       act_now = it >= 1;
     if (active && act_now) {
-      double part = live ? m * Zr * Zr : 0.0;
-      for (int cc = r; cc < M; cc += G) part += SD(S.C, cc) * LD(L.o_cZ, cc) * LD(L.o_cZ, cc);
-      const double I = 0.5 * gsum<G>(part, lane);
-      const double sqI = sqrt(I);
-      const double A = L.debyeA, B = L.debyeB, Bd = L.debyeBdot;
-      if (live) {
-        gam = fabs(Zr) > 1e-10 ? exp((-Zr * Zr * sqI * A / (1.0 + a0r * B * sqI) + Bd * I) * PFX_LOG_TO_LN)
-                               : 1.0;
-        lng = log(gam);
-      }
-      for (int cc = r; cc < M; cc += G) {
-        const double z = LD(L.o_cZ, cc);
-        SD(S.gsec, cc) = fabs(z) > 1e-10
-                             ? exp((-z * z * sqI * A / (1.0 + LD(L.o_ca0, cc) * B * sqI) + Bd * I) * PFX_LOG_TO_LN)
-                             : 1.0;
+      // with updates every Newton iteration, the step's first pass updates
+      // once at the previous state (reaction.F90:3407-3409) and once more at
+      // the initial iterate, as iteration 1 does (reaction.F90:3415-3418)
+      const int npass = (MODE == PFX_MODE_REACT && phase == 1 && L.act_freq == PFX_ACT_NEWTON_ITER) ? 2 : 1;
+      for (int pass = 0; pass < npass; pass++) {
+        if (MODE == PFX_MODE_REACT && pass == 1) {
+          phase = 2;
+          if (live) {
+            if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
+              m = P.guess[cell * N + r];
+            else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
+          }
+        }
+        const double A = L.debyeA, B = L.debyeB, Bd = L.debyeBdot;
+        if (L.act_alg == PFX_ACT_ALG_NEWTON) {
+          // Newton on the ionic strength that also updates the complex
+          // molalities at fixed primary activities (reaction.F90:3846-3991)
+          SD(S.la, r) = live ? log(m) + lng : 0.0;
+          const double fpri = gsum<G>(live ? m * Zr * Zr : 0.0, lane);
+          double II = 0.0;
+          int ait = 0;
+          wave_sync();
+          for (;;) {
+            if (++ait > 50) {  // reference: NaN poisoning (reaction.F90:3866-3877)
+              cell_flag = PFX_CELL_NONFINITE;
+              break;
+            }
+            double part = 0.0;
+            for (int cc = r; cc < M; cc += G) part += SD(S.C, cc) * LD(L.o_cZ, cc) * LD(L.o_cZ, cc);
+            double I = 0.5 * (fpri + gsum<G>(part, lane));
+            const double f = I;
+            if (fabs(I - II) < 1.0e-6 * I) break;
+            if (M > 0) {
+              double sqI = sqrt(I);
+              // d log10(gamma_j)/dI of the primaries (0 for neutral ones)
+              if (live) {
+                const double tj = 1.0 + B * a0r * sqI;
+                SD(S.im, r) = fabs(Zr) > 0.0 ? -0.5 * A * Zr * Zr / (sqI * tj * tj) + Bd : 0.0;
+              }
+              wave_sync();
+              double dpart = 0.0;
+              for (int cc = r; cc < M; cc += G) {
+                const double z = LD(L.o_cZ, cc);
+                if (fabs(z) > 0.0) {
+                  const double t = 1.0 + B * LD(L.o_ca0, cc) * sqI;
+                  double sum = 0.5 * A * z * z / (sqI * t * t) - Bd;
+                  const int nS = L.S;
+                  for (int s = 0; s < nS; s++) {
+                    const int sp = LI(L.o_cspec, cc * nS + s);
+                    if (sp >= 0 && fabs(LD(L.o_Z, sp)) > 0.0) sum += LD(L.o_cnu, cc * nS + s) * SD(S.im, sp);
+                  }
+                  const double dcdi = SD(S.C, cc) * PFX_LOG_TO_LN * sum;
+                  dpart += 0.5 * z * z * dcdi;
+                }
+              }
+              const double didi = gsum<G>(dpart, lane);
+              const double dd = 1.0 - didi;
+              II = fabs(dd) > 0.0 ? (f - I * didi) / dd : f;
+            } else {
+              II = f;
+            }
+            I = II;
+            const double sqI = sqrt(I);
+            if (live) {
+              gam = fabs(Zr) > 0.0 ? exp((-Zr * Zr * sqI * A / (1.0 + a0r * B * sqI) + Bd * I) * PFX_LOG_TO_LN)
+                                   : 1.0;
+              lng = log(gam);
+            }
+            for (int cc = r; cc < M; cc += G) {
+              const double z = LD(L.o_cZ, cc);
+              const double gs =
+                  fabs(z) > 0.0 ? exp((-z * z * sqI * A / (1.0 + LD(L.o_ca0, cc) * B * sqI) + Bd * I) * PFX_LOG_TO_LN)
+                                : 1.0;
+              SD(S.gsec, cc) = gs;
+              double lnQK = SD(S.lnK, cc);
+              const int nS = L.S;
+              for (int s = 0; s < nS; s++) {
+                const int sp = LI(L.o_cspec, cc * nS + s);
+                if (sp >= 0) lnQK += LD(L.o_cnu, cc * nS + s) * SD(S.la, sp);
+              }
+              SD(S.C, cc) = exp(lnQK) / gs;
+            }
+            wave_sync();
+          }
+        } else {
+          // LAG: one pass (reaction.F90:3992-4051)
+          double part = live ? m * Zr * Zr : 0.0;
+          for (int cc = r; cc < M; cc += G) part += SD(S.C, cc) * LD(L.o_cZ, cc) * LD(L.o_cZ, cc);
+          const double I = 0.5 * gsum<G>(part, lane);
+          const double sqI = sqrt(I);
+          if (live) {
+            gam = fabs(Zr) > 1e-10 ? exp((-Zr * Zr * sqI * A / (1.0 + a0r * B * sqI) + Bd * I) * PFX_LOG_TO_LN)
+                                   : 1.0;
+            lng = log(gam);
+          }
+          for (int cc = r; cc < M; cc += G) {
+            const double z = LD(L.o_cZ, cc);
+            SD(S.gsec, cc) = fabs(z) > 1e-10
+                                 ? exp((-z * z * sqI * A / (1.0 + LD(L.o_ca0, cc) * B * sqI) + Bd * I) * PFX_LOG_TO_LN)
+                                 : 1.0;
+          }
+        }
       }
       if (MODE == PFX_MODE_REACT && phase == 1) {
         phase = 2;
@@ -697,7 +785,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       if (ctype == PFX_CONSTRAINT_PH || ctype == PFX_CONSTRAINT_CHARGE) SD(S.mv, r) = m;
       if (it > 3 && it < 9) use_log = use_log ? (it % 2 == 0) : 0;  // reaction.F90:1797-1804
       rr = gmax<G>(live ? fabs(Rr) : 0.0, lane);  // max |Res| (mol/L)
-      if (!(rr == rr) || rr == INFINITY) {
+      if (cell_flag != PFX_CELL_CONVERGED) {
+        finish = true;
+        status = cell_flag;
+      } else if (!(rr == rr) || rr == INFINITY) {
         finish = true;
         status = PFX_CELL_NONFINITE;
       } else if (stop_next) {

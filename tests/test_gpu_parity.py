@@ -100,8 +100,12 @@ def test_react_ragged_and_previous_guess(net15, n):
     assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
 
 
-def test_equilibrate_parity(net15):
+@pytest.mark.parametrize("alg", [abi.PFX_ACT_ALG_LAG, abi.PFX_ACT_ALG_NEWTON])
+def test_equilibrate_parity(net15, alg):
     t = net15
+    if alg != t.act_alg:
+        t = synth.random_network(15, 50, 3, ionx=True)
+        t.act_alg = alg
     n = 256
     cells, ctype, conc, ok = speciated_state(t, n, seed=11)
     tab = runtime.Tables(t)
@@ -120,6 +124,34 @@ def test_equilibrate_parity(net15):
     assert rel_err(out["pri_molal"], cells["pri_molal"]) <= 1e-10
     assert rel_err(out["pri_act_coef"], cells["pri_act_coef"]) <= 1e-10
     assert rel_err(out["total"][:, 1:], cells["total"][:, 1:]) <= 1e-10
+
+
+@pytest.mark.parametrize("freq", [abi.PFX_ACT_TIMESTEP, abi.PFX_ACT_NEWTON_ITER])
+@pytest.mark.parametrize("alg", [abi.PFX_ACT_ALG_LAG, abi.PFX_ACT_ALG_NEWTON])
+def test_react_parity_activity_algorithms(alg, freq):
+    # activity coefficients once per step or every Newton iteration, by the
+    # one-pass LAG form or the ionic-strength iteration (reaction.F90:3812-4053).
+    # Updates every iteration start from the previous free ions, as RReact
+    # does: from the 1e-9 default most cells stop in the exchange or
+    # ionic-strength iterations, in the oracle and on the GPU alike.
+    t = synth.random_network(15, 50, 3, ionx=True, act_freq=freq)
+    t.act_alg = alg
+    n = 256
+    cells, _, _, _ = speciated_state(t, n, seed=23)
+    guess = abi.PFX_GUESS_PREVIOUS if freq == abi.PFX_ACT_NEWTON_ITER else abi.PFX_GUESS_DEFAULT
+    ref = cells.copy()
+    orc.react_batch(t, ref, synth.react_opts(extra_iterations=1, guess_mode=guess))
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1, guess_mode=guess))
+    out = dc.to_host()
+    assert np.array_equal(out["status"], ref["status"])
+    ok = ref["status"] == 0
+    assert ok.mean() > 0.9
+    assert rel_err(out["pri_molal"][ok], ref["pri_molal"][ok]) <= 1e-10
+    assert rel_err(out["pri_act_coef"][ok], ref["pri_act_coef"][ok]) <= 1e-10
+    assert rel_err(out["sec_act_coef"][ok], ref["sec_act_coef"][ok]) <= 1e-10
+    assert np.mean(np.abs(out["num_iterations"][ok] - ref["num_iterations"][ok])) < 0.5
 
 
 def _gpu_speciate(t, setup, ncells=1):

@@ -252,3 +252,111 @@ def test_oracle_fallback_restarts_from_the_default_guess():
     assert np.array_equal(a["pri_molal"], b["pri_molal"])
     assert np.array_equal(a["status"], b["status"])
     assert np.array_equal(a["num_iterations"], b["num_iterations"] + 3)
+
+
+# ---------------------------------------------------------------- activity, NEWTON algorithm
+def _activity(t, m, sec, gam, gsec):
+    d = t.desc()
+    g, gs, so = gam.copy(), gsec.copy(), np.zeros(max(t.M, 1))
+    st = orc.load().or_activity_cell(C.byref(d), np.ascontiguousarray(m).ctypes.data,
+                                     np.ascontiguousarray(sec).ctypes.data, g.ctypes.data, gs.ctypes.data,
+                                     so.ctypes.data)
+    return st, g, gs, so[:t.M]
+
+
+def _dh_log10(z, a0, I, A, B, Bd):
+    return -z * z * np.sqrt(I) * A / (1.0 + a0 * B * np.sqrt(I)) + Bd * I
+
+
+def _ionic_strength_root(t, m, gam0):
+    """The fixed point that reaction.F90:3846-3991 iterates to (its update
+    II = (f - I didi)/(1 - didi) with f = I is I itself: successive
+    substitution, which converges where the map contracts), found by
+    bisection in extended precision: I = (sum_j z_j^2 m_j + sum_i z_i^2 C_i(I))/2
+    with C_i(I) = exp(lnK_i + sum_j nu_ij ln(m_j gam0_j)) / gamma_i(I); the
+    primary activities inside C stay at their values on entry."""
+    A, B, Bd = t.debye
+    la = np.log(m) + np.log(gam0)
+    lnq = np.array([-t.cplx_logK[i] * LN10_TRUNC +
+                    sum(t.cplx_stoich[i, s] * la[t.cplx_specid[i, 1 + s]] for s in range(t.cplx_specid[i, 0]))
+                    for i in range(t.M)], dtype=np.longdouble)
+    fpri = np.longdouble(np.sum(m * t.Z * t.Z))
+
+    def g(I):
+        lg = np.array([_dh_log10(np.longdouble(z), np.longdouble(a), I, A, B, Bd) for z, a in zip(t.cplx_Z, t.cplx_a0)])
+        C_ = np.exp(lnq - lg * np.longdouble(LN10_TRUNC))
+        return 0.5 * (fpri + np.sum(np.asarray(t.cplx_Z, dtype=np.longdouble) ** 2 * C_)) - I
+
+    # the first sign change on a log grid (far above it the B-dot term can
+    # raise the complexes again), then bisection
+    grid = np.logspace(-8, 1, 400).astype(np.longdouble)
+    k = next(i for i in range(len(grid)) if g(grid[i]) < 0)
+    lo, hi = grid[k - 1], grid[k]
+    assert g(lo) > 0 > g(hi)
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if g(mid) > 0 else (lo, mid)
+    return float(0.5 * (lo + hi))
+
+
+@pytest.mark.parametrize("case", ["ca_carbonate", "network"])
+def test_newton_activity_reaches_the_ionic_strength_fixed_point(case):
+    """The returned coefficients are Debye-Hueckel at the self-consistent
+    ionic strength (to the reference's own stop, |dI| < 1e-6 I), with the
+    primary activities inside the complexes held at their entry values; an
+    oracle that refreshed them, dropped the complexes' charge or iterated to
+    the LAG single pass would miss the root by far more than 1e-6."""
+    from chemdefs import ca_carbonate
+    rng = np.random.default_rng(17)
+    if case == "ca_carbonate":
+        t = ca_carbonate(True)[0]
+        m = np.array([2.5e-3, 8e-2, 3e-7])  # HCO3-, Ca++, H+: I ~ 0.2, strong Ca complexing
+    else:
+        t = synth.random_network(15, 50, 3, ionx=False)
+        m = 10.0 ** rng.uniform(-4, -2.5, size=t.N)  # I ~ 0.016, complexes ~2% of it
+    t.act_alg = abi.PFX_ACT_ALG_NEWTON
+    gam0 = rng.uniform(0.5, 0.9, size=t.N)
+    _, _, sec0 = _rtotal(t, m, gam0)
+    sec0 = sec0[:t.M] * rng.uniform(0.2, 5.0, size=t.M)  # a stale complex state to start from
+    st, gam, gsec, sec = _activity(t, m, sec0, gam0, np.ones(t.M))
+    assert st == 0
+    Istar = _ionic_strength_root(t, m, gam0)
+    A, B, Bd = t.debye
+    want = np.where(np.abs(t.Z) > 0, 10.0 ** _dh_log10(t.Z, t.a0, Istar, A, B, Bd), 1.0)
+    # d ln(gamma)/d ln(I) is below 3 here, so 1e-6 in I is below 3e-6 in gamma
+    err = np.max(np.abs(np.log(gam / want)))
+    assert err <= 3e-6
+    wsec = np.where(np.abs(t.cplx_Z) > 0, 10.0 ** _dh_log10(t.cplx_Z, t.cplx_a0, Istar, A, B, Bd), 1.0)
+    assert np.max(np.abs(np.log(gsec / wsec))) <= 3e-6
+    # complexes at the returned coefficients, entry primary activities
+    la = np.log(m) + np.log(gam0)
+    for i in range(t.M):
+        ns = t.cplx_specid[i, 0]
+        lnq = -t.cplx_logK[i] * LN10_TRUNC + sum(t.cplx_stoich[i, s] * la[t.cplx_specid[i, 1 + s]] for s in range(ns))
+        assert sec[i] == pytest.approx(np.exp(lnq) / gsec[i], rel=1e-13)
+    # the single LAG pass from the stale complexes is not the fixed point
+    t.act_alg = abi.PFX_ACT_ALG_LAG
+    _, gl, _, _ = _activity(t, m, sec0, gam0, np.ones(t.M))
+    assert np.max(np.abs(np.log(gl / want))) > max(10.0 * err, 3e-6)
+
+
+def test_newton_activity_with_neutral_complexes_is_the_lag_pass():
+    # with uncharged complexes the ionic strength is the primaries' alone:
+    # one Newton step lands on it and both algorithms give the same bits
+    t = synth.random_network(8, 20, 0, ionx=False)
+    t.cplx_Z[:] = 0.0
+    rng = np.random.default_rng(4)
+    m = 10.0 ** rng.uniform(-4, -2, size=t.N)
+    sec = 10.0 ** rng.uniform(-6, -3, size=t.M)
+    gam0 = np.ones(t.N)
+    out = []
+    for alg in (abi.PFX_ACT_ALG_LAG, abi.PFX_ACT_ALG_NEWTON):
+        t.act_alg = alg
+        st, g, gs, _ = _activity(t, m, sec, gam0, np.ones(t.M))
+        assert st == 0
+        out.append((g, gs))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    A, B, Bd = t.debye
+    I = 0.5 * np.sum(m * t.Z ** 2)
+    assert np.allclose(out[1][0], np.where(np.abs(t.Z) > 0, 10.0 ** _dh_log10(t.Z, t.a0, I, A, B, Bd), 1.0),
+                       rtol=1e-12)
