@@ -137,7 +137,14 @@ typedef struct pfx_reaction_desc {
   int32_t use_log_formulation;
   int32_t act_coef_update_frequency;  /* PFX_ACT_*     */
   int32_t act_coef_update_algorithm;  /* PFX_ACT_ALG_* */
-  int32_t reserved0;
+  int32_t numerical_jacobian;         /* 1: the reaction part of the per-cell
+                                         Jacobian by forward differences,
+                                         perturbation 1e-5 m_j (NUMERICAL_JACOBIAN,
+                                         RReactionDerivative reaction.F90:3625-3684);
+                                         applies to pfx_residual_jacobian and the
+                                         global-implicit assembly, as in the
+                                         reference; the reaction step's Newton
+                                         stays analytic (reaction.F90:3438-3439) */
   double max_dlnC;                    /* 5                         */
   double max_relative_change_tolerance;  /* 1e-6                   */
   double max_residual_tolerance;         /* 1e-12                  */

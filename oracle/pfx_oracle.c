@@ -17,7 +17,10 @@
  * Pinning status per function (checks live in tests/):
  *   or_rtotal            golds (carbonate/ca-carbonate speciation), complex-step dT/dm
  *   or_activity_lag      golds (gamma H+, HCO3-, Ca++), closed form
- *   or_activity_newton   closed-form fixed point of I (self-consistency)
+ *   or_activity_newton   the ionic-strength fixed point by extended-precision bisection;
+ *                        neutral complexes reduce it to the LAG pass bit for bit
+ *   or_reaction_fd       the analytic reaction Jacobian (den = 1000) to the forward
+ *                        difference's truncation; zero columns exact
  *   or_ionx              gold (ion-exchange-valocchi sorbed totals), charge invariant,
  *                        equal-valence closed form
  *   or_kinetic_mineral   golds (calcite-kinetics rate, pH, VF), finite-difference Jacobian
@@ -552,11 +555,38 @@ static void or_accumulation(const pfx_reaction_desc *d, const ocell *c, double *
     for (int j = 0; j < N; j++) acc[j] += c->sorb[j] * c->volume;
 }
 
+/* RReactionDerivative, numerical branch (reaction.F90:3625-3684): the
+   residual as in the analytic case; the reaction Jacobian column j by
+   forward differences of the rates of a copy of the cell with m_j raised by
+   1e-5 m_j (perturbation_tolerance, reaction.F90:36), the residual of each
+   taken from zero; entries below 1e-40 in magnitude are zeroed. */
+static void or_reaction_fd(const pfx_reaction_desc *d, ocell *c, double *R, double J[NMAX][NMAX]) {
+  const int N = d->naqcomp;
+  double unused[NMAX][NMAX];
+  or_kinetic_mineral(d, c, R, unused, 0);
+  double res_orig[NMAX], Jr[NMAX][NMAX];
+  for (int i = 0; i < N; i++) res_orig[i] = 0.0;
+  or_kinetic_mineral(d, c, res_orig, unused, 0);
+  for (int j = 0; j < N; j++) {
+    ocell p = *c;
+    double pert = p.m[j] * 1.0e-5;
+    p.m[j] = p.m[j] + pert;
+    double rp[NMAX];
+    for (int i = 0; i < N; i++) rp[i] = 0.0;
+    or_kinetic_mineral(d, &p, rp, unused, 0);
+    for (int i = 0; i < N; i++) Jr[i][j] = (rp[i] - res_orig[i]) / pert;
+  }
+  for (int i = 0; i < N; i++)
+    for (int k = 0; k < N; k++) J[i][k] += fabs(Jr[i][k]) < 1.0e-40 ? 0.0 : Jr[i][k];
+}
+
 /* dt-consistent residual (mol/s) and Jacobian (kg/s) of one cell, the
    per-cell part of RTResidualNonFlux/RTJacobianNonFlux
-   (reactive_transport.F90:2539-2587, 2735-2758, 3342-3465). */
+   (reactive_transport.F90:2539-2587, 2735-2758, 3342-3465).  numerical:
+   the reaction part of the Jacobian by differences (the global-implicit
+   path only; RReact's Newton is analytic, reaction.F90:3438-3439). */
 static void or_res_jac(const pfx_reaction_desc *d, ocell *c, const double *fixed_accum, double dt,
-                       double *R, double J[NMAX][NMAX], double *scale) {
+                       double *R, double J[NMAX][NMAX], double *scale, int numerical) {
   const int N = d->naqcomp;
   or_auxvar_compute(d, c);
   double acc[NMAX];
@@ -570,7 +600,10 @@ static void or_res_jac(const pfx_reaction_desc *d, ocell *c, const double *fixed
       if (d->neqionxrxn > 0) J[j][k] += c->dsorb[j][k] * v_t;
     }
   }
-  if (d->nkinmnrl > 0) or_kinetic_mineral(d, c, R, J, 1);
+  if (d->nkinmnrl > 0) {
+    if (numerical) or_reaction_fd(d, c, R, J);
+    else or_kinetic_mineral(d, c, R, J, 1);
+  }
   if (scale) {
     /* magnitude of every term in component j (mol/s): the denominator of the
        scaled residual rel_res = max_j |R_j| / scale_j */
@@ -703,7 +736,7 @@ restart:
          RTAuxVarCompute (the previous iterate) */
       or_activity(d, &c);
     }
-    or_res_jac(d, &c, fixed, dt, Rv, J, sc);
+    or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0);
     rr = or_rel_res(Rv, sc, N);
     if (!isfinite(rr)) { status = PFX_CELL_NONFINITE; break; }
     if (rr <= o->rel_tol) {
@@ -738,7 +771,7 @@ restart:
     }
     for (int j = 0; j < N; j++) maxrel = fmax(maxrel, fabs((c.m[j] - prev[j]) / prev[j]));
     if (o->reference_stop && maxrel < d->max_relative_change_tolerance) {
-      or_res_jac(d, &c, fixed, dt, Rv, J, sc);
+      or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0);
       rr = or_rel_res(Rv, sc, N);
       status = PFX_CELL_CONVERGED;
       done = 1;
@@ -849,13 +882,15 @@ int or_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
         if (d->neqionxrxn > 0 && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
       }
     }
-    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL);
+    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL, d->numerical_jacobian);
     for (int j = 0; j < N; j++) {
       residual[ci * N + j] = Rv[j];
       for (int k = 0; k < N; k++) jacobian[ci * N * N + j + (size_t)k * N] = J[j][k];
     }
     if (cl->sec_molal)
       for (int i = 0; i < d->neqcplx; i++) cl->sec_molal[ci * d->neqcplx + i] = c.sec[i];
+    if (cl->mnrl_rate)
+      for (int k = 0; k < d->nkinmnrl; k++) cl->mnrl_rate[ci * d->nkinmnrl + k] = c.rate[k];
   }
   (void)nthreads;
   return 0;
@@ -1238,7 +1273,7 @@ int or_gi_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx
       fixed[j] = psv_t * cl->total[ci * N + j];
       if (d->neqionxrxn > 0 && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
     }
-    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL);
+    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL, d->numerical_jacobian);
     for (int i = 0; i < N; i++) {
       tot[ci * N + i] = c.total[i];
       rnf[ci * N + i] = Rv[i];

@@ -44,7 +44,7 @@ class ReactionDesc(C.Structure):
         ("eqionx_rxn_to_surf", _ip),
         ("debyeA", C.c_double), ("debyeB", C.c_double), ("debyeBdot", C.c_double),
         ("use_log_formulation", C.c_int32), ("act_coef_update_frequency", C.c_int32),
-        ("act_coef_update_algorithm", C.c_int32), ("reserved0", C.c_int32),
+        ("act_coef_update_algorithm", C.c_int32), ("numerical_jacobian", C.c_int32),
         ("max_dlnC", C.c_double), ("max_relative_change_tolerance", C.c_double),
         ("max_residual_tolerance", C.c_double),
     ]

@@ -542,7 +542,8 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
     return PFX_E_ARG;
   if (R > 0 && (!d->eqionx_rxn_cationid || !d->eqionx_rxn_k || !d->eqionx_rxn_CEC)) return PFX_E_ARG;
   if (d->act_coef_update_frequency < PFX_ACT_OFF || d->act_coef_update_frequency > PFX_ACT_NEWTON_ITER ||
-      (d->act_coef_update_algorithm != PFX_ACT_ALG_LAG && d->act_coef_update_algorithm != PFX_ACT_ALG_NEWTON))
+      (d->act_coef_update_algorithm != PFX_ACT_ALG_LAG && d->act_coef_update_algorithm != PFX_ACT_ALG_NEWTON) ||
+      (d->numerical_jacobian != 0 && d->numerical_jacobian != 1))
     return PFX_E_ARG;
   for (int i = 0; i < M; i++)
     if (d->eqcplxh2ostoich && d->eqcplxh2ostoich[i] != 0.0) {
@@ -722,6 +723,7 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   L.use_log = d->use_log_formulation;
   L.act_freq = d->act_coef_update_frequency;
   L.act_alg = d->act_coef_update_algorithm;
+  L.num_jac = d->numerical_jacobian;
   L.max_dlnC = d->max_dlnC;
   L.max_rel_change_tol = d->max_relative_change_tolerance;
   L.max_residual_tol = d->max_residual_tolerance;

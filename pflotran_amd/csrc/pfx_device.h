@@ -27,7 +27,7 @@ struct pfx_dev_layout {
   double ionx_ref_k[PFX_MAX_IONX_RXN];
   int32_t ionx_ref_lane[PFX_MAX_IONX_RXN];
   double debyeA, debyeB, debyeBdot;
-  int32_t use_log, act_freq, act_alg;
+  int32_t use_log, act_freq, act_alg, num_jac;
   double max_dlnC, max_rel_change_tol, max_residual_tol;
   int32_t mnrl_flags;           // bit0 threshold, bit1 limiter, bit2 power, bit3 Temkin
 

@@ -210,6 +210,74 @@ __device__ __forceinline__ double pow_ratio(double x, double z) {
 // used instead of all being held in SGPRs across the whole loop.
 typedef const __attribute__((address_space(4))) pfx_kernel_params *KP;
 
+// Rate (mol/s/m^3 bulk) of kinetic mineral k at ln(Q/K), 0 where the
+// reference skips the mineral (reaction_mineral.F90:690-816).  Group-uniform.
+__device__ __forceinline__ double mineral_rate(KP Pp, const Scr &S, int k, double lnQK) {
+  const double QK = lnQK <= 6.90776 ? exp(lnQK) : 1.0e3;
+  const bool temkin = Pp->L.mnrl_flags & 8;
+  double aff = temkin ? 1.0 - pow(QK, 1.0 / LD(Pp->L.o_mtemkin, k)) : 1.0 - QK;
+  const double sgn = copysign(1.0, aff);
+  if (!(SD(S.mvf, k) > 0.0 || sgn < 0.0)) return 0.0;
+  const double thr = (Pp->L.mnrl_flags & 1) ? LD(Pp->L.o_mthr, k) : 0.0;
+  if (thr > 0.0 && sgn < 0.0 && QK < thr) return 0.0;
+  const double lim = (Pp->L.mnrl_flags & 2) ? LD(Pp->L.o_mlim, k) : 0.0;
+  if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
+  const double Im_const = -SD(S.marea, k);
+  const double rate = SD(S.mkeff, k);
+  return (Pp->L.mnrl_flags & 4) ? Im_const * sgn * pow(fabs(aff), LD(Pp->L.o_mpow, k)) * rate
+                             : Im_const * sgn * fabs(aff) * rate;
+}
+
+// Reaction part of the Jacobian by forward differences (NUMERICAL_JACOBIAN,
+// RReactionDerivative reaction.F90:3625-3684): column j is
+// (Res(m + e_j m_j 1e-5) - Res(m)) / (m_j 1e-5), Res = sum_k nu_ik Im_k V in
+// mineral order from zero; entries below 1e-40 in magnitude become 0.  The
+// perturbed ln activities use the unperturbed coefficients.  Adds to row[].
+template <int G>
+__device__ __forceinline__ void mineral_fd_jacobian(KP Pp, const Scr &S, int r, bool live, double m, double lng,
+                                                    double vol, double *row) {
+  const int K = Pp->L.K, nS = Pp->L.S;
+  {
+#pragma clang fp contract(off)
+    if (live) SD(S.xb, r) = log(m + m * 1.0e-5) + lng;  // perturbed ln a_r
+  }
+  wave_sync();
+  double ro = 0.0;
+  for (int k = 0; k < K; k++) {
+    double lnQK = SD(S.mlnK, k);
+    for (int s = 0; s < nS; s++) {
+      const int sp = LI(Pp->L.o_mspec, k * nS + s);
+      if (sp >= 0) lnQK += LD(Pp->L.o_mnu, k * nS + s) * SD(S.la, sp);
+    }
+    const double nur = live ? LD(Pp->L.o_mnuG, k * G + r) : 0.0;
+    ro += nur * (mineral_rate(Pp, S, k, lnQK) * vol);
+  }
+#pragma unroll
+  for (int j = 0; j < G; j++) {
+    if (j < Pp->L.N) {
+      double rp = 0.0;
+      for (int k = 0; k < K; k++) {
+        double lnQK = SD(S.mlnK, k);
+        for (int s = 0; s < nS; s++) {
+          const int sp = LI(Pp->L.o_mspec, k * nS + s);
+          if (sp >= 0) lnQK += LD(Pp->L.o_mnu, k * nS + s) * (sp == j ? SD(S.xb, j) : SD(S.la, sp));
+        }
+        const double nur = live ? LD(Pp->L.o_mnuG, k * G + r) : 0.0;
+        rp += nur * (mineral_rate(Pp, S, k, lnQK) * vol);
+      }
+      double pert;
+      {
+#pragma clang fp contract(off)
+        pert = SD(S.mv, j) * 1.0e-5;
+      }
+      double d = (rp - ro) / pert;
+      if (fabs(d) < 1.0e-40) d = 0.0;
+      if (live) row[j] += d;
+    }
+  }
+  wave_sync();
+}
+
 template <int G, int MODE>
 __device__ __forceinline__ void react_body(uint64_t praw) {
   const KP P0 = (KP)praw;
@@ -684,6 +752,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         scale = psv * (m + absum) / dt + fabs(Sr) * vol / dt + fabs(fixed) / dt;
       }
       // kinetic minerals (group-uniform rates, row-local residual/Jacobian)
+      const bool fd_jac = MODE == PFX_MODE_EVAL && L.num_jac;
       for (int k = 0; k < K; k++) {
         double lnQK = SD(S.mlnK, k);
         for (int s = 0; s < L.S; s++) {
@@ -717,8 +786,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
               const double dd = 1.0 + (1.0 - aff) / lim;
               f = f * (1.0 + QK / lim / dd) / dd;
             }
+            if (!fd_jac) {
 #pragma unroll
-            for (int kk = 0; kk < G; kk++) row[kk] += f * LD(L.o_mnuG, k * G + kk) * SD(S.im, kk);
+              for (int kk = 0; kk < G; kk++) row[kk] += f * LD(L.o_mnuG, k * G + kk) * SD(S.im, kk);
+            }
           } else if (r == 0) {
             SD(S.mrate, k) = 0.0;
           }
@@ -726,6 +797,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           SD(S.mrate, k) = 0.0;
         }
       }
+      if (MODE == PFX_MODE_EVAL && fd_jac && K > 0) mineral_fd_jacobian<G>(Pp, S, r, live, m, lng, vol, row);
       rr = gmax<G>(live ? fabs(Rr) / fmax(scale, 1e-300) : 0.0, lane);
       if (MODE == PFX_MODE_EVAL) {
         finish = true;

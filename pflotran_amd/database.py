@@ -312,6 +312,7 @@ class ReactionTables:
     use_log_formulation: int = 1
     act_freq: int = abi.PFX_ACT_OFF
     act_alg: int = abi.PFX_ACT_ALG_LAG
+    numerical_jacobian: int = 0
     max_dlnC: float = 5.0
     max_rel_change_tol: float = 1e-6
     max_residual_tol: float = 1e-12
@@ -370,6 +371,7 @@ class ReactionTables:
         d.use_log_formulation = self.use_log_formulation
         d.act_coef_update_frequency = self.act_freq
         d.act_coef_update_algorithm = self.act_alg
+        d.numerical_jacobian = self.numerical_jacobian
         d.max_dlnC = self.max_dlnC
         d.max_relative_change_tolerance = self.max_rel_change_tol
         d.max_residual_tolerance = self.max_residual_tol

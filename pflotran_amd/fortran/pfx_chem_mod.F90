@@ -43,7 +43,7 @@ module pfx_chem_module
     integer(c_int32_t) :: use_log_formulation
     integer(c_int32_t) :: act_coef_update_frequency
     integer(c_int32_t) :: act_coef_update_algorithm
-    integer(c_int32_t) :: reserved0
+    integer(c_int32_t) :: numerical_jacobian
     real(c_double) :: max_dlnC
     real(c_double) :: max_relative_change_tolerance
     real(c_double) :: max_residual_tolerance

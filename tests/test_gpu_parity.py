@@ -26,6 +26,30 @@ def net15():
     return synth.random_network(15, 50, 3, ionx=True)
 
 
+def _jacobian_bound(t, cells, jac_o):
+    """Per-entry bound of the analytic Jacobian's rounding: 1e-12 of the
+    magnitude of the terms summed into the entry (the complex terms
+    |nu_r nu_k| C_c / m_k cancel in off-diagonal entries, so either
+    implementation's rounding is relative to that sum, not to |J|); the
+    exchanger rows additionally carry the inner Newton's own 1e-12 relative
+    tolerance on KDj (reaction.F90:4407): 2e-12 of their row's largest entry.
+    cells must hold sec_molal at the state."""
+    m = cells["pri_molal"]
+    sec = cells["sec_molal"][:, :t.M]
+    absnu = np.zeros((t.M, t.N))
+    for i in range(t.M):
+        for s_ in range(t.cplx_specid[i, 0]):
+            absnu[i, t.cplx_specid[i, 1 + s_]] += abs(t.cplx_stoich[i, s_])
+    psvd = 0.25 * 1.0 * 1000.0 * 1.0 / synth.STEP_DT * (synth.DEN_25C * 1e-3)
+    terms = psvd * (np.eye(t.N)[None] + np.einsum("ci,ck,nc->nik", absnu, absnu, sec) / m[:, None, :])
+    bound = 1e-12 * (terms + np.abs(jac_o))
+    cation = np.zeros(t.N, dtype=bool)
+    cation[t.ionx_cationid[0, 1:1 + t.ionx_cationid[0, 0]]] = True
+    rowmax = np.max(np.abs(jac_o), axis=2, keepdims=True)
+    bound[:, cation, :] = np.broadcast_to(2e-12 * rowmax, bound.shape)[:, cation, :]
+    return bound
+
+
 def test_residual_jacobian_parity(net15):
     t = net15
     n = 333
@@ -34,30 +58,47 @@ def test_residual_jacobian_parity(net15):
     o = synth.react_opts()
     # evaluate at a perturbed iterate so kinetics and exchange are far from balance
     cells["pri_molal"][:] *= np.exp(0.3 * np.sin(np.arange(n * t.N)).reshape(n, t.N))
-    res_o, jac_o = orc.residual_jacobian(t, cells, o)
+    res_o, jac_o = orc.residual_jacobian(t, cells, o)  # also fills sec_molal at the state
     tab = runtime.Tables(t)
     dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
     res_g, jac_g = runtime.residual_jacobian(tab, dc, synth.react_opts())
-    # error bound per entry: the magnitude of the terms summed into it (the
-    # complex terms |nu_r nu_k| C_c / m_k cancel in off-diagonal entries, so
-    # either implementation's rounding is relative to that sum, not to |J|)
-    m = cells["pri_molal"]
-    _, _ = orc.residual_jacobian(t, cells, o)  # fills sec_molal at the state
-    sec = cells["sec_molal"][:, :t.M]
-    absnu = np.zeros((t.M, t.N))
-    for i in range(t.M):
-        for s_ in range(t.cplx_specid[i, 0]):
-            absnu[i, t.cplx_specid[i, 1 + s_]] += abs(t.cplx_stoich[i, s_])
-    psvd = 0.25 * 1.0 * 1000.0 * 1.0 / synth.STEP_DT * (synth.DEN_25C * 1e-3)
-    terms = psvd * (np.eye(t.N)[None] + np.einsum("ci,ck,nc->nik", absnu, absnu, sec) / m[:, None, :])
-    err = np.abs(jac_g - jac_o) / (terms + np.abs(jac_o) + 1e-300)
-    # exchanger rows additionally carry the inner Newton's own 1e-12 relative
-    # tolerance on KDj (reaction.F90:4407): compared against their row scale
-    cation = np.zeros(t.N, dtype=bool)
-    cation[t.ionx_cationid[0, 1:1 + t.ionx_cationid[0, 0]]] = True
-    assert np.max(err[:, ~cation, :]) <= 1e-12
-    rowmax = np.max(np.abs(jac_o), axis=2, keepdims=True)
-    assert np.max((np.abs(jac_g - jac_o) / rowmax)[:, cation, :]) <= 2e-12
+    assert np.all(np.abs(jac_g - jac_o) <= _jacobian_bound(t, cells, jac_o))
+    psv = 0.25 * 1.0 * 1000.0 * 1.0
+    acc = psv * np.abs(cells["total"]) / synth.STEP_DT
+    assert np.max(np.abs(res_g - res_o) / (acc + np.abs(res_o) + 1e-300)) <= 1e-12
+
+
+def test_residual_jacobian_parity_numerical():
+    # NUMERICAL_JACOBIAN: the reaction block by forward differences
+    # (reaction.F90:3625-3684).  Each implementation's rounding of the
+    # residual sums, a few ulp of S_i = sum_k |nu_ik Im_k| V, is divided by the
+    # perturbation 1e-5 m_j: entries are held to 1e-10 S_i / m_j (about 4 ulp
+    # each side) plus 1e-8 of the differenced column
+    t = synth.random_network(15, 50, 3, ionx=True)
+    t.numerical_jacobian = 1
+    n = 200
+    cells, _, _, ok = speciated_state(t, n)
+    cells["pri_molal"][:] *= np.exp(0.3 * np.sin(np.arange(n * t.N)).reshape(n, t.N))
+    o = synth.react_opts()
+    res_o, jac_o = orc.residual_jacobian(t, cells, o)
+    bare = cells.copy()
+    bare["mnrl_area"][:] = 0.0
+    _, jac_0 = orc.residual_jacobian(t, bare, o)
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    res_g, jac_g = runtime.residual_jacobian(tab, dc, synth.react_opts())
+    fdscale = np.max(np.abs(jac_o - jac_0), axis=1, keepdims=True)
+    assert np.count_nonzero(fdscale) > 0
+    _, _ = orc.residual_jacobian(t, cells, o)  # sec_molal and mineral rates at the state
+    nu = np.zeros((t.K, t.N))
+    for k in range(t.K):
+        for s_ in range(t.mnrl_specid[k, 0]):
+            nu[k, t.mnrl_specid[k, 1 + s_]] = t.mnrl_stoich[k, s_]
+    S = np.abs(cells["mnrl_rate"]) @ np.abs(nu)               # [cell][row], V = 1 m^3
+    noise = 1e-10 * S[:, :, None] / cells["pri_molal"][:, None, :]
+    excess = np.abs(jac_g - jac_o) - _jacobian_bound(t, cells, jac_o) - noise - 1e-8 * fdscale
+    worst = np.unravel_index(np.argmax(excess), excess.shape)
+    assert excess[worst] <= 0.0, (worst, jac_g[worst], jac_o[worst], fdscale[worst[0], 0, worst[2]])
     psv = 0.25 * 1.0 * 1000.0 * 1.0
     acc = psv * np.abs(cells["total"]) / synth.STEP_DT
     assert np.max(np.abs(res_g - res_o) / (acc + np.abs(res_o) + 1e-300)) <= 1e-12

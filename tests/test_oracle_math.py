@@ -360,3 +360,43 @@ def test_newton_activity_with_neutral_complexes_is_the_lag_pass():
     I = 0.5 * np.sum(m * t.Z ** 2)
     assert np.allclose(out[1][0], np.where(np.abs(t.Z) > 0, 10.0 ** _dh_log10(t.Z, t.a0, I, A, B, Bd), 1.0),
                        rtol=1e-12)
+
+
+# ---------------------------------------------------------------- numerical reaction Jacobian
+def test_numerical_reaction_jacobian_matches_the_analytic_one():
+    """NUMERICAL_JACOBIAN (reaction.F90:3625-3684): forward differences with a
+    1e-5 relative perturbation agree with the analytic mineral Jacobian to
+    the difference's truncation error; columns of species in no mineral
+    are untouched bit for bit; the residual is the analytic one.  The
+    differences carry no den/1000 factor: at another water density the
+    analytic mineral block is the differenced one times den/1000
+    (reaction_mineral.F90:866-870)."""
+    t = synth.random_network(15, 50, 3, ionx=False)
+    rng = np.random.default_rng(5)
+    cells = orc.HostCells.allocate(t, 1, temperature=True)
+    cells["temperature"][:] = 31.0
+    cells["pri_molal"][0] = 1e-4 * 10.0 ** rng.uniform(-0.3, 0.3, size=t.N)
+    cells["pri_act_coef"][0] = rng.uniform(0.7, 1.0, size=t.N)
+    cells["mnrl_volfrac"][:] = 0.05
+    cells["mnrl_area"][:] = 1.0
+    in_mineral = np.zeros(t.N, dtype=bool)
+    for k in range(t.K):
+        in_mineral[t.mnrl_specid[k, 1:1 + t.mnrl_specid[k, 0]]] = True
+    assert (~in_mineral).any()
+    for den in (1000.0, 997.16):
+        o = synth.react_opts(den_kg=den)
+        t.numerical_jacobian = 0
+        Ra, Ja = orc.residual_jacobian(t, cells, o)
+        bare = cells.copy()
+        bare["mnrl_area"][:] = 0.0
+        _, J0 = orc.residual_jacobian(t, bare, o)
+        t.numerical_jacobian = 1
+        Rn, Jn = orc.residual_jacobian(t, cells, o)
+        t.numerical_jacobian = 0
+        assert np.array_equal(Rn, Ra)
+        assert np.array_equal(Jn[0][:, ~in_mineral], Ja[0][:, ~in_mineral])
+        an = (Ja[0] - J0[0]) * (1000.0 / den)
+        fd = Jn[0] - J0[0]
+        scale = np.max(np.abs(an), axis=0)
+        assert np.all(scale[in_mineral] > 0)
+        assert np.max(np.abs(fd - an)[:, in_mineral] / scale[in_mineral]) <= 1e-4
