@@ -72,6 +72,9 @@ extern "C" {
 #define PFX_CONSTRAINT_PH    3   /* pH (component must be H+)              */
 #define PFX_CONSTRAINT_CHARGE 4  /* charge balance: sum z_j T_j = 0; the value
                                     is the starting free concentration      */
+#define PFX_CONSTRAINT_TOTAL_SORB 5 /* sorbed (ion-exchange) total, mol/m^3 bulk
+                                       (reaction.F90:1490-1492, 1608-1614);
+                                       needs an exchange site                */
 
 /* initial Newton iterate */
 #define PFX_GUESS_PREVIOUS  0   /* previous free ions (RReact's guess, reaction.F90:3370) */

@@ -23,6 +23,7 @@ PFX_ACT_ALG_LAG, PFX_ACT_ALG_NEWTON = 0, 1
 PFX_CELL_CONVERGED, PFX_CELL_MAX_ITERATIONS, PFX_CELL_NONFINITE, PFX_CELL_SINGULAR = 0, 1, 2, 3
 PFX_GUESS_PREVIOUS, PFX_GUESS_ARRAY, PFX_GUESS_DEFAULT, PFX_GUESS_ARRAY_FALLBACK = 0, 1, 2, 3
 PFX_CONSTRAINT_TOTAL, PFX_CONSTRAINT_FREE, PFX_CONSTRAINT_LOG, PFX_CONSTRAINT_PH, PFX_CONSTRAINT_CHARGE = 0, 1, 2, 3, 4
+PFX_CONSTRAINT_TOTAL_SORB = 5
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

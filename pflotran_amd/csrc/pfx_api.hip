@@ -820,7 +820,8 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
   int nph = 0;
   for (int j = 0; j < t->L.N; j++) {
     const int c = constraint_type[j];
-    if (c < PFX_CONSTRAINT_TOTAL || c > PFX_CONSTRAINT_CHARGE) return PFX_E_UNSUP;
+    if (c < PFX_CONSTRAINT_TOTAL || c > PFX_CONSTRAINT_TOTAL_SORB) return PFX_E_UNSUP;
+    if (c == PFX_CONSTRAINT_TOTAL_SORB && t->L.R == 0) return PFX_E_ARG;
     nph += c == PFX_CONSTRAINT_PH;
   }
   if (nph > 1) return PFX_E_ARG;
@@ -833,6 +834,8 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
   for (int j = 0; j < t->L.N; j++) P.ctype[j] = constraint_type[j];
   P.has_charge = 0;
   for (int j = 0; j < t->L.N; j++) P.has_charge |= constraint_type[j] == PFX_CONSTRAINT_CHARGE;
+  P.has_sorb = 0;
+  for (int j = 0; j < t->L.N; j++) P.has_sorb |= constraint_type[j] == PFX_CONSTRAINT_TOTAL_SORB;
   P.molal = molal;
   if (P.max_iterations <= 0) P.max_iterations = 10000;
   int rc = use_scratch(t, P, cells->ncells);

@@ -392,6 +392,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             free_conc = 1.0e-9;
             tcon = 0.0;
             if (ctype == PFX_CONSTRAINT_TOTAL) tcon = conc * molal_to_molar;
+            else if (ctype == PFX_CONSTRAINT_TOTAL_SORB) tcon = conc;  // mol/m^3 bulk (reaction.F90:1490-1492)
             else if (ctype == PFX_CONSTRAINT_FREE) free_conc = conc * molar_to_molal;
             else if (ctype == PFX_CONSTRAINT_LOG) free_conc = pow(10.0, conc) * molar_to_molal;
             else if (ctype == PFX_CONSTRAINT_PH) { free_conc = pow(10.0, -conc); tcon = conc; }
@@ -670,6 +671,11 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     const bool need_ionx = (MODE == PFX_MODE_REACT || MODE == PFX_MODE_EVAL) ? true : false;
     const bool ionx_final = (MODE == PFX_MODE_AUX || MODE == PFX_MODE_EQUIL);
     double Sr = 0.0;
+    // d(sorbed total of row r)/dm for a total-sorbed constraint row
+    // (reaction.F90:1608-1614); only the constraint kernel keeps it
+    double srow[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) srow[k] = 0.0;
     const double dt = P.dt;
     const double psv_t = porosity * sat * 1000.0 * vol;
     if (active && R > 0 && (need_ionx || ionx_final)) {
@@ -727,6 +733,16 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             const double Xk = SD(S.xb, k);
             const double d = (k == r) ? t1 * (1.0 - (t2 * Xk)) : (-t1) * t2 * Xk;
             row[k] += d * SD(S.im, k) * wscale;
+          }
+          wave_sync();
+        } else if (MODE == PFX_MODE_EQUIL && P.has_sorb) {
+          SD(S.xb, r) = X;
+          wave_sync();
+#pragma unroll
+          for (int k = 0; k < G; k++) {
+            const double Xk = SD(S.xb, k);
+            const double d = (k == r) ? t1 * (1.0 - (t2 * Xk)) : (-t1) * t2 * Xk;
+            srow[k] += d * SD(S.im, k);
           }
           wave_sync();
         }
@@ -846,6 +862,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       }
       if (ctype == PFX_CONSTRAINT_TOTAL) {
         Rr = tot_out - tcon;
+      } else if (ctype == PFX_CONSTRAINT_TOTAL_SORB) {
+        Rr = sorb_out - tcon;
+#pragma unroll
+        for (int k = 0; k < G; k++) row[k] = srow[k];
       } else if (ctype == PFX_CONSTRAINT_CHARGE) {
         // row set above
       } else {

@@ -22,7 +22,7 @@ module pfx_chem_module
   integer(c_int), parameter, public :: PFX_CONSTRAINT_TOTAL = 0, &
                                        PFX_CONSTRAINT_FREE = 1, &
                                        PFX_CONSTRAINT_LOG = 2, PFX_CONSTRAINT_PH = 3, &
-                                       PFX_CONSTRAINT_CHARGE = 4
+                                       PFX_CONSTRAINT_CHARGE = 4, PFX_CONSTRAINT_TOTAL_SORB = 5
   integer(c_int), parameter, public :: PFX_GUESS_PREVIOUS = 0, PFX_GUESS_ARRAY = 1, &
                                        PFX_GUESS_DEFAULT = 2, PFX_GUESS_ARRAY_FALLBACK = 3
 

@@ -7,7 +7,7 @@ from pflotran_amd.database import IonExchangeRxn, MineralKinetics, build_tables,
 
 DB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "db")
 
-CON_TOTAL, CON_FREE, CON_LOG, CON_PH, CON_CHARGE = 0, 1, 2, 3, 4
+CON_TOTAL, CON_FREE, CON_LOG, CON_PH, CON_CHARGE, CON_TOTAL_SORB = 0, 1, 2, 3, 4, 5
 
 
 def carbonate(activity):
@@ -45,6 +45,17 @@ def valocchi():
                      [IonExchangeRxn([("Na+", 1.0), ("Ca++", 3.38638672536), ("Mg++", 6.00240096038)], 750.0)])
     return t, dict(ctype=[CON_TOTAL, CON_TOTAL, CON_TOTAL, CON_CHARGE], conc=[8.65e-2, 1.82e-2, 1.11e-2, 2e-3],
                    porosity=0.25, vf=1e-5, area=100.0)
+
+
+def valocchi_sorbed(golds_sorbed):
+    """ion-exchange-valocchi with Ca++ and Mg++ constrained by their sorbed
+    totals (mol/m^3 bulk; the reference's TOTAL_SORB constraint) instead of
+    their aqueous totals.  The three sorbed totals alone are not independent
+    (sum z S = CEC), so Na+ keeps its aqueous total."""
+    t, setup = valocchi()
+    setup = dict(setup, ctype=[CON_TOTAL, CON_TOTAL_SORB, CON_TOTAL_SORB, CON_CHARGE],
+                 conc=[setup["conc"][0], golds_sorbed[0], golds_sorbed[1], setup["conc"][3]])
+    return t, setup
 
 
 def carbonate_column():

@@ -294,3 +294,23 @@ def test_gpu_gold_valocchi_charge_balance(golds):
         assert abs(out["total_sorb"][0][j] - g_s) <= 1e-12 * max(abs(g_s), 1.0)
     # charge balance holds at the solution: sum z_j T_j = 0
     assert abs(np.dot(t.Z, out["total"][0])) < 1e-11
+
+
+def test_gpu_sorbed_total_constraints(golds):
+    """Ca++ and Mg++ constrained by the valocchi gold's sorbed totals on the
+    GPU: the gold's state again, and the oracle's solution of the same rows."""
+    from chemdefs import valocchi_sorbed
+    name = "ion-exchange-valocchi"
+    sorbed = [gold(golds, name, f"CONCENTRATION: Total Sorbed {sp}") for sp in ("Ca++", "Mg++")]
+    t, setup = valocchi_sorbed(sorbed)
+    tab, dc = _gpu_speciate(t, setup)
+    out = dc.to_host()
+    for j, sp in enumerate(["Na+", "Ca++", "Mg++", "Cl-"]):
+        g_t = gold(golds, name, f"CONCENTRATION: Total {sp}")
+        assert abs(out["total"][0][j] - g_t) <= 1e-9 * abs(g_t)
+    ref = orc.HostCells.allocate(t, 1)
+    ref["mnrl_volfrac"][0, 0] = setup["vf"]
+    ref["mnrl_area"][0, 0] = setup["area"]
+    rc, _ = orc.equilibrate(t, setup["ctype"], np.array(setup["conc"]), ref, synth.react_opts(porosity=setup["porosity"]))
+    assert rc == 0
+    assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from chemdefs import calcite_kinetics, ca_carbonate, carbonate, valocchi
+from chemdefs import calcite_kinetics, ca_carbonate, carbonate, valocchi, valocchi_sorbed
 from conftest import gold
 from pflotran_amd import abi
 from pflotran_amd.database import water_density_ifc67
@@ -65,6 +65,23 @@ def test_valocchi_ion_exchange(golds):
     # charge invariant of the exchanger: sum z_i S_i = CEC
     Z = t.Z
     assert abs(np.dot(Z, cells["total_sorb"][0]) - 750.0) < 1e-10
+
+
+def test_sorbed_total_constraints_recover_the_valocchi_state(golds):
+    """Sorbed-total constraint rows (reaction.F90:1490-1492, 1608-1614): with
+    Ca++ and Mg++ constrained by the gold's sorbed totals, the solution is the
+    gold's state again (aqueous totals, and the sorbed totals themselves);
+    the sorbed rows' Jacobian is the exchanger's dS/dm, so a wrong row would
+    stall or land elsewhere."""
+    name = "ion-exchange-valocchi"
+    sorbed = [gold(golds, name, f"CONCENTRATION: Total Sorbed {sp}") for sp in ("Ca++", "Mg++")]
+    t, setup = valocchi_sorbed(sorbed)
+    cells, _ = speciate(t, setup)
+    for j, sp in enumerate(["Na+", "Ca++", "Mg++", "Cl-"]):
+        g_t = gold(golds, name, f"CONCENTRATION: Total {sp}")
+        assert abs(cells["total"][0][j] - g_t) <= 1e-9 * abs(g_t)
+        g_s = gold(golds, name, f"CONCENTRATION: Total Sorbed {sp}")
+        assert abs(cells["total_sorb"][0][j] - g_s) <= 1e-9 * max(abs(g_s), 1.0)
 
 
 def test_calcite_kinetics(golds):
