@@ -137,9 +137,10 @@ extern __shared__ __align__(16) unsigned char smem[];
 // 5-term log K(T) (reaction_aux.F90:1461-1488) evaluated without fused
 // multiply-add, in the reference's order: the fitted coefficients cancel to
 // ~1e-3 of their terms, so contraction alone would move K by ~1e-12.
-__device__ __forceinline__ double logk_5term(int cfo, double tk) {
+// ln_tk = log(tk), computed once per cell.
+__device__ __forceinline__ double logk_5term(int cfo, double tk, double ln_tk) {
 #pragma clang fp contract(off)
-  double v = LD(cfo, 0) * log(tk);
+  double v = LD(cfo, 0) * ln_tk;
   v = v + LD(cfo, 1);
   v = v + LD(cfo, 2) * tk;
   v = v + LD(cfo, 3) / tk;
@@ -382,6 +383,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           den = P.den_kg ? P.den_kg[c] : P.def_den;
           const double tempc = P.temperature ? P.temperature[c] : 25.0;
           const double tk = tempc + 273.15;
+          const double ln_tk = log(tk);
           if (MODE == PFX_MODE_EQUIL) {
             gam = 1.0;
             for (int cc = r; cc < M; cc += G) SD(S.gsec, cc) = 1.0;
@@ -418,7 +420,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             double lk = LD(L.o_clogK, cc);
             if (P.temperature && L.has_logk_coef) {
               const int cfo = (int)L.o_clogKcoef + 5 * cc * 8;
-              lk = logk_5term(cfo, tk);
+              lk = logk_5term(cfo, tk, ln_tk);
             }
             SD(S.lnK, cc) = -lk * PFX_LOG_TO_LN;
           }
@@ -426,7 +428,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             double lk = LD(L.o_mlogK, k);
             if (P.temperature && L.has_mnrl_coef) {
               const int cfo = (int)L.o_mlogKcoef + 5 * k * 8;
-              lk = logk_5term(cfo, tk);
+              lk = logk_5term(cfo, tk, ln_tk);
             }
             SD(S.mlnK, k) = -lk * PFX_LOG_TO_LN;
             double arr = 1.0;
@@ -609,10 +611,12 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     wave_sync();
 
     // phase B: totals, |contributions|, and row r of dT/dm (kg/L)
+    // (not needed at the previous state, whose complexes only feed the
+    // activity update)
     double row[G];
     double Tsum = 0.0, absum = 0.0;
     const double den_kg_per_L = den * 1.0e-3;
-    if (active) {
+    if (active && !(MODE == PFX_MODE_REACT && phase == 0)) {
       double diag = 0.0;
       const int maxdeg = L.maxdeg;
 #pragma unroll 2
