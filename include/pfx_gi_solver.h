@@ -1,0 +1,135 @@
+/*
+ * pfx_gi_solver.h -- the global-implicit time step closed on the device:
+ * Newton's method on the residual/Jacobian of pfx_gi_residual_jacobian with
+ * a preconditioned BiCGStab linear solve, so a transport step needs no host
+ * round trip of the 7-point block system.
+ *
+ * Reference interfaces replaced (src/pflotran/):
+ *   SNESSolve of the transport process model   timestepper_BE.F90:326
+ *     (PETSc SNES NEWTONLS, basic line search, SNESConvergedDefault through
+ *      ConvergenceTest, convergence.F90:63-200)
+ *   KSPBCGS, the transport linear solver       solver.F90:161, 237
+ *   PMRTCheckUpdatePre (max_dlnC clip, or the
+ *     0.99 min-ratio scaling in linear form)   pm_rt.F90:734-840
+ * PETSc itself is not part of the reference tree (pinned commit 1a9d3c3c50);
+ * its published algorithms are restated here: SNESConvergedDefault's order
+ * of tests and KSPBCGS (van der Vorst's BiCGStab).  The preconditioner is the
+ * point-block Jacobi of the N x N diagonal blocks (PETSc PCPBJACOBI), not
+ * PETSc's serial default ILU(0): it is exact to set up in parallel and costs
+ * one block product per application; ILU(0)'s triangular solves serialise
+ * along the grid's hyperplanes.
+ *
+ * All pointers are DEVICE pointers unless marked (host).  Return codes as in
+ * pfx_chem.h (0 success, <0 PFX_E_*); solver outcomes are in the stats.
+ */
+#ifndef PFX_GI_SOLVER_H
+#define PFX_GI_SOLVER_H
+
+#include "pfx_chem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* KSPSetTolerances meanings (solver.F90:237); zeros select PETSc's defaults
+   rtol 1e-5, atol 1e-50, dtol 1e5, max_iterations 10000.  Convergence:
+   ||r||_2 <= max(rtol ||b||_2, atol), r the true (unpreconditioned)
+   residual, the preconditioner applied on the right. */
+typedef struct pfx_krylov_opts {
+  double rtol;
+  double atol;
+  double dtol;
+  int32_t max_iterations;
+  int32_t reserved;
+} pfx_krylov_opts;
+
+/* KSPConvergedReason values */
+#define PFX_KSP_CONVERGED_RTOL       2
+#define PFX_KSP_CONVERGED_ATOL       3
+#define PFX_KSP_DIVERGED_ITS       (-3)
+#define PFX_KSP_DIVERGED_DTOL      (-4)
+#define PFX_KSP_DIVERGED_BREAKDOWN (-5)
+#define PFX_KSP_DIVERGED_NANORINF  (-9)
+#define PFX_KSP_DIVERGED_PC_FAILED (-11)  /* a singular diagonal block */
+
+typedef struct pfx_krylov_stats {
+  int32_t iterations;
+  int32_t reason;         /* PFX_KSP_* */
+  double rnorm0;          /* ||b||_2 */
+  double rnorm;           /* final ||r||_2 */
+  double ms;              /* device time of the solve (events) */
+} pfx_krylov_stats;
+
+/* A block-sparse (BSR) system with column-major N x N blocks, the layout of
+   pfx_gi_structure / PETSc BAIJ (MAT_ROW_ORIENTED = FALSE); the handle owns
+   the solver's work vectors and the inverted diagonal blocks.  row_ptr
+   [nrows+1] and col_idx [nnzb] (device, int32) must stay valid while the
+   handle lives; every row must hold its diagonal block. */
+typedef struct pfx_bsr pfx_bsr;
+int pfx_bsr_create(int64_t nrows, int32_t N, int64_t nnzb, const int32_t *row_ptr, const int32_t *col_idx,
+                   pfx_bsr **out);
+int pfx_bsr_destroy(pfx_bsr *h);
+
+/* y = A x (values [nnzb][N][N] column-major). */
+int pfx_bsr_spmv(pfx_bsr *h, const double *values, const double *x, double *y, void *stream);
+
+/* Solve A x = b by BiCGStab with point-block Jacobi preconditioning from
+   x = 0.  The diagonal blocks are inverted (partial pivoting) first. */
+int pfx_bsr_solve(pfx_bsr *h, const double *values, const double *b, double *x, const pfx_krylov_opts *opts,
+                  void *stream, pfx_krylov_stats *stats);
+
+/* SNESSetTolerances meanings (solver.F90:256); zeros select PETSc's
+   defaults atol 1e-50, rtol 1e-8, stol 1e-8, max_iterations 50. */
+typedef struct pfx_newton_opts {
+  double atol;
+  double rtol;
+  double stol;
+  int32_t max_iterations;
+  int32_t reserved;
+  pfx_krylov_opts krylov;
+} pfx_newton_opts;
+
+/* SNESConvergedReason values */
+#define PFX_SNES_CONVERGED_FNORM_ABS        2
+#define PFX_SNES_CONVERGED_FNORM_RELATIVE   3
+#define PFX_SNES_CONVERGED_SNORM_RELATIVE   4
+#define PFX_SNES_DIVERGED_LINEAR_SOLVE    (-3)
+#define PFX_SNES_DIVERGED_FNORM_NAN       (-4)
+#define PFX_SNES_DIVERGED_MAX_IT          (-5)
+
+typedef struct pfx_newton_stats {
+  int32_t iterations;         /* Newton updates taken */
+  int32_t reason;             /* PFX_SNES_* */
+  int32_t linear_iterations;  /* BiCGStab iterations, all Newton steps */
+  int32_t reserved;
+  double fnorm0, fnorm;       /* ||F||_2 at the start and at the end */
+  double ms_assembly;         /* device time in residual/Jacobian evaluations */
+  double ms_linear;           /* device time in the linear solves */
+  double ms_total;            /* host wall time of the step */
+} pfx_newton_stats;
+
+/* One global-implicit time step (the SNESSolve of a transport step).
+   The grid's BSR structure and matrix are held by the handle; create it
+   once per grid shape and network. */
+typedef struct pfx_gi_solver pfx_gi_solver;
+int pfx_gi_solver_create(pfx_tables *tables, const pfx_grid *grid, pfx_gi_solver **out);
+int pfx_gi_solver_destroy(pfx_gi_solver *s);
+
+/* Newton from the free ions in cells->pri_molal (in: initial iterate, the
+   old state as timestepper_BE starts it; out: the last iterate).
+   cells->total / total_sorb hold the OLD state's accumulation, as for
+   pfx_gi_residual_jacobian; activity coefficients are held fixed (the
+   once-per-step update).  Each iteration: F, J at the iterate; J y = F by
+   pfx_bsr_solve; y clipped as PMRTCheckUpdatePre does (|y| <= max_dlnC in
+   log form, else scaled by 0.99 min(m / y) where that is below 1); x -= y
+   (m <- m exp(-y) in log form); tests in SNESConvergedDefault's order:
+   NaN, ||F|| < atol, ||F|| <= rtol ||F0||, ||y|| < stol ||x||, the
+   iteration limit.  Totals at the solution come from pfx_auxvar_batch. */
+int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *grid, const pfx_react_opts *opts,
+                       const pfx_newton_opts *nopts, pfx_newton_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFX_GI_SOLVER_H */

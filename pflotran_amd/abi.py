@@ -107,6 +107,39 @@ class ZonePlan(C.Structure):
     ]
 
 
+class KrylovOpts(C.Structure):
+    """pfx_krylov_opts (include/pfx_gi_solver.h); zeros = PETSc defaults."""
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
+                ("max_iterations", C.c_int32), ("reserved", C.c_int32)]
+
+
+class KrylovStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("reason", C.c_int32), ("rnorm0", C.c_double), ("rnorm", C.c_double),
+                ("ms", C.c_double)]
+
+
+class NewtonOpts(C.Structure):
+    """pfx_newton_opts; zeros = PETSc SNES defaults."""
+    _fields_ = [("atol", C.c_double), ("rtol", C.c_double), ("stol", C.c_double),
+                ("max_iterations", C.c_int32), ("reserved", C.c_int32), ("krylov", KrylovOpts)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("reason", C.c_int32), ("linear_iterations", C.c_int32),
+                ("reserved", C.c_int32), ("fnorm0", C.c_double), ("fnorm", C.c_double),
+                ("ms_assembly", C.c_double), ("ms_linear", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+PFX_KSP_CONVERGED_RTOL, PFX_KSP_CONVERGED_ATOL = 2, 3
+PFX_KSP_DIVERGED_ITS, PFX_KSP_DIVERGED_DTOL, PFX_KSP_DIVERGED_BREAKDOWN = -3, -4, -5
+PFX_KSP_DIVERGED_NANORINF, PFX_KSP_DIVERGED_PC_FAILED = -9, -11
+PFX_SNES_CONVERGED_FNORM_ABS, PFX_SNES_CONVERGED_FNORM_RELATIVE, PFX_SNES_CONVERGED_SNORM_RELATIVE = 2, 3, 4
+PFX_SNES_DIVERGED_LINEAR_SOLVE, PFX_SNES_DIVERGED_FNORM_NAN, PFX_SNES_DIVERGED_MAX_IT = -3, -4, -5
+
+
 CELL_FIELDS = [f for f, _ in Cells._fields_ if f != "ncells"]
 
 

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpfx_chem.so")
-SOURCES = ["pfx_react.hip", "pfx_api.hip", "pfx_mlp.hip", "pfx_gi.hip"]
+SOURCES = ["pfx_react.hip", "pfx_api.hip", "pfx_mlp.hip", "pfx_gi.hip", "pfx_krylov.hip"]
 ARCH = os.environ.get("PFX_OFFLOAD_ARCH", "gfx950")
 
 

@@ -514,6 +514,8 @@ int pfx_internal::timed_launch_end(pfx_tables *t, hipStream_t s, int slot) {
   return 0;
 }
 
+void pfx_internal::table_max_dlnC(const pfx_tables *t, double *max_dlnC) { *max_dlnC = t->L.max_dlnC; }
+
 void pfx_internal::table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R) {
   *N = t->L.N;
   *M = t->L.M;

@@ -100,6 +100,7 @@ struct pfx_kernel_params {
 // for the other translation units of the library.
 namespace pfx_internal {
 void table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R);
+void table_max_dlnC(const pfx_tables *t, double *max_dlnC);
 int gi_scratch(pfx_tables *t, int64_t ncells, double **rnf, double **jnf, double **tot, double **dtot);
 int gi_evaluate_cells(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, int col_scale, double *rnf,
                       double *jnf, double *tot, double *dtot);

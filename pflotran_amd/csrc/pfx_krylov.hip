@@ -1,0 +1,774 @@
+// Block-sparse kernels, BiCGStab with point-block Jacobi preconditioning, and
+// the global-implicit Newton step on the device (include/pfx_gi_solver.h).
+//
+// Layout: a block row of the BSR matrix is a group of G lanes (G = 8, 16 or
+// 32 >= N); lane r owns row r of the block row, so the N lanes of a group
+// read the N consecutive doubles of one column of a column-major block at a
+// time and the whole block stream is read once, in order.  Every kernel is
+// HBM-bound: the matrix (7 N^2 doubles per block row) dominates the bytes of
+// a Krylov iteration, two products per iteration.
+//
+// Reductions are deterministic: each workgroup writes partial sums, one small
+// kernel adds them in a fixed order, and the host reads the 16-byte result
+// (BiCGStab's scalars are needed on the host to steer the iteration anyway).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "pfx_device.h"
+#include "../../include/pfx_gi_solver.h"
+
+namespace {
+
+#define KCHK(x)                                                               \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess) {                                                   \
+      fprintf(stderr, "pfx_krylov: %s failed: %s (%s:%d)\n", #x,              \
+              hipGetErrorString(e_), __FILE__, __LINE__);                     \
+      return PFX_E_DEVICE;                                                    \
+    }                                                                         \
+  } while (0)
+
+constexpr int BLOCK = 256;
+constexpr int MAX_GRID = 4096;  // workgroups of the grid-stride kernels
+
+// sum of (a, b) and (c, d) over the workgroup into partial[2 * blockIdx.x]
+__device__ __forceinline__ void block_reduce2(double a, double b, double *partial) {
+  __shared__ double sh[2][BLOCK / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = a;
+    sh[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = 0; i < BLOCK / 64; i++) {
+      s0 += sh[0][i];
+      s1 += sh[1][i];
+    }
+    partial[2 * blockIdx.x] = s0;
+    partial[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+// out[0..1] = sums of the nparts partial pairs, in a fixed order
+__global__ __launch_bounds__(BLOCK) void k_reduce2(int nparts, const double *__restrict__ partial,
+                                                   double *__restrict__ out) {
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += BLOCK) {
+    a += partial[2 * i];
+    b += partial[2 * i + 1];
+  }
+  __shared__ double sh[2][BLOCK];
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sh[0][0];
+    out[1] = sh[1][0];
+  }
+}
+
+// min over the workgroup / over the partials (the linear-form update ratio)
+__global__ __launch_bounds__(BLOCK) void k_min_ratio(int64_t n, const double *__restrict__ m,
+                                                     const double *__restrict__ y, double *__restrict__ partial) {
+  double mn = 1.0e20;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const double c = m[i], d = y[i];
+    if (c <= d) mn = fmin(mn, fabs(c / d));
+  }
+  for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o, 64));
+  __shared__ double sh[BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = sh[0];
+    for (int i = 1; i < BLOCK / 64; i++) v = fmin(v, sh[i]);
+    partial[2 * blockIdx.x] = v;
+    partial[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reduce_min(int nparts, const double *__restrict__ partial,
+                                                      double *__restrict__ out) {
+  double v = 1.0e20;
+  for (int i = threadIdx.x; i < nparts; i += BLOCK) v = fmin(v, partial[2 * i]);
+  __shared__ double sh[BLOCK];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sh[0];
+    out[1] = 0.0;
+  }
+}
+
+// (a, b) and (c, d) over n entries; c == nullptr skips the second
+__global__ __launch_bounds__(BLOCK) void k_dot2(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                                const double *__restrict__ c, const double *__restrict__ d,
+                                                double *__restrict__ partial) {
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    s0 += a[i] * b[i];
+    if (c) s1 += c[i] * d[i];
+  }
+  block_reduce2(s0, s1, partial);
+}
+
+// y = A x, optionally with the partial sums of (y, a1) and (y, y)
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_bsr_spmv(int64_t nrows, int N, const int32_t *__restrict__ rp,
+                                                    const int32_t *__restrict__ ci,
+                                                    const double *__restrict__ vals, const double *__restrict__ x,
+                                                    double *__restrict__ y, const double *__restrict__ a1,
+                                                    int self2, double *__restrict__ partial) {
+  const int r = threadIdx.x % G;
+  const int64_t NN = (int64_t)N * N;
+  double d1 = 0.0, d2 = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G; row < nrows;
+       row += (int64_t)gridDim.x * (BLOCK / G)) {
+    if (r < N) {
+      double acc0 = 0.0, acc1 = 0.0;
+      const int b0 = rp[row], b1 = rp[row + 1];
+      for (int bk = b0; bk < b1; bk++) {
+        const double *A = vals + (int64_t)bk * NN + r;
+        const double *xc = x + (int64_t)ci[bk] * N;
+        int k = 0;
+        for (; k + 1 < N; k += 2) {
+          acc0 += __builtin_nontemporal_load(A + (int64_t)k * N) * xc[k];
+          acc1 += __builtin_nontemporal_load(A + (int64_t)(k + 1) * N) * xc[k + 1];
+        }
+        if (k < N) acc0 += __builtin_nontemporal_load(A + (int64_t)k * N) * xc[k];
+      }
+      const double yv = acc0 + acc1;
+      y[row * N + r] = yv;
+      if (a1) d1 += a1[row * N + r] * yv;
+      if (self2) d2 += yv * yv;
+    }
+  }
+  if (partial) block_reduce2(d1, d2, partial);
+}
+
+// group argmax (ties to the larger lane) within G lanes
+template <int G>
+__device__ __forceinline__ void group_argmax(double &v, int &idx) {
+  for (int o = G / 2; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, G);
+    const int j = __shfl_xor(idx, o, G);
+    if (w > v || (w == v && j > idx)) {
+      v = w;
+      idx = j;
+    }
+  }
+}
+
+// Inverse of each diagonal block by Gauss-Jordan with partial pivoting,
+// lane r holding row r of the block and of the growing inverse; the lane
+// that pivots at step j ends with row j of the inverse.  A zero pivot
+// counts in *fail.
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_diag_inverse(int64_t nrows, int N, const int32_t *__restrict__ rp,
+                                                        const int32_t *__restrict__ ci,
+                                                        const double *__restrict__ vals, double *__restrict__ dinv,
+                                                        int *__restrict__ fail) {
+  const int r = threadIdx.x % G;
+  const bool live = r < N;
+  const int64_t NN = (int64_t)N * N;
+  for (int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G; row < nrows;
+       row += (int64_t)gridDim.x * (BLOCK / G)) {
+    int db = -1;
+    for (int bk = rp[row]; bk < rp[row + 1]; bk++)
+      if (ci[bk] == row) db = bk;
+    double a[G], e[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      a[k] = (live && k < N && db >= 0) ? vals[(int64_t)db * NN + (int64_t)k * N + r] : (k == r ? 1.0 : 0.0);
+      e[k] = (k == r) ? 1.0 : 0.0;
+    }
+    bool done = !live;
+    int mystep = -1;
+    bool singular = db < 0;
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      if (j < N) {
+        double cand = done ? -1.0 : fabs(a[j]);
+        int p = r;
+        group_argmax<G>(cand, p);
+        double pa[G], pe[G];
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+          pa[k] = __shfl(a[k], p, G);
+          pe[k] = __shfl(e[k], p, G);
+        }
+        double piv = pa[j];
+        if (piv == 0.0) {
+          singular = true;
+          piv = 1.0;
+        }
+        const double rinv = 1.0 / piv;
+        if (r == p) {
+          done = true;
+          mystep = j;
+#pragma unroll
+          for (int k = 0; k < G; k++) {
+            a[k] = pa[k] * rinv;
+            e[k] = pe[k] * rinv;
+          }
+        } else {
+          const double f = a[j] * rinv;
+#pragma unroll
+          for (int k = 0; k < G; k++) {
+            a[k] -= f * pa[k];
+            e[k] -= f * pe[k];
+          }
+        }
+      }
+    }
+    if (live && mystep >= 0) {
+#pragma unroll
+      for (int k = 0; k < G; k++)
+        if (k < N) dinv[row * NN + (int64_t)k * N + mystep] = e[k];
+    }
+    if (singular && r == 0) atomicAdd(fail, 1);
+  }
+}
+
+// out = D^-1 v for the group's block row, v_r held by lane r
+template <int G>
+__device__ __forceinline__ double apply_dinv(const double *__restrict__ D, int N, int r, double v) {
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const double vk = __shfl(v, k, G);
+    if (k < N && r < N) acc += D[(int64_t)k * N + r] * vk;
+  }
+  return acc;
+}
+
+// BiCGStab start from x = 0: r = rhat = b, p = v = 0; partial (b, b)
+__global__ __launch_bounds__(BLOCK) void k_bcgs_init(int64_t n, const double *__restrict__ b, double *__restrict__ x,
+                                                     double *__restrict__ r, double *__restrict__ rhat,
+                                                     double *__restrict__ p, double *__restrict__ v,
+                                                     double *__restrict__ partial) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const double bi = b[i];
+    x[i] = 0.0;
+    r[i] = bi;
+    rhat[i] = bi;
+    p[i] = 0.0;
+    v[i] = 0.0;
+    s += bi * bi;
+  }
+  block_reduce2(s, 0.0, partial);
+}
+
+// p = r + beta (p - omega v);  phat = D^-1 p
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_bcgs_p(int64_t nrows, int N, const double *__restrict__ r,
+                                                  const double *__restrict__ v, double *__restrict__ p,
+                                                  double beta, double omega, const double *__restrict__ dinv,
+                                                  double *__restrict__ phat) {
+  const int l = threadIdx.x % G;
+  const int64_t NN = (int64_t)N * N;
+  for (int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G; row < nrows;
+       row += (int64_t)gridDim.x * (BLOCK / G)) {
+    const int64_t i = row * N + l;
+    double pv = 0.0;
+    if (l < N) {
+      pv = r[i] + beta * (p[i] - omega * v[i]);
+      p[i] = pv;
+    }
+    const double z = apply_dinv<G>(dinv + row * NN, N, l, pv);
+    if (l < N) phat[i] = z;
+  }
+}
+
+// s = r - alpha v;  shat = D^-1 s;  partial (s, s)
+template <int G>
+__global__ __launch_bounds__(BLOCK) void k_bcgs_s(int64_t nrows, int N, const double *__restrict__ r,
+                                                  const double *__restrict__ v, double alpha,
+                                                  const double *__restrict__ dinv, double *__restrict__ s,
+                                                  double *__restrict__ shat, double *__restrict__ partial) {
+  const int l = threadIdx.x % G;
+  const int64_t NN = (int64_t)N * N;
+  double ss = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G; row < nrows;
+       row += (int64_t)gridDim.x * (BLOCK / G)) {
+    const int64_t i = row * N + l;
+    double sv = 0.0;
+    if (l < N) {
+      sv = r[i] - alpha * v[i];
+      s[i] = sv;
+      ss += sv * sv;
+    }
+    const double z = apply_dinv<G>(dinv + row * NN, N, l, sv);
+    if (l < N) shat[i] = z;
+  }
+  block_reduce2(ss, 0.0, partial);
+}
+
+// x += alpha phat + omega shat;  r = s - omega t;  partial (r, r), (rhat, r)
+__global__ __launch_bounds__(BLOCK) void k_bcgs_xr(int64_t n, double *__restrict__ x, const double *__restrict__ phat,
+                                                   const double *__restrict__ shat, const double *__restrict__ s,
+                                                   const double *__restrict__ t, double alpha, double omega,
+                                                   double *__restrict__ r, const double *__restrict__ rhat,
+                                                   double *__restrict__ partial) {
+  double rr = 0.0, hr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    x[i] += alpha * phat[i] + omega * shat[i];
+    const double rv = s[i] - omega * t[i];
+    r[i] = rv;
+    rr += rv * rv;
+    hr += rhat[i] * rv;
+  }
+  block_reduce2(rr, hr, partial);
+}
+
+__global__ void k_zero_int(int *__restrict__ v) { *v = 0; }
+
+// x += alpha phat (BiCGStab's early exit on a small s)
+__global__ __launch_bounds__(BLOCK) void k_axpy(int64_t n, double alpha, const double *__restrict__ xp,
+                                                double *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    y[i] += alpha * xp[i];
+}
+
+// Newton update with PMRTCheckUpdatePre's change to the step (pm_rt.F90:
+// 734-840): log form y <- sign(y) min(|y|, max_dlnC), m <- m exp(-y); linear
+// form y <- y * scale, m <- m - y.  partial (y, y) and (x, x), x = ln m or m.
+__global__ __launch_bounds__(BLOCK) void k_newton_update(int64_t n, double *__restrict__ m, double *__restrict__ y,
+                                                         int use_log, double max_dlnC, double scale,
+                                                         double *__restrict__ partial) {
+  double yy = 0.0, xx = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    double d = y[i];
+    double mi = m[i];
+    if (use_log) {
+      d = copysign(1.0, d) * fmin(fabs(d), max_dlnC);
+      mi = mi * exp(-d);
+      const double lm = log(mi);
+      xx += lm * lm;
+    } else {
+      d = d * scale;
+      mi = mi - d;
+      xx += mi * mi;
+    }
+    y[i] = d;
+    m[i] = mi;
+    yy += d * d;
+  }
+  block_reduce2(yy, xx, partial);
+}
+
+int grid_for(int64_t work_items) {
+  int64_t g = (work_items + BLOCK - 1) / BLOCK;
+  if (g > MAX_GRID) g = MAX_GRID;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+int group_size_for(int N) { return N <= 8 ? 8 : (N <= 16 ? 16 : 32); }
+
+}  // namespace
+
+struct pfx_bsr {
+  int device = 0;
+  int64_t nrows = 0, nnzb = 0;
+  int32_t N = 0, G = 16;
+  const int32_t *rp = nullptr, *ci = nullptr;
+  double *dinv = nullptr;
+  double *r = nullptr, *rhat = nullptr, *p = nullptr, *v = nullptr, *s = nullptr, *t = nullptr;
+  double *phat = nullptr, *shat = nullptr;
+  double *partial = nullptr, *red = nullptr, *host2 = nullptr;
+  int *fail = nullptr, *host_fail = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+namespace {
+
+// launch helpers over the block-row groups
+int rows_grid(const pfx_bsr *h) {
+  const int64_t rows_per_block = BLOCK / h->G;
+  int64_t g = (h->nrows + rows_per_block - 1) / rows_per_block;
+  if (g > MAX_GRID) g = MAX_GRID;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+int read2(pfx_bsr *h, int nparts, hipStream_t s, double *a, double *b) {
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(BLOCK), 0, s, nparts, h->partial, h->red);
+  KCHK(hipGetLastError());
+  KCHK(hipMemcpyAsync(h->host2, h->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  KCHK(hipStreamSynchronize(s));
+  *a = h->host2[0];
+  if (b) *b = h->host2[1];
+  return 0;
+}
+
+int spmv_launch(pfx_bsr *h, const double *vals, const double *x, double *y, const double *a1, int self2,
+                double *partial, hipStream_t s) {
+  const int g = rows_grid(h);
+  switch (h->G) {
+    case 8:
+      hipLaunchKernelGGL(k_bsr_spmv<8>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y, a1,
+                         self2, partial);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_bsr_spmv<16>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y, a1,
+                         self2, partial);
+      break;
+    default:
+      hipLaunchKernelGGL(k_bsr_spmv<32>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y, a1,
+                         self2, partial);
+  }
+  KCHK(hipGetLastError());
+  return g;
+}
+
+template <int G>
+void launch_rows(pfx_bsr *h, int which, hipStream_t s, int g, const double *vals, double beta, double omega,
+                 double alpha) {
+  if (which == 0)
+    hipLaunchKernelGGL(k_diag_inverse<G>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, h->dinv,
+                       h->fail);
+  else if (which == 1)
+    hipLaunchKernelGGL(k_bcgs_p<G>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->r, h->v, h->p, beta, omega,
+                       h->dinv, h->phat);
+  else
+    hipLaunchKernelGGL(k_bcgs_s<G>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->r, h->v, alpha, h->dinv, h->s,
+                       h->shat, h->partial);
+}
+
+int rows_kernel(pfx_bsr *h, int which, hipStream_t s, const double *vals, double beta, double omega, double alpha) {
+  const int g = rows_grid(h);
+  if (h->G == 8) launch_rows<8>(h, which, s, g, vals, beta, omega, alpha);
+  else if (h->G == 16) launch_rows<16>(h, which, s, g, vals, beta, omega, alpha);
+  else launch_rows<32>(h, which, s, g, vals, beta, omega, alpha);
+  KCHK(hipGetLastError());
+  return g;
+}
+
+}  // namespace
+
+extern "C" int pfx_bsr_create(int64_t nrows, int32_t N, int64_t nnzb, const int32_t *row_ptr,
+                              const int32_t *col_idx, pfx_bsr **out) {
+  if (!out || !row_ptr || !col_idx || nrows < 1 || N < 1 || N > 32 || nnzb < nrows) return PFX_E_ARG;
+  if (nrows > (int64_t)0x7fffffff || nnzb > (int64_t)0x7fffffff) return PFX_E_ARG;
+  *out = nullptr;
+  pfx_bsr *h = new pfx_bsr;
+  KCHK(hipGetDevice(&h->device));
+  h->nrows = nrows;
+  h->nnzb = nnzb;
+  h->N = N;
+  h->G = group_size_for(N);
+  h->rp = row_ptr;
+  h->ci = col_idx;
+  const size_t nv = (size_t)nrows * N * sizeof(double);
+  double **vecs[] = {&h->r, &h->rhat, &h->p, &h->v, &h->s, &h->t, &h->phat, &h->shat};
+  bool ok = hipMalloc((void **)&h->dinv, (size_t)nrows * N * N * sizeof(double)) == hipSuccess;
+  for (double **vp : vecs) ok = ok && hipMalloc((void **)vp, nv) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->partial, 2 * MAX_GRID * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->red, 2 * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&h->fail, sizeof(int)) == hipSuccess;
+  ok = ok && hipHostMalloc((void **)&h->host2, 2 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipHostMalloc((void **)&h->host_fail, sizeof(int), hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipEventCreate(&h->e0) == hipSuccess && hipEventCreate(&h->e1) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    pfx_bsr_destroy(h);
+    return PFX_E_NOMEM;
+  }
+  *out = h;
+  return 0;
+}
+
+extern "C" int pfx_bsr_destroy(pfx_bsr *h) {
+  if (!h) return 0;
+  double *vecs[] = {h->dinv, h->r, h->rhat, h->p, h->v, h->s, h->t, h->phat, h->shat, h->partial, h->red};
+  for (double *v : vecs)
+    if (v) (void)hipFree(v);
+  if (h->fail) (void)hipFree(h->fail);
+  if (h->host2) (void)hipHostFree(h->host2);
+  if (h->host_fail) (void)hipHostFree(h->host_fail);
+  if (h->e0) (void)hipEventDestroy(h->e0);
+  if (h->e1) (void)hipEventDestroy(h->e1);
+  delete h;
+  return 0;
+}
+
+extern "C" int pfx_bsr_spmv(pfx_bsr *h, const double *values, const double *x, double *y, void *stream) {
+  if (!h || !values || !x || !y) return PFX_E_ARG;
+  KCHK(hipSetDevice(h->device));
+  hipStream_t s = (hipStream_t)stream;
+  const int g = spmv_launch(h, values, x, y, nullptr, 0, nullptr, s);
+  if (g < 0) return g;
+  KCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int pfx_bsr_solve(pfx_bsr *h, const double *values, const double *b, double *x, const pfx_krylov_opts *o,
+                             void *stream, pfx_krylov_stats *st) {
+  if (!h || !values || !b || !x) return PFX_E_ARG;
+  const double rtol = (o && o->rtol > 0.0) ? o->rtol : 1.0e-5;
+  const double atol = (o && o->atol > 0.0) ? o->atol : 1.0e-50;
+  const double dtol = (o && o->dtol > 0.0) ? o->dtol : 1.0e5;
+  const int maxit = (o && o->max_iterations > 0) ? o->max_iterations : 10000;
+  KCHK(hipSetDevice(h->device));
+  hipStream_t s = (hipStream_t)stream;
+  pfx_krylov_stats S;
+  memset(&S, 0, sizeof(S));
+  const int64_t n = h->nrows * h->N;
+  const int ge = grid_for(n);
+  KCHK(hipEventRecord(h->e0, s));
+
+  // point-block Jacobi: invert the diagonal blocks
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->fail);
+  int rc = rows_kernel(h, 0, s, values, 0.0, 0.0, 0.0);
+  if (rc < 0) return rc;
+  KCHK(hipMemcpyAsync(h->host_fail, h->fail, sizeof(int), hipMemcpyDeviceToHost, s));
+  hipLaunchKernelGGL(k_bcgs_init, dim3(ge), dim3(BLOCK), 0, s, n, b, x, h->r, h->rhat, h->p, h->v, h->partial);
+  KCHK(hipGetLastError());
+  double bb = 0.0;
+  if ((rc = read2(h, ge, s, &bb, nullptr))) return rc;
+  S.rnorm0 = sqrt(bb);
+  S.rnorm = S.rnorm0;
+  const double tol = fmax(rtol * S.rnorm0, atol);
+  const int conv = rtol * S.rnorm0 >= atol ? PFX_KSP_CONVERGED_RTOL : PFX_KSP_CONVERGED_ATOL;
+  if (*h->host_fail > 0) {
+    S.reason = PFX_KSP_DIVERGED_PC_FAILED;
+  } else if (!std::isfinite(S.rnorm0)) {
+    S.reason = PFX_KSP_DIVERGED_NANORINF;
+  } else if (S.rnorm0 <= tol) {
+    S.reason = conv;
+  }
+  double rho = 1.0, alpha = 1.0, omega = 1.0, rho_new = bb;
+  int it = 0;
+  while (S.reason == 0) {
+    if (it >= maxit) {
+      S.reason = PFX_KSP_DIVERGED_ITS;
+      break;
+    }
+    if (rho_new == 0.0 || omega == 0.0) {
+      S.reason = PFX_KSP_DIVERGED_BREAKDOWN;
+      break;
+    }
+    it++;
+    const double beta = (rho_new / rho) * (alpha / omega);
+    if ((rc = rows_kernel(h, 1, s, values, beta, omega, 0.0)) < 0) return rc;
+    int g = spmv_launch(h, values, h->phat, h->v, h->rhat, 0, h->partial, s);
+    if (g < 0) return g;
+    double rv = 0.0;
+    if ((rc = read2(h, g, s, &rv, nullptr))) return rc;
+    if (rv == 0.0 || !std::isfinite(rv)) {
+      S.reason = std::isfinite(rv) ? PFX_KSP_DIVERGED_BREAKDOWN : PFX_KSP_DIVERGED_NANORINF;
+      break;
+    }
+    alpha = rho_new / rv;
+    if ((g = rows_kernel(h, 2, s, values, 0.0, 0.0, alpha)) < 0) return g;
+    double ss = 0.0;
+    if ((rc = read2(h, g, s, &ss, nullptr))) return rc;
+    const double snorm = sqrt(ss);
+    if (!std::isfinite(snorm)) {
+      S.reason = PFX_KSP_DIVERGED_NANORINF;
+      break;
+    }
+    if (snorm <= tol) {
+      hipLaunchKernelGGL(k_axpy, dim3(ge), dim3(BLOCK), 0, s, n, alpha, h->phat, x);
+      KCHK(hipGetLastError());
+      S.rnorm = snorm;
+      S.reason = conv;
+      break;
+    }
+    if ((g = spmv_launch(h, values, h->shat, h->t, h->s, 1, h->partial, s)) < 0) return g;
+    double ts = 0.0, tt = 0.0;
+    if ((rc = read2(h, g, s, &ts, &tt))) return rc;
+    if (tt == 0.0 || !std::isfinite(tt)) {
+      S.reason = std::isfinite(tt) ? PFX_KSP_DIVERGED_BREAKDOWN : PFX_KSP_DIVERGED_NANORINF;
+      break;
+    }
+    omega = ts / tt;
+    hipLaunchKernelGGL(k_bcgs_xr, dim3(ge), dim3(BLOCK), 0, s, n, x, h->phat, h->shat, h->s, h->t, alpha, omega,
+                       h->r, h->rhat, h->partial);
+    KCHK(hipGetLastError());
+    double rr = 0.0, hr = 0.0;
+    if ((rc = read2(h, ge, s, &rr, &hr))) return rc;
+    rho = rho_new;
+    rho_new = hr;
+    S.rnorm = sqrt(rr);
+    if (!std::isfinite(S.rnorm)) S.reason = PFX_KSP_DIVERGED_NANORINF;
+    else if (S.rnorm <= tol) S.reason = conv;
+    else if (S.rnorm >= dtol * S.rnorm0) S.reason = PFX_KSP_DIVERGED_DTOL;
+  }
+  S.iterations = it;
+  KCHK(hipEventRecord(h->e1, s));
+  KCHK(hipEventSynchronize(h->e1));
+  float ms = 0.f;
+  KCHK(hipEventElapsedTime(&ms, h->e0, h->e1));
+  S.ms = ms;
+  if (st) *st = S;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// the global-implicit Newton step
+
+struct pfx_gi_solver {
+  pfx_tables *tables = nullptr;
+  int device = 0;
+  int32_t nx = 0, ny = 0, nz = 0, N = 0;
+  int64_t n = 0, nnzb = 0;
+  int32_t *rp = nullptr, *ci = nullptr;
+  double *vals = nullptr, *res = nullptr, *y = nullptr;
+  pfx_bsr *bsr = nullptr;
+};
+
+extern "C" int pfx_gi_solver_destroy(pfx_gi_solver *s) {
+  if (!s) return 0;
+  if (s->bsr) pfx_bsr_destroy(s->bsr);
+  if (s->rp) (void)hipFree(s->rp);
+  if (s->ci) (void)hipFree(s->ci);
+  if (s->vals) (void)hipFree(s->vals);
+  if (s->res) (void)hipFree(s->res);
+  if (s->y) (void)hipFree(s->y);
+  delete s;
+  return 0;
+}
+
+extern "C" int pfx_gi_solver_create(pfx_tables *tables, const pfx_grid *grid, pfx_gi_solver **out) {
+  if (!tables || !grid || !out || grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return PFX_E_ARG;
+  *out = nullptr;
+  int N, M, K, R;
+  pfx_internal::table_sizes(tables, &N, &M, &K, &R);
+  int64_t nnzb = 0;
+  int rc = pfx_gi_structure(grid, nullptr, nullptr, &nnzb);
+  if (rc) return rc;
+  pfx_gi_solver *s = new pfx_gi_solver;
+  KCHK(hipGetDevice(&s->device));
+  s->tables = tables;
+  s->nx = grid->nx;
+  s->ny = grid->ny;
+  s->nz = grid->nz;
+  s->N = N;
+  s->n = (int64_t)grid->nx * grid->ny * grid->nz;
+  s->nnzb = nnzb;
+  bool ok = hipMalloc((void **)&s->rp, (s->n + 1) * sizeof(int32_t)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&s->ci, nnzb * sizeof(int32_t)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&s->vals, (size_t)nnzb * N * N * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&s->res, (size_t)s->n * N * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&s->y, (size_t)s->n * N * sizeof(double)) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    pfx_gi_solver_destroy(s);
+    return PFX_E_NOMEM;
+  }
+  if ((rc = pfx_gi_structure(grid, s->rp, s->ci, &nnzb)) || (rc = pfx_bsr_create(s->n, N, nnzb, s->rp, s->ci, &s->bsr))) {
+    pfx_gi_solver_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return 0;
+}
+
+extern "C" int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *grid, const pfx_react_opts *o,
+                                  const pfx_newton_opts *no, pfx_newton_stats *st) {
+  if (!s || !cells || !grid || !o || !cells->pri_molal) return PFX_E_ARG;
+  if (grid->nx != s->nx || grid->ny != s->ny || grid->nz != s->nz || cells->ncells != s->n) return PFX_E_ARG;
+  const auto wall0 = std::chrono::steady_clock::now();
+  const double atol = (no && no->atol > 0.0) ? no->atol : 1.0e-50;
+  const double rtol = (no && no->rtol > 0.0) ? no->rtol : 1.0e-8;
+  const double stol = (no && no->stol > 0.0) ? no->stol : 1.0e-8;
+  const int maxit = (no && no->max_iterations > 0) ? no->max_iterations : 50;
+  pfx_krylov_opts ko;
+  memset(&ko, 0, sizeof(ko));
+  if (no) ko = no->krylov;
+  KCHK(hipSetDevice(s->device));
+  hipStream_t hs = (hipStream_t)o->stream;
+  pfx_newton_stats S;
+  memset(&S, 0, sizeof(S));
+  pfx_bsr *h = s->bsr;
+  const int64_t n = s->n * s->N;
+  const int ge = grid_for(n);
+  double tms[2];
+
+  auto assemble = [&](double *fnorm) -> int {
+    int rc = pfx_gi_residual_jacobian(s->tables, cells, grid, o, s->rp, s->res, s->vals);
+    if (rc) return rc;
+    pfx_tables_timing(s->tables, tms);
+    S.ms_assembly += tms[0] + tms[1];
+    hipLaunchKernelGGL(k_dot2, dim3(ge), dim3(BLOCK), 0, hs, n, s->res, s->res, nullptr, nullptr, h->partial);
+    KCHK(hipGetLastError());
+    double ff = 0.0;
+    if ((rc = read2(h, ge, hs, &ff, nullptr))) return rc;
+    *fnorm = sqrt(ff);
+    return 0;
+  };
+
+  int rc = assemble(&S.fnorm0);
+  if (rc) return rc;
+  S.fnorm = S.fnorm0;
+  if (!std::isfinite(S.fnorm0)) S.reason = PFX_SNES_DIVERGED_FNORM_NAN;
+  else if (S.fnorm0 < atol) S.reason = PFX_SNES_CONVERGED_FNORM_ABS;
+  const int use_log = grid->log_formulation ? 1 : 0;
+  // max_dlnC of the network (reaction%max_dlnC)
+  double max_dlnC = 5.0;
+  pfx_internal::table_max_dlnC(s->tables, &max_dlnC);
+  while (S.reason == 0) {
+    if (S.iterations >= maxit) {
+      S.reason = PFX_SNES_DIVERGED_MAX_IT;
+      break;
+    }
+    pfx_krylov_stats ks;
+    if ((rc = pfx_bsr_solve(h, s->vals, s->res, s->y, &ko, (void *)hs, &ks))) return rc;
+    S.linear_iterations += ks.iterations;
+    S.ms_linear += ks.ms;
+    if (ks.reason < 0) {
+      S.reason = PFX_SNES_DIVERGED_LINEAR_SOLVE;
+      break;
+    }
+    double scale = 1.0;
+    if (!use_log) {
+      hipLaunchKernelGGL(k_min_ratio, dim3(ge), dim3(BLOCK), 0, hs, n, cells->pri_molal, s->y, h->partial);
+      hipLaunchKernelGGL(k_reduce_min, dim3(1), dim3(BLOCK), 0, hs, ge, h->partial, h->red);
+      KCHK(hipGetLastError());
+      KCHK(hipMemcpyAsync(h->host2, h->red, 2 * sizeof(double), hipMemcpyDeviceToHost, hs));
+      KCHK(hipStreamSynchronize(hs));
+      const double min_ratio = h->host2[0];
+      if (min_ratio < 1.0) scale = min_ratio * 0.99;
+    }
+    hipLaunchKernelGGL(k_newton_update, dim3(ge), dim3(BLOCK), 0, hs, n, cells->pri_molal, s->y, use_log, max_dlnC,
+                       scale, h->partial);
+    KCHK(hipGetLastError());
+    double yy = 0.0, xx = 0.0;
+    if ((rc = read2(h, ge, hs, &yy, &xx))) return rc;
+    S.iterations++;
+    if ((rc = assemble(&S.fnorm))) return rc;
+    // SNESConvergedDefault's order
+    if (!std::isfinite(S.fnorm)) S.reason = PFX_SNES_DIVERGED_FNORM_NAN;
+    else if (S.fnorm < atol) S.reason = PFX_SNES_CONVERGED_FNORM_ABS;
+    else if (S.fnorm <= rtol * S.fnorm0) S.reason = PFX_SNES_CONVERGED_FNORM_RELATIVE;
+    else if (sqrt(yy) < stol * sqrt(xx)) S.reason = PFX_SNES_CONVERGED_SNORM_RELATIVE;
+  }
+  S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  if (st) *st = S;
+  return 0;
+}

@@ -42,6 +42,10 @@ def test_null_arguments_rejected_without_gpu_work():
     assert lib.pfx_mlp_destroy(None) == abi.PFX_E_ARG
     assert lib.pfx_mlp_forward(None, None, 0, None, None) == abi.PFX_E_ARG
     assert lib.pfx_surrogate_guess(None, None, None, None) == abi.PFX_E_ARG
+    assert lib.pfx_bsr_create(0, 15, 0, None, None, None) == abi.PFX_E_ARG
+    assert lib.pfx_bsr_solve(None, None, None, None, None, None, None) == abi.PFX_E_ARG
+    assert lib.pfx_gi_solver_create(None, None, None) == abi.PFX_E_ARG
+    assert lib.pfx_gi_solver_step(None, None, None, None, None, None) == abi.PFX_E_ARG
 
 
 def test_every_header_compiles_as_c(tmp_path):
@@ -52,10 +56,12 @@ def test_every_header_compiles_as_c(tmp_path):
 
 @pytest.mark.parametrize("struct,cname", [(abi.ReactionDesc, "pfx_reaction_desc"), (abi.Cells, "pfx_cells"),
                                           (abi.ReactOpts, "pfx_react_opts"), (abi.Stats, "pfx_stats"),
-                                          (abi.Grid, "pfx_grid"), (abi.ZonePlan, "pfx_zone_plan")])
+                                          (abi.Grid, "pfx_grid"), (abi.ZonePlan, "pfx_zone_plan"),
+                                          (abi.KrylovOpts, "pfx_krylov_opts"), (abi.KrylovStats, "pfx_krylov_stats"),
+                                          (abi.NewtonOpts, "pfx_newton_opts"), (abi.NewtonStats, "pfx_newton_stats")])
 def test_struct_layout_matches_header(tmp_path, struct, cname):
     fields = [f for f, _ in struct._fields_]
-    src = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', 'int main(void){',
+    src = ['#include <stddef.h>', '#include <stdio.h>'] + [f'#include "{h}"' for h in HEADERS] + ['int main(void){',
            f'printf("%zu\\n", sizeof({cname}));']
     src += [f'printf("%zu\\n", offsetof({cname}, {f}));' for f in fields]
     src.append("return 0;}")
