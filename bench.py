@@ -319,10 +319,55 @@ def gi_leg(t, tab, args, log):
            "assembly_bytes": bytes_asm, "assembly_GBs": bytes_asm / (a_ms * 1e-3) / 1e9,
            "assembly_frac_hbm": bytes_asm / (a_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
            "cells_per_s": n / (float(np.mean(wall)) * 1e-3), "finite": ok}
-    del vals, res, dc
-    torch.cuda.empty_cache()
     log(f"gi: {out['ms_per_evaluation']:.2f} ms/evaluation, assembly {a_ms:.2f} ms "
         f"({out['assembly_GBs']:.0f} GB/s)")
+
+    # the block product of the Krylov solve on this Jacobian: HBM-bound,
+    # algorithmic bytes = every block once, x and y once, the structure once
+    bsr = runtime.Bsr(row_ptr, col_idx, N)
+    xv = torch.rand((n, N), dtype=torch.float64, device="cuda")
+    yv = torch.empty_like(xv)
+    bsr.spmv(vals, xv, yv)
+    sp_ms = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        bsr.spmv(vals, xv, yv)
+        e1.record()
+        torch.cuda.synchronize()
+        sp_ms.append(e0.elapsed_time(e1))
+    spmv_bytes = 8 * (nnzb * N * N + 2 * n * N) + 4 * (nnzb + n + 1)
+    out["spmv_ms"] = float(np.mean(sp_ms))
+    out["spmv_GBs"] = spmv_bytes / (out["spmv_ms"] * 1e-3) / 1e9
+    out["spmv_frac_hbm"] = out["spmv_GBs"] / PEAK_HBM_GBS
+    bsr.close()
+    del vals, res, xv, yv
+    torch.cuda.empty_cache()
+    log(f"gi: block product {out['spmv_ms']:.2f} ms ({out['spmv_GBs']:.0f} GB/s)")
+
+    # one time step closed on the device: Newton from the old state with
+    # point-block Jacobi BiCGStab, PETSc's default tolerances, with the
+    # analytic reaction Jacobian and with NUMERICAL_JACOBIAN
+    snap = workload.snapshot(dc)
+    for key, numerical in (("newton", 0), ("newton_numerical_jacobian", 1)):
+        workload.restore(dc, snap)
+        t.numerical_jacobian = numerical
+        tab_n = runtime.Tables(t)
+        t.numerical_jacobian = 0
+        gs = runtime.GiSolver(tab_n, g)
+        st = gs.step(dc, g, synth.react_opts())
+        gs.close()
+        out[key] = {"reason": st.reason, "iterations": st.iterations, "linear_iterations": st.linear_iterations,
+                    "fnorm0": st.fnorm0, "fnorm": st.fnorm, "ms_total": st.ms_total,
+                    "ms_assembly": st.ms_assembly, "ms_linear": st.ms_linear,
+                    "ms_per_newton_iteration": st.ms_total / max(st.iterations, 1),
+                    "tolerances": "SNES rtol 1e-8 / stol 1e-8 / 50 its, KSP rtol 1e-5 (PETSc defaults)"}
+        log(f"gi: {key} {st.ms_total:.1f} ms, {st.iterations} its ({st.linear_iterations} BiCGStab), "
+            f"reason {st.reason}, |F| {st.fnorm0:.2e} -> {st.fnorm:.2e}, assembly {st.ms_assembly:.1f} ms, "
+            f"linear {st.ms_linear:.1f} ms")
+        del tab_n
+    del dc, snap
+    torch.cuda.empty_cache()
     return out
 
 

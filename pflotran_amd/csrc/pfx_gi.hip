@@ -122,9 +122,46 @@ __global__ __launch_bounds__(256) void k_gi_assemble(const gi_params P) {
   }
   if (comp) P.res[c * N + lane] = res;
 
-  // blocks of the row, in column order
+  // blocks of the row, in column order.  All source words are loaded
+  // before the first store (up to 8 x CH loads in flight per lane); the
+  // row's blocks are written once and not read back here, so they bypass
+  // the caches that hold the neighbours' dT/dm blocks.
   int64_t b = P.row_ptr[c];
   const int order[7] = {0, 1, 2, -1, 3, 4, 5};
+  constexpr int CH = 4;  // 64-wide chunks of an N x N block, N <= 16
+  if (NN <= CH * 64) {
+    double v[7][CH], jn[CH];
+#pragma unroll
+    for (int s = 0; s < 7; s++) {
+      const int o = order[s];
+      const bool on = o < 0 || has[o];
+      const double *src = P.dtot + (o < 0 ? c : (on ? nb[o] : c)) * NN;
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int e = lane + 64 * q;
+        v[s][q] = (on && e < NN) ? src[e] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CH; q++) {
+      const int e = lane + 64 * q;
+      jn[q] = e < NN ? __builtin_nontemporal_load(P.jnf + c * NN + e) : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 7; s++) {
+      const int o = order[s];
+      if (o < 0 || has[o]) {
+        double *out = P.vals + b * NN;
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+          const int e = lane + 64 * q;
+          if (e < NN) __builtin_nontemporal_store(o < 0 ? jn[q] + s_diag * v[s][q] : tnb[o] * v[s][q], out + e);
+        }
+        b++;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < 7; s++) {
     const int o = order[s];

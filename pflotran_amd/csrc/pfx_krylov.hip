@@ -134,13 +134,16 @@ __global__ __launch_bounds__(BLOCK) void k_dot2(int64_t n, const double *__restr
   block_reduce2(s0, s1, partial);
 }
 
-// y = A x, optionally with the partial sums of (y, a1) and (y, y)
-template <int G>
+// y = A x, optionally with the partial sums of (y, a1) and (y, y).  NT > 0
+// fixes N at compile time: a block's N column loads are then all issued
+// before the first multiply (N loads in flight per lane).
+template <int G, int NT>
 __global__ __launch_bounds__(BLOCK) void k_bsr_spmv(int64_t nrows, int N, const int32_t *__restrict__ rp,
                                                     const int32_t *__restrict__ ci,
                                                     const double *__restrict__ vals, const double *__restrict__ x,
                                                     double *__restrict__ y, const double *__restrict__ a1,
                                                     int self2, double *__restrict__ partial) {
+  if (NT > 0) N = NT;
   const int r = threadIdx.x % G;
   const int64_t NN = (int64_t)N * N;
   double d1 = 0.0, d2 = 0.0;
@@ -152,12 +155,26 @@ __global__ __launch_bounds__(BLOCK) void k_bsr_spmv(int64_t nrows, int N, const 
       for (int bk = b0; bk < b1; bk++) {
         const double *A = vals + (int64_t)bk * NN + r;
         const double *xc = x + (int64_t)ci[bk] * N;
-        int k = 0;
-        for (; k + 1 < N; k += 2) {
-          acc0 += __builtin_nontemporal_load(A + (int64_t)k * N) * xc[k];
-          acc1 += __builtin_nontemporal_load(A + (int64_t)(k + 1) * N) * xc[k + 1];
+        if constexpr (NT > 0) {
+          double a[NT], xv[NT];
+#pragma unroll
+          for (int k = 0; k < NT; k++) {
+            a[k] = __builtin_nontemporal_load(A + k * NT);
+            xv[k] = xc[k];
+          }
+#pragma unroll
+          for (int k = 0; k < NT; k++) {
+            if (k & 1) acc1 += a[k] * xv[k];
+            else acc0 += a[k] * xv[k];
+          }
+        } else {
+          int k = 0;
+          for (; k + 1 < N; k += 2) {
+            acc0 += __builtin_nontemporal_load(A + (int64_t)k * N) * xc[k];
+            acc1 += __builtin_nontemporal_load(A + (int64_t)(k + 1) * N) * xc[k + 1];
+          }
+          if (k < N) acc0 += __builtin_nontemporal_load(A + (int64_t)k * N) * xc[k];
         }
-        if (k < N) acc0 += __builtin_nontemporal_load(A + (int64_t)k * N) * xc[k];
       }
       const double yv = acc0 + acc1;
       y[row * N + r] = yv;
@@ -181,10 +198,19 @@ __device__ __forceinline__ void group_argmax(double &v, int &idx) {
   }
 }
 
+// LDS written by lanes of this wavefront becomes visible to the wavefront
+// (a group of G lanes never spans two wavefronts)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Inverse of each diagonal block by Gauss-Jordan with partial pivoting,
 // lane r holding row r of the block and of the growing inverse; the lane
-// that pivots at step j ends with row j of the inverse.  A zero pivot
-// counts in *fail.
+// that pivots at step j ends with row j of the inverse.  The pivot row is
+// broadcast through LDS (one store by its lane, same-address reads by the
+// group).  A zero pivot counts in *fail.
 template <int G>
 __global__ __launch_bounds__(BLOCK) void k_diag_inverse(int64_t nrows, int N, const int32_t *__restrict__ rp,
                                                         const int32_t *__restrict__ ci,
@@ -193,6 +219,8 @@ __global__ __launch_bounds__(BLOCK) void k_diag_inverse(int64_t nrows, int N, co
   const int r = threadIdx.x % G;
   const bool live = r < N;
   const int64_t NN = (int64_t)N * N;
+  __shared__ double prow[BLOCK / G][2 * G];
+  double *pr = prow[threadIdx.x / G];
   for (int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G; row < nrows;
        row += (int64_t)gridDim.x * (BLOCK / G)) {
     int db = -1;
@@ -213,12 +241,21 @@ __global__ __launch_bounds__(BLOCK) void k_diag_inverse(int64_t nrows, int N, co
         double cand = done ? -1.0 : fabs(a[j]);
         int p = r;
         group_argmax<G>(cand, p);
+        // columns before j of the pivot row are already zero
+        if (r == p) {
+#pragma unroll
+          for (int k = j; k < G; k++) pr[k] = a[k];
+#pragma unroll
+          for (int k = 0; k < G; k++) pr[G + k] = e[k];
+        }
+        wave_sync();
         double pa[G], pe[G];
 #pragma unroll
         for (int k = 0; k < G; k++) {
-          pa[k] = __shfl(a[k], p, G);
-          pe[k] = __shfl(e[k], p, G);
+          pa[k] = k >= j ? pr[k] : 0.0;
+          pe[k] = pr[G + k];
         }
+        wave_sync();
         double piv = pa[j];
         if (piv == 0.0) {
           singular = true;
@@ -237,9 +274,10 @@ __global__ __launch_bounds__(BLOCK) void k_diag_inverse(int64_t nrows, int N, co
           const double f = a[j] * rinv;
 #pragma unroll
           for (int k = 0; k < G; k++) {
-            a[k] -= f * pa[k];
+            if (k > j) a[k] -= f * pa[k];
             e[k] -= f * pe[k];
           }
+          a[j] = 0.0;
         }
       }
     }
@@ -428,19 +466,18 @@ int read2(pfx_bsr *h, int nparts, hipStream_t s, double *a, double *b) {
 int spmv_launch(pfx_bsr *h, const double *vals, const double *x, double *y, const double *a1, int self2,
                 double *partial, hipStream_t s) {
   const int g = rows_grid(h);
-  switch (h->G) {
-    case 8:
-      hipLaunchKernelGGL(k_bsr_spmv<8>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y, a1,
-                         self2, partial);
-      break;
-    case 16:
-      hipLaunchKernelGGL(k_bsr_spmv<16>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y, a1,
-                         self2, partial);
-      break;
-    default:
-      hipLaunchKernelGGL(k_bsr_spmv<32>, dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y, a1,
-                         self2, partial);
-  }
+  if (h->N == 15)
+    hipLaunchKernelGGL((k_bsr_spmv<16, 15>), dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y,
+                       a1, self2, partial);
+  else if (h->G == 8)
+    hipLaunchKernelGGL((k_bsr_spmv<8, 0>), dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y, a1,
+                       self2, partial);
+  else if (h->G == 16)
+    hipLaunchKernelGGL((k_bsr_spmv<16, 0>), dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y,
+                       a1, self2, partial);
+  else
+    hipLaunchKernelGGL((k_bsr_spmv<32, 0>), dim3(g), dim3(BLOCK), 0, s, h->nrows, h->N, h->rp, h->ci, vals, x, y,
+                       a1, self2, partial);
   KCHK(hipGetLastError());
   return g;
 }
