@@ -11,7 +11,10 @@ program pfx_abi_probe
   type(pfx_stats) :: stats
   type(pfx_grid) :: grid
   type(pfx_zone_plan) :: plan
-  type(c_ptr) :: tables
+  type(pfx_newton_opts) :: nopts
+  type(pfx_newton_stats) :: nstats
+  type(pfx_krylov_stats) :: kstats
+  type(c_ptr) :: tables, solver
   integer(c_int) :: rc
 
   print '(a,i0)', 'abi ', pfx_abi_version()
@@ -28,4 +31,9 @@ program pfx_abi_probe
   print '(a,i0)', 'sizeof_stats ', c_sizeof(stats)
   print '(a,i0)', 'sizeof_grid ', c_sizeof(grid)
   print '(a,i0)', 'sizeof_plan ', c_sizeof(plan)
+  rc = pfx_gi_solver_create(c_null_ptr, grid, solver)
+  print '(a,i0)', 'solver_null ', rc
+  print '(a,i0)', 'sizeof_newton_opts ', c_sizeof(nopts)
+  print '(a,i0)', 'sizeof_newton_stats ', c_sizeof(nstats)
+  print '(a,i0)', 'sizeof_krylov_stats ', c_sizeof(kstats)
 end program pfx_abi_probe

@@ -90,6 +90,31 @@ module pfx_chem_module
     integer(c_int64_t) :: fallbacks
   end type pfx_stats
 
+  ! global-implicit step on the device (include/pfx_gi_solver.h)
+  type, bind(c), public :: pfx_krylov_opts
+    real(c_double) :: rtol, atol, dtol
+    integer(c_int32_t) :: max_iterations, reserved
+  end type pfx_krylov_opts
+
+  type, bind(c), public :: pfx_krylov_stats
+    integer(c_int32_t) :: iterations, reason
+    real(c_double) :: rnorm0, rnorm, ms
+  end type pfx_krylov_stats
+
+  type, bind(c), public :: pfx_newton_opts
+    real(c_double) :: atol, rtol, stol
+    integer(c_int32_t) :: max_iterations, reserved
+    type(pfx_krylov_opts) :: krylov
+  end type pfx_newton_opts
+
+  type, bind(c), public :: pfx_newton_stats
+    integer(c_int32_t) :: iterations, reason, linear_iterations, reserved
+    real(c_double) :: fnorm0, fnorm, ms_assembly, ms_linear, ms_total
+  end type pfx_newton_stats
+
+  public :: pfx_bsr_create, pfx_bsr_destroy, pfx_bsr_spmv, pfx_bsr_solve, &
+            pfx_gi_solver_create, pfx_gi_solver_destroy, pfx_gi_solver_step
+
   public :: pfx_abi_version, pfx_tables_create, pfx_tables_destroy, &
             pfx_react_batch, pfx_residual_jacobian, pfx_equilibrate_batch, &
             pfx_auxvar_batch, pfx_update_kinetic_state, pfx_tables_timing, &
@@ -253,6 +278,62 @@ module pfx_chem_module
       type(pfx_zone_plan), intent(out) :: plan
       integer(c_int) :: pfx_graph_stats
     end function pfx_graph_stats
+
+    function pfx_bsr_create(nrows, n, nnzb, row_ptr, col_idx, h) bind(c, name='pfx_bsr_create')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      integer(c_int64_t), value :: nrows
+      integer(c_int32_t), value :: n
+      integer(c_int64_t), value :: nnzb
+      type(c_ptr), value :: row_ptr, col_idx
+      type(c_ptr), intent(out) :: h
+      integer(c_int) :: pfx_bsr_create
+    end function pfx_bsr_create
+
+    function pfx_bsr_destroy(h) bind(c, name='pfx_bsr_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int) :: pfx_bsr_destroy
+    end function pfx_bsr_destroy
+
+    function pfx_bsr_spmv(h, values, x, y, stream) bind(c, name='pfx_bsr_spmv')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, values, x, y, stream
+      integer(c_int) :: pfx_bsr_spmv
+    end function pfx_bsr_spmv
+
+    function pfx_bsr_solve(h, values, b, x, opts, stream, stats) bind(c, name='pfx_bsr_solve')
+      import :: c_int, c_ptr, pfx_krylov_opts, pfx_krylov_stats
+      type(c_ptr), value :: h, values, b, x
+      type(pfx_krylov_opts), intent(in) :: opts
+      type(c_ptr), value :: stream
+      type(pfx_krylov_stats), intent(out) :: stats
+      integer(c_int) :: pfx_bsr_solve
+    end function pfx_bsr_solve
+
+    function pfx_gi_solver_create(tables, grid, solver) bind(c, name='pfx_gi_solver_create')
+      import :: c_int, c_ptr, pfx_grid
+      type(c_ptr), value :: tables
+      type(pfx_grid), intent(in) :: grid
+      type(c_ptr), intent(out) :: solver
+      integer(c_int) :: pfx_gi_solver_create
+    end function pfx_gi_solver_create
+
+    function pfx_gi_solver_destroy(solver) bind(c, name='pfx_gi_solver_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solver
+      integer(c_int) :: pfx_gi_solver_destroy
+    end function pfx_gi_solver_destroy
+
+    function pfx_gi_solver_step(solver, cells, grid, opts, nopts, stats) bind(c, name='pfx_gi_solver_step')
+      import :: c_int, c_ptr, pfx_cells, pfx_grid, pfx_react_opts, pfx_newton_opts, pfx_newton_stats
+      type(c_ptr), value :: solver
+      type(pfx_cells), intent(inout) :: cells
+      type(pfx_grid), intent(in) :: grid
+      type(pfx_react_opts), intent(in) :: opts
+      type(pfx_newton_opts), intent(in) :: nopts
+      type(pfx_newton_stats), intent(out) :: stats
+      integer(c_int) :: pfx_gi_solver_step
+    end function pfx_gi_solver_step
 
     function pfx_graph_destroy(graph) bind(c, name='pfx_graph_destroy')
       import :: c_int, c_ptr

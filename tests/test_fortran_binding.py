@@ -33,3 +33,7 @@ def test_fortran_module_binds_the_c_abi(tmp_path):
     assert int(vals["sizeof_stats"]) == C.sizeof(abi.Stats)
     assert int(vals["sizeof_grid"]) == C.sizeof(abi.Grid)
     assert int(vals["sizeof_plan"]) == C.sizeof(abi.ZonePlan)
+    assert int(vals["solver_null"]) == abi.PFX_E_ARG
+    assert int(vals["sizeof_newton_opts"]) == C.sizeof(abi.NewtonOpts)
+    assert int(vals["sizeof_newton_stats"]) == C.sizeof(abi.NewtonStats)
+    assert int(vals["sizeof_krylov_stats"]) == C.sizeof(abi.KrylovStats)
