@@ -208,8 +208,41 @@ def main():
                                  "frac": mlp_flops * n / (mlp_avg * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
                                  "kernel": "k_surrogate_guess"}}
         value, ms_per_step, st_main, kern_ms = warm_value, warm_ms_step, st_warm, st_warm.kernel_ms
-        gpu_m = dc.t["pri_molal"]
-        gpu_status = dc.t["status"]
+        gpu_m = dc.t["pri_molal"].clone()
+        gpu_status = dc.t["status"].clone()
+
+        # both methods again with PFX_JAC_CAPPED_QK (no mineral derivative
+        # above the Q/K cap, where the rate is constant): same residual and
+        # acceptance test, a consistent Jacobian for the capped cells
+        cap = {}
+        for key, opts in (("plain", synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT,
+                                                     jacobian_flags=abi.PFX_JAC_CAPPED_QK)),
+                          ("warm", synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK,
+                                                    fallback_after=args.fallback_after,
+                                                    jacobian_flags=abi.PFX_JAC_CAPPED_QK))):
+            def step(opts=opts, key=key):
+                restore()
+                if key == "warm":
+                    surrogate.guess(tab, mlp, dc)
+                box["rc"], box["st"] = runtime.react_batch(tab, dc, opts)
+            for _ in range(args.warmup):
+                step()
+            ms = float(np.mean(timed_steps(torch, args.steps, step)))
+            st = box["st"]
+            cap[key] = {"cells_per_s": st.converged / (ms * 1e-3), "ms_per_step": ms, "kernel_ms": st.kernel_ms,
+                        "converged": int(st.converged), "mean_iterations": st.total_iterations / n,
+                        "max_iterations": st.max_iterations, "max_rel_res": st.max_rel_res,
+                        "fallbacks": int(st.fallbacks)}
+            log(f"{key} with PFX_JAC_CAPPED_QK: {ms:.2f} ms/step, {cap[key]['cells_per_s']:.4g} cells/s, "
+                f"{cap[key]['mean_iterations']:.2f} it/cell")
+        cap["warm_speedup_vs_plain"] = cap["warm"]["cells_per_s"] / cap["plain"]["cells_per_s"]
+        cap["warm_speedup_vs_reference_jacobian_plain"] = cap["warm"]["cells_per_s"] / plain_value
+        both = ((dc.t["status"] == 0) & (gpu_status == 0)).unsqueeze(1)
+        rel = ((dc.t["pri_molal"] - gpu_m).abs() / gpu_m.abs()).masked_fill(~both, 0.0)
+        cap["max_rel_diff_free_ions_vs_reference_jacobian_warm"] = float(rel.max().item())
+        cap["p99_rel_diff_free_ions_vs_reference_jacobian_warm"] = float(
+            torch.quantile(rel.amax(dim=1)[:200_000].float(), 0.99).item())
+        warm["capped_qk"] = cap
     else:
         value, ms_per_step, st_main, kern_ms = plain_value, plain_ms_step, st_plain, plain_kernel_ms
         plain_step()
@@ -268,6 +301,13 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         line["copy_calibration_bytes"] = x.numel() * 8
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        line["copy_GBs"] = 2 * x.numel() * 8 * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(t, dc, snap, gpu_m, gpu_status, plain_m, plain_status, n_cpu)
     print(json.dumps(line), flush=True)

@@ -193,8 +193,17 @@ typedef struct pfx_react_opts {
   void *stream;           /* hipStream_t, NULL = default stream               */
   int32_t fallback_after; /* PFX_GUESS_ARRAY_FALLBACK: iterations of the warm
                              attempt (0 = max_iterations)                     */
-  int32_t reserved1;
+  int32_t jacobian_flags; /* PFX_JAC_*; 0 = the reference's Jacobian          */
 } pfx_react_opts;
+
+/* pfx_react_opts.jacobian_flags.  PFX_JAC_CAPPED_QK: where ln(Q/K) of a
+   kinetic mineral exceeds the reference's cap 6.90776 (Q/K held at 1e3,
+   reaction_mineral.F90:699-703) the rate does not change with m, and its
+   derivative is left out.  The reference keeps the uncapped derivative there
+   (reaction_mineral.F90:860-875), with which Newton converges only linearly
+   for such cells.  The residual, and so the solution, are unchanged.  No
+   counterpart in the reference. */
+#define PFX_JAC_CAPPED_QK 1
 
 typedef struct pfx_stats {
   int64_t ncells;

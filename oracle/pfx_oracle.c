@@ -475,7 +475,9 @@ void or_ionx(const pfx_reaction_desc *d, ocell *c) {
 }
 
 /* reaction_mineral.F90:564-1000 RKineticMineral (no prefactor species).
-   Res (mol/s) and Jac (kg water/s) accumulate; rates stored per m^3 bulk. */
+   Res (mol/s) and Jac (kg water/s) accumulate; rates stored per m^3 bulk.
+   deriv: 0 residual only, 1 the reference's derivative, 1 | 2 without the
+   derivative where ln(Q/K) is above the cap (PFX_JAC_CAPPED_QK). */
 void or_kinetic_mineral(const pfx_reaction_desc *d, ocell *c, double *Res, double Jac[NMAX][NMAX], int deriv) {
   const int N = d->naqcomp;
   double ln_conc[NMAX], ln_act[NMAX];
@@ -513,6 +515,7 @@ void or_kinetic_mineral(const pfx_reaction_desc *d, ocell *c, double *Res, doubl
     Im *= c->volume;
     for (int s = 0; s < ns; s++) Res[MNRL_ID(d, k, s)] += MNRL_NU(d, k, s) * Im;
     if (!deriv) continue;
+    if ((deriv & 2) && lnQK > 6.90776) continue;
     double dIm_dQK;
     if (d->kinmnrl_affinity_power)
       dIm_dQK = -Im * d->kinmnrl_affinity_power[k] / fabs(aff);
@@ -586,7 +589,7 @@ static void or_reaction_fd(const pfx_reaction_desc *d, ocell *c, double *R, doub
    the reaction part of the Jacobian by differences (the global-implicit
    path only; RReact's Newton is analytic, reaction.F90:3438-3439). */
 static void or_res_jac(const pfx_reaction_desc *d, ocell *c, const double *fixed_accum, double dt,
-                       double *R, double J[NMAX][NMAX], double *scale, int numerical) {
+                       double *R, double J[NMAX][NMAX], double *scale, int numerical, int jflags) {
   const int N = d->naqcomp;
   or_auxvar_compute(d, c);
   double acc[NMAX];
@@ -602,7 +605,7 @@ static void or_res_jac(const pfx_reaction_desc *d, ocell *c, const double *fixed
   }
   if (d->nkinmnrl > 0) {
     if (numerical) or_reaction_fd(d, c, R, J);
-    else or_kinetic_mineral(d, c, R, J, 1);
+    else or_kinetic_mineral(d, c, R, J, (jflags & PFX_JAC_CAPPED_QK) ? 3 : 1);
   }
   if (scale) {
     /* magnitude of every term in component j (mol/s): the denominator of the
@@ -736,7 +739,7 @@ restart:
          RTAuxVarCompute (the previous iterate) */
       or_activity(d, &c);
     }
-    or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0);
+    or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0, o->jacobian_flags);
     rr = or_rel_res(Rv, sc, N);
     if (!isfinite(rr)) { status = PFX_CELL_NONFINITE; break; }
     if (rr <= o->rel_tol) {
@@ -771,7 +774,7 @@ restart:
     }
     for (int j = 0; j < N; j++) maxrel = fmax(maxrel, fabs((c.m[j] - prev[j]) / prev[j]));
     if (o->reference_stop && maxrel < d->max_relative_change_tolerance) {
-      or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0);
+      or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0, o->jacobian_flags);
       rr = or_rel_res(Rv, sc, N);
       status = PFX_CELL_CONVERGED;
       done = 1;
@@ -882,7 +885,7 @@ int or_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
         if (d->neqionxrxn > 0 && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
       }
     }
-    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL, d->numerical_jacobian);
+    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL, d->numerical_jacobian, o->jacobian_flags);
     for (int j = 0; j < N; j++) {
       residual[ci * N + j] = Rv[j];
       for (int k = 0; k < N; k++) jacobian[ci * N * N + j + (size_t)k * N] = J[j][k];
@@ -1273,7 +1276,7 @@ int or_gi_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx
       fixed[j] = psv_t * cl->total[ci * N + j];
       if (d->neqionxrxn > 0 && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
     }
-    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL, d->numerical_jacobian);
+    or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL, d->numerical_jacobian, o->jacobian_flags);
     for (int i = 0; i < N; i++) {
       tot[ci * N + i] = c.total[i];
       rnf[ci * N + i] = Rv[i];

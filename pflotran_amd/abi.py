@@ -24,6 +24,7 @@ PFX_CELL_CONVERGED, PFX_CELL_MAX_ITERATIONS, PFX_CELL_NONFINITE, PFX_CELL_SINGUL
 PFX_GUESS_PREVIOUS, PFX_GUESS_ARRAY, PFX_GUESS_DEFAULT, PFX_GUESS_ARRAY_FALLBACK = 0, 1, 2, 3
 PFX_CONSTRAINT_TOTAL, PFX_CONSTRAINT_FREE, PFX_CONSTRAINT_LOG, PFX_CONSTRAINT_PH, PFX_CONSTRAINT_CHARGE = 0, 1, 2, 3, 4
 PFX_CONSTRAINT_TOTAL_SORB = 5
+PFX_JAC_CAPPED_QK = 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -70,7 +71,7 @@ class ReactOpts(C.Structure):
         ("extra_iterations", C.c_int32), ("guess_mode", C.c_int32), ("reference_stop", C.c_int32),
         ("default_guess", C.c_double), ("porosity", C.c_double), ("saturation", C.c_double),
         ("volume", C.c_double), ("den_kg", C.c_double), ("stream", C.c_void_p),
-        ("fallback_after", C.c_int32), ("reserved1", C.c_int32),
+        ("fallback_after", C.c_int32), ("jacobian_flags", C.c_int32),
     ]
 
 
@@ -145,7 +146,8 @@ CELL_FIELDS = [f for f, _ in Cells._fields_ if f != "ncells"]
 
 def default_opts(dt=1.0, rel_tol=1e-10, max_iterations=100, extra_iterations=0,
                  guess_mode=PFX_GUESS_PREVIOUS, reference_stop=0, default_guess=1e-9,
-                 porosity=0.25, saturation=1.0, volume=1.0, den_kg=997.0, stream=None, fallback_after=0):
+                 porosity=0.25, saturation=1.0, volume=1.0, den_kg=997.0, stream=None, fallback_after=0,
+                 jacobian_flags=0):
     o = ReactOpts()
     o.dt = dt
     o.rel_tol = rel_tol
@@ -157,4 +159,5 @@ def default_opts(dt=1.0, rel_tol=1e-10, max_iterations=100, extra_iterations=0,
     o.porosity, o.saturation, o.volume, o.den_kg = porosity, saturation, volume, den_kg
     o.stream = stream
     o.fallback_after = fallback_after
+    o.jacobian_flags = jacobian_flags
     return o

@@ -249,6 +249,7 @@ void fill_params(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, pfx_k
   P.guess_mode = o->guess_mode;
   P.reference_stop = o->reference_stop;
   P.fallback_after = o->fallback_after > 0 ? o->fallback_after : o->max_iterations;
+  P.jac_flags = o->jacobian_flags;
   P.fallback_count = t->d_queue + 1;
 }
 
@@ -262,6 +263,7 @@ int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o
     return PFX_E_ARG;
   if (o->guess_mode < PFX_GUESS_PREVIOUS || o->guess_mode > PFX_GUESS_ARRAY_FALLBACK) return PFX_E_ARG;
   if (o->fallback_after < 0) return PFX_E_ARG;
+  if (o->jacobian_flags & ~PFX_JAC_CAPPED_QK) return PFX_E_ARG;
   if (!(o->dt > 0.0)) return PFX_E_ARG;
   return 0;
 }

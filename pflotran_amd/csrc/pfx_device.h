@@ -79,6 +79,7 @@ struct pfx_kernel_params {
   double def_porosity, def_saturation, def_volume, def_den;
   int32_t max_iterations, extra_iterations, guess_mode, reference_stop;
   int32_t fallback_after;       // warm-attempt iteration cap (PFX_GUESS_ARRAY_FALLBACK)
+  int32_t jac_flags;            // PFX_JAC_*
   unsigned int *fallback_count; // cells restarted from the default guess
   double *res_out, *jac_out;    // residual/Jacobian outputs
   double *dtot_out;             // EVAL: dT/dm blocks [ncells][N][N] column-major (kg/L), may be NULL

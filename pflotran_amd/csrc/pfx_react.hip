@@ -806,7 +806,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
               const double dd = 1.0 + (1.0 - aff) / lim;
               f = f * (1.0 + QK / lim / dd) / dd;
             }
-            if (!fd_jac) {
+            if (!fd_jac && !((P.jac_flags & PFX_JAC_CAPPED_QK) && lnQK > 6.90776)) {
 #pragma unroll
               for (int kk = 0; kk < G; kk++) row[kk] += f * LD(L.o_mnuG, k * G + kk) * SD(S.im, kk);
             }

@@ -64,7 +64,7 @@ module pfx_chem_module
     real(c_double) :: default_guess
     real(c_double) :: porosity, saturation, volume, den_kg
     type(c_ptr) :: stream
-    integer(c_int32_t) :: fallback_after, reserved1
+    integer(c_int32_t) :: fallback_after, jacobian_flags
   end type pfx_react_opts
 
   type, bind(c), public :: pfx_grid

@@ -127,6 +127,29 @@ def test_react_parity_default_guess(net15):
     assert np.mean(np.abs(out["num_iterations"] - ref["num_iterations"])) < 0.5
 
 
+def test_react_parity_capped_mineral_derivative(net15):
+    # PFX_JAC_CAPPED_QK: the oracle's Newton with the same option, and the
+    # same solution as the reference Jacobian's in fewer iterations
+    t = net15
+    n = 512
+    cells, _, _, _ = speciated_state(t, n)
+    oc = dict(extra_iterations=1, jacobian_flags=abi.PFX_JAC_CAPPED_QK)
+    ref = cells.copy()
+    orc.react_batch(t, ref, synth.react_opts(**oc))
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    runtime.react_batch(tab, dc, synth.react_opts(**oc))
+    out = dc.to_host()
+    assert np.array_equal(out["status"], ref["status"]) and np.all(out["status"] == 0)
+    assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
+    assert np.mean(np.abs(out["num_iterations"] - ref["num_iterations"])) < 0.5
+    dr = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    runtime.react_batch(tab, dr, synth.react_opts(extra_iterations=1))
+    plain = dr.to_host()
+    assert rel_err(out["pri_molal"], plain["pri_molal"]) <= 1e-9
+    assert out["num_iterations"].mean() < 0.8 * plain["num_iterations"].mean()
+
+
 @pytest.mark.parametrize("n", [1, 7, 33, 1000])
 def test_react_ragged_and_previous_guess(net15, n):
     t = net15

@@ -400,3 +400,55 @@ def test_numerical_reaction_jacobian_matches_the_analytic_one():
         scale = np.max(np.abs(an), axis=0)
         assert np.all(scale[in_mineral] > 0)
         assert np.max(np.abs(fd - an)[:, in_mineral] / scale[in_mineral]) <= 1e-4
+
+
+# ---------------------------------------------------------------- Jacobian at the Q/K cap
+def test_capped_mineral_derivative_option():
+    """PFX_JAC_CAPPED_QK: above the reference's cap ln(Q/K) = 6.90776 the
+    rate is constant, so the forward differences of the residual carry no
+    mineral term there; the option's analytic Jacobian matches them, the
+    reference's does not.  Newton then reaches the same solution (the
+    residual is untouched) in fewer iterations."""
+    t = synth.random_network(15, 50, 3, ionx=False)
+    rng = np.random.default_rng(9)
+    cells = orc.HostCells.allocate(t, 1, temperature=True)
+    cells["temperature"][:] = 25.0
+    cells["mnrl_volfrac"][:] = 0.05
+    cells["mnrl_area"][:] = 1.0
+    la_need = None
+    # raise every primary until some mineral is far above its cap
+    for scale in 10.0 ** np.arange(-6, 0, 0.25):
+        m = scale * 10.0 ** rng.uniform(-0.2, 0.2, size=t.N)
+        la = np.log(m)
+        logk_c = logk_eval(t.mnrl_logKcoef, 25.0)
+        lnq = np.array([-logk_c[k] * LN10_TRUNC + sum(t.mnrl_stoich[k, s] * la[t.mnrl_specid[k, 1 + s]]
+                                                      for s in range(t.mnrl_specid[k, 0])) for k in range(t.K)])
+        if (lnq > 8.0).any() and (lnq < 6.0).any():
+            la_need = m
+            break
+    assert la_need is not None
+    cells["pri_molal"][0] = la_need
+    cells["pri_act_coef"][0] = 1.0
+    o = synth.react_opts(den_kg=1000.0)
+    oc = synth.react_opts(den_kg=1000.0, jacobian_flags=abi.PFX_JAC_CAPPED_QK)
+    R_ref, J_ref = orc.residual_jacobian(t, cells, o)
+    R_cap, J_cap = orc.residual_jacobian(t, cells, oc)
+    t.numerical_jacobian = 1
+    _, J_fd = orc.residual_jacobian(t, cells, o)
+    t.numerical_jacobian = 0
+    assert np.array_equal(R_ref, R_cap)
+    # the capped minerals' analytic term is what separates the two Jacobians
+    d_ref = np.max(np.abs(J_ref[0] - J_fd[0]))
+    d_cap = np.max(np.abs(J_cap[0] - J_fd[0]))
+    assert d_ref > 1e-3 * np.max(np.abs(J_ref[0] - J_cap[0])) > 0.0
+    assert d_cap <= 1e-3 * d_ref
+    # same solutions, fewer iterations
+    cs, _, _, _ = orc.speciated_state(synth.random_network(15, 50, 3, ionx=True), 300, seed=5)
+    tn = synth.random_network(15, 50, 3, ionx=True)
+    a, b = cs.copy(), cs.copy()
+    orc.react_batch(tn, a, synth.react_opts(extra_iterations=1))
+    orc.react_batch(tn, b, synth.react_opts(extra_iterations=1, jacobian_flags=abi.PFX_JAC_CAPPED_QK))
+    ok = (a["status"] == 0) & (b["status"] == 0)
+    assert ok.mean() > 0.99
+    assert np.max(np.abs(a["pri_molal"][ok] - b["pri_molal"][ok]) / a["pri_molal"][ok]) <= 1e-8
+    assert b["num_iterations"].mean() < 0.8 * a["num_iterations"].mean()
