@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PFX_ABI_VERSION 2
+#define PFX_ABI_VERSION 3
 
 /* largest primary-species count handled by one cell group (row per lane) */
 #define PFX_MAX_PRIMARY 32
@@ -75,6 +75,30 @@ extern "C" {
 #define PFX_CONSTRAINT_TOTAL_SORB 5 /* sorbed (ion-exchange) total, mol/m^3 bulk
                                        (reaction.F90:1490-1492, 1608-1614);
                                        needs an exchange site                */
+#define PFX_CONSTRAINT_MINERAL 6 /* equilibrium with a mineral, ln(Q/K) = 0
+                                    (reaction.F90:1694-1718); the value is the
+                                    starting free concentration; the reaction
+                                    is constraint_rxn[j]                      */
+#define PFX_CONSTRAINT_GAS     7 /* equilibrium with a gas at the value's
+                                    partial pressure (bar; a value <= 0 is
+                                    log10 of it), ln(Q/K) = ln p
+                                    (reaction.F90:1531-1534, 1720-1745); the
+                                    reaction is constraint_rxn[j]             */
+
+/* The reaction of a mineral or gas constraint row: the primaries it holds
+   (0-based ids; water activity is 1 and drops out), their stoichiometry,
+   log10 K at the reference temperature and, with has_logK_coef, the 5
+   Maier-Kelly coefficients used for cells that carry a temperature
+   (mineral_reaction%mnrl_logK / gas%paseqlogK after
+   RUpdateTempDependentCoefs, reaction.F90:1472-1474). */
+typedef struct pfx_constraint_rxn {
+  int32_t nspec;
+  int32_t has_logK_coef;
+  int32_t specid[PFX_MAX_RXN_SPEC];
+  double stoich[PFX_MAX_RXN_SPEC];
+  double logK;
+  double logK_coef[5];
+} pfx_constraint_rxn;
 
 /* initial Newton iterate */
 #define PFX_GUESS_PREVIOUS  0   /* previous free ions (RReact's guess, reaction.F90:3370) */
@@ -271,7 +295,9 @@ int pfx_residual_jacobian(pfx_tables *tables, pfx_cells *cells,
    (reaction.F90:1308-2046) for every cell, as called per cell by
    CondControlAssignTranInitCond (condition_control.F90:725-750).
    constraint_type[N] (PFX_CONSTRAINT_*) is shared by all cells;
-   constraint_conc[ncells*N] (device) holds each cell's values.  Starts from
+   constraint_conc[ncells*N] (device) holds each cell's values;
+   constraint_rxn[N] (host; NULL when no component has a mineral or gas
+   constraint) the reactions of the mineral and gas rows.  Starts from
    1e-9 molal with unit activity coefficients, switches activity on after the
    first convergence (LAG, every iteration), and stops on the reference
    criteria max|R| < max_residual_tolerance and max|dm/m| <
@@ -280,8 +306,8 @@ int pfx_residual_jacobian(pfx_tables *tables, pfx_cells *cells,
    ionx_ref_sorbed at the solution, num_iterations, status.
    molal != 0: concentrations in mol/kg water (CHEMISTRY MOLAL). */
 int pfx_equilibrate_batch(pfx_tables *tables, pfx_cells *cells, const int32_t *constraint_type,
-                          const double *constraint_conc, int32_t molal,
-                          const pfx_react_opts *opts, pfx_stats *stats);
+                          const double *constraint_conc, const pfx_constraint_rxn *constraint_rxn,
+                          int32_t molal, const pfx_react_opts *opts, pfx_stats *stats);
 
 /* Auxiliary-variable refresh at the given free ions: RTUpdateAuxVars's cell
    loop (reactive_transport.F90:3790-3846): RTAuxVarCompute (totals, sorbed

@@ -34,7 +34,7 @@ def load():
         vp = C.c_void_p
         lib.or_react_batch.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.ReactOpts), P(abi.Stats), C.c_int]
         lib.or_residual_jacobian.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.ReactOpts), vp, vp, vp, C.c_int]
-        lib.or_equilibrate_batch.argtypes = [P(abi.ReactionDesc), vp, vp, P(abi.Cells), P(abi.ReactOpts), C.c_int,
+        lib.or_equilibrate_batch.argtypes = [P(abi.ReactionDesc), vp, vp, vp, P(abi.Cells), P(abi.ReactOpts), C.c_int,
                                              vp, C.c_int]
         lib.or_auxvar_batch.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.ReactOpts), C.c_int, C.c_int]
         lib.or_update_kinetic_state.argtypes = [P(abi.ReactionDesc), P(abi.Cells), P(abi.ReactOpts), vp, C.c_int]
@@ -123,15 +123,16 @@ def residual_jacobian(t: ReactionTables, cells: HostCells, opts: abi.ReactOpts, 
 
 
 def equilibrate(t: ReactionTables, ctype: Sequence[int], conc: np.ndarray, cells: HostCells,
-                opts: abi.ReactOpts, molal=False, nthreads=0):
+                opts: abi.ReactOpts, molal=False, nthreads=0, rxn=None):
+    """rxn: ctypes array [N] of abi.ConstraintRxn for mineral / gas rows."""
     lib = load()
     d = t.desc()
     ct = np.ascontiguousarray(ctype, dtype=np.int32)
     cc = np.ascontiguousarray(conc, dtype=np.float64).reshape(cells.ncells, t.N)
     its = np.zeros(cells.ncells, dtype=np.int32)
     cs = cells.struct()
-    rc = lib.or_equilibrate_batch(C.byref(d), _ptr(ct), _ptr(cc), C.byref(cs), C.byref(opts), int(molal),
-                                  _ptr(its), nthreads)
+    rc = lib.or_equilibrate_batch(C.byref(d), _ptr(ct), _ptr(cc), None if rxn is None else C.byref(rxn),
+                                  C.byref(cs), C.byref(opts), int(molal), _ptr(its), nthreads)
     return rc, its
 
 

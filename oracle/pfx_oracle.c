@@ -910,9 +910,12 @@ int or_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
 #define OR_CON_PH 3
 #define OR_CON_CHARGE 4
 #define OR_CON_TOTAL_SORB 5
+#define OR_CON_MINERAL 6
+#define OR_CON_GAS 7
 
 static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype, const double *conc_in,
-                               ocell *c, int use_prev, int molal, int *iters) {
+                               const pfx_constraint_rxn *rxn, int has_temp, ocell *c, int use_prev, int molal,
+                               int *iters) {
   const int N = d->naqcomp;
   double conc[NMAX], total_conc[NMAX], free_conc[NMAX];
   double molal_to_molar = molal ? c->den / 1000.0 : 1.0;
@@ -928,7 +931,20 @@ static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype,
       case OR_CON_LOG: free_conc[j] = pow(10.0, conc[j]) * molar_to_molal; break;
       case OR_CON_CHARGE: if (!use_prev) free_conc[j] = conc[j] * molar_to_molal; break;
       case OR_CON_PH: free_conc[j] = pow(10.0, -conc[j]); break;
+      case OR_CON_MINERAL: if (!use_prev) free_conc[j] = conc[j] * molar_to_molal; break; /* a guess */
+      case OR_CON_GAS: if (conc[j] <= 0.0) conc[j] = pow(10.0, conc[j]); break; /* log10 partial pressure */
       default: return -1;
+    }
+  }
+  /* ln K of the mineral / gas reactions at the cell's temperature
+     (RUpdateTempDependentCoefs, reaction.F90:1472-1474) */
+  double rxn_lnK[NMAX];
+  for (int j = 0; j < N; j++) {
+    rxn_lnK[j] = 0.0;
+    if (ctype[j] == OR_CON_MINERAL || ctype[j] == OR_CON_GAS) {
+      const pfx_constraint_rxn *x = &rxn[j];
+      const double lk = (x->has_logK_coef && has_temp) ? or_logk_eval(x->logK_coef, c->temp) : x->logK;
+      rxn_lnK[j] = -lk * OR_LOG_TO_LN;
     }
   }
   for (int j = 0; j < N; j++) c->m[j] = free_conc[j];
@@ -975,6 +991,19 @@ static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype,
           c->m[j] = pow(10.0, -conc[j]) / c->gam[j];
           J[j][j] = 1.0;
           break;
+        case OR_CON_MINERAL:
+        case OR_CON_GAS: {
+          /* reaction.F90:1694-1745: ln(Q/K), less ln p for a gas; row nu/m */
+          const pfx_constraint_rxn *x = &rxn[j];
+          double lnQK = rxn_lnK[j];
+          for (int s = 0; s < x->nspec; s++) {
+            const int jc = x->specid[s];
+            lnQK += x->stoich[s] * log(c->m[jc] * c->gam[jc]);
+          }
+          Res[j] = ctype[j] == OR_CON_MINERAL ? lnQK : lnQK - log(conc[j]);
+          for (int s = 0; s < x->nspec; s++) J[j][x->specid[s]] = x->stoich[s] / c->m[x->specid[s]];
+          break;
+        }
       }
     }
     double max_res = 0.0;
@@ -1024,9 +1053,12 @@ static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype,
    ctype[N] shared by all cells, conc[ncells*N] per cell.  Outputs written to
    cells->pri_molal/pri_act_coef/sec_act_coef/sec_molal/total/total_sorb/
    ionx_ref_sorbed. */
-int or_equilibrate_batch(const pfx_reaction_desc *d, const int32_t *ctype, const double *conc, pfx_cells *cl,
-                         const pfx_react_opts *o, int molal, int32_t *iters, int nthreads) {
+int or_equilibrate_batch(const pfx_reaction_desc *d, const int32_t *ctype, const double *conc,
+                         const pfx_constraint_rxn *rxn, pfx_cells *cl, const pfx_react_opts *o, int molal,
+                         int32_t *iters, int nthreads) {
   if (!d || !cl || !o) return PFX_E_ARG;
+  for (int j = 0; j < d->naqcomp; j++)
+    if ((ctype[j] == OR_CON_MINERAL || ctype[j] == OR_CON_GAS) && !rxn) return PFX_E_ARG;
   const int N = d->naqcomp, M = d->neqcplx, R = d->neqionxrxn;
   int64_t bad = 0;
 #ifdef _OPENMP
@@ -1040,7 +1072,7 @@ int or_equilibrate_batch(const pfx_reaction_desc *d, const int32_t *ctype, const
     for (int i = 0; i < M; i++) c.gsec[i] = 1.0;
     for (int r = 0; r < R; r++) c.ionx_ref[r] = 1e-9; /* reactive_transport_aux.F90:304 */
     int it = 0;
-    int st = or_equilibrate_cell(d, ctype + 0, conc + ci * N, &c, 0, molal, &it);
+    int st = or_equilibrate_cell(d, ctype + 0, conc + ci * N, rxn, cl->temperature != NULL, &c, 0, molal, &it);
     if (st) bad++;
     /* initial-condition refresh of the cell auxvar, which (isothermal deck,
        no datasets) is not the constraint auxvar: it starts from

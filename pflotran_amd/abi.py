@@ -5,7 +5,7 @@ the field offsets against the header compiled with the host compiler.
 """
 import ctypes as C
 
-PFX_ABI_VERSION = 2
+PFX_ABI_VERSION = 3
 PFX_MAX_PRIMARY = 32
 PFX_MAX_CPLX = 256
 PFX_MAX_RXN_SPEC = 8
@@ -24,6 +24,7 @@ PFX_CELL_CONVERGED, PFX_CELL_MAX_ITERATIONS, PFX_CELL_NONFINITE, PFX_CELL_SINGUL
 PFX_GUESS_PREVIOUS, PFX_GUESS_ARRAY, PFX_GUESS_DEFAULT, PFX_GUESS_ARRAY_FALLBACK = 0, 1, 2, 3
 PFX_CONSTRAINT_TOTAL, PFX_CONSTRAINT_FREE, PFX_CONSTRAINT_LOG, PFX_CONSTRAINT_PH, PFX_CONSTRAINT_CHARGE = 0, 1, 2, 3, 4
 PFX_CONSTRAINT_TOTAL_SORB = 5
+PFX_CONSTRAINT_MINERAL, PFX_CONSTRAINT_GAS = 6, 7
 PFX_JAC_CAPPED_QK = 1
 
 _dp = C.POINTER(C.c_double)
@@ -106,6 +107,12 @@ class ZonePlan(C.Structure):
         ("nzones", C.c_int32), ("blocks", C.c_int32 * PFX_MAX_ZONES), ("zone_ms", C.c_double * PFX_MAX_ZONES),
         ("kernel_ms", C.c_double), ("variant", C.c_int32), ("reserved", C.c_int32),
     ]
+
+
+class ConstraintRxn(C.Structure):
+    """pfx_constraint_rxn: the reaction of a mineral or gas constraint row."""
+    _fields_ = [("nspec", C.c_int32), ("has_logK_coef", C.c_int32), ("specid", C.c_int32 * PFX_MAX_RXN_SPEC),
+                ("stoich", C.c_double * PFX_MAX_RXN_SPEC), ("logK", C.c_double), ("logK_coef", C.c_double * 5)]
 
 
 class KrylovOpts(C.Structure):

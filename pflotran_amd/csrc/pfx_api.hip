@@ -31,6 +31,7 @@ struct pfx_tables {
   int64_t scratch_cells = 0;
   unsigned long long *d_stats = nullptr;  // reduction buffer
   double *d_gi = nullptr;                 // global-implicit per-cell scratch
+  pfx_constraint_rxn *d_crxn = nullptr;   // mineral / gas constraint reactions [N]
   int64_t gi_cells = 0;
   double last_ms[2] = {0.0, 0.0};         // device time of the last call's launches
   int variant[4] = {0, 0, 0, 0};          // register-cap variant per kernel mode (pfx_tables_tune)
@@ -761,6 +762,7 @@ extern "C" int pfx_tables_destroy(pfx_tables *t) {
   (void)hipFree(t->d_queue);
   (void)hipFree(t->d_stats);
   if (t->d_gi) (void)hipFree(t->d_gi);
+  if (t->d_crxn) (void)hipFree(t->d_crxn);
   if (t->d_zone_params) (void)hipFree(t->d_zone_params);
   if (t->d_finish) (void)hipFree(t->d_finish);
   if (t->d_iters) (void)hipFree(t->d_iters);
@@ -811,8 +813,8 @@ extern "C" int pfx_residual_jacobian(pfx_tables *t, pfx_cells *cells, const pfx_
 }
 
 extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int32_t *constraint_type,
-                                     const double *constraint_conc, int32_t molal, const pfx_react_opts *o,
-                                     pfx_stats *stats) {
+                                     const double *constraint_conc, const pfx_constraint_rxn *constraint_rxn,
+                                     int32_t molal, const pfx_react_opts *o, pfx_stats *stats) {
   if (!t || !cells || !o || !constraint_type || !constraint_conc || !cells->pri_molal || !cells->total)
     return PFX_E_ARG;
   if (cells->ncells < 0 || cells->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
@@ -821,12 +823,20 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
     memset(stats, 0, sizeof(*stats));
     stats->ncells = cells->ncells;
   }
-  int nph = 0;
+  int nph = 0, nrxn = 0;
   for (int j = 0; j < t->L.N; j++) {
     const int c = constraint_type[j];
-    if (c < PFX_CONSTRAINT_TOTAL || c > PFX_CONSTRAINT_TOTAL_SORB) return PFX_E_UNSUP;
+    if (c < PFX_CONSTRAINT_TOTAL || c > PFX_CONSTRAINT_GAS) return PFX_E_UNSUP;
     if (c == PFX_CONSTRAINT_TOTAL_SORB && t->L.R == 0) return PFX_E_ARG;
     nph += c == PFX_CONSTRAINT_PH;
+    if (c == PFX_CONSTRAINT_MINERAL || c == PFX_CONSTRAINT_GAS) {
+      if (!constraint_rxn) return PFX_E_ARG;
+      const pfx_constraint_rxn &x = constraint_rxn[j];
+      if (x.nspec < 1 || x.nspec > PFX_MAX_RXN_SPEC) return PFX_E_ARG;
+      for (int s = 0; s < x.nspec; s++)
+        if (x.specid[s] < 0 || x.specid[s] >= t->L.N) return PFX_E_ARG;
+      nrxn++;
+    }
   }
   if (nph > 1) return PFX_E_ARG;
   if (cells->ncells == 0) return 0;
@@ -840,6 +850,13 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
   for (int j = 0; j < t->L.N; j++) P.has_charge |= constraint_type[j] == PFX_CONSTRAINT_CHARGE;
   P.has_sorb = 0;
   for (int j = 0; j < t->L.N; j++) P.has_sorb |= constraint_type[j] == PFX_CONSTRAINT_TOTAL_SORB;
+  P.crxn = nullptr;
+  if (nrxn > 0) {
+    if (!t->d_crxn) HIPCHK(hipMalloc((void **)&t->d_crxn, PFX_MAX_PRIMARY * sizeof(pfx_constraint_rxn)));
+    HIPCHK(hipMemcpyAsync(t->d_crxn, constraint_rxn, t->L.N * sizeof(pfx_constraint_rxn), hipMemcpyHostToDevice,
+                          stream));
+    P.crxn = t->d_crxn;
+  }
   P.molal = molal;
   if (P.max_iterations <= 0) P.max_iterations = 10000;
   int rc = use_scratch(t, P, cells->ncells);

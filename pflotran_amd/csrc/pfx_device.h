@@ -90,6 +90,7 @@ struct pfx_kernel_params {
   int32_t ctype[PFX_MAX_PRIMARY];  // PFX_CONSTRAINT_* per component
   int32_t has_charge;              // some component carries the charge balance
   int32_t has_sorb;                // some component is constrained by its sorbed total
+  const pfx_constraint_rxn *crxn;  // [N] mineral / gas constraint reactions (device), or null
   int32_t molal;                   // constraint concentrations in mol/kg
   int32_t aux_passes;              // activity passes of the aux-var refresh
   // work queue

@@ -147,6 +147,16 @@ __device__ __forceinline__ double logk_5term(int cfo, double tk, double ln_tk) {
   v = v + LD(cfo, 4) / (tk * tk);
   return v;
 }
+// the same from a coefficient array in global memory
+__device__ __forceinline__ double logk_5term_ptr(const double *cf, double tk, double ln_tk) {
+#pragma clang fp contract(off)
+  double v = cf[0] * ln_tk;
+  v = v + cf[1];
+  v = v + cf[2] * tk;
+  v = v + cf[3] / tk;
+  v = v + cf[4] / (tk * tk);
+  return v;
+}
 
 // per-group scratch slice (doubles), as byte offsets from the slice base:
 //   C[M+1] complex molalities (C[M] = 0, padding slot), gsec[M] complex
@@ -322,6 +332,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   int ctype = PFX_CONSTRAINT_FREE, nit_on = 0;
   bool compute_act = false;
   double tcon = 0.0, free_conc = 1e-9;
+  double clnK = 0.0;  // -ln K of the lane's mineral / gas constraint reaction
   double ionx_state[PFX_MAX_IONX_RXN];
 #pragma unroll
   for (int q = 0; q < PFX_MAX_IONX_RXN; q++) ionx_state[q] = 1e-9;
@@ -399,6 +410,13 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             else if (ctype == PFX_CONSTRAINT_LOG) free_conc = pow(10.0, conc) * molar_to_molal;
             else if (ctype == PFX_CONSTRAINT_PH) { free_conc = pow(10.0, -conc); tcon = conc; }
             else if (ctype == PFX_CONSTRAINT_CHARGE) free_conc = conc * molar_to_molal;
+            else if (ctype == PFX_CONSTRAINT_MINERAL) free_conc = conc * molar_to_molal;
+            else if (ctype == PFX_CONSTRAINT_GAS) tcon = conc <= 0.0 ? pow(10.0, conc) : conc;  // bar
+            if ((ctype == PFX_CONSTRAINT_MINERAL || ctype == PFX_CONSTRAINT_GAS) && P.crxn) {
+              const pfx_constraint_rxn &X = P.crxn[r];
+              const double lk = (X.has_logK_coef && P.temperature) ? logk_5term_ptr(X.logK_coef, tk, ln_tk) : X.logK;
+              clnK = -lk * PFX_LOG_TO_LN;
+            }
             m = live ? free_conc : 1.0;
           } else {
             if (live) {
@@ -870,6 +888,21 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         Rr = sorb_out - tcon;
 #pragma unroll
         for (int k = 0; k < G; k++) row[k] = srow[k];
+      } else if (ctype == PFX_CONSTRAINT_MINERAL || ctype == PFX_CONSTRAINT_GAS) {
+        // ln(Q/K) of the mineral, or of the gas less ln p (reaction.F90:1694-1745)
+        const pfx_constraint_rxn &X = P.crxn[r];
+        double lnQK = clnK;
+        for (int s = 0; s < X.nspec; s++) lnQK += X.stoich[s] * SD(S.la, X.specid[s]);
+        Rr = ctype == PFX_CONSTRAINT_MINERAL ? lnQK : lnQK - log(tcon);
+#pragma unroll
+        for (int k = 0; k < G; k++) row[k] = 0.0;
+        for (int s = 0; s < X.nspec; s++) {
+          const int sp = X.specid[s];
+          const double v = X.stoich[s];
+#pragma unroll
+          for (int k = 0; k < G; k++)
+            if (k == sp) row[k] += v / SD(S.mv, k);
+        }
       } else if (ctype == PFX_CONSTRAINT_CHARGE) {
         // row set above
       } else {

@@ -398,6 +398,46 @@ class ReactionTables:
         return {"flops": flops, "transcendentals": transc}
 
 
+def constraint_rxn(db: Database, name: str, primaries: Sequence[str], kind: str = "mineral",
+                   temperature: float = 25.0):
+    """abi.ConstraintRxn of a mineral or gas constraint (the species of the
+    database reaction, water dropped; log K at the reference temperature as
+    build_tables takes it, plus the Maier-Kelly coefficients)."""
+    r = (db.minerals if kind == "mineral" else db.gases)[name]
+    pidx = {n: i for i, n in enumerate(primaries)}
+    x = abi.ConstraintRxn()
+    k = 0
+    for st, sp in r.species:
+        if sp in H2O_NAMES:
+            continue
+        if sp not in pidx:
+            raise ValueError(f"{name}: species {sp} is not a primary (basis swap unsupported)")
+        x.specid[k] = pidx[sp]
+        x.stoich[k] = st
+        k += 1
+    x.nspec = k
+    temps = db.temperatures
+    itemp = next((i for i, t in enumerate(temps) if abs(t - temperature) < 1e-10), None)
+    coef = logk_fit(temps, r.logK)
+    if itemp is not None and abs(r.logK[itemp] - 500.0) > 1e-10:
+        x.logK = r.logK[itemp]
+    else:
+        x.logK = float(logk_eval(coef, temperature))
+    x.has_logK_coef = 1
+    for i in range(5):
+        x.logK_coef[i] = coef[i]
+    return x
+
+
+def constraint_rxns(t: "ReactionTables", rxns: Dict[int, "abi.ConstraintRxn"]):
+    """ctypes array [N] of constraint reactions (components without one hold
+    an empty entry)."""
+    arr = (abi.ConstraintRxn * t.N)()
+    for j, x in rxns.items():
+        arr[j] = x
+    return arr
+
+
 def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[str] = (),
                  minerals: Dict[str, MineralKinetics] | None = None,
                  ion_exchange: Sequence[IonExchangeRxn] = (), *, temperature: float = 25.0,

@@ -14,6 +14,7 @@ program pfx_abi_probe
   type(pfx_newton_opts) :: nopts
   type(pfx_newton_stats) :: nstats
   type(pfx_krylov_stats) :: kstats
+  type(pfx_constraint_rxn) :: crxn
   type(c_ptr) :: tables, solver
   integer(c_int) :: rc
 
@@ -36,4 +37,5 @@ program pfx_abi_probe
   print '(a,i0)', 'sizeof_newton_opts ', c_sizeof(nopts)
   print '(a,i0)', 'sizeof_newton_stats ', c_sizeof(nstats)
   print '(a,i0)', 'sizeof_krylov_stats ', c_sizeof(kstats)
+  print '(a,i0)', 'sizeof_constraint_rxn ', c_sizeof(crxn)
 end program pfx_abi_probe

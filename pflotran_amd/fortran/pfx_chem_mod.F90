@@ -10,7 +10,7 @@ module pfx_chem_module
 
   private
 
-  integer(c_int), parameter, public :: PFX_ABI_EXPECTED = 2
+  integer(c_int), parameter, public :: PFX_ABI_EXPECTED = 3
   integer(c_int), parameter, public :: PFX_E_ARG = -1, PFX_E_DEVICE = -2, &
                                        PFX_E_NOMEM = -3, PFX_E_UNSUP = -4
   integer(c_int), parameter, public :: PFX_ACT_OFF = 0, PFX_ACT_TIMESTEP = 1, &
@@ -22,7 +22,8 @@ module pfx_chem_module
   integer(c_int), parameter, public :: PFX_CONSTRAINT_TOTAL = 0, &
                                        PFX_CONSTRAINT_FREE = 1, &
                                        PFX_CONSTRAINT_LOG = 2, PFX_CONSTRAINT_PH = 3, &
-                                       PFX_CONSTRAINT_CHARGE = 4, PFX_CONSTRAINT_TOTAL_SORB = 5
+                                       PFX_CONSTRAINT_CHARGE = 4, PFX_CONSTRAINT_TOTAL_SORB = 5, &
+                                       PFX_CONSTRAINT_MINERAL = 6, PFX_CONSTRAINT_GAS = 7
   integer(c_int), parameter, public :: PFX_GUESS_PREVIOUS = 0, PFX_GUESS_ARRAY = 1, &
                                        PFX_GUESS_DEFAULT = 2, PFX_GUESS_ARRAY_FALLBACK = 3
 
@@ -66,6 +67,15 @@ module pfx_chem_module
     type(c_ptr) :: stream
     integer(c_int32_t) :: fallback_after, jacobian_flags
   end type pfx_react_opts
+
+  ! the reaction of a mineral or gas constraint row
+  type, bind(c), public :: pfx_constraint_rxn
+    integer(c_int32_t) :: nspec, has_logK_coef
+    integer(c_int32_t) :: specid(8)
+    real(c_double) :: stoich(8)
+    real(c_double) :: logK
+    real(c_double) :: logK_coef(5)
+  end type pfx_constraint_rxn
 
   type, bind(c), public :: pfx_grid
     integer(c_int32_t) :: nx, ny, nz, log_formulation
@@ -166,13 +176,14 @@ module pfx_chem_module
     end function pfx_residual_jacobian
 
     function pfx_equilibrate_batch(tables, cells, constraint_type, &
-                                   constraint_conc, molal, opts, stats) &
+                                   constraint_conc, constraint_rxn, molal, opts, stats) &
         bind(c, name='pfx_equilibrate_batch')
       import :: c_int, c_int32_t, c_ptr, pfx_cells, pfx_react_opts, pfx_stats
       type(c_ptr), value :: tables
       type(pfx_cells), intent(inout) :: cells
       integer(c_int32_t), intent(in) :: constraint_type(*)
       type(c_ptr), value :: constraint_conc
+      type(c_ptr), value :: constraint_rxn   ! pfx_constraint_rxn(N) or c_null_ptr
       integer(c_int32_t), value :: molal
       type(pfx_react_opts), intent(in) :: opts
       type(pfx_stats), intent(out) :: stats

@@ -57,7 +57,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_tables_destroy.argtypes = [C.c_void_p]
     lib.pfx_react_batch.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), P(abi.Stats)]
     lib.pfx_residual_jacobian.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), C.c_void_p, C.c_void_p]
-    lib.pfx_equilibrate_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_void_p, C.c_void_p, C.c_int32,
+    lib.pfx_equilibrate_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                           P(abi.ReactOpts), P(abi.Stats)]
     lib.pfx_auxvar_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_int32, P(abi.ReactOpts)]
     lib.pfx_update_kinetic_state.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts)]
@@ -211,8 +211,10 @@ def residual_jacobian(tables: Tables, cells: DeviceCells, opts: abi.ReactOpts):
 
 
 def equilibrate(tables: Tables, cells: DeviceCells, constraint_type, constraint_conc, opts: abi.ReactOpts,
-                molal: bool = False):
-    """pfx_equilibrate_batch; constraint_conc is a [n, N] array (host or device)."""
+                molal: bool = False, rxn=None):
+    """pfx_equilibrate_batch; constraint_conc is a [n, N] array (host or
+    device); rxn a ctypes array [N] of abi.ConstraintRxn for mineral / gas
+    rows (database.constraint_rxns)."""
     lib = load_library()
     torch = _torch()
     ct = np.ascontiguousarray(constraint_type, dtype=np.int32)
@@ -223,8 +225,8 @@ def equilibrate(tables: Tables, cells: DeviceCells, constraint_type, constraint_
     st = abi.Stats()
     if opts.stream is None:
         opts.stream = torch.cuda.current_stream().cuda_stream
-    rc = lib.pfx_equilibrate_batch(tables.handle, C.byref(cs), ct.ctypes.data, cc.data_ptr(), int(molal),
-                                   C.byref(opts), C.byref(st))
+    rc = lib.pfx_equilibrate_batch(tables.handle, C.byref(cs), ct.ctypes.data, cc.data_ptr(),
+                                   None if rxn is None else C.byref(rxn), int(molal), C.byref(opts), C.byref(st))
     if rc < 0:
         raise PfxError(f"pfx_equilibrate_batch failed: {rc}")
     return rc, st

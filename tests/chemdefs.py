@@ -3,11 +3,12 @@
 import os
 
 from pflotran_amd import abi
-from pflotran_amd.database import IonExchangeRxn, MineralKinetics, build_tables, read_database
+from pflotran_amd.database import (IonExchangeRxn, MineralKinetics, build_tables, constraint_rxn, constraint_rxns,
+                                   read_database)
 
 DB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "db")
 
-CON_TOTAL, CON_FREE, CON_LOG, CON_PH, CON_CHARGE, CON_TOTAL_SORB = 0, 1, 2, 3, 4, 5
+CON_TOTAL, CON_FREE, CON_LOG, CON_PH, CON_CHARGE, CON_TOTAL_SORB, CON_MINERAL, CON_GAS = 0, 1, 2, 3, 4, 5, 6, 7
 
 
 def carbonate(activity):
@@ -56,6 +57,20 @@ def valocchi_sorbed(golds_sorbed):
     setup = dict(setup, ctype=[CON_TOTAL, CON_TOTAL_SORB, CON_TOTAL_SORB, CON_CHARGE],
                  conc=[setup["conc"][0], golds_sorbed[0], golds_sorbed[1], setup["conc"][3]])
     return t, setup
+
+
+def calcite_column_initial(activity=True):
+    """The initial constraint of the 1-D calcite transport decks
+    (shortcourse/1D_Calcite/calcite_tran_only.in, ascem/1d/1d-calcite):
+    H+ 1e-8 free, HCO3- in equilibrium with CO2(g) at 1e-3 bar, Ca++ in
+    equilibrium with calcite (5e-4 the starting value), porosity 0.25."""
+    db = read_database(os.path.join(DB, "hanford_subset.dat"))
+    prim = ["H+", "HCO3-", "Ca++"]
+    t = build_tables(db, prim, ["OH-", "CO3--", "CO2(aq)", "CaCO3(aq)", "CaHCO3+", "CaOH+"],
+                     act_freq=abi.PFX_ACT_TIMESTEP if activity else abi.PFX_ACT_OFF)
+    rxn = constraint_rxns(t, {1: constraint_rxn(db, "CO2(g)", prim, "gas"),
+                              2: constraint_rxn(db, "Calcite", prim, "mineral")})
+    return t, dict(ctype=[CON_FREE, CON_GAS, CON_MINERAL], conc=[1e-8, 1e-3, 5e-4], porosity=0.25, rxn=rxn)
 
 
 def carbonate_column():

@@ -19,9 +19,10 @@ BATCH = os.path.join(REF, "regression_tests", "ascem", "batch")
 for name in ("carbonate.dat", "ca-carbonate.dat", "calcite.dat"):
     shutil.copy(os.path.join(BATCH, name), os.path.join(HERE, "db", name))
 
-# excerpt of database/hanford.dat: the species of the 1-D calcite column
-# (5 primaries, 8 complexes, calcite) and of the Valocchi ion-exchange test
-KEEP = {"H2O", "H+", "HCO3-", "Ca++", "Mg++", "Cl-", "Na+",
+# excerpt of database/hanford.dat: the species of the 1-D calcite columns
+# (5 primaries, 8 complexes, calcite; CaOH+ and the CO2(g) constraint of the
+# calcite transport decks) and of the Valocchi ion-exchange test
+KEEP = {"H2O", "H+", "HCO3-", "Ca++", "Mg++", "Cl-", "Na+", "CO2(g)", "CaOH+",
         "OH-", "CO2(aq)", "CO3--", "CaHCO3+", "CaCO3(aq)", "MgHCO3+", "MgCO3(aq)", "CaCl+",
         "Calcite", "Halite"}
 out = []

@@ -58,7 +58,8 @@ def test_every_header_compiles_as_c(tmp_path):
                                           (abi.ReactOpts, "pfx_react_opts"), (abi.Stats, "pfx_stats"),
                                           (abi.Grid, "pfx_grid"), (abi.ZonePlan, "pfx_zone_plan"),
                                           (abi.KrylovOpts, "pfx_krylov_opts"), (abi.KrylovStats, "pfx_krylov_stats"),
-                                          (abi.NewtonOpts, "pfx_newton_opts"), (abi.NewtonStats, "pfx_newton_stats")])
+                                          (abi.NewtonOpts, "pfx_newton_opts"), (abi.NewtonStats, "pfx_newton_stats"),
+                                          (abi.ConstraintRxn, "pfx_constraint_rxn")])
 def test_struct_layout_matches_header(tmp_path, struct, cname):
     fields = [f for f, _ in struct._fields_]
     src = ['#include <stddef.h>', '#include <stdio.h>'] + [f'#include "{h}"' for h in HEADERS] + ['int main(void){',

@@ -37,3 +37,4 @@ def test_fortran_module_binds_the_c_abi(tmp_path):
     assert int(vals["sizeof_newton_opts"]) == C.sizeof(abi.NewtonOpts)
     assert int(vals["sizeof_newton_stats"]) == C.sizeof(abi.NewtonStats)
     assert int(vals["sizeof_krylov_stats"]) == C.sizeof(abi.KrylovStats)
+    assert int(vals["sizeof_constraint_rxn"]) == C.sizeof(abi.ConstraintRxn)

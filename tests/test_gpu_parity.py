@@ -227,7 +227,7 @@ def _gpu_speciate(t, setup, ncells=1):
     dc = runtime.DeviceCells.from_host(arr, ncells)
     o = synth.react_opts(porosity=setup["porosity"])
     conc = np.tile(np.array(setup["conc"], dtype=np.float64), (ncells, 1))
-    rc, st = runtime.equilibrate(tab, dc, setup["ctype"], conc, o)
+    rc, st = runtime.equilibrate(tab, dc, setup["ctype"], conc, o, rxn=setup.get("rxn"))
     assert rc == 0, st.as_dict()
     dc.t["pri_act_coef"].fill_(1.0)
     dc.t["sec_act_coef"].fill_(1.0)
@@ -317,6 +317,25 @@ def test_gpu_gold_valocchi_charge_balance(golds):
         assert abs(out["total_sorb"][0][j] - g_s) <= 1e-12 * max(abs(g_s), 1.0)
     # charge balance holds at the solution: sum z_j T_j = 0
     assert abs(np.dot(t.Z, out["total"][0])) < 1e-11
+
+
+@pytest.mark.parametrize("activity", [False, True])
+def test_gpu_mineral_and_gas_constraints(activity):
+    """The calcite decks' initial constraint (H+ free, HCO3- by CO2(g) at
+    1e-3 bar, Ca++ by calcite) on the GPU against the oracle's solution."""
+    from chemdefs import calcite_column_initial
+    t, setup = calcite_column_initial(activity)
+    tab, dc = _gpu_speciate(t, setup, ncells=3)
+    out = dc.to_host()
+    ref = orc.HostCells.allocate(t, 3)
+    rc, _ = orc.equilibrate(t, setup["ctype"], np.tile(np.array(setup["conc"]), (3, 1)), ref,
+                            synth.react_opts(porosity=setup["porosity"]), rxn=setup["rxn"])
+    assert rc == 0
+    assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
+    if not activity:
+        la = np.log(out["pri_molal"])
+        assert np.max(np.abs(-1.8487 * 2.30258509299 - la[:, 0] + la[:, 2] + la[:, 1])) <= 1e-10
+        assert np.max(np.abs(7.8136 * 2.30258509299 + la[:, 0] + la[:, 1] - np.log(1e-3))) <= 1e-10
 
 
 def test_gpu_sorbed_total_constraints(golds):

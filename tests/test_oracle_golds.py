@@ -21,7 +21,7 @@ def speciate(t, setup):
         cells["mnrl_volfrac"][0, 0] = setup["vf"]
         cells["mnrl_area"][0, 0] = setup["area"]
     o = abi.default_opts(porosity=setup["porosity"], den_kg=DEN)
-    rc, its = orc.equilibrate(t, setup["ctype"], np.array(setup["conc"]), cells, o)
+    rc, its = orc.equilibrate(t, setup["ctype"], np.array(setup["conc"]), cells, o, rxn=setup.get("rxn"))
     assert rc == 0 and its[0] > 0
     return cells, o
 
@@ -82,6 +82,35 @@ def test_sorbed_total_constraints_recover_the_valocchi_state(golds):
         assert abs(cells["total"][0][j] - g_t) <= 1e-9 * abs(g_t)
         g_s = gold(golds, name, f"CONCENTRATION: Total Sorbed {sp}")
         assert abs(cells["total_sorb"][0][j] - g_s) <= 1e-9 * max(abs(g_s), 1.0)
+
+
+@pytest.mark.parametrize("activity", [False, True])
+def test_mineral_and_gas_constraints(activity):
+    """Mineral and gas constraint rows (reaction.F90:1694-1745) on the calcite
+    decks' initial constraint.  With unit activity the solution has
+    ln(Q/K) = 0 for calcite and ln(Q/K) = ln(1e-3 bar) for CO2(g) (from the
+    database's log K, independently of the oracle); with activity on, the
+    coefficients of the solve are not those of the refresh that follows, so
+    the pin there is the inverse: speciating again from the solution's totals
+    (TOTAL rows, H+ free) returns the same free ions."""
+    from chemdefs import CON_FREE, CON_TOTAL, calcite_column_initial
+    t, setup = calcite_column_initial(activity)
+    cells, _ = speciate(t, setup)
+    m = cells["pri_molal"][0]
+    assert abs(m[0] - 1e-8 * 1000.0 / DEN) <= 1e-20
+    if not activity:
+        # hanford.dat at 25 C: Calcite = -H+ + Ca++ + HCO3-, log K 1.8487;
+        # CO2(g) = -H2O + H+ + HCO3-, log K -7.8136
+        la = np.log(m)
+        assert abs(-1.8487 * 2.30258509299 - la[0] + la[2] + la[1]) <= 1e-10
+        assert abs(7.8136 * 2.30258509299 + la[0] + la[1] - np.log(1e-3)) <= 1e-10
+    again = dict(setup, ctype=[CON_FREE, CON_TOTAL, CON_TOTAL],
+                 conc=[setup["conc"][0], cells["total"][0][1], cells["total"][0][2]])
+    again.pop("rxn")
+    cells2, _ = speciate(t, again)
+    # the activity-coupled stop (|dm/m| < 1e-6 one iteration after activity
+    # is on) leaves both solutions within a few 1e-9 of the fixed point
+    assert np.max(np.abs(cells2["pri_molal"][0] - m) / m) <= (1e-8 if activity else 1e-10)
 
 
 def test_calcite_kinetics(golds):
