@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="plain method only")
     ap.add_argument("--no-gi", action="store_true", help="skip the global-implicit assembly leg")
     ap.add_argument("--no-zones", action="store_true", help="skip the chemistry-zones leg")
+    ap.add_argument("--no-capped", action="store_true",
+                    help="skip the legs with the consistent derivative above the Q/K cap")
     ap.add_argument("--zones-scale", type=float, default=1.0, help="fraction of the 2M zoned cells")
     ap.add_argument("--gi-side", type=int, default=128, help="cells per side of the global-implicit grid")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -215,7 +217,7 @@ def main():
         # above the Q/K cap, where the rate is constant): same residual and
         # acceptance test, a consistent Jacobian for the capped cells
         cap = {}
-        for key, opts in (("plain", synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT,
+        for key, opts in () if args.no_capped else (("plain", synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT,
                                                      jacobian_flags=abi.PFX_JAC_CAPPED_QK)),
                           ("warm", synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK,
                                                     fallback_after=args.fallback_after,
@@ -235,14 +237,15 @@ def main():
                         "fallbacks": int(st.fallbacks)}
             log(f"{key} with PFX_JAC_CAPPED_QK: {ms:.2f} ms/step, {cap[key]['cells_per_s']:.4g} cells/s, "
                 f"{cap[key]['mean_iterations']:.2f} it/cell")
-        cap["warm_speedup_vs_plain"] = cap["warm"]["cells_per_s"] / cap["plain"]["cells_per_s"]
-        cap["warm_speedup_vs_reference_jacobian_plain"] = cap["warm"]["cells_per_s"] / plain_value
-        both = ((dc.t["status"] == 0) & (gpu_status == 0)).unsqueeze(1)
-        rel = ((dc.t["pri_molal"] - gpu_m).abs() / gpu_m.abs()).masked_fill(~both, 0.0)
-        cap["max_rel_diff_free_ions_vs_reference_jacobian_warm"] = float(rel.max().item())
-        cap["p99_rel_diff_free_ions_vs_reference_jacobian_warm"] = float(
-            torch.quantile(rel.amax(dim=1)[:200_000].float(), 0.99).item())
-        warm["capped_qk"] = cap
+        if cap:
+            cap["warm_speedup_vs_plain"] = cap["warm"]["cells_per_s"] / cap["plain"]["cells_per_s"]
+            cap["warm_speedup_vs_reference_jacobian_plain"] = cap["warm"]["cells_per_s"] / plain_value
+            both = ((dc.t["status"] == 0) & (gpu_status == 0)).unsqueeze(1)
+            rel = ((dc.t["pri_molal"] - gpu_m).abs() / gpu_m.abs()).masked_fill(~both, 0.0)
+            cap["max_rel_diff_free_ions_vs_reference_jacobian_warm"] = float(rel.max().item())
+            cap["p99_rel_diff_free_ions_vs_reference_jacobian_warm"] = float(
+                torch.quantile(rel.amax(dim=1)[:200_000].float(), 0.99).item())
+            warm["capped_qk"] = cap
     else:
         value, ms_per_step, st_main, kern_ms = plain_value, plain_ms_step, st_plain, plain_kernel_ms
         plain_step()
@@ -282,7 +285,8 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": "profiles/pmc_traffic_pfx_react_kernel_1M.json (FETCH_SIZE x2.0, "
                                        "WRITE_SIZE x1.0 from a 256 MiB copy)" if traffic else None,
-                     "kernel": "pfx_react_kernel<16,0>", "kernel_ms": kern_ms,
+                     "kernel": f"pfx_react_kernel<16, 0, {tab.tune(abi.PFX_KERNEL_REACT) + 1}>",
+                     "kernel_ms": kern_ms,
                      "flops_per_iteration": fl["flops"], "transcendentals_per_iteration": fl["transcendentals"],
                      "note": "FP64 VALU-bound Newton; MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s",
                      "algorithmic_bytes_per_cell": bytes_per_cell,
