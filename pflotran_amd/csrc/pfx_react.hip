@@ -162,13 +162,17 @@ __device__ __forceinline__ double logk_5term_ptr(const double *cf, double tk, do
 //   C[M+1] complex molalities (C[M] = 0, padding slot), gsec[M] complex
 //   activity coefficients, lnK[M] = -log10 K * LOG_TO_LN of this cell,
 //   la[G] ln activities, mv[G] free ions, im[G] inverse free ions,
-//   xb[G] ion-exchange fractions, mlnK/mkeff/marea/mvf/mrate[K] per mineral
+//   xb[G] ion-exchange fractions, mlnK/mkeff/marea/mvf/mrate[K] per mineral,
+//   cs[4] the cell's porosity, saturation, volume and water density,
+//   ionx[PFX_MAX_IONX_RXN] the exchange state, fx[G] the fixed accumulation
+//   of each row.  The last three hold cell state that lives across Newton
+//   passes: in LDS it does not occupy registers for the whole loop.
 struct Scr {
-  int C, gsec, lnK, la, mv, im, xb, mlnK, mkeff, marea, mvf, mrate;
+  int C, gsec, lnK, la, mv, im, xb, mlnK, mkeff, marea, mvf, mrate, cs, ionx, fx;
 };
 
 __host__ __device__ inline int scr_doubles(int M, int G, int K) {
-  int n = (M + 1) + 2 * M + 4 * G + 5 * K;
+  int n = (M + 1) + 2 * M + 5 * G + 5 * K + 4 + PFX_MAX_IONX_RXN;
   return (n + 1) & ~1;
 }
 
@@ -186,7 +190,10 @@ __device__ __forceinline__ Scr make_scr(int base, int M, int G, int K) {
   s.mkeff = p; p += K * 8;
   s.marea = p; p += K * 8;
   s.mvf = p; p += K * 8;
-  s.mrate = p;
+  s.mrate = p; p += K * 8;
+  s.cs = p; p += 4 * 8;
+  s.ionx = p; p += PFX_MAX_IONX_RXN * 8;
+  s.fx = p;
   return s;
 }
 
@@ -309,11 +316,13 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   const int gbase = lane - r;
   const bool live = r < N;  // padding lanes carry an identity row
   const int sdoubles = scr_doubles(M, G, K);
-  const Scr S = make_scr(lds_tab + (int)(threadIdx.x / G) * sdoubles * 8, M, G, K);
-  if (r == 0) SD(S.C, M) = 0.0;
-
-  const double Zr = live ? LD(P0->L.o_Z, r) : 0.0;
-  const double a0r = live ? LD(P0->L.o_a0, r) : 0.0;
+  // The group's scratch offsets, and the lane's charge and ion size, are
+  // rebuilt every pass from opaque copies of their base values: held across
+  // the persistent loop they cost ~15 registers that the register-capped
+  // variants spilled to scratch and reloaded on every pass.
+  int sbase = lds_tab + (int)(threadIdx.x / G) * sdoubles * 8;
+  int r_opq = r;
+  if (r == 0) SD(make_scr(sbase, M, G, K).C, M) = 0.0;
 
   // wave-level work queue state (identical in all lanes of the wave)
   long long q_next = 0, q_end = 0;
@@ -326,22 +335,23 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   bool fell_back = false;   // warm attempt abandoned, restarted from the default guess
   bool stop_next = false;   // reference relative-change stop: finish at the next evaluation
   int cell_flag = PFX_CELL_CONVERGED;
-  double m = 1.0, gam = 1.0, lng = 0.0, fixed = 0.0, tot_out = 0.0, sorb_out = 0.0;
-  double porosity = 0, sat = 0, vol = 0, den = 0;
+  double m = 1.0, gam = 1.0, lng = 0.0;
   // constraint speciation state
   int ctype = PFX_CONSTRAINT_FREE, nit_on = 0;
   bool compute_act = false;
   double tcon = 0.0, free_conc = 1e-9;
   double clnK = 0.0;  // -ln K of the lane's mineral / gas constraint reaction
-  double ionx_state[PFX_MAX_IONX_RXN];
-#pragma unroll
-  for (int q = 0; q < PFX_MAX_IONX_RXN; q++) ionx_state[q] = 1e-9;
 
   for (;;) {
     asm volatile("" : "+s"(praw));
     KP Pp = (KP)praw;
 #define P (*Pp)
 #define L (Pp->L)
+    asm volatile("" : "+v"(sbase), "+v"(r_opq));
+    const Scr S = make_scr(sbase, L.M, G, L.K);
+    const double Zr = live ? LD(L.o_Z, r_opq) : 0.0;
+    const double a0r = live ? LD(L.o_a0, r_opq) : 0.0;
+    double tot_out = 0.0, sorb_out = 0.0;  // totals and sorbed totals of this pass
     // ---------------------------------------------------------------
     // refill: groups without a cell take the next ones from the queue
     {
@@ -388,10 +398,16 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           compute_act = false;
           nit_on = 0;
           // ---- load the cell
-          porosity = P.porosity ? P.porosity[c] : P.def_porosity;
-          sat = P.saturation ? P.saturation[c] : P.def_saturation;
-          vol = P.volume ? P.volume[c] : P.def_volume;
-          den = P.den_kg ? P.den_kg[c] : P.def_den;
+          const double porosity = P.porosity ? P.porosity[c] : P.def_porosity;
+          const double sat = P.saturation ? P.saturation[c] : P.def_saturation;
+          const double vol = P.volume ? P.volume[c] : P.def_volume;
+          const double den = P.den_kg ? P.den_kg[c] : P.def_den;
+          if (r == 0) {
+            SD(S.cs, 0) = porosity;
+            SD(S.cs, 1) = sat;
+            SD(S.cs, 2) = vol;
+            SD(S.cs, 3) = den;
+          }
           const double tempc = P.temperature ? P.temperature[c] : 25.0;
           const double tk = tempc + 273.15;
           const double ln_tk = log(tk);
@@ -425,12 +441,12 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
               if (MODE != PFX_MODE_AUX) {
                 const double tot_in = P.total[c * N + r];
                 const double sorb_old = (R > 0 && P.total_sorb) ? P.total_sorb[c * N + r] : 0.0;
-                fixed = porosity * sat * 1000.0 * vol * tot_in + (R > 0 ? sorb_old * vol : 0.0);
+                SD(S.fx, r) = porosity * sat * 1000.0 * vol * tot_in + (R > 0 ? sorb_old * vol : 0.0);
               }
             } else {
               m = 1.0;
               gam = 1.0;
-              fixed = 0.0;
+              SD(S.fx, r) = 0.0;
             }
             for (int cc = r; cc < M; cc += G) SD(S.gsec, cc) = P.sec_act_coef ? P.sec_act_coef[c * M + cc] : 1.0;
           }
@@ -459,7 +475,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           }
 #pragma unroll
           for (int q = 0; q < PFX_MAX_IONX_RXN; q++)
-            ionx_state[q] = (MODE != PFX_MODE_EQUIL && q < R && P.ionx_ref) ? P.ionx_ref[c * R + q] : 1e-9;
+            SD(S.ionx, q) = (MODE != PFX_MODE_EQUIL && q < R && P.ionx_ref) ? P.ionx_ref[c * R + q] : 1e-9;
           lng = log(gam);
           if (MODE == PFX_MODE_REACT && L.act_freq == PFX_ACT_OFF) {
             phase = 2;
@@ -633,7 +649,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     // activity update)
     double row[G];
     double Tsum = 0.0, absum = 0.0;
-    const double den_kg_per_L = den * 1.0e-3;
+    const double den_kg_per_L = SD(S.cs, 3) * 1.0e-3;
     if (active && !(MODE == PFX_MODE_REACT && phase == 0)) {
       double diag = 0.0;
       const int maxdeg = L.maxdeg;
@@ -699,7 +715,8 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 #pragma unroll
     for (int k = 0; k < G; k++) srow[k] = 0.0;
     const double dt = P.dt;
-    const double psv_t = porosity * sat * 1000.0 * vol;
+    const double vol = SD(S.cs, 2);
+    const double psv_t = SD(S.cs, 0) * SD(S.cs, 1) * 1000.0 * vol;
     if (active && R > 0 && (need_ionx || ionx_final)) {
       const double ionx_w = need_ionx ? vol / dt : 0.0;
       for (int q = 0; q < R; q++) {
@@ -713,7 +730,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           const double ref_conc = bperm_d(m * gam, gbase + rl);
           const double ref_Z = LD(L.o_Z, rl);
           const double ref_k = L.ionx_ref_k[q];
-          double ref_X = ref_Z * ionx_state[q] / omega;
+          double ref_X = ref_Z * SD(S.ionx, q) / omega;
           double KDj = ref_X / (ref_k * ref_conc);
           bool one_more = false;
           const double zratio = Zr / ref_Z;
@@ -736,7 +753,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             KDj = fmax(KDj, 1e-40);
             if (fabs(delta / KDj) < 1e-12) one_more = true;
           }
-          ionx_state[q] = ref_X * omega / ref_Z;
+          SD(S.ionx, q) = ref_X * omega / ref_Z;
         } else {
           X = role >= 0 ? m * gam * kq : 0.0;
           const double sumkm = gsum<G>(X, lane);
@@ -785,8 +802,9 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 #pragma unroll
       for (int k = 0; k < G; k++) row[k] = live ? row[k] * (psv_t / dt) : row[k];
       if (live) {
+        const double fixed = SD(S.fx, r);
         Rr = (psv_t * tot_out + (R > 0 ? Sr * vol : 0.0)) / dt - fixed / dt;
-        const double psv = porosity * sat * den * vol;
+        const double psv = SD(S.cs, 0) * SD(S.cs, 1) * SD(S.cs, 3) * vol;
         scale = psv * (m + absum) / dt + fabs(Sr) * vol / dt + fabs(fixed) / dt;
       }
       // kinetic minerals (group-uniform rates, row-local residual/Jacobian)
@@ -819,7 +837,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             Rr += nur * ImV;
             double dIm_dQK = pw ? -ImV * LD(L.o_mpow, k) / fabs(aff) : -Im_const * rate;
             if (temkin) dIm_dQK = dIm_dQK * (1.0 / tk_) / QK * (1.0 - aff);
-            double f = nur * dIm_dQK * QK * den * 1.0e-3;
+            double f = nur * dIm_dQK * QK * SD(S.cs, 3) * 1.0e-3;
             if (lim > 0.0) {
               const double dd = 1.0 + (1.0 - aff) / lim;
               f = f * (1.0 + QK / lim / dd) / dd;
@@ -1056,7 +1074,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       if (live) m = P.default_guess;
 #pragma unroll
       for (int q = 0; q < PFX_MAX_IONX_RXN; q++)
-        ionx_state[q] = (q < R && P.ionx_ref) ? P.ionx_ref[cell * R + q] : 1e-9;
+        SD(S.ionx, q) = (q < R && P.ionx_ref) ? P.ionx_ref[cell * R + q] : 1e-9;
       if (r == 0) atomicAdd(P.fallback_count, 1u);
     }
 
@@ -1090,7 +1108,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           for (int k = r; k < K; k += G) P.mnrl_rate[c * K + k] = SD(S.mrate, k);
         if (r == 0) {
           for (int q = 0; q < R; q++)
-            if (P.ionx_ref) P.ionx_ref[c * R + q] = ionx_state[q];
+            if (P.ionx_ref) P.ionx_ref[c * R + q] = SD(S.ionx, q);
           if (MODE != PFX_MODE_AUX) {
             if (P.num_iterations) P.num_iterations[c] = it;
             if (P.status) P.status[c] = status;
