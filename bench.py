@@ -324,7 +324,7 @@ def gi_leg(t, tab, args, log):
     import numpy as np
     import torch
 
-    from pflotran_amd import runtime, synth, workload
+    from pflotran_amd import abi, runtime, synth, workload
 
     side = args.gi_side
     n = side ** 3
@@ -393,13 +393,13 @@ def gi_leg(t, tab, args, log):
     # point-block Jacobi BiCGStab, PETSc's default tolerances, with the
     # analytic reaction Jacobian and with NUMERICAL_JACOBIAN
     snap = workload.snapshot(dc)
-    for key, numerical in (("newton", 0), ("newton_numerical_jacobian", 1)):
+    for key, numerical, jflags in (("newton", 0, 0), ("newton_numerical_jacobian", 1, 0)):
         workload.restore(dc, snap)
         t.numerical_jacobian = numerical
         tab_n = runtime.Tables(t)
         t.numerical_jacobian = 0
         gs = runtime.GiSolver(tab_n, g)
-        st = gs.step(dc, g, synth.react_opts())
+        st = gs.step(dc, g, synth.react_opts(jacobian_flags=jflags))
         gs.close()
         out[key] = {"reason": st.reason, "iterations": st.iterations, "linear_iterations": st.linear_iterations,
                     "fnorm0": st.fnorm0, "fnorm": st.fnorm, "ms_total": st.ms_total,
