@@ -1,0 +1,601 @@
+"""Reader of the chemistry part of a PFLOTRAN input deck.
+
+A Fortran host hands the library tables that PFLOTRAN's own input pass has
+already built (ReactionReadPass1/Pass2, src/pflotran/reaction.F90:113-1048;
+TranConstraintRead, src/pflotran/transport_constraint.F90:163-685).  This
+module gives Python callers the same route from a deck: it reads the cards
+that fix the per-cell reaction system and its initial constraints, and turns
+them into ``database.build_tables`` tables, ``pfx_equilibrate_batch``
+constraint arrays and a time-step schedule.
+
+Cards read (everything else in the deck is skipped line by line):
+
+* ``CHEMISTRY``: PRIMARY_SPECIES, SECONDARY_SPECIES, GAS_SPECIES /
+  PASSIVE_GAS_SPECIES / ACTIVE_GAS_SPECIES, MINERALS, MINERAL_KINETICS
+  (RATE_CONSTANT [units], ACTIVATION_ENERGY [units], AFFINITY_THRESHOLD,
+  RATE_LIMITER), SORPTION / ION_EXCHANGE_RXN (MINERAL, CEC, CATIONS with one
+  REFERENCE cation of k = 1), DATABASE, LOG_FORMULATION,
+  ACTIVITY_COEFFICIENTS [OFF|LAG|NEWTON|TIMESTEP|NEWTON_ITERATION], NO_BDOT,
+  MOLAL, MAX_DLNC, MAX_RELATIVE_CHANGE_TOLERANCE, MAX_RESIDUAL_TOLERANCE;
+  OUTPUT and REDOX_SPECIES are skipped as the reference skips them.
+* ``CONSTRAINT name``: CONCENTRATIONS (F/FREE, T/TOTAL, TOTAL_SORB, P/PH,
+  L/LOG, M/MINERAL/MNRL name, G/GAS name, Z/CHG; TOTAL when the type is
+  left out) and MINERALS (volume fraction, area [units]).
+* ``MATERIAL_PROPERTY name``: POROSITY.
+* ``SUBSURFACE_TRANSPORT`` options: GLOBAL_IMPLICIT, NUMERICAL_JACOBIAN.
+* ``TIME``: FINAL_TIME, INITIAL_TIMESTEP_SIZE, MAXIMUM_TIMESTEP_SIZE.
+* ``REFERENCE_TEMPERATURE``.
+
+Chemistry outside the per-cell path (surface complexation, general,
+microbial and decay reactions, immobile species, colloids, sandboxes,
+porosity updates, supercritical CO2 and dataset constraints, free-ion
+guesses) raises ``DeckError`` naming the keyword rather than being dropped.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+from typing import Dict, Iterator, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import abi
+from .database import (Database, IonExchangeRxn, MineralKinetics, ReactionTables, build_tables, constraint_rxn,
+                       constraint_rxns, read_database)
+
+
+class DeckError(ValueError):
+    """A deck the reader cannot turn into the per-cell reaction system."""
+
+
+# --------------------------------------------------------------------------
+# units (units.F90:400-475, the SI factors of the categories used here)
+
+_SI = {
+    # volume -> m^3
+    "cm^3": 1e-6, "ml": 1e-6, "mL": 1e-6, "l": 1e-3, "L": 1e-3, "dm^3": 1e-3, "m^3": 1.0,
+    # area -> m^2
+    "cm^2": 1e-4, "dm^2": 1e-2, "m^2": 1.0, "km^2": 1e6,
+    # length -> m
+    "km": 1e3, "m": 1.0, "met": 1.0, "meter": 1.0, "dm": 1e-1, "cm": 1e-2, "mm": 1e-3,
+    # time -> s
+    "s": 1.0, "sec": 1.0, "second": 1.0, "min": 60.0, "minute": 60.0,
+    "h": 3600.0, "hr": 3600.0, "hour": 3600.0, "d": 86400.0, "day": 86400.0,
+    "w": 7 * 86400.0, "week": 7 * 86400.0,
+    "mo": 365.0 / 12.0 * 86400.0, "month": 365.0 / 12.0 * 86400.0,
+    "y": 365.0 * 86400.0, "yr": 365.0 * 86400.0, "year": 365.0 * 86400.0,
+    # energy -> J
+    "J": 1.0, "kJ": 1e3, "MJ": 1e6,
+    # amount -> mol
+    "mol": 1.0, "mole": 1.0, "moles": 1.0, "kmol": 1e3,
+    # mass -> kg
+    "ug": 1e-9, "mg": 1e-6, "g": 1e-3, "kg": 1.0,
+}
+
+
+def _part_si(part: str, unit: str) -> float:
+    """SI factor of one side of a unit, its dash-joined units multiplied
+    (UnitsConvert, units.F90:216-330)."""
+    f = 1.0
+    for tok in part.split("-"):
+        if tok not in _SI:
+            raise DeckError(f"unit '{tok}' of '{unit}' is not a unit this reader converts")
+        f = f * _SI[tok]
+    return f
+
+
+def units_factor(unit: str, internal: str) -> float:
+    """Factor taking a value in `unit` to `internal` (UnitsConvertToInternal
+    and UnitsConvertParse, units.F90:17-160): the numerators' ratio over the
+    denominators' ratio, each ratio user SI over internal SI."""
+    u, i = unit.split("/"), internal.split("/")
+    if len(u) > 2 or len(u) != len(i):
+        raise DeckError(f"unit '{unit}' does not have the form of '{internal}'")
+    num = _part_si(u[0], unit) / _part_si(i[0], internal)
+    den = _part_si(u[1], unit) / _part_si(i[1], internal) if len(u) == 2 else 1.0
+    return num / den
+
+
+# --------------------------------------------------------------------------
+# lines
+
+
+def _num(tok: str) -> float:
+    try:
+        return float(tok.lower().replace("d", "e"))
+    except ValueError:
+        raise DeckError(f"'{tok}' is not a number") from None
+
+
+def _lines(text: str) -> List[List[str]]:
+    """Token lists of the deck's lines as InputReadPflotranString yields
+    them (input_aux.F90:642-732): lines opening with '#' or '!' and blank
+    lines skipped, a line cut at its first '#' or '!', SKIP ... NOSKIP
+    regions (nested) dropped."""
+    out, skip = [], 0
+    for raw in text.splitlines():
+        s = raw.strip()
+        if not s or s[0] in "#!":
+            continue
+        for c in "#!":
+            i = s.find(c)
+            if i >= 0:
+                s = s[:i]
+        toks = s.replace("\t", " ").split()
+        if not toks:
+            continue
+        w = toks[0].upper()
+        if w.startswith("SKIP"):
+            skip += 1
+            continue
+        if skip:
+            if w.startswith("NOSK"):
+                skip -= 1
+            continue
+        if w.startswith("NOSK"):
+            continue
+        out.append(toks)
+    return out
+
+
+def _is_exit(toks: List[str]) -> bool:
+    """InputCheckExit (input_aux.F90:1236-1273): '/', END or END_*."""
+    w = toks[0]
+    return w.startswith("/") or w.upper() == "END" or w.upper().startswith("END_")
+
+
+class _Cursor:
+    def __init__(self, lines: List[List[str]]):
+        self.lines, self.i = lines, 0
+
+    def next(self, where: str) -> List[str]:
+        if self.i >= len(self.lines):
+            raise DeckError(f"end of deck inside {where}")
+        t = self.lines[self.i]
+        self.i += 1
+        return t
+
+    def block(self, where: str) -> Iterator[List[str]]:
+        """Lines of a block up to its exit line (consumed)."""
+        while True:
+            t = self.next(where)
+            if _is_exit(t):
+                return
+            yield t
+
+    def skip_block(self, where: str) -> None:
+        for _ in self.block(where):
+            pass
+
+
+# --------------------------------------------------------------------------
+# deck contents
+
+
+@dataclass
+class SpeciesConstraint:
+    name: str
+    value: float
+    ctype: int
+    aux: Optional[str] = None      # mineral or gas of an M/G constraint
+
+
+@dataclass
+class MineralConstraint:
+    name: str
+    volume_fraction: float
+    area: float                    # m^2/m^3 bulk
+
+
+@dataclass
+class Constraint:
+    name: str
+    concentrations: List[SpeciesConstraint] = field(default_factory=list)
+    minerals: List[MineralConstraint] = field(default_factory=list)
+
+
+@dataclass
+class Chemistry:
+    primaries: List[str] = field(default_factory=list)
+    secondaries: List[str] = field(default_factory=list)
+    gases: List[str] = field(default_factory=list)
+    minerals: List[str] = field(default_factory=list)
+    kinetics: Dict[str, MineralKinetics] = field(default_factory=dict)
+    ion_exchange: List[IonExchangeRxn] = field(default_factory=list)
+    database: Optional[str] = None
+    log_formulation: bool = False                 # reaction_aux.F90:433
+    act_freq: int = abi.PFX_ACT_OFF               # reaction_aux.F90:418
+    act_alg: int = abi.PFX_ACT_ALG_LAG            # reaction_aux.F90:419
+    use_bdot: bool = True                         # reaction_aux.F90:417
+    molal: bool = False
+    max_dlnC: float = 5.0                         # reaction_aux.F90:569
+    max_rel_change_tol: float = 1e-6
+    max_residual_tol: float = 1e-12
+
+
+@dataclass
+class TimeCard:
+    final: Optional[float] = None                 # s
+    dt_init: Optional[float] = None
+    dt_max: Optional[float] = None
+
+
+@dataclass
+class Deck:
+    path: Optional[str]
+    chemistry: Chemistry
+    constraints: Dict[str, Constraint]
+    porosity: Dict[str, float]
+    time: TimeCard
+    temperature: float = 25.0
+    numerical_jacobian: bool = False
+
+    # ---------------------------------------------------------------- builds
+
+    def database(self, path: Optional[str] = None) -> Database:
+        """The thermodynamic database the deck names, resolved against the
+        deck's folder (or `path` when given)."""
+        p = path or self.chemistry.database
+        if p is None:
+            raise DeckError("CHEMISTRY has no DATABASE")
+        if not os.path.isabs(p) and self.path:
+            p = os.path.join(os.path.dirname(os.path.abspath(self.path)), p)
+        return read_database(p)
+
+    def tables(self, db: Database, fit_logk: bool = False) -> ReactionTables:
+        c = self.chemistry
+        kin = {n: c.kinetics[n] for n in self.kinetic_minerals()}
+        t = build_tables(db, c.primaries, c.secondaries, kin, c.ion_exchange, temperature=self.temperature,
+                         log_formulation=c.log_formulation, act_freq=c.act_freq, act_alg=c.act_alg,
+                         use_bdot=c.use_bdot, fit_logk=fit_logk)
+        t.max_dlnC = c.max_dlnC
+        t.max_rel_change_tol = c.max_rel_change_tol
+        t.max_residual_tol = c.max_residual_tol
+        t.numerical_jacobian = int(self.numerical_jacobian)
+        return t
+
+    def kinetic_minerals(self) -> List[str]:
+        """Kinetic minerals in MINERALS-list order (the basis numbers
+        kinetic minerals as it walks the mineral list)."""
+        c = self.chemistry
+        missing = [n for n in c.kinetics if n not in c.minerals]
+        if missing:
+            raise DeckError(f"MINERAL_KINETICS names minerals missing from MINERALS: {missing}")
+        return [n for n in c.minerals if n in c.kinetics]
+
+    def constraint_setup(self, name: str, tables: ReactionTables, db: Database) -> Dict:
+        """Arrays of one constraint in primary-species order: ``ctype``,
+        ``conc``, ``rxn`` (ctypes array of mineral/gas reactions, or None),
+        ``vf`` and ``area`` per kinetic mineral (None when the constraint
+        has no MINERALS block), ``molal``."""
+        if name not in self.constraints:
+            raise DeckError(f"no CONSTRAINT {name}")
+        con = self.constraints[name]
+        c = self.chemistry
+        by_name = {s.name: s for s in con.concentrations}
+        extra = [n for n in by_name if n not in c.primaries]
+        if extra:
+            raise DeckError(f"CONSTRAINT {name}: {extra} are not primary species")
+        missing = [n for n in c.primaries if n not in by_name]
+        if missing:
+            raise DeckError(f"CONSTRAINT {name}: no concentration for {missing}")
+        ctype, conc, rxns = [], [], {}
+        for j, n in enumerate(c.primaries):
+            s = by_name[n]
+            ctype.append(s.ctype)
+            conc.append(s.value)
+            if s.ctype in (abi.PFX_CONSTRAINT_MINERAL, abi.PFX_CONSTRAINT_GAS):
+                kind = "mineral" if s.ctype == abi.PFX_CONSTRAINT_MINERAL else "gas"
+                rxns[j] = constraint_rxn(db, s.aux, c.primaries, kind, self.temperature)
+        kin = self.kinetic_minerals()
+        vf = area = None
+        if con.minerals:
+            # a MINERALS block must cover every kinetic mineral
+            # (transport_constraint.F90:525-534); a constraint without one
+            # (a boundary's) leaves the minerals alone
+            vf, area = np.zeros(len(kin)), np.zeros(len(kin))
+            got = {m.name: m for m in con.minerals}
+            lack = [n for n in kin if n not in got]
+            if lack:
+                raise DeckError(f"CONSTRAINT {name}: MINERALS gives no volume fraction/area for {lack}")
+            for k, n in enumerate(kin):
+                vf[k], area[k] = got[n].volume_fraction, got[n].area
+        return dict(ctype=ctype, conc=conc, rxn=constraint_rxns(tables, rxns) if rxns else None,
+                    vf=vf, area=area, molal=int(c.molal))
+
+    def time_steps(self) -> List[float]:
+        """Step sizes from 0 to FINAL_TIME for a run whose steps are never
+        cut and whose only waypoint is the final time: min(initial, maximum)
+        steps, the last one shortened to land on the final time
+        (TimestepperBaseSetTargetTime, timestepper_base.F90:320-521, with
+        its 0.1 time-step tolerance).  Growth between the initial and the
+        maximum size is not modelled: decks with INITIAL < MAXIMUM raise."""
+        tc = self.time
+        if tc.final is None or tc.dt_init is None:
+            raise DeckError("TIME needs FINAL_TIME and INITIAL_TIMESTEP_SIZE")
+        dt_max = tc.dt_max if tc.dt_max is not None else tc.dt_init
+        if tc.dt_init < dt_max:
+            raise DeckError("time-step growth (INITIAL_TIMESTEP_SIZE < MAXIMUM_TIMESTEP_SIZE) is not modelled")
+        dt, t, steps = dt_max, 0.0, []
+        while True:
+            if t + dt + 0.1 * dt >= tc.final:
+                steps.append(tc.final - t)
+                return steps
+            steps.append(dt)
+            t += dt
+
+
+# --------------------------------------------------------------------------
+# parser
+
+_CONSTRAINT_TYPES = {
+    "F": abi.PFX_CONSTRAINT_FREE, "FREE": abi.PFX_CONSTRAINT_FREE,
+    "T": abi.PFX_CONSTRAINT_TOTAL, "TOTAL": abi.PFX_CONSTRAINT_TOTAL,
+    "TOTAL_SORB": abi.PFX_CONSTRAINT_TOTAL_SORB,
+    "P": abi.PFX_CONSTRAINT_PH, "PH": abi.PFX_CONSTRAINT_PH,
+    "L": abi.PFX_CONSTRAINT_LOG, "LOG": abi.PFX_CONSTRAINT_LOG,
+    "M": abi.PFX_CONSTRAINT_MINERAL, "MINERAL": abi.PFX_CONSTRAINT_MINERAL, "MNRL": abi.PFX_CONSTRAINT_MINERAL,
+    "G": abi.PFX_CONSTRAINT_GAS, "GAS": abi.PFX_CONSTRAINT_GAS,
+    "Z": abi.PFX_CONSTRAINT_CHARGE, "CHG": abi.PFX_CONSTRAINT_CHARGE,
+}
+
+# CHEMISTRY keywords of chemistry outside the per-cell path
+_CHEM_UNSUPPORTED = {
+    "GENERAL_REACTION", "MICROBIAL_REACTION", "RADIOACTIVE_DECAY_REACTION", "IMMOBILE_SPECIES",
+    "COLLOIDS", "REACTION_SANDBOX", "CLM_REACTION", "SOLID_SOLUTIONS", "UPDATE_POROSITY",
+    "UPDATE_TORTUOSITY", "UPDATE_PERMEABILITY", "UPDATE_MINERAL_SURFACE_AREA", "ACTIVITY_H2O",
+    "ACTIVITY_WATER", "TRUNCATE_CONCENTRATION", "OPERATOR_SPLIT", "OPERATOR_SPLITTING", "GEOTHERMAL_HPT",
+    "UPDATE_MNRL_SURF_AREA_WITH_POR", "UPDATE_ARMOR_MINERAL_SURFACE", "UPDATE_ARMOR_MINERAL_SURFACE_FLAG",
+    "EXPLICIT_ADVECTION", "DECOUPLED_EQUILIBRIUM_MINERALS", "USE_FULL_GEOCHEMISTRY", "MINIMUM_POROSITY",
+}
+# keywords with no effect on the per-cell system
+_CHEM_IGNORED = {"NO_CHECKPOINT_ACT_COEFS", "NO_RESTART_MINERAL_VOL_FRAC", "NO_CHECK_UPDATE"}
+
+
+def _names(cur: _Cursor, where: str) -> List[str]:
+    return [t[0] for t in cur.block(where)]
+
+
+def _value_units(toks: List[str], i: int, internal: str, where: str) -> float:
+    if len(toks) <= i:
+        raise DeckError(f"{where}: value missing")
+    v = _num(toks[i])
+    if len(toks) > i + 1:
+        v *= units_factor(toks[i + 1], internal)
+    return v
+
+
+def _mineral_kinetics(cur: _Cursor) -> Dict[str, MineralKinetics]:
+    """MINERAL_KINETICS (reaction.F90:415-455 pass 1, reaction_mineral.F90
+    MineralReadKinetics for the values)."""
+    out = {}
+    for t in cur.block("MINERAL_KINETICS"):
+        name = t[0]
+        k = MineralKinetics(rate_constant=float("nan"))
+        for s in cur.block(f"MINERAL_KINETICS {name}"):
+            w = s[0].upper()
+            if w == "RATE_CONSTANT":
+                rate = _num(s[1])
+                if rate < 0.0:
+                    rate = 10.0 ** rate
+                if len(s) > 2:
+                    rate *= units_factor(s[2], "mol/m^2-sec")
+                k.rate_constant = rate
+            elif w == "ACTIVATION_ENERGY":
+                k.activation_energy = _value_units(s, 1, "J/mol", "ACTIVATION_ENERGY")
+            elif w == "AFFINITY_THRESHOLD":
+                k.affinity_threshold = _num(s[1])
+            elif w == "RATE_LIMITER":
+                k.rate_limiter = _num(s[1])
+            else:
+                raise DeckError(f"MINERAL_KINETICS {name}: {w} is outside the per-cell path")
+        if not np.isfinite(k.rate_constant):
+            raise DeckError(f"MINERAL_KINETICS {name}: no RATE_CONSTANT")
+        out[name] = k
+    return out
+
+
+def _ion_exchange(cur: _Cursor) -> IonExchangeRxn:
+    """ION_EXCHANGE_RXN (reaction.F90:639-735): the REFERENCE cation is moved
+    to the front and must have k = 1; CEC is read without units (eq/m^3)."""
+    mineral, cec, cations, ref = None, None, [], None
+    for t in cur.block("ION_EXCHANGE_RXN"):
+        w = t[0].upper()
+        if w == "MINERAL":
+            mineral = t[1]
+        elif w == "CEC":
+            cec = _num(t[1])
+        elif w == "CATIONS":
+            for s in cur.block("CATIONS"):
+                cations.append((s[0], _num(s[1])))
+                if len(s) > 2:
+                    if s[2].upper() != "REFERENCE":
+                        raise DeckError(f"CATIONS: unknown word {s[2]}")
+                    ref = s[0]
+        else:
+            raise DeckError(f"ION_EXCHANGE_RXN: {w} unknown")
+    if ref is None:
+        raise DeckError("Reference cation missing in Ion Exchange reaction.")
+    i = next(i for i, (n, _) in enumerate(cations) if n == ref)
+    if abs(cations[i][1] - 1.0) > 1e-40:
+        raise DeckError(f'Reference cation "{ref}" must have k = 1.d0.')
+    cations.insert(0, cations.pop(i))
+    if cec is None:
+        raise DeckError("ION_EXCHANGE_RXN without CEC")
+    if mineral is not None:
+        raise DeckError("ION_EXCHANGE_RXN on a mineral surface is outside the per-cell path")
+    return IonExchangeRxn(cations, cec)
+
+
+def _chemistry(cur: _Cursor) -> Chemistry:
+    c = Chemistry()
+    for t in cur.block("CHEMISTRY"):
+        w = t[0].upper()
+        if w == "PRIMARY_SPECIES":
+            c.primaries += _names(cur, w)
+        elif w == "SECONDARY_SPECIES":
+            c.secondaries += _names(cur, w)
+        elif w in ("GAS_SPECIES", "PASSIVE_GAS_SPECIES", "ACTIVE_GAS_SPECIES"):
+            c.gases += _names(cur, w)
+        elif w == "MINERALS":
+            c.minerals += _names(cur, w)
+        elif w == "MINERAL_KINETICS":
+            c.kinetics.update(_mineral_kinetics(cur))
+        elif w == "SORPTION":
+            for s in cur.block("SORPTION"):
+                v = s[0].upper()
+                if v == "ION_EXCHANGE_RXN":
+                    c.ion_exchange.append(_ion_exchange(cur))
+                elif v in ("SURFACE_COMPLEXATION_RXN", "ISOTHERM_REACTIONS", "DISTRIBUTION_COEFFICIENTS"):
+                    raise DeckError(f"SORPTION {v} is outside the per-cell path")
+                elif v in ("JUMPSTART_KINETIC_SORPTION", "NO_CHECKPOINT_KINETIC_SORPTION",
+                           "NO_RESTART_KINETIC_SORPTION"):
+                    pass
+                else:
+                    raise DeckError(f"SORPTION: {v} unknown")
+        elif w == "DATABASE":
+            c.database = t[1]
+        elif w == "LOG_FORMULATION":
+            c.log_formulation = True
+        elif w == "ACTIVITY_COEFFICIENTS":
+            # reaction.F90:768-789: the keyword alone means LAG at every step
+            c.act_alg, c.act_freq = abi.PFX_ACT_ALG_LAG, abi.PFX_ACT_TIMESTEP
+            for v in (x.upper() for x in t[1:]):
+                if v == "OFF":
+                    c.act_freq = abi.PFX_ACT_OFF
+                elif v == "LAG":
+                    c.act_alg = abi.PFX_ACT_ALG_LAG
+                elif v == "NEWTON":
+                    c.act_alg = abi.PFX_ACT_ALG_NEWTON
+                elif v == "TIMESTEP":
+                    c.act_freq = abi.PFX_ACT_TIMESTEP
+                elif v == "NEWTON_ITERATION":
+                    c.act_freq = abi.PFX_ACT_NEWTON_ITER
+                else:
+                    raise DeckError(f"CHEMISTRY,ACTIVITY_COEFFICIENTS: {v} unknown")
+        elif w == "NO_BDOT":
+            c.use_bdot = False
+        elif w in ("MOLAL", "MOLALITY"):
+            c.molal = True
+        elif w == "MAX_DLNC":
+            c.max_dlnC = _num(t[1])
+        elif w in ("MAX_RELATIVE_CHANGE_TOLERANCE", "REACTION_TOLERANCE"):
+            c.max_rel_change_tol = _num(t[1])
+        elif w == "MAX_RESIDUAL_TOLERANCE":
+            c.max_residual_tol = _num(t[1])
+        elif w in ("OUTPUT", "REDOX_SPECIES"):
+            cur.skip_block(w)
+        elif w in _CHEM_IGNORED:
+            pass
+        elif w in _CHEM_UNSUPPORTED:
+            raise DeckError(f"CHEMISTRY {w} is outside the per-cell path")
+        else:
+            raise DeckError(f"CHEMISTRY: keyword {w} unknown")
+    return c
+
+
+def _constraint(cur: _Cursor, name: str) -> Constraint:
+    """CONSTRAINT (transport_constraint.F90:163-685)."""
+    con = Constraint(name)
+    for t in cur.block(f"CONSTRAINT {name}"):
+        w = t[0].upper()
+        if w in ("CONC", "CONCENTRATIONS"):
+            for s in cur.block("CONCENTRATIONS"):
+                if len(s) < 2:
+                    raise DeckError(f"CONSTRAINT {name}: '{' '.join(s)}' lacks a value")
+                ctype, aux = abi.PFX_CONSTRAINT_TOTAL, None
+                if len(s) > 2:
+                    v = s[2].upper()
+                    if v == "S":
+                        raise DeckError('"S" constraint type no longer supported as of March 4, 2013.')
+                    if v not in _CONSTRAINT_TYPES:
+                        raise DeckError(f"CONSTRAINT {name}: type {s[2]} is outside the per-cell path")
+                    ctype = _CONSTRAINT_TYPES[v]
+                    if ctype in (abi.PFX_CONSTRAINT_MINERAL, abi.PFX_CONSTRAINT_GAS):
+                        if len(s) < 4:
+                            raise DeckError(f"CONSTRAINT {name}: {s[0]} needs a mineral or gas name")
+                        aux = s[3]
+                    elif len(s) > 3 and s[3].upper() == "DATASET":
+                        raise DeckError(f"CONSTRAINT {name}: DATASET concentrations are not read")
+                con.concentrations.append(SpeciesConstraint(s[0], _num(s[1]), ctype, aux))
+        elif w in ("MNRL", "MINERALS"):
+            for s in cur.block("MINERALS"):
+                if len(s) < 3 or "DATASET" in (x.upper() for x in s):
+                    raise DeckError(f"CONSTRAINT {name}: mineral line '{' '.join(s)}' needs a volume fraction "
+                                    "and an area (datasets are not read)")
+                area = _num(s[2])
+                if len(s) > 3:
+                    area *= units_factor(s[3], "m^2/m^3")
+                con.minerals.append(MineralConstraint(s[0], _num(s[1]), area))
+        elif w == "FREE_ION_GUESS":
+            raise DeckError(f"CONSTRAINT {name}: FREE_ION_GUESS is not taken by pfx_equilibrate_batch")
+        else:
+            raise DeckError(f"CONSTRAINT {name}: {w} is outside the per-cell path")
+    return con
+
+
+def _time(cur: _Cursor) -> TimeCard:
+    tc = TimeCard()
+    for t in cur.block("TIME"):
+        w = t[0].upper()
+        if w == "FINAL_TIME":
+            tc.final = _value_units(t, 1, "s", w)
+        elif w == "INITIAL_TIMESTEP_SIZE":
+            tc.dt_init = _value_units(t, 1, "s", w)
+        elif w == "MAXIMUM_TIMESTEP_SIZE":
+            if len(t) > 3:
+                raise DeckError("MAXIMUM_TIMESTEP_SIZE ... at <time> waypoints are not modelled")
+            tc.dt_max = _value_units(t, 1, "s", w)
+        elif w == "STEADY_STATE":
+            raise DeckError("TIME STEADY_STATE is outside the per-cell path")
+    return tc
+
+
+def parse_deck(text: str, path: Optional[str] = None) -> Deck:
+    cur = _Cursor(_lines(text))
+    chem, constraints, porosity = None, {}, {}
+    tc, temperature, numjac = TimeCard(), 25.0, False
+    while cur.i < len(cur.lines):
+        t = cur.next("deck")
+        w = t[0].upper()
+        if w == "CHEMISTRY":
+            if chem is not None:
+                raise DeckError("two CHEMISTRY cards")
+            chem = _chemistry(cur)
+        elif w == "CONSTRAINT" and len(t) > 1:
+            constraints[t[1]] = _constraint(cur, t[1])
+        elif w == "MATERIAL_PROPERTY" and len(t) > 1:
+            name = t[1]
+            depth = 0
+            for s in cur.lines[cur.i:]:
+                cur.i += 1
+                v = s[0].upper()
+                if _is_exit(s):
+                    if depth == 0:
+                        break
+                    depth -= 1
+                elif v in ("PERMEABILITY", "SECONDARY_CONTINUUM", "GEOMECHANICS_SUBSURFACE_PROPS"):
+                    depth += 1
+                elif v == "POROSITY" and depth == 0:
+                    if len(s) > 2 and s[1].upper() == "DATASET":
+                        raise DeckError(f"MATERIAL_PROPERTY {name}: POROSITY DATASET is not read")
+                    porosity[name] = _num(s[1])
+        elif w == "SUBSURFACE_TRANSPORT":
+            for s in cur.block("SUBSURFACE_TRANSPORT"):
+                v = s[0].upper()
+                if v == "NUMERICAL_JACOBIAN":
+                    numjac = True
+                elif v in ("OPERATOR_SPLIT", "OPERATOR_SPLITTING"):
+                    raise DeckError("operator-split transport is not reimplemented in the reference either")
+        elif w == "TIME":
+            tc = _time(cur)
+        elif w == "REFERENCE_TEMPERATURE":
+            temperature = _num(t[1])
+    if chem is None:
+        raise DeckError("deck has no CHEMISTRY card")
+    return Deck(path, chem, constraints, porosity, tc, temperature, numjac)
+
+
+def read_deck(path: str) -> Deck:
+    with open(path) as f:
+        return parse_deck(f.read(), path)

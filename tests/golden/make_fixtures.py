@@ -19,6 +19,17 @@ BATCH = os.path.join(REF, "regression_tests", "ascem", "batch")
 for name in ("carbonate.dat", "ca-carbonate.dat", "calcite.dat"):
     shutil.copy(os.path.join(BATCH, name), os.path.join(HERE, "db", name))
 
+# input decks of those tests (data: the inputs whose outputs the golds
+# hold), and the 1-D calcite column deck for its mineral, gas, pH and
+# charge-balance constraints
+os.makedirs(os.path.join(HERE, "decks"), exist_ok=True)
+for name in ("carbonate-unit-activity", "carbonate-debye-huckel-activity",
+             "ca-carbonate-unit-activity", "ca-carbonate-debye-huckel-activity",
+             "calcite-kinetics", "calcite-kinetics-volume-fractions", "ion-exchange-valocchi"):
+    shutil.copy(os.path.join(BATCH, f"{name}.in"), os.path.join(HERE, "decks", f"{name}.in"))
+shutil.copy(os.path.join(REF, "regression_tests", "ascem", "1d", "1d-calcite", "1d-calcite.in"),
+            os.path.join(HERE, "decks", "1d-calcite.in"))
+
 # excerpt of database/hanford.dat: the species of the 1-D calcite columns
 # (5 primaries, 8 complexes, calcite; CaOH+ and the CO2(g) constraint of the
 # calcite transport decks) and of the Valocchi ion-exchange test
