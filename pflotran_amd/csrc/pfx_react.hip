@@ -47,16 +47,18 @@ __device__ __forceinline__ int bperm_i(int v, int src_lane) {
 //   G = 32: the G = 16 steps, then one permute across the two rows.
 // Every step combines a commutative pair, so all lanes of the group end
 // with the bitwise-same value.
+// Every lane of a row has a source lane under these controls, so no lane
+// keeps an old value and the moves need no initialised destination.
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
 
 constexpr int DPP_ROR8 = 0x128, DPP_ROR4 = 0x124, DPP_ROR2 = 0x122, DPP_ROR1 = 0x121;
@@ -98,10 +100,9 @@ __device__ __forceinline__ double gmin(double v, int lane) { return greduce<G>(v
 
 // argmax with ties resolved to the larger row index
 __device__ __forceinline__ void amax_step(double &v, int &idx, double w, int j) {
-  if (w > v || (w == v && j > idx)) {
-    v = w;
-    idx = j;
-  }
+  const bool take = (w > v) | ((w == v) & (j > idx));  // no short circuit: selects
+  v = take ? w : v;
+  idx = take ? j : idx;
 }
 template <int G>
 __device__ __forceinline__ void gargmax(double &v, int &idx, int lane) {
@@ -948,18 +949,26 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     // RSolve: row equilibration, ln-column scaling, pivoted LU, solve
     double upd = 0.0;
     if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
+      // Straight-line code: the steps run over all G rows with selects
+      // instead of branches (a divergent branch, or a step skipped for
+      // j >= N, makes the compiler copy the whole row at every merge).
+      // Padding lanes (r >= N) hold identity rows with a zero right-hand
+      // side; they are no pivot candidates before step N and pivot on
+      // themselves after it, where every multiplier is zero, so the live
+      // rows see exactly the N-step elimination.
       double b = 0.0;
-      if (live) {
+      {
         double mx = 0.0;
 #pragma unroll
         for (int k = 0; k < G; k++) mx = fmax(mx, fabs(row[k]));
-        const double norm = 1.0 / fmax(1.0, mx);
-        b = Rr * norm;
+        const double norm = live ? 1.0 / fmax(1.0, mx) : 1.0;
+        b = live ? Rr * norm : 0.0;
+        const bool lg = live && use_log;
 #pragma unroll
-        for (int k = 0; k < G; k++) row[k] *= norm;
-        if (use_log) {
-#pragma unroll
-          for (int k = 0; k < G; k++) row[k] *= SD(S.mv, k);
+        for (int k = 0; k < G; k++) {
+          row[k] = row[k] * norm;
+          const double sk = SD(S.mv, k);
+          row[k] = lg ? row[k] * sk : row[k];
         }
       }
       double vv = 0.0;
@@ -970,48 +979,43 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         vv = mx > 0.0 ? 1.0 / mx : 0.0;
         if (live && !(mx > 0.0)) status = PFX_CELL_SINGULAR;
       }
-      bool pivoted = !live;
+      bool pivoted = false;
       int mystep = -1;  // elimination step at which this lane's row was the pivot
 #pragma unroll
       for (int j = 0; j < G; j++) {
-        if (j < N) {
-          double cand = pivoted ? -1.0 : vv * fabs(row[j]);
-          int p = r;
-          gargmax<G>(cand, p, lane);
-          const int src = gbase + p;
-          // stream the pivot row one element at a time (lane to lane; an LDS
-          // broadcast of the row measured slower: it adds a store-to-load
-          // wait to every step)
-          double piv = bperm_d(row[j], src);
-          if (piv == 0.0) piv = 1.0e-20;
-          const bool elim = (r != p) && !pivoted;
-          const double f = elim ? row[j] / piv : 0.0;  // 0 leaves the row unchanged below
-          if (r == p) {
-            pivoted = true;
-            mystep = j;
-            row[j] = piv;
-          } else if (elim) {
-            row[j] = f;
-          }
+        const bool in_play = live || j >= N;  // padding rows join after step N
+        double cand = (pivoted || !in_play) ? -1.0 : vv * fabs(row[j]);
+        int p = r;
+        gargmax<G>(cand, p, lane);
+        const int src = gbase + p;
+        // stream the pivot row one element at a time (lane to lane; an LDS
+        // broadcast of the row measured slower: it adds a store-to-load
+        // wait to every step)
+        double piv = bperm_d(row[j], src);
+        piv = piv == 0.0 ? 1.0e-20 : piv;
+        const bool is_piv = r == p;
+        const bool elim = !is_piv && !pivoted && in_play;
+        const double q = row[j] / piv;
+        const double f = elim ? q : 0.0;  // 0 leaves the row unchanged below
+        row[j] = is_piv ? piv : (elim ? q : row[j]);
+        mystep = is_piv ? j : mystep;
+        pivoted = pivoted || is_piv;
 #pragma unroll
-          for (int k = j + 1; k < G; k++) {
-            const double pk = bperm_d(row[k], src);
-            row[k] -= f * pk;
-          }
-          const double pb = bperm_d(b, src);
-          b -= f * pb;
+        for (int k = j + 1; k < G; k++) {
+          const double pk = bperm_d(row[k], src);
+          row[k] -= f * pk;
         }
+        const double pb = bperm_d(b, src);
+        b -= f * pb;
       }
       // back substitution: the pivot row of step j is held by the lane whose
       // mystep is j, the only non-zero term of the group sum that spreads
       // x_j; rows pivoted before step j still need x_j, later ones are done
 #pragma unroll
       for (int j = G - 1; j >= 0; j--) {
-        if (j < N) {
-          const double xj = gsum<G>(mystep == j ? b / row[j] : 0.0, lane);
-          if (r == j) upd = xj;
-          b -= row[j] * xj;
-        }
+        const double xj = gsum<G>(mystep == j ? b / row[j] : 0.0, lane);
+        upd = (r == j && live) ? xj : upd;
+        b -= row[j] * xj;
       }
       const double bad = gmax<G>(status == PFX_CELL_SINGULAR ? 1.0 : 0.0, lane);
       if (bad > 0.0) {
