@@ -177,24 +177,28 @@ __host__ __device__ inline int scr_doubles(int M, int G, int K) {
   return (n + 1) & ~1;
 }
 
+// The fields whose size is fixed by the group size come first, at offsets
+// known at compile time: their LDS accesses take the offset as an
+// immediate of the slice base, and no per-field address stays live in a
+// register.  The complex and mineral arrays (sizes M, K) follow.
 __device__ __forceinline__ Scr make_scr(int base, int M, int G, int K) {
   Scr s;
-  int p = base;
+  s.la = base;
+  s.mv = base + 8 * G;
+  s.im = base + 16 * G;
+  s.xb = base + 24 * G;
+  s.fx = base + 32 * G;
+  s.cs = base + 40 * G;
+  s.ionx = base + 40 * G + 32;
+  int p = base + 40 * G + 32 + 8 * PFX_MAX_IONX_RXN;
   s.C = p; p += (M + 1) * 8;
   s.gsec = p; p += M * 8;
   s.lnK = p; p += M * 8;
-  s.la = p; p += G * 8;
-  s.mv = p; p += G * 8;
-  s.im = p; p += G * 8;
-  s.xb = p; p += G * 8;
   s.mlnK = p; p += K * 8;
   s.mkeff = p; p += K * 8;
   s.marea = p; p += K * 8;
   s.mvf = p; p += K * 8;
-  s.mrate = p; p += K * 8;
-  s.cs = p; p += 4 * 8;
-  s.ionx = p; p += PFX_MAX_IONX_RXN * 8;
-  s.fx = p;
+  s.mrate = p;
   return s;
 }
 
@@ -1004,6 +1008,9 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         for (int k = j + 1; k < G; k++) {
           const double pk = bperm_d(row[k], src);
           row[k] -= f * pk;
+          // at most 8 pivot-row elements in flight: hoisting all of them
+          // costs 15-20 more registers (and spills at 3 wavefronts/SIMD)
+          if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
         const double pb = bperm_d(b, src);
         b -= f * pb;
