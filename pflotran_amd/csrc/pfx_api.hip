@@ -618,26 +618,32 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
     }
   L.o_adj_c = B.add(adj_c);
   L.o_adj_nu = B.add(adj_nu);
-  int total_off = 0;
-  for (int k = 0; k < G; k++) {
-    L.off_start[k] = total_off;
+  // chunks of 4 columns, each with as many slots as its longest list; the
+  // kernel reads a slot's 4 complexes (16-bit) and 4 coefficients with
+  // one 8-byte and two 16-byte LDS loads per lane
+  const int nq = G / 4;
+  int rows = 0;
+  for (int q = 0; q < nq; q++) {
+    L.offq_start[q] = rows;
     int mx = 0;
-    for (int r = 0; r < G; r++) mx = std::max<int>(mx, (int)off[k][r].size());
-    total_off += mx;
+    for (int j = 0; j < 4; j++)
+      for (int r = 0; r < G; r++) mx = std::max<int>(mx, (int)off[4 * q + j][r].size());
+    rows += mx;
   }
-  L.off_start[G] = total_off;
-  for (int k = G + 1; k <= PFX_MAX_PRIMARY; k++) L.off_start[k] = total_off;
-  std::vector<int32_t> off_c((size_t)std::max(total_off, 1) * G, M);
-  std::vector<double> off_coef((size_t)std::max(total_off, 1) * G, 0.0);
-  for (int k = 0; k < G; k++)
-    for (int r = 0; r < G; r++)
-      for (size_t e = 0; e < off[k][r].size(); e++) {
-        const size_t row = L.off_start[k] + e;
-        off_c[row * G + r] = off[k][r][e].first;
-        off_coef[row * G + r] = off[k][r][e].second;
-      }
-  L.o_off_c = B.add(off_c);
-  L.o_off_coef = B.add(off_coef);
+  L.offq_start[nq] = rows;
+  for (int q = nq + 1; q <= PFX_MAX_PRIMARY / 4; q++) L.offq_start[q] = rows;
+  std::vector<uint16_t> offq_c((size_t)std::max(rows, 1) * G * 4, (uint16_t)M);
+  std::vector<double> offq_coef((size_t)std::max(rows, 1) * G * 4, 0.0);
+  for (int q = 0; q < nq; q++)
+    for (int j = 0; j < 4; j++)
+      for (int r = 0; r < G; r++)
+        for (size_t e = 0; e < off[4 * q + j][r].size(); e++) {
+          const size_t row = (size_t)L.offq_start[q] + e;
+          offq_c[(row * G + r) * 4 + j] = (uint16_t)off[4 * q + j][r][e].first;
+          offq_coef[((row * 2 + j / 2) * G + r) * 2 + (j % 2)] = off[4 * q + j][r][e].second;
+        }
+  L.o_offq_c = B.add(offq_c);
+  L.o_offq_coef = B.add(offq_coef);
 
   // minerals
   auto mid = [&](int k, int s) { return d->kinmnrlspecid[(size_t)k * (S0 + 1) + 1 + s]; };

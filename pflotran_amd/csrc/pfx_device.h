@@ -18,7 +18,10 @@ struct pfx_dev_layout {
   int32_t N, M, K, R, G, S;     // sizes; S = species slots per complex/mineral
   int32_t Q;                    // complex rounds per lane = ceil(M/G)
   int32_t maxdeg;               // max complexes touching one primary
-  int32_t off_start[PFX_MAX_PRIMARY + 1];  // per-column off-diagonal list offsets
+  // off-diagonal co-occurrence lists of dT_r/dm_k in chunks of 4 columns:
+  // chunk q owns rows [offq_start[q], offq_start[q+1]) of the chunked
+  // tables, one row per slot, as many slots as its longest list
+  int32_t offq_start[PFX_MAX_PRIMARY / 4 + 1];
   int32_t has_logk_coef, has_mnrl_coef;
   int32_t ionx_ncat[PFX_MAX_IONX_RXN];
   int32_t ionx_zflag[PFX_MAX_IONX_RXN];
@@ -41,8 +44,8 @@ struct pfx_dev_layout {
   int64_t o_ch2o;                    // [M]
   int64_t o_adj_c;                   // int32 [maxdeg][G], -> M = zero slot
   int64_t o_adj_nu;                  // [maxdeg][G]
-  int64_t o_off_c;                   // int32 [off_total][G]
-  int64_t o_off_coef;                // [off_total][G]
+  int64_t o_offq_c;                  // uint16 [rows][G][4] complex (M = zero slot)
+  int64_t o_offq_coef;               // [rows][2][G][2] nu_r * nu_k
   int64_t o_mspec;                   // int32 [K][S]
   int64_t o_mnu;                     // [K][S]
   int64_t o_mnuG;                    // [K][G] dense stoichiometry per lane

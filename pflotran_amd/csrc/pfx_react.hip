@@ -667,22 +667,31 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         absum += fabs(nu) * Cc;
         diag += nu * nu * Cc;
       }
+      // off-diagonal sums, 4 columns per chunk: one slot is one 8-byte load
+      // of 4 complex numbers (16-bit), two 16-byte loads of coefficients and
+      // 4 independent gathers; a column with a shorter list reads the zero
+      // slot C[M] with coefficient 0
 #pragma unroll
-      for (int k = 0; k < G; k++) {
-        // two independent partial sums keep two LDS gathers in flight
-        double acc0 = 0.0, acc1 = 0.0;
-        const int e0 = L.off_start[k], e1 = L.off_start[k + 1];
-        int e = e0;
-        for (; e + 1 < e1; e += 2) {
-          const int c0 = LI(L.o_off_c, e * G + r), c1 = LI(L.o_off_c, (e + 1) * G + r);
-          const double f0 = LD(L.o_off_coef, e * G + r), f1 = LD(L.o_off_coef, (e + 1) * G + r);
-          acc0 += f0 * SD(S.C, c0);
-          acc1 += f1 * SD(S.C, c1);
+      for (int q = 0; q < G / 4; q++) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        const int s1 = L.offq_start[q + 1];
+        for (int s = L.offq_start[q]; s < s1; s++) {
+          const uint2 w = *(const uint2 *)(smem + L.o_offq_c + (s * G + r) * 8);
+          const double2 fa = *(const double2 *)(smem + L.o_offq_coef + ((2 * s) * G + r) * 16);
+          const double2 fb = *(const double2 *)(smem + L.o_offq_coef + ((2 * s + 1) * G + r) * 16);
+          a0 += fa.x * SD(S.C, (int)(w.x & 0xffffu));
+          a1 += fa.y * SD(S.C, (int)(w.x >> 16));
+          a2 += fb.x * SD(S.C, (int)(w.y & 0xffffu));
+          a3 += fb.y * SD(S.C, (int)(w.y >> 16));
         }
-        if (e < e1) acc0 += LD(L.o_off_coef, e * G + r) * SD(S.C, LI(L.o_off_c, e * G + r));
-        const double acc = acc0 + acc1;
-        row[k] = (k == r) ? (live ? 1.0 + diag * SD(S.im, k) : 1.0) : acc * SD(S.im, k);
+        row[4 * q] = a0;
+        row[4 * q + 1] = a1;
+        row[4 * q + 2] = a2;
+        row[4 * q + 3] = a3;
       }
+#pragma unroll
+      for (int k = 0; k < G; k++)
+        row[k] = (k == r) ? (live ? 1.0 + diag * SD(S.im, k) : 1.0) : row[k] * SD(S.im, k);
 #pragma unroll
       for (int k = 0; k < G; k++) row[k] = live ? row[k] * den_kg_per_L : row[k];
       tot_out = (m + Tsum) * den_kg_per_L;
