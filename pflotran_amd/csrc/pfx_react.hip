@@ -119,6 +119,23 @@ __device__ __forceinline__ void gargmax(double &v, int &idx, int lane) {
   }
 }
 
+struct OpMaxNum {  // a NaN candidate never wins the pivot search
+  __device__ __forceinline__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+
+// Pivot row of an elimination step: the group's largest candidate, ties to
+// the larger row (the reference's `>=` scan keeps the last maximum,
+// utility.F90:451-452).  A max reduction, then the highest lane of the
+// group whose candidate equals it (one ballot); a group without a finite
+// candidate keeps its own row (the cell is non-finite and stops).
+template <int G>
+__device__ __forceinline__ int gargmax_row(double cand, int lane, int gbase) {
+  const double vmax = greduce<G>(cand, lane, OpMaxNum());
+  const unsigned long long msk = __ballot(cand == vmax);
+  const unsigned int gm = (unsigned int)(msk >> gbase) & (G == 32 ? 0xffffffffu : ((1u << G) - 1u));
+  return gm ? 31 - __clz((int)gm) : lane - gbase;
+}
+
 __device__ __forceinline__ void wave_sync() {
   // LDS written by other lanes of this wavefront becomes visible to the
   // whole wavefront (lanes of one group never span wavefronts).
@@ -301,6 +318,12 @@ __device__ __forceinline__ void mineral_fd_jacobian(KP Pp, const Scr &S, int r, 
   wave_sync();
 }
 
+// A load issued whether or not the array is given: a missing array reads
+// `dummy` (any valid device address) and the caller selects its default.
+__device__ __forceinline__ double ld_or(const double *a, long long i, const unsigned char *dummy) {
+  return *(a ? a + i : (const double *)dummy);
+}
+
 template <int G, int MODE>
 __device__ __forceinline__ void react_body(uint64_t praw) {
   const KP P0 = (KP)praw;
@@ -341,6 +364,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   bool stop_next = false;   // reference relative-change stop: finish at the next evaluation
   int cell_flag = PFX_CELL_CONVERGED;
   double m = 1.0, gam = 1.0, lng = 0.0;
+  double mguess = 1.0;  // the cell's warm-start guess, loaded with its inputs
   // constraint speciation state
   int ctype = PFX_CONSTRAINT_FREE, nit_on = 0;
   bool compute_act = false;
@@ -402,18 +426,48 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           cell_flag = PFX_CELL_CONVERGED;
           compute_act = false;
           nit_on = 0;
-          // ---- load the cell
-          const double porosity = P.porosity ? P.porosity[c] : P.def_porosity;
-          const double sat = P.saturation ? P.saturation[c] : P.def_saturation;
-          const double vol = P.volume ? P.volume[c] : P.def_volume;
-          const double den = P.den_kg ? P.den_kg[c] : P.def_den;
+          // ---- load the cell.  Every global load is issued before any is
+          // used: a missing optional array reads the table blob instead and
+          // its default is selected afterwards, so the loads are in flight
+          // together instead of one round trip per array.
+          const unsigned char *dmy = P.tables;
+          const int rr = live ? r : 0;
+          const long long ci = c * N + rr;
+          const double v_por = ld_or(P.porosity, c, dmy);
+          const double v_sat = ld_or(P.saturation, c, dmy);
+          const double v_vol = ld_or(P.volume, c, dmy);
+          const double v_den = ld_or(P.den_kg, c, dmy);
+          const double v_tmp = ld_or(P.temperature, c, dmy);
+          double v_m = 1.0, v_gam = 1.0, v_tot = 0.0, v_sorb = 0.0, v_conc = 0.0;
+          if (MODE == PFX_MODE_EQUIL) {
+            v_conc = ld_or(P.constraint_conc, ci, dmy);
+          } else {
+            v_m = ld_or(P.pri_molal, ci, dmy);
+            v_gam = ld_or(P.pri_act_coef, ci, dmy);
+            if (MODE != PFX_MODE_AUX) {
+              v_tot = ld_or(P.total, ci, dmy);
+              v_sorb = ld_or(R > 0 ? P.total_sorb : nullptr, ci, dmy);
+            }
+          }
+          if (MODE == PFX_MODE_REACT)
+            mguess = ld_or((P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK) ? P.guess
+                                                                                                          : nullptr,
+                           ci, dmy);
+          double v_ionx[PFX_MAX_IONX_RXN];
+#pragma unroll
+          for (int q = 0; q < PFX_MAX_IONX_RXN; q++)
+            v_ionx[q] = ld_or((MODE != PFX_MODE_EQUIL && q < R) ? P.ionx_ref : nullptr, c * R + q, dmy);
+          const double porosity = P.porosity ? v_por : P.def_porosity;
+          const double sat = P.saturation ? v_sat : P.def_saturation;
+          const double vol = P.volume ? v_vol : P.def_volume;
+          const double den = P.den_kg ? v_den : P.def_den;
           if (r == 0) {
             SD(S.cs, 0) = porosity;
             SD(S.cs, 1) = sat;
             SD(S.cs, 2) = vol;
             SD(S.cs, 3) = den;
           }
-          const double tempc = P.temperature ? P.temperature[c] : 25.0;
+          const double tempc = P.temperature ? v_tmp : 25.0;
           const double tk = tempc + 273.15;
           const double ln_tk = log(tk);
           if (MODE == PFX_MODE_EQUIL) {
@@ -422,7 +476,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             const double molal_to_molar = P.molal ? den / 1000.0 : 1.0;
             const double molar_to_molal = P.molal ? 1.0 : 1000.0 / den;
             ctype = live ? P.ctype[r] : PFX_CONSTRAINT_FREE;
-            const double conc = live ? P.constraint_conc[c * N + r] : 0.0;
+            const double conc = live ? v_conc : 0.0;
             free_conc = 1.0e-9;
             tcon = 0.0;
             if (ctype == PFX_CONSTRAINT_TOTAL) tcon = conc * molal_to_molar;
@@ -440,20 +494,18 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             }
             m = live ? free_conc : 1.0;
           } else {
-            if (live) {
-              m = P.pri_molal[c * N + r];
-              gam = P.pri_act_coef ? P.pri_act_coef[c * N + r] : 1.0;
-              if (MODE != PFX_MODE_AUX) {
-                const double tot_in = P.total[c * N + r];
-                const double sorb_old = (R > 0 && P.total_sorb) ? P.total_sorb[c * N + r] : 0.0;
-                SD(S.fx, r) = porosity * sat * 1000.0 * vol * tot_in + (R > 0 ? sorb_old * vol : 0.0);
-              }
-            } else {
-              m = 1.0;
-              gam = 1.0;
-              SD(S.fx, r) = 0.0;
+            m = live ? v_m : 1.0;
+            gam = (live && P.pri_act_coef) ? v_gam : 1.0;
+            if (MODE != PFX_MODE_AUX) {
+              const double sorb_old = (R > 0 && P.total_sorb) ? v_sorb : 0.0;
+              SD(S.fx, r) = live ? porosity * sat * 1000.0 * vol * v_tot + (R > 0 ? sorb_old * vol : 0.0) : 0.0;
             }
-            for (int cc = r; cc < M; cc += G) SD(S.gsec, cc) = P.sec_act_coef ? P.sec_act_coef[c * M + cc] : 1.0;
+            // up to 4 complexes per pass with their loads issued together
+#pragma unroll 4
+            for (int cc = r; cc < M; cc += G) {
+              const double g = ld_or(P.sec_act_coef, c * M + cc, dmy);
+              SD(S.gsec, cc) = P.sec_act_coef ? g : 1.0;
+            }
           }
           for (int cc = r; cc < M; cc += G) {
             double lk = LD(L.o_clogK, cc);
@@ -464,6 +516,8 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             SD(S.lnK, cc) = -lk * PFX_LOG_TO_LN;
           }
           for (int k = r; k < K; k += G) {
+            const double v_area = ld_or(P.mnrl_area, c * K + k, dmy);
+            const double v_vf = ld_or(P.mnrl_volfrac, c * K + k, dmy);
             double lk = LD(L.o_mlogK, k);
             if (P.temperature && L.has_mnrl_coef) {
               const int cfo = (int)L.o_mlogKcoef + 5 * k * 8;
@@ -474,19 +528,19 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             const double Ea = LD(L.o_mEa, k);
             if (Ea > 0.0) arr = exp(Ea / PFX_IDEAL_GAS * (1.0 / (25.0 + 273.15) - 1.0 / (tempc + 273.15)));
             SD(S.mkeff, k) = LD(L.o_mk, k) * arr;
-            SD(S.marea, k) = P.mnrl_area ? P.mnrl_area[c * K + k] : 0.0;
-            SD(S.mvf, k) = P.mnrl_volfrac ? P.mnrl_volfrac[c * K + k] : 0.0;
+            SD(S.marea, k) = P.mnrl_area ? v_area : 0.0;
+            SD(S.mvf, k) = P.mnrl_volfrac ? v_vf : 0.0;
             SD(S.mrate, k) = 0.0;
           }
 #pragma unroll
           for (int q = 0; q < PFX_MAX_IONX_RXN; q++)
-            SD(S.ionx, q) = (MODE != PFX_MODE_EQUIL && q < R && P.ionx_ref) ? P.ionx_ref[c * R + q] : 1e-9;
+            SD(S.ionx, q) = (MODE != PFX_MODE_EQUIL && q < R && P.ionx_ref) ? v_ionx[q] : 1e-9;
           lng = log(gam);
           if (MODE == PFX_MODE_REACT && L.act_freq == PFX_ACT_OFF) {
             phase = 2;
             if (live) {
               if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
-                m = P.guess[c * N + r];
+                m = mguess;
               else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
             }
           }
@@ -518,7 +572,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           phase = 2;
           if (live) {
             if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
-              m = P.guess[cell * N + r];
+              m = mguess;
             else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
           }
         }
@@ -616,7 +670,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         phase = 2;
         if (live) {
           if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
-            m = P.guess[cell * N + r];
+            m = mguess;
           else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
         }
       }
@@ -979,9 +1033,11 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         const bool lg = live && use_log;
 #pragma unroll
         for (int k = 0; k < G; k++) {
-          row[k] = row[k] * norm;
+          // a multiply by 1 when the columns are not scaled (exact): a
+          // select of the product made the compiler branch around the load
           const double sk = SD(S.mv, k);
-          row[k] = lg ? row[k] * sk : row[k];
+          row[k] = row[k] * norm;
+          row[k] = row[k] * (lg ? sk : 1.0);
         }
       }
       double vv = 0.0;
@@ -998,8 +1054,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       for (int j = 0; j < G; j++) {
         const bool in_play = live || j >= N;  // padding rows join after step N
         double cand = (pivoted || !in_play) ? -1.0 : vv * fabs(row[j]);
-        int p = r;
-        gargmax<G>(cand, p, lane);
+        const int p = gargmax_row<G>(cand, lane, gbase);
         const int src = gbase + p;
         // stream the pivot row one element at a time (lane to lane; an LDS
         // broadcast of the row measured slower: it adds a store-to-load
