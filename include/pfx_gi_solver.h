@@ -128,6 +128,30 @@ int pfx_gi_solver_destroy(pfx_gi_solver *s);
 int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *grid, const pfx_react_opts *opts,
                        const pfx_newton_opts *nopts, pfx_newton_stats *stats);
 
+/* The transport half of an operator-split step (the reference's
+   deprecated/timestepper.F90:2156-2245 sequence): one implicit step of the
+   totals of every component, A T_new = b, with A the scalar transport
+   matrix of RTCalculateTransportMatrix (reactive_transport.F90:1379-1601):
+   upwind face coefficients (TFluxCoef, transport.F90:756-819), boundary
+   faces on the diagonal, accumulation phi*s*1000*V/dt (phi, s, V from
+   cells or the opts defaults, dt = opts->dt); and b of RTCalculateRHS_t0
+   (:1085-1146) plus T_up * T_bc on boundary faces with a boundary total
+   (RTCalculateRHS_t1, :1232-1258).  A zero-gradient boundary face (no
+   boundary total) carries the cell's new total, on the diagonal (the
+   reference's RTCalculateRHS_t1 stops with "must be refactored", so it
+   fixes no other choice).  A is the same for all components: it is
+   assembled once and each component solved by pfx_bsr_solve.
+   h: a handle of block size 1 on pfx_gi_structure's pattern for the grid;
+   cells->total [ncells][ncomp] (device): in the old totals, out the
+   transported ones; work: device doubles [nnzb + 2 ncells].  stats sum
+   the components' iterations and keep the worst reason.  Returns 0, 1 if
+   a component's solve did not converge, or a negative PFX_E_* code.
+   No counterpart in the current reference build (its operator split is
+   disabled). */
+int pfx_os_transport_step(pfx_bsr *h, const pfx_grid *grid, pfx_cells *cells, int32_t ncomp,
+                          const pfx_react_opts *opts, double *work, const pfx_krylov_opts *kopts,
+                          pfx_krylov_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -326,3 +326,76 @@ def gi_dispersion(grid: HostGrid, porosity, saturation=None, tortuosity=None, di
                          bt.ctypes.data, ddp, bcp)
     return dd, bc
 
+
+
+# --------------------------------------------------------------------------
+# operator-split transport half (numpy/scipy restatement)
+
+
+def _flux_coef(area, q, dd):
+    """TFluxCoef with upwinding (transport.F90:756-819): (T_up, T_dn) in L/s."""
+    up = np.where(q > 0.0, dd + q, dd)
+    dn = np.where(q > 0.0, -dd, -dd + q)
+    return up * area * 1000.0, dn * area * 1000.0
+
+
+def os_transport_matrix(grid: HostGrid, acc: np.ndarray):
+    """Scalar transport matrix of RTCalculateTransportMatrix
+    (reactive_transport.F90:1379-1601) as a scipy CSR matrix, natural cell
+    order: per internal face (up, dn) A[up,up] += T_up, A[up,dn] += T_dn,
+    A[dn,dn] -= T_dn, A[dn,up] -= T_up (:1442-1486); per boundary face
+    A[c,c] -= T_dn (:1489-1518), and -= T_up as well when the face carries
+    the cell's own total (zero gradient); accumulation phi*s*1000*V/dt on the
+    diagonal (:1526-1532)."""
+    import scipy.sparse as sp
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    n = nx * ny * nz
+    cid = np.arange(n).reshape(nz, ny, nx)
+    rows, cols, vals = [np.arange(n)], [np.arange(n)], [np.asarray(acc, dtype=np.float64).ravel()]
+    slices = [(np.s_[:, :, :-1], np.s_[:, :, 1:]), (np.s_[:, :-1, :], np.s_[:, 1:, :]),
+              (np.s_[:-1, :, :], np.s_[1:, :, :])]
+    for d in range(3):
+        if grid.q[d].size == 0:
+            continue
+        tu, td = _flux_coef(grid.area[d], grid.q[d], grid.dd[d])
+        up, dn = cid[slices[d][0]].ravel(), cid[slices[d][1]].ravel()
+        tu, td = tu.ravel(), td.ravel()
+        rows += [up, up, dn, dn]
+        cols += [up, dn, dn, up]
+        vals += [tu, td, -td, -tu]
+    for side in range(2):
+        if grid.bc_q[side] is None:
+            continue
+        tu, td = _flux_coef(grid.area[0], grid.bc_q[side], grid.bc_dd[side])
+        c = cid[:, :, 0 if side == 0 else nx - 1].ravel()
+        diag = -td.ravel() - (tu.ravel() if grid.bc_total[side] is None else 0.0)
+        rows.append(c)
+        cols.append(c)
+        vals.append(diag)
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    return A.tocsr()
+
+
+def os_transport_rhs(grid: HostGrid, acc: np.ndarray, total: np.ndarray):
+    """RTCalculateRHS_t0 (reactive_transport.F90:1085-1146): acc * T_old,
+    plus RTCalculateRHS_t1's T_up * T_bc on boundary faces with a boundary
+    total (:1232-1258); [ncells][N]."""
+    b = np.asarray(acc, dtype=np.float64).reshape(-1, 1) * total
+    nx = grid.nx
+    cid = np.arange(grid.ncells).reshape(grid.nz, grid.ny, nx)
+    for side in range(2):
+        if grid.bc_q[side] is None or grid.bc_total[side] is None:
+            continue
+        tu, _ = _flux_coef(grid.area[0], grid.bc_q[side], grid.bc_dd[side])
+        c = cid[:, :, 0 if side == 0 else nx - 1].ravel()
+        b[c] += tu.ravel()[:, None] * grid.bc_total[side][None, :]
+    return b
+
+
+def os_transport_step(grid: HostGrid, acc: np.ndarray, total: np.ndarray):
+    """One implicit transport step of every component (direct solve)."""
+    import scipy.sparse.linalg as spla
+    A = os_transport_matrix(grid, acc).tocsc()
+    b = os_transport_rhs(grid, acc, total)
+    lu = spla.splu(A)
+    return np.stack([lu.solve(b[:, j]) for j in range(b.shape[1])], axis=1)

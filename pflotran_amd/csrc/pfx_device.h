@@ -103,12 +103,18 @@ struct pfx_kernel_params {
 
 // Sizes of a network held by an opaque pfx_tables (defined in pfx_api.hip)
 // for the other translation units of the library.
+struct pfx_bsr;  // include/pfx_gi_solver.h
 namespace pfx_internal {
 void table_sizes(const pfx_tables *t, int *N, int *M, int *K, int *R);
 void table_max_dlnC(const pfx_tables *t, double *max_dlnC);
 int gi_scratch(pfx_tables *t, int64_t ncells, double **rnf, double **jnf, double **tot, double **dtot);
 int gi_evaluate_cells(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, int col_scale, double *rnf,
                       double *jnf, double *tot, double *dtot);
+// shape of a pfx_bsr handle (defined in pfx_krylov.hip)
+const int32_t *bsr_row_ptr(const ::pfx_bsr *h);
+int64_t bsr_rows(const ::pfx_bsr *h);
+int32_t bsr_block(const ::pfx_bsr *h);
+int64_t bsr_nnzb(const ::pfx_bsr *h);
 // event-timed launch bracket; the time goes to the tables' last_ms[slot]
 int timed_launch_begin(pfx_tables *t, hipStream_t s);
 int timed_launch_end(pfx_tables *t, hipStream_t s, int slot);

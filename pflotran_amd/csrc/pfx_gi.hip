@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "pfx_device.h"
+#include "../../include/pfx_gi_solver.h"
 
 namespace {
 
@@ -255,6 +256,118 @@ static bool grid_ok(const pfx_grid *g) {
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// The transport half of an operator-split step: the scalar transport matrix
+// of RTCalculateTransportMatrix (reactive_transport.F90:1379-1601), the same
+// for every component, and the right-hand sides of RTCalculateRHS_t0/_t1
+// (:1085-1375).  Rows in the order of pfx_gi_structure (block size 1).
+
+struct os_params {
+  int32_t nx, ny, nz, N;
+  double area[3];
+  const double *q[3], *dd[3];
+  const double *bc_q[2], *bc_dd[2], *bc_total[2];
+  const int32_t *row_ptr;
+  const double *porosity, *saturation, *volume;
+  double def_porosity, def_saturation, def_volume, dt;
+  double *vals;        // [nnzb] matrix
+  double *rhs, *x;     // [ncells] one component
+  double *total;       // [ncells][N] in: old totals; out: transported
+  int32_t comp;
+};
+
+__device__ __forceinline__ double os_accum(const os_params &P, int64_t c) {
+  const double phi = P.porosity ? P.porosity[c] : P.def_porosity;
+  const double sat = P.saturation ? P.saturation[c] : P.def_saturation;
+  const double vol = P.volume ? P.volume[c] : P.def_volume;
+  return phi * sat * 1000.0 * vol / P.dt;  // reactive_transport.F90:1526-1530
+}
+
+// one thread per cell: the row's values, neighbours in column order
+__global__ void k_os_matrix(const os_params P) {
+  const int64_t nx = P.nx, ny = P.ny, nz = P.nz, n = nx * ny * nz;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+  const int64_t idx[3] = {i, j, k}, ext[3] = {nx, ny, nz};
+  double diag = os_accum(P, c);
+  double off[6];
+  bool has[6];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int64_t ex = nx - (d == 0), ey = ny - (d == 1);
+    const int64_t stride = d == 0 ? 1 : (d == 1 ? nx : nx * ny);
+    const int lo = 2 - d, hi = 3 + d;
+    has[lo] = idx[d] > 0;
+    off[lo] = 0.0;
+    if (has[lo]) {  // c is the downstream cell of its lower face
+      const int64_t up = c - stride;
+      const int64_t f = (up % nx) + ex * (((up / nx) % ny) + ey * (up / (nx * ny)));
+      double Tup, Tdn;
+      flux_coef(P.area[d], P.q[d][f], P.dd[d][f], Tup, Tdn);
+      diag -= Tdn;
+      off[lo] = -Tup;
+    }
+    has[hi] = idx[d] < ext[d] - 1;
+    off[hi] = 0.0;
+    if (has[hi]) {  // c is the upstream cell of its upper face
+      const int64_t f = i + ex * (j + ey * k);
+      double Tup, Tdn;
+      flux_coef(P.area[d], P.q[d][f], P.dd[d][f], Tup, Tdn);
+      diag += Tup;
+      off[hi] = Tdn;
+    }
+  }
+  // boundary connections on the x faces (boundary up, cell down): the
+  // diagonal takes -T_dn; a zero-gradient face (no boundary total) carries
+  // the cell's own total, so its T_up joins the diagonal too
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (P.bc_q[side] && i == (side == 0 ? 0 : nx - 1)) {
+      const int64_t f = j + ny * k;
+      double Tup, Tdn;
+      flux_coef(P.area[0], P.bc_q[side][f], P.bc_dd[side][f], Tup, Tdn);
+      diag -= Tdn;
+      if (!P.bc_total[side]) diag -= Tup;
+    }
+  }
+  int64_t b = P.row_ptr[c];
+  const int order[7] = {0, 1, 2, -1, 3, 4, 5};
+#pragma unroll
+  for (int s = 0; s < 7; s++) {
+    const int o = order[s];
+    if (o < 0) P.vals[b++] = diag;
+    else if (has[o]) P.vals[b++] = off[o];
+  }
+}
+
+// right-hand side of one component: accum * T_old plus T_up * T_bc on faces
+// with a boundary total; the old totals start the solve's bookkeeping only
+__global__ void k_os_rhs(const os_params P) {
+  const int64_t nx = P.nx, ny = P.ny, n = nx * ny * (int64_t)P.nz;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+  double r = os_accum(P, c) * P.total[c * P.N + P.comp];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (P.bc_q[side] && P.bc_total[side] && i == (side == 0 ? 0 : nx - 1)) {
+      const int64_t f = j + ny * k;
+      double Tup, Tdn;
+      flux_coef(P.area[0], P.bc_q[side][f], P.bc_dd[side][f], Tup, Tdn);
+      r += Tup * P.bc_total[side][P.comp];
+    }
+  }
+  P.rhs[c] = r;
+}
+
+__global__ void k_os_scatter(const os_params P) {
+  const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  P.total[c * P.N + P.comp] = P.x[c];
+}
+
 extern "C" int pfx_gi_structure(const pfx_grid *g, int32_t *row_ptr, int32_t *col_idx, int64_t *nnzb) {
   if (!g || !nnzb || g->nx < 1 || g->ny < 1 || g->nz < 1) return PFX_E_ARG;
   const int64_t nx = g->nx, ny = g->ny, nz = g->nz, n = nx * ny * nz;
@@ -359,4 +472,77 @@ extern "C" int pfx_gi_dispersion(const pfx_grid *g, const double *porosity, cons
   GCHK(hipGetLastError());
   GCHK(hipStreamSynchronize(s));
   return 0;
+}
+
+extern "C" int pfx_os_transport_step(pfx_bsr *h, const pfx_grid *g, pfx_cells *cells, int32_t ncomp,
+                                     const pfx_react_opts *o, double *work, const pfx_krylov_opts *kopts,
+                                     pfx_krylov_stats *stats) {
+  if (!h || !cells || !o || !work || !cells->total || ncomp < 1 || !grid_ok(g) || !(o->dt > 0.0))
+    return PFX_E_ARG;
+  const int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  if (cells->ncells != n) return PFX_E_ARG;
+  for (int side = 0; side < 2; side++)
+    if (g->bc_q[side] && !g->bc_dd[side]) return PFX_E_ARG;
+  int64_t nnzb = 0;
+  int rc = pfx_gi_structure(g, nullptr, nullptr, &nnzb);
+  if (rc) return rc;
+  os_params P;
+  memset(&P, 0, sizeof(P));
+  P.nx = g->nx;
+  P.ny = g->ny;
+  P.nz = g->nz;
+  P.N = ncomp;
+  for (int d = 0; d < 3; d++) {
+    P.area[d] = g->area[d];
+    P.q[d] = g->q[d];
+    P.dd[d] = g->dd[d];
+  }
+  for (int s = 0; s < 2; s++) {
+    P.bc_q[s] = g->bc_q[s];
+    P.bc_dd[s] = g->bc_dd[s];
+    P.bc_total[s] = g->bc_total[s];
+  }
+  P.row_ptr = nullptr;
+  P.porosity = cells->porosity;
+  P.saturation = cells->saturation;
+  P.volume = cells->volume;
+  P.def_porosity = o->porosity;
+  P.def_saturation = o->saturation;
+  P.def_volume = o->volume;
+  P.dt = o->dt;
+  P.vals = work;
+  P.rhs = work + nnzb;
+  P.x = work + nnzb + n;
+  P.total = cells->total;
+  P.row_ptr = pfx_internal::bsr_row_ptr(h);
+  // the handle must hold this grid's pattern at block size 1: the matrix
+  // kernel writes nnzb values through its row pointers
+  if (!P.row_ptr || pfx_internal::bsr_rows(h) != n || pfx_internal::bsr_block(h) != 1 ||
+      pfx_internal::bsr_nnzb(h) != nnzb)
+    return PFX_E_ARG;
+  hipStream_t s = (hipStream_t)o->stream;
+  const unsigned gb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_os_matrix, dim3(gb), dim3(256), 0, s, P);
+  GCHK(hipGetLastError());
+  pfx_krylov_stats acc;
+  memset(&acc, 0, sizeof(acc));
+  acc.reason = PFX_KSP_CONVERGED_RTOL;
+  for (int j = 0; j < ncomp; j++) {
+    P.comp = j;
+    hipLaunchKernelGGL(k_os_rhs, dim3(gb), dim3(256), 0, s, P);
+    GCHK(hipGetLastError());
+    pfx_krylov_stats st;
+    rc = pfx_bsr_solve(h, P.vals, P.rhs, P.x, kopts, o->stream, &st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_os_scatter, dim3(gb), dim3(256), 0, s, P);
+    GCHK(hipGetLastError());
+    // the components' worst outcome: a divergence reason wins
+    acc.iterations += st.iterations;
+    if (st.reason < 0 || (acc.reason > 0 && st.reason == PFX_KSP_CONVERGED_ATOL)) acc.reason = st.reason;
+    acc.rnorm0 = fmax(acc.rnorm0, st.rnorm0);
+    acc.rnorm = fmax(acc.rnorm, st.rnorm);
+    acc.ms += st.ms;
+  }
+  if (stats) *stats = acc;
+  return acc.reason < 0 ? 1 : 0;
 }

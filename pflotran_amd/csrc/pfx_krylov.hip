@@ -442,6 +442,14 @@ struct pfx_bsr {
   hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
+// shape of a handle, for the other translation units of the library
+namespace pfx_internal {
+const int32_t *bsr_row_ptr(const pfx_bsr *h) { return h ? h->rp : nullptr; }
+int64_t bsr_rows(const pfx_bsr *h) { return h ? h->nrows : -1; }
+int32_t bsr_block(const pfx_bsr *h) { return h ? h->N : -1; }
+int64_t bsr_nnzb(const pfx_bsr *h) { return h ? h->nnzb : -1; }
+}  // namespace pfx_internal
+
 namespace {
 
 // launch helpers over the block-row groups

@@ -123,7 +123,8 @@ module pfx_chem_module
   end type pfx_newton_stats
 
   public :: pfx_bsr_create, pfx_bsr_destroy, pfx_bsr_spmv, pfx_bsr_solve, &
-            pfx_gi_solver_create, pfx_gi_solver_destroy, pfx_gi_solver_step
+            pfx_gi_solver_create, pfx_gi_solver_destroy, pfx_gi_solver_step, &
+            pfx_os_transport_step
 
   public :: pfx_abi_version, pfx_tables_create, pfx_tables_destroy, &
             pfx_react_batch, pfx_residual_jacobian, pfx_equilibrate_batch, &
@@ -345,6 +346,21 @@ module pfx_chem_module
       type(pfx_newton_stats), intent(out) :: stats
       integer(c_int) :: pfx_gi_solver_step
     end function pfx_gi_solver_step
+
+    ! transport half of an operator-split step (include/pfx_gi_solver.h)
+    function pfx_os_transport_step(bsr, grid, cells, ncomp, opts, work, kopts, stats) &
+        bind(c, name='pfx_os_transport_step')
+      import :: c_int, c_int32_t, c_ptr, pfx_cells, pfx_grid, pfx_react_opts, pfx_krylov_opts, pfx_krylov_stats
+      type(c_ptr), value :: bsr
+      type(pfx_grid), intent(in) :: grid
+      type(pfx_cells), intent(inout) :: cells
+      integer(c_int32_t), value :: ncomp
+      type(pfx_react_opts), intent(in) :: opts
+      type(c_ptr), value :: work
+      type(pfx_krylov_opts), intent(in) :: kopts
+      type(pfx_krylov_stats), intent(out) :: stats
+      integer(c_int) :: pfx_os_transport_step
+    end function pfx_os_transport_step
 
     function pfx_graph_destroy(graph) bind(c, name='pfx_graph_destroy')
       import :: c_int, c_ptr

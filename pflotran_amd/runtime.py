@@ -30,7 +30,7 @@ EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_r
             "pfx_surrogate_guess_from",
             # global-implicit step on the device (include/pfx_gi_solver.h)
             "pfx_bsr_create", "pfx_bsr_destroy", "pfx_bsr_spmv", "pfx_bsr_solve", "pfx_gi_solver_create",
-            "pfx_gi_solver_destroy", "pfx_gi_solver_step"]
+            "pfx_gi_solver_destroy", "pfx_gi_solver_step", "pfx_os_transport_step"]
 
 _lib = None
 
@@ -95,6 +95,8 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_gi_solver_destroy.argtypes = [vp]
     lib.pfx_gi_solver_step.argtypes = [vp, P(abi.Cells), P(abi.Grid), P(abi.ReactOpts), P(abi.NewtonOpts),
                                        P(abi.NewtonStats)]
+    lib.pfx_os_transport_step.argtypes = [vp, P(abi.Grid), P(abi.Cells), i32, P(abi.ReactOpts), vp,
+                                          P(abi.KrylovOpts), P(abi.KrylovStats)]
     if lib.pfx_abi_version() != abi.PFX_ABI_VERSION:
         raise PfxError("ABI version mismatch between libpfx_chem.so and pflotran_amd.abi")
     _lib = lib
@@ -384,6 +386,40 @@ class Bsr:
             self.close()
         except Exception:
             pass
+
+
+class OsTransport:
+    """pfx_os_transport_step: the implicit transport half of an
+    operator-split step on a grid (the transport matrix with block size 1 on
+    the grid's 7-point pattern, one BiCGStab solve per component)."""
+
+    def __init__(self, grid: DeviceGrid):
+        torch = _torch()
+        self.lib = load_library()
+        self.row_ptr, self.col_idx = gi_structure(grid)
+        self.bsr = Bsr(self.row_ptr, self.col_idx, 1)
+        n = grid.ncells
+        self.work = torch.empty(int(self.col_idx.numel()) + 2 * n, dtype=torch.float64, device="cuda")
+
+    def step(self, cells: DeviceCells, grid: DeviceGrid, ncomp: int, opts: abi.ReactOpts, rtol: float = 1e-12,
+             max_iterations: int = 0):
+        """Transport cells' totals by one step of opts.dt (in place):
+        (return code 0 or 1 = some component did not converge, KrylovStats)."""
+        torch = _torch()
+        if opts.stream is None:
+            opts.stream = torch.cuda.current_stream().cuda_stream
+        ko = abi.KrylovOpts(rtol=rtol, max_iterations=max_iterations)
+        st = abi.KrylovStats()
+        cs = cells.struct()
+        g = grid.struct()
+        rc = self.lib.pfx_os_transport_step(self.bsr.handle, C.byref(g), C.byref(cs), int(ncomp), C.byref(opts),
+                                            self.work.data_ptr(), C.byref(ko), C.byref(st))
+        if rc < 0:
+            raise PfxError(f"pfx_os_transport_step failed: {rc}")
+        return rc, st
+
+    def close(self):
+        self.bsr.close()
 
 
 class GiSolver:

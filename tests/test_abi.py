@@ -46,6 +46,7 @@ def test_null_arguments_rejected_without_gpu_work():
     assert lib.pfx_bsr_solve(None, None, None, None, None, None, None) == abi.PFX_E_ARG
     assert lib.pfx_gi_solver_create(None, None, None) == abi.PFX_E_ARG
     assert lib.pfx_gi_solver_step(None, None, None, None, None, None) == abi.PFX_E_ARG
+    assert lib.pfx_os_transport_step(None, None, None, 0, None, None, None, None) == abi.PFX_E_ARG
 
 
 def test_every_header_compiles_as_c(tmp_path):
