@@ -364,7 +364,6 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   bool stop_next = false;   // reference relative-change stop: finish at the next evaluation
   int cell_flag = PFX_CELL_CONVERGED;
   double m = 1.0, gam = 1.0, lng = 0.0;
-  double mguess = 1.0;  // the cell's warm-start guess, loaded with its inputs
   // constraint speciation state
   int ctype = PFX_CONSTRAINT_FREE, nit_on = 0;
   bool compute_act = false;
@@ -449,10 +448,14 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
               v_sorb = ld_or(R > 0 ? P.total_sorb : nullptr, ci, dmy);
             }
           }
+          double mguess = 1.0;
           if (MODE == PFX_MODE_REACT)
             mguess = ld_or((P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK) ? P.guess
                                                                                                           : nullptr,
                            ci, dmy);
+          // the warm-start guess waits in the group's exchange slot (unused
+          // until the first ion-exchange pass) instead of a register
+          SD(S.xb, r) = mguess;
           double v_ionx[PFX_MAX_IONX_RXN];
 #pragma unroll
           for (int q = 0; q < PFX_MAX_IONX_RXN; q++)
@@ -540,7 +543,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             phase = 2;
             if (live) {
               if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
-                m = mguess;
+                m = SD(S.xb, r);
               else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
             }
           }
@@ -572,7 +575,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           phase = 2;
           if (live) {
             if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
-              m = mguess;
+              m = SD(S.xb, r);
             else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
           }
         }
@@ -670,7 +673,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         phase = 2;
         if (live) {
           if (P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK)
-            m = mguess;
+            m = SD(S.xb, r);
           else if (P.guess_mode == PFX_GUESS_DEFAULT) m = P.default_guess;
         }
       }
