@@ -257,6 +257,7 @@ void fill_params(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, pfx_k
 int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o) {
   if (!c || !o) return PFX_E_ARG;
   if (c->ncells < 0 || c->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
+  if (c->ncells == 0) return 0;  // an empty batch needs no arrays (RTReact over no cells)
   if (!c->total || !c->pri_molal) return PFX_E_ARG;
   if (t->L.R > 0 && !c->total_sorb) return PFX_E_ARG;
   if (t->L.K > 0 && (!c->mnrl_volfrac || !c->mnrl_area)) return PFX_E_ARG;
@@ -821,10 +822,11 @@ extern "C" int pfx_residual_jacobian(pfx_tables *t, pfx_cells *cells, const pfx_
 extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int32_t *constraint_type,
                                      const double *constraint_conc, const pfx_constraint_rxn *constraint_rxn,
                                      int32_t molal, const pfx_react_opts *o, pfx_stats *stats) {
-  if (!t || !cells || !o || !constraint_type || !constraint_conc || !cells->pri_molal || !cells->total)
-    return PFX_E_ARG;
+  if (!t || !cells || !o || !constraint_type) return PFX_E_ARG;
   if (cells->ncells < 0 || cells->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
-  if (t->L.R > 0 && !cells->total_sorb) return PFX_E_ARG;
+  // an empty batch needs no device arrays
+  if (cells->ncells > 0 && (!constraint_conc || !cells->pri_molal || !cells->total)) return PFX_E_ARG;
+  if (cells->ncells > 0 && t->L.R > 0 && !cells->total_sorb) return PFX_E_ARG;
   if (stats) {
     memset(stats, 0, sizeof(*stats));
     stats->ncells = cells->ncells;
@@ -873,10 +875,11 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
 }
 
 extern "C" int pfx_auxvar_batch(pfx_tables *t, pfx_cells *cells, int32_t activity_passes, const pfx_react_opts *o) {
-  if (!t || !cells || !o || !cells->pri_molal || !cells->total || activity_passes < 0) return PFX_E_ARG;
+  if (!t || !cells || !o || activity_passes < 0) return PFX_E_ARG;
   if (cells->ncells < 0 || cells->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
+  if (cells->ncells == 0) return 0;  // an empty batch needs no device arrays
+  if (!cells->pri_molal || !cells->total) return PFX_E_ARG;
   if (t->L.R > 0 && !cells->total_sorb) return PFX_E_ARG;
-  if (cells->ncells == 0) return 0;
   HIPCHK(hipSetDevice(t->device));
   pfx_kernel_params P;
   fill_params(t, cells, o, P);
