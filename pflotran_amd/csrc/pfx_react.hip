@@ -30,6 +30,24 @@ namespace {
 
 constexpr int BLOCK = 256;
 
+// Profiling build (-DPFX_SECTION_TIMERS, pflotran_amd/build.py build_profiling):
+// shader-clock cycles per section of the Newton pass, summed per wavefront
+// and added into a device array when the wavefront exits; the product build
+// compiles every marker to nothing.  Sections: 0 refill/load, 1 activity,
+// 2 complexes, 3 totals and dT/dm, 4 ion exchange, 5 residual, minerals and
+// convergence, 6 constraint rows, 7 LU solve, 8 update and retire.
+#ifdef PFX_SECTION_TIMERS
+__device__ unsigned long long g_section_cycles[16];
+#define PFX_PT(i)                                  \
+  {                                                \
+    const unsigned long long t_ = clock64();       \
+    pt_acc[i] += t_ - pt_last;                     \
+    pt_last = t_;                                  \
+  }
+#else
+#define PFX_PT(i)
+#endif
+
 __device__ __forceinline__ double bperm_d(double v, int src_lane) {
   int lo = __double2loint(v), hi = __double2hiint(v);
   lo = __builtin_amdgcn_ds_bpermute(src_lane << 2, lo);
@@ -356,6 +374,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   long long q_next = 0, q_end = 0;
   bool exhausted = false;
 
+#ifdef PFX_SECTION_TIMERS
+  unsigned long long pt_acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long pt_last = clock64();
+#endif
   // per-group cell state
   long long cell = -1;
   int phase = 0, it = 0, extra = 0;
@@ -372,6 +394,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 
   for (;;) {
     asm volatile("" : "+s"(praw));
+    PFX_PT(8);
     KP Pp = (KP)praw;
 #define P (*Pp)
 #define L (Pp->L)
@@ -551,6 +574,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       }
     }
     if (__ballot(cell >= 0) == 0ull) break;
+    PFX_PT(0);
     wave_sync();
     const bool active = cell >= 0;
 
@@ -678,6 +702,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         }
       }
     }
+    PFX_PT(1);
     if (MODE == PFX_MODE_EQUIL && active && live &&
         (ctype == PFX_CONSTRAINT_FREE || ctype == PFX_CONSTRAINT_LOG))
       m = free_conc;  // reaction.F90:1552-1558
@@ -706,6 +731,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     }
     wave_sync();
 
+    PFX_PT(2);
     // phase B: totals, |contributions|, and row r of dT/dm (kg/L)
     // (not needed at the previous state, whose complexes only feed the
     // activity update)
@@ -753,6 +779,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       for (int k = 0; k < G; k++) row[k] = live ? row[k] * den_kg_per_L : row[k];
       tot_out = (m + Tsum) * den_kg_per_L;
     }
+    PFX_PT(3);
     if (MODE == PFX_MODE_EVAL && active && live) {
       // aqueous totals and dT/dm of the cell for the flux terms of the
       // global-implicit assembly (RTAuxVarCompute's total and dtotal)
@@ -859,6 +886,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       }
     }
     sorb_out = Sr;
+    PFX_PT(4);
 
     // ---------------------------------------------------------------
     // residual row, Jacobian row, convergence test
@@ -951,6 +979,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         }
       }
     }
+    PFX_PT(5);
     if (MODE == PFX_MODE_EQUIL && active) {
       // constraint rows (reaction.F90:1597-1660)
       if (P.has_charge) {
@@ -1016,6 +1045,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     wave_sync();
 
     // ---------------------------------------------------------------
+    PFX_PT(6);
     // RSolve: row equilibration, ln-column scaling, pivoted LU, solve
     double upd = 0.0;
     if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
@@ -1099,6 +1129,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     }
 
     // ---------------------------------------------------------------
+    PFX_PT(7);
     // Newton update (reaction.F90:3454-3470; constraint form :1806-1826)
     if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
       const double mprev = m;
@@ -1200,6 +1231,13 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 #undef P
 #undef L
   }
+#ifdef PFX_SECTION_TIMERS
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 10; i++) atomicAdd(&g_section_cycles[i], pt_acc[i]);
+    atomicAdd(&g_section_cycles[15], 1ull);
+  }
+#endif
 }
 
 // W: minimum wavefronts per SIMD the register allocation must allow (1 =
@@ -1256,3 +1294,21 @@ PFX_INST(16)
 PFX_INST(32)
 
 extern "C" int pfx_scr_doubles(int M, int G, int K) { return scr_doubles(M, G, K); }
+
+#ifdef PFX_SECTION_TIMERS
+// Profiling build only (no counterpart in the reference): read and clear the
+// per-section cycle sums of the current device; out[0..9] the sections,
+// out[15] the number of wavefronts that added to them.
+extern "C" int pfx_section_cycles(unsigned long long *out, int32_t clear) {
+  if (!out) return PFX_E_ARG;
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_section_cycles), sizeof(unsigned long long) * 16, 0,
+                          hipMemcpyDeviceToHost) != hipSuccess)
+    return PFX_E_DEVICE;
+  if (clear) {
+    unsigned long long z[16] = {0};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_section_cycles), z, sizeof(z), 0, hipMemcpyHostToDevice) != hipSuccess)
+      return PFX_E_DEVICE;
+  }
+  return 0;
+}
+#endif
