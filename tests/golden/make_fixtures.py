@@ -1,7 +1,7 @@
 """Regenerate the committed fixtures under tests/golden/ from the reference
 tree (run where the reference tree is present; the device runs never read
-the reference).  Fixtures are data: database excerpts and the regression-gold
-values of the reference's own batch chemistry tests.
+the reference).  Fixtures are data: database excerpts, the input decks and
+the regression-gold values of the reference's own batch chemistry tests.
 
     python tests/golden/make_fixtures.py [/root/reference]
 """
