@@ -107,11 +107,12 @@ def _num(tok: str) -> float:
         raise DeckError(f"'{tok}' is not a number") from None
 
 
-def _lines(text: str) -> List[List[str]]:
+def _lines(text: str, base_dir: Optional[str] = None, depth: int = 0) -> List[List[str]]:
     """Token lists of the deck's lines as InputReadPflotranString yields
     them (input_aux.F90:642-732): lines opening with '#' or '!' and blank
     lines skipped, a line cut at its first '#' or '!', SKIP ... NOSKIP
-    regions (nested) dropped."""
+    regions (nested) dropped, an EXTERNAL_FILE line replaced by that file's
+    lines (relative to the deck's folder)."""
     out, skip = [], 0
     for raw in text.splitlines():
         s = raw.strip()
@@ -133,6 +134,15 @@ def _lines(text: str) -> List[List[str]]:
                 skip -= 1
             continue
         if w.startswith("NOSK"):
+            continue
+        if w.startswith("EXTERNAL_FILE"):
+            if base_dir is None or len(toks) < 2:
+                raise DeckError("EXTERNAL_FILE needs the deck's path (read_deck)")
+            if depth > 8:
+                raise DeckError("EXTERNAL_FILE nested too deeply")
+            sub = os.path.join(base_dir, toks[1])
+            with open(sub) as f:
+                out += _lines(f.read(), os.path.dirname(os.path.abspath(sub)), depth + 1)
             continue
         out.append(toks)
     return out
@@ -218,6 +228,7 @@ class TimeCard:
     final: Optional[float] = None                 # s
     dt_init: Optional[float] = None
     dt_max: Optional[float] = None
+    waypoint_dt: bool = False                     # MAXIMUM_TIMESTEP_SIZE ... at <time> lines
 
 
 @dataclass
@@ -313,6 +324,8 @@ class Deck:
         tc = self.time
         if tc.final is None or tc.dt_init is None:
             raise DeckError("TIME needs FINAL_TIME and INITIAL_TIMESTEP_SIZE")
+        if tc.waypoint_dt:
+            raise DeckError("MAXIMUM_TIMESTEP_SIZE ... at <time> waypoints are not modelled")
         dt_max = tc.dt_max if tc.dt_max is not None else tc.dt_init
         if tc.dt_init < dt_max:
             raise DeckError("time-step growth (INITIAL_TIMESTEP_SIZE < MAXIMUM_TIMESTEP_SIZE) is not modelled")
@@ -543,16 +556,17 @@ def _time(cur: _Cursor) -> TimeCard:
         elif w == "INITIAL_TIMESTEP_SIZE":
             tc.dt_init = _value_units(t, 1, "s", w)
         elif w == "MAXIMUM_TIMESTEP_SIZE":
-            if len(t) > 3:
-                raise DeckError("MAXIMUM_TIMESTEP_SIZE ... at <time> waypoints are not modelled")
-            tc.dt_max = _value_units(t, 1, "s", w)
+            if len(t) > 3 and t[3].upper() == "AT":
+                tc.waypoint_dt = True   # a later maximum from a waypoint on
+            else:
+                tc.dt_max = _value_units(t, 1, "s", w)
         elif w == "STEADY_STATE":
             raise DeckError("TIME STEADY_STATE is outside the per-cell path")
     return tc
 
 
 def parse_deck(text: str, path: Optional[str] = None) -> Deck:
-    cur = _Cursor(_lines(text))
+    cur = _Cursor(_lines(text, os.path.dirname(os.path.abspath(path)) if path else None))
     chem, constraints, porosity = None, {}, {}
     tc, temperature, numjac = TimeCard(), 25.0, False
     while cur.i < len(cur.lines):

@@ -237,3 +237,20 @@ def test_parser_errors(edit, message):
         d = parse_deck(MINI.replace(*edit))
         # a constraint that misses a primary fails where its arrays are built
         d.constraint_setup("c1", None, None)
+
+
+def test_external_file_and_waypoint_times(tmp_path):
+    """EXTERNAL_FILE lines are replaced by the named file's lines
+    (input_aux.F90:682-686); a TIME card with a waypoint maximum parses, and
+    only the schedule, which does not model waypoints, refuses it."""
+    (tmp_path / "chem.txt").write_text("CHEMISTRY\n  PRIMARY_SPECIES\n    H+\n    HCO3-\n  /\n  DATABASE x.dat\nEND\n")
+    (tmp_path / "main.in").write_text(
+        "EXTERNAL_FILE chem.txt\nCONSTRAINT c\n  CONCENTRATIONS\n    H+ 7. P\n    HCO3- 1.d-3 T\n  /\nEND\n"
+        "TIME\n  FINAL_TIME 1 d\n  INITIAL_TIMESTEP_SIZE 1 h\n  MAXIMUM_TIMESTEP_SIZE 1 h\n"
+        "  MAXIMUM_TIMESTEP_SIZE 6 h at 12 h\nEND\n")
+    d = read_deck(str(tmp_path / "main.in"))
+    assert d.chemistry.primaries == ["H+", "HCO3-"] and list(d.constraints) == ["c"]
+    with pytest.raises(DeckError, match="waypoints"):
+        d.time_steps()
+    with pytest.raises(DeckError, match="EXTERNAL_FILE"):
+        parse_deck((tmp_path / "main.in").read_text())
