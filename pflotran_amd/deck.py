@@ -40,8 +40,8 @@ from typing import Dict, Iterator, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import abi
-from .database import (Database, IonExchangeRxn, MineralKinetics, ReactionTables, build_tables, constraint_rxn,
-                       constraint_rxns, read_database)
+from .database import (Database, DbSpecies, IonExchangeRxn, MineralKinetics, ReactionTables, build_tables,
+                       constraint_rxn, constraint_rxns, read_database)
 
 
 class DeckError(ValueError):
@@ -243,12 +243,22 @@ class Deck:
 
     # ---------------------------------------------------------------- builds
 
+    def tracer_only(self) -> bool:
+        """No complexes, minerals, sorption or gases: the reference does not
+        switch on its full geochemistry (reaction.F90:899-906)."""
+        c = self.chemistry
+        return not (c.secondaries or c.minerals or c.kinetics or c.ion_exchange or c.gases)
+
     def database(self, path: Optional[str] = None) -> Database:
         """The thermodynamic database the deck names, resolved against the
-        deck's folder (or `path` when given)."""
+        deck's folder (or `path` when given).  A tracer-only deck may name
+        none: its primaries are uncharged and take part in no reaction."""
         p = path or self.chemistry.database
         if p is None:
-            raise DeckError("CHEMISTRY has no DATABASE")
+            if not self.tracer_only():
+                raise DeckError("CHEMISTRY has no DATABASE")
+            return Database(temperatures=[25.0],
+                            primary={n: DbSpecies(n, 0.0, 0.0, 0.0) for n in self.chemistry.primaries})
         if not os.path.isabs(p) and self.path:
             p = os.path.join(os.path.dirname(os.path.abspath(self.path)), p)
         return read_database(p)
@@ -256,8 +266,10 @@ class Deck:
     def tables(self, db: Database, fit_logk: bool = False) -> ReactionTables:
         c = self.chemistry
         kin = {n: c.kinetics[n] for n in self.kinetic_minerals()}
+        # without a database the activity update is off (reaction.F90:920-921)
+        act_freq = c.act_freq if c.database is not None else abi.PFX_ACT_OFF
         t = build_tables(db, c.primaries, c.secondaries, kin, c.ion_exchange, temperature=self.temperature,
-                         log_formulation=c.log_formulation, act_freq=c.act_freq, act_alg=c.act_alg,
+                         log_formulation=c.log_formulation, act_freq=act_freq, act_alg=c.act_alg,
                          use_bdot=c.use_bdot, fit_logk=fit_logk)
         t.max_dlnC = c.max_dlnC
         t.max_rel_change_tol = c.max_rel_change_tol

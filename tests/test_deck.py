@@ -254,3 +254,29 @@ def test_external_file_and_waypoint_times(tmp_path):
         d.time_steps()
     with pytest.raises(DeckError, match="EXTERNAL_FILE"):
         parse_deck((tmp_path / "main.in").read_text())
+
+
+def test_tracer_deck_without_database():
+    """A deck with primaries only and no DATABASE card: the reference skips
+    full geochemistry (reaction.F90:899-906), turns the activity update off
+    (:920-921) and takes a total as molarity, so the free ion is
+    T 1000 / rho_w (:3376-3385).  Any reaction without a database is an
+    error."""
+    text = ("CHEMISTRY\n  PRIMARY_SPECIES\n    Tracer\n    Tracer2\n  /\n  OUTPUT\n    ALL\n  /\nEND\n"
+            "CONSTRAINT initial\n  CONCENTRATIONS\n    Tracer 1.d-3 T\n    Tracer2 2.5d-7 T\n  /\nEND\n")
+    d = parse_deck(text)
+    assert d.tracer_only()
+    db = d.database()
+    t = d.tables(db)
+    assert t.act_freq == abi.PFX_ACT_OFF and np.all(np.asarray(t.Z) == 0.0)
+    s = d.constraint_setup("initial", t, db)
+    cells = orc.HostCells.allocate(t, 1)
+    o = abi.default_opts(den_kg=DEN)
+    rc, _ = orc.equilibrate(t, s["ctype"], np.array(s["conc"]), cells, o, molal=s["molal"], rxn=s["rxn"])
+    assert rc == 0
+    assert np.allclose(cells["pri_molal"][0], np.array([1e-3, 2.5e-7]) * 1000.0 / DEN, rtol=1e-13, atol=0)
+    assert np.allclose(cells["total"][0], [1e-3, 2.5e-7], rtol=1e-13, atol=0)
+    assert np.all(cells["pri_act_coef"][0] == 1.0)
+    with_mineral = text.replace("  OUTPUT", "  MINERALS\n    Calcite\n  /\n  OUTPUT")
+    with pytest.raises(DeckError, match="no DATABASE"):
+        parse_deck(with_mineral).database()
