@@ -35,7 +35,7 @@ from __future__ import annotations
 
 import os
 from dataclasses import dataclass, field
-from typing import Dict, Iterator, List, Optional, Sequence, Tuple
+from typing import Callable, Dict, Iterator, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -229,6 +229,10 @@ class TimeCard:
     dt_init: Optional[float] = None
     dt_max: Optional[float] = None
     waypoint_dt: bool = False                     # MAXIMUM_TIMESTEP_SIZE ... at <time> lines
+    # TIMESTEPPER TRANSPORT: TS_ACCELERATION and DT_FACTOR, with the
+    # reference's defaults (timestepper_BE.F90:124-133)
+    iaccel: int = 5
+    tfac: Tuple[float, ...] = (2.0, 2.0, 2.0, 2.0, 2.0, 1.8, 1.6, 1.4, 1.2, 1.0, 1.0, 1.0, 1.0)
 
 
 @dataclass
@@ -326,28 +330,48 @@ class Deck:
         return dict(ctype=ctype, conc=conc, rxn=constraint_rxns(tables, rxns) if rxns else None,
                     vf=vf, area=area, molal=int(c.molal))
 
-    def time_steps(self) -> List[float]:
+    def next_dt(self, dt: float, newton_iterations: int) -> float:
+        """The step size after a step of size `dt` that converged in
+        `newton_iterations` Newton iterations and was not cut
+        (PMRTUpdateTimestep without the volume-fraction governor,
+        pm_rt.F90:631-647, called from TimestepperBEUpdateDT,
+        timestepper_BE.F90:238-246): tfac(n) dt for n up to TS_ACCELERATION
+        and the length of tfac, half of dt beyond, never more than 2 dt or
+        the maximum step.  TS_ACCELERATION 0 keeps dt.  A count below 1 is
+        taken as 1 (the reference would index tfac(0))."""
+        tc = self.time
+        if tc.iaccel == 0:
+            return dt
+        dt_max = tc.dt_max if tc.dt_max is not None else tc.dt_init
+        n = max(int(newton_iterations), 1)
+        dtt = tc.tfac[n - 1] * dt if (n <= tc.iaccel and n <= len(tc.tfac)) else 0.5 * dt
+        return max(min(dtt, 2.0 * dt, dt_max), 1e-20)
+
+    def time_steps(self, newton_iterations: Optional[Callable[[int, float], int]] = None) -> List[float]:
         """Step sizes from 0 to FINAL_TIME for a run whose steps are never
-        cut and whose only waypoint is the final time: min(initial, maximum)
-        steps, the last one shortened to land on the final time
-        (TimestepperBaseSetTargetTime, timestepper_base.F90:320-521, with
-        its 0.1 time-step tolerance).  Growth between the initial and the
-        maximum size is not modelled: decks with INITIAL < MAXIMUM raise."""
+        cut and whose only waypoint is the final time.  Each step is taken
+        at the current size, shortened to land on the final time when it
+        comes within 0.1 of a step of it (TimestepperBaseSetTargetTime,
+        timestepper_base.F90:320-521), and the size is then updated by
+        `next_dt` from the step's Newton count, which
+        `newton_iterations(step_index, dt)` returns (1 when not given: the
+        schedule of a run whose steps all converge within the accelerated
+        counts).  The first step is min(INITIAL, MAXIMUM)."""
         tc = self.time
         if tc.final is None or tc.dt_init is None:
             raise DeckError("TIME needs FINAL_TIME and INITIAL_TIMESTEP_SIZE")
         if tc.waypoint_dt:
             raise DeckError("MAXIMUM_TIMESTEP_SIZE ... at <time> waypoints are not modelled")
         dt_max = tc.dt_max if tc.dt_max is not None else tc.dt_init
-        if tc.dt_init < dt_max:
-            raise DeckError("time-step growth (INITIAL_TIMESTEP_SIZE < MAXIMUM_TIMESTEP_SIZE) is not modelled")
-        dt, t, steps = dt_max, 0.0, []
+        dt, t, steps = min(tc.dt_init, dt_max), 0.0, []
         while True:
             if t + dt + 0.1 * dt >= tc.final:
                 steps.append(tc.final - t)
                 return steps
             steps.append(dt)
             t += dt
+            n = 1 if newton_iterations is None else newton_iterations(len(steps) - 1, dt)
+            dt = self.next_dt(dt, n)
 
 
 # --------------------------------------------------------------------------
@@ -580,7 +604,7 @@ def _time(cur: _Cursor) -> TimeCard:
 def parse_deck(text: str, path: Optional[str] = None) -> Deck:
     cur = _Cursor(_lines(text, os.path.dirname(os.path.abspath(path)) if path else None))
     chem, constraints, porosity = None, {}, {}
-    tc, temperature, numjac = TimeCard(), 25.0, False
+    tc, temperature, numjac, stepper = TimeCard(), 25.0, False, {}
     while cur.i < len(cur.lines):
         t = cur.next("deck")
         w = t[0].upper()
@@ -615,10 +639,21 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
                     raise DeckError("operator-split transport is not reimplemented in the reference either")
         elif w == "TIME":
             tc = _time(cur)
+        elif w == "TIMESTEPPER" and (len(t) < 2 or t[1].upper() == "TRANSPORT"):
+            for s in cur.block("TIMESTEPPER"):
+                v = s[0].upper()
+                if v == "TS_ACCELERATION":
+                    stepper["iaccel"] = int(_num(s[1]))
+                elif v == "DT_FACTOR":
+                    if len(s) < 2:
+                        raise DeckError("TIMESTEPPER DT_FACTOR: the factors go on the keyword's line")
+                    stepper["tfac"] = tuple(_num(x) for x in s[1:])
         elif w == "REFERENCE_TEMPERATURE":
             temperature = _num(t[1])
     if chem is None:
         raise DeckError("deck has no CHEMISTRY card")
+    for k, v in stepper.items():
+        setattr(tc, k, v)
     return Deck(path, chem, constraints, porosity, tc, temperature, numjac)
 
 

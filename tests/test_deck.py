@@ -280,3 +280,34 @@ def test_tracer_deck_without_database():
     with_mineral = text.replace("  OUTPUT", "  MINERALS\n    Calcite\n  /\n  OUTPUT")
     with pytest.raises(DeckError, match="no DATABASE"):
         parse_deck(with_mineral).database()
+
+
+def test_time_step_growth():
+    """TS_ACCELERATION and the default tfac (timestepper_BE.F90:124-133)
+    with PMRTUpdateTimestep's rule (pm_rt.F90:631-647), pinned by closed
+    forms: a step converged in 1-5 iterations doubles dt, up to the maximum;
+    the schedule then lands on the final time.  The exact sequence of a
+    reference run also depends on its Newton counts, which the caller
+    supplies; no reference output records them, so the growth itself is
+    parity unpinned beyond these rules."""
+    d, _ = load("ca-carbonate-debye-huckel-activity")   # TS_ACCELERATION 8, 1e-6 y .. 0.1 y, 1 y
+    assert d.time.iaccel == 8
+    y = units_factor("y", "s")
+    steps = d.time_steps()
+    k = int(np.ceil(np.log2(0.1 / 1e-6)))            # doublings before the cap
+    assert steps[:k] == pytest.approx([1e-6 * y * 2 ** i for i in range(k)], rel=1e-14)
+    assert all(s == pytest.approx(0.1 * y, rel=1e-14) for s in steps[k:-1])
+    assert sum(steps) == pytest.approx(1.0 * y, rel=1e-14) and steps[-1] > 0.1 * 0.1 * y
+    # iteration counts beyond 5: tfac 1.8, 1.6, 1.4 up to TS_ACCELERATION 8, halving beyond
+    dt = 1000.0
+    assert [d.next_dt(dt, n) for n in (1, 5, 6, 7, 8, 9, 20)] == pytest.approx(
+        [2000.0, 2000.0, 1800.0, 1600.0, 1400.0, 500.0, 500.0], rel=1e-15)
+    assert d.next_dt(0.09 * y, 1) == pytest.approx(0.1 * y, rel=1e-15)   # capped at the maximum
+    # the default TS_ACCELERATION (5): a 6-iteration step halves dt
+    d2 = parse_deck("CHEMISTRY\n  PRIMARY_SPECIES\n    A\n  /\nEND\nTIME\n  FINAL_TIME 10 s\n"
+                    "  INITIAL_TIMESTEP_SIZE 1 s\n  MAXIMUM_TIMESTEP_SIZE 4 s\nEND\n"
+                    "TIMESTEPPER TRANSPORT\n  DT_FACTOR 1.5 1.5 1.5\nEND\n")
+    assert d2.time.iaccel == 5 and d2.time.tfac == (1.5, 1.5, 1.5)
+    assert d2.next_dt(2.0, 3) == 3.0 and d2.next_dt(2.0, 4) == 1.0 and d2.next_dt(2.0, 6) == 1.0
+    # steps of 1, 1.5, 2.25, 3.375: the next would pass 10 s within 0.1 of a step
+    assert d2.time_steps() == pytest.approx([1.0, 1.5, 2.25, 3.375, 10.0 - 8.125], rel=1e-15)
