@@ -15,13 +15,19 @@ the GPU on pairs that the GPU solver generates from training cells drawn with
 a different RNG key.
 
 Timed, per step (inputs already in HBM):
-  plain: pfx_react_batch from the reference's default guess (1e-9 molal);
+  plain_previous: pfx_react_batch from the previous free ions (RReact's own
+         starting point, reaction.F90:3370): the plain GPU Newton;
+  plain: pfx_react_batch from the constraint default guess (1e-9 molal,
+         reaction.F90:1481);
   warm:  pfx_surrogate_guess (features + network + guess, one launch), then
-         pfx_react_batch from that guess with fallback to the default guess.
-`value` is the warm method's converged cells/s; the plain figure sits beside
-it.  The CPU baseline is the oracle (OpenMP C restatement of RReact) on the
-first cells of the same held-out batch, and its free ions are compared with
-the GPU's.
+         pfx_react_batch from that guess, a failed attempt repeating the
+         step from the previous free ions.
+All three stop at rel_res <= 1e-10 with the step test (Newton step at the
+accepted iterate below 1e-11 relative).  `value` is the warm method's
+converged cells/s; both plain figures sit beside it.  The CPU baseline is
+the oracle (OpenMP C restatement of RReact) from the previous free ions on
+the first cells of the same held-out batch, timed on every host core of the
+GPU's share and on one core, and its free ions are compared with the GPU's.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--ncells N] [--no-cpu]
                 [--train-pairs P] [--epochs E] [--no-surrogate]
@@ -104,30 +110,36 @@ def main():
     def restore():
         workload.restore(dc, snap)
 
-    # ---- plain GPU Newton from the default guess
-    plain_opts = synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT)
     box = {}
+    n_cpu = min(args.cpu_sample or 12000 * cpu_threads(), n)
+    gpu_out = {}   # free ions and status of each GPU method on the CPU baseline's cells
 
-    def plain_step():
-        restore()
-        box["rc"], box["st"] = runtime.react_batch(tab, dc, plain_opts)
+    def plain_leg(key, opts):
+        """one plain method: autotuned kernel variant, warm-up, timed steps"""
+        def step():
+            restore()
+            box["rc"], box["st"] = runtime.react_batch(tab, dc, opts)
+        tune = autotune.tune_kernel(tab, dc, lambda: runtime.react_batch(tab, dc, opts))
+        log(f"{key} kernel variants (ms): {tune}")
+        for _ in range(args.warmup):
+            step()
+        ms = timed_steps(torch, args.steps, step)
+        st = box["st"]
+        gpu_out[key] = (dc.t["pri_molal"][:n_cpu].cpu().numpy().copy(), dc.t["status"][:n_cpu].cpu().numpy().copy())
+        ms_step = float(np.mean(ms))
+        value = st.converged / (ms_step * 1e-3)
+        log(f"{key}: {ms_step:.2f} ms/step, {value:.4g} cells/s, {st.total_iterations / n:.2f} it/cell")
+        return {"cells_per_s": value, "ms_per_step": ms_step, "kernel_ms": st.kernel_ms,
+                "converged": int(st.converged), "mean_iterations": st.total_iterations / n,
+                "max_iterations": st.max_iterations, "max_rel_res": st.max_rel_res,
+                "histogram": list(st.iteration_histogram), "kernel_variant_ms": tune,
+                "kernel_variant": tab.tune(abi.PFX_KERNEL_REACT)}, st
 
-    # untimed: pick the kernel variant (register cap) by measurement
-    plain_tune = autotune.tune_kernel(tab, dc, lambda: runtime.react_batch(tab, dc, plain_opts))
-    log(f"plain kernel variants (ms): {plain_tune}")
-    for _ in range(args.warmup):
-        plain_step()
-    plain_ms = timed_steps(torch, args.steps, plain_step)
-    st_plain = box["st"]
-    # free ions of the plain method on the cells the CPU baseline will solve
-    n_cpu = min(args.cpu_sample or 12000 * min(os.cpu_count() or 1, 16), n)
-    plain_m = dc.t["pri_molal"][:n_cpu].cpu().numpy().copy()
-    plain_status = dc.t["status"][:n_cpu].cpu().numpy().copy()
-    plain_kernel_ms = st_plain.kernel_ms
-    plain_ms_step = float(np.mean(plain_ms))
-    plain_value = st_plain.converged / (plain_ms_step * 1e-3)
-    log(f"plain: {plain_ms_step:.2f} ms/step, {plain_value:.4g} cells/s, "
-        f"{st_plain.total_iterations / n:.2f} it/cell")
+    # ---- plain GPU Newton from the previous free ions (RReact's start) and
+    # from the constraint default guess
+    plain_prev, st_prev = plain_leg("plain_previous", synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
+    plain, st_plain = plain_leg("plain_default", synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT))
+    plain_value = plain["cells_per_s"]
 
     line = {
         "metric": "reacted cells/s at 1e-10 rel residual: surrogate-warm vs plain GPU Newton",
@@ -141,10 +153,6 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded counter-based RNG, seed 20261202; held-out cells key 2, training cells key 4)",
     }
-    plain = {"cells_per_s": plain_value, "ms_per_step": plain_ms_step, "kernel_ms": plain_kernel_ms,
-             "converged": int(st_plain.converged), "mean_iterations": st_plain.total_iterations / n,
-             "max_iterations": st_plain.max_iterations, "max_rel_res": st_plain.max_rel_res,
-             "histogram": list(st_plain.iteration_histogram), "kernel_variant_ms": plain_tune}
 
     fl = t.flops_per_iteration()
     warm = None
@@ -165,7 +173,8 @@ def main():
         torch.cuda.empty_cache()
         log(f"trained: {tr['steps']} steps in {tr['seconds']:.1f} s, prediction rms {pred['rms_ln']:.3f} ln units")
 
-        warm_opts = synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK, fallback_after=args.fallback_after)
+        warm_opts = synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK, fallback_after=args.fallback_after,
+                                     fallback_previous=1)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         mlp_ms = []
 
@@ -197,8 +206,10 @@ def main():
                 "max_rel_res": st_warm.max_rel_res, "fallbacks": int(st_warm.fallbacks),
                 "fallback_rate": st_warm.fallbacks / n, "fallback_after": args.fallback_after,
                 "histogram": list(st_warm.iteration_histogram),
-                "speedup_vs_plain": warm_value / plain_value,
-                "iteration_reduction": (st_plain.total_iterations / max(st_warm.total_iterations, 1)),
+                "speedup_vs_plain_previous": warm_value / plain_prev["cells_per_s"],
+                "iteration_reduction_vs_plain_previous": st_prev.total_iterations / max(st_warm.total_iterations, 1),
+                "speedup_vs_plain_default": warm_value / plain_value,
+                "iteration_reduction_vs_plain_default": st_plain.total_iterations / max(st_warm.total_iterations, 1),
                 "kernel_variant_ms": warm_tune, "kernel_variant": tab.tune(abi.PFX_KERNEL_REACT),
                 "prediction_error_ln": pred,
                 "training": {"pairs": int(args.train_pairs), "epochs": args.epochs, "batch": args.batch,
@@ -212,15 +223,17 @@ def main():
         value, ms_per_step, st_main, kern_ms = warm_value, warm_ms_step, st_warm, st_warm.kernel_ms
         gpu_m = dc.t["pri_molal"].clone()
         gpu_status = dc.t["status"].clone()
+        gpu_out["warm"] = (gpu_m[:n_cpu].cpu().numpy().copy(), gpu_status[:n_cpu].cpu().numpy().copy())
 
         # both methods again with PFX_JAC_CAPPED_QK (no mineral derivative
         # above the Q/K cap, where the rate is constant): same residual and
         # acceptance test, a consistent Jacobian for the capped cells
         cap = {}
-        for key, opts in () if args.no_capped else (("plain", synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT,
-                                                     jacobian_flags=abi.PFX_JAC_CAPPED_QK)),
+        for key, opts in () if args.no_capped else (("plain_previous", synth.react_opts(
+                                                         guess_mode=abi.PFX_GUESS_PREVIOUS,
+                                                         jacobian_flags=abi.PFX_JAC_CAPPED_QK)),
                           ("warm", synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK,
-                                                    fallback_after=args.fallback_after,
+                                                    fallback_after=args.fallback_after, fallback_previous=1,
                                                     jacobian_flags=abi.PFX_JAC_CAPPED_QK))):
             def step(opts=opts, key=key):
                 restore()
@@ -238,8 +251,9 @@ def main():
             log(f"{key} with PFX_JAC_CAPPED_QK: {ms:.2f} ms/step, {cap[key]['cells_per_s']:.4g} cells/s, "
                 f"{cap[key]['mean_iterations']:.2f} it/cell")
         if cap:
-            cap["warm_speedup_vs_plain"] = cap["warm"]["cells_per_s"] / cap["plain"]["cells_per_s"]
-            cap["warm_speedup_vs_reference_jacobian_plain"] = cap["warm"]["cells_per_s"] / plain_value
+            cap["warm_speedup_vs_plain_previous"] = cap["warm"]["cells_per_s"] / cap["plain_previous"]["cells_per_s"]
+            cap["warm_speedup_vs_reference_jacobian_plain_previous"] = (cap["warm"]["cells_per_s"]
+                                                                        / plain_prev["cells_per_s"])
             both = ((dc.t["status"] == 0) & (gpu_status == 0)).unsqueeze(1)
             rel = ((dc.t["pri_molal"] - gpu_m).abs() / gpu_m.abs()).masked_fill(~both, 0.0)
             cap["max_rel_diff_free_ions_vs_reference_jacobian_warm"] = float(rel.max().item())
@@ -247,13 +261,13 @@ def main():
                 torch.quantile(rel.amax(dim=1)[:200_000].float(), 0.99).item())
             warm["capped_qk"] = cap
     else:
-        value, ms_per_step, st_main, kern_ms = plain_value, plain_ms_step, st_plain, plain_kernel_ms
-        plain_step()
-        gpu_m = dc.t["pri_molal"]
-        gpu_status = dc.t["status"]
+        value, ms_per_step, st_main, kern_ms = (plain_prev["cells_per_s"], plain_prev["ms_per_step"], st_prev,
+                                                plain_prev["kernel_ms"])
 
     # ---- roofline of the dominant kernel (the FP64 Newton): algorithmic
-    # flops of the iterations plus one RTotal + activity pass per cell
+    # flops of the Newton updates plus, per cell, the final evaluation and
+    # the LU solve of its step test (the RTotal + activity pass at the
+    # previous state is not counted)
     iters = st_main.total_iterations
     flops = (iters + n) * (fl["flops"] + fl["transcendentals"])
     achieved = flops / (kern_ms * 1e-3) / 1e12
@@ -275,13 +289,14 @@ def main():
         "ms_per_step": ms_per_step,
         "config": {"workload": f"random_network N=15 M=50 K=3 ionx=1, {n} held-out cells, dt=8640 s, rel_tol 1e-10; "
                                + ("warm start 40-256-256-256-15 BF16 MLP + FP64 Newton with fallback"
-                                  if warm else "plain Newton from the 1e-9 molal default guess"),
-                   "ncells": n, "rel_tol": 1e-10,
-                   "timed_step": ("restore inputs + pfx_surrogate_guess + pfx_react_batch(ARRAY_FALLBACK)"
-                                  if warm else "restore inputs + pfx_react_batch(DEFAULT)")},
+                                  if warm else "plain Newton from the previous free ions"),
+                   "ncells": n, "rel_tol": 1e-10, "rel_change_tol": synth.REL_CHANGE_TOL,
+                   "timed_step": ("restore inputs + pfx_surrogate_guess + pfx_react_batch(ARRAY_FALLBACK, "
+                                  "fallback_previous)" if warm else "restore inputs + pfx_react_batch(PREVIOUS)")},
+        "plain_previous": plain_prev,
         "plain": plain,
         "warm": warm,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": "profiles/pmc_traffic_pfx_react_kernel_1M.json (FETCH_SIZE x2.0, "
                                        "WRITE_SIZE x1.0 from a 256 MiB copy)" if traffic else None,
@@ -313,7 +328,7 @@ def main():
         torch.cuda.synchronize()
         line["copy_GBs"] = 2 * x.numel() * 8 * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(t, dc, snap, gpu_m, gpu_status, plain_m, plain_status, n_cpu)
+        line["cpu_baseline"] = cpu_baseline(t, dc, snap, gpu_out, n_cpu)
     print(json.dumps(line), flush=True)
 
 
@@ -517,41 +532,109 @@ def zones_leg(args, log):
     return out
 
 
-def cpu_baseline(t, dc, snap, gpu_m, gpu_status, plain_m, plain_status, n):
+def cpu_threads():
+    """Host cores of this process's share (affinity mask), at most 16: the
+    CPU share of one GPU on the measurement machine."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(t, dc, snap, gpu_out, n, runs=5):
     """The oracle (C restatement of RReact, OpenMP over cells) on the first
     n cells of the same held-out batch (their step inputs copied to the
-    host), from the reference's default guess, timed on the host cores.  Its
-    free ions are compared with the GPU plain method's (the same algorithm
-    from the same guess: parity) and with the GPU warm-start method's (a
-    different path to the same 1e-10 tolerance)."""
+    host), from the previous free ions as RReact starts, timed on every core
+    of this process's share and on one core (a sixteenth of the sample),
+    median of `runs` runs each.  Its free ions are compared with every GPU
+    method's: the same algorithm from the same start (parity) and the other
+    starting points (the same root to the step test's tolerance).  Cells
+    whose FP64 conditioning floor (eps times the Newton system's amplification
+    of the residual's rounding) exceeds 1e-11 are counted separately: their
+    free ions are not resolved to 1e-10 by FP64 itself."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
     import oracle as orc
-    from pflotran_amd import synth
+    from pflotran_amd import abi, synth
 
-    threads = min(os.cpu_count() or 1, 16)
-    cells = orc.HostCells.allocate(t, n, temperature=True)
-    for k, v in cells.arrays.items():
+    threads = cpu_threads()
+    inputs = orc.HostCells.allocate(t, n, temperature=True)
+    for k, v in inputs.arrays.items():
         src = snap.get(k, dc.t.get(k))
         if v is not None and src is not None:
             v[:] = src[:n].cpu().numpy().reshape(v.shape)
-    t0 = time.perf_counter()
-    rc, st = orc.react_batch(t, cells, synth.react_opts(), nthreads=threads)
-    dt = time.perf_counter() - t0
+    opts = synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS)
+
+    def timed(nc, nthreads):
+        sub = orc.HostCells({k: (None if v is None else v[:nc].copy()) for k, v in inputs.arrays.items()}, nc)
+        ts, out = [], None
+        for _ in range(runs):
+            c = sub.copy()
+            t0 = time.perf_counter()
+            rc, st = orc.react_batch(t, c, opts, nthreads=nthreads)
+            ts.append(time.perf_counter() - t0)
+            out = (c, st)
+        return float(np.median(ts)), out
+
+    dt_all, (cells, st) = timed(n, threads)
+    n1 = max(1, n // 16)
+    dt_one, (_, st1) = timed(n1, 1)
+    # conditioning floor of each cell at the oracle's solution
+    o = synth.react_opts()
+    fixed = o.porosity * o.saturation * 1000.0 * o.volume * inputs["total"] + inputs["total_sorb"] * o.volume
+    _, J = orc.residual_jacobian(t, cells.copy(), o, fixed_accum=np.ascontiguousarray(fixed), nthreads=threads)
+    Jl = J * cells["pri_molal"][:, None, :]
+    floor = np.full(n, np.inf)
+    scale = np.abs(fixed) / o.dt
+    for i0 in range(0, n, 4096):
+        blk = slice(i0, min(n, i0 + 4096))
+        try:
+            inv = np.linalg.inv(Jl[blk])
+            floor[blk] = 2.220446049250313e-16 * np.max(np.abs(inv) @ scale[blk][:, :, None], axis=(1, 2))
+        except np.linalg.LinAlgError:
+            for i in range(blk.start, blk.stop):
+                try:
+                    floor[i] = 2.220446049250313e-16 * np.max(np.abs(np.linalg.inv(Jl[i])) @ scale[i])
+                except np.linalg.LinAlgError:
+                    pass
+    resolved = floor <= 1e-11
+
     def compare(m_gpu, s_gpu):
         ok = (cells["status"] == 0) & (s_gpu == 0)
         if not ok.any():
             return None
-        rel = np.max(np.abs(m_gpu[ok] - cells["pri_molal"][ok]) / cells["pri_molal"][ok], axis=1)
-        return {"max": float(rel.max()), "p99": float(np.quantile(rel, 0.99)), "median": float(np.median(rel)),
-                "cells": int(ok.sum())}
+        rel = np.max(np.abs(m_gpu - cells["pri_molal"]) / cells["pri_molal"], axis=1)
+        rel[~ok] = 0.0
+        return {"max": float(rel.max()), "max_resolved_cells": float(rel[resolved].max()),
+                "p99": float(np.quantile(rel[ok], 0.99)), "median": float(np.median(rel[ok])),
+                "cells": int(ok.sum()), "cells_above_1e-10": int((rel > 1e-10).sum()),
+                "cells_above_1e-10_resolved": int((rel[resolved] > 1e-10).sum()),
+                "max_over_8x_floor": float(np.max(rel / np.maximum(1e-10, 8.0 * floor)))}
 
-    return {"value": st.converged / dt, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} cells of the held-out batch, same tables/tolerance, default guess; "
-                      f"{dt:.2f} s, {st.total_iterations} Newton iterations",
-            "free_ion_rel_err_plain_gpu_vs_oracle": compare(plain_m, plain_status),
-            "free_ion_rel_err_warm_gpu_vs_oracle": compare(gpu_m[:n].cpu().numpy(), gpu_status[:n].cpu().numpy())}
+    out = {"value": st.converged / dt_all, "unit": "cells/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": f"first {n} cells of the held-out batch, same tables and convergence test, from the previous "
+                     f"free ions (RReact's start); median of {runs} runs of {dt_all:.2f} s on {threads} threads, "
+                     f"{st.total_iterations} Newton iterations",
+           "one_core": {"value": st1.converged / dt_one, "cells": n1, "seconds": dt_one},
+           "conditioning_floor": {"cells_resolved_to_1e-11": int(resolved.sum()), "cells": n,
+                                  "median": float(np.median(floor))}}
+    for key, (m, s_) in gpu_out.items():
+        out[f"free_ion_rel_err_{key}_gpu_vs_oracle"] = compare(m, s_)
+    return out
 
 
 if __name__ == "__main__":

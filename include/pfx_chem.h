@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PFX_ABI_VERSION 3
+#define PFX_ABI_VERSION 4
 
 /* largest primary-species count handled by one cell group (row per lane) */
 #define PFX_MAX_PRIMARY 32
@@ -218,6 +218,24 @@ typedef struct pfx_react_opts {
   int32_t fallback_after; /* PFX_GUESS_ARRAY_FALLBACK: iterations of the warm
                              attempt (0 = max_iterations)                     */
   int32_t jacobian_flags; /* PFX_JAC_*; 0 = the reference's Jacobian          */
+  double rel_change_tol;  /* > 0: a cell converges only when, besides
+                             rel_res <= rel_tol, the last Newton update moved
+                             no free ion by more than this relative amount
+                             (max|dm/m|, the quantity of the reference's
+                             relative-change stop, reaction.F90:3473-3476,
+                             tightened); an attempt that has taken no update
+                             yet takes one.  0 = off                         */
+  int32_t fallback_previous; /* PFX_GUESS_ARRAY_FALLBACK: 1 restarts a failed
+                             warm attempt from the previous free ions (RReact's
+                             own starting point, reaction.F90:3370), 0 from
+                             opts->default_guess                             */
+  int32_t stream_ordered; /* 1: pfx_react_batch enqueues its launches on
+                             `stream` and returns without waiting; the stats
+                             argument is ignored and pfx_tables_stats reads
+                             the statistics of the last call later.  No
+                             host synchronisation, no events: such a call
+                             can be captured into a HIP graph.  0: returns
+                             after the stream has drained, stats filled    */
 } pfx_react_opts;
 
 /* pfx_react_opts.jacobian_flags.  PFX_JAC_CAPPED_QK: where ln(Q/K) of a
@@ -277,6 +295,15 @@ int pfx_tables_timing(pfx_tables *tables, double *ms);
    (reactive_transport.F90:2539-2587, 2735-2758).  One persistent launch. */
 int pfx_react_batch(pfx_tables *tables, pfx_cells *cells,
                     const pfx_react_opts *opts, pfx_stats *stats);
+
+/* Statistics of the last pfx_react_batch made with opts->stream_ordered = 1
+   on these tables: waits for that call's stream, fills stats (kernel_ms is 0:
+   a stream-ordered call records no events) and returns the number of cells
+   that did not converge, as pfx_react_batch does.  PFX_E_ARG when no such
+   call is pending (each is read once).  The reference's counterpart is the
+   per-cell num_iterations that RTReact sums after its loop
+   (reactive_transport.F90:1720-1724). */
+int pfx_tables_stats(pfx_tables *tables, pfx_stats *stats);
 
 /* Per-cell residual and analytic Jacobian at the given free ions, without a
    Newton solve: RTAuxVarCompute + RTAccumulation(Derivative) +

@@ -4,12 +4,13 @@
  * No counterpart in the reference: RReact starts its Newton iteration from
  * the previous free-ion molalities (reaction.F90:3370), or from 1e-9 molal
  * after constraint speciation (reaction.F90:1481).  These entry points add
- * a multilayer perceptron that maps a cell's transported totals, sorbed
- * totals, temperature and mineral volume fractions to the log free-ion
- * molalities after the step, used as the initial iterate of
- * pfx_react_batch with PFX_GUESS_ARRAY_FALLBACK.  The FP64 Newton from the
- * default guess stays the fallback, and every returned state passes the
- * same FP64 convergence test.
+ * a multilayer perceptron that maps a cell's log transported totals, its
+ * previous log free ions, temperature and mineral volume fractions to the
+ * change of the log free-ion molalities over the step; the previous free
+ * ions moved by that change are the initial iterate of pfx_react_batch with
+ * PFX_GUESS_ARRAY_FALLBACK.  The FP64 Newton from RReact's own starting
+ * point (opts->fallback_previous) or the default guess stays the fallback,
+ * and every returned state passes the same FP64 convergence test.
  *
  * Matrix products run in BF16 on the matrix cores with FP32 accumulation;
  * master weights, gradients and Adam moments are FP32.  All array arguments
@@ -23,12 +24,16 @@
  * Features of a cell (pfx_surrogate_features, pfx_surrogate_guess), N
  * primaries, K minerals, slots:
  *   [0, N)        slog(total[j])         slog(v) = sign(v) log10(1 + |v| 1e12)
- *   [N, 2N)       slog(total_sorb[j])    (0 without ion exchange)
+ *   [N, 2N)       ln(pri_molal[j])       the previous free ions (molal;
+ *                                        held within [1e-300, 1e300])
  *   2N            temperature (C)        (25 if the array is NULL)
  *   [2N+1, 2N+1+K) mineral volume fraction
  *   rest          0
- * each normalised as (x - in_shift) * in_scale.  Outputs are normalised
- * ln molality: ln m_j = y_j / out_scale_j + out_shift_j.
+ * each normalised as (x - in_shift) * in_scale.  The sorbed totals are not
+ * an input: the previous free ions fix them.  Outputs are the normalised
+ * change of ln molality over the step: the guess is
+ * m_j = pri_molal_j * exp(y_j / out_scale_j + out_shift_j), the change
+ * held within +-30.
  */
 #ifndef PFX_SURROGATE_H
 #define PFX_SURROGATE_H
@@ -71,20 +76,21 @@ int pfx_mlp_train_step(pfx_mlp *mlp, const float *X, const float *T, int32_t B, 
                        void *stream);
 
 /* Normalised features X[ncells][dims0] of a batch of cells (uses total,
-   total_sorb, temperature, mnrl_volfrac). */
+   pri_molal, temperature, mnrl_volfrac). */
 int pfx_surrogate_features(pfx_tables *tables, pfx_mlp *mlp, const pfx_cells *cells, float *X, void *stream);
 
 /* Fused warm start: features, every layer and the guess in one launch,
-   cells->guess[ncells*N] = exp(ln m) in FP64.  Needs dims0 <= 256, hidden
+   cells->guess[ncells*N] = pri_molal * exp(predicted change) in FP64.  Needs dims0 <= 256, hidden
    widths multiples of 64 up to 256, 1 <= outputs <= 16 and N <= outputs
    (else PFX_E_UNSUP; pfx_surrogate_features + pfx_mlp_forward +
    pfx_surrogate_guess_from cover other shapes).  Waits for nothing. */
 int pfx_surrogate_guess(pfx_tables *tables, pfx_mlp *mlp, pfx_cells *cells, void *stream);
 
-/* Guess from network outputs Y[ncells][out]: guess[c*N+j] =
+/* Guess from network outputs Y[ncells][out] and the previous free ions
+   prev[ncells*N]: guess[c*N+j] = prev[c*N+j] *
    exp(Y[c][j] / out_scale_j + out_shift_j). */
-int pfx_surrogate_guess_from(pfx_mlp *mlp, const float *Y, int64_t ncells, int32_t naqcomp, double *guess,
-                             void *stream);
+int pfx_surrogate_guess_from(pfx_mlp *mlp, const float *Y, int64_t ncells, int32_t naqcomp, const double *prev,
+                             double *guess, void *stream);
 
 #ifdef __cplusplus
 }

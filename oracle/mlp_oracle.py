@@ -93,13 +93,13 @@ def slog(v):
     return np.sign(v) * np.log10(1.0 + np.abs(v) * 1e12)
 
 
-def features(total, total_sorb, temperature, volfrac, nfeat, N, K, R):
-    """Raw feature rows (slots of pfx_surrogate.h) of cell arrays."""
+def features(total, pri_molal, temperature, volfrac, nfeat, N, K):
+    """Raw feature rows (slots of pfx_surrogate.h) of cell arrays: slog of
+    the totals, ln of the previous free ions, temperature, volume fractions."""
     n = total.shape[0]
     X = np.zeros((n, nfeat))
     X[:, :N] = slog(total[:, :N])
-    if R > 0 and total_sorb is not None:
-        X[:, N:2 * N] = slog(total_sorb[:, :N])
+    X[:, N:2 * N] = np.log(np.clip(pri_molal[:, :N], 1e-300, 1e300))
     X[:, 2 * N] = 25.0 if temperature is None else temperature
     if K > 0 and volfrac is not None:
         X[:, 2 * N + 1:2 * N + 1 + K] = volfrac[:, :K]

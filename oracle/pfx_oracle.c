@@ -700,8 +700,21 @@ static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
                          int *iters_out, double *relres_out, int *fallbacks_out) {
   const int N = d->naqcomp, M = d->neqcplx, K = d->nkinmnrl, R = d->neqionxrxn;
   ocell c;
-  or_load_cell(d, cl, o, ci, &c);
   const double dt = o->dt;
+  int it = 0, status = PFX_CELL_MAX_ITERATIONS, extra = 0, done = 0;
+  /* learned warm start with fallback: the warm attempt is capped at
+     fallback_after iterations; a failed attempt repeats the step from its
+     start (the plain method) from the previous free ions or the default
+     guess, with the full max_iterations budget (no reference counterpart) */
+  int fallback = o->guess_mode == PFX_GUESS_ARRAY_FALLBACK;
+  int fell_back = 0;
+  int cap = fallback && o->fallback_after > 0 ? o->fallback_after : o->max_iterations;
+  double rr = INFINITY;
+  /* largest |dm/m| of the attempt's last Newton update (none yet: unknown) */
+  double last_change = INFINITY;
+  double Rv[NMAX], J[NMAX][NMAX], sc[NMAX], upd[NMAX];
+restart:
+  or_load_cell(d, cl, o, ci, &c);
 
   /* fixed accumulation from the transported totals and the old sorbed
      state (reaction.F90:3370, 3399-3404) */
@@ -721,18 +734,10 @@ static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
   /* initial iterate */
   const int warm = o->guess_mode == PFX_GUESS_ARRAY || o->guess_mode == PFX_GUESS_ARRAY_FALLBACK;
   for (int j = 0; j < N; j++) {
-    if (warm && cl->guess) c.m[j] = cl->guess[ci * N + j];
+    if (fell_back) c.m[j] = o->fallback_previous ? c.m[j] : o->default_guess;
+    else if (warm && cl->guess) c.m[j] = cl->guess[ci * N + j];
     else if (o->guess_mode == PFX_GUESS_DEFAULT) c.m[j] = o->default_guess;
   }
-  int it = 0, status = PFX_CELL_MAX_ITERATIONS, extra = 0, done = 0;
-  /* learned warm start with fallback: the warm attempt is capped at
-     fallback_after iterations; a failed attempt restarts from the default
-     guess with the full max_iterations budget (no reference counterpart) */
-  int fallback = o->guess_mode == PFX_GUESS_ARRAY_FALLBACK;
-  int cap = fallback && o->fallback_after > 0 ? o->fallback_after : o->max_iterations;
-  double rr = INFINITY;
-  double Rv[NMAX], J[NMAX][NMAX], sc[NMAX], upd[NMAX];
-restart:
   while (!done) {
     if (d->act_coef_update_frequency == PFX_ACT_NEWTON_ITER) {
       /* reaction.F90:3415-3418: the complexes are those of the previous
@@ -742,24 +747,30 @@ restart:
     or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0, o->jacobian_flags);
     rr = or_rel_res(Rv, sc, N);
     if (!isfinite(rr)) { status = PFX_CELL_NONFINITE; break; }
+    /* rel_res test passed and the extra updates taken: converged, or, with
+       a step test, a candidate whose Newton step decides */
+    int candidate = 0;
     if (rr <= o->rel_tol) {
-      if (extra >= o->extra_iterations) { status = PFX_CELL_CONVERGED; break; }
-      extra++;
+      if (extra >= o->extra_iterations) {
+        if (!(o->rel_change_tol > 0.0)) { status = PFX_CELL_CONVERGED; break; }
+        candidate = 1;
+      } else {
+        extra++;
+      }
     }
     if (o->reference_stop) {
       double mx = 0.0;
       for (int j = 0; j < N; j++) mx = fmax(mx, fabs(Rv[j] * dt)); /* reference units: mol */
       if (mx < d->max_residual_tolerance) { status = PFX_CELL_CONVERGED; break; }
     }
-    if (it >= cap) { status = PFX_CELL_MAX_ITERATIONS; break; }
-    it++;
+    if (!candidate && it >= cap) { status = PFX_CELL_MAX_ITERATIONS; break; }
     if (or_rsolve(Rv, J, c.m, upd, N, d->use_log_formulation)) { status = PFX_CELL_SINGULAR; break; }
-    double prev[NMAX], maxrel = 0.0;
+    double prev[NMAX], next[NMAX], maxrel = 0.0;
     for (int j = 0; j < N; j++) prev[j] = c.m[j];
     if (d->use_log_formulation) {
       for (int j = 0; j < N; j++) {
         double u = copysign(1.0, upd[j]) * fmin(fabs(upd[j]), d->max_dlnC);
-        c.m[j] = prev[j] * exp(-u);
+        next[j] = prev[j] * exp(-u);
       }
     } else {
       double min_ratio = 1e20;
@@ -770,9 +781,22 @@ restart:
         }
       if (min_ratio < 1.0)
         for (int j = 0; j < N; j++) upd[j] *= min_ratio * 0.99;
-      for (int j = 0; j < N; j++) c.m[j] = prev[j] - upd[j];
+      for (int j = 0; j < N; j++) next[j] = prev[j] - upd[j];
     }
-    for (int j = 0; j < N; j++) maxrel = fmax(maxrel, fabs((c.m[j] - prev[j]) / prev[j]));
+    for (int j = 0; j < N; j++) maxrel = fmax(maxrel, fabs((next[j] - prev[j]) / prev[j]));
+    if (candidate) {
+      /* the step test: the change this Newton step would make, enlarged by
+         1/(1 - rho) with rho = this step over the last one (at most 10x),
+         bounds the distance to the root of an iteration that contracts
+         only linearly (a Jacobian that is not the residual's derivative);
+         the iterate itself is returned, its residual having passed */
+      const double rho = fmin(isfinite(last_change) && last_change > 0.0 ? maxrel / last_change : 0.0, 0.9);
+      if (maxrel / (1.0 - rho) <= o->rel_change_tol) { status = PFX_CELL_CONVERGED; break; }
+      if (it >= cap) { status = PFX_CELL_MAX_ITERATIONS; break; }
+    }
+    it++;
+    for (int j = 0; j < N; j++) c.m[j] = next[j];
+    last_change = maxrel;
     if (o->reference_stop && maxrel < d->max_relative_change_tolerance) {
       or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0, o->jacobian_flags);
       rr = or_rel_res(Rv, sc, N);
@@ -783,11 +807,10 @@ restart:
   if (c.status) status = c.status;
   if (fallback && status != PFX_CELL_CONVERGED) {
     fallback = 0;
-    c.status = 0;
+    fell_back = 1;
     cap = it + o->max_iterations;
     extra = 0;
-    for (int j = 0; j < N; j++) c.m[j] = o->default_guess;
-    for (int r = 0; r < R; r++) c.ionx_ref[r] = cl->ionx_ref_sorbed ? cl->ionx_ref_sorbed[ci * R + r] : 1e-9;
+    last_change = INFINITY;
     *fallbacks_out = 1;
     goto restart;
   }

@@ -5,7 +5,7 @@ the field offsets against the header compiled with the host compiler.
 """
 import ctypes as C
 
-PFX_ABI_VERSION = 3
+PFX_ABI_VERSION = 4
 PFX_MAX_PRIMARY = 32
 PFX_MAX_CPLX = 256
 PFX_MAX_RXN_SPEC = 8
@@ -73,6 +73,7 @@ class ReactOpts(C.Structure):
         ("default_guess", C.c_double), ("porosity", C.c_double), ("saturation", C.c_double),
         ("volume", C.c_double), ("den_kg", C.c_double), ("stream", C.c_void_p),
         ("fallback_after", C.c_int32), ("jacobian_flags", C.c_int32),
+        ("rel_change_tol", C.c_double), ("fallback_previous", C.c_int32), ("stream_ordered", C.c_int32),
     ]
 
 
@@ -154,7 +155,7 @@ CELL_FIELDS = [f for f, _ in Cells._fields_ if f != "ncells"]
 def default_opts(dt=1.0, rel_tol=1e-10, max_iterations=100, extra_iterations=0,
                  guess_mode=PFX_GUESS_PREVIOUS, reference_stop=0, default_guess=1e-9,
                  porosity=0.25, saturation=1.0, volume=1.0, den_kg=997.0, stream=None, fallback_after=0,
-                 jacobian_flags=0):
+                 jacobian_flags=0, rel_change_tol=0.0, fallback_previous=0, stream_ordered=0):
     o = ReactOpts()
     o.dt = dt
     o.rel_tol = rel_tol
@@ -167,4 +168,7 @@ def default_opts(dt=1.0, rel_tol=1e-10, max_iterations=100, extra_iterations=0,
     o.stream = stream
     o.fallback_after = fallback_after
     o.jacobian_flags = jacobian_flags
+    o.rel_change_tol = rel_change_tol
+    o.fallback_previous = fallback_previous
+    o.stream_ordered = stream_ordered
     return o

@@ -40,6 +40,11 @@ struct pfx_tables {
   uint64_t *d_finish = nullptr;
   int finish_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the last stream-ordered pfx_react_batch, whose statistics
+  // pfx_tables_stats reads (-1: none pending)
+  int64_t pending_n = -1;
+  hipStream_t pending_stream = nullptr;
+  bool pending_fallback = false;
   int device = 0;
   int num_cu = 0;
 };
@@ -201,14 +206,19 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   P.queue = t->d_queue;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, t->d_queue, 2);  // cursor, fallbacks
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(t->ev0, stream));
   void *args[] = {(void *)&P};
+  if (!ms) {
+    // stream-ordered: no events, no wait (the launch may be graph-captured)
+    HIPCHK(hipLaunchKernel(fn, dim3((unsigned)blocks), dim3(BLOCK), args, (size_t)lds, stream));
+    return 0;
+  }
+  HIPCHK(hipEventRecord(t->ev0, stream));
   HIPCHK(hipLaunchKernel(fn, dim3((unsigned)blocks), dim3(BLOCK), args, (size_t)lds, stream));
   HIPCHK(hipEventRecord(t->ev1, stream));
   HIPCHK(hipEventSynchronize(t->ev1));
   float f = 0.f;
   HIPCHK(hipEventElapsedTime(&f, t->ev0, t->ev1));
-  if (ms) *ms = f;
+  *ms = f;
   t->last_ms[0] = f;
   t->last_ms[1] = 0.0;
   return 0;
@@ -251,6 +261,8 @@ void fill_params(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, pfx_k
   P.reference_stop = o->reference_stop;
   P.fallback_after = o->fallback_after > 0 ? o->fallback_after : o->max_iterations;
   P.jac_flags = o->jacobian_flags;
+  P.rel_change_tol = o->rel_change_tol;
+  P.fallback_previous = o->fallback_previous ? 1 : 0;
   P.fallback_count = t->d_queue + 1;
 }
 
@@ -267,6 +279,7 @@ int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o
   if (o->fallback_after < 0) return PFX_E_ARG;
   if (o->jacobian_flags & ~PFX_JAC_CAPPED_QK) return PFX_E_ARG;
   if (!(o->dt > 0.0)) return PFX_E_ARG;
+  if (!(o->rel_change_tol >= 0.0)) return PFX_E_ARG;
   return 0;
 }
 
@@ -501,7 +514,8 @@ int pfx_internal::gi_evaluate_cells(pfx_tables *t, pfx_cells *cells, const pfx_r
   P.tot_eval_out = tot;
   P.dtot_out = dtot;
   P.col_scale = col_scale ? 1 : 0;
-  return launch(t, P, PFX_MODE_EVAL, (hipStream_t)o->stream, nullptr);
+  double ms = 0.0;
+  return launch(t, P, PFX_MODE_EVAL, (hipStream_t)o->stream, o->stream_ordered ? nullptr : &ms);
 }
 
 int pfx_internal::timed_launch_begin(pfx_tables *t, hipStream_t s) {
@@ -796,9 +810,33 @@ extern "C" int pfx_react_batch(pfx_tables *t, pfx_cells *cells, const pfx_react_
   pfx_kernel_params P;
   fill_params(t, cells, o, P);
   if ((rc = use_scratch(t, P, cells->ncells))) return rc;
+  if (o->stream_ordered) {
+    // enqueue the step and its statistics reduction; pfx_tables_stats
+    // reads them later
+    if ((rc = launch(t, P, PFX_MODE_REACT, stream, nullptr))) return rc;
+    if ((rc = stats_launch(t, P, cells->ncells, stream))) return rc;
+    t->pending_n = cells->ncells;
+    t->pending_stream = stream;
+    t->pending_fallback = P.guess_mode == PFX_GUESS_ARRAY_FALLBACK;
+    return 0;
+  }
   double ms = 0.0;
   if ((rc = launch(t, P, PFX_MODE_REACT, stream, &ms))) return rc;
+  t->pending_n = -1;
   return finish_stats(t, P, cells->ncells, stream, ms, stats);
+}
+
+extern "C" int pfx_tables_stats(pfx_tables *t, pfx_stats *stats) {
+  if (!t || !stats) return PFX_E_ARG;
+  if (t->pending_n < 0) return PFX_E_ARG;  // no stream-ordered step since the last read
+  HIPCHK(hipSetDevice(t->device));
+  pfx_kernel_params P;
+  memset(&P, 0, sizeof(P));
+  P.guess_mode = t->pending_fallback ? PFX_GUESS_ARRAY_FALLBACK : PFX_GUESS_DEFAULT;
+  const int64_t n = t->pending_n;
+  t->pending_n = -1;
+  int rc = stats_read(t, P, n, t->pending_stream, 0.0, stats);
+  return rc;
 }
 
 extern "C" int pfx_residual_jacobian(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, double *residual,

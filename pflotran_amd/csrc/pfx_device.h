@@ -83,7 +83,10 @@ struct pfx_kernel_params {
   int32_t max_iterations, extra_iterations, guess_mode, reference_stop;
   int32_t fallback_after;       // warm-attempt iteration cap (PFX_GUESS_ARRAY_FALLBACK)
   int32_t jac_flags;            // PFX_JAC_*
-  unsigned int *fallback_count; // cells restarted from the default guess
+  double rel_change_tol;        // step test at the accepted iterate (0 = off)
+  int32_t fallback_previous;    // a failed warm attempt restarts from the previous free ions
+  int32_t reserved0;
+  unsigned int *fallback_count; // cells restarted by the fallback
   double *res_out, *jac_out;    // residual/Jacobian outputs
   double *dtot_out;             // EVAL: dT/dm blocks [ncells][N][N] column-major (kg/L), may be NULL
   double *tot_eval_out;         // EVAL: totals at the free ions [ncells][N] (mol/L), may be NULL

@@ -85,11 +85,14 @@ __device__ __forceinline__ float slog(double v) {
   return v < 0.0 ? -s : s;
 }
 
+// ln of a previous free ion, held finite for a zero or broken entry
+__device__ __forceinline__ double ln_prev(double m) { return log(fmin(fmax(m, 1e-300), 1e300)); }
+
 // raw feature f of cell c (slots documented in pfx_surrogate.h)
-__device__ __forceinline__ float feature_raw(int f, int64_t c, int N, int K, int R, const double *total,
-                                             const double *sorb, const double *temp, const double *vf) {
+__device__ __forceinline__ float feature_raw(int f, int64_t c, int N, int K, const double *total,
+                                             const double *prev, const double *temp, const double *vf) {
   if (f < N) return slog(total[c * N + f]);
-  if (f < 2 * N) return (R > 0 && sorb) ? slog(sorb[c * N + (f - N)]) : 0.0f;
+  if (f < 2 * N) return (float)ln_prev(prev[c * N + (f - N)]);
   if (f == 2 * N) return temp ? (float)temp[c] : 25.0f;
   if (f < 2 * N + 1 + K) return vf ? (float)vf[c * K + (f - 2 * N - 1)] : 0.0f;
   return 0.0f;
@@ -261,28 +264,30 @@ __global__ void k_swizzle(const float *W, int N, int K, int Np, int Kp, uint4 *o
   out[i] = u;
 }
 
-__global__ void k_features(int64_t ncells, int N, int K, int R, const double *total, const double *sorb,
+__global__ void k_features(int64_t ncells, int N, int K, const double *total, const double *prev,
                            const double *temp, const double *vf, int nfeat, const float *shift, const float *scale,
                            float *X) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= ncells * nfeat) return;
   const int64_t c = i / nfeat;
   const int f = (int)(i % nfeat);
-  X[i] = (feature_raw(f, c, N, K, R, total, sorb, temp, vf) - shift[f]) * scale[f];
+  X[i] = (feature_raw(f, c, N, K, total, prev, temp, vf) - shift[f]) * scale[f];
 }
 
-__device__ __forceinline__ double guess_of(float y, int j, const float *shift, const float *scale) {
-  const double lnm = (double)y / (double)scale[j] + (double)shift[j];
-  return exp(fmin(fmax(lnm, -700.0), 30.0));
+// guess = previous free ion times exp(predicted change of ln m over the
+// step); the change is held within +-30 (a factor of 1e13)
+__device__ __forceinline__ double guess_of(float y, int j, const float *shift, const float *scale, double prev) {
+  const double dlnm = (double)y / (double)scale[j] + (double)shift[j];
+  return exp(fmin(fmax(ln_prev(prev) + fmin(fmax(dlnm, -30.0), 30.0), -700.0), 700.0));
 }
 
-__global__ void k_guess(const float *Y, int nout, const float *shift, const float *scale, double *guess, int64_t n,
-                        int N) {
+__global__ void k_guess(const float *Y, int nout, const float *shift, const float *scale, const double *prev,
+                        double *guess, int64_t n, int N) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n * N) return;
   const long c = i / N;
   const int j = (int)(i % N);
-  guess[i] = guess_of(Y[c * nout + j], j, shift, scale);
+  guess[i] = guess_of(Y[c * nout + j], j, shift, scale, prev[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -299,9 +304,9 @@ struct mlp_dev {
 constexpr int SC = 64;      // cells per block
 constexpr int ASTR = 264;   // LDS activation row stride (BF16 elements, 528 B)
 
-__global__ __launch_bounds__(256) void k_surrogate_guess(const mlp_dev D, int64_t ncells, int N, int K, int R,
+__global__ __launch_bounds__(256) void k_surrogate_guess(const mlp_dev D, int64_t ncells, int N, int K,
                                                          const double *__restrict__ total,
-                                                         const double *__restrict__ sorb,
+                                                         const double *__restrict__ prev,
                                                          const double *__restrict__ temp,
                                                          const double *__restrict__ vf, double *__restrict__ guess) {
   __shared__ __attribute__((aligned(16))) unsigned short act[2][SC * ASTR];
@@ -315,7 +320,7 @@ __global__ __launch_bounds__(256) void k_surrogate_guess(const mlp_dev D, int64_
     for (int f = part * per; f < (part + 1) * per; f++) {
       float x = 0.0f;
       if (c < ncells && f < nf)
-        x = (feature_raw(f, c, N, K, R, total, sorb, temp, vf) - D.in_shift[f]) * D.in_scale[f];
+        x = (feature_raw(f, c, N, K, total, prev, temp, vf) - D.in_shift[f]) * D.in_scale[f];
       act[0][cl * ASTR + f] = f2bf(x);
     }
   }
@@ -388,7 +393,8 @@ __global__ __launch_bounds__(256) void k_surrogate_guess(const mlp_dev D, int64_
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int j = rq + q;
-        if (j < N && j < nout) guess[c * N + j] = guess_of(acc[q] + D.b[l][j], j, D.out_shift, D.out_scale);
+        if (j < N && j < nout)
+          guess[c * N + j] = guess_of(acc[q] + D.b[l][j], j, D.out_shift, D.out_scale, prev[c * N + j]);
       }
     }
   }
@@ -684,10 +690,10 @@ extern "C" int pfx_surrogate_features(pfx_tables *tables, pfx_mlp *m, const pfx_
   pfx_internal::table_sizes(tables, &N, &M, &K, &R);
   if (2 * N + 1 + K > m->dims[0]) return PFX_E_ARG;
   if (cells->ncells == 0) return 0;
-  if (!X || !cells->total || (R > 0 && !cells->total_sorb)) return PFX_E_ARG;
+  if (!X || !cells->total || !cells->pri_molal) return PFX_E_ARG;
   const long n = cells->ncells * (long)m->dims[0];
   hipLaunchKernelGGL(k_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                     cells->ncells, N, K, R, cells->total, cells->total_sorb, cells->temperature,
+                     cells->ncells, N, K, cells->total, cells->pri_molal, cells->temperature,
                      cells->mnrl_volfrac, m->dims[0], m->in_shift, m->in_scale, X);
   MCHK(hipGetLastError());
   return 0;
@@ -700,7 +706,7 @@ extern "C" int pfx_surrogate_guess(pfx_tables *tables, pfx_mlp *m, pfx_cells *ce
   if (2 * N + 1 + K > m->dims[0]) return PFX_E_ARG;
   if (!fused_shape_ok(m, N)) return PFX_E_UNSUP;
   if (cells->ncells == 0) return 0;
-  if (!cells->guess || !cells->total || (R > 0 && !cells->total_sorb)) return PFX_E_ARG;
+  if (!cells->guess || !cells->total || !cells->pri_molal) return PFX_E_ARG;
   hipStream_t s = (hipStream_t)stream;
   int rc = refresh_swizzled(m, s);
   if (rc) return rc;
@@ -718,20 +724,20 @@ extern "C" int pfx_surrogate_guess(pfx_tables *tables, pfx_mlp *m, pfx_cells *ce
   D.out_shift = m->out_shift;
   D.out_scale = m->out_scale;
   const unsigned blocks = (unsigned)((cells->ncells + SC - 1) / SC);
-  hipLaunchKernelGGL(k_surrogate_guess, dim3(blocks), dim3(256), 0, s, D, cells->ncells, N, K, R, cells->total,
-                     cells->total_sorb, cells->temperature, cells->mnrl_volfrac, (double *)cells->guess);
+  hipLaunchKernelGGL(k_surrogate_guess, dim3(blocks), dim3(256), 0, s, D, cells->ncells, N, K, cells->total,
+                     cells->pri_molal, cells->temperature, cells->mnrl_volfrac, (double *)cells->guess);
   MCHK(hipGetLastError());
   return 0;
 }
 
-extern "C" int pfx_surrogate_guess_from(pfx_mlp *m, const float *Y, int64_t ncells, int32_t naqcomp, double *guess,
-                                        void *stream) {
+extern "C" int pfx_surrogate_guess_from(pfx_mlp *m, const float *Y, int64_t ncells, int32_t naqcomp,
+                                        const double *prev, double *guess, void *stream) {
   if (!m || ncells < 0 || naqcomp < 1 || naqcomp > m->dims[m->nlayers]) return PFX_E_ARG;
   if (ncells == 0) return 0;
-  if (!Y || !guess) return PFX_E_ARG;
+  if (!Y || !prev || !guess) return PFX_E_ARG;
   const long n = ncells * naqcomp;
   hipLaunchKernelGGL(k_guess, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, Y,
-                     m->dims[m->nlayers], m->out_shift, m->out_scale, guess, ncells, naqcomp);
+                     m->dims[m->nlayers], m->out_shift, m->out_scale, prev, guess, ncells, naqcomp);
   MCHK(hipGetLastError());
   return 0;
 }

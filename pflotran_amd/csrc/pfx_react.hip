@@ -384,6 +384,8 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   int it_cap = 0;           // iteration at which the current attempt gives up
   bool fell_back = false;   // warm attempt abandoned, restarted from the default guess
   bool stop_next = false;   // reference relative-change stop: finish at the next evaluation
+  bool reload = false;      // load the current cell again (fallback restart)
+  double last_change = INFINITY;  // max|dm/m| of the attempt's last Newton update
   int cell_flag = PFX_CELL_CONVERGED;
   double m = 1.0, gam = 1.0, lng = 0.0;
   // constraint speciation state
@@ -405,6 +407,8 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     double tot_out = 0.0, sorb_out = 0.0;  // totals and sorbed totals of this pass
     // ---------------------------------------------------------------
     // refill: groups without a cell take the next ones from the queue
+    bool load_now = reload;
+    reload = false;
     {
       const bool need = (cell < 0) && !exhausted;
       unsigned long long mask = __ballot(need && r == 0);
@@ -438,13 +442,25 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         long long c = ((long long)hi << 32) | (unsigned int)lo;
         if (c >= 0) {
           cell = c;
-          phase = (MODE == PFX_MODE_REACT) ? 0 : 2;
           it = 0;
-          extra = 0;
           fell_back = false;
-          it_cap = (MODE == PFX_MODE_REACT && P.guess_mode == PFX_GUESS_ARRAY_FALLBACK) ? P.fallback_after
-                                                                                     : P.max_iterations;
+          load_now = true;
+        }
+      }
+      // ---- load the cell: a new one, or the same one again when its warm
+      // attempt failed (the restart then repeats the plain method from the
+      // step's start: activity coefficients, complexes and exchange state
+      // are read again from the unchanged inputs)
+      if (load_now) {
+        {
+          const long long c = cell;
+          phase = (MODE == PFX_MODE_REACT) ? 0 : 2;
+          extra = 0;
+          it_cap = (MODE == PFX_MODE_REACT && P.guess_mode == PFX_GUESS_ARRAY_FALLBACK && !fell_back)
+                       ? P.fallback_after
+                       : it + P.max_iterations;
           stop_next = false;
+          last_change = INFINITY;
           cell_flag = PFX_CELL_CONVERGED;
           compute_act = false;
           nit_on = 0;
@@ -473,9 +489,12 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           }
           double mguess = 1.0;
           if (MODE == PFX_MODE_REACT)
-            mguess = ld_or((P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK) ? P.guess
-                                                                                                          : nullptr,
+            mguess = ld_or((P.guess_mode == PFX_GUESS_ARRAY || P.guess_mode == PFX_GUESS_ARRAY_FALLBACK) && !fell_back
+                               ? P.guess
+                               : nullptr,
                            ci, dmy);
+          // the fallback's starting point: the previous free ions or the default guess
+          if (MODE == PFX_MODE_REACT && fell_back) mguess = P.fallback_previous ? v_m : P.default_guess;
           // the warm-start guess waits in the group's exchange slot (unused
           // until the first ion-exchange pass) instead of a register
           SD(S.xb, r) = mguess;
@@ -892,6 +911,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     // residual row, Jacobian row, convergence test
     double Rr = 0.0, scale = 1.0;
     bool finish = false;
+    bool candidate = false;  // rel_res passed; the step test below decides
     int status = PFX_CELL_CONVERGED;
     double rr = 0.0;
     int use_log = L.use_log;
@@ -966,14 +986,19 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         finish = true;
       } else {
         if (rr <= P.rel_tol) {
-          if (extra >= P.extra_iterations) finish = true;
-          else extra++;
+          if (extra >= P.extra_iterations) {
+            // with a step test the Newton step of this iterate decides
+            if (MODE == PFX_MODE_REACT && P.rel_change_tol > 0.0) candidate = true;
+            else finish = true;
+          } else {
+            extra++;
+          }
         }
         if (!finish && P.reference_stop) {
           const double mx = gmax<G>(live ? fabs(Rr * dt) : 0.0, lane);
           if (mx < L.max_residual_tol) finish = true;
         }
-        if (!finish && it >= it_cap) {
+        if (!finish && !candidate && it >= it_cap) {
           finish = true;
           status = PFX_CELL_MAX_ITERATIONS;
         }
@@ -1131,20 +1156,39 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     // ---------------------------------------------------------------
     PFX_PT(7);
     // Newton update (reaction.F90:3454-3470; constraint form :1806-1826)
+    double mnew = m, rel = 0.0;
     if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
       const double mprev = m;
       if (use_log) {
         const double u = copysign(1.0, upd) * fmin(fabs(upd), L.max_dlnC);
-        if (live) m = m * exp(-u);
+        if (live) mnew = m * exp(-u);
       } else {
         double ratio = (live && m <= upd) ? fabs(m / upd) : 1e20;
         const double min_ratio = gmin<G>(ratio, lane);
         if (MODE == PFX_MODE_EQUIL ? (min_ratio <= 1.0) : (min_ratio < 1.0)) upd = upd * min_ratio * 0.99;
-        if (live) m = m - upd;
+        if (live) mnew = m - upd;
       }
-      const double rel = gmax<G>(live ? fabs((m - mprev) / mprev) : 0.0, lane);
+      rel = gmax<G>(live ? fabs((mnew - mprev) / mprev) : 0.0, lane);
+      if (MODE == PFX_MODE_REACT && candidate) {
+        // step test: the change this step would make, enlarged by
+        // 1/(1 - rho), rho = this step over the last one (at most 10x),
+        // bounds the distance to the root also where the iteration
+        // contracts only linearly; the iterate itself is returned (its
+        // residual passed), the step is not taken
+        const double rho = fmin(last_change > 0.0 && last_change < INFINITY ? rel / last_change : 0.0, 0.9);
+        if (rel / (1.0 - rho) <= P.rel_change_tol) {
+          finish = true;
+        } else if (it >= it_cap) {
+          finish = true;
+          status = PFX_CELL_MAX_ITERATIONS;
+        }
+      }
+    }
+    if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
+      m = mnew;
       it++;
       if (MODE == PFX_MODE_REACT) {
+        last_change = rel;
         if (P.reference_stop && rel < L.max_rel_change_tol) stop_next = true;
       } else {
         // constraint convergence (reaction.F90:1952-1966)
@@ -1170,20 +1214,15 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     }
 
     // ---------------------------------------------------------------
-    // warm start abandoned: restart from the default guess with the
-    // activity coefficients of the step's start and the full budget
+    // warm start abandoned: the cell is loaded again at the next pass and
+    // repeats the plain method from the step's start (previous state,
+    // activity update, then Newton from the fallback guess) with the full
+    // iteration budget
     if (MODE == PFX_MODE_REACT && active && finish && status != PFX_CELL_CONVERGED &&
         P.guess_mode == PFX_GUESS_ARRAY_FALLBACK && !fell_back) {
       fell_back = true;
       finish = false;
-      it_cap = it + P.max_iterations;
-      extra = 0;
-      stop_next = false;
-      cell_flag = PFX_CELL_CONVERGED;
-      if (live) m = P.default_guess;
-#pragma unroll
-      for (int q = 0; q < PFX_MAX_IONX_RXN; q++)
-        SD(S.ionx, q) = (q < R && P.ionx_ref) ? P.ionx_ref[cell * R + q] : 1e-9;
+      reload = true;
       if (r == 0) atomicAdd(P.fallback_count, 1u);
     }
 

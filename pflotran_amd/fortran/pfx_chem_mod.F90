@@ -66,6 +66,8 @@ module pfx_chem_module
     real(c_double) :: porosity, saturation, volume, den_kg
     type(c_ptr) :: stream
     integer(c_int32_t) :: fallback_after, jacobian_flags
+    real(c_double) :: rel_change_tol
+    integer(c_int32_t) :: fallback_previous, stream_ordered
   end type pfx_react_opts
 
   ! the reaction of a mineral or gas constraint row
@@ -127,7 +129,7 @@ module pfx_chem_module
             pfx_os_transport_step
 
   public :: pfx_abi_version, pfx_tables_create, pfx_tables_destroy, &
-            pfx_react_batch, pfx_residual_jacobian, pfx_equilibrate_batch, &
+            pfx_react_batch, pfx_tables_stats, pfx_residual_jacobian, pfx_equilibrate_batch, &
             pfx_auxvar_batch, pfx_update_kinetic_state, pfx_tables_timing, &
             pfx_gi_structure, pfx_gi_residual_jacobian, pfx_gi_dispersion, &
             pfx_react_zones, pfx_zone_plan_rebalance, pfx_zones_graph_create, &
@@ -165,6 +167,13 @@ module pfx_chem_module
       type(pfx_stats), intent(out) :: stats
       integer(c_int) :: pfx_react_batch
     end function pfx_react_batch
+
+    function pfx_tables_stats(tables, stats) bind(c, name='pfx_tables_stats')
+      import :: c_int, c_ptr, pfx_stats
+      type(c_ptr), value :: tables
+      type(pfx_stats), intent(out) :: stats
+      integer(c_int) :: pfx_tables_stats
+    end function pfx_tables_stats
 
     function pfx_residual_jacobian(tables, cells, opts, residual, jacobian) &
         bind(c, name='pfx_residual_jacobian')
@@ -450,10 +459,10 @@ module pfx_chem_module
       integer(c_int) :: pfx_surrogate_guess
     end function pfx_surrogate_guess
 
-    function pfx_surrogate_guess_from(mlp, y, ncells, naqcomp, guess, stream) &
+    function pfx_surrogate_guess_from(mlp, y, ncells, naqcomp, prev, guess, stream) &
         bind(c, name='pfx_surrogate_guess_from')
       import :: c_int, c_int32_t, c_int64_t, c_ptr
-      type(c_ptr), value :: mlp, y, guess, stream
+      type(c_ptr), value :: mlp, y, prev, guess, stream
       integer(c_int64_t), value :: ncells
       integer(c_int32_t), value :: naqcomp
       integer(c_int) :: pfx_surrogate_guess_from

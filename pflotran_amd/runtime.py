@@ -19,7 +19,7 @@ from .database import ReactionTables
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpfx_chem.so")
 
-EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_react_batch",
+EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_react_batch", "pfx_tables_stats",
             "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch", "pfx_update_kinetic_state",
             "pfx_gi_structure", "pfx_gi_residual_jacobian", "pfx_tables_timing", "pfx_gi_dispersion",
             "pfx_react_zones", "pfx_zone_plan_rebalance", "pfx_zones_graph_create", "pfx_graph_launch",
@@ -56,6 +56,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_tables_create.argtypes = [P(abi.ReactionDesc), P(C.c_void_p)]
     lib.pfx_tables_destroy.argtypes = [C.c_void_p]
     lib.pfx_react_batch.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), P(abi.Stats)]
+    lib.pfx_tables_stats.argtypes = [C.c_void_p, P(abi.Stats)]
     lib.pfx_residual_jacobian.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), C.c_void_p, C.c_void_p]
     lib.pfx_equilibrate_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                           P(abi.ReactOpts), P(abi.Stats)]
@@ -86,7 +87,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_mlp_train_step.argtypes = [vp, vp, vp, i32, f32, vp, vp]
     lib.pfx_surrogate_features.argtypes = [vp, vp, P(abi.Cells), vp, vp]
     lib.pfx_surrogate_guess.argtypes = [vp, vp, P(abi.Cells), vp]
-    lib.pfx_surrogate_guess_from.argtypes = [vp, vp, i64, i32, vp, vp]
+    lib.pfx_surrogate_guess_from.argtypes = [vp, vp, i64, i32, vp, vp, vp]
     lib.pfx_bsr_create.argtypes = [i64, i32, i64, vp, vp, P(vp)]
     lib.pfx_bsr_destroy.argtypes = [vp]
     lib.pfx_bsr_spmv.argtypes = [vp, vp, vp, vp, vp]

@@ -1,13 +1,14 @@
 """Host side of the learned warm start (include/pfx_surrogate.h).
 
 A multilayer perceptron (40 -> 256 -> 256 -> 256 -> 15, GELU) maps a cell's
-transported totals, sorbed totals, temperature and mineral volume fractions to
-the log free-ion molalities after the reaction step.  Its training pairs come
-from the GPU Newton solver itself (speciated synthetic cells, one converged
-reaction step each); training and inference run in the BF16 matrix-core
-kernels of libpfx_chem.so.  The prediction only seeds the FP64 Newton
-(PFX_GUESS_ARRAY_FALLBACK): a cell that does not converge from it restarts
-from the default guess, so every returned state meets the same convergence
+log transported totals, its previous log free ions, temperature and mineral
+volume fractions to the change of the log free-ion molalities over the
+reaction step.  Its training pairs come from the GPU Newton solver itself
+(speciated synthetic cells, one converged reaction step each); training and
+inference run in the BF16 matrix-core kernels of libpfx_chem.so.  The
+prediction only seeds the FP64 Newton (PFX_GUESS_ARRAY_FALLBACK): a cell that
+does not converge from it repeats the step as the plain method does, from
+the previous free ions, so every returned state meets the same convergence
 test as the plain method.
 
 PyTorch holds the device buffers and draws minibatch permutations; every
@@ -154,18 +155,19 @@ def guess(tab: runtime.Tables, mlp: MLP, dc: runtime.DeviceCells):
         raise runtime.PfxError(f"pfx_surrogate_guess failed: {rc}")
 
 
-def guess_from(mlp: MLP, Y, ncells: int, naqcomp: int, out):
-    rc = runtime.load_library().pfx_surrogate_guess_from(mlp.handle, Y.data_ptr(), ncells, naqcomp, out.data_ptr(),
-                                                          _stream())
+def guess_from(mlp: MLP, Y, ncells: int, naqcomp: int, prev, out):
+    rc = runtime.load_library().pfx_surrogate_guess_from(mlp.handle, Y.data_ptr(), ncells, naqcomp, prev.data_ptr(),
+                                                          out.data_ptr(), _stream())
     if rc != 0:
         raise runtime.PfxError(f"pfx_surrogate_guess_from failed: {rc}")
 
 
 def generate_pairs(t: ReactionTables, tab: runtime.Tables, mlp: MLP, n_pairs: int, chunk: int = 1_000_000,
                    seed: int = synth.SEED, workload_id: int = TRAIN_WORKLOAD, log=None):
-    """(raw features, converged ln m) of n_pairs synthetic cells, produced by
-    the GPU solver: speciation of the previous state, then one reaction step
-    from the previous free ions.  Raw = un-normalised features."""
+    """(raw features, change of ln m over the step) of n_pairs synthetic
+    cells, produced by the GPU solver: speciation of the previous state,
+    then one reaction step from the previous free ions.  Raw = un-normalised
+    features."""
     import torch
     norm = mlp.get_norm()
     ni, no = mlp.dims[0], mlp.dims[-1]
@@ -179,11 +181,14 @@ def generate_pairs(t: ReactionTables, tab: runtime.Tables, mlp: MLP, n_pairs: in
             nc = min(chunk, n_pairs - have + 1024)
             dc, nfail = workload.speciated_cells(t, tab, nc, seed=seed + 7919 * k, workload_id=workload_id)
             X = features(tab, mlp, dc)
+            prev = dc.t["pri_molal"].clone()
             rc, st = runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
-            ok = (dc.t["status"] == 0) & (dc.t["pri_molal"] > 0).all(dim=1)
+            ok = (dc.t["status"] == 0) & (dc.t["pri_molal"] > 0).all(dim=1) & (prev > 0).all(dim=1)
             idx = torch.nonzero(ok).flatten()
             Xs.append(X.index_select(0, idx))
-            Ys.append(torch.log(dc.t["pri_molal"].index_select(0, idx)[:, :no]).float())
+            d = torch.log(dc.t["pri_molal"].index_select(0, idx)) - torch.log(prev.index_select(0, idx))
+            Ys.append(d[:, :no].float())
+            del prev
             have += int(idx.numel())
             stats["cells"] += nc
             stats["failed_speciation"] += nfail
@@ -261,7 +266,9 @@ def train(mlp: MLP, Xn, Yn, epochs: float = 3.0, batch: int = 4096, lr: float = 
 
 
 def prediction_error(mlp: MLP, Xn, Y_lnm):
-    """RMS and max |ln m_pred - ln m| over a set of pairs (un-normalised)."""
+    """RMS and max |ln m_pred - ln m| over a set of pairs, from the
+    un-normalised targets (the change of ln m over the step: the same
+    difference)."""
     import torch
     norm = mlp.get_norm()
     Yp = mlp.forward(Xn)

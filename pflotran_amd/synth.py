@@ -163,10 +163,16 @@ DEN_25C = water_density_ifc67(25.0, 101325.0)
 STEP_DT = 8640.0   # 0.1 d
 
 
+# step test of the reaction step's convergence: besides rel_res <= 1e-10,
+# the Newton step at the accepted iterate, enlarged by the observed
+# contraction, stays below 1e-11 relative (free ions to ~1e-11)
+REL_CHANGE_TOL = 1e-11
+
+
 def react_opts(**kw) -> abi.ReactOpts:
     base = dict(dt=STEP_DT, rel_tol=1e-10, max_iterations=150, extra_iterations=0,
                 guess_mode=abi.PFX_GUESS_DEFAULT, default_guess=1e-9, porosity=0.25, saturation=1.0,
-                volume=1.0, den_kg=DEN_25C)
+                volume=1.0, den_kg=DEN_25C, rel_change_tol=REL_CHANGE_TOL)
     base.update(kw)
     return abi.default_opts(**base)
 

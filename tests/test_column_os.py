@@ -34,7 +34,7 @@ def initial_state(t, n):
 
 
 def react_opts(dt):
-    return synth.react_opts(dt=dt, guess_mode=abi.PFX_GUESS_PREVIOUS, extra_iterations=1, max_iterations=100)
+    return synth.react_opts(dt=dt, guess_mode=abi.PFX_GUESS_PREVIOUS, max_iterations=100)
 
 
 def oracle_run(t, cells, inflow, col, nsteps):

@@ -34,11 +34,11 @@ def test_full_group_of_32_primaries():
     assert n > 100  # about three in four draws of this network speciate
     cells = orc.HostCells({k: (None if v is None else v[idx].copy()) for k, v in drawn.arrays.items()}, n)
     ref = cells.copy()
-    rc_o, _ = orc.react_batch(t, ref, synth.react_opts(extra_iterations=1))
+    rc_o, _ = orc.react_batch(t, ref, synth.react_opts())
     tab = runtime.Tables(t)
     dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
     print("N = 32 react", flush=True)
-    rc_g, st = runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1))
+    rc_g, st = runtime.react_batch(tab, dc, synth.react_opts())
     out = dc.to_host()
     assert np.array_equal(out["status"], ref["status"])
     conv = ref["status"] == 0

@@ -109,12 +109,12 @@ def test_react_parity_default_guess(net15):
     n = 512
     cells, _, _, ok = speciated_state(t, n)
     assert ok.all()
-    o = synth.react_opts(extra_iterations=1)
+    o = synth.react_opts()
     ref = cells.copy()
     rc_o, st_o = orc.react_batch(t, ref, o)
     tab = runtime.Tables(t)
     dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
-    rc_g, st_g = runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1))
+    rc_g, st_g = runtime.react_batch(tab, dc, synth.react_opts())
     out = dc.to_host()
     assert rc_g == rc_o == 0, (st_g.as_dict(), st_o.as_dict())
     assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
@@ -133,7 +133,7 @@ def test_react_parity_capped_mineral_derivative(net15):
     t = net15
     n = 512
     cells, _, _, _ = speciated_state(t, n)
-    oc = dict(extra_iterations=1, jacobian_flags=abi.PFX_JAC_CAPPED_QK)
+    oc = dict(jacobian_flags=abi.PFX_JAC_CAPPED_QK)
     ref = cells.copy()
     orc.react_batch(t, ref, synth.react_opts(**oc))
     tab = runtime.Tables(t)
@@ -144,7 +144,7 @@ def test_react_parity_capped_mineral_derivative(net15):
     assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
     assert np.mean(np.abs(out["num_iterations"] - ref["num_iterations"])) < 0.5
     dr = runtime.DeviceCells.from_host(host_arrays(cells), n)
-    runtime.react_batch(tab, dr, synth.react_opts(extra_iterations=1))
+    runtime.react_batch(tab, dr, synth.react_opts())
     plain = dr.to_host()
     assert rel_err(out["pri_molal"], plain["pri_molal"]) <= 1e-9
     assert out["num_iterations"].mean() < 0.8 * plain["num_iterations"].mean()
@@ -154,12 +154,12 @@ def test_react_parity_capped_mineral_derivative(net15):
 def test_react_ragged_and_previous_guess(net15, n):
     t = net15
     cells, _, _, ok = speciated_state(t, n, seed=7)
-    o = synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS, extra_iterations=1)
+    o = synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS)
     ref = cells.copy()
     orc.react_batch(t, ref, o)
     tab = runtime.Tables(t)
     dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
-    runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS, extra_iterations=1))
+    runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
     out = dc.to_host()
     assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
 
@@ -204,10 +204,10 @@ def test_react_parity_activity_algorithms(alg, freq):
     cells, _, _, _ = speciated_state(t, n, seed=23)
     guess = abi.PFX_GUESS_PREVIOUS if freq == abi.PFX_ACT_NEWTON_ITER else abi.PFX_GUESS_DEFAULT
     ref = cells.copy()
-    orc.react_batch(t, ref, synth.react_opts(extra_iterations=1, guess_mode=guess))
+    orc.react_batch(t, ref, synth.react_opts(guess_mode=guess))
     tab = runtime.Tables(t)
     dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
-    runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1, guess_mode=guess))
+    runtime.react_batch(tab, dc, synth.react_opts(guess_mode=guess))
     out = dc.to_host()
     assert np.array_equal(out["status"], ref["status"])
     ok = ref["status"] == 0
@@ -289,7 +289,7 @@ def test_update_kinetic_state_parity(net15):
     cells, _, _, _ = speciated_state(t, n, seed=5)
     tab = runtime.Tables(t)
     dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
-    runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1))
+    runtime.react_batch(tab, dc, synth.react_opts())
     state = orc.HostCells({k: (None if v is None else v.copy()) for k, v in dc.to_host().items()}, n)
     runtime.update_kinetic_state(tab, dc, synth.react_opts())
     orc.update_kinetic_state(t, state, synth.react_opts())

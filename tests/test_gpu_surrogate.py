@@ -10,14 +10,18 @@
 * the fallback guess mode: bit-identical to the plain method when every
   warm attempt fails, and equal to the CPU oracle's fallback;
 * a trained network's gain: fewer Newton iterations than the plain method
-  from the default guess, same convergence, same free ions (1e-10).
+  from the previous free ions (RReact's own start) and from the default
+  guess, same convergence, and free ions of every method against the
+  oracle to 1e-10 over 20,000 cells of the timed workload (the largest
+  error, not a percentile), wherever FP64 resolves the free ions that
+  finely (8x the per-cell conditioning floor elsewhere).
 """
 import numpy as np
 import pytest
 
 import mlp_oracle as mo
 import oracle as orc
-from gpu_cases import host_arrays, speciated_state
+from gpu_cases import conditioning_floor, host_arrays, speciated_state
 from pflotran_amd import abi, runtime, surrogate, synth, workload
 
 pytestmark = pytest.mark.gpu
@@ -113,8 +117,8 @@ def test_features_match_oracle(torch, net15):
     dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
     mlp = make_mlp(batch_max=64)
     X = surrogate.features(tab, mlp, dc).cpu().numpy()
-    Xo = mo.features(cells["total"], cells["total_sorb"], cells["temperature"], cells["mnrl_volfrac"], 40, t.N,
-                     t.K, t.R)
+    Xo = mo.features(cells["total"], cells["pri_molal"], cells["temperature"], cells["mnrl_volfrac"], 40, t.N,
+                     t.K)
     assert np.max(np.abs(X - Xo) / np.maximum(np.abs(Xo), 1.0)) <= 1e-6
 
 
@@ -127,31 +131,35 @@ def test_fused_guess_matches_unfused_and_oracle(torch, net15):
     rng = np.random.default_rng(9)
     mlp = make_mlp(batch_max=512)
     perturb_biases(mlp, rng)
-    Xo = mo.features(cells["total"], cells["total_sorb"], cells["temperature"], cells["mnrl_volfrac"], 40, t.N,
-                     t.K, t.R)
+    Xo = mo.features(cells["total"], cells["pri_molal"], cells["temperature"], cells["mnrl_volfrac"], 40, t.N,
+                     t.K)
     sd = Xo.std(0)
     in_scale = np.where(sd > 1e-6, 1.0 / np.where(sd > 1e-6, sd, 1.0), 1.0)
-    out_shift = rng.uniform(-20, -5, size=15)
-    out_scale = rng.uniform(0.1, 1.0, size=15)
+    # outputs are changes of ln m: kept well inside the guess's +-30 hold
+    out_shift = rng.uniform(-2.0, 2.0, size=15)
+    out_scale = rng.uniform(0.5, 2.0, size=15)
     mlp.set_norm(Xo.mean(0), in_scale, out_shift, out_scale)
     surrogate.guess(tab, mlp, dc)
     g_fused = dc.t["guess"].cpu().numpy()
     X = surrogate.features(tab, mlp, dc)
     Y = mlp.forward(X)
     g2 = torch.zeros((n, t.N), dtype=torch.float64, device="cuda")
-    surrogate.guess_from(mlp, Y, n, t.N, g2)
+    surrogate.guess_from(mlp, Y, n, t.N, dc.t["pri_molal"], g2)
     g_unfused = g2.cpu().numpy()
     nrm = mlp.get_norm()
-    y_fused = (np.log(g_fused) - nrm["out_shift"]) * nrm["out_scale"]
-    y_unfused = (np.log(g_unfused) - nrm["out_shift"]) * nrm["out_scale"]
+    prev = np.log(cells["pri_molal"])
+    y_fused = (np.log(g_fused) - prev - nrm["out_shift"]) * nrm["out_scale"]
+    y_unfused = (np.log(g_unfused) - prev - nrm["out_shift"]) * nrm["out_scale"]
     Yq, _ = mo.forward(mlp.weights(), X.cpu().numpy(), bf16=True)
     assert np.all(np.isfinite(g_fused)) and np.all(g_fused > 0)
     assert rel_to_max(y_fused, y_unfused) <= 2e-3
     assert rel_to_max(y_fused, Yq) <= 2e-3
 
 
-def test_fallback_is_the_plain_method_when_every_warm_attempt_fails(torch, net15):
-    t = net15
+@pytest.mark.parametrize("act_freq,fallback_previous", [(abi.PFX_ACT_TIMESTEP, 0), (abi.PFX_ACT_NEWTON_ITER, 0),
+                                                          (abi.PFX_ACT_TIMESTEP, 1)])
+def test_fallback_is_the_plain_method_when_every_warm_attempt_fails(torch, act_freq, fallback_previous):
+    t = synth.random_network(15, 50, 3, ionx=True, act_freq=act_freq)
     n = 300
     cells, _, _, _ = speciated_state(t, n)
     tab = runtime.Tables(t)
@@ -159,23 +167,25 @@ def test_fallback_is_the_plain_method_when_every_warm_attempt_fails(torch, net15
     arr["guess"][:] = 1e3   # far from every solution: no cell converges within 3 iterations
     dc = runtime.DeviceCells.from_host(arr, n)
     snap = workload.snapshot(dc)
-    rc, st_fb = runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK,
-                                                              fallback_after=3, extra_iterations=1))
-    fb = {k: dc.t[k].cpu().numpy() for k in ("pri_molal", "num_iterations", "status", "total")}
+    plain_mode = abi.PFX_GUESS_PREVIOUS if fallback_previous else abi.PFX_GUESS_DEFAULT
+    fb_opts = dict(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK, fallback_after=3, fallback_previous=fallback_previous)
+    rc, st_fb = runtime.react_batch(tab, dc, synth.react_opts(**fb_opts))
+    fb = {k: dc.t[k].cpu().numpy() for k in ("pri_molal", "num_iterations", "status", "total", "pri_act_coef",
+                                             "sec_act_coef", "sec_molal")}
     workload.restore(dc, snap)
-    rc, st_pl = runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT, extra_iterations=1))
-    pl = {k: dc.t[k].cpu().numpy() for k in ("pri_molal", "num_iterations", "status", "total")}
+    rc, st_pl = runtime.react_batch(tab, dc, synth.react_opts(guess_mode=plain_mode))
+    pl = {k: dc.t[k].cpu().numpy() for k in fb}
     assert st_fb.fallbacks == n
-    assert np.array_equal(fb["pri_molal"], pl["pri_molal"])
-    assert np.array_equal(fb["total"], pl["total"])
-    assert np.array_equal(fb["status"], pl["status"])
+    # the restart repeats the plain method from the step's start: activity
+    # coefficients, complexes and free ions bit for bit
+    for k in ("pri_molal", "total", "status", "pri_act_coef", "sec_act_coef", "sec_molal"):
+        assert np.array_equal(fb[k], pl[k]), k
     extra = fb["num_iterations"] - pl["num_iterations"]   # iterations of the abandoned attempt
     assert np.all((extra >= 1) & (extra <= 3))
     # the oracle's fallback takes the same path
     oc = cells.copy()
     oc["guess"][:] = 1e3
-    rc, st_o = orc.react_batch(t, oc, synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK, fallback_after=3,
-                                                       extra_iterations=1))
+    rc, st_o = orc.react_batch(t, oc, synth.react_opts(**fb_opts))
     assert st_o.fallbacks == n
     # iteration counts agree closely (rounding can move a long clipped path)
     assert np.mean(np.abs(oc["num_iterations"] - fb["num_iterations"])) < 0.5
@@ -211,40 +221,56 @@ def test_fallback_untouched_by_a_good_guess(torch, net15):
 
 
 def test_trained_network_reduces_newton_iterations(torch, net15):
+    """A network trained on GPU-generated pairs against both plain starting
+    points, on 20,000 held-out cells of the 1M-cell workload, with the timed
+    path's options: fewer Newton iterations than RReact's own start (the
+    previous free ions) and than the default guess; every cell converged by
+    every method; free ions of all three against the oracle's (default
+    guess) to 1e-10 wherever FP64 resolves them that finely."""
     t = net15
     tab = runtime.Tables(t)
     mlp = surrogate.MLP(surrogate.DIMS, batch_max=2048, seed=123)
-    X_raw, Y, gst = surrogate.generate_pairs(t, tab, mlp, 120_000, chunk=60_000)
+    X_raw, Y, gst = surrogate.generate_pairs(t, tab, mlp, 200_000, chunk=100_000)
     Xn, Yn = surrogate.fit_normalisation(mlp, X_raw, Y)
-    tr = surrogate.train(mlp, Xn, Yn, epochs=4, batch=2048, lr=2e-3)
+    tr = surrogate.train(mlp, Xn, Yn, epochs=6, batch=2048, lr=2e-3)
     assert tr["loss"][-1][1] < 0.5 * tr["loss"][0][1]
     # held-out cells of the bench workload (RNG key 2, not the training key)
-    n = 4000
+    n = 20_000
     dc, _ = workload.speciated_cells(t, tab, n)
     snap = workload.snapshot(dc)
-    rc, st_plain = runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT,
-                                                                 extra_iterations=1))
-    m_plain = dc.t["pri_molal"].cpu().numpy()
-    s_plain = dc.t["status"].cpu().numpy()
-    workload.restore(dc, snap)
-    surrogate.guess(tab, mlp, dc)
-    rc, st_warm = runtime.react_batch(tab, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK,
-                                                                fallback_after=30, extra_iterations=1))
-    m_warm = dc.t["pri_molal"].cpu().numpy()
-    s_warm = dc.t["status"].cpu().numpy()
-    print(f"plain {st_plain.total_iterations / n:.2f} it/cell, warm {st_warm.total_iterations / n:.2f} it/cell, "
-          f"fallbacks {st_warm.fallbacks}, converged {st_plain.converged}/{st_warm.converged}")
-    assert st_warm.converged >= st_plain.converged
-    assert st_warm.total_iterations < 0.8 * st_plain.total_iterations
+    res = {}
+    for key, opts in (("default", synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT)),
+                      ("previous", synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS)),
+                      ("warm", synth.react_opts(guess_mode=abi.PFX_GUESS_ARRAY_FALLBACK, fallback_after=50,
+                                                fallback_previous=1))):
+        workload.restore(dc, snap)
+        if key == "warm":
+            surrogate.guess(tab, mlp, dc)
+        rc, st = runtime.react_batch(tab, dc, opts)
+        res[key] = (st, dc.t["pri_molal"].cpu().numpy(), dc.t["status"].cpu().numpy(), dc.t["rel_res"].cpu().numpy())
+        print(f"{key}: {st.total_iterations / n:.3f} it/cell, converged {st.converged}, fallbacks {st.fallbacks}")
+    st_d, st_p, st_w = res["default"][0], res["previous"][0], res["warm"][0]
+    assert st_w.converged == st_p.converged == n
+    assert st_w.total_iterations < 0.9 * st_p.total_iterations
+    assert st_w.total_iterations < 0.5 * st_d.total_iterations
     # every returned state passes the same FP64 acceptance test
-    rr_warm = dc.t["rel_res"].cpu().numpy()
-    assert np.all(rr_warm[s_warm == 0] <= 1e-10)
-    # both methods reach the same free ions; cells whose Newton converges
-    # only linearly (capped mineral lnQK in the Jacobian) stop at different
-    # iterates within the tolerance
-    both = (s_plain == 0) & (s_warm == 0)
-    assert both.sum() >= 0.99 * n
-    rel = np.max(np.abs(m_warm[both] - m_plain[both]) / m_plain[both], axis=1)
-    print(f"free-ion difference: median {np.median(rel):.2e}, 99% {np.quantile(rel, 0.99):.2e}, max {rel.max():.2e}")
-    assert np.quantile(rel, 0.99) <= 1e-10
-    assert rel.max() <= 1e-6
+    assert np.all(res["warm"][3] <= 1e-10)
+    # the oracle from the default guess on the same cells
+    workload.restore(dc, snap)
+    inputs = orc.HostCells({k: (None if v is None else v.cpu().numpy().copy()) for k, v in dc.t.items()}, n)
+    ref = inputs.copy()
+    orc.react_batch(t, ref, synth.react_opts(guess_mode=abi.PFX_GUESS_DEFAULT))
+    floor = conditioning_floor(t, ref, inputs, synth.STEP_DT)
+    print(f"conditioning floor: median {np.median(floor):.1e}, cells above 1e-11: {(floor > 1e-11).sum()}")
+    for key in ("default", "previous", "warm"):
+        m, s_ = res[key][1], res[key][2]
+        both = (ref["status"] == 0) & (s_ == 0)
+        assert both.sum() >= n - 10
+        rel = np.max(np.abs(m - ref["pri_molal"]) / ref["pri_molal"], axis=1)
+        rel[~both] = 0.0
+        bound = np.maximum(1e-10, 8.0 * floor)
+        print(f"{key} vs oracle: max {rel.max():.2e}, over cells resolved to 1e-11: {rel[floor <= 1e-11].max():.2e}, "
+              f"cells above 1e-10: {(rel > 1e-10).sum()}")
+        worst = int(np.argmax(rel / bound))
+        assert rel[worst] <= bound[worst], (key, worst, rel[worst], floor[worst])
+        assert rel[floor <= 1e-11].max() <= 1e-10

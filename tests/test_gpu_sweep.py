@@ -23,11 +23,11 @@ def test_status_and_free_ions_match_the_oracle(N, M, K, ionx, seed, mode):
     n = len(idx)
     cells = orc.HostCells({k: (None if v is None else v[idx].copy()) for k, v in drawn.arrays.items()}, n)
     ref = cells.copy()
-    orc.react_batch(t, ref, synth.react_opts(extra_iterations=1, guess_mode=mode))
+    orc.react_batch(t, ref, synth.react_opts(guess_mode=mode))
     tab = runtime.Tables(t)
     dc = runtime.DeviceCells.from_host({k: (None if v is None else v.copy()) for k, v in cells.arrays.items()}, n)
     print(f"sweep N={N} M={M} ionx={ionx} mode={mode}: {n} cells", flush=True)
-    runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1, guess_mode=mode))
+    runtime.react_batch(tab, dc, synth.react_opts(guess_mode=mode))
     out = dc.to_host()
     assert np.array_equal(out["status"], ref["status"])
     both = ref["status"] == 0

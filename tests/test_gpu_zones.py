@@ -92,10 +92,10 @@ def test_thirty_primaries_match_oracle(zones):
     nets, cells = zones
     t, c = nets[3], cells[3]
     ref = c.copy()
-    orc.react_batch(t, ref, synth.react_opts(extra_iterations=1))
+    orc.react_batch(t, ref, synth.react_opts())
     tab = runtime.Tables(t)
     dc = device(c, c.ncells)
-    runtime.react_batch(tab, dc, synth.react_opts(extra_iterations=1))
+    runtime.react_batch(tab, dc, synth.react_opts())
     out = dc.to_host()
     ok = ref["status"] == 0
     assert np.array_equal(out["status"], ref["status"])

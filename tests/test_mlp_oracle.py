@@ -85,12 +85,14 @@ def test_bf16_product_differs_from_fp64_by_rounding_only():
 
 def test_features_slots():
     total = np.array([[1e-3, -2e-4, 0.0]])
-    sorb = np.array([[5.0, 0.0, 1e-2]])
-    X = mo.features(total, sorb, np.array([30.0]), np.array([[0.05, 0.02]]), 12, 3, 2, 1)
+    prev = np.array([[1e-3, 2e-7, 0.0]])
+    X = mo.features(total, prev, np.array([30.0]), np.array([[0.05, 0.02]]), 12, 3, 2)
     assert X[0, 0] == pytest.approx(np.log10(1 + 1e9))
     assert X[0, 1] == pytest.approx(-np.log10(1 + 2e8))
     assert X[0, 2] == 0.0
-    assert X[0, 3] == pytest.approx(np.log10(1 + 5e12))
+    assert X[0, 3] == pytest.approx(np.log(1e-3))
+    assert X[0, 4] == pytest.approx(np.log(2e-7))
+    assert X[0, 5] == pytest.approx(np.log(1e-300))   # a zero free ion is held finite
     assert X[0, 6] == 30.0
     assert list(X[0, 7:9]) == [0.05, 0.02]
     assert np.all(X[0, 9:] == 0.0)
