@@ -405,25 +405,37 @@ def gi_leg(t, tab, args, log):
     log(f"gi: block product {out['spmv_ms']:.2f} ms ({out['spmv_GBs']:.0f} GB/s)")
 
     # one time step closed on the device: Newton from the old state with
-    # point-block Jacobi BiCGStab, PETSc's default tolerances, with the
-    # analytic reaction Jacobian and with NUMERICAL_JACOBIAN
+    # point-block Jacobi BiCGStab, PETSc's default tolerances, and the
+    # reference's time-step cut (halve dt and retry from the old state, at
+    # most 16 times) when Newton does not converge; with the reference's
+    # reaction Jacobian, with the capped-Q/K derivative (first 12 Newton
+    # iterations), and with NUMERICAL_JACOBIAN
     snap = workload.snapshot(dc)
-    for key, numerical, jflags in (("newton", 0, 0), ("newton_numerical_jacobian", 1, 0)):
+    for key, numerical, jflags in (("newton", 0, 0), ("newton_capped_qk", 0, abi.PFX_JAC_CAPPED_QK),
+                                   ("newton_numerical_jacobian", 1, 0)):
         workload.restore(dc, snap)
         t.numerical_jacobian = numerical
         tab_n = runtime.Tables(t)
         t.numerical_jacobian = 0
         gs = runtime.GiSolver(tab_n, g)
-        st = gs.step(dc, g, synth.react_opts(jacobian_flags=jflags))
+        dt_want = synth.STEP_DT
+        st = gs.timestep(dc, g, synth.react_opts(dt=dt_want, jacobian_flags=jflags))
         gs.close()
-        out[key] = {"reason": st.reason, "iterations": st.iterations, "linear_iterations": st.linear_iterations,
-                    "fnorm0": st.fnorm0, "fnorm": st.fnorm, "ms_total": st.ms_total,
-                    "ms_assembly": st.ms_assembly, "ms_linear": st.ms_linear,
-                    "ms_per_newton_iteration": st.ms_total / max(st.iterations, 1),
-                    "tolerances": "SNES rtol 1e-8 / stol 1e-8 / 50 its, KSP rtol 1e-5 (PETSc defaults)"}
-        log(f"gi: {key} {st.ms_total:.1f} ms, {st.iterations} its ({st.linear_iterations} BiCGStab), "
-            f"reason {st.reason}, |F| {st.fnorm0:.2e} -> {st.fnorm:.2e}, assembly {st.ms_assembly:.1f} ms, "
-            f"linear {st.ms_linear:.1f} ms")
+        ns = st.newton
+        out[key] = {"reason": st.reason, "converged": st.reason > 0, "dt_wanted": dt_want, "dt_taken": st.dt,
+                    "cuts": st.cuts, "newton_iterations": st.newton_iterations,
+                    "wasted_newton_iterations": st.wasted_newton_iterations,
+                    "linear_iterations": st.linear_iterations, "ms_total": st.ms_total,
+                    "last_solve": {"iterations": ns.iterations, "fnorm0": ns.fnorm0, "fnorm": ns.fnorm,
+                                   "ms_assembly": ns.ms_assembly, "ms_linear": ns.ms_linear},
+                    "ms_per_newton_iteration": st.ms_total / max(st.newton_iterations, 1),
+                    "tolerances": "SNES rtol 1e-8 / stol 1e-8 / 50 its, KSP rtol 1e-5 (PETSc defaults); "
+                                  "cut factor 0.5, at most 16 cuts (timestepper defaults)"}
+        if st.reason <= 0:
+            out[key]["note"] = "NOT CONVERGED after the cut budget: the time is not a step time"
+        log(f"gi: {key} {st.ms_total:.1f} ms, reason {st.reason}, dt {st.dt:.0f} s after {st.cuts} cuts, "
+            f"{st.newton_iterations} Newton its ({st.wasted_newton_iterations} wasted), "
+            f"{st.linear_iterations} BiCGStab, last |F| {ns.fnorm0:.2e} -> {ns.fnorm:.2e}")
         del tab_n
 
     # the transport half of an operator-split step on the same grid: the

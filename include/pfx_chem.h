@@ -246,6 +246,13 @@ typedef struct pfx_react_opts {
    for such cells.  The residual, and so the solution, are unchanged.  No
    counterpart in the reference. */
 #define PFX_JAC_CAPPED_QK 1
+/* The capped derivative is used for the first PFX_JAC_CAPPED_ITERS Newton
+   iterations of an attempt, the reference's after them: the residual has
+   a kink at the cap, across which Newton with the capped derivative can
+   cycle (rare cells); the reference's derivative, larger there, does not. */
+#ifndef PFX_JAC_CAPPED_ITERS
+#define PFX_JAC_CAPPED_ITERS 12
+#endif
 
 typedef struct pfx_stats {
   int64_t ncells;

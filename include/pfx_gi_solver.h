@@ -128,6 +128,29 @@ int pfx_gi_solver_destroy(pfx_gi_solver *s);
 int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *grid, const pfx_react_opts *opts,
                        const pfx_newton_opts *nopts, pfx_newton_stats *stats);
 
+/* One time step of the global-implicit solve with the reference's time-step
+   cut (timestepper_BE.F90:344-399): pfx_gi_solver_step with opts->dt; a
+   step whose Newton solve does not converge (reason <= 0) restores the old
+   free ions and exchange state (RTTimeCut, reactive_transport.F90:58-96),
+   multiplies dt by reduction_factor and is solved again, at most max_cuts
+   times (< 0: the reference's 16; a factor outside (0,1): its 0.5,
+   timestepper_base.F90:144-145).  The step taken is stats->dt; the caller
+   advances time by it (the reference's target time moves with the cut). */
+typedef struct pfx_gi_step_stats {
+  pfx_newton_stats newton;          /* the last Newton solve                 */
+  int32_t reason;                   /* its PFX_SNES_* reason                 */
+  int32_t cuts;                     /* time-step cuts taken                  */
+  int32_t newton_iterations;        /* over every attempt                    */
+  int32_t wasted_newton_iterations; /* in the attempts that were cut         */
+  int32_t linear_iterations;
+  int32_t reserved;
+  double dt;                        /* the step taken (s)                    */
+  double ms_total;                  /* host wall time                        */
+} pfx_gi_step_stats;
+int pfx_gi_solver_timestep(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *grid, const pfx_react_opts *opts,
+                           const pfx_newton_opts *nopts, int32_t max_cuts, double reduction_factor,
+                           pfx_gi_step_stats *stats);
+
 /* The transport half of an operator-split step (the reference's
    deprecated/timestepper.F90:2156-2245 sequence): one implicit step of the
    totals of every component, A T_new = b, with A the scalar transport

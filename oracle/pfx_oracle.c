@@ -634,14 +634,16 @@ static double or_rel_res(const double *R, const double *scale, int N) {
 
 /* reaction.F90:4835-4880 RSolve: row equilibration, log-column scaling,
    ludcmp/lubksb.  J is consumed. */
-static int or_rsolve(const double *R, double J[NMAX][NMAX], const double *conc, double *update, int N, int uselog) {
+static int or_rsolve2(const double *R, double J[NMAX][NMAX], const double *conc, double *update, int N, int uselog,
+                      const double *R2, double *update2) {
   double A[NMAX * NMAX];
-  double rhs[NMAX];
+  double rhs[NMAX], rhs2[NMAX];
   for (int i = 0; i < N; i++) {
     double norm = 0.0;
     for (int k = 0; k < N; k++) norm = fmax(norm, fabs(J[i][k]));
     norm = 1.0 / fmax(1.0, norm);
     rhs[i] = R[i] * norm;
+    if (R2) rhs2[i] = R2[i] * norm;
     for (int k = 0; k < N; k++) J[i][k] *= norm;
   }
   if (uselog)
@@ -654,7 +656,15 @@ static int or_rsolve(const double *R, double J[NMAX][NMAX], const double *conc, 
   if (st) return st;
   or_lubksb(A, N, indx, rhs);
   for (int i = 0; i < N; i++) update[i] = rhs[i];
+  if (R2) {
+    or_lubksb(A, N, indx, rhs2);
+    for (int i = 0; i < N; i++) update2[i] = rhs2[i];
+  }
   return 0;
+}
+
+static int or_rsolve(const double *R, double J[NMAX][NMAX], const double *conc, double *update, int N, int uselog) {
+  return or_rsolve2(R, J, conc, update, N, uselog, NULL, NULL);
 }
 
 /* per-cell temperature-dependent log K (RUpdateTempDependentCoefs,
@@ -713,7 +723,9 @@ static int or_react_cell(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
   /* largest |dm/m| of the attempt's last Newton update (none yet: unknown) */
   double last_change = INFINITY;
   double Rv[NMAX], J[NMAX][NMAX], sc[NMAX], upd[NMAX];
+  int start_it = 0;
 restart:
+  start_it = it;
   or_load_cell(d, cl, o, ci, &c);
 
   /* fixed accumulation from the transported totals and the old sorbed
@@ -744,7 +756,10 @@ restart:
          RTAuxVarCompute (the previous iterate) */
       or_activity(d, &c);
     }
-    or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0, o->jacobian_flags);
+    /* the capped-Q/K derivative for the attempt's first iterations, the
+       reference's after them */
+    const int jf = (it - start_it) < PFX_JAC_CAPPED_ITERS ? o->jacobian_flags : 0;
+    or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0, jf);
     rr = or_rel_res(Rv, sc, N);
     if (!isfinite(rr)) { status = PFX_CELL_NONFINITE; break; }
     /* rel_res test passed and the extra updates taken: converged, or, with
@@ -764,7 +779,15 @@ restart:
       if (mx < d->max_residual_tolerance) { status = PFX_CELL_CONVERGED; break; }
     }
     if (!candidate && it >= cap) { status = PFX_CELL_MAX_ITERATIONS; break; }
-    if (or_rsolve(Rv, J, c.m, upd, N, d->use_log_formulation)) { status = PFX_CELL_SINGULAR; break; }
+    /* at a candidate the factors also solve for the rounding level of each
+       residual row (unit roundoff times the row's term magnitude): the
+       FP64 resolution of the free ions (no reference counterpart) */
+    double rf[NMAX], fupd[NMAX];
+    for (int j = 0; j < N; j++) rf[j] = 2.220446049250313e-16 * sc[j];
+    if (or_rsolve2(Rv, J, c.m, upd, N, d->use_log_formulation, candidate ? rf : NULL, fupd)) {
+      status = PFX_CELL_SINGULAR;
+      break;
+    }
     double prev[NMAX], next[NMAX], maxrel = 0.0;
     for (int j = 0; j < N; j++) prev[j] = c.m[j];
     if (d->use_log_formulation) {
@@ -791,14 +814,17 @@ restart:
          only linearly (a Jacobian that is not the residual's derivative);
          the iterate itself is returned, its residual having passed */
       const double rho = fmin(isfinite(last_change) && last_change > 0.0 ? maxrel / last_change : 0.0, 0.9);
-      if (maxrel / (1.0 - rho) <= o->rel_change_tol) { status = PFX_CELL_CONVERGED; break; }
+      /* a step within twice the FP64 resolution is rounding noise */
+      double frel = 0.0;
+      for (int j = 0; j < N; j++) frel = fmax(frel, fabs(d->use_log_formulation ? fupd[j] : fupd[j] / prev[j]));
+      if (maxrel / (1.0 - rho) <= o->rel_change_tol || maxrel <= 2.0 * frel) { status = PFX_CELL_CONVERGED; break; }
       if (it >= cap) { status = PFX_CELL_MAX_ITERATIONS; break; }
     }
     it++;
     for (int j = 0; j < N; j++) c.m[j] = next[j];
     last_change = maxrel;
     if (o->reference_stop && maxrel < d->max_relative_change_tolerance) {
-      or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0, o->jacobian_flags);
+      or_res_jac(d, &c, fixed, dt, Rv, J, sc, 0, 0);
       rr = or_rel_res(Rv, sc, N);
       status = PFX_CELL_CONVERGED;
       done = 1;

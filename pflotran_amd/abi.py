@@ -142,6 +142,18 @@ class NewtonStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+class GiStepStats(C.Structure):
+    _fields_ = [("newton", NewtonStats), ("reason", C.c_int32), ("cuts", C.c_int32),
+                ("newton_iterations", C.c_int32), ("wasted_newton_iterations", C.c_int32),
+                ("linear_iterations", C.c_int32), ("reserved", C.c_int32), ("dt", C.c_double),
+                ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("reserved", "newton")}
+        d["newton"] = self.newton.as_dict()
+        return d
+
+
 PFX_KSP_CONVERGED_RTOL, PFX_KSP_CONVERGED_ATOL = 2, 3
 PFX_KSP_DIVERGED_ITS, PFX_KSP_DIVERGED_DTOL, PFX_KSP_DIVERGED_BREAKDOWN = -3, -4, -5
 PFX_KSP_DIVERGED_NANORINF, PFX_KSP_DIVERGED_PC_FAILED = -9, -11

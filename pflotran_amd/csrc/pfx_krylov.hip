@@ -684,6 +684,9 @@ struct pfx_gi_solver {
   int64_t n = 0, nnzb = 0;
   int32_t *rp = nullptr, *ci = nullptr;
   double *vals = nullptr, *res = nullptr, *y = nullptr;
+  // the old state a cut time step restores: free ions and exchange state
+  double *old_m = nullptr, *old_ionx = nullptr;
+  int32_t R = 0;
   pfx_bsr *bsr = nullptr;
 };
 
@@ -695,6 +698,8 @@ extern "C" int pfx_gi_solver_destroy(pfx_gi_solver *s) {
   if (s->vals) (void)hipFree(s->vals);
   if (s->res) (void)hipFree(s->res);
   if (s->y) (void)hipFree(s->y);
+  if (s->old_m) (void)hipFree(s->old_m);
+  if (s->old_ionx) (void)hipFree(s->old_ionx);
   delete s;
   return 0;
 }
@@ -714,6 +719,7 @@ extern "C" int pfx_gi_solver_create(pfx_tables *tables, const pfx_grid *grid, pf
   s->ny = grid->ny;
   s->nz = grid->nz;
   s->N = N;
+  s->R = R;
   s->n = (int64_t)grid->nx * grid->ny * grid->nz;
   s->nnzb = nnzb;
   bool ok = hipMalloc((void **)&s->rp, (s->n + 1) * sizeof(int32_t)) == hipSuccess;
@@ -721,6 +727,8 @@ extern "C" int pfx_gi_solver_create(pfx_tables *tables, const pfx_grid *grid, pf
   ok = ok && hipMalloc((void **)&s->vals, (size_t)nnzb * N * N * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void **)&s->res, (size_t)s->n * N * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void **)&s->y, (size_t)s->n * N * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&s->old_m, (size_t)s->n * N * sizeof(double)) == hipSuccess;
+  if (R > 0) ok = ok && hipMalloc((void **)&s->old_ionx, (size_t)s->n * R * sizeof(double)) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
     pfx_gi_solver_destroy(s);
@@ -754,9 +762,14 @@ extern "C" int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_
   const int64_t n = s->n * s->N;
   const int ge = grid_for(n);
   double tms[2];
+  // the capped-Q/K derivative for the first PFX_JAC_CAPPED_ITERS Newton
+  // iterations, the reference's after them
+  pfx_react_opts oe = *o;
 
   auto assemble = [&](double *fnorm) -> int {
-    int rc = pfx_gi_residual_jacobian(s->tables, cells, grid, o, s->rp, s->res, s->vals);
+    oe.jacobian_flags = S.iterations < PFX_JAC_CAPPED_ITERS ? o->jacobian_flags
+                                                            : (o->jacobian_flags & ~PFX_JAC_CAPPED_QK);
+    int rc = pfx_gi_residual_jacobian(s->tables, cells, grid, &oe, s->rp, s->res, s->vals);
     if (rc) return rc;
     pfx_tables_timing(s->tables, tms);
     S.ms_assembly += tms[0] + tms[1];
@@ -813,6 +826,52 @@ extern "C" int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_
     else if (S.fnorm <= rtol * S.fnorm0) S.reason = PFX_SNES_CONVERGED_FNORM_RELATIVE;
     else if (sqrt(yy) < stol * sqrt(xx)) S.reason = PFX_SNES_CONVERGED_SNORM_RELATIVE;
   }
+  S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  if (st) *st = S;
+  return 0;
+}
+
+// One time step with the reference's cut and retry (timestepper_BE.F90:
+// 344-399): a Newton solve that does not converge restores the old state
+// (RTTimeCut, reactive_transport.F90:58-96: tran_xx <- tran_yy; here the
+// free ions and the exchange state), scales dt by reduction_factor and
+// tries again, until max_cuts cuts have been taken.
+extern "C" int pfx_gi_solver_timestep(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *grid,
+                                      const pfx_react_opts *o, const pfx_newton_opts *no, int32_t max_cuts,
+                                      double reduction_factor, pfx_gi_step_stats *st) {
+  if (!s || !cells || !grid || !o || !cells->pri_molal || !(o->dt > 0.0)) return PFX_E_ARG;
+  if (cells->ncells != s->n) return PFX_E_ARG;
+  if (max_cuts < 0) max_cuts = 16;                       // timestepper_base.F90:144
+  if (!(reduction_factor > 0.0 && reduction_factor < 1.0)) reduction_factor = 0.5;  // :145
+  const auto wall0 = std::chrono::steady_clock::now();
+  KCHK(hipSetDevice(s->device));
+  hipStream_t hs = (hipStream_t)o->stream;
+  const size_t bm = (size_t)s->n * s->N * sizeof(double);
+  const size_t bx = (size_t)s->n * s->R * sizeof(double);
+  const bool ionx = s->R > 0 && cells->ionx_ref_sorbed;
+  KCHK(hipMemcpyAsync(s->old_m, cells->pri_molal, bm, hipMemcpyDeviceToDevice, hs));
+  if (ionx) KCHK(hipMemcpyAsync(s->old_ionx, cells->ionx_ref_sorbed, bx, hipMemcpyDeviceToDevice, hs));
+  pfx_gi_step_stats S;
+  memset(&S, 0, sizeof(S));
+  pfx_react_opts oc = *o;
+  for (;;) {
+    pfx_newton_stats ns;
+    int rc = pfx_gi_solver_step(s, cells, grid, &oc, no, &ns);
+    if (rc) return rc;
+    S.newton = ns;
+    S.newton_iterations += ns.iterations;
+    S.linear_iterations += ns.linear_iterations;
+    if (ns.reason > 0) break;
+    S.wasted_newton_iterations += ns.iterations;
+    if (S.cuts >= max_cuts || oc.dt * reduction_factor < 1.0e-20) break;   // dt_min, :159
+    S.cuts++;
+    oc.dt *= reduction_factor;
+    KCHK(hipMemcpyAsync(cells->pri_molal, s->old_m, bm, hipMemcpyDeviceToDevice, hs));
+    if (ionx) KCHK(hipMemcpyAsync(cells->ionx_ref_sorbed, s->old_ionx, bx, hipMemcpyDeviceToDevice, hs));
+  }
+  KCHK(hipStreamSynchronize(hs));
+  S.reason = S.newton.reason;
+  S.dt = oc.dt;
   S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
   if (st) *st = S;
   return 0;

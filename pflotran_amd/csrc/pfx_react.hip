@@ -928,6 +928,11 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       }
       // kinetic minerals (group-uniform rates, row-local residual/Jacobian)
       const bool fd_jac = MODE == PFX_MODE_EVAL && L.num_jac;
+      // the capped-Q/K derivative for the attempt's first iterations only
+      // (the attempt started at it_cap less its budget)
+      const int att_it = it - (it_cap - ((MODE == PFX_MODE_REACT && P.guess_mode == PFX_GUESS_ARRAY_FALLBACK &&
+                                          !fell_back) ? P.fallback_after : P.max_iterations));
+      const bool capped_jac = (P.jac_flags & PFX_JAC_CAPPED_QK) && (MODE == PFX_MODE_EVAL || att_it < PFX_JAC_CAPPED_ITERS);
       for (int k = 0; k < K; k++) {
         double lnQK = SD(S.mlnK, k);
         for (int s = 0; s < L.S; s++) {
@@ -961,7 +966,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
               const double dd = 1.0 + (1.0 - aff) / lim;
               f = f * (1.0 + QK / lim / dd) / dd;
             }
-            if (!fd_jac && !((P.jac_flags & PFX_JAC_CAPPED_QK) && lnQK > 6.90776)) {
+            if (!fd_jac && !(capped_jac && lnQK > 6.90776)) {
 #pragma unroll
               for (int kk = 0; kk < G; kk++) row[kk] += f * LD(L.o_mnuG, k * G + kk) * SD(S.im, kk);
             }
@@ -1073,7 +1078,15 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     PFX_PT(6);
     // RSolve: row equilibration, ln-column scaling, pivoted LU, solve
     double upd = 0.0;
+    double fl = 0.0;  // floor solve: the step a rounding-level residual would cause
     if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
+      // At a step-test candidate the same factors also solve for b2, the
+      // rounding level of each residual row (unit roundoff times the
+      // magnitude of the row's terms): its solution is the resolution of
+      // the free ions in FP64.  Wave-uniform switch, so the lane moves of
+      // the extra right-hand side run with every lane of each group.
+      const bool floor_solve = MODE == PFX_MODE_REACT && __ballot(candidate) != 0ull;
+      double b2 = 0.0;
       // Straight-line code: the steps run over all G rows with selects
       // instead of branches (a divergent branch, or a step skipped for
       // j >= N, makes the compiler copy the whole row at every merge).
@@ -1088,6 +1101,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         for (int k = 0; k < G; k++) mx = fmax(mx, fabs(row[k]));
         const double norm = live ? 1.0 / fmax(1.0, mx) : 1.0;
         b = live ? Rr * norm : 0.0;
+        b2 = (live && candidate) ? 2.220446049250313e-16 * scale * norm : 0.0;
         const bool lg = live && use_log;
 #pragma unroll
         for (int k = 0; k < G; k++) {
@@ -1136,6 +1150,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         }
         const double pb = bperm_d(b, src);
         b -= f * pb;
+        if (floor_solve) {
+          const double pb2 = bperm_d(b2, src);
+          b2 -= f * pb2;
+        }
       }
       // back substitution: the pivot row of step j is held by the lane whose
       // mystep is j, the only non-zero term of the group sum that spreads
@@ -1145,6 +1163,14 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         const double xj = gsum<G>(mystep == j ? b / row[j] : 0.0, lane);
         upd = (r == j && live) ? xj : upd;
         b -= row[j] * xj;
+      }
+      if (floor_solve) {
+#pragma unroll
+        for (int j = G - 1; j >= 0; j--) {
+          const double xj = gsum<G>(mystep == j ? b2 / row[j] : 0.0, lane);
+          fl = (r == j && live) ? xj : fl;
+          b2 -= row[j] * xj;
+        }
       }
       const double bad = gmax<G>(status == PFX_CELL_SINGULAR ? 1.0 : 0.0, lane);
       if (bad > 0.0) {
@@ -1174,9 +1200,12 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         // 1/(1 - rho), rho = this step over the last one (at most 10x),
         // bounds the distance to the root also where the iteration
         // contracts only linearly; the iterate itself is returned (its
-        // residual passed), the step is not taken
+        // residual passed), the step is not taken.  A step within twice
+        // the FP64 resolution of the free ions (the floor solve) is
+        // rounding noise: the iterate is as good as FP64 makes it.
         const double rho = fmin(last_change > 0.0 && last_change < INFINITY ? rel / last_change : 0.0, 0.9);
-        if (rel / (1.0 - rho) <= P.rel_change_tol) {
+        const double frel = gmax<G>(live ? fabs(use_log ? fl : fl / m) : 0.0, lane);
+        if (rel / (1.0 - rho) <= P.rel_change_tol || rel <= 2.0 * frel) {
           finish = true;
         } else if (it >= it_cap) {
           finish = true;

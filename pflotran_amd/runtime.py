@@ -30,7 +30,7 @@ EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_r
             "pfx_surrogate_guess_from",
             # global-implicit step on the device (include/pfx_gi_solver.h)
             "pfx_bsr_create", "pfx_bsr_destroy", "pfx_bsr_spmv", "pfx_bsr_solve", "pfx_gi_solver_create",
-            "pfx_gi_solver_destroy", "pfx_gi_solver_step", "pfx_os_transport_step"]
+            "pfx_gi_solver_destroy", "pfx_gi_solver_step", "pfx_gi_solver_timestep", "pfx_os_transport_step"]
 
 _lib = None
 
@@ -96,6 +96,8 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_gi_solver_destroy.argtypes = [vp]
     lib.pfx_gi_solver_step.argtypes = [vp, P(abi.Cells), P(abi.Grid), P(abi.ReactOpts), P(abi.NewtonOpts),
                                        P(abi.NewtonStats)]
+    lib.pfx_gi_solver_timestep.argtypes = [vp, P(abi.Cells), P(abi.Grid), P(abi.ReactOpts), P(abi.NewtonOpts), i32,
+                                           C.c_double, P(abi.GiStepStats)]
     lib.pfx_os_transport_step.argtypes = [vp, P(abi.Grid), P(abi.Cells), i32, P(abi.ReactOpts), vp,
                                           P(abi.KrylovOpts), P(abi.KrylovStats)]
     if lib.pfx_abi_version() != abi.PFX_ABI_VERSION:
@@ -449,6 +451,23 @@ class GiSolver:
                                          C.byref(st))
         if rc != 0:
             raise PfxError(f"pfx_gi_solver_step failed: {rc}")
+        return st
+
+    def timestep(self, cells: DeviceCells, grid: DeviceGrid, opts: abi.ReactOpts,
+                 newton: Optional[abi.NewtonOpts] = None, max_cuts: int = -1, reduction_factor: float = 0.5):
+        """pfx_gi_solver_timestep: the step with the time-step cut and
+        retry; returns abi.GiStepStats (dt = the step taken)."""
+        torch = _torch()
+        if opts.stream is None:
+            opts.stream = torch.cuda.current_stream().cuda_stream
+        no = newton if newton is not None else abi.NewtonOpts()
+        st = abi.GiStepStats()
+        cs = cells.struct()
+        g = grid.struct()
+        rc = self.lib.pfx_gi_solver_timestep(self.handle, C.byref(cs), C.byref(g), C.byref(opts), C.byref(no),
+                                             C.c_int32(max_cuts), C.c_double(reduction_factor), C.byref(st))
+        if rc != 0:
+            raise PfxError(f"pfx_gi_solver_timestep failed: {rc}")
         return st
 
     def close(self):

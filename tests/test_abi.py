@@ -46,6 +46,7 @@ def test_null_arguments_rejected_without_gpu_work():
     assert lib.pfx_bsr_solve(None, None, None, None, None, None, None) == abi.PFX_E_ARG
     assert lib.pfx_gi_solver_create(None, None, None) == abi.PFX_E_ARG
     assert lib.pfx_gi_solver_step(None, None, None, None, None, None) == abi.PFX_E_ARG
+    assert lib.pfx_gi_solver_timestep(None, None, None, None, None, -1, 0.5, None) == abi.PFX_E_ARG
     assert lib.pfx_os_transport_step(None, None, None, 0, None, None, None, None) == abi.PFX_E_ARG
 
 
@@ -60,6 +61,7 @@ def test_every_header_compiles_as_c(tmp_path):
                                           (abi.Grid, "pfx_grid"), (abi.ZonePlan, "pfx_zone_plan"),
                                           (abi.KrylovOpts, "pfx_krylov_opts"), (abi.KrylovStats, "pfx_krylov_stats"),
                                           (abi.NewtonOpts, "pfx_newton_opts"), (abi.NewtonStats, "pfx_newton_stats"),
+                                          (abi.GiStepStats, "pfx_gi_step_stats"),
                                           (abi.ConstraintRxn, "pfx_constraint_rxn")])
 def test_struct_layout_matches_header(tmp_path, struct, cname):
     fields = [f for f, _ in struct._fields_]

@@ -177,3 +177,65 @@ def test_one_closed_cell_is_the_reaction_step():
     m_gi = dg.to_host()["pri_molal"]
     assert np.max(np.abs(m_gi - m_react) / m_react) <= 1e-9
     gs.close()
+
+
+def test_time_step_cut_restores_the_old_state_and_retries():
+    # a Newton budget of 3 iterations fails at the full step; the step is
+    # cut by halves (timestepper_BE.F90:344-399) until 3 suffice, each retry
+    # from the restored old state, so the accepted attempt is the plain
+    # Newton solve of the old state at the cut step
+    t = synth.random_network(15, 50, 3, ionx=True)
+    nx, ny, nz = 5, 4, 3
+    n = nx * ny * nz
+    rng = np.random.default_rng(53)
+    q, dd = faces(rng, nx, ny, nz)
+    cells, _, _, _ = speciated_state(t, n, seed=59)
+    tab = runtime.Tables(t)
+    g = runtime.DeviceGrid(nx, ny, nz, q, dd, log_formulation=True)
+    gs = runtime.GiSolver(tab, g)
+    no = abi.NewtonOpts(rtol=1e-10, max_iterations=3)
+    no.krylov.rtol = 1e-13
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    dt0 = 10 * synth.STEP_DT
+    st = gs.timestep(dc, g, synth.react_opts(dt=dt0), no)
+    assert st.reason > 0 and st.cuts >= 1, st.as_dict()
+    assert st.dt == dt0 * 0.5 ** st.cuts
+    assert st.newton_iterations == st.wasted_newton_iterations + st.newton.iterations
+    assert 0 < st.wasted_newton_iterations <= 3 * st.cuts
+    out = dc.to_host()
+    ref = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    st1 = gs.step(ref, g, synth.react_opts(dt=st.dt), no)
+    assert st1.reason == st.reason and st1.iterations == st.newton.iterations
+    assert np.array_equal(ref.to_host()["pri_molal"], out["pri_molal"])
+    assert np.array_equal(ref.to_host()["ionx_ref_sorbed"], out["ionx_ref_sorbed"])
+    # a cut budget of 0 leaves the failed step in place and reports it
+    dz = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    st0 = gs.timestep(dz, g, synth.react_opts(dt=dt0), no, max_cuts=0)
+    assert st0.reason == abi.PFX_SNES_DIVERGED_MAX_IT and st0.cuts == 0 and st0.dt == dt0
+    gs.close()
+
+
+def test_capped_derivative_gi_newton_converges_quadratically():
+    # PFX_JAC_CAPPED_QK in the global Newton: the same converged state as
+    # the reference derivative's Newton (the residual is the same), reached
+    # in no more iterations
+    t = synth.random_network(15, 50, 3, ionx=True)
+    nx, ny, nz = 6, 5, 4
+    n = nx * ny * nz
+    rng = np.random.default_rng(61)
+    q, dd = faces(rng, nx, ny, nz)
+    cells, _, _, _ = speciated_state(t, n, seed=67)
+    tab = runtime.Tables(t)
+    g = runtime.DeviceGrid(nx, ny, nz, q, dd, log_formulation=True)
+    gs = runtime.GiSolver(tab, g)
+    no = abi.NewtonOpts(rtol=1e-12, max_iterations=60)
+    no.krylov.rtol = 1e-14
+    res = {}
+    for jf in (0, abi.PFX_JAC_CAPPED_QK):
+        dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+        st = gs.step(dc, g, synth.react_opts(jacobian_flags=jf), no)
+        assert st.reason > 0, (jf, st.as_dict())
+        res[jf] = (st.iterations, dc.to_host()["pri_molal"])
+    assert res[abi.PFX_JAC_CAPPED_QK][0] <= res[0][0]
+    assert np.max(np.abs(res[0][1] - res[1][1]) / res[0][1]) <= 1e-8
+    gs.close()
