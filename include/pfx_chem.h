@@ -192,7 +192,8 @@ typedef struct pfx_cells {
   double *ionx_ref_sorbed;/* [ncells*R] In/Out eqionx_ref_cation_sorbed_conc   */
   const double *mnrl_volfrac; /* [ncells*K]                                  */
   const double *mnrl_area;    /* [ncells*K] m^2/m^3 bulk                      */
-  double *mnrl_rate;      /* [ncells*K] Out mol/m^3/s, may be NULL             */
+  double *mnrl_rate;      /* [ncells*K] Out mol/m^3/s at the returned state (also
+                             from the residual evaluations), may be NULL     */
   const double *temperature;  /* [ncells] deg C; NULL -> table log K (isothermal) */
   const double *porosity;     /* [ncells] or NULL -> opts default             */
   const double *saturation;   /* [ncells] or NULL                              */

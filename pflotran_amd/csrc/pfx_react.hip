@@ -1266,6 +1266,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           for (int k = 0; k < G; k++)
             if (k < N) P.jac_out[c * N * N + (long long)k * N + r] = P.col_scale ? row[k] * SD(S.mv, k) : row[k];
         }
+        // the mineral rates at this iterate (the last evaluation of a
+        // global-implicit solve leaves those of its solution)
+        if (P.mnrl_rate)
+          for (int k = r; k < K; k += G) P.mnrl_rate[c * K + k] = SD(S.mrate, k);
       } else {
         if (MODE == PFX_MODE_EQUIL && R > 0) {
           // sorbed totals at the final free ions (reaction.F90:1970-1975)

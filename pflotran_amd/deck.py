@@ -236,6 +236,61 @@ class TimeCard:
 
 
 @dataclass
+class RegionCard:
+    name: str
+    face: Optional[str] = None                    # WEST, EAST, SOUTH, NORTH, BOTTOM, TOP
+    coordinates: Optional[Tuple[Tuple[float, float, float], Tuple[float, float, float]]] = None
+
+
+@dataclass
+class TransportConditionCard:
+    name: str
+    type: str                                     # DIRICHLET, DIRICHLET_ZERO_GRADIENT, ZERO_GRADIENT, ...
+    constraint: str
+
+
+@dataclass
+class TransportCard:
+    """The structured box and its transport: GRID, UNIFORM_VELOCITY,
+    FLUID_PROPERTY, MATERIAL_PROPERTY tortuosity and dispersivity, REGION,
+    TRANSPORT_CONDITION, INITIAL_CONDITION, BOUNDARY_CONDITION, STRATA and
+    the OUTPUT times that make waypoints."""
+    nxyz: Optional[Tuple[int, int, int]] = None
+    bounds: Optional[Tuple[Tuple[float, float, float], Tuple[float, float, float]]] = None
+    velocity: Tuple[float, float, float] = (0.0, 0.0, 0.0)   # m/s Darcy (UNIFORM_VELOCITY)
+    diffusion: float = 1e-9                       # m^2/s, fluid.F90:57
+    tortuosity: Dict[str, float] = field(default_factory=dict)      # material -> (default 1)
+    dispersivity: Dict[str, float] = field(default_factory=dict)    # material -> longitudinal (m)
+    regions: Dict[str, RegionCard] = field(default_factory=dict)
+    conditions: Dict[str, TransportConditionCard] = field(default_factory=dict)
+    initial: List[Tuple[str, str]] = field(default_factory=list)    # (transport condition, region)
+    boundaries: List[Tuple[str, str]] = field(default_factory=list)
+    strata: List[Tuple[str, str]] = field(default_factory=list)     # (region, material)
+    output_times: List[float] = field(default_factory=list)         # s
+    output_periods: List[float] = field(default_factory=list)       # s
+
+
+@dataclass
+class BoxSetup:
+    """A 1-D structured box along x ready for the global-implicit run."""
+    nx: int
+    dx: float
+    area: float                                   # face area dy dz (m^2)
+    porosity: float
+    tortuosity: float
+    dispersivity: float
+    diffusion: float
+    darcy: float                                  # m/s along +x
+    initial: str                                  # constraint of every cell
+    west: Optional[Tuple[str, str]] = None        # (condition type, constraint) of the x-min face
+    east: Optional[Tuple[str, str]] = None
+
+    @property
+    def volume(self) -> float:
+        return self.dx * self.area
+
+
+@dataclass
 class Deck:
     path: Optional[str]
     chemistry: Chemistry
@@ -244,6 +299,7 @@ class Deck:
     time: TimeCard
     temperature: float = 25.0
     numerical_jacobian: bool = False
+    transport: TransportCard = field(default_factory=TransportCard)
 
     # ---------------------------------------------------------------- builds
 
@@ -347,31 +403,85 @@ class Deck:
         dtt = tc.tfac[n - 1] * dt if (n <= tc.iaccel and n <= len(tc.tfac)) else 0.5 * dt
         return max(min(dtt, 2.0 * dt, dt_max), 1e-20)
 
-    def time_steps(self, newton_iterations: Optional[Callable[[int, float], int]] = None) -> List[float]:
-        """Step sizes from 0 to FINAL_TIME for a run whose steps are never
-        cut and whose only waypoint is the final time.  Each step is taken
-        at the current size, shortened to land on the final time when it
-        comes within 0.1 of a step of it (TimestepperBaseSetTargetTime,
-        timestepper_base.F90:320-521), and the size is then updated by
-        `next_dt` from the step's Newton count, which
-        `newton_iterations(step_index, dt)` returns (1 when not given: the
-        schedule of a run whose steps all converge within the accelerated
-        counts).  The first step is min(INITIAL, MAXIMUM)."""
+    def waypoints(self):
+        """Waypoints of the run (stepper.build_waypoints): the TIME card's
+        maximum step, the final time and the OUTPUT times."""
+        from .stepper import build_waypoints
         tc = self.time
         if tc.final is None or tc.dt_init is None:
             raise DeckError("TIME needs FINAL_TIME and INITIAL_TIMESTEP_SIZE")
         if tc.waypoint_dt:
             raise DeckError("MAXIMUM_TIMESTEP_SIZE ... at <time> waypoints are not modelled")
         dt_max = tc.dt_max if tc.dt_max is not None else tc.dt_init
-        dt, t, steps = min(tc.dt_init, dt_max), 0.0, []
-        while True:
-            if t + dt + 0.1 * dt >= tc.final:
-                steps.append(tc.final - t)
-                return steps
+        tr = self.transport
+        wps = build_waypoints(tc.final, dt_max, tr.output_times, tr.output_periods[0] if tr.output_periods else None)
+        for per in tr.output_periods[1:]:
+            more = build_waypoints(tc.final, dt_max, (), per)
+            merged = {w.time: w for w in wps}
+            for w in more:
+                if w.time in merged:
+                    merged[w.time].force = merged[w.time].force or w.force
+                else:
+                    merged[w.time] = w
+            wps = [merged[k] for k in sorted(merged)]
+        return wps
+
+    def stepper(self):
+        """The transport time-step controller of the run (stepper.TimeStepper)."""
+        from .stepper import TimeStepper
+        tc = self.time
+        wps = self.waypoints()
+        dt_max = tc.dt_max if tc.dt_max is not None else tc.dt_init
+        return TimeStepper(tc.final, tc.dt_init, dt_max, wps, iaccel=tc.iaccel, tfac=tuple(tc.tfac))
+
+    def time_steps(self, newton_iterations: Optional[Callable[[int, float], int]] = None) -> List[float]:
+        """Step sizes from 0 to FINAL_TIME for a run whose steps are never
+        cut (stepper.TimeStepper: waypoints, the final time, growth by
+        `next_dt` from each step's Newton count, which
+        `newton_iterations(step_index, dt)` returns; 1 when not given)."""
+        ts = self.stepper()
+        steps = []
+        while not ts.done:
+            dt = ts.set_target_time()
+            n = 1 if newton_iterations is None else newton_iterations(len(steps), dt)
             steps.append(dt)
-            t += dt
-            n = 1 if newton_iterations is None else newton_iterations(len(steps) - 1, dt)
-            dt = self.next_dt(dt, n)
+            ts.accept(dt, n, 0)
+        return steps
+
+    def box(self) -> BoxSetup:
+        """The deck's structured box as a 1-D column along x: one material,
+        one initial condition over the whole box, boundary conditions on the
+        WEST / EAST faces only (the faces pfx_grid carries), a uniform
+        velocity along x."""
+        tr = self.transport
+        if tr.nxyz is None or tr.bounds is None:
+            raise DeckError("GRID with NXYZ and BOUNDS (or DXYZ) expected")
+        nx, ny, nz = tr.nxyz
+        if ny != 1 or nz != 1:
+            raise DeckError("only columns along x (NXYZ n 1 1) are set up")
+        (x0, y0, z0), (x1, y1, z1) = tr.bounds
+        if abs(tr.velocity[1]) > 0.0 or abs(tr.velocity[2]) > 0.0:
+            raise DeckError("UNIFORM_VELOCITY across the column is not modelled")
+        mats = {m for _, m in tr.strata} or set(self.porosity)
+        if len(mats) != 1:
+            raise DeckError("one material expected")
+        mat = next(iter(mats))
+        if len(tr.initial) != 1:
+            raise DeckError("one INITIAL_CONDITION expected")
+        ic = tr.conditions[tr.initial[0][0]].constraint
+        faces = {}
+        for cname, rname in tr.boundaries:
+            reg = tr.regions.get(rname)
+            if reg is None or reg.face not in ("WEST", "EAST"):
+                raise DeckError(f"BOUNDARY_CONDITION on region {rname}: only WEST / EAST faces are modelled")
+            cond = tr.conditions[cname]
+            if cond.type not in ("DIRICHLET", "DIRICHLET_ZERO_GRADIENT", "ZERO_GRADIENT"):
+                raise DeckError(f"TRANSPORT_CONDITION {cname}: TYPE {cond.type} is not modelled")
+            faces[reg.face] = (cond.type, cond.constraint)
+        return BoxSetup(nx=nx, dx=(x1 - x0) / nx, area=(y1 - y0) * (z1 - z0), porosity=self.porosity[mat],
+                        tortuosity=tr.tortuosity.get(mat, 1.0), dispersivity=tr.dispersivity.get(mat, 0.0),
+                        diffusion=tr.diffusion, darcy=tr.velocity[0], initial=ic, west=faces.get("WEST"),
+                        east=faces.get("EAST"))
 
 
 # --------------------------------------------------------------------------
@@ -471,9 +581,9 @@ def _ion_exchange(cur: _Cursor) -> IonExchangeRxn:
     cations.insert(0, cations.pop(i))
     if cec is None:
         raise DeckError("ION_EXCHANGE_RXN without CEC")
-    if mineral is not None:
-        raise DeckError("ION_EXCHANGE_RXN on a mineral surface is outside the per-cell path")
-    return IonExchangeRxn(cations, cec)
+    # on a mineral surface the site density is CEC times the mineral's
+    # volume fraction (reaction.F90:4356-4363)
+    return IonExchangeRxn(cations, cec, mineral)
 
 
 def _chemistry(cur: _Cursor) -> Chemistry:
@@ -601,10 +711,131 @@ def _time(cur: _Cursor) -> TimeCard:
     return tc
 
 
+def _xyz(toks: List[str], where: str) -> Tuple[float, float, float]:
+    if len(toks) < 3:
+        raise DeckError(f"{where}: three coordinates expected")
+    return (_num(toks[0]), _num(toks[1]), _num(toks[2]))
+
+
+def _grid(cur: _Cursor, tr: TransportCard) -> None:
+    """GRID (grid.F90 GridRead): a structured box by NXYZ and BOUNDS (or a
+    uniform DXYZ)."""
+    dxyz = None
+    for t in cur.block("GRID"):
+        w = t[0].upper()
+        if w == "TYPE":
+            if len(t) < 2 or t[1].upper() != "STRUCTURED":
+                raise DeckError("GRID: only TYPE structured boxes are read")
+        elif w == "NXYZ":
+            tr.nxyz = (int(_num(t[1])), int(_num(t[2])), int(_num(t[3])))
+        elif w == "BOUNDS":
+            lo = _xyz(cur.next("BOUNDS"), "BOUNDS")
+            hi = _xyz(cur.next("BOUNDS"), "BOUNDS")
+            cur.skip_block("BOUNDS")
+            tr.bounds = (lo, hi)
+        elif w == "DXYZ":
+            vals = []
+            for s_ in cur.block("DXYZ"):
+                vals.append(s_)
+            if len(vals) != 3 or any(len(v) != 1 for v in vals):
+                raise DeckError("GRID DXYZ: only one uniform spacing per direction is read")
+            dxyz = tuple(_num(v[0]) for v in vals)
+        elif w == "ORIGIN":
+            pass
+        else:
+            raise DeckError(f"GRID: {w} is not read")
+    if dxyz is not None and tr.nxyz is not None:
+        tr.bounds = ((0.0, 0.0, 0.0), tuple(n * d for n, d in zip(tr.nxyz, dxyz)))
+
+
+def _region(cur: _Cursor, name: str) -> RegionCard:
+    """REGION (region.F90 RegionRead): FACE and COORDINATES."""
+    reg = RegionCard(name)
+    for t in cur.block(f"REGION {name}"):
+        w = t[0].upper()
+        if w == "FACE":
+            reg.face = t[1].upper()
+        elif w == "COORDINATES":
+            a = _xyz(cur.next("COORDINATES"), "COORDINATES")
+            b = _xyz(cur.next("COORDINATES"), "COORDINATES")
+            cur.skip_block("COORDINATES")
+            reg.coordinates = (a, b)
+        elif w == "COORDINATE":
+            reg.coordinates = (_xyz(t[1:], "COORDINATE"),) * 2
+        else:
+            raise DeckError(f"REGION {name}: {w} is not read")
+    return reg
+
+
+def _transport_condition(cur: _Cursor, name: str) -> TransportConditionCard:
+    """TRANSPORT_CONDITION (condition.F90 TranConditionRead): TYPE and the
+    first entry of CONSTRAINT_LIST (a time-varying list is refused)."""
+    typ, cons = None, []
+    for t in cur.block(f"TRANSPORT_CONDITION {name}"):
+        w = t[0].upper()
+        if w == "TYPE":
+            typ = t[1].upper()
+        elif w == "CONSTRAINT_LIST":
+            for s_ in cur.block("CONSTRAINT_LIST"):
+                cons.append((_num(s_[0]), s_[1]))
+        elif w == "CONSTRAINT":
+            raise DeckError(f"TRANSPORT_CONDITION {name}: inline CONSTRAINT blocks are not read")
+        elif w == "TIME_UNITS":
+            pass
+        else:
+            raise DeckError(f"TRANSPORT_CONDITION {name}: {w} is not read")
+    if typ is None or not cons:
+        raise DeckError(f"TRANSPORT_CONDITION {name}: TYPE and CONSTRAINT_LIST expected")
+    if len({c for _, c in cons}) > 1:
+        raise DeckError(f"TRANSPORT_CONDITION {name}: constraints that change in time are not modelled")
+    return TransportConditionCard(name, typ, cons[0][1])
+
+
+def _coupler(cur: _Cursor, where: str) -> Tuple[str, str]:
+    """INITIAL_CONDITION / BOUNDARY_CONDITION: (transport condition, region)."""
+    cond = reg = None
+    for t in cur.block(where):
+        w = t[0].upper()
+        if w == "TRANSPORT_CONDITION":
+            cond = t[1]
+        elif w == "REGION":
+            reg = t[1]
+        elif w == "FLOW_CONDITION":
+            pass
+        else:
+            raise DeckError(f"{where}: {w} is not read")
+    if cond is None or reg is None:
+        raise DeckError(f"{where}: TRANSPORT_CONDITION and REGION expected")
+    return cond, reg
+
+
+def _output(cur: _Cursor, tr: TransportCard, depth: int = 0) -> None:
+    """OUTPUT (output.F90:225-300 and factory_subsurface.F90:2270-2370): the
+    times that become waypoints with output (TIMES, PERIODIC TIME,
+    PERIODIC_OBSERVATION TIME), in the card or its SNAPSHOT_FILE,
+    OBSERVATION_FILE and MASS_BALANCE_FILE blocks; the rest is output
+    format and is not read."""
+    for t in cur.block("OUTPUT"):
+        w = t[0].upper()
+        if w == "TIMES":
+            f = units_factor(t[1], "s")
+            vals = [_num(x) for x in t[2:]]
+            tr.output_times += [v * f for v in vals]
+        elif w in ("PERIODIC", "PERIODIC_OBSERVATION") and len(t) > 1 and t[1].upper() == "TIME":
+            if len(t) > 4 and t[4].lower() == "between":
+                raise DeckError("OUTPUT PERIODIC TIME ... between is not modelled")
+            tr.output_periods.append(_num(t[2]) * units_factor(t[3], "s"))
+        elif w in ("SNAPSHOT_FILE", "OBSERVATION_FILE", "MASS_BALANCE_FILE") and depth == 0:
+            _output(cur, tr, depth + 1)
+        elif w in ("VARIABLES", "AVERAGE_VARIABLES"):
+            cur.skip_block(w)
+
+
 def parse_deck(text: str, path: Optional[str] = None) -> Deck:
     cur = _Cursor(_lines(text, os.path.dirname(os.path.abspath(path)) if path else None))
     chem, constraints, porosity = None, {}, {}
     tc, temperature, numjac, stepper = TimeCard(), 25.0, False, {}
+    tr = TransportCard()
     while cur.i < len(cur.lines):
         t = cur.next("deck")
         w = t[0].upper()
@@ -630,6 +861,10 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
                     if len(s) > 2 and s[1].upper() == "DATASET":
                         raise DeckError(f"MATERIAL_PROPERTY {name}: POROSITY DATASET is not read")
                     porosity[name] = _num(s[1])
+                elif v == "TORTUOSITY" and depth == 0:
+                    tr.tortuosity[name] = _num(s[1])
+                elif v == "LONGITUDINAL_DISPERSIVITY" and depth == 0:
+                    tr.dispersivity[name] = _num(s[1])
         elif w == "SUBSURFACE_TRANSPORT":
             for s in cur.block("SUBSURFACE_TRANSPORT"):
                 v = s[0].upper()
@@ -650,11 +885,41 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
                     stepper["tfac"] = tuple(_num(x) for x in s[1:])
         elif w == "REFERENCE_TEMPERATURE":
             temperature = _num(t[1])
+        elif w == "GRID":
+            _grid(cur, tr)
+        elif w == "UNIFORM_VELOCITY":
+            v = _xyz(t[1:], "UNIFORM_VELOCITY")
+            f = units_factor(t[4], "m/s") if len(t) > 4 else 1.0
+            tr.velocity = tuple(x * f for x in v)
+        elif w == "FLUID_PROPERTY":
+            for s_ in cur.block("FLUID_PROPERTY"):
+                if s_[0].upper() in ("DIFFUSION_COEFFICIENT", "LIQUID_DIFFUSION_COEFFICIENT"):
+                    tr.diffusion = _num(s_[1])
+        elif w == "REGION" and len(t) > 1:
+            tr.regions[t[1]] = _region(cur, t[1])
+        elif w == "TRANSPORT_CONDITION" and len(t) > 1:
+            tr.conditions[t[1]] = _transport_condition(cur, t[1])
+        elif w == "INITIAL_CONDITION":
+            tr.initial.append(_coupler(cur, "INITIAL_CONDITION"))
+        elif w == "BOUNDARY_CONDITION":
+            tr.boundaries.append(_coupler(cur, "BOUNDARY_CONDITION"))
+        elif w == "STRATA":
+            reg = mat = None
+            for s_ in cur.block("STRATA"):
+                if s_[0].upper() == "REGION":
+                    reg = s_[1]
+                elif s_[0].upper() == "MATERIAL":
+                    mat = s_[1]
+            tr.strata.append((reg, mat))
+        elif w == "OUTPUT":
+            _output(cur, tr)
+        elif w in ("OBSERVATION", "SOURCE_SINK", "INTEGRAL_FLUX"):
+            cur.skip_block(w)   # their REGION lines are not region cards
     if chem is None:
         raise DeckError("deck has no CHEMISTRY card")
     for k, v in stepper.items():
         setattr(tc, k, v)
-    return Deck(path, chem, constraints, porosity, tc, temperature, numjac)
+    return Deck(path, chem, constraints, porosity, tc, temperature, numjac, tr)
 
 
 def read_deck(path: str) -> Deck:

@@ -20,15 +20,18 @@ for name in ("carbonate.dat", "ca-carbonate.dat", "calcite.dat"):
     shutil.copy(os.path.join(BATCH, name), os.path.join(HERE, "db", name))
 
 # input decks of those tests (data: the inputs whose outputs the golds
-# hold), and the 1-D calcite column deck for its mineral, gas, pH and
-# charge-balance constraints
+# hold)
 os.makedirs(os.path.join(HERE, "decks"), exist_ok=True)
 for name in ("carbonate-unit-activity", "carbonate-debye-huckel-activity",
              "ca-carbonate-unit-activity", "ca-carbonate-debye-huckel-activity",
              "calcite-kinetics", "calcite-kinetics-volume-fractions", "ion-exchange-valocchi"):
     shutil.copy(os.path.join(BATCH, f"{name}.in"), os.path.join(HERE, "decks", f"{name}.in"))
-shutil.copy(os.path.join(REF, "regression_tests", "ascem", "1d", "1d-calcite", "1d-calcite.in"),
-            os.path.join(HERE, "decks", "1d-calcite.in"))
+# global-implicit column decks whose golds pin the device time loop
+COLUMNS = {"1d-calcite": os.path.join("ascem", "1d", "1d-calcite"),
+           "calcite_tran_only": os.path.join("shortcourse", "1D_Calcite"),
+           "valocchi_ionx": os.path.join("default", "column")}
+for name, sub in COLUMNS.items():
+    shutil.copy(os.path.join(REF, "regression_tests", sub, f"{name}.in"), os.path.join(HERE, "decks", f"{name}.in"))
 
 # excerpt of database/hanford.dat: the species of the 1-D calcite columns
 # (5 primaries, 8 complexes, calcite; CaOH+ and the CO2(g) constraint of the
@@ -73,4 +76,11 @@ for test in ("carbonate-unit-activity", "carbonate-debye-huckel-activity",
 golds["_source"] = "regression_tests/ascem/batch/*.regression.gold (tolerance 1e-12 absolute, batch.cfg)"
 with open(os.path.join(HERE, "batch_golds.json"), "w") as f:
     json.dump(golds, f, indent=1, sort_keys=True)
+
+columns = {name: parse_gold(os.path.join(REF, "regression_tests", sub, f"{name}.regression.gold"))
+           for name, sub in COLUMNS.items()}
+columns["_source"] = ("regression_tests/{ascem/1d/1d-calcite,shortcourse/1D_Calcite,default/column}/"
+                      "*.regression.gold (tolerance 1e-12 absolute in their .cfg files)")
+with open(os.path.join(HERE, "column_golds.json"), "w") as f:
+    json.dump(columns, f, indent=1, sort_keys=True)
 print("fixtures written to", HERE)
