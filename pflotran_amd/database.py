@@ -269,6 +269,16 @@ class MineralKinetics:
     activation_energy: float = 0.0  # J/mol
     affinity_threshold: float = 0.0
     rate_limiter: float = 0.0
+    affinity_power: Optional[float] = None   # beta (AFFINITY_POWER)
+    temkin: Optional[float] = None           # sigma (TEMKIN_CONSTANT)
+
+
+def _optional_per_mineral(vals):
+    """An optional per-mineral coefficient: absent (None) unless a mineral
+    sets it, then set for all with 1 for those that do not."""
+    if all(v is None for v in vals):
+        return None
+    return np.array([1.0 if v is None else float(v) for v in vals], dtype=np.float64)
 
 
 @dataclass
@@ -316,6 +326,11 @@ class ReactionTables:
     max_dlnC: float = 5.0
     max_rel_change_tol: float = 1e-6
     max_residual_tol: float = 1e-12
+    # AFFINITY_POWER / TEMKIN_CONSTANT: allocated for every kinetic mineral
+    # when any mineral sets one, 1 for the others (reaction_database.F90
+    # tables built in BasisInit); None when no mineral sets it
+    mnrl_affinity_power: Optional[np.ndarray] = None
+    mnrl_temkin: Optional[np.ndarray] = None
     _desc: Optional[abi.ReactionDesc] = None
 
     @property
@@ -363,8 +378,8 @@ class ReactionTables:
         d.kinmnrl_activation_energy = dp(self.mnrl_activation_energy)
         d.kinmnrl_affinity_threshold = dp(self.mnrl_affinity_threshold)
         d.kinmnrl_rate_limiter = dp(self.mnrl_rate_limiter)
-        d.kinmnrl_affinity_power = dp(None)
-        d.kinmnrl_Temkin_const = dp(None)
+        d.kinmnrl_affinity_power = dp(self.mnrl_affinity_power)
+        d.kinmnrl_Temkin_const = dp(self.mnrl_temkin)
         d.eqionx_rxn_cationid, d.eqionx_rxn_k = ip(self.ionx_cationid), dp(self.ionx_k)
         d.eqionx_rxn_CEC, d.eqionx_rxn_to_surf = dp(self.ionx_CEC), ip(self.ionx_to_surf)
         d.debyeA, d.debyeB, d.debyeBdot = self.debye
@@ -515,6 +530,8 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
         mnrl_activation_energy=np.array([minerals[n].activation_energy for n in mnames], dtype=np.float64),
         mnrl_affinity_threshold=np.array([minerals[n].affinity_threshold for n in mnames], dtype=np.float64),
         mnrl_rate_limiter=np.array([minerals[n].rate_limiter for n in mnames], dtype=np.float64),
+        mnrl_affinity_power=_optional_per_mineral([minerals[n].affinity_power for n in mnames]),
+        mnrl_temkin=_optional_per_mineral([minerals[n].temkin for n in mnames]),
         ionx_cationid=ionx_id, ionx_k=ionx_k,
         ionx_CEC=np.array([rx.CEC for rx in ion_exchange], dtype=np.float64),
         ionx_to_surf=np.array([mnames.index(rx.mineral) if rx.mineral else -1 for rx in ion_exchange],

@@ -546,6 +546,10 @@ def _mineral_kinetics(cur: _Cursor) -> Dict[str, MineralKinetics]:
                 k.affinity_threshold = _num(s[1])
             elif w == "RATE_LIMITER":
                 k.rate_limiter = _num(s[1])
+            elif w == "AFFINITY_POWER":
+                k.affinity_power = _num(s[1])
+            elif w == "TEMKIN_CONSTANT":
+                k.temkin = _num(s[1])
             else:
                 raise DeckError(f"MINERAL_KINETICS {name}: {w} is outside the per-cell path")
         if not np.isfinite(k.rate_constant):

@@ -374,3 +374,48 @@ def test_gpu_sorbed_total_constraints(golds):
     rc, _ = orc.equilibrate(t, setup["ctype"], np.array(setup["conc"]), ref, synth.react_opts(porosity=setup["porosity"]))
     assert rc == 0
     assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
+
+
+def _with_mineral_options(t):
+    """Every rate-law option of RKineticMineral on: mineral 0 with an
+    activation energy, a precipitation threshold and a rate limiter,
+    mineral 1 with an affinity power and a Temkin constant, mineral 2 with
+    both defaults of the allocated arrays (1)."""
+    t.mnrl_activation_energy[:] = [45000.0, 0.0, 20000.0]
+    t.mnrl_affinity_threshold[:] = [2.0, 0.0, 0.0]
+    t.mnrl_rate_limiter[:] = [0.5, 0.0, 0.0]
+    t.mnrl_affinity_power = np.array([1.0, 1.5, 1.0])
+    t.mnrl_temkin = np.array([1.0, 2.0, 1.0])
+    return t
+
+
+def test_mineral_option_parity():
+    # the kernel's rate-law branches against the oracle's (which the
+    # closed-form and finite-difference tests pin): residual, Jacobian and
+    # rates at a perturbed iterate, then the reaction step's free ions
+    t = _with_mineral_options(synth.random_network(15, 50, 3, ionx=True))
+    n = 400
+    cells, _, _, _ = speciated_state(t, n)
+    cells["pri_molal"][:] *= np.exp(0.3 * np.sin(np.arange(n * t.N)).reshape(n, t.N))
+    o = synth.react_opts()
+    res_o, jac_o = orc.residual_jacobian(t, cells, o)
+    rate_o = cells["mnrl_rate"].copy()
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    res_g, jac_g = runtime.residual_jacobian(tab, dc, synth.react_opts())
+    rate_g = dc.to_host()["mnrl_rate"]
+    assert np.count_nonzero(rate_o) > n and np.all(np.abs(rate_g - rate_o) <= 1e-12 * np.abs(rate_o))
+    assert np.all(np.abs(jac_g - jac_o) <= _jacobian_bound(t, cells, jac_o))
+    psv = 0.25 * 1.0 * 1000.0 * 1.0
+    acc = psv * np.abs(cells["total"]) / synth.STEP_DT
+    assert np.max(np.abs(res_g - res_o) / (acc + np.abs(res_o) + 1e-300)) <= 1e-12
+    # the reaction step from the previous free ions
+    start, _, _, _ = speciated_state(t, n)
+    ref = start.copy()
+    orc.react_batch(t, ref, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
+    dr = runtime.DeviceCells.from_host(host_arrays(start), n)
+    runtime.react_batch(tab, dr, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
+    out = dr.to_host()
+    assert np.array_equal(out["status"], ref["status"]) and (ref["status"] == 0).mean() > 0.95
+    ok = ref["status"] == 0
+    assert rel_err(out["pri_molal"][ok], ref["pri_molal"][ok]) <= 1e-10
