@@ -304,6 +304,23 @@ int pfx_tables_timing(pfx_tables *tables, double *ms);
 int pfx_react_batch(pfx_tables *tables, pfx_cells *cells,
                     const pfx_react_opts *opts, pfx_stats *stats);
 
+/* No counterpart in the reference.  pfx_react_batch for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc): the work-queue counters, the
+   statistics reduction and, when cells->num_iterations, status or rel_res is
+   NULL, the per-cell iteration counts, status and scaled residuals live in
+   a fixed device array of the library (PFX_NOALLOC_SCRATCH_BYTES on each
+   GPU, held until the process ends; its address is taken at
+   pfx_tables_create), never in memory allocated by the call.  Limit: with
+   any of those per-cell outputs NULL, at most PFX_NOALLOC_MAX_CELLS cells
+   (PFX_E_ARG beyond).  Every call on one GPU shares the array, which keeps
+   the last call's contents: calls that overlap on one GPU race.
+   opts->stream_ordered must be 0 (the statistics are read back before the
+   call returns).  Otherwise identical to pfx_react_batch. */
+#define PFX_NOALLOC_MAX_CELLS (1 << 20)
+#define PFX_NOALLOC_SCRATCH_BYTES (1024 + 16 * PFX_NOALLOC_MAX_CELLS)
+int pfx_react_batch_noalloc(pfx_tables *tables, pfx_cells *cells, const pfx_react_opts *opts, pfx_stats *stats);
+
 /* Statistics of the last pfx_react_batch made with opts->stream_ordered = 1
    on these tables: waits for that call's stream, fills stats (kernel_ms is 0:
    a stream-ordered call records no events) and returns the number of cells

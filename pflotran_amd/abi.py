@@ -26,6 +26,7 @@ PFX_CONSTRAINT_TOTAL, PFX_CONSTRAINT_FREE, PFX_CONSTRAINT_LOG, PFX_CONSTRAINT_PH
 PFX_CONSTRAINT_TOTAL_SORB = 5
 PFX_CONSTRAINT_MINERAL, PFX_CONSTRAINT_GAS = 6, 7
 PFX_JAC_CAPPED_QK = 1
+PFX_NOALLOC_MAX_CELLS = 1 << 20
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -49,7 +49,23 @@ struct pfx_tables {
   int num_cu = 0;
 };
 
+// ---------------------------------------------------------------------------
+// Fixed scratch of pfx_react_batch_noalloc: the work-queue cursor and fallback
+// counter, the statistics reduction and, for callers that pass no per-cell
+// outputs, the per-cell iteration counts, status and scaled residuals.  Each
+// GPU holds its own copy until the process ends; every call on a GPU shares
+// it (overlapping calls on one GPU race).
+constexpr size_t kNoallocQueueOff = 0;      // uint32 [2]
+constexpr size_t kNoallocStatsOff = 256;    // uint64 [64]
+constexpr size_t kNoallocCellsOff = 1024;   // int32 iters [cap], int32 status [cap], double rel_res [cap]
+static __device__ __align__(256) unsigned char scratch[PFX_NOALLOC_SCRATCH_BYTES];
+static_assert(kNoallocCellsOff + (size_t)PFX_NOALLOC_MAX_CELLS * 16 <= PFX_NOALLOC_SCRATCH_BYTES,
+              "scratch too small for its per-cell slots");
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+unsigned char *g_scratch_ptr[kMaxDevices] = {nullptr};
 
 constexpr int BLOCK = 256;
 constexpr int64_t kMaxLds = 160 * 1024;   // gfx950 LDS per workgroup
@@ -66,6 +82,17 @@ int g_last_hip_error = 0;
       return PFX_E_DEVICE;                                          \
     }                                                               \
   } while (0)
+
+// hipGetSymbolAddress of the scratch array on `device` (current), once
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return PFX_E_UNSUP;
+  if (g_scratch_ptr[device]) return 0;
+  void *ptr = nullptr;
+  HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return PFX_E_DEVICE;
+  g_scratch_ptr[device] = (unsigned char *)ptr;
+  return 0;
+}
 
 int group_size(int N) {
   if (N <= 8) return 8;
@@ -186,7 +213,11 @@ const void *zones_kernel_ptr(int v) {
        : v == 1 ? (const void *)pfx_react_zones_kernel<2> : (const void *)pfx_react_zones_kernel<1>;
 }
 
-int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, double *ms) {
+// queue: the work-queue cursor and fallback counter of this launch (the
+// tables' own, or the fixed scratch array of pfx_react_batch_noalloc)
+int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, double *ms,
+           unsigned int *queue = nullptr) {
+  if (!queue) queue = t->d_queue;
   const int G = t->L.G;
   const int groups = BLOCK / G;
   const int64_t lds = t->L.lds_table_bytes + (int64_t)groups * pfx_scr_doubles(t->L.M, G, t->L.K) * 8;
@@ -203,8 +234,9 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   // to balance the tail
   int64_t per_wave = std::max<int64_t>(1, P.ncells / (blocks * (BLOCK / 64)));
   P.chunk = (int32_t)std::max<int64_t>(64 / G, std::min<int64_t>(64, per_wave / 8 + 1));
-  P.queue = t->d_queue;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, t->d_queue, 2);  // cursor, fallbacks
+  P.queue = queue;
+  P.fallback_count = queue + 1;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, queue, 2);  // cursor, fallbacks
   HIPCHK(hipGetLastError());
   void *args[] = {(void *)&P};
   if (!ms) {
@@ -284,21 +316,25 @@ int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o
 }
 
 // statistics reduction into t->d_stats (kernels only: capturable)
-int stats_launch(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream_t stream) {
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, t->d_stats, 64);
+int stats_launch(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream_t stream,
+                 unsigned long long *sbuf = nullptr) {
+  if (!sbuf) sbuf = t->d_stats;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, sbuf, 64);
   int sblocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
   hipLaunchKernelGGL(k_stats, dim3(sblocks), dim3(256), 0, stream, P.num_iterations, P.status, P.rel_res, n,
-                     t->d_stats);
+                     sbuf);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
 int stats_read(pfx_tables *t, const pfx_kernel_params &P, int64_t n, hipStream_t stream, double ms,
-               pfx_stats *stats) {
+               pfx_stats *stats, const unsigned long long *sbuf = nullptr, const unsigned int *queue = nullptr) {
+  if (!sbuf) sbuf = t->d_stats;
+  if (!queue) queue = t->d_queue;
   unsigned long long h[64];
   unsigned int nfb = 0;
-  HIPCHK(hipMemcpyAsync(h, t->d_stats, sizeof(h), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipMemcpyAsync(&nfb, t->d_queue + 1, sizeof(nfb), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(h, sbuf, sizeof(h), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(&nfb, queue + 1, sizeof(nfb), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
   const int64_t conv = (int64_t)h[0];
   if (stats) {
@@ -773,6 +809,15 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   HIPCHK(hipMalloc(&t->d_stats, 64 * sizeof(unsigned long long)));
   HIPCHK(hipEventCreate(&t->ev0));
   HIPCHK(hipEventCreate(&t->ev1));
+  {
+    // the device address of this GPU's fixed scratch array, once per device
+    // (the array's name is a host stand-in, not a device address)
+    int rc = noalloc_scratch_init(t->device);
+    if (rc) {
+      pfx_tables_destroy(t);
+      return rc;
+    }
+  }
   *out = t;
   return 0;
 }
@@ -837,6 +882,48 @@ extern "C" int pfx_tables_stats(pfx_tables *t, pfx_stats *stats) {
   t->pending_n = -1;
   int rc = stats_read(t, P, n, t->pending_stream, 0.0, stats);
   return rc;
+}
+
+extern "C" int pfx_react_batch_noalloc(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o,
+                                       pfx_stats *stats) {
+  if (!t) return PFX_E_ARG;
+  int rc = check_cells(t, cells, o);
+  if (rc) return rc;
+  if (o->max_iterations < 0 || o->extra_iterations < 0) return PFX_E_ARG;
+  if (o->stream_ordered) return PFX_E_ARG;   // the statistics are read back before returning
+  const bool per_cell_scratch = !cells->num_iterations || !cells->status || !cells->rel_res;
+  if (per_cell_scratch && cells->ncells > PFX_NOALLOC_MAX_CELLS) {
+    fprintf(stderr,
+            "pfx_react_batch_noalloc: %lld cells without per-cell outputs exceed the fixed scratch "
+            "(at most %d; pass num_iterations, status and rel_res for larger batches)\n",
+            (long long)cells->ncells, PFX_NOALLOC_MAX_CELLS);
+    return PFX_E_ARG;
+  }
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->ncells = cells->ncells;
+  }
+  if (cells->ncells == 0) return 0;
+  if (t->device < 0 || t->device >= kMaxDevices || !g_scratch_ptr[t->device]) return PFX_E_DEVICE;
+  HIPCHK(hipSetDevice(t->device));
+  unsigned char *base = g_scratch_ptr[t->device];
+  unsigned int *queue = (unsigned int *)(base + kNoallocQueueOff);
+  unsigned long long *sbuf = (unsigned long long *)(base + kNoallocStatsOff);
+  hipStream_t stream = (hipStream_t)o->stream;
+  pfx_kernel_params P;
+  fill_params(t, cells, o, P);
+  if (per_cell_scratch) {
+    // per-cell slots: every slot a launch uses is written by it before the
+    // statistics read it, so nothing needs clearing
+    const int64_t cap = PFX_NOALLOC_MAX_CELLS;
+    if (!P.num_iterations) P.num_iterations = (int32_t *)(base + kNoallocCellsOff);
+    if (!P.status) P.status = (int32_t *)(base + kNoallocCellsOff + cap * 4);
+    if (!P.rel_res) P.rel_res = (double *)(base + kNoallocCellsOff + cap * 8);
+  }
+  double ms = 0.0;
+  if ((rc = launch(t, P, PFX_MODE_REACT, stream, &ms, queue))) return rc;
+  if ((rc = stats_launch(t, P, cells->ncells, stream, sbuf))) return rc;
+  return stats_read(t, P, cells->ncells, stream, ms, stats, sbuf, queue);
 }
 
 extern "C" int pfx_residual_jacobian(pfx_tables *t, pfx_cells *cells, const pfx_react_opts *o, double *residual,

@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpfx_chem.so")
 
 EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_react_batch", "pfx_tables_stats",
+            "pfx_react_batch_noalloc",
             "pfx_residual_jacobian", "pfx_equilibrate_batch", "pfx_auxvar_batch", "pfx_update_kinetic_state",
             "pfx_gi_structure", "pfx_gi_residual_jacobian", "pfx_tables_timing", "pfx_gi_dispersion",
             "pfx_react_zones", "pfx_zone_plan_rebalance", "pfx_zones_graph_create", "pfx_graph_launch",
@@ -57,6 +58,7 @@ def load_library(path: str = LIB_PATH):
     lib.pfx_tables_destroy.argtypes = [C.c_void_p]
     lib.pfx_react_batch.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), P(abi.Stats)]
     lib.pfx_tables_stats.argtypes = [C.c_void_p, P(abi.Stats)]
+    lib.pfx_react_batch_noalloc.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), P(abi.Stats)]
     lib.pfx_residual_jacobian.argtypes = [C.c_void_p, P(abi.Cells), P(abi.ReactOpts), C.c_void_p, C.c_void_p]
     lib.pfx_equilibrate_batch.argtypes = [C.c_void_p, P(abi.Cells), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
                                           P(abi.ReactOpts), P(abi.Stats)]
@@ -196,6 +198,18 @@ def react_batch(tables: Tables, cells: DeviceCells, opts: abi.ReactOpts):
     if rc < 0:
         raise PfxError(f"pfx_react_batch failed: {rc}")
     return rc, st
+
+
+def react_batch_noalloc(tables: Tables, cells: DeviceCells, opts: abi.ReactOpts):
+    """pfx_react_batch_noalloc (no device allocation in the call): returns
+    (rc, stats); rc < 0 is returned, not raised."""
+    lib = load_library()
+    torch = _torch()
+    cs = cells.struct()
+    st = abi.Stats()
+    if opts.stream is None:
+        opts.stream = torch.cuda.current_stream().cuda_stream
+    return lib.pfx_react_batch_noalloc(tables.handle, C.byref(cs), C.byref(opts), C.byref(st)), st
 
 
 def residual_jacobian(tables: Tables, cells: DeviceCells, opts: abi.ReactOpts):
