@@ -78,6 +78,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="plain method only")
     ap.add_argument("--no-gi", action="store_true", help="skip the global-implicit assembly leg")
     ap.add_argument("--no-zones", action="store_true", help="skip the chemistry-zones leg")
+    ap.add_argument("--no-column", action="store_true", help="skip the 100-cell operator-split column leg")
     ap.add_argument("--no-capped", action="store_true",
                     help="skip the legs with the consistent derivative above the Q/K cap")
     ap.add_argument("--zones-scale", type=float, default=1.0, help="fraction of the 2M zoned cells")
@@ -311,6 +312,8 @@ def main():
         line["gi"] = gi_leg(t, tab, args, log)
     if not args.no_zones:
         line["zones"] = zones_leg(args, log)
+    if not args.no_column:
+        line["column_os"] = column_leg(args, log)
     if args.calibrate_copy:
         # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
         # calibrate the byte counters (256 MiB read + 256 MiB written)
@@ -463,7 +466,136 @@ def gi_leg(t, tab, args, log):
         f"reason {ks.reason}")
     del dc, snap
     torch.cuda.empty_cache()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_gi(t, tab, log)
     return out
+
+
+def cpu_baseline_gi(t, tab, log, side=16):
+    """The global-implicit assembly against the oracle on a side^3 box
+    drawn from the same distributions as the 128^3 grid (cells, log-normal
+    permeability, boundary): relative errors of the residual and of the BSR
+    blocks, and the oracle's assembly rate on every host core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import torch
+
+    import oracle as orc
+    from pflotran_amd import runtime, synth, workload
+    n = side ** 3
+    dc, _ = workload.speciated_cells(t, tab, n, workload_id=5)
+    bc_total = dc.t["total"].mean(dim=0).cpu().numpy()
+    gd = workload.gi_grid(side, side, side, bc_total=bc_total)
+    g = runtime.DeviceGrid(**gd)
+    rp, ci = runtime.gi_structure(g)
+    N = t.N
+    res = torch.empty((n, N), dtype=torch.float64, device="cuda")
+    vals = torch.empty((int(ci.numel()), N, N), dtype=torch.float64, device="cuda")
+    runtime.gi_residual_jacobian(tab, dc, g, synth.react_opts(), rp, res, vals)
+    host = workload.host_arrays(dc)
+    cells = orc.HostCells(host, n)
+    hg = orc.HostGrid(gd["nx"], gd["ny"], gd["nz"], gd["q"], gd["dd"], area=gd["area"], bc_q=gd["bc_q"],
+                      bc_dd=gd["bc_dd"], bc_total=gd["bc_total"], log_formulation=gd["log_formulation"])
+    threads = cpu_threads()
+    t0 = time.perf_counter()
+    r_o, v_o, _, _ = orc.gi_residual_jacobian(t, cells, hg, synth.react_opts(), nthreads=threads)
+    cpu_s = time.perf_counter() - t0
+    r_g = res.cpu().numpy()
+    v_g = np.ascontiguousarray(vals.cpu().numpy().transpose(0, 2, 1))   # [block][row][col] as the oracle's
+    acc = np.abs(r_o) + 0.25 * 1000.0 * np.abs(host["total"]) / synth.STEP_DT
+    rowmax = np.max(np.abs(v_o), axis=2, keepdims=True)
+    out = {"box": f"{side}^3 box drawn from the same distributions", "cores": threads, "kind": "port",
+           "oracle_assembly_cells_per_s": n / cpu_s,
+           "residual_max_rel_err": float(np.max(np.abs(r_g - r_o) / acc)),
+           "jacobian_max_err_rel_to_row_max": float(np.max(np.abs(v_g - v_o) / np.maximum(rowmax, 1e-300)))}
+    log(f"gi vs oracle on {side}^3: {out}")
+    return out
+
+
+def column_leg(args, log):
+    """The CPU-runnable configuration: a 1-D column of 100 cells (dx 1 m,
+    porosity 0.25, Darcy flux 1 m/yr), 100 operator-split steps of 0.1 d,
+    carbonate chemistry (5 primaries, 8 complexes, kinetic calcite from the
+    hanford database excerpt), seeded totals log-uniform [1e-5, 1e-2] and pH
+    U[6, 9] speciated on the device.  Each step: the explicit upwind
+    transport of the totals, the batched reaction step from the previous free
+    ions, the volume-fraction update.  Timed on the device (the whole run,
+    launch-bound at 100 cells); the oracle's run of the same start on one
+    core is the CPU baseline beside it."""
+    import numpy as np
+    import torch
+
+    from pflotran_amd import abi, column, runtime, synth
+    from pflotran_amd.database import MineralKinetics, build_tables, read_database
+
+    db = read_database(os.path.join(ROOT, "tests", "golden", "db", "hanford_subset.dat"))
+    t = build_tables(db, ["H+", "HCO3-", "Ca++", "Mg++", "Cl-"],
+                     ["OH-", "CO2(aq)", "CO3--", "CaHCO3+", "CaCO3(aq)", "MgHCO3+", "MgCO3(aq)", "CaCl+"],
+                     {"Calcite": MineralKinetics(1e-9)}, act_freq=abi.PFX_ACT_TIMESTEP)
+    col = column.Column()
+    n, nsteps, dt = col.ncells, 100, 8640.0
+    tab = runtime.Tables(t)
+    ci = synth.random_cells(t, n + 1, workload_id=1, total_range=(1e-5, 1e-2), ph_range=(6.0, 9.0))
+    arr = synth.host_state_arrays(t, n + 1, with_temperature=False)
+    arr["mnrl_volfrac"][:, 0] = ci.vf[:, 0]
+    arr["mnrl_area"][:, 0] = 1.0
+    conc = ci.totals.copy()
+    conc[:, 0] = ci.ph
+    sp = runtime.DeviceCells.from_host(arr, n + 1)
+    rc, _ = runtime.equilibrate(tab, sp, [abi.PFX_CONSTRAINT_PH] + [abi.PFX_CONSTRAINT_TOTAL] * (t.N - 1), conc,
+                                synth.react_opts())
+    host = sp.to_host()
+    inflow = host["total"][n].copy()
+    start = {k: (None if v is None else v[:n].copy()) for k, v in host.items()}
+
+    def opts(step_dt):
+        return synth.react_opts(dt=step_dt, guess_mode=abi.PFX_GUESS_PREVIOUS, max_iterations=100)
+
+    gpu_ms = []
+    for _ in range(3):
+        dc = runtime.DeviceCells.from_host({k: (None if v is None else v.copy()) for k, v in start.items()}, n)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = column.os_steps_device(tab, dc, col, inflow, [dt] * nsteps, opts)
+        torch.cuda.synchronize()
+        gpu_ms.append((time.perf_counter() - t0) * 1e3)
+    out = dc.to_host()
+    g = float(np.median(gpu_ms))
+    res = {"workload": "100-cell 1-D column, dx 1 m, q 1 m/yr, 100 OS steps of 0.1 d, 5 primaries / 8 complexes / "
+                       "kinetic calcite, Debye-Hueckel",
+           "cell_steps": n * nsteps, "gpu_ms_run": g, "gpu_cell_steps_per_s": n * nsteps / (g * 1e-3),
+           "newton_iterations": int(sum(st.total_iterations for st in stats)),
+           "converged_every_step": bool(all(st.converged == n for st in stats)),
+           "note": "launch-bound: 100 cells per launch; transport is the explicit upwind step on both sides"}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_column(t, col, start, inflow, nsteps, dt, opts, out["pri_molal"])
+    log(f"column: GPU {g:.1f} ms for {nsteps} steps; {res.get('cpu_baseline')}")
+    return res
+
+
+def cpu_baseline_column(t, col, start, inflow, nsteps, dt, opts, m_gpu):
+    """The oracle's run of the column from the same start on one core
+    (median of 3), and the final free ions of the device run against it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as orc
+    from pflotran_amd import column
+    n = col.ncells
+    cpu_s = []
+    ref = None
+    for _ in range(3):
+        ref = orc.HostCells({k: (None if v is None else v.copy()) for k, v in start.items()}, n)
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            ref.arrays["total"] = np.ascontiguousarray(column.transport_step(col, ref["total"], inflow, dt))
+            orc.react_batch(t, ref, opts(dt), nthreads=1)
+            orc.update_kinetic_state(t, ref, opts(dt), nthreads=1)
+        cpu_s.append(time.perf_counter() - t0)
+    c = float(np.median(cpu_s))
+    rel = float(np.max(np.abs(m_gpu - ref["pri_molal"]) / ref["pri_molal"]))
+    return {"value": n * nsteps / c, "unit": "cell-steps/s", "cores": 1, "kind": "port", "seconds_run": c,
+            "free_ion_rel_err_gpu_vs_oracle": rel}
 
 
 def zones_leg(args, log):
@@ -512,9 +644,22 @@ def zones_leg(args, log):
     # starts from cells * N^2 and is rebalanced from the measured zone times
     plan, history = autotune.tune_zones(tabs, dcs, opts)
     best = {"variant": plan.variant, "blocks": list(plan.blocks[:4])}
+    # the tuned plan eagerly and as a graph replay, alternated, so the two
+    # see the same device state
     graph = runtime.ZonesGraph(tabs, dcs, opts, plan)
-    g_ms, k_ms = [], []
+    g_ms, k_ms, e_ms, ek_ms = [], [], [], []
     for _ in range(5):
+        restore()
+        torch.cuda.synchronize()
+        p2 = abi.ZonePlan()
+        p2.nzones, p2.variant = plan.nzones, plan.variant
+        for z in range(plan.nzones):
+            p2.blocks[z] = plan.blocks[z]
+        t1 = time.perf_counter()
+        runtime.react_zones(tabs, dcs, opts, p2)
+        torch.cuda.synchronize()
+        e_ms.append((time.perf_counter() - t1) * 1e3)
+        ek_ms.append(p2.kernel_ms)
         restore()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -532,15 +677,49 @@ def zones_leg(args, log):
            "ncells": n, "zone_cells": sizes, "near_saturation_fraction": inside,
            "graph_ms_per_step": float(np.mean(g_ms)), "graph_kernel_ms": float(np.mean(k_ms)),
            "cells_per_s": conv / (float(np.mean(g_ms)) * 1e-3),
-           "separate_launches_ms": float(np.mean(sep)), "autotune_history": history, "blocks": best["blocks"],
+           "separate_launches_ms": float(np.mean(sep)), "eager_tuned_plan_ms": float(np.mean(e_ms)),
+           "eager_tuned_plan_kernel_ms": float(np.mean(ek_ms)), "autotune_history": history, "blocks": best["blocks"],
            "variant": best["variant"], "graph_plan": {"variant": gp.variant, "blocks": list(gp.blocks[:4]),
                                                       "zone_ms": list(gp.zone_ms[:4])},
            "mean_iterations_per_zone": its, "iteration_histogram": hist.tolist(),
            "converged": int(conv)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_zones(nets, dcs, snaps, log)
     log(f"zones: graph {out['graph_ms_per_step']:.2f} ms/step (kernel {out['graph_kernel_ms']:.2f}), "
         f"separate {out['separate_launches_ms']:.2f} ms, its/zone {[round(x, 2) for x in its]}")
     del dcs, snaps
     torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline_zones(nets, dcs, snaps, log, sample=4000):
+    """Each zone's first `sample` cells by the oracle from the same start
+    (the inputs restored), against the device's free ions after the graph
+    replay; and the oracle's rate on every host core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as orc
+    from pflotran_amd import abi, synth
+    threads = cpu_threads()
+    out = {"cores": threads, "kind": "port", "sample_per_zone": sample, "zones": []}
+    for t, dc, sn in zip(nets, dcs, snaps):
+        k = min(sample, dc.ncells)
+        gpu_m = dc.t["pri_molal"][:k].cpu().numpy()
+        gpu_st = dc.t["status"][:k].cpu().numpy()
+        arr = {key: (None if v is None else v[:k].cpu().numpy().copy()) for key, v in dc.t.items()}
+        for key, v in sn.items():
+            arr[key] = v[:k].cpu().numpy().copy()
+        cells = orc.HostCells(arr, k)
+        t0 = time.perf_counter()
+        orc.react_batch(t, cells, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS), nthreads=threads)
+        sec = time.perf_counter() - t0
+        ok = (cells["status"] == 0) & (gpu_st == 0)
+        rel = np.max(np.abs(gpu_m[ok] - cells["pri_molal"][ok]) / cells["pri_molal"][ok], axis=1)
+        out["zones"].append({"N": t.N, "cells": k, "status_equal": bool(np.array_equal(gpu_st, cells["status"])),
+                             "free_ion_max_rel_err": float(rel.max()), "p99": float(np.quantile(rel, 0.99)),
+                             "cells_above_1e-10": int((rel > 1e-10).sum()), "oracle_cells_per_s": k / sec})
+    log(f"zones vs oracle: {out}")
     return out
 
 
@@ -610,7 +789,17 @@ def cpu_baseline(t, dc, snap, gpu_out, n, runs=5):
     _, J = orc.residual_jacobian(t, cells.copy(), o, fixed_accum=np.ascontiguousarray(fixed), nthreads=threads)
     Jl = J * cells["pri_molal"][:, None, :]
     floor = np.full(n, np.inf)
-    scale = np.abs(fixed) / o.dt
+    # each row's rounding is relative to the magnitude of its terms: the
+    # fixed accumulation, phi s rho V (m + sum |nu| C) and V |S|, over dt
+    absnu = np.zeros((t.M, t.N))
+    for i in range(t.M):
+        for s_ in range(t.cplx_specid[i, 0]):
+            absnu[i, t.cplx_specid[i, 1 + s_]] += abs(t.cplx_stoich[i, s_])
+    ev = cells.copy()
+    orc.residual_jacobian(t, ev, o, fixed_accum=np.ascontiguousarray(fixed), nthreads=threads)  # complexes
+    terms = ev["pri_molal"] + ev["sec_molal"][:, :t.M] @ absnu
+    scale = (o.porosity * o.saturation * o.den_kg * o.volume * terms + np.abs(fixed)
+             + np.abs(ev["total_sorb"]) * o.volume) / o.dt
     for i0 in range(0, n, 4096):
         blk = slice(i0, min(n, i0 + 4096))
         try:

@@ -337,11 +337,22 @@ __global__ __launch_bounds__(256) void k_surrogate_guess(const mlp_dev D, int64_
     for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int j = 0; j < 4; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    // weight fragments of step kt + 1 are loaded while step kt's products
+    // run: the fragments come from L2 and each block reuses them for only
+    // 64 cells, so their latency, not the matrix cores, bounds the layer
+    uint4 wa[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      wa[i] = i < ntw ? Wl[((long)(w * ntw + i) * kts + 0) * 64 + lane] : make_uint4(0, 0, 0, 0);
     for (int kt = 0; kt < kts; kt++) {
       bf16x8 a[4], b[4];
 #pragma unroll
-      for (int i = 0; i < 4; i++)
-        a[i] = i < ntw ? as_frag(Wl[((long)(w * ntw + i) * kts + kt) * 64 + lane]) : as_frag(make_uint4(0, 0, 0, 0));
+      for (int i = 0; i < 4; i++) a[i] = as_frag(wa[i]);
+      if (kt + 1 < kts) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (i < ntw) wa[i] = Wl[((long)(w * ntw + i) * kts + kt + 1) * 64 + lane];
+      }
 #pragma unroll
       for (int j = 0; j < 4; j++) b[j] = as_frag(*(const uint4 *)&in[(j * 16 + col) * ASTR + kt * 32 + kq]);
 #pragma unroll

@@ -130,11 +130,16 @@ def mineral_log_omega(t: ReactionTables, dc: runtime.DeviceCells, k: int = 0):
 
 
 def near_saturation_cells(t: ReactionTables, tab: runtime.Tables, n: int, seed: int, workload_id: int,
-                          fraction: float = 0.5, window: float = 0.05, passes: int = 3, oversample: float = 1.5):
-    """n speciated cells; the first `fraction` of them moved toward
-    |log Omega| <= window for mineral 0 by rescaling the total of its
-    dominant non-H+ primary and re-speciating (a few passes).  Returns the
-    cells, the final log Omega and the mask of cells inside the window."""
+                          fraction: float = 0.5, window: float = 0.05, passes: int = 3, oversample: float = 2.2):
+    """n speciated cells of which exactly round(fraction * n) lie within
+    |log Omega| <= window of mineral 0 and the rest are ordinary draws.
+    The first 60% of the oversampled draws are moved toward a target in the
+    window by rescaling the total of mineral 0's dominant non-H+ primary and
+    re-speciating (a few passes); after the initial aux-var refresh (two
+    activity passes, as the reaction step sees the cells) the cells are
+    taken in draw order: the first in-window cells of the moved draws and the
+    first cells of the others.  Returns the cells, their log Omega and the
+    mask of cells inside the window."""
     import numpy as np
     import torch
     n_draw = int(n * oversample) + 64
@@ -151,8 +156,8 @@ def near_saturation_cells(t: ReactionTables, tab: runtime.Tables, n: int, seed: 
     cand = [(abs(t.mnrl_stoich[0, s_]), int(t.mnrl_specid[0, 1 + s_]), t.mnrl_stoich[0, s_]) for s_ in range(ns)
             if int(t.mnrl_specid[0, 1 + s_]) != 0]
     _, jp, nu = max(cand)
-    near = np.zeros(n_draw, dtype=bool)
-    near[:int(fraction * n_draw)] = True
+    moved = np.zeros(n_draw, dtype=bool)
+    moved[:int(0.6 * n_draw)] = True
     target = synth.rng(workload_id, "perturb", seed).uniform(-window, window, size=n_draw)
     dc = None
     for it in range(passes + 1):
@@ -161,19 +166,25 @@ def near_saturation_cells(t: ReactionTables, tab: runtime.Tables, n: int, seed: 
         if it == passes:
             break
         lo = mineral_log_omega(t, dc)
-        shift = np.where(near & np.isfinite(lo), (target - lo) / nu, 0.0)
+        shift = np.where(moved & np.isfinite(lo), (target - lo) / nu, 0.0)
         conc[:, jp] *= 10.0 ** np.clip(shift, -2.0, 2.0)
         conc[:, jp] = np.clip(conc[:, jp], 1e-12, 1.0)
     m = dc.t["pri_molal"]
-    ok = (dc.t["status"] == 0) & torch.isfinite(m).all(dim=1) & (m > 0).all(dim=1)
-    idx = torch.nonzero(ok).flatten()[:n]
-    if idx.numel() < n:
-        raise runtime.PfxError(f"only {idx.numel()} of {n} cells speciated")
-    dc = subset(dc, idx)
+    ok = ((dc.t["status"] == 0) & torch.isfinite(m).all(dim=1) & (m > 0).all(dim=1)).cpu().numpy()
+    # the initial refresh of every draw, then the selection
     dc.t["pri_act_coef"].fill_(1.0)
     dc.t["sec_act_coef"].fill_(1.0)
     dc.t["ionx_ref_sorbed"].fill_(1e-9)
     runtime.auxvar(tab, dc, 2, synth.react_opts())
-    lo = mineral_log_omega(t, dc)
-    inside = near[idx.cpu().numpy()] & (np.abs(lo) <= window)
+    lo_all = mineral_log_omega(t, dc)
+    inside_all = ok & np.isfinite(lo_all) & (np.abs(lo_all) <= window)
+    n_near = int(round(fraction * n))
+    sel_near = np.nonzero(moved & inside_all)[0][:n_near]
+    sel_far = np.nonzero(~moved & ok)[0][:n - n_near]
+    if len(sel_near) < n_near or len(sel_far) < n - n_near:
+        raise runtime.PfxError(f"only {len(sel_near)} near-saturation and {len(sel_far)} other cells of {n}")
+    idx = np.sort(np.concatenate([sel_near, sel_far]))
+    dc = subset(dc, torch.as_tensor(idx, device=dc.t["pri_molal"].device))
+    lo = lo_all[idx]
+    inside = inside_all[idx] & moved[idx]
     return dc, lo, inside

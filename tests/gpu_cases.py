@@ -6,14 +6,31 @@ def host_arrays(cells):
     return {k: (None if v is None else v.copy()) for k, v in cells.arrays.items()}
 
 
+def row_term_scale(t, solved, fixed, o):
+    """Per cell and row, the magnitude of the terms summed into the residual
+    row (mol/s): the fixed accumulation, the free ion and every complex it
+    enters (|nu| C) times phi s rho V, and the sorbed total times V, over
+    dt; the denominator of the scaled residual."""
+    import numpy as np
+    absnu = np.zeros((t.M, t.N))
+    for i in range(t.M):
+        for s_ in range(t.cplx_specid[i, 0]):
+            absnu[i, t.cplx_specid[i, 1 + s_]] += abs(t.cplx_stoich[i, s_])
+    terms = solved["pri_molal"] + solved["sec_molal"][:, :t.M] @ absnu
+    sc = o.porosity * o.saturation * o.den_kg * o.volume * terms + np.abs(fixed)
+    if solved.arrays.get("total_sorb") is not None:
+        sc = sc + np.abs(solved["total_sorb"]) * o.volume
+    return sc / o.dt
+
+
 def conditioning_floor(t, solved, step_inputs, dt, nthreads=0):
     """Per cell, the FP64 resolution of the free ions at a solution: the
     residual of a row cannot be evaluated to better than the unit roundoff
-    times its largest term (the fixed accumulation over dt), and the Newton
-    system in ln m maps that to eps * max_k sum_j |J_log^-1|_kj |fixed_j|/dt
-    in ln m_k.  `solved` holds the solution (free ions and the step's
-    activity coefficients), `step_inputs` the step's inputs (totals and
-    sorbed totals of the previous state); opts as synth.react_opts."""
+    times the magnitude of its terms (row_term_scale), and the Newton system
+    in ln m maps that to eps * max_k sum_j |J_log^-1|_kj scale_j in ln m_k.
+    `solved` holds the solution (free ions, the step's activity
+    coefficients and the complexes), `step_inputs` the step's inputs (totals
+    and sorbed totals of the previous state); opts as synth.react_opts."""
     import numpy as np
 
     import oracle as orc
@@ -26,7 +43,7 @@ def conditioning_floor(t, solved, step_inputs, dt, nthreads=0):
     _, J = orc.residual_jacobian(t, ev, o, fixed_accum=np.ascontiguousarray(fixed), nthreads=nthreads)
     Jl = J * solved["pri_molal"][:, None, :]
     fl = np.full(len(Jl), np.inf)
-    s = np.abs(fixed) / dt
+    s = row_term_scale(t, ev, fixed, o)
     with np.errstate(all="ignore"):
         for i in range(len(Jl)):
             try:

@@ -177,11 +177,18 @@ def test_fallback_is_the_plain_method_when_every_warm_attempt_fails(torch, act_f
     pl = {k: dc.t[k].cpu().numpy() for k in fb}
     assert st_fb.fallbacks == n
     # the restart repeats the plain method from the step's start: activity
-    # coefficients, complexes and free ions bit for bit
+    # coefficients, complexes and free ions bit for bit (with activity every
+    # iteration from the 1e-9 default guess, cells that end non-finite end
+    # so in both)
     for k in ("pri_molal", "total", "status", "pri_act_coef", "sec_act_coef", "sec_molal"):
-        assert np.array_equal(fb[k], pl[k]), k
-    extra = fb["num_iterations"] - pl["num_iterations"]   # iterations of the abandoned attempt
-    assert np.all((extra >= 1) & (extra <= 3))
+        assert np.array_equal(fb[k], pl[k], equal_nan=fb[k].dtype.kind == "f"), k
+    # iterations of the abandoned attempt: up to fallback_after; none where
+    # the attempt turns non-finite at its first evaluation (with activity
+    # every iteration, the ionic strength at 1e3 molal does so in every cell)
+    extra = fb["num_iterations"] - pl["num_iterations"]
+    assert np.all((extra >= 0) & (extra <= 3))
+    if act_freq == abi.PFX_ACT_TIMESTEP:
+        assert np.mean(extra >= 1) > 0.5
     # the oracle's fallback takes the same path
     oc = cells.copy()
     oc["guess"][:] = 1e3

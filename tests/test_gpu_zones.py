@@ -107,8 +107,8 @@ def test_near_saturation_workload():
     tab = runtime.Tables(t)
     dc, lo, inside = workload.near_saturation_cells(t, tab, 2000, seed=synth.SEED, workload_id=3)
     assert dc.ncells == 2000
-    # most of the first half sits inside the window, the rest is unconstrained
-    assert inside.sum() >= 0.6 * 1000
+    # exactly half of the cells sit inside the window after the refresh
+    assert inside.sum() == 1000
     assert np.all(np.abs(lo[inside]) <= 0.05)
 
 
