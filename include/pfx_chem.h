@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PFX_ABI_VERSION 4
+#define PFX_ABI_VERSION 5
 
 /* largest primary-species count handled by one cell group (row per lane) */
 #define PFX_MAX_PRIMARY 32
@@ -49,6 +49,12 @@ extern "C" {
 #define PFX_E_DEVICE   (-2)   /* HIP runtime error */
 #define PFX_E_NOMEM    (-3)
 #define PFX_E_UNSUP    (-4)   /* table shape beyond the compiled limits */
+
+/* isotherm (K_D) sorption types: pflotran_constants.F90:94-96 SORPTION_* */
+#define PFX_SORPTION_LINEAR     1   /* S = K_D m                          */
+#define PFX_SORPTION_LANGMUIR   2   /* S = K_D m b / (1 + K_D m)          */
+#define PFX_SORPTION_FREUNDLICH 3   /* S = K_D m^(1/n)                    */
+#define PFX_MAX_KD_RXN 32
 
 /* activity-coefficient update frequency: reaction_aux.F90 ACT_COEF_FREQUENCY_* */
 #define PFX_ACT_OFF          0
@@ -175,6 +181,21 @@ typedef struct pfx_reaction_desc {
   double max_dlnC;                    /* 5                         */
   double max_relative_change_tolerance;  /* 1e-6                   */
   double max_residual_tolerance;         /* 1e-12                  */
+
+  /* equilibrium isotherm sorption: reaction%eqkd* (reaction.F90:505-600),
+     evaluated by RTotalSorbKD (:4220-4301); at most one reaction per
+     primary species.  The sorbed totals add to those of ion exchange. */
+  int32_t neqkdrxn;                   /* D: isotherm reactions (0: none)      */
+  int32_t reserved_kd;
+  const int32_t *eqkdspecid;          /* [D] primary id                        */
+  const int32_t *eqkdtype;            /* [D] PFX_SORPTION_*                    */
+  const double *eqkddistcoef;         /* [D] K_D: kg water/m^3 bulk, or with a
+                                         mineral mL water/g soil (L/kg)       */
+  const double *eqkdlangmuirb;        /* [D] Langmuir b (mol/m^3 bulk) or NULL */
+  const double *eqkdfreundlichn;      /* [D] Freundlich n or NULL              */
+  const int32_t *eqkdmineral;         /* [D] kinetic mineral whose volume
+                                         fraction scales K_D, -1 none; NULL = none */
+  double soil_particle_density;       /* kg/m^3 (ROCK_DENSITY), with eqkdmineral */
 } pfx_reaction_desc;
 
 /* Per-cell state, device pointers, cell-major.  In = read, Out = written. */

@@ -96,6 +96,9 @@ typedef struct pfx_newton_opts {
 #define PFX_SNES_DIVERGED_LINEAR_SOLVE    (-3)
 #define PFX_SNES_DIVERGED_FNORM_NAN       (-4)
 #define PFX_SNES_DIVERGED_MAX_IT          (-5)
+/* PFLOTRAN's ConvergenceTest: residual infinity norm above max_norm 1e20
+   (convergence.F90:230-232, solver.F90:137) */
+#define PFX_SNES_DIVERGED_MAX_NORM       (-10)
 
 typedef struct pfx_newton_stats {
   int32_t iterations;         /* Newton updates taken */
@@ -124,7 +127,11 @@ int pfx_gi_solver_destroy(pfx_gi_solver *s);
    log form, else scaled by 0.99 min(m / y) where that is below 1); x -= y
    (m <- m exp(-y) in log form); tests in SNESConvergedDefault's order:
    NaN, ||F|| < atol, ||F|| <= rtol ||F0||, ||y|| < stol ||x||, the
-   iteration limit.  Totals at the solution come from pfx_auxvar_batch. */
+   iteration limit, then PFLOTRAN's ConvergenceTest (convergence.F90:
+   170-245): ||F||_inf > 1e20 diverges, and no test but NaN applies to the
+   initial residual (newton_min_iterations 1: one Newton iteration at
+   least, also from a zero residual).  Totals at the solution come from
+   pfx_auxvar_batch. */
 int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *grid, const pfx_react_opts *opts,
                        const pfx_newton_opts *nopts, pfx_newton_stats *stats);
 

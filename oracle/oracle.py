@@ -73,7 +73,7 @@ class HostCells:
         N, M, K, R = t.N, t.M, t.K, t.R
         z = lambda *s: np.zeros(s, dtype=np.float64)  # noqa: E731
         a = {
-            "total": z(n, N), "total_sorb": z(n, N) if (R > 0 if sorb is None else sorb) else None,
+            "total": z(n, N), "total_sorb": z(n, N) if (t.has_sorption if sorb is None else sorb) else None,
             "pri_molal": np.full((n, N), 1e-9), "pri_act_coef": np.ones((n, N)),
             "sec_act_coef": np.ones((n, max(M, 1))), "sec_molal": z(n, max(M, 1)),
             "ionx_ref_sorbed": np.full((n, max(R, 1)), 1e-9), "mnrl_volfrac": z(n, max(K, 1)),
@@ -412,8 +412,12 @@ def gi_newton(t: ReactionTables, cells: HostCells, grid: HostGrid, opts: abi.Rea
     reference's ILU(0) on a 1-D block-tridiagonal matrix, or its DIRECT
     solver, is exact): PMRTCheckUpdatePre's clip of the log-form update
     (pm_rt.F90:734-840), then SNESConvergedDefault's tests in its order
-    (||F|| < atol, ||F|| <= rtol ||F0||, ||y|| < stol ||x||, the limit).
-    Updates cells["pri_molal"]; returns (iterations, reason)."""
+    (||F|| < atol, ||F|| <= rtol ||F0||, ||y|| < stol ||x||, the limit),
+    after which PFLOTRAN's ConvergenceTest (convergence.F90:170-245) marks a
+    residual of infinity norm above 1e20 diverged and takes no decision at
+    iteration 0 (newton_min_iterations 1: a zero initial residual still
+    takes one Newton iteration).  Updates cells["pri_molal"]; returns
+    (iterations, reason)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     N = t.N
@@ -422,8 +426,6 @@ def gi_newton(t: ReactionTables, cells: HostCells, grid: HostGrid, opts: abi.Rea
     f0 = np.linalg.norm(r)
     if not np.isfinite(f0):
         return 0, abi.PFX_SNES_DIVERGED_FNORM_NAN
-    if f0 < atol:
-        return 0, abi.PFX_SNES_CONVERGED_FNORM_ABS
     it = 0
     while True:
         if it >= max_it:
@@ -435,7 +437,8 @@ def gi_newton(t: ReactionTables, cells: HostCells, grid: HostGrid, opts: abi.Rea
             m = cells["pri_molal"].ravel() * np.exp(-y)
         else:
             m0 = cells["pri_molal"].ravel()
-            ratio = np.where(m0 <= y, np.abs(m0 / np.where(y == 0, 1.0, y)), 1e20)
+            with np.errstate(over="ignore"):
+                ratio = np.where(m0 <= y, np.abs(m0 / np.where(y == 0, 1.0, y)), 1e20)
             mr = ratio.min() if ratio.size else 1e20
             if mr < 1.0:
                 y = y * mr * 0.99
@@ -447,6 +450,8 @@ def gi_newton(t: ReactionTables, cells: HostCells, grid: HostGrid, opts: abi.Rea
         x = np.log(m) if grid.log_formulation else m
         if not np.isfinite(f):
             return it, abi.PFX_SNES_DIVERGED_FNORM_NAN
+        if np.max(np.abs(r)) > 1e20:
+            return it, abi.PFX_SNES_DIVERGED_MAX_NORM
         if f < atol:
             return it, abi.PFX_SNES_CONVERGED_FNORM_ABS
         if f <= rtol * f0:
@@ -461,12 +466,11 @@ def gi_column_run(deck, db=None, max_cuts=16):
     PMRTInitializeTimestep, SNESSolve with the time-step cut,
     PMRTUpdateSolution, TimestepperBEUpdateDT), the step controller being
     pflotran_amd.stepper.  Returns (stepper, cells, tables)."""
-    from pflotran_amd.database import water_density_ifc67
     from pflotran_amd.gi_run import face_data
     db = db if db is not None else deck.database()
     t = deck.tables(db)
     box = deck.box()
-    den = water_density_ifc67()
+    den = deck.water_density()
     n = box.nx
 
     def opts(dt):

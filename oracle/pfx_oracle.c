@@ -542,10 +542,57 @@ void or_kinetic_mineral(const pfx_reaction_desc *d, ocell *c, double *Res, doubl
   }
 }
 
+/* reaction.F90:4220-4301 RTotalSorbKD: isotherm sorption of a primary,
+   added to the sorbed totals; the derivative is diagonal.  With a mineral,
+   K_D (mL water/g soil) is scaled by the water density, the solid fraction,
+   the soil particle density and the mineral's volume fraction. */
+static void or_kd(const pfx_reaction_desc *d, ocell *c) {
+  for (int q = 0; q < d->neqkdrxn; q++) {
+    const int j = d->eqkdspecid[q];
+    const double molality = c->m[j];
+    double kd = d->eqkddistcoef[q];
+    if (d->eqkdmineral && d->eqkdmineral[q] >= 0)
+      kd = d->eqkddistcoef[q] * c->den * (1.0 - c->porosity) * d->soil_particle_density * 1.0e-3 *
+           c->vf[d->eqkdmineral[q]];
+    double res = 0.0, dres = 0.0;
+    if (d->eqkdtype[q] == PFX_SORPTION_LINEAR) {
+      res = kd * molality;
+      dres = kd;
+    } else if (d->eqkdtype[q] == PFX_SORPTION_LANGMUIR) {
+      const double t = kd * molality;
+      res = t * d->eqkdlangmuirb[q] / (1.0 + t);
+      dres = res / molality - res / (1.0 + t) * t / molality;
+    } else if (d->eqkdtype[q] == PFX_SORPTION_FREUNDLICH) {
+      const double one_over_n = 1.0 / d->eqkdfreundlichn[q];
+      res = kd * pow(molality, one_over_n);
+      dres = res / molality * one_over_n;
+    }
+    c->sorb[j] += res;
+    c->dsorb[j][j] += dres;
+  }
+}
+
+static int or_has_sorb(const pfx_reaction_desc *d) { return d->neqionxrxn > 0 || d->neqkdrxn > 0; }
+
+/* RTotalSorb (reaction.F90:4182-4216): the sorbed totals from zero, ion
+   exchange then isotherms */
+static void or_total_sorb(const pfx_reaction_desc *d, ocell *c) {
+  const int N = d->naqcomp;
+  if (d->neqionxrxn > 0) {
+    or_ionx(d, c);
+  } else {
+    for (int j = 0; j < N; j++) {
+      c->sorb[j] = 0.0;
+      for (int k = 0; k < N; k++) c->dsorb[j][k] = 0.0;
+    }
+  }
+  if (d->neqkdrxn > 0) or_kd(d, c);
+}
+
 /* RTAuxVarCompute (reaction.F90:4969-5006): RTotal then RTotalSorb */
 static void or_auxvar_compute(const pfx_reaction_desc *d, ocell *c) {
   or_rtotal(d, c);
-  if (d->neqionxrxn > 0) or_ionx(d, c);
+  if (or_has_sorb(d)) or_total_sorb(d, c);
 }
 
 /* Accumulation (mol), RTAccumulation reaction.F90:5072-5148 +
@@ -554,7 +601,7 @@ static void or_accumulation(const pfx_reaction_desc *d, const ocell *c, double *
   const int N = d->naqcomp;
   double psv_t = c->porosity * c->sat * 1000.0 * c->volume;
   for (int j = 0; j < N; j++) acc[j] = psv_t * c->total[j];
-  if (d->neqionxrxn > 0)
+  if (or_has_sorb(d))
     for (int j = 0; j < N; j++) acc[j] += c->sorb[j] * c->volume;
 }
 
@@ -600,7 +647,7 @@ static void or_res_jac(const pfx_reaction_desc *d, ocell *c, const double *fixed
     R[j] = acc[j] / dt - fixed_accum[j] / dt;
     for (int k = 0; k < N; k++) {
       J[j][k] = c->dtotal[j][k] * psvd_t;
-      if (d->neqionxrxn > 0) J[j][k] += c->dsorb[j][k] * v_t;
+      if (or_has_sorb(d)) J[j][k] += c->dsorb[j][k] * v_t;
     }
   }
   if (d->nkinmnrl > 0) {
@@ -617,7 +664,7 @@ static void or_res_jac(const pfx_reaction_desc *d, ocell *c, const double *fixed
       for (int s = 0; s < CPLX_N(d, i); s++) sc[CPLX_ID(d, i, s)] += fabs(CPLX_NU(d, i, s)) * c->sec[i];
     for (int j = 0; j < N; j++) {
       double s = psv * sc[j];
-      if (d->neqionxrxn > 0) s += fabs(c->sorb[j]) * c->volume;
+      if (or_has_sorb(d)) s += fabs(c->sorb[j]) * c->volume;
       scale[j] = s / dt + fabs(fixed_accum[j]) / dt;
     }
   }
@@ -734,7 +781,7 @@ restart:
   double psv_t = c.porosity * c.sat * 1000.0 * c.volume;
   for (int j = 0; j < N; j++) {
     fixed[j] = psv_t * cl->total[ci * N + j];
-    if (R > 0) fixed[j] += (cl->total_sorb ? cl->total_sorb[ci * N + j] : 0.0) * c.volume;
+    if (or_has_sorb(d)) fixed[j] += (cl->total_sorb ? cl->total_sorb[ci * N + j] : 0.0) * c.volume;
   }
   /* activity coefficients once per step from the previous state
      (reaction.F90:3407-3409); sec_molal of the previous state is
@@ -846,7 +893,7 @@ restart:
     cl->pri_molal[ci * N + j] = c.m[j];
     cl->total[ci * N + j] = c.total[j];
     if (cl->pri_act_coef) cl->pri_act_coef[ci * N + j] = c.gam[j];
-    if (cl->total_sorb && R > 0) cl->total_sorb[ci * N + j] = c.sorb[j];
+    if (cl->total_sorb && or_has_sorb(d)) cl->total_sorb[ci * N + j] = c.sorb[j];
   }
   for (int i = 0; i < M; i++) {
     if (cl->sec_act_coef) cl->sec_act_coef[ci * M + i] = c.gsec[i];
@@ -931,7 +978,7 @@ int or_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
       double psv_t = c.porosity * c.sat * 1000.0 * c.volume;
       for (int j = 0; j < N; j++) {
         fixed[j] = psv_t * cl->total[ci * N + j];
-        if (d->neqionxrxn > 0 && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
+        if (or_has_sorb(d) && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
       }
     }
     or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL, d->numerical_jacobian, o->jacobian_flags);
@@ -1005,7 +1052,7 @@ static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype,
       if (ctype[j] == OR_CON_FREE || ctype[j] == OR_CON_LOG) c->m[j] = free_conc[j];
     if (d->act_coef_update_frequency != PFX_ACT_OFF && compute_act) or_activity(d, c);
     or_rtotal(d, c);
-    if (d->neqionxrxn > 0) or_ionx(d, c);
+    if (or_has_sorb(d)) or_total_sorb(d, c);
     /* the reference stops on these failures (reaction.F90:3866-3877, 4382-4385) */
     if (c->status) return c->status;
     for (int j = 0; j < N; j++) {
@@ -1090,7 +1137,7 @@ static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype,
       compute_act = 1;
     }
   }
-  if (d->neqionxrxn > 0) or_ionx(d, c);
+  if (or_has_sorb(d)) or_total_sorb(d, c);
   *iters = nit;
   return 0;
 }
@@ -1135,19 +1182,19 @@ int or_equilibrate_batch(const pfx_reaction_desc *d, const int32_t *ctype, const
       c.m[j] = molarity / c.den * 1000.0;
     }
     or_rtotal(d, &c);
-    if (R > 0) or_ionx(d, &c);
+    if (or_has_sorb(d)) or_total_sorb(d, &c);
     if (d->act_coef_update_frequency != PFX_ACT_OFF) {
       for (int pass = 0; pass < 2; pass++) {
         or_activity(d, &c);
         or_rtotal(d, &c);
-        if (R > 0) or_ionx(d, &c);
+        if (or_has_sorb(d)) or_total_sorb(d, &c);
       }
     }
     for (int j = 0; j < N; j++) {
       cl->pri_molal[ci * N + j] = c.m[j];
       if (cl->pri_act_coef) cl->pri_act_coef[ci * N + j] = c.gam[j];
       if (cl->total) cl->total[ci * N + j] = c.total[j];
-      if (cl->total_sorb) cl->total_sorb[ci * N + j] = R > 0 ? c.sorb[j] : 0.0;
+      if (cl->total_sorb) cl->total_sorb[ci * N + j] = or_has_sorb(d) ? c.sorb[j] : 0.0;
     }
     for (int i = 0; i < M; i++) {
       if (cl->sec_act_coef) cl->sec_act_coef[ci * M + i] = c.gsec[i];
@@ -1178,10 +1225,10 @@ int or_auxvar_batch(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_react_o
       or_activity(d, &c);
     }
     or_rtotal(d, &c);
-    if (R > 0) or_ionx(d, &c);
+    if (or_has_sorb(d)) or_total_sorb(d, &c);
     for (int j = 0; j < N; j++) {
       if (cl->total) cl->total[ci * N + j] = c.total[j];
-      if (cl->total_sorb) cl->total_sorb[ci * N + j] = R > 0 ? c.sorb[j] : 0.0;
+      if (cl->total_sorb) cl->total_sorb[ci * N + j] = or_has_sorb(d) ? c.sorb[j] : 0.0;
       if (cl->pri_act_coef) cl->pri_act_coef[ci * N + j] = c.gam[j];
     }
     for (int i = 0; i < M; i++) {
@@ -1355,7 +1402,7 @@ int or_gi_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx
     const double psv_t = c.porosity * c.sat * 1000.0 * c.volume;
     for (int j = 0; j < N; j++) {
       fixed[j] = psv_t * cl->total[ci * N + j];
-      if (d->neqionxrxn > 0 && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
+      if (or_has_sorb(d) && cl->total_sorb) fixed[j] += cl->total_sorb[ci * N + j] * c.volume;
     }
     or_res_jac(d, &c, fixed, o->dt, Rv, J, NULL, d->numerical_jacobian, o->jacobian_flags);
     for (int i = 0; i < N; i++) {

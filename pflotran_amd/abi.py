@@ -5,7 +5,7 @@ the field offsets against the header compiled with the host compiler.
 """
 import ctypes as C
 
-PFX_ABI_VERSION = 4
+PFX_ABI_VERSION = 5
 PFX_MAX_PRIMARY = 32
 PFX_MAX_CPLX = 256
 PFX_MAX_RXN_SPEC = 8
@@ -26,6 +26,7 @@ PFX_CONSTRAINT_TOTAL, PFX_CONSTRAINT_FREE, PFX_CONSTRAINT_LOG, PFX_CONSTRAINT_PH
 PFX_CONSTRAINT_TOTAL_SORB = 5
 PFX_CONSTRAINT_MINERAL, PFX_CONSTRAINT_GAS = 6, 7
 PFX_JAC_CAPPED_QK = 1
+PFX_SORPTION_LINEAR, PFX_SORPTION_LANGMUIR, PFX_SORPTION_FREUNDLICH = 1, 2, 3
 PFX_NOALLOC_MAX_CELLS = 1 << 20
 
 _dp = C.POINTER(C.c_double)
@@ -51,6 +52,9 @@ class ReactionDesc(C.Structure):
         ("act_coef_update_algorithm", C.c_int32), ("numerical_jacobian", C.c_int32),
         ("max_dlnC", C.c_double), ("max_relative_change_tolerance", C.c_double),
         ("max_residual_tolerance", C.c_double),
+        ("neqkdrxn", C.c_int32), ("reserved_kd", C.c_int32), ("eqkdspecid", _ip), ("eqkdtype", _ip),
+        ("eqkddistcoef", _dp), ("eqkdlangmuirb", _dp), ("eqkdfreundlichn", _dp), ("eqkdmineral", _ip),
+        ("soil_particle_density", C.c_double),
     ]
 
 
@@ -160,6 +164,7 @@ PFX_KSP_DIVERGED_ITS, PFX_KSP_DIVERGED_DTOL, PFX_KSP_DIVERGED_BREAKDOWN = -3, -4
 PFX_KSP_DIVERGED_NANORINF, PFX_KSP_DIVERGED_PC_FAILED = -9, -11
 PFX_SNES_CONVERGED_FNORM_ABS, PFX_SNES_CONVERGED_FNORM_RELATIVE, PFX_SNES_CONVERGED_SNORM_RELATIVE = 2, 3, 4
 PFX_SNES_DIVERGED_LINEAR_SOLVE, PFX_SNES_DIVERGED_FNORM_NAN, PFX_SNES_DIVERGED_MAX_IT = -3, -4, -5
+PFX_SNES_DIVERGED_MAX_NORM = -10
 
 
 CELL_FIELDS = [f for f, _ in Cells._fields_ if f != "ncells"]

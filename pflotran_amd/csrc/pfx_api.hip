@@ -303,7 +303,7 @@ int check_cells(const pfx_tables *t, const pfx_cells *c, const pfx_react_opts *o
   if (c->ncells < 0 || c->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
   if (c->ncells == 0) return 0;  // an empty batch needs no arrays (RTReact over no cells)
   if (!c->total || !c->pri_molal) return PFX_E_ARG;
-  if (t->L.R > 0 && !c->total_sorb) return PFX_E_ARG;
+  if (t->L.sorb_any && !c->total_sorb) return PFX_E_ARG;
   if (t->L.K > 0 && (!c->mnrl_volfrac || !c->mnrl_area)) return PFX_E_ARG;
   if ((o->guess_mode == PFX_GUESS_ARRAY || o->guess_mode == PFX_GUESS_ARRAY_FALLBACK) && !c->guess)
     return PFX_E_ARG;
@@ -767,6 +767,36 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   }
   L.o_ionx_k = B.add(ionx_k);
   L.o_ionx_role = B.add(ionx_role);
+  // isotherm sorption: one reaction per primary, per-lane parameters
+  {
+    const int D = d->neqkdrxn;
+    if (D < 0 || D > PFX_MAX_KD_RXN) return PFX_E_UNSUP;
+    if (D > 0 && (!d->eqkdspecid || !d->eqkdtype || !d->eqkddistcoef)) return PFX_E_ARG;
+    std::vector<int32_t> kd_type(G, 0), kd_mnrl(G, -1);
+    std::vector<double> kd_coef(G, 0.0), kd_b(G, 0.0), kd_invn(G, 1.0);
+    for (int q = 0; q < D; q++) {
+      const int sp = d->eqkdspecid[q];
+      const int ty = d->eqkdtype[q];
+      if (sp < 0 || sp >= N || ty < PFX_SORPTION_LINEAR || ty > PFX_SORPTION_FREUNDLICH) return PFX_E_ARG;
+      if (kd_type[sp] != 0) return PFX_E_UNSUP;  // a second isotherm on one species
+      if (ty == PFX_SORPTION_LANGMUIR && !d->eqkdlangmuirb) return PFX_E_ARG;
+      if (ty == PFX_SORPTION_FREUNDLICH && !d->eqkdfreundlichn) return PFX_E_ARG;
+      kd_type[sp] = ty;
+      kd_coef[sp] = d->eqkddistcoef[q];
+      kd_b[sp] = d->eqkdlangmuirb ? d->eqkdlangmuirb[q] : 0.0;
+      kd_invn[sp] = (ty == PFX_SORPTION_FREUNDLICH) ? 1.0 / d->eqkdfreundlichn[q] : 1.0;
+      kd_mnrl[sp] = d->eqkdmineral ? d->eqkdmineral[q] : -1;
+      if (kd_mnrl[sp] >= K) return PFX_E_ARG;
+    }
+    L.kd_any = D > 0 ? 1 : 0;
+    L.sorb_any = (R > 0 || D > 0) ? 1 : 0;
+    L.soil_density = d->soil_particle_density;
+    L.o_kd_type = B.add(kd_type);
+    L.o_kd_mnrl = B.add(kd_mnrl);
+    L.o_kd_coef = B.add(kd_coef);
+    L.o_kd_b = B.add(kd_b);
+    L.o_kd_invn = B.add(kd_invn);
+  }
   // not needed inside the Newton loop: kept past the LDS prefix
   L.lds_table_bytes = ((int64_t)B.bytes.size() + 15) & ~int64_t(15);
   L.o_clogKcoef = B.add(ccoef);
@@ -951,7 +981,7 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
   if (cells->ncells < 0 || cells->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
   // an empty batch needs no device arrays
   if (cells->ncells > 0 && (!constraint_conc || !cells->pri_molal || !cells->total)) return PFX_E_ARG;
-  if (cells->ncells > 0 && t->L.R > 0 && !cells->total_sorb) return PFX_E_ARG;
+  if (cells->ncells > 0 && t->L.sorb_any && !cells->total_sorb) return PFX_E_ARG;
   if (stats) {
     memset(stats, 0, sizeof(*stats));
     stats->ncells = cells->ncells;
@@ -960,7 +990,7 @@ extern "C" int pfx_equilibrate_batch(pfx_tables *t, pfx_cells *cells, const int3
   for (int j = 0; j < t->L.N; j++) {
     const int c = constraint_type[j];
     if (c < PFX_CONSTRAINT_TOTAL || c > PFX_CONSTRAINT_GAS) return PFX_E_UNSUP;
-    if (c == PFX_CONSTRAINT_TOTAL_SORB && t->L.R == 0) return PFX_E_ARG;
+    if (c == PFX_CONSTRAINT_TOTAL_SORB && !t->L.sorb_any) return PFX_E_ARG;
     nph += c == PFX_CONSTRAINT_PH;
     if (c == PFX_CONSTRAINT_MINERAL || c == PFX_CONSTRAINT_GAS) {
       if (!constraint_rxn) return PFX_E_ARG;
@@ -1004,7 +1034,7 @@ extern "C" int pfx_auxvar_batch(pfx_tables *t, pfx_cells *cells, int32_t activit
   if (cells->ncells < 0 || cells->ncells > (int64_t)0x7fffffff) return PFX_E_ARG;
   if (cells->ncells == 0) return 0;  // an empty batch needs no device arrays
   if (!cells->pri_molal || !cells->total) return PFX_E_ARG;
-  if (t->L.R > 0 && !cells->total_sorb) return PFX_E_ARG;
+  if (t->L.sorb_any && !cells->total_sorb) return PFX_E_ARG;
   HIPCHK(hipSetDevice(t->device));
   pfx_kernel_params P;
   fill_params(t, cells, o, P);

@@ -33,6 +33,9 @@ struct pfx_dev_layout {
   int32_t use_log, act_freq, act_alg, num_jac;
   double max_dlnC, max_rel_change_tol, max_residual_tol;
   int32_t mnrl_flags;           // bit0 threshold, bit1 limiter, bit2 power, bit3 Temkin
+  int32_t kd_any;               // some primary carries isotherm (K_D) sorption
+  int32_t sorb_any;             // sorbed totals present: ion exchange or isotherms
+  double soil_density;          // kg/m^3, for K_D scaled by a mineral's volume fraction
 
   // byte offsets into the table blob (doubles unless noted)
   int64_t o_Z, o_a0;                 // [G]
@@ -53,6 +56,9 @@ struct pfx_dev_layout {
   int64_t o_mk, o_mmolvol, o_mEa, o_mthr, o_mlim, o_mpow, o_mtemkin;  // [K]
   int64_t o_ionx_k;                  // [R][G] selectivity of the lane's cation (0 if none)
   int64_t o_ionx_role;               // int32 [R][G] cation slot (0 = reference) or -1
+  int64_t o_kd_type;                 // int32 [G] PFX_SORPTION_* of the lane's primary (0 = none)
+  int64_t o_kd_mnrl;                 // int32 [G] mineral scaling the lane's K_D, or -1
+  int64_t o_kd_coef, o_kd_b, o_kd_invn;  // [G] K_D, Langmuir b, 1/n (Freundlich)
   int64_t blob_bytes;
   int64_t lds_table_bytes;           // prefix of the blob staged into LDS
 };

@@ -122,6 +122,42 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_min(int nparts, const double *
   }
 }
 
+// max |v| over the workgroup / over the partials (the residual's infinity
+// norm for the divergence bound)
+__global__ __launch_bounds__(BLOCK) void k_max_abs(int64_t n, const double *__restrict__ v,
+                                                   double *__restrict__ partial) {
+  double mx = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    mx = fmax(mx, fabs(v[i]));
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  __shared__ double sh[BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = sh[0];
+    for (int i = 1; i < BLOCK / 64; i++) r = fmax(r, sh[i]);
+    partial[2 * blockIdx.x] = r;
+    partial[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reduce_max(int nparts, const double *__restrict__ partial,
+                                                      double *__restrict__ out) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += BLOCK) v = fmax(v, partial[2 * i]);
+  __shared__ double sh[BLOCK];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sh[0];
+    out[1] = 0.0;
+  }
+}
+
 // (a, b) and (c, d) over n entries; c == nullptr skips the second
 __global__ __launch_bounds__(BLOCK) void k_dot2(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
                                                 const double *__restrict__ c, const double *__restrict__ d,
@@ -471,6 +507,20 @@ int read2(pfx_bsr *h, int nparts, hipStream_t s, double *a, double *b) {
   return 0;
 }
 
+// max |v_i| over n entries (nparts workgroups)
+int max_abs(pfx_bsr *h, int nparts, hipStream_t s, int64_t n, const double *v, double *out) {
+  hipLaunchKernelGGL(k_max_abs, dim3(nparts), dim3(BLOCK), 0, s, n, v, h->partial);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(BLOCK), 0, s, nparts, h->partial, h->red);
+  KCHK(hipGetLastError());
+  KCHK(hipMemcpyAsync(h->host2, h->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  KCHK(hipStreamSynchronize(s));
+  *out = h->host2[0];
+  return 0;
+}
+
+// ConvergenceTest's divergence bound on the residual (solver.F90:137)
+constexpr double kMaxNorm = 1.0e20;
+
 int spmv_launch(pfx_bsr *h, const double *vals, const double *x, double *y, const double *a1, int self2,
                 double *partial, hipStream_t s) {
   const int g = rows_grid(h);
@@ -784,8 +834,9 @@ extern "C" int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_
   int rc = assemble(&S.fnorm0);
   if (rc) return rc;
   S.fnorm = S.fnorm0;
+  // the initial residual decides nothing but NaN: one iteration at least
+  // (newton_min_iterations, convergence.F90:242-245)
   if (!std::isfinite(S.fnorm0)) S.reason = PFX_SNES_DIVERGED_FNORM_NAN;
-  else if (S.fnorm0 < atol) S.reason = PFX_SNES_CONVERGED_FNORM_ABS;
   const int use_log = grid->log_formulation ? 1 : 0;
   // max_dlnC of the network (reaction%max_dlnC)
   double max_dlnC = 5.0;
@@ -825,6 +876,13 @@ extern "C" int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_
     else if (S.fnorm < atol) S.reason = PFX_SNES_CONVERGED_FNORM_ABS;
     else if (S.fnorm <= rtol * S.fnorm0) S.reason = PFX_SNES_CONVERGED_FNORM_RELATIVE;
     else if (sqrt(yy) < stol * sqrt(xx)) S.reason = PFX_SNES_CONVERGED_SNORM_RELATIVE;
+    // then ConvergenceTest's max_norm on ||F||_inf (<= ||F||_2, so only a
+    // 2-norm above the bound needs the infinity norm)
+    if (std::isfinite(S.fnorm) && S.fnorm > kMaxNorm) {
+      double finf = 0.0;
+      if ((rc = max_abs(h, ge, hs, n, s->res, &finf))) return rc;
+      if (finf > kMaxNorm) S.reason = PFX_SNES_DIVERGED_MAX_NORM;
+    }
   }
   S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
   if (st) *st = S;

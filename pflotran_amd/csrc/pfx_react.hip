@@ -484,7 +484,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             v_gam = ld_or(P.pri_act_coef, ci, dmy);
             if (MODE != PFX_MODE_AUX) {
               v_tot = ld_or(P.total, ci, dmy);
-              v_sorb = ld_or(R > 0 ? P.total_sorb : nullptr, ci, dmy);
+              v_sorb = ld_or(L.sorb_any ? P.total_sorb : nullptr, ci, dmy);
             }
           }
           double mguess = 1.0;
@@ -542,8 +542,8 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             m = live ? v_m : 1.0;
             gam = (live && P.pri_act_coef) ? v_gam : 1.0;
             if (MODE != PFX_MODE_AUX) {
-              const double sorb_old = (R > 0 && P.total_sorb) ? v_sorb : 0.0;
-              SD(S.fx, r) = live ? porosity * sat * 1000.0 * vol * v_tot + (R > 0 ? sorb_old * vol : 0.0) : 0.0;
+              const double sorb_old = (L.sorb_any && P.total_sorb) ? v_sorb : 0.0;
+              SD(S.fx, r) = live ? porosity * sat * 1000.0 * vol * v_tot + (L.sorb_any ? sorb_old * vol : 0.0) : 0.0;
             }
             // up to 4 complexes per pass with their loads issued together
 #pragma unroll 4
@@ -904,6 +904,39 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         }
       }
     }
+    // isotherm sorption (RTotalSorbKD, reaction.F90:4220-4301): the lane's
+    // own primary only, so its derivative is diagonal
+    if (active && L.kd_any && (need_ionx || ionx_final)) {
+      const int kty = live ? LI(L.o_kd_type, r) : 0;
+      if (kty > 0) {
+        double kd = LD(L.o_kd_coef, r);
+        const int km = LI(L.o_kd_mnrl, r);
+        // K_D in mL water/g soil scaled by the mineral's volume fraction
+        if (km >= 0) kd = kd * SD(S.cs, 3) * (1.0 - SD(S.cs, 0)) * L.soil_density * 1.0e-3 * SD(S.mvf, km);
+        double res, dres;
+        if (kty == PFX_SORPTION_LINEAR) {
+          res = kd * m;
+          dres = kd;
+        } else if (kty == PFX_SORPTION_LANGMUIR) {
+          const double tr = kd * m;
+          res = tr * LD(L.o_kd_b, r) / (1.0 + tr);
+          dres = res / m - res / (1.0 + tr) * tr / m;
+        } else {
+          const double inv_n = LD(L.o_kd_invn, r);
+          res = kd * pow(m, inv_n);
+          dres = res / m * inv_n;
+        }
+        Sr += res;
+        const double jw = need_ionx ? vol / psv_t : 1.0;  // row[] is scaled by psv_t/dt later
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+          if (k == r) {
+            if (need_ionx) row[k] += dres * jw;
+            else if (MODE == PFX_MODE_EQUIL && P.has_sorb) srow[k] += dres;
+          }
+        }
+      }
+    }
     sorb_out = Sr;
     PFX_PT(4);
 
@@ -922,7 +955,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       for (int k = 0; k < G; k++) row[k] = live ? row[k] * (psv_t / dt) : row[k];
       if (live) {
         const double fixed = SD(S.fx, r);
-        Rr = (psv_t * tot_out + (R > 0 ? Sr * vol : 0.0)) / dt - fixed / dt;
+        Rr = (psv_t * tot_out + (L.sorb_any ? Sr * vol : 0.0)) / dt - fixed / dt;
         const double psv = SD(S.cs, 0) * SD(S.cs, 1) * SD(S.cs, 3) * vol;
         scale = psv * (m + absum) / dt + fabs(Sr) * vol / dt + fabs(fixed) / dt;
       }
@@ -1279,7 +1312,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           P.pri_molal[c * N + r] = m;
           P.total[c * N + r] = tot_out;
           if (P.pri_act_coef) P.pri_act_coef[c * N + r] = gam;
-          if (R > 0 && P.total_sorb) P.total_sorb[c * N + r] = sorb_out;
+          if (L.sorb_any && P.total_sorb) P.total_sorb[c * N + r] = sorb_out;
         }
         for (int cc = r; cc < M; cc += G) {
           if (P.sec_act_coef) P.sec_act_coef[c * M + cc] = SD(S.gsec, cc);

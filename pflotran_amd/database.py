@@ -289,6 +289,20 @@ class IonExchangeRxn:
 
 
 @dataclass
+class KDRxn:
+    """An isotherm reaction of SORPTION / ISOTHERM_REACTIONS (kd_rxn_type,
+    reaction_aux.F90:80-90).  K_D is in kg water/m^3 bulk, or with a
+    mineral in mL water/g soil (L/kg) scaled by the soil density and the
+    mineral's volume fraction."""
+    species: str
+    kd: float
+    type: int = abi.PFX_SORPTION_LINEAR
+    langmuir_b: float = 0.0
+    freundlich_n: float = 0.0
+    mineral: Optional[str] = None
+
+
+@dataclass
 class ReactionTables:
     """numpy form of pfx_reaction_desc; `desc()` returns the ctypes struct
     (the arrays must outlive it: they are kept on this object)."""
@@ -331,6 +345,15 @@ class ReactionTables:
     # tables built in BasisInit); None when no mineral sets it
     mnrl_affinity_power: Optional[np.ndarray] = None
     mnrl_temkin: Optional[np.ndarray] = None
+    # isotherm sorption (reaction%eqkd*, reaction_database.F90:3393-3450):
+    # [D] arrays, empty without ISOTHERM_REACTIONS
+    kd_specid: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    kd_type: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    kd_coef: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    kd_langmuir_b: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    kd_freundlich_n: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    kd_mineral: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    soil_particle_density: float = 0.0            # kg/m^3 (ROCK_DENSITY)
     _desc: Optional[abi.ReactionDesc] = None
 
     @property
@@ -348,6 +371,16 @@ class ReactionTables:
     @property
     def R(self) -> int:
         return self.ionx_CEC.shape[0]
+
+    @property
+    def D(self) -> int:
+        return self.kd_specid.shape[0]
+
+    @property
+    def has_sorption(self) -> bool:
+        """Sorbed totals are carried (reaction%neqsorb > 0): ion exchange or
+        isotherms."""
+        return self.R > 0 or self.D > 0
 
     def desc(self) -> abi.ReactionDesc:
         d = abi.ReactionDesc()
@@ -390,6 +423,12 @@ class ReactionTables:
         d.max_dlnC = self.max_dlnC
         d.max_relative_change_tolerance = self.max_rel_change_tol
         d.max_residual_tolerance = self.max_residual_tol
+        d.neqkdrxn = self.D
+        if self.D:
+            d.eqkdspecid, d.eqkdtype = ip(self.kd_specid), ip(self.kd_type)
+            d.eqkddistcoef, d.eqkdlangmuirb = dp(self.kd_coef), dp(self.kd_langmuir_b)
+            d.eqkdfreundlichn, d.eqkdmineral = dp(self.kd_freundlich_n), ip(self.kd_mineral)
+        d.soil_particle_density = self.soil_particle_density
         self._desc = d
         return d
 
@@ -455,7 +494,8 @@ def constraint_rxns(t: "ReactionTables", rxns: Dict[int, "abi.ConstraintRxn"]):
 
 def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[str] = (),
                  minerals: Dict[str, MineralKinetics] | None = None,
-                 ion_exchange: Sequence[IonExchangeRxn] = (), *, temperature: float = 25.0,
+                 ion_exchange: Sequence[IonExchangeRxn] = (), *, isotherms: Sequence[KDRxn] = (),
+                 soil_particle_density: float = 0.0, temperature: float = 25.0,
                  log_formulation: bool = True, act_freq: int = abi.PFX_ACT_OFF,
                  act_alg: int = abi.PFX_ACT_ALG_LAG, use_bdot: bool = True,
                  fit_logk: bool = False) -> ReactionTables:
@@ -463,7 +503,10 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
     every complex/mineral species must be a primary or H2O).  log K at the
     reference temperature is the tabulated value when the temperature is a
     database temperature (ReactionInitializeLogK, reaction_aux.F90:1438-1450),
-    otherwise the Maier-Kelly fit."""
+    otherwise the Maier-Kelly fit.  `isotherms` are the ISOTHERM_REACTIONS
+    (their mineral names resolved among the kinetic minerals,
+    GetKineticMineralIDFromName as reaction_database.F90:3434-3442 uses it);
+    `soil_particle_density` is the ROCK_DENSITY they scale by."""
     minerals = dict(minerals or {})
     pidx = {n: i for i, n in enumerate(primaries)}
     N, M, K, R = len(primaries), len(secondaries), len(minerals), len(ion_exchange)
@@ -517,6 +560,20 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
         for j, (nm, k) in enumerate(rx.cations):
             ionx_id[r, 1 + j] = pidx[nm]
             ionx_k[r, j] = k
+    mnames_ = list(minerals)
+    for kr in isotherms:
+        if kr.species not in pidx:
+            raise ValueError(f"species {kr.species} in kd reaction not found among primary species list")
+        if kr.mineral and kr.mineral not in mnames_:
+            raise ValueError(f"mineral {kr.mineral} listed in kd reaction not found in the kinetic minerals")
+    kd_kw = dict(
+        kd_specid=np.array([pidx[kr.species] for kr in isotherms], dtype=np.int32),
+        kd_type=np.array([kr.type for kr in isotherms], dtype=np.int32),
+        kd_coef=np.array([kr.kd for kr in isotherms], dtype=np.float64),
+        kd_langmuir_b=np.array([kr.langmuir_b for kr in isotherms], dtype=np.float64),
+        kd_freundlich_n=np.array([kr.freundlich_n for kr in isotherms], dtype=np.float64),
+        kd_mineral=np.array([mnames_.index(kr.mineral) if kr.mineral else -1 for kr in isotherms], dtype=np.int32),
+        soil_particle_density=float(soil_particle_density))
     return ReactionTables(
         primary_names=list(primaries), complex_names=list(secondaries), mineral_names=mnames,
         Z=Z, a0=a0, cplx_specid=cs, cplx_stoich=cn, cplx_h2ostoich=ch,
@@ -537,4 +594,4 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
         ionx_to_surf=np.array([mnames.index(rx.mineral) if rx.mineral else -1 for rx in ion_exchange],
                               dtype=np.int32),
         debye=debye_params(temperature, use_bdot), use_log_formulation=int(log_formulation),
-        act_freq=act_freq, act_alg=act_alg)
+        act_freq=act_freq, act_alg=act_alg, **kd_kw)

@@ -14,14 +14,17 @@ Cards read (everything else in the deck is skipped line by line):
   PASSIVE_GAS_SPECIES / ACTIVE_GAS_SPECIES, MINERALS, MINERAL_KINETICS
   (RATE_CONSTANT [units], ACTIVATION_ENERGY [units], AFFINITY_THRESHOLD,
   RATE_LIMITER), SORPTION / ION_EXCHANGE_RXN (MINERAL, CEC, CATIONS with one
-  REFERENCE cation of k = 1), DATABASE, LOG_FORMULATION,
+  REFERENCE cation of k = 1), SORPTION / ISOTHERM_REACTIONS (TYPE,
+  DISTRIBUTION_COEFFICIENT [units], LANGMUIR_B, FREUNDLICH_N,
+  KD_MINERAL_NAME), DATABASE, LOG_FORMULATION,
   ACTIVITY_COEFFICIENTS [OFF|LAG|NEWTON|TIMESTEP|NEWTON_ITERATION], NO_BDOT,
   MOLAL, MAX_DLNC, MAX_RELATIVE_CHANGE_TOLERANCE, MAX_RESIDUAL_TOLERANCE;
   OUTPUT and REDOX_SPECIES are skipped as the reference skips them.
 * ``CONSTRAINT name``: CONCENTRATIONS (F/FREE, T/TOTAL, TOTAL_SORB, P/PH,
   L/LOG, M/MINERAL/MNRL name, G/GAS name, Z/CHG; TOTAL when the type is
   left out) and MINERALS (volume fraction, area [units]).
-* ``MATERIAL_PROPERTY name``: POROSITY.
+* ``MATERIAL_PROPERTY name``: POROSITY, ROCK_DENSITY.
+* ``REFERENCE_DENSITY`` [units]: the water density of a run without flow.
 * ``SUBSURFACE_TRANSPORT`` options: GLOBAL_IMPLICIT, NUMERICAL_JACOBIAN.
 * ``TIME``: FINAL_TIME, INITIAL_TIMESTEP_SIZE, MAXIMUM_TIMESTEP_SIZE.
 * ``REFERENCE_TEMPERATURE``.
@@ -40,8 +43,8 @@ from typing import Callable, Dict, Iterator, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import abi
-from .database import (Database, DbSpecies, IonExchangeRxn, MineralKinetics, ReactionTables, build_tables,
-                       constraint_rxn, constraint_rxns, read_database)
+from .database import (Database, DbSpecies, IonExchangeRxn, KDRxn, MineralKinetics, ReactionTables, build_tables,
+                       constraint_rxn, constraint_rxns, read_database, water_density_ifc67)
 
 
 class DeckError(ValueError):
@@ -212,6 +215,7 @@ class Chemistry:
     minerals: List[str] = field(default_factory=list)
     kinetics: Dict[str, MineralKinetics] = field(default_factory=dict)
     ion_exchange: List[IonExchangeRxn] = field(default_factory=list)
+    isotherms: List[KDRxn] = field(default_factory=list)
     database: Optional[str] = None
     log_formulation: bool = False                 # reaction_aux.F90:433
     act_freq: int = abi.PFX_ACT_OFF               # reaction_aux.F90:418
@@ -300,6 +304,8 @@ class Deck:
     temperature: float = 25.0
     numerical_jacobian: bool = False
     transport: TransportCard = field(default_factory=TransportCard)
+    reference_density: Optional[float] = None     # kg/m^3, REFERENCE_DENSITY
+    rock_density: Dict[str, float] = field(default_factory=dict)   # material -> kg/m^3
 
     # ---------------------------------------------------------------- builds
 
@@ -307,7 +313,26 @@ class Deck:
         """No complexes, minerals, sorption or gases: the reference does not
         switch on its full geochemistry (reaction.F90:899-906)."""
         c = self.chemistry
-        return not (c.secondaries or c.minerals or c.kinetics or c.ion_exchange or c.gases)
+        return not (c.secondaries or c.minerals or c.kinetics or c.ion_exchange or c.isotherms or c.gases)
+
+    def water_density(self) -> float:
+        """den_kg of a run without flow: REFERENCE_DENSITY, else the water
+        density at the reference temperature and pressure
+        (factory_subsurface.F90:995-1000, init_subsurface_transport.F90:50)."""
+        if self.reference_density is not None:
+            return self.reference_density
+        return water_density_ifc67(self.temperature)
+
+    def soil_particle_density(self) -> float:
+        """ROCK_DENSITY of the deck's material (material.F90:1371-1374), which
+        scales the isotherms tied to a mineral; 0 when none is needed."""
+        if not any(k.mineral for k in self.chemistry.isotherms):
+            return 0.0
+        mats = {m for _, m in self.transport.strata if m} or set(self.porosity)
+        vals = {self.rock_density[m] for m in mats if m in self.rock_density}
+        if len(vals) != 1 or len(mats) != 1:
+            raise DeckError("an isotherm scaled by a mineral needs one material with a ROCK_DENSITY")
+        return vals.pop()
 
     def database(self, path: Optional[str] = None) -> Database:
         """The thermodynamic database the deck names, resolved against the
@@ -328,7 +353,8 @@ class Deck:
         kin = {n: c.kinetics[n] for n in self.kinetic_minerals()}
         # without a database the activity update is off (reaction.F90:920-921)
         act_freq = c.act_freq if c.database is not None else abi.PFX_ACT_OFF
-        t = build_tables(db, c.primaries, c.secondaries, kin, c.ion_exchange, temperature=self.temperature,
+        t = build_tables(db, c.primaries, c.secondaries, kin, c.ion_exchange, isotherms=c.isotherms,
+                         soil_particle_density=self.soil_particle_density(), temperature=self.temperature,
                          log_formulation=c.log_formulation, act_freq=act_freq, act_alg=c.act_alg,
                          use_bdot=c.use_bdot, fit_logk=fit_logk)
         t.max_dlnC = c.max_dlnC
@@ -590,8 +616,54 @@ def _ion_exchange(cur: _Cursor) -> IonExchangeRxn:
     return IonExchangeRxn(cations, cec, mineral)
 
 
+def _isotherms(cur: _Cursor, units: List[str]) -> List[KDRxn]:
+    """ISOTHERM_REACTIONS (reaction.F90:505-633).  As the reference reads
+    them: every keyword line first resets the type to LINEAR, so the last
+    line decides it (LANGMUIR_B and FREUNDLICH_N set theirs); units given
+    after a DISTRIBUTION_COEFFICIENT stay in force for the reactions read
+    after it (`units` carries them), converting K_D to kg/m^3 bulk, or to
+    L/kg with a mineral."""
+    out = []
+    for t in cur.block("ISOTHERM_REACTIONS"):
+        k = KDRxn(t[0], 0.0)
+        for s in cur.block(f"ISOTHERM_REACTIONS {t[0]}"):
+            w = s[0].upper()
+            k.type = abi.PFX_SORPTION_LINEAR
+            if w == "TYPE":
+                v = s[1]
+                types = {"LINEAR": abi.PFX_SORPTION_LINEAR, "LANGMUIR": abi.PFX_SORPTION_LANGMUIR,
+                         "FREUNDLICH": abi.PFX_SORPTION_FREUNDLICH}
+                if v not in types:    # compared as written
+                    raise DeckError(f"CHEMISTRY,SORPTION,ISOTHERM_REACTIONS,TYPE: {v} unknown")
+                k.type = types[v]
+            elif w in ("DISTRIBUTION_COEFFICIENT", "KD"):
+                k.kd = _num(s[1])
+                if len(s) > 2:
+                    units[0] = s[2]
+            elif w == "LANGMUIR_B":
+                k.langmuir_b = _num(s[1])
+                k.type = abi.PFX_SORPTION_LANGMUIR
+            elif w == "FREUNDLICH_N":
+                k.freundlich_n = _num(s[1])
+                k.type = abi.PFX_SORPTION_FREUNDLICH
+            elif w == "KD_MINERAL_NAME":
+                k.mineral = s[1]
+            elif w in ("SEC_CONT_DISTRIBUTION_COEFFICIENT", "SEC_CONT_KD"):
+                raise DeckError("Make sure MULTIPLE_CONTINUUM keyword is set, SECONDARY_CONTINUUM_KD.")
+            else:
+                raise DeckError(f"CHEMISTRY,SORPTION,ISOTHERM_REACTIONS: {w} unknown")
+        if units[0]:
+            k.kd *= units_factor(units[0], "L/kg" if k.mineral else "kg/m^3")
+        # a one-letter mineral name is not looked up (reaction_database.F90:3434)
+        if k.mineral is not None and len(k.mineral) < 2:
+            k.mineral = None
+        out.append(k)
+    return out
+
+
 def _chemistry(cur: _Cursor) -> Chemistry:
     c = Chemistry()
+    kd_units = [""]
     for t in cur.block("CHEMISTRY"):
         w = t[0].upper()
         if w == "PRIMARY_SPECIES":
@@ -609,7 +681,9 @@ def _chemistry(cur: _Cursor) -> Chemistry:
                 v = s[0].upper()
                 if v == "ION_EXCHANGE_RXN":
                     c.ion_exchange.append(_ion_exchange(cur))
-                elif v in ("SURFACE_COMPLEXATION_RXN", "ISOTHERM_REACTIONS", "DISTRIBUTION_COEFFICIENTS"):
+                elif v == "ISOTHERM_REACTIONS":
+                    c.isotherms += _isotherms(cur, kd_units)
+                elif v in ("SURFACE_COMPLEXATION_RXN", "DISTRIBUTION_COEFFICIENTS"):
                     raise DeckError(f"SORPTION {v} is outside the per-cell path")
                 elif v in ("JUMPSTART_KINETIC_SORPTION", "NO_CHECKPOINT_KINETIC_SORPTION",
                            "NO_RESTART_KINETIC_SORPTION"):
@@ -840,6 +914,7 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
     chem, constraints, porosity = None, {}, {}
     tc, temperature, numjac, stepper = TimeCard(), 25.0, False, {}
     tr = TransportCard()
+    ref_den, rock = None, {}
     while cur.i < len(cur.lines):
         t = cur.next("deck")
         w = t[0].upper()
@@ -867,6 +942,9 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
                     porosity[name] = _num(s[1])
                 elif v == "TORTUOSITY" and depth == 0:
                     tr.tortuosity[name] = _num(s[1])
+                elif v == "ROCK_DENSITY" and depth == 0:
+                    f = units_factor(s[2], "kg/m^3") if len(s) > 2 else 1.0
+                    rock[name] = _num(s[1]) * f
                 elif v == "LONGITUDINAL_DISPERSIVITY" and depth == 0:
                     tr.dispersivity[name] = _num(s[1])
         elif w == "SUBSURFACE_TRANSPORT":
@@ -889,6 +967,8 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
                     stepper["tfac"] = tuple(_num(x) for x in s[1:])
         elif w == "REFERENCE_TEMPERATURE":
             temperature = _num(t[1])
+        elif w == "REFERENCE_DENSITY":
+            ref_den = _num(t[1]) * (units_factor(t[2], "kg/m^3") if len(t) > 2 else 1.0)
         elif w == "GRID":
             _grid(cur, tr)
         elif w == "UNIFORM_VELOCITY":
@@ -923,7 +1003,7 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
         raise DeckError("deck has no CHEMISTRY card")
     for k, v in stepper.items():
         setattr(tc, k, v)
-    return Deck(path, chem, constraints, porosity, tc, temperature, numjac, tr)
+    return Deck(path, chem, constraints, porosity, tc, temperature, numjac, tr, ref_den, rock)
 
 
 def read_deck(path: str) -> Deck:

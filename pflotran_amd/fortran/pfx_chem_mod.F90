@@ -10,7 +10,7 @@ module pfx_chem_module
 
   private
 
-  integer(c_int), parameter, public :: PFX_ABI_EXPECTED = 3
+  integer(c_int), parameter, public :: PFX_ABI_EXPECTED = 5
   integer(c_int), parameter, public :: PFX_E_ARG = -1, PFX_E_DEVICE = -2, &
                                        PFX_E_NOMEM = -3, PFX_E_UNSUP = -4
   integer(c_int), parameter, public :: PFX_ACT_OFF = 0, PFX_ACT_TIMESTEP = 1, &
@@ -48,6 +48,10 @@ module pfx_chem_module
     real(c_double) :: max_dlnC
     real(c_double) :: max_relative_change_tolerance
     real(c_double) :: max_residual_tolerance
+    integer(c_int32_t) :: neqkdrxn, reserved_kd
+    type(c_ptr) :: eqkdspecid, eqkdtype, eqkddistcoef, eqkdlangmuirb
+    type(c_ptr) :: eqkdfreundlichn, eqkdmineral
+    real(c_double) :: soil_particle_density
   end type pfx_reaction_desc
 
   type, bind(c), public :: pfx_cells

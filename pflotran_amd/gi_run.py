@@ -40,7 +40,6 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import abi
-from .database import water_density_ifc67
 from .deck import Deck
 
 
@@ -95,7 +94,7 @@ class ColumnRun:
         self.t = deck.tables(self.db)
         self.box = deck.box()
         self.tab = runtime.Tables(self.t)
-        self.den = water_density_ifc67()
+        self.den = deck.water_density()
         self.jacobian_flags = jacobian_flags
         box, t = self.box, self.t
         n = box.nx

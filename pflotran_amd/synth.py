@@ -27,7 +27,7 @@ from .database import (ReactionTables, debye_params, logk_eval, logk_fit, water_
 SEED = 20261202
 DB_TEMPS = [0.0, 25.0, 60.0, 100.0, 150.0, 200.0, 250.0, 300.0]
 
-FIELDS = {"table": 1, "totals": 2, "ph": 3, "temp": 4, "vf": 5, "guess": 6, "perturb": 7, "perm": 8}
+FIELDS = {"table": 1, "totals": 2, "ph": 3, "temp": 4, "vf": 5, "guess": 6, "perturb": 7, "perm": 8, "isotherms": 9}
 
 
 def rng(workload_id: int, field: str, seed: int = SEED) -> np.random.Generator:
@@ -120,6 +120,27 @@ def random_network(N: int = 15, M: int = 50, K: int = 3, ionx: bool = True, seed
         debye=debye_params(25.0), use_log_formulation=1, act_freq=act_freq, act_alg=abi.PFX_ACT_ALG_LAG)
 
 
+def with_isotherms(t: ReactionTables, seed: int = SEED, workload_id: int = 2,
+                   soil_particle_density: float = 2650.0) -> ReactionTables:
+    """The network with isotherm (K_D) sorption on four non-H+ primaries:
+    linear, Langmuir, Freundlich, and linear scaled by kinetic mineral 0's
+    volume fraction (K_D in mL/g).  Sized so that the sorbed and aqueous
+    amounts of a species are comparable over 1e-9..1e-2 molal."""
+    from dataclasses import replace
+    g = rng(workload_id, "isotherms", seed)
+    sp = np.sort(g.choice(np.arange(1, t.N), size=4, replace=False)).astype(np.int32)
+    kd = np.array([10.0 ** g.uniform(1.0, 3.0), 10.0 ** g.uniform(2.0, 5.0), 10.0 ** g.uniform(0.0, 2.0),
+                   10.0 ** g.uniform(-2.0, 0.0)])
+    return replace(
+        t, kd_specid=sp,
+        kd_type=np.array([abi.PFX_SORPTION_LINEAR, abi.PFX_SORPTION_LANGMUIR, abi.PFX_SORPTION_FREUNDLICH,
+                          abi.PFX_SORPTION_LINEAR], dtype=np.int32),
+        kd_coef=kd, kd_langmuir_b=np.array([0.0, 10.0 ** g.uniform(-3.0, 0.0), 0.0, 0.0]),
+        kd_freundlich_n=np.array([0.0, 0.0, g.uniform(1.2, 3.0), 0.0]),
+        kd_mineral=np.array([-1, -1, -1, 0 if t.K > 0 else -1], dtype=np.int32),
+        soil_particle_density=soil_particle_density, _desc=None)
+
+
 @dataclass
 class CellInputs:
     """Constraint definition of each cell: totals (mol/kg) of every primary
@@ -148,7 +169,7 @@ def host_state_arrays(t: ReactionTables, n: int, with_temperature: bool) -> Dict
     N, M, K, R = t.N, t.M, t.K, t.R
     z = lambda *s: np.zeros(s, dtype=np.float64)  # noqa: E731
     return {
-        "total": z(n, N), "total_sorb": z(n, N) if R > 0 else None, "pri_molal": np.full((n, N), 1e-9),
+        "total": z(n, N), "total_sorb": z(n, N) if t.has_sorption else None, "pri_molal": np.full((n, N), 1e-9),
         "pri_act_coef": np.ones((n, N)), "sec_act_coef": np.ones((n, max(M, 1))), "sec_molal": z(n, max(M, 1)),
         "ionx_ref_sorbed": np.full((n, max(R, 1)), 1e-9), "mnrl_volfrac": z(n, max(K, 1)),
         "mnrl_area": z(n, max(K, 1)), "mnrl_rate": z(n, max(K, 1)),

@@ -29,16 +29,21 @@ for name in ("carbonate-unit-activity", "carbonate-debye-huckel-activity",
 # global-implicit column decks whose golds pin the device time loop
 COLUMNS = {"1d-calcite": os.path.join("ascem", "1d", "1d-calcite"),
            "calcite_tran_only": os.path.join("shortcourse", "1D_Calcite"),
-           "valocchi_ionx": os.path.join("default", "column")}
+           "valocchi_ionx": os.path.join("default", "column"),
+           # isotherm sorption: the linear K_D column and its 1-cell decks
+           "solute_KD": os.path.join("default", "column"),
+           "solute_KD_w_mineral": os.path.join("default", "batch"),
+           "solute_KD_wo_mineral": os.path.join("default", "batch")}
 for name, sub in COLUMNS.items():
     shutil.copy(os.path.join(REF, "regression_tests", sub, f"{name}.in"), os.path.join(HERE, "decks", f"{name}.in"))
 
 # excerpt of database/hanford.dat: the species of the 1-D calcite columns
 # (5 primaries, 8 complexes, calcite; CaOH+ and the CO2(g) constraint of the
-# calcite transport decks) and of the Valocchi ion-exchange test
+# calcite transport decks), of the Valocchi ion-exchange test and of the
+# K_D decks (A(aq), A(s))
 KEEP = {"H2O", "H+", "HCO3-", "Ca++", "Mg++", "Cl-", "Na+", "CO2(g)", "CaOH+",
         "OH-", "CO2(aq)", "CO3--", "CaHCO3+", "CaCO3(aq)", "MgHCO3+", "MgCO3(aq)", "CaCl+",
-        "Calcite", "Halite"}
+        "Calcite", "Halite", "A(aq)", "A(s)"}
 out = []
 with open(os.path.join(REF, "database", "hanford.dat")) as f:
     for i, ln in enumerate(f):
@@ -79,8 +84,8 @@ with open(os.path.join(HERE, "batch_golds.json"), "w") as f:
 
 columns = {name: parse_gold(os.path.join(REF, "regression_tests", sub, f"{name}.regression.gold"))
            for name, sub in COLUMNS.items()}
-columns["_source"] = ("regression_tests/{ascem/1d/1d-calcite,shortcourse/1D_Calcite,default/column}/"
-                      "*.regression.gold (tolerance 1e-12 absolute in their .cfg files)")
+columns["_source"] = ("regression_tests/{ascem/1d/1d-calcite,shortcourse/1D_Calcite,default/column,"
+                      "default/batch}/*.regression.gold (tolerance 1e-12 absolute in their .cfg files)")
 with open(os.path.join(HERE, "column_golds.json"), "w") as f:
     json.dump(columns, f, indent=1, sort_keys=True)
 print("fixtures written to", HERE)

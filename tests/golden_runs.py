@@ -15,6 +15,9 @@ RUN_DECKS = {
     "1d-calcite": ("hanford_subset.dat", "column_golds.json"),
     "calcite_tran_only": ("hanford_subset.dat", "column_golds.json"),
     "valocchi_ionx": ("hanford_subset.dat", "column_golds.json"),
+    "solute_KD": ("hanford_subset.dat", "column_golds.json"),
+    "solute_KD_w_mineral": ("hanford_subset.dat", "column_golds.json"),
+    "solute_KD_wo_mineral": ("hanford_subset.dat", "column_golds.json"),
 }
 
 
@@ -48,6 +51,13 @@ def outputs(t, state, den):
         out[f"VOLUME_FRACTION: {mn} VF"] = state["mnrl_volfrac"][:, k]
         out[f"RATE: {mn} Rate"] = state["mnrl_rate"][:, k]
     return out
+
+
+def solution_norm(state):
+    """The transport solution's 2-norm as the gold's SOLUTION block prints
+    it: the solution vector holds every cell's free-ion molalities, also
+    under the log formulation (which changes the update, not the vector)."""
+    return float(np.linalg.norm(np.asarray(state["pri_molal"])))
 
 
 def differences(g, out):

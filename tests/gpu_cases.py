@@ -52,3 +52,30 @@ def conditioning_floor(t, solved, step_inputs, dt, nthreads=0):
                 continue   # singular: the free ions are not resolved at all
             fl[i] = 2.220446049250313e-16 * np.max(np.abs(Jinv) @ s[i])
     return np.where(np.isfinite(fl), fl, np.inf)
+
+
+def jacobian_bound(t, cells, jac_o):
+    """Per-entry bound of the analytic Jacobian's rounding: 1e-12 of the
+    magnitude of the terms summed into the entry (the complex terms
+    |nu_r nu_k| C_c / m_k cancel in off-diagonal entries, so either
+    implementation's rounding is relative to that sum, not to |J|); the
+    exchanger rows additionally carry the inner Newton's own 1e-12 relative
+    tolerance on KDj (reaction.F90:4407): 2e-12 of their row's largest entry.
+    cells must hold sec_molal at the state."""
+    import numpy as np
+
+    from pflotran_amd import synth
+    m = cells["pri_molal"]
+    sec = cells["sec_molal"][:, :t.M]
+    absnu = np.zeros((t.M, t.N))
+    for i in range(t.M):
+        for s_ in range(t.cplx_specid[i, 0]):
+            absnu[i, t.cplx_specid[i, 1 + s_]] += abs(t.cplx_stoich[i, s_])
+    psvd = 0.25 * 1.0 * 1000.0 * 1.0 / synth.STEP_DT * (synth.DEN_25C * 1e-3)
+    terms = psvd * (np.eye(t.N)[None] + np.einsum("ci,ck,nc->nik", absnu, absnu, sec) / m[:, None, :])
+    bound = 1e-12 * (terms + np.abs(jac_o))
+    cation = np.zeros(t.N, dtype=bool)
+    cation[t.ionx_cationid[0, 1:1 + t.ionx_cationid[0, 0]]] = True
+    rowmax = np.max(np.abs(jac_o), axis=2, keepdims=True)
+    bound[:, cation, :] = np.broadcast_to(2e-12 * rowmax, bound.shape)[:, cation, :]
+    return bound

@@ -8,7 +8,7 @@ import pytest
 import oracle as orc
 from chemdefs import calcite_kinetics, ca_carbonate, carbonate, carbonate_column, valocchi
 from conftest import gold
-from gpu_cases import host_arrays, speciated_state
+from gpu_cases import host_arrays, jacobian_bound, speciated_state
 from pflotran_amd import abi, runtime, synth
 from pflotran_amd.database import water_density_ifc67
 
@@ -26,30 +26,6 @@ def net15():
     return synth.random_network(15, 50, 3, ionx=True)
 
 
-def _jacobian_bound(t, cells, jac_o):
-    """Per-entry bound of the analytic Jacobian's rounding: 1e-12 of the
-    magnitude of the terms summed into the entry (the complex terms
-    |nu_r nu_k| C_c / m_k cancel in off-diagonal entries, so either
-    implementation's rounding is relative to that sum, not to |J|); the
-    exchanger rows additionally carry the inner Newton's own 1e-12 relative
-    tolerance on KDj (reaction.F90:4407): 2e-12 of their row's largest entry.
-    cells must hold sec_molal at the state."""
-    m = cells["pri_molal"]
-    sec = cells["sec_molal"][:, :t.M]
-    absnu = np.zeros((t.M, t.N))
-    for i in range(t.M):
-        for s_ in range(t.cplx_specid[i, 0]):
-            absnu[i, t.cplx_specid[i, 1 + s_]] += abs(t.cplx_stoich[i, s_])
-    psvd = 0.25 * 1.0 * 1000.0 * 1.0 / synth.STEP_DT * (synth.DEN_25C * 1e-3)
-    terms = psvd * (np.eye(t.N)[None] + np.einsum("ci,ck,nc->nik", absnu, absnu, sec) / m[:, None, :])
-    bound = 1e-12 * (terms + np.abs(jac_o))
-    cation = np.zeros(t.N, dtype=bool)
-    cation[t.ionx_cationid[0, 1:1 + t.ionx_cationid[0, 0]]] = True
-    rowmax = np.max(np.abs(jac_o), axis=2, keepdims=True)
-    bound[:, cation, :] = np.broadcast_to(2e-12 * rowmax, bound.shape)[:, cation, :]
-    return bound
-
-
 def test_residual_jacobian_parity(net15):
     t = net15
     n = 333
@@ -62,7 +38,7 @@ def test_residual_jacobian_parity(net15):
     tab = runtime.Tables(t)
     dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
     res_g, jac_g = runtime.residual_jacobian(tab, dc, synth.react_opts())
-    assert np.all(np.abs(jac_g - jac_o) <= _jacobian_bound(t, cells, jac_o))
+    assert np.all(np.abs(jac_g - jac_o) <= jacobian_bound(t, cells, jac_o))
     psv = 0.25 * 1.0 * 1000.0 * 1.0
     acc = psv * np.abs(cells["total"]) / synth.STEP_DT
     assert np.max(np.abs(res_g - res_o) / (acc + np.abs(res_o) + 1e-300)) <= 1e-12
@@ -96,7 +72,7 @@ def test_residual_jacobian_parity_numerical():
             nu[k, t.mnrl_specid[k, 1 + s_]] = t.mnrl_stoich[k, s_]
     S = np.abs(cells["mnrl_rate"]) @ np.abs(nu)               # [cell][row], V = 1 m^3
     noise = 1e-10 * S[:, :, None] / cells["pri_molal"][:, None, :]
-    excess = np.abs(jac_g - jac_o) - _jacobian_bound(t, cells, jac_o) - noise - 1e-8 * fdscale
+    excess = np.abs(jac_g - jac_o) - jacobian_bound(t, cells, jac_o) - noise - 1e-8 * fdscale
     worst = np.unravel_index(np.argmax(excess), excess.shape)
     assert excess[worst] <= 0.0, (worst, jac_g[worst], jac_o[worst], fdscale[worst[0], 0, worst[2]])
     psv = 0.25 * 1.0 * 1000.0 * 1.0
@@ -405,7 +381,7 @@ def test_mineral_option_parity():
     res_g, jac_g = runtime.residual_jacobian(tab, dc, synth.react_opts())
     rate_g = dc.to_host()["mnrl_rate"]
     assert np.count_nonzero(rate_o) > n and np.all(np.abs(rate_g - rate_o) <= 1e-12 * np.abs(rate_o))
-    assert np.all(np.abs(jac_g - jac_o) <= _jacobian_bound(t, cells, jac_o))
+    assert np.all(np.abs(jac_g - jac_o) <= jacobian_bound(t, cells, jac_o))
     psv = 0.25 * 1.0 * 1000.0 * 1.0
     acc = psv * np.abs(cells["total"]) / synth.STEP_DT
     assert np.max(np.abs(res_g - res_o) / (acc + np.abs(res_o) + 1e-300)) <= 1e-12
