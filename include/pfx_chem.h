@@ -332,14 +332,18 @@ int pfx_react_batch(pfx_tables *tables, pfx_cells *cells,
    NULL, the per-cell iteration counts, status and scaled residuals live in
    a fixed device array of the library (PFX_NOALLOC_SCRATCH_BYTES on each
    GPU, held until the process ends; its address is taken at
-   pfx_tables_create), never in memory allocated by the call.  Limit: with
-   any of those per-cell outputs NULL, at most PFX_NOALLOC_MAX_CELLS cells
-   (PFX_E_ARG beyond).  Every call on one GPU shares the array, which keeps
-   the last call's contents: calls that overlap on one GPU race.
-   opts->stream_ordered must be 0 (the statistics are read back before the
-   call returns).  Otherwise identical to pfx_react_batch. */
+   pfx_tables_create), never in memory allocated by the call, and so does
+   the list of cells left to the pivoting launch.  Limit: with any of those
+   per-cell outputs NULL, at most PFX_NOALLOC_MAX_CELLS cells (PFX_E_ARG
+   beyond); larger batches (per-cell outputs given) factor with pivoting
+   switched on per cell inside one launch instead of a follow-up launch,
+   which gives the same converged cells up to rounding.  Every call on one
+   GPU shares the array, which keeps the last call's contents: calls that
+   overlap on one GPU race.  opts->stream_ordered must be 0 (the statistics
+   are read back before the call returns).  Otherwise identical to
+   pfx_react_batch. */
 #define PFX_NOALLOC_MAX_CELLS (1 << 20)
-#define PFX_NOALLOC_SCRATCH_BYTES (1024 + 16 * PFX_NOALLOC_MAX_CELLS)
+#define PFX_NOALLOC_SCRATCH_BYTES (1280 + 20 * PFX_NOALLOC_MAX_CELLS)
 int pfx_react_batch_noalloc(pfx_tables *tables, pfx_cells *cells, const pfx_react_opts *opts, pfx_stats *stats);
 
 /* Statistics of the last pfx_react_batch made with opts->stream_ordered = 1

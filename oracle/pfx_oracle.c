@@ -861,10 +861,14 @@ restart:
          only linearly (a Jacobian that is not the residual's derivative);
          the iterate itself is returned, its residual having passed */
       const double rho = fmin(isfinite(last_change) && last_change > 0.0 ? maxrel / last_change : 0.0, 0.9);
-      /* a step within twice the FP64 resolution is rounding noise */
+      /* where the steps stagnate (rho >= 1/2), a step within twice the FP64
+         resolution is rounding noise */
       double frel = 0.0;
       for (int j = 0; j < N; j++) frel = fmax(frel, fabs(d->use_log_formulation ? fupd[j] : fupd[j] / prev[j]));
-      if (maxrel / (1.0 - rho) <= o->rel_change_tol || maxrel <= 2.0 * frel) { status = PFX_CELL_CONVERGED; break; }
+      if (maxrel / (1.0 - rho) <= o->rel_change_tol || (rho >= 0.5 && maxrel <= 2.0 * frel)) {
+        status = PFX_CELL_CONVERGED;
+        break;
+      }
       if (it >= cap) { status = PFX_CELL_MAX_ITERATIONS; break; }
     }
     it++;

@@ -14,7 +14,7 @@
 
 #include "pfx_device.h"
 
-template <int G, int MODE, int W>
+template <int G, int MODE, int W, int LUP>
 __global__ void pfx_react_kernel(const pfx_kernel_params P);
 template <int W>
 __global__ void pfx_react_zones_kernel(const pfx_kernel_params *zone_params, const pfx_zone_launch Z,
@@ -29,6 +29,8 @@ struct pfx_tables {
   int32_t *d_iters = nullptr, *d_status = nullptr;
   double *d_relres = nullptr;
   int64_t scratch_cells = 0;
+  unsigned int *d_defer = nullptr;        // deferred-factorisation list: count, cells
+  int64_t defer_cells = 0;
   unsigned long long *d_stats = nullptr;  // reduction buffer
   double *d_gi = nullptr;                 // global-implicit per-cell scratch
   pfx_constraint_rxn *d_crxn = nullptr;   // mineral / gas constraint reactions [N]
@@ -55,11 +57,12 @@ struct pfx_tables {
 // outputs, the per-cell iteration counts, status and scaled residuals.  Each
 // GPU holds its own copy until the process ends; every call on a GPU shares
 // it (overlapping calls on one GPU race).
-constexpr size_t kNoallocQueueOff = 0;      // uint32 [2]
+constexpr size_t kNoallocQueueOff = 0;      // uint32 [3]
 constexpr size_t kNoallocStatsOff = 256;    // uint64 [64]
 constexpr size_t kNoallocCellsOff = 1024;   // int32 iters [cap], int32 status [cap], double rel_res [cap]
+constexpr size_t kNoallocDeferOff = kNoallocCellsOff + (size_t)PFX_NOALLOC_MAX_CELLS * 16;  // uint32 [cap + 1]
 static __device__ __align__(256) unsigned char scratch[PFX_NOALLOC_SCRATCH_BYTES];
-static_assert(kNoallocCellsOff + (size_t)PFX_NOALLOC_MAX_CELLS * 16 <= PFX_NOALLOC_SCRATCH_BYTES,
+static_assert(kNoallocDeferOff + ((size_t)PFX_NOALLOC_MAX_CELLS + 1) * 4 <= PFX_NOALLOC_SCRATCH_BYTES,
               "scratch too small for its per-cell slots");
 
 namespace {
@@ -118,6 +121,12 @@ __global__ void k_zero_u32(unsigned int *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
 }
+// a reaction launch's counters: queue cursors and fallback count, and the
+// count of the deferred list (if any)
+__global__ void k_zero_launch(unsigned int *queue, unsigned int *defer) {
+  if (threadIdx.x < 3) queue[threadIdx.x] = 0u;
+  if (threadIdx.x == 0 && defer) defer[0] = 0u;
+}
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -171,6 +180,16 @@ __global__ void k_kinetic_state(const unsigned char *blob, int64_t o_molvol, int
   volfrac[i] = v < 0.0 ? 0.0 : v;
 }
 
+// the deferred-factorisation list for n cells (count + n indices)
+int ensure_defer(pfx_tables *t, int64_t n) {
+  if (n <= t->defer_cells) return 0;
+  if (t->d_defer) HIPCHK(hipFree(t->d_defer));
+  t->d_defer = nullptr;
+  HIPCHK(hipMalloc(&t->d_defer, (size_t)(n + 1) * sizeof(unsigned int)));
+  t->defer_cells = n;
+  return 0;
+}
+
 int ensure_scratch(pfx_tables *t, int64_t n) {
   if (n <= t->scratch_cells) return 0;
   if (t->d_iters) HIPCHK(hipFree(t->d_iters));
@@ -186,25 +205,34 @@ int ensure_scratch(pfx_tables *t, int64_t n) {
   return 0;
 }
 
-template <int MODE, int W>
+template <int MODE, int W, int LUP>
 const void *kernel_for(int G) {
-  return G == 8 ? (const void *)pfx_react_kernel<8, MODE, W>
-       : G == 16 ? (const void *)pfx_react_kernel<16, MODE, W>
-                 : (const void *)pfx_react_kernel<32, MODE, W>;
+  return G == 8 ? (const void *)pfx_react_kernel<8, MODE, W, LUP>
+       : G == 16 ? (const void *)pfx_react_kernel<16, MODE, W, LUP>
+                 : (const void *)pfx_react_kernel<32, MODE, W, LUP>;
 }
 
 // variant v caps the registers at v + 1 wavefronts per SIMD (v = 0: none);
-// only the Newton modes have the capped variants
-const void *kernel_ptr(int G, int mode, int v) {
+// only the Newton modes have the capped variants.  lup: the reaction
+// kernel's factorisation policy (PFX_LU_*; the pivoting one has no capped
+// variants, the other modes always pivot)
+const void *kernel_ptr(int G, int mode, int v, int lup = PFX_LU_DEFER) {
   switch (mode) {
     case PFX_MODE_REACT:
-      return v == 2 ? kernel_for<PFX_MODE_REACT, 3>(G)
-           : v == 1 ? kernel_for<PFX_MODE_REACT, 2>(G) : kernel_for<PFX_MODE_REACT, 1>(G);
-    case PFX_MODE_EVAL: return kernel_for<PFX_MODE_EVAL, 1>(G);
+      if (lup == PFX_LU_PIVOT) return kernel_for<PFX_MODE_REACT, 1, PFX_LU_PIVOT>(G);
+      if (lup == PFX_LU_ONDEMAND)
+        return v == 2 ? kernel_for<PFX_MODE_REACT, 3, PFX_LU_ONDEMAND>(G)
+             : v == 1 ? kernel_for<PFX_MODE_REACT, 2, PFX_LU_ONDEMAND>(G)
+                      : kernel_for<PFX_MODE_REACT, 1, PFX_LU_ONDEMAND>(G);
+      return v == 2 ? kernel_for<PFX_MODE_REACT, 3, PFX_LU_DEFER>(G)
+           : v == 1 ? kernel_for<PFX_MODE_REACT, 2, PFX_LU_DEFER>(G)
+                    : kernel_for<PFX_MODE_REACT, 1, PFX_LU_DEFER>(G);
+    case PFX_MODE_EVAL: return kernel_for<PFX_MODE_EVAL, 1, PFX_LU_PIVOT>(G);
     case PFX_MODE_EQUIL:
-      return v == 2 ? kernel_for<PFX_MODE_EQUIL, 3>(G)
-           : v == 1 ? kernel_for<PFX_MODE_EQUIL, 2>(G) : kernel_for<PFX_MODE_EQUIL, 1>(G);
-    default: return kernel_for<PFX_MODE_AUX, 1>(G);
+      return v == 2 ? kernel_for<PFX_MODE_EQUIL, 3, PFX_LU_PIVOT>(G)
+           : v == 1 ? kernel_for<PFX_MODE_EQUIL, 2, PFX_LU_PIVOT>(G)
+                    : kernel_for<PFX_MODE_EQUIL, 1, PFX_LU_PIVOT>(G);
+    default: return kernel_for<PFX_MODE_AUX, 1, PFX_LU_PIVOT>(G);
   }
 }
 
@@ -213,8 +241,12 @@ const void *zones_kernel_ptr(int v) {
        : v == 1 ? (const void *)pfx_react_zones_kernel<2> : (const void *)pfx_react_zones_kernel<1>;
 }
 
-// queue: the work-queue cursor and fallback counter of this launch (the
-// tables' own, or the fixed scratch array of pfx_react_batch_noalloc)
+// queue: the work-queue cursors and fallback counter of this launch (the
+// tables' own, or the fixed scratch array of pfx_react_batch_noalloc).
+// A reaction step with a deferred list (P.defer) is two launches: the
+// unpivoted kernel over the batch, then the pivoting kernel over the cells
+// it listed (an empty list ends that launch at once); otherwise a reaction
+// step factors with on-demand pivoting in one launch.
 int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, double *ms,
            unsigned int *queue = nullptr) {
   if (!queue) queue = t->d_queue;
@@ -223,8 +255,14 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   const int64_t lds = t->L.lds_table_bytes + (int64_t)groups * pfx_scr_doubles(t->L.M, G, t->L.K) * 8;
   if (lds > kMaxLds) return PFX_E_UNSUP;
   int occ = 0;
-  const void *fn = kernel_ptr(G, mode, t->variant[mode & 3]);
-  if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const bool deferring = mode == PFX_MODE_REACT && P.defer != nullptr;
+  const int lup = mode != PFX_MODE_REACT ? PFX_LU_PIVOT : (deferring ? PFX_LU_DEFER : PFX_LU_ONDEMAND);
+  const void *fn = kernel_ptr(G, mode, t->variant[mode & 3], lup);
+  const void *fn_piv = deferring ? kernel_ptr(G, mode, 0, PFX_LU_PIVOT) : nullptr;
+  if (lds > 64 * 1024) {
+    HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (fn_piv) HIPCHK(hipFuncSetAttribute(fn_piv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BLOCK, (size_t)lds));
   if (occ < 1) occ = 1;
   int64_t need_blocks = (P.ncells + groups - 1) / groups;
@@ -236,16 +274,25 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   P.chunk = (int32_t)std::max<int64_t>(64 / G, std::min<int64_t>(64, per_wave / 8 + 1));
   P.queue = queue;
   P.fallback_count = queue + 1;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, queue, 2);  // cursor, fallbacks
+  // the pivoting launch: its own cursor, one cell per fetch, one workgroup
+  // per compute unit at most (the list is short)
+  pfx_kernel_params P2 = P;
+  P2.queue = queue + 2;
+  P2.chunk = 1;
+  const int64_t blocks2 = std::max<int64_t>(1, std::min<int64_t>(need_blocks, (int64_t)t->num_cu));
+  hipLaunchKernelGGL(k_zero_launch, dim3(1), dim3(64), 0, stream, queue, P.defer);
   HIPCHK(hipGetLastError());
   void *args[] = {(void *)&P};
+  void *args2[] = {(void *)&P2};
   if (!ms) {
-    // stream-ordered: no events, no wait (the launch may be graph-captured)
+    // stream-ordered: no events, no wait (the launches may be graph-captured)
     HIPCHK(hipLaunchKernel(fn, dim3((unsigned)blocks), dim3(BLOCK), args, (size_t)lds, stream));
+    if (fn_piv) HIPCHK(hipLaunchKernel(fn_piv, dim3((unsigned)blocks2), dim3(BLOCK), args2, (size_t)lds, stream));
     return 0;
   }
   HIPCHK(hipEventRecord(t->ev0, stream));
   HIPCHK(hipLaunchKernel(fn, dim3((unsigned)blocks), dim3(BLOCK), args, (size_t)lds, stream));
+  if (fn_piv) HIPCHK(hipLaunchKernel(fn_piv, dim3((unsigned)blocks2), dim3(BLOCK), args2, (size_t)lds, stream));
   HIPCHK(hipEventRecord(t->ev1, stream));
   HIPCHK(hipEventSynchronize(t->ev1));
   float f = 0.f;
@@ -408,7 +455,10 @@ int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, c
     fill_params(t, &cells[z], o, zs.P[z]);
     int rc = use_scratch(t, zs.P[z], cells[z].ncells);
     if (rc) return rc;
+    if ((rc = ensure_defer(t, cells[z].ncells))) return rc;
+    zs.P[z].defer = t->d_defer;
     zs.P[z].queue = t->d_queue;
+    zs.P[z].fallback_count = t->d_queue + 1;
     zs.n[z] = cells[z].ncells;
     zs.Z.G[z] = t->L.G;
     zs.lds = std::max<int64_t>(zs.lds, zone_lds(t));
@@ -447,16 +497,30 @@ int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, c
   return 0;
 }
 
-// queue resets, start stamp, zoned kernel, statistics (all kernels)
+// queue resets, start stamp, zoned kernel, each zone's pivoting launch over
+// the cells the zoned kernel deferred, statistics (all kernels)
 int zones_enqueue(pfx_tables *const *tables, zone_setup &zs, const pfx_kernel_params *d_params, uint64_t *d_finish,
                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   for (int z = 0; z < zs.nzones; z++)
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, tables[z]->d_queue, 2);
+    hipLaunchKernelGGL(k_zero_launch, dim3(1), dim3(64), 0, s, tables[z]->d_queue, zs.P[z].defer);
   hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, d_finish + zs.total_blocks);
   HIPCHK(hipGetLastError());
   if (ev0) HIPCHK(hipEventRecord(ev0, s));
   void *args[] = {(void *)&d_params, (void *)&zs.Z, (void *)&d_finish};
   HIPCHK(hipLaunchKernel(zs.fn, dim3((unsigned)zs.total_blocks), dim3(BLOCK), args, (size_t)zs.lds, s));
+  for (int z = 0; z < zs.nzones; z++) {
+    pfx_tables *t = tables[z];
+    pfx_kernel_params P2 = zs.P[z];
+    P2.queue = t->d_queue + 2;
+    P2.chunk = 1;
+    const int64_t lds = zone_lds(t);
+    const void *fn_piv = kernel_ptr(t->L.G, PFX_MODE_REACT, 0, PFX_LU_PIVOT);
+    if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn_piv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int64_t need = (zs.n[z] + (BLOCK / t->L.G) - 1) / (BLOCK / t->L.G);
+    const int64_t blocks2 = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)t->num_cu));
+    void *args2[] = {(void *)&P2};
+    HIPCHK(hipLaunchKernel(fn_piv, dim3((unsigned)blocks2), dim3(BLOCK), args2, (size_t)lds, s));
+  }
   if (ev1) HIPCHK(hipEventRecord(ev1, s));
   for (int z = 0; z < zs.nzones; z++) {
     int rc = stats_launch(tables[z], zs.P[z], zs.n[z], s);
@@ -864,6 +928,7 @@ extern "C" int pfx_tables_destroy(pfx_tables *t) {
   if (t->d_iters) (void)hipFree(t->d_iters);
   if (t->d_status) (void)hipFree(t->d_status);
   if (t->d_relres) (void)hipFree(t->d_relres);
+  if (t->d_defer) (void)hipFree(t->d_defer);
   if (t->ev0) (void)hipEventDestroy(t->ev0);
   if (t->ev1) (void)hipEventDestroy(t->ev1);
   delete t;
@@ -885,6 +950,8 @@ extern "C" int pfx_react_batch(pfx_tables *t, pfx_cells *cells, const pfx_react_
   pfx_kernel_params P;
   fill_params(t, cells, o, P);
   if ((rc = use_scratch(t, P, cells->ncells))) return rc;
+  if ((rc = ensure_defer(t, cells->ncells))) return rc;
+  P.defer = t->d_defer;
   if (o->stream_ordered) {
     // enqueue the step and its statistics reduction; pfx_tables_stats
     // reads them later
@@ -950,6 +1017,8 @@ extern "C" int pfx_react_batch_noalloc(pfx_tables *t, pfx_cells *cells, const pf
     if (!P.status) P.status = (int32_t *)(base + kNoallocCellsOff + cap * 4);
     if (!P.rel_res) P.rel_res = (double *)(base + kNoallocCellsOff + cap * 8);
   }
+  // the deferred list where it fits (its count is cleared by the launch)
+  if (cells->ncells <= PFX_NOALLOC_MAX_CELLS) P.defer = (unsigned int *)(base + kNoallocDeferOff);
   double ms = 0.0;
   if ((rc = launch(t, P, PFX_MODE_REACT, stream, &ms, queue))) return rc;
   if ((rc = stats_launch(t, P, cells->ncells, stream, sbuf))) return rc;

@@ -108,7 +108,23 @@ struct pfx_kernel_params {
   // work queue
   unsigned int *queue;          // global cell cursor
   int32_t chunk;                // cells taken per fetch
+  // cells whose unpivoted factorisation broke down: defer[0] their count,
+  // defer[1..] their indices (written by the unpivoted reaction kernel,
+  // worked through by the pivoting one); null where the kernel pivots
+  // itself
+  unsigned int *defer;
 };
+
+// Factorisation policy of a reaction-step kernel (template argument):
+//   PFX_LU_DEFER     unpivoted; a cell whose factorisation breaks down, or
+//                    that does not converge, is left untouched and listed in
+//                    defer[] for a pivoting launch
+//   PFX_LU_PIVOT     partial pivoting; with defer[] given the launch works
+//                    through the listed cells instead of the batch
+//   PFX_LU_ONDEMAND  unpivoted; such a cell pivots from its next pass on
+#define PFX_LU_DEFER 0
+#define PFX_LU_PIVOT 1
+#define PFX_LU_ONDEMAND 2
 
 // Sizes of a network held by an opaque pfx_tables (defined in pfx_api.hip)
 // for the other translation units of the library.

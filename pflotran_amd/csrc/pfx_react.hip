@@ -154,6 +154,36 @@ __device__ __forceinline__ int gargmax_row(double cand, int lane, int gbase) {
   return gm ? 31 - __clz((int)gm) : lane - gbase;
 }
 
+// compile-time loop: f(Ic<J>) for J = B .. E-1
+template <int V>
+struct Ic {
+  static constexpr int value = V;
+};
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(Ic<B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Lane j's value of a group, to every lane of the group.  G = 16: one DPP
+// row broadcast (a VALU move, no LDS); G = 8: the two groups of a DPP row
+// take lane j and lane 8 + j; G = 32: a lane permute (the group spans two
+// DPP rows).  J is a compile-time lane index.
+template <int G, int J>
+__device__ __forceinline__ double gbcast(double v, int gbase) {
+  if constexpr (G == 16) {
+    return dpp_d<0x150 + J>(v);
+  } else if constexpr (G == 8) {
+    const double a = dpp_d<0x150 + J>(v);
+    const double b = dpp_d<0x150 + 8 + J>(v);
+    return (gbase & 8) ? b : a;
+  } else {
+    return bperm_d(v, gbase + J);
+  }
+}
+
 __device__ __forceinline__ void wave_sync() {
   // LDS written by other lanes of this wavefront becomes visible to the
   // whole wavefront (lanes of one group never span wavefronts).
@@ -342,10 +372,14 @@ __device__ __forceinline__ double ld_or(const double *a, long long i, const unsi
   return *(a ? a + i : (const double *)dummy);
 }
 
-template <int G, int MODE>
+template <int G, int MODE, int LUP>
 __device__ __forceinline__ void react_body(uint64_t praw) {
   const KP P0 = (KP)praw;
   const int N = P0->L.N, M = P0->L.M, K = P0->L.K, R = P0->L.R;
+  // a pivoting reaction launch given the deferred list works through it
+  constexpr bool kCanDefer = MODE == PFX_MODE_REACT && LUP == PFX_LU_DEFER;
+  const bool from_list = MODE == PFX_MODE_REACT && LUP == PFX_LU_PIVOT && P0->defer != nullptr;
+  const long long ncl = from_list ? (long long)P0->defer[0] : P0->ncells;
   const int lds_tab = (int)P0->L.lds_table_bytes;
 
   // stage the reaction tables into LDS (16-byte words)
@@ -385,6 +419,13 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   bool fell_back = false;   // warm attempt abandoned, restarted from the default guess
   bool stop_next = false;   // reference relative-change stop: finish at the next evaluation
   bool reload = false;      // load the current cell again (fallback restart)
+  bool lu_pivot = false;    // this cell factors with row pivoting (its unpivoted LU broke down)
+  bool deferred = false;    // left to the pivoting launch (factorisation broke down, or not converged)
+  // step test at a stagnating candidate: this pass solves for the FP64
+  // resolution of the free ions at the candidate iterate, whose step
+  // (m_cand, its size rel_cand) waits
+  bool floor_pass = false;
+  double rel_cand = 0.0, m_cand = 0.0;
   double last_change = INFINITY;  // max|dm/m| of the attempt's last Newton update
   int cell_flag = PFX_CELL_CONVERGED;
   double m = 1.0, gam = 1.0, lng = 0.0;
@@ -423,8 +464,8 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           base = __shfl(base, 0);
           q_next = base;
           q_end = (long long)base + P.chunk;
-          if (q_end > P.ncells) q_end = P.ncells;
-          if (q_next >= P.ncells) {
+          if (q_end > ncl) q_end = ncl;
+          if (q_next >= ncl) {
             exhausted = true;
             break;
           }
@@ -436,14 +477,26 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         nneed -= (int)take;
         q_next += take;
       }
+      // a listed cell: its index, and bit 31 set where it failed after its
+      // warm attempt had been given up (the pivoting launch then repeats the
+      // plain attempt, counting on from the iterations already spent)
+      bool listed_fb = false;
+      if (from_list && mycell >= 0) mycell = (long long)P.defer[1 + mycell];
       if (need) {
         int lo = bperm_i((int)(mycell & 0xffffffff), gbase);
         int hi = bperm_i((int)(mycell >> 32), gbase);
         long long c = ((long long)hi << 32) | (unsigned int)lo;
+        if (from_list && c >= 0) {
+          listed_fb = (c >> 31) & 1;
+          c &= 0x7fffffff;
+        }
         if (c >= 0) {
           cell = c;
-          it = 0;
-          fell_back = false;
+          it = listed_fb ? P.num_iterations[c] : 0;
+          fell_back = listed_fb;
+          lu_pivot = false;
+          deferred = false;
+          floor_pass = false;
           load_now = true;
         }
       }
@@ -603,7 +656,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     // reaction.F90:3812-4053)
     bool act_now = false;
     if (MODE == PFX_MODE_REACT)
-      act_now = phase == 1 || (phase == 2 && L.act_freq == PFX_ACT_NEWTON_ITER && !stop_next);
+      act_now = phase == 1 || (phase == 2 && L.act_freq == PFX_ACT_NEWTON_ITER && !stop_next && !floor_pass);
     else if (MODE == PFX_MODE_EQUIL)
       act_now = compute_act && L.act_freq != PFX_ACT_OFF;
     else if (MODE == PFX_MODE_AUX)
@@ -1111,15 +1164,12 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     PFX_PT(6);
     // RSolve: row equilibration, ln-column scaling, pivoted LU, solve
     double upd = 0.0;
-    double fl = 0.0;  // floor solve: the step a rounding-level residual would cause
+    bool redo = false;  // the unpivoted LU broke down
     if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
-      // At a step-test candidate the same factors also solve for b2, the
-      // rounding level of each residual row (unit roundoff times the
-      // magnitude of the row's terms): its solution is the resolution of
-      // the free ions in FP64.  Wave-uniform switch, so the lane moves of
-      // the extra right-hand side run with every lane of each group.
-      const bool floor_solve = MODE == PFX_MODE_REACT && __ballot(candidate) != 0ull;
-      double b2 = 0.0;
+      // A floor pass solves for the rounding level of each residual row
+      // (unit roundoff times the magnitude of the row's terms) instead of
+      // the residual: its solution is the resolution of the free ions in
+      // FP64.
       // Straight-line code: the steps run over all G rows with selects
       // instead of branches (a divergent branch, or a step skipped for
       // j >= N, makes the compiler copy the whole row at every merge).
@@ -1133,8 +1183,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 #pragma unroll
         for (int k = 0; k < G; k++) mx = fmax(mx, fabs(row[k]));
         const double norm = live ? 1.0 / fmax(1.0, mx) : 1.0;
-        b = live ? Rr * norm : 0.0;
-        b2 = (live && candidate) ? 2.220446049250313e-16 * scale * norm : 0.0;
+        b = live ? (floor_pass ? 2.220446049250313e-16 * scale : Rr) * norm : 0.0;
         const bool lg = live && use_log;
 #pragma unroll
         for (int k = 0; k < G; k++) {
@@ -1153,56 +1202,82 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         vv = mx > 0.0 ? 1.0 / mx : 0.0;
         if (live && !(mx > 0.0)) status = PFX_CELL_SINGULAR;
       }
-      bool pivoted = false;
-      int mystep = -1;  // elimination step at which this lane's row was the pivot
+      // Elimination with the pivot row of step j in lane j, broadcast across
+      // the group.  The reaction step's Jacobian in ln m is the
+      // accumulation's diagonal plus positive semi-definite complexation,
+      // exchange and mineral terms (nu^T diag(C) nu, diag(X) - X X^T,
+      // nu nu^T dI/dlnQ) with rows scaled by positive factors, so it needs
+      // no row exchanges, and the reaction kernels factor without a pivot
+      // search.  A zero pivot or a non-finite multiplier (far from the
+      // solution, or where a rate-law option makes a mineral term
+      // indefinite) breaks the unpivoted factorisation down: under
+      // PFX_LU_DEFER the cell is listed and left untouched for the pivoting
+      // launch, which redoes its step from the start; under
+      // PFX_LU_ONDEMAND its wave pivots from the cell's next pass on, at
+      // the same iterate.  Pivoting (the constraint solve always, and
+      // PFX_LU_PIVOT): before step j the largest scaled candidate of the
+      // rows not yet eliminated (implicit row scaling vv, ties to the larger
+      // row as the reference's `>=` scan, utility.F90:451-452) trades places
+      // with row j, lane to lane, with its right-hand side and scale.
+      // Padding lanes (r >= N) hold identity rows with a zero right-hand
+      // side and zeros in the live rows' columns: they are never exchanged
+      // and leave the live rows' elimination unchanged.
+      constexpr bool kCanPivot = MODE != PFX_MODE_REACT || LUP != PFX_LU_DEFER;
+      const bool pivoting = MODE != PFX_MODE_REACT || LUP == PFX_LU_PIVOT ||
+                            (LUP == PFX_LU_ONDEMAND && __ballot(active && !finish && lu_pivot) != 0ull);
+      {
+        bool lbad = false;
+        double inv_diag = 0.0;  // 1 / U_rr, kept by lane r for the back substitution
+        static_for<0, G>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if (kCanPivot && pivoting && j < N) {
+            const double cand = (r >= j && live) ? vv * fabs(row[j]) : -1.0;
+            const int pr = gargmax_row<G>(cand, lane, gbase);
+            const int src = gbase + ((r == j) ? pr : ((r == pr) ? j : r));
 #pragma unroll
-      for (int j = 0; j < G; j++) {
-        const bool in_play = live || j >= N;  // padding rows join after step N
-        double cand = (pivoted || !in_play) ? -1.0 : vv * fabs(row[j]);
-        const int p = gargmax_row<G>(cand, lane, gbase);
-        const int src = gbase + p;
-        // stream the pivot row one element at a time (lane to lane; an LDS
-        // broadcast of the row measured slower: it adds a store-to-load
-        // wait to every step)
-        double piv = bperm_d(row[j], src);
-        piv = piv == 0.0 ? 1.0e-20 : piv;
-        const bool is_piv = r == p;
-        const bool elim = !is_piv && !pivoted && in_play;
-        const double q = row[j] / piv;
-        const double f = elim ? q : 0.0;  // 0 leaves the row unchanged below
-        row[j] = is_piv ? piv : (elim ? q : row[j]);
-        mystep = is_piv ? j : mystep;
-        pivoted = pivoted || is_piv;
+            for (int k = 0; k < G; k++) {
+              row[k] = bperm_d(row[k], src);
+              if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+            }
+            b = bperm_d(b, src);
+            vv = bperm_d(vv, src);
+          }
+          double piv = gbcast<G, j>(row[j], gbase);
+          if (pivoting) piv = piv == 0.0 ? 1.0e-20 : piv;  // ludcmp's TINY (utility.F90:462)
+          const double ip = 1.0 / piv;
+          const bool elim = r > j;
+          // pivoting divides, as the reference's elimination does; the
+          // unpivoted path multiplies by the reciprocal
+          const double f = elim ? (pivoting ? row[j] / piv : row[j] * ip) : 0.0;
+          lbad = lbad || (r == j && !(fabs(piv) > 0.0)) || !(fabs(f) <= 1.0e300);
+          inv_diag = (r == j) ? ip : inv_diag;
+          row[j] = (r == j) ? piv : (elim ? f : row[j]);
 #pragma unroll
-        for (int k = j + 1; k < G; k++) {
-          const double pk = bperm_d(row[k], src);
-          row[k] -= f * pk;
-          // at most 8 pivot-row elements in flight: hoisting all of them
-          // costs 15-20 more registers (and spills at 3 wavefronts/SIMD)
-          if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-        }
-        const double pb = bperm_d(b, src);
-        b -= f * pb;
-        if (floor_solve) {
-          const double pb2 = bperm_d(b2, src);
-          b2 -= f * pb2;
-        }
-      }
-      // back substitution: the pivot row of step j is held by the lane whose
-      // mystep is j, the only non-zero term of the group sum that spreads
-      // x_j; rows pivoted before step j still need x_j, later ones are done
-#pragma unroll
-      for (int j = G - 1; j >= 0; j--) {
-        const double xj = gsum<G>(mystep == j ? b / row[j] : 0.0, lane);
-        upd = (r == j && live) ? xj : upd;
-        b -= row[j] * xj;
-      }
-      if (floor_solve) {
-#pragma unroll
-        for (int j = G - 1; j >= 0; j--) {
-          const double xj = gsum<G>(mystep == j ? b2 / row[j] : 0.0, lane);
-          fl = (r == j && live) ? xj : fl;
-          b2 -= row[j] * xj;
+          for (int k = j + 1; k < G; k++) {
+            const double pk = gbcast<G, j>(row[k], gbase);
+            row[k] -= f * pk;
+            if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+          }
+          const double pb = gbcast<G, j>(b, gbase);
+          b -= f * pb;
+        });
+        static_for<0, G>([&](auto jc) {
+          constexpr int j = G - 1 - decltype(jc)::value;
+          const double xj = gbcast<G, j>(pivoting ? b / row[j] : b * inv_diag, gbase);
+          upd = (r == j && live) ? xj : upd;
+          b -= row[j] * xj;
+        });
+        if (!pivoting) {
+          redo = gmax<G>(lbad ? 1.0 : 0.0, lane) > 0.0;
+          if (LUP == PFX_LU_ONDEMAND) lu_pivot = lu_pivot || redo;
+          if (kCanDefer && redo) {
+            // a warm attempt is given up as one that fails (the plain
+            // attempt follows in this launch); otherwise the cell is left as
+            // it came in, for the pivoting launch to redo
+            finish = true;
+            if (P.guess_mode == PFX_GUESS_ARRAY_FALLBACK && !fell_back) status = PFX_CELL_SINGULAR;
+            else deferred = true;
+          }
         }
       }
       const double bad = gmax<G>(status == PFX_CELL_SINGULAR ? 1.0 : 0.0, lane);
@@ -1216,7 +1291,22 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     PFX_PT(7);
     // Newton update (reaction.F90:3454-3470; constraint form :1806-1826)
     double mnew = m, rel = 0.0;
-    if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
+    if (MODE == PFX_MODE_REACT && active && !finish && !redo && floor_pass) {
+      // the solve gave the resolution of the free ions at the candidate: a
+      // waiting step within twice it is rounding noise, and the iterate is
+      // as good as FP64 makes it; otherwise the step is taken now
+      floor_pass = false;
+      const double frel = gmax<G>(live ? fabs(use_log ? upd : upd / m) : 0.0, lane);
+      if (rel_cand <= 2.0 * frel) {
+        finish = true;
+      } else if (it >= it_cap) {
+        finish = true;
+        status = PFX_CELL_MAX_ITERATIONS;
+      } else {
+        mnew = m_cand;
+        rel = rel_cand;
+      }
+    } else if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish && !redo) {
       const double mprev = m;
       if (use_log) {
         const double u = copysign(1.0, upd) * fmin(fabs(upd), L.max_dlnC);
@@ -1233,20 +1323,24 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
         // 1/(1 - rho), rho = this step over the last one (at most 10x),
         // bounds the distance to the root also where the iteration
         // contracts only linearly; the iterate itself is returned (its
-        // residual passed), the step is not taken.  A step within twice
-        // the FP64 resolution of the free ions (the floor solve) is
-        // rounding noise: the iterate is as good as FP64 makes it.
+        // residual passed), the step is not taken.  Where the steps
+        // stagnate (rho >= 1/2) a step within twice the FP64 resolution of
+        // the free ions is rounding noise: the next pass, at the same
+        // iterate, solves for that resolution and decides.
         const double rho = fmin(last_change > 0.0 && last_change < INFINITY ? rel / last_change : 0.0, 0.9);
-        const double frel = gmax<G>(live ? fabs(use_log ? fl : fl / m) : 0.0, lane);
-        if (rel / (1.0 - rho) <= P.rel_change_tol || rel <= 2.0 * frel) {
+        if (rel / (1.0 - rho) <= P.rel_change_tol) {
           finish = true;
+        } else if (rho >= 0.5) {
+          floor_pass = true;
+          rel_cand = rel;
+          m_cand = mnew;
         } else if (it >= it_cap) {
           finish = true;
           status = PFX_CELL_MAX_ITERATIONS;
         }
       }
     }
-    if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish) {
+    if ((MODE == PFX_MODE_REACT || MODE == PFX_MODE_EQUIL) && active && !finish && !redo && !floor_pass) {
       m = mnew;
       it++;
       if (MODE == PFX_MODE_REACT) {
@@ -1289,10 +1383,23 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
     }
 
     // ---------------------------------------------------------------
+    // a cell the unpivoted kernel did not converge is redone from its
+    // inputs by the pivoting launch (the reference's factorisation), so a
+    // failure is only reported where partial pivoting fails too
+    if (kCanDefer && active && finish && status != PFX_CELL_CONVERGED) deferred = true;
+    if (kCanDefer && active && finish && deferred && r == 0) {
+      const unsigned int slot = atomicAdd(P.defer, 1u);
+      P.defer[1 + slot] = (unsigned int)cell | (fell_back ? 0x80000000u : 0u);
+      if (fell_back) P.num_iterations[cell] = it_cap - P.max_iterations;  // the warm attempt's iterations
+    }
+
+    // ---------------------------------------------------------------
     // retire the cell: write the state at this iterate
     if (active && finish) {
       const long long c = cell;
-      if (MODE == PFX_MODE_EVAL) {
+      if (deferred) {
+        // nothing written: the inputs stay for the pivoting launch
+      } else if (MODE == PFX_MODE_EVAL) {
         if (live) {
           P.res_out[c * N + r] = Rr;
 #pragma unroll
@@ -1349,11 +1456,11 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 // as many registers as the code wants; 2, 3 spill what does not fit).  The
 // fastest choice depends on the group size and mode and is picked by
 // measurement (pfx_tables_tune).
-template <int G, int MODE, int W>
+template <int G, int MODE, int W, int LUP>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W))) void pfx_react_kernel(
     const pfx_kernel_params Pk) {
   (void)Pk;
-  react_body<G, MODE>((uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr());
+  react_body<G, MODE, LUP>((uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr());
 }
 
 // Several chemistry zones in one persistent launch: workgroups
@@ -1366,9 +1473,9 @@ __device__ __forceinline__ void zones_body(const pfx_kernel_params *zone_params,
   while (z + 1 < Z.nzones && (int)blockIdx.x >= Z.block_start[z + 1]) z++;
   const uint64_t praw = (uint64_t)(uintptr_t)(zone_params + z);
   switch (Z.G[z]) {
-    case 8: react_body<8, PFX_MODE_REACT>(praw); break;
-    case 16: react_body<16, PFX_MODE_REACT>(praw); break;
-    default: react_body<32, PFX_MODE_REACT>(praw); break;
+    case 8: react_body<8, PFX_MODE_REACT, PFX_LU_DEFER>(praw); break;
+    case 16: react_body<16, PFX_MODE_REACT, PFX_LU_DEFER>(praw); break;
+    default: react_body<32, PFX_MODE_REACT, PFX_LU_DEFER>(praw); break;
   }
   __syncthreads();
   if (threadIdx.x == 0 && finish) finish[blockIdx.x] = wall_clock64();
@@ -1385,15 +1492,21 @@ template __global__ void pfx_react_zones_kernel<1>(const pfx_kernel_params *, co
 template __global__ void pfx_react_zones_kernel<2>(const pfx_kernel_params *, const pfx_zone_launch, uint64_t *);
 template __global__ void pfx_react_zones_kernel<3>(const pfx_kernel_params *, const pfx_zone_launch, uint64_t *);
 
-#define PFX_INST(G)                                                                    \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_REACT, 1>(const pfx_kernel_params); \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_REACT, 2>(const pfx_kernel_params); \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_REACT, 3>(const pfx_kernel_params); \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_EVAL, 1>(const pfx_kernel_params);  \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_EQUIL, 1>(const pfx_kernel_params); \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_EQUIL, 2>(const pfx_kernel_params); \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_EQUIL, 3>(const pfx_kernel_params); \
-  template __global__ void pfx_react_kernel<G, PFX_MODE_AUX, 1>(const pfx_kernel_params);
+#define PFX_INST_LU(G, MODE, W, LUP) \
+  template __global__ void pfx_react_kernel<G, MODE, W, LUP>(const pfx_kernel_params);
+#define PFX_INST(G)                                     \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_DEFER)       \
+  PFX_INST_LU(G, PFX_MODE_REACT, 2, PFX_LU_DEFER)       \
+  PFX_INST_LU(G, PFX_MODE_REACT, 3, PFX_LU_DEFER)       \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_ONDEMAND)    \
+  PFX_INST_LU(G, PFX_MODE_REACT, 2, PFX_LU_ONDEMAND)    \
+  PFX_INST_LU(G, PFX_MODE_REACT, 3, PFX_LU_ONDEMAND)    \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_PIVOT)       \
+  PFX_INST_LU(G, PFX_MODE_EVAL, 1, PFX_LU_PIVOT)        \
+  PFX_INST_LU(G, PFX_MODE_EQUIL, 1, PFX_LU_PIVOT)       \
+  PFX_INST_LU(G, PFX_MODE_EQUIL, 2, PFX_LU_PIVOT)       \
+  PFX_INST_LU(G, PFX_MODE_EQUIL, 3, PFX_LU_PIVOT)       \
+  PFX_INST_LU(G, PFX_MODE_AUX, 1, PFX_LU_PIVOT)
 PFX_INST(8)
 PFX_INST(16)
 PFX_INST(32)
