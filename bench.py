@@ -157,6 +157,7 @@ def main():
 
     fl = t.flops_per_iteration()
     warm = None
+    mlp = None
     if not args.no_surrogate:
         # ---- untimed: training pairs from the GPU solver, then training
         mlp = surrogate.MLP(surrogate.DIMS, batch_max=args.batch, seed=synth.SEED)
@@ -309,7 +310,7 @@ def main():
                      "achieved_GBs": bytes_per_cell * n / (kern_ms * 1e-3) / 1e9},
     })
     if not args.no_gi:
-        line["gi"] = gi_leg(t, tab, args, log)
+        line["gi"] = gi_leg(t, tab, args, log, mlp=mlp)
     if not args.no_zones:
         line["zones"] = zones_leg(args, log)
     if not args.no_column:
@@ -335,7 +336,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def gi_leg(t, tab, args, log):
+def gi_leg(t, tab, args, log, mlp=None):
     """Global-implicit assembly of a side^3 grid (7-point, upwind): the
     first residual/Jacobian evaluation of a time step (x = old state), the
     BSR Jacobian with column-major 15x15 blocks.  HBM-bound."""
@@ -413,33 +414,74 @@ def gi_leg(t, tab, args, log):
     # most 16 times) when Newton does not converge; with the reference's
     # reaction Jacobian, with the capped-Q/K derivative (first 12 Newton
     # iterations), and with NUMERICAL_JACOBIAN
+    # The same step from a predicted iterate (runtime.gi_initial_iterate):
+    # every cell's own reaction step first (pfx_react_batch, from the old
+    # free ions or from the learned warm start), then the global Newton from
+    # its free ions; same residual, Jacobian and tests, so the same solution
+    # within SNES's tolerances.  Timed: predictor + global step.
     snap = workload.snapshot(dc)
-    for key, numerical, jflags in (("newton", 0, 0), ("newton_capped_qk", 0, abi.PFX_JAC_CAPPED_QK),
-                                   ("newton_numerical_jacobian", 1, 0)):
+    ref_m = {}
+    cases = (("newton", 0, 0, None), ("newton_capped_qk", 0, abi.PFX_JAC_CAPPED_QK, None),
+             ("newton_numerical_jacobian", 1, 0, None), ("newton_predicted_previous", 0, 0, "previous"),
+             ("newton_predicted_warm", 0, 0, "warm"),
+             ("newton_capped_qk_predicted_warm", 0, abi.PFX_JAC_CAPPED_QK, "warm"))
+    for key, numerical, jflags, pred in cases:
+        if pred == "warm" and mlp is None:
+            continue
         workload.restore(dc, snap)
         t.numerical_jacobian = numerical
         tab_n = runtime.Tables(t)
         t.numerical_jacobian = 0
         gs = runtime.GiSolver(tab_n, g)
         dt_want = synth.STEP_DT
-        st = gs.timestep(dc, g, synth.react_opts(dt=dt_want, jacobian_flags=jflags))
+        o = synth.react_opts(dt=dt_want, jacobian_flags=jflags)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        pst = None
+        if pred is not None:
+            warm = None
+            if pred == "warm":
+                from pflotran_amd import surrogate
+
+                def warm(tb, cells):
+                    surrogate.guess(tb, mlp, cells)
+            pst = runtime.gi_initial_iterate(tab_n, dc, o, warm=warm)
+            torch.cuda.synchronize()
+        pred_ms = (time.perf_counter() - t1) * 1e3
+        st = gs.timestep(dc, g, o)
+        torch.cuda.synchronize()
+        total_ms = (time.perf_counter() - t1) * 1e3
         gs.close()
         ns = st.newton
         out[key] = {"reason": st.reason, "converged": st.reason > 0, "dt_wanted": dt_want, "dt_taken": st.dt,
                     "cuts": st.cuts, "newton_iterations": st.newton_iterations,
                     "wasted_newton_iterations": st.wasted_newton_iterations,
-                    "linear_iterations": st.linear_iterations, "ms_total": st.ms_total,
+                    "linear_iterations": st.linear_iterations, "ms_total": total_ms,
                     "last_solve": {"iterations": ns.iterations, "fnorm0": ns.fnorm0, "fnorm": ns.fnorm,
                                    "ms_assembly": ns.ms_assembly, "ms_linear": ns.ms_linear},
                     "ms_per_newton_iteration": st.ms_total / max(st.newton_iterations, 1),
                     "tolerances": "SNES rtol 1e-8 / stol 1e-8 / 50 its, KSP rtol 1e-5 (PETSc defaults); "
                                   "cut factor 0.5, at most 16 cuts (timestepper defaults)"}
+        if pst is not None:
+            out[key]["predictor"] = {"start": pred, "ms": pred_ms, "converged_cells": int(pst.converged),
+                                     "mean_iterations": pst.total_iterations / n}
         if st.reason <= 0:
             out[key]["note"] = "NOT CONVERGED after the cut budget: the time is not a step time"
-        log(f"gi: {key} {st.ms_total:.1f} ms, reason {st.reason}, dt {st.dt:.0f} s after {st.cuts} cuts, "
-            f"{st.newton_iterations} Newton its ({st.wasted_newton_iterations} wasted), "
+        m_end = dc.t["pri_molal"].clone()
+        jkey = "capped" if jflags else "reference"
+        if pred is None and not numerical and st.dt == dt_want and st.reason > 0:
+            ref_m[jkey] = m_end
+        if pred is not None and jkey in ref_m:
+            rel = ((m_end - ref_m[jkey]).abs() / ref_m[jkey].abs()).amax(dim=1)
+            out[key]["free_ions_vs_unpredicted_step"] = {
+                "max_rel": float(rel.max().item()),
+                "p99_rel": float(torch.quantile(rel[:1_000_000].float(), 0.99).item())}
+        del m_end
+        log(f"gi: {key} {total_ms:.1f} ms (predictor {pred_ms:.1f}), reason {st.reason}, dt {st.dt:.0f} s after "
+            f"{st.cuts} cuts, {st.newton_iterations} Newton its ({st.wasted_newton_iterations} wasted), "
             f"{st.linear_iterations} BiCGStab, last |F| {ns.fnorm0:.2e} -> {ns.fnorm:.2e}")
         del tab_n
+    del ref_m
 
     # the transport half of an operator-split step on the same grid: the
     # scalar transport matrix once, one BiCGStab solve per component (Jacobi,
@@ -641,7 +683,10 @@ def zones_leg(args, log):
         torch.cuda.synchronize()
         sep.append((time.perf_counter() - t1) * 1e3)
     # zoned launch, autotuned: per kernel variant (register cap) the split
-    # starts from cells * N^2 and is rebalanced from the measured zone times
+    # starts from cells * N^2 and is rebalanced from the measured zone times;
+    # the trials start from the step's inputs, not from the reacted state the
+    # separate launches left
+    restore()
     plan, history = autotune.tune_zones(tabs, dcs, opts)
     best = {"variant": plan.variant, "blocks": list(plan.blocks[:4])}
     # the tuned plan eagerly and as a graph replay, alternated, so the two

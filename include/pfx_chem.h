@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PFX_ABI_VERSION 5
+#define PFX_ABI_VERSION 6
 
 /* largest primary-species count handled by one cell group (row per lane) */
 #define PFX_MAX_PRIMARY 32
@@ -42,6 +42,9 @@ extern "C" {
 #define PFX_MAX_MINERALS 16
 #define PFX_MAX_IONX_RXN 4
 #define PFX_MAX_IONX_CATION 8
+/* rate prefactors per kinetic mineral, and species per prefactor */
+#define PFX_MAX_PREFACTORS 4
+#define PFX_MAX_PREFACTOR_SPECIES 4
 #define PFX_MAX_ZONES 8
 
 /* error codes */
@@ -196,6 +199,30 @@ typedef struct pfx_reaction_desc {
   const int32_t *eqkdmineral;         /* [D] kinetic mineral whose volume
                                          fraction scales K_D, -1 none; NULL = none */
   double soil_particle_density;       /* kg/m^3 (ROCK_DENSITY), with eqkdmineral */
+
+  /* Rate prefactors of the kinetic minerals (mineral%kinmnrl_num_prefactors,
+     kinmnrl_prefactor_id, kinmnrl_pref_*; RKineticMineral
+     reaction_mineral.F90:743-786 and its derivative :887-944).  A mineral
+     with prefactors takes as its rate constant
+       sum_p k_p A_p(T) prod_s a_s^alpha_ps / (1 + atten_ps a_s^beta_ps),
+     a_s the activity of primary species s and A_p the Arrhenius factor of
+     the prefactor's activation energy (none where <= 0); a mineral without
+     uses kinmnrl_rate_constant.  Prefactor species must be primaries (the
+     reference's negative ids for complexes are not supported). */
+  int32_t max_prefactors;             /* P: leading dim; 0 = no prefactors    */
+  int32_t max_prefactor_species;      /* Q: leading dim                        */
+  const int32_t *kinmnrl_num_prefactors;        /* [K]                         */
+  const int32_t *kinmnrl_prefactor_id;          /* [K][P][Q+1]; [k][p][0] = count,
+                                                   then 0-based primary ids   */
+  const double *kinmnrl_pref_alpha;             /* [K][P][Q]                   */
+  const double *kinmnrl_pref_beta;              /* [K][P][Q]                   */
+  const double *kinmnrl_pref_atten_coef;        /* [K][P][Q]                   */
+  const double *kinmnrl_pref_rate;              /* [K][P] mol/m^2/s            */
+  const double *kinmnrl_pref_activation_energy; /* [K][P] J/mol, or NULL       */
+  /* kinmnrl_min_scale_factor f (reaction_mineral.F90:704-716, 800-802,
+     848-858): affinity 1 - (Q/K)^(1/(f sigma)) and the rate divided by f;
+     NULL = none */
+  const double *kinmnrl_min_scale_factor;       /* [K] or NULL                 */
 } pfx_reaction_desc;
 
 /* Per-cell state, device pointers, cell-major.  In = read, Out = written. */
@@ -375,7 +402,10 @@ int pfx_residual_jacobian(pfx_tables *tables, pfx_cells *cells,
    constraint_conc[ncells*N] (device) holds each cell's values;
    constraint_rxn[N] (host; NULL when no component has a mineral or gas
    constraint) the reactions of the mineral and gas rows.  Starts from
-   1e-9 molal with unit activity coefficients, switches activity on after the
+   1e-9 molal, or with opts->guess_mode == PFX_GUESS_ARRAY from
+   cells->guess (a FREE_ION_GUESS, reaction.F90:1476-1482; rows fixed by
+   their own value still start from it), with unit activity coefficients,
+   switches activity on after the
    first convergence (LAG, every iteration), and stops on the reference
    criteria max|R| < max_residual_tolerance and max|dm/m| <
    max_relative_change_tolerance.  Out: pri_molal, pri_act_coef,

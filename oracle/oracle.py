@@ -482,7 +482,11 @@ def gi_column_run(deck, db=None, max_cuts=16):
         if t.K and setup.get("vf") is not None:
             c["mnrl_volfrac"][0, :] = setup["vf"]
             c["mnrl_area"][0, :] = setup["area"]
-        rc, its = equilibrate(t, setup["ctype"], np.asarray([setup["conc"]], dtype=np.float64), c, opts(1.0),
+        o = opts(1.0)
+        if setup.get("guess") is not None:
+            c["guess"][0, :] = setup["guess"]
+            o.guess_mode = abi.PFX_GUESS_ARRAY
+        rc, its = equilibrate(t, setup["ctype"], np.asarray([setup["conc"]], dtype=np.float64), c, o,
                               molal=bool(setup.get("molal", 0)), rxn=setup.get("rxn"))
         if rc != 0:
             raise RuntimeError(f"constraint {name} did not speciate")

@@ -474,10 +474,43 @@ void or_ionx(const pfx_reaction_desc *d, ocell *c) {
   }
 }
 
-/* reaction_mineral.F90:564-1000 RKineticMineral (no prefactor species).
+/* The rate constant of kinetic mineral k with rate prefactors
+   (reaction_mineral.F90:743-786): sum over the prefactors of
+   exp(sum_s alpha ln a_s - ln(1 + exp(ln atten + beta ln a_s))) * k_p *
+   Arrhenius_p; prefactor_p and the per-species ln terms are returned for
+   the derivative (:887-944).  Primary prefactor species only. */
+static double or_prefactor_sum(const pfx_reaction_desc *d, int k, const double *ln_act, double temp,
+                               double prefactor[PFX_MAX_PREFACTORS],
+                               double ln_spec[PFX_MAX_PREFACTORS][PFX_MAX_PREFACTOR_SPECIES]) {
+  const int P = d->max_prefactors, Q = d->max_prefactor_species;
+  double sum = 0.0;
+  for (int p = 0; p < d->kinmnrl_num_prefactors[k]; p++) {
+    const size_t kp = (size_t)k * P + p;
+    double ln_pref = 0.0;
+    for (int q = 0; q < d->kinmnrl_prefactor_id[kp * (Q + 1)]; q++) {
+      const int ic = d->kinmnrl_prefactor_id[kp * (Q + 1) + 1 + q];
+      const double la = ln_act[ic];
+      const double num = d->kinmnrl_pref_alpha[kp * Q + q] * la;
+      const double den = log(1.0 + exp(log(d->kinmnrl_pref_atten_coef[kp * Q + q]) +
+                                       d->kinmnrl_pref_beta[kp * Q + q] * la));
+      ln_pref = ln_pref + num;
+      ln_pref = ln_pref - den;
+      ln_spec[p][q] = num - den;
+    }
+    prefactor[p] = exp(ln_pref);
+    double arr = 1.0;
+    const double Ea = d->kinmnrl_pref_activation_energy ? d->kinmnrl_pref_activation_energy[kp] : 0.0;
+    if (Ea > 0.0) arr = exp(Ea / OR_IDEAL_GAS * (1.0 / (25.0 + 273.15) - 1.0 / (temp + 273.15)));
+    sum = sum + prefactor[p] * d->kinmnrl_pref_rate[kp] * arr;
+  }
+  return sum;
+}
+
+/* reaction_mineral.F90:564-1000 RKineticMineral (primary prefactor species).
    Res (mol/s) and Jac (kg water/s) accumulate; rates stored per m^3 bulk.
    deriv: 0 residual only, 1 the reference's derivative, 1 | 2 without the
-   derivative where ln(Q/K) is above the cap (PFX_JAC_CAPPED_QK). */
+   Q/K derivative where ln(Q/K) is above the cap (PFX_JAC_CAPPED_QK; the
+   prefactor derivative stays). */
 void or_kinetic_mineral(const pfx_reaction_desc *d, ocell *c, double *Res, double Jac[NMAX][NMAX], int deriv) {
   const int N = d->naqcomp;
   double ln_conc[NMAX], ln_act[NMAX];
@@ -492,19 +525,34 @@ void or_kinetic_mineral(const pfx_reaction_desc *d, ocell *c, double *Res, doubl
     const int ns = MNRL_N(d, k);
     for (int s = 0; s < ns; s++) lnQK += MNRL_NU(d, k, s) * ln_act[MNRL_ID(d, k, s)];
     double QK = lnQK <= 6.90776 ? exp(lnQK) : 1.0e3;
+    const double *msf = d->kinmnrl_min_scale_factor;
     double temkin = d->kinmnrl_Temkin_const ? d->kinmnrl_Temkin_const[k] : 0.0;
-    double aff = d->kinmnrl_Temkin_const ? 1.0 - pow(QK, 1.0 / temkin) : 1.0 - QK;
+    double aff;
+    if (d->kinmnrl_Temkin_const)
+      aff = msf ? 1.0 - pow(QK, 1.0 / (msf[k] * temkin)) : 1.0 - pow(QK, 1.0 / temkin);
+    else if (msf)
+      aff = 1.0 - pow(QK, 1.0 / msf[k]);
+    else
+      aff = 1.0 - QK;
     double sgn = copysign(1.0, aff);
     if (!(c->vf[k] > 0.0 || sgn < 0.0)) continue;
     double thr = d->kinmnrl_affinity_threshold ? d->kinmnrl_affinity_threshold[k] : 0.0;
     if (thr > 0.0 && sgn < 0.0 && QK < thr) continue;
     double lim = d->kinmnrl_rate_limiter ? d->kinmnrl_rate_limiter[k] : 0.0;
     if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
-    double arr = 1.0;
-    double Ea = d->kinmnrl_activation_energy ? d->kinmnrl_activation_energy[k] : 0.0;
-    if (Ea > 0.0) arr = exp(Ea / OR_IDEAL_GAS * (1.0 / (25.0 + 273.15) - 1.0 / (c->temp + 273.15)));
-    double sum_pref = d->kinmnrl_rate_constant[k] * arr;
+    const int npref = d->kinmnrl_num_prefactors && d->max_prefactors > 0 ? d->kinmnrl_num_prefactors[k] : 0;
+    double prefactor[PFX_MAX_PREFACTORS], ln_spec[PFX_MAX_PREFACTORS][PFX_MAX_PREFACTOR_SPECIES];
+    double sum_pref;
+    if (npref > 0) {
+      sum_pref = or_prefactor_sum(d, k, ln_act, c->temp, prefactor, ln_spec);
+    } else {
+      double arr = 1.0;
+      double Ea = d->kinmnrl_activation_energy ? d->kinmnrl_activation_energy[k] : 0.0;
+      if (Ea > 0.0) arr = exp(Ea / OR_IDEAL_GAS * (1.0 / (25.0 + 273.15) - 1.0 / (c->temp + 273.15)));
+      sum_pref = d->kinmnrl_rate_constant[k] * arr;
+    }
     double Im_const = -c->area[k];
+    if (msf) Im_const = Im_const / msf[k];
     double Im;
     if (d->kinmnrl_affinity_power)
       Im = Im_const * sgn * pow(fabs(aff), d->kinmnrl_affinity_power[k]) * sum_pref;
@@ -515,14 +563,20 @@ void or_kinetic_mineral(const pfx_reaction_desc *d, ocell *c, double *Res, doubl
     Im *= c->volume;
     for (int s = 0; s < ns; s++) Res[MNRL_ID(d, k, s)] += MNRL_NU(d, k, s) * Im;
     if (!deriv) continue;
-    if ((deriv & 2) && lnQK > 6.90776) continue;
+    const int skip_qk = (deriv & 2) && lnQK > 6.90776;
     double dIm_dQK;
     if (d->kinmnrl_affinity_power)
       dIm_dQK = -Im * d->kinmnrl_affinity_power[k] / fabs(aff);
     else
       dIm_dQK = -Im_const * sum_pref;
-    if (d->kinmnrl_Temkin_const) dIm_dQK = dIm_dQK * (1.0 / temkin) / QK * (1.0 - aff);
-    if (lim <= 0.0) {
+    if (d->kinmnrl_Temkin_const)
+      dIm_dQK = msf ? dIm_dQK * (1.0 / (msf[k] * temkin)) / QK * (1.0 - aff)
+                    : dIm_dQK * (1.0 / temkin) / QK * (1.0 - aff);
+    else if (msf)
+      dIm_dQK = dIm_dQK * (1.0 / msf[k]) / QK * (1.0 - aff);
+    if (skip_qk) {
+      /* the rate is constant in Q/K above the cap: no Q/K derivative */
+    } else if (lim <= 0.0) {
       for (int t = 0; t < ns; t++) {
         int jc = MNRL_ID(d, k, t);
         double dQK_dCj = MNRL_NU(d, k, t) * QK * exp(-ln_conc[jc]);
@@ -537,6 +591,31 @@ void or_kinetic_mineral(const pfx_reaction_desc *d, ocell *c, double *Res, doubl
         double dQK_dmj = dQK_dCj * c->den * 1.0e-3;
         for (int s = 0; s < ns; s++)
           Jac[MNRL_ID(d, k, s)][jc] += MNRL_NU(d, k, s) * dIm_dQK * (1.0 + QK / lim / den) * dQK_dmj / den;
+      }
+    }
+    if (npref > 0) {
+      /* reaction_mineral.F90:887-944: through the prefactor species */
+      const int P = d->max_prefactors, Q = d->max_prefactor_species;
+      const double dIm_dsum = Im / sum_pref;
+      for (int p = 0; p < npref; p++) {
+        const size_t kp = (size_t)k * P + p;
+        double arr = 1.0;
+        const double Ea = d->kinmnrl_pref_activation_energy ? d->kinmnrl_pref_activation_energy[kp] : 0.0;
+        if (Ea > 0.0) arr = exp(Ea / OR_IDEAL_GAS * (1.0 / (25.0 + 273.15) - 1.0 / (c->temp + 273.15)));
+        const double ln_pref = log(prefactor[p]);
+        for (int q = 0; q < d->kinmnrl_prefactor_id[kp * (Q + 1)]; q++) {
+          const double dpref = exp(ln_pref - ln_spec[p][q]);
+          const int ic = d->kinmnrl_prefactor_id[kp * (Q + 1) + 1 + q];
+          const double la = ln_act[ic];
+          const double dnum = d->kinmnrl_pref_alpha[kp * Q + q] * exp(ln_spec[p][q] - la);
+          const double atten = d->kinmnrl_pref_atten_coef[kp * Q + q], beta = d->kinmnrl_pref_beta[kp * Q + q];
+          const double lgb = beta * la;
+          const double denom = 1.0 + exp(log(atten) + lgb);
+          const double dden = -1.0 * exp(ln_spec[p][q]) / denom * atten * beta * exp(lgb - la);
+          const double dspec = (dnum + dden) * c->gam[ic];
+          const double dIm = dIm_dsum * dpref * dspec * d->kinmnrl_pref_rate[kp] * arr;
+          for (int s = 0; s < ns; s++) Jac[MNRL_ID(d, k, s)][ic] += MNRL_NU(d, k, s) * dIm;
+        }
       }
     }
   }
@@ -1015,13 +1094,15 @@ int or_residual_jacobian(const pfx_reaction_desc *d, pfx_cells *cl, const pfx_re
 
 static int or_equilibrate_cell(const pfx_reaction_desc *d, const int32_t *ctype, const double *conc_in,
                                const pfx_constraint_rxn *rxn, int has_temp, ocell *c, int use_prev, int molal,
-                               int *iters) {
+                               int *iters, const double *guess) {
   const int N = d->naqcomp;
   double conc[NMAX], total_conc[NMAX], free_conc[NMAX];
   double molal_to_molar = molal ? c->den / 1000.0 : 1.0;
   double molar_to_molal = molal ? 1.0 : 1000.0 / c->den;
   for (int j = 0; j < N; j++) conc[j] = conc_in[j];
-  for (int j = 0; j < N; j++) free_conc[j] = use_prev ? c->m[j] : 1.0e-9;
+  /* the starting free ions: the previous solution, a FREE_ION_GUESS
+     (guess), or 1e-9 (reaction.F90:1476-1482) */
+  for (int j = 0; j < N; j++) free_conc[j] = use_prev ? c->m[j] : guess ? guess[j] : 1.0e-9;
   for (int j = 0; j < N; j++) {
     total_conc[j] = 0.0;
     switch (ctype[j]) {
@@ -1172,7 +1253,9 @@ int or_equilibrate_batch(const pfx_reaction_desc *d, const int32_t *ctype, const
     for (int i = 0; i < M; i++) c.gsec[i] = 1.0;
     for (int r = 0; r < R; r++) c.ionx_ref[r] = 1e-9; /* reactive_transport_aux.F90:304 */
     int it = 0;
-    int st = or_equilibrate_cell(d, ctype + 0, conc + ci * N, rxn, cl->temperature != NULL, &c, 0, molal, &it);
+    const double *guess = (o->guess_mode == PFX_GUESS_ARRAY && cl->guess) ? cl->guess + ci * N : NULL;
+    int st = or_equilibrate_cell(d, ctype + 0, conc + ci * N, rxn, cl->temperature != NULL, &c, 0, molal, &it,
+                                 guess);
     if (st) bad++;
     /* initial-condition refresh of the cell auxvar, which (isothermal deck,
        no datasets) is not the constraint auxvar: it starts from

@@ -5,13 +5,15 @@ the field offsets against the header compiled with the host compiler.
 """
 import ctypes as C
 
-PFX_ABI_VERSION = 5
+PFX_ABI_VERSION = 6
 PFX_MAX_PRIMARY = 32
 PFX_MAX_CPLX = 256
 PFX_MAX_RXN_SPEC = 8
 PFX_MAX_MINERALS = 16
 PFX_MAX_IONX_RXN = 4
 PFX_MAX_IONX_CATION = 8
+PFX_MAX_PREFACTORS = 4
+PFX_MAX_PREFACTOR_SPECIES = 4
 PFX_MAX_ZONES = 8
 
 PFX_E_ARG, PFX_E_DEVICE, PFX_E_NOMEM, PFX_E_UNSUP = -1, -2, -3, -4
@@ -55,6 +57,10 @@ class ReactionDesc(C.Structure):
         ("neqkdrxn", C.c_int32), ("reserved_kd", C.c_int32), ("eqkdspecid", _ip), ("eqkdtype", _ip),
         ("eqkddistcoef", _dp), ("eqkdlangmuirb", _dp), ("eqkdfreundlichn", _dp), ("eqkdmineral", _ip),
         ("soil_particle_density", C.c_double),
+        ("max_prefactors", C.c_int32), ("max_prefactor_species", C.c_int32),
+        ("kinmnrl_num_prefactors", _ip), ("kinmnrl_prefactor_id", _ip), ("kinmnrl_pref_alpha", _dp),
+        ("kinmnrl_pref_beta", _dp), ("kinmnrl_pref_atten_coef", _dp), ("kinmnrl_pref_rate", _dp),
+        ("kinmnrl_pref_activation_energy", _dp), ("kinmnrl_min_scale_factor", _dp),
     ]
 
 

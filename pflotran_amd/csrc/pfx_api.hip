@@ -787,9 +787,49 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
     if (d->kinmnrl_affinity_power) mpow[k] = d->kinmnrl_affinity_power[k];
     if (d->kinmnrl_Temkin_const) mtk[k] = d->kinmnrl_Temkin_const[k];
   }
+  // rate prefactors (reaction_mineral.F90:743-786) and min scale factor
+  const int PP = d->kinmnrl_num_prefactors ? d->max_prefactors : 0;
+  const int PQ = PP > 0 ? d->max_prefactor_species : 0;
+  if (PP < 0 || PP > PFX_MAX_PREFACTORS || PQ < 0 || PQ > PFX_MAX_PREFACTOR_SPECIES) return PFX_E_UNSUP;
+  if (PP > 0 && (PQ < 1 || !d->kinmnrl_prefactor_id || !d->kinmnrl_pref_alpha || !d->kinmnrl_pref_beta ||
+                 !d->kinmnrl_pref_atten_coef || !d->kinmnrl_pref_rate))
+    return PFX_E_ARG;
+  const int PPa = std::max(PP, 1), PQa = std::max(PQ, 1);
+  std::vector<int32_t> mpn(std::max(K, 1), 0), mpid((size_t)std::max(K, 1) * PPa * PQa, -1);
+  std::vector<double> mpalpha((size_t)std::max(K, 1) * PPa * PQa, 0.0), mpbeta(mpalpha), mpatten(mpalpha);
+  std::vector<double> mprate((size_t)std::max(K, 1) * PPa, 0.0), mpEa(mprate), mmsf(std::max(K, 1), 1.0);
+  bool any_pref = false;
+  for (int k = 0; k < K && PP > 0; k++) {
+    const int np = d->kinmnrl_num_prefactors[k];
+    if (np < 0 || np > PP) return PFX_E_ARG;
+    mpn[k] = np;
+    any_pref = any_pref || np > 0;
+    for (int p = 0; p < np; p++) {
+      const size_t kp = (size_t)k * PP + p;
+      mprate[k * PPa + p] = d->kinmnrl_pref_rate[kp];
+      mpEa[k * PPa + p] = d->kinmnrl_pref_activation_energy ? d->kinmnrl_pref_activation_energy[kp] : 0.0;
+      const int ns = d->kinmnrl_prefactor_id[kp * (PQ + 1)];
+      if (ns < 0 || ns > PQ) return PFX_E_ARG;
+      for (int q = 0; q < ns; q++) {
+        const int sp = d->kinmnrl_prefactor_id[kp * (PQ + 1) + 1 + q];
+        if (sp < 0) return PFX_E_UNSUP;  // complexes as prefactor species
+        if (sp >= N) return PFX_E_ARG;
+        const size_t o = ((size_t)k * PPa + p) * PQa + q;
+        mpid[o] = sp;
+        mpalpha[o] = d->kinmnrl_pref_alpha[kp * PQ + q];
+        mpbeta[o] = d->kinmnrl_pref_beta[kp * PQ + q];
+        mpatten[o] = d->kinmnrl_pref_atten_coef[kp * PQ + q];
+      }
+    }
+  }
+  if (d->kinmnrl_min_scale_factor)
+    for (int k = 0; k < K; k++) mmsf[k] = d->kinmnrl_min_scale_factor[k];
+  L.npref = any_pref ? PPa : 0;
+  L.npref_spec = any_pref ? PQa : 0;
   L.has_mnrl_coef = d->kinmnrl_logKcoef != nullptr;
   L.mnrl_flags = (d->kinmnrl_affinity_threshold ? 1 : 0) | (d->kinmnrl_rate_limiter ? 2 : 0) |
-                 (d->kinmnrl_affinity_power ? 4 : 0) | (d->kinmnrl_Temkin_const ? 8 : 0);
+                 (d->kinmnrl_affinity_power ? 4 : 0) | (d->kinmnrl_Temkin_const ? 8 : 0) |
+                 (any_pref ? 16 : 0) | (d->kinmnrl_min_scale_factor ? 32 : 0);
   L.o_mspec = B.add(mspec);
   L.o_mnu = B.add(mnu);
   L.o_mnuG = B.add(mnuG);
@@ -800,6 +840,14 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   L.o_mlim = B.add(mlim);
   L.o_mpow = B.add(mpow);
   L.o_mtemkin = B.add(mtk);
+  L.o_mmsf = B.add(mmsf);
+  L.o_mpn = B.add(mpn);
+  L.o_mpid = B.add(mpid);
+  L.o_mpalpha = B.add(mpalpha);
+  L.o_mpbeta = B.add(mpbeta);
+  L.o_mpatten = B.add(mpatten);
+  L.o_mprate = B.add(mprate);
+  L.o_mpEa = B.add(mpEa);
   // ion exchange
   std::vector<double> ionx_k((size_t)std::max(R, 1) * G, 0.0);
   std::vector<int32_t> ionx_role((size_t)std::max(R, 1) * G, -1);

@@ -32,7 +32,9 @@ struct pfx_dev_layout {
   double debyeA, debyeB, debyeBdot;
   int32_t use_log, act_freq, act_alg, num_jac;
   double max_dlnC, max_rel_change_tol, max_residual_tol;
-  int32_t mnrl_flags;           // bit0 threshold, bit1 limiter, bit2 power, bit3 Temkin
+  int32_t mnrl_flags;           // bit0 threshold, bit1 limiter, bit2 power, bit3 Temkin,
+                                // bit4 rate prefactors, bit5 min scale factor
+  int32_t npref, npref_spec;    // P, Q of the prefactor tables (0 without prefactors)
   int32_t kd_any;               // some primary carries isotherm (K_D) sorption
   int32_t sorb_any;             // sorbed totals present: ion exchange or isotherms
   double soil_density;          // kg/m^3, for K_D scaled by a mineral's volume fraction
@@ -54,6 +56,11 @@ struct pfx_dev_layout {
   int64_t o_mnuG;                    // [K][G] dense stoichiometry per lane
   int64_t o_mlogK, o_mlogKcoef, o_mh2o;
   int64_t o_mk, o_mmolvol, o_mEa, o_mthr, o_mlim, o_mpow, o_mtemkin;  // [K]
+  int64_t o_mmsf;                    // [K] min scale factor (1 where unset)
+  int64_t o_mpn;                     // int32 [K] rate prefactors of the mineral
+  int64_t o_mpid;                    // int32 [K][P][Q] primary id of each prefactor species, -1 empty
+  int64_t o_mpalpha, o_mpbeta, o_mpatten;  // [K][P][Q]
+  int64_t o_mprate, o_mpEa;          // [K][P] rate constant (mol/m^2/s), activation energy (J/mol)
   int64_t o_ionx_k;                  // [R][G] selectivity of the lane's cation (0 if none)
   int64_t o_ionx_role;               // int32 [R][G] cation slot (0 = reference) or -1
   int64_t o_kd_type;                 // int32 [G] PFX_SORPTION_* of the lane's primary (0 = none)

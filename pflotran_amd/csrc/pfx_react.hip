@@ -229,7 +229,8 @@ __device__ __forceinline__ double logk_5term_ptr(const double *cf, double tk, do
 //   activity coefficients, lnK[M] = -log10 K * LOG_TO_LN of this cell,
 //   la[G] ln activities, mv[G] free ions, im[G] inverse free ions,
 //   xb[G] ion-exchange fractions, mlnK/mkeff/marea/mvf/mrate[K] per mineral,
-//   cs[4] the cell's porosity, saturation, volume and water density,
+//   cs[6] the cell's porosity, saturation, volume, water density and
+//   temperature (C) and a spare slot,
 //   ionx[PFX_MAX_IONX_RXN] the exchange state, fx[G] the fixed accumulation
 //   of each row.  The last three hold cell state that lives across Newton
 //   passes: in LDS it does not occupy registers for the whole loop.
@@ -238,7 +239,7 @@ struct Scr {
 };
 
 __host__ __device__ inline int scr_doubles(int M, int G, int K) {
-  int n = (M + 1) + 2 * M + 5 * G + 5 * K + 4 + PFX_MAX_IONX_RXN;
+  int n = (M + 1) + 2 * M + 5 * G + 5 * K + 6 + PFX_MAX_IONX_RXN;
   return (n + 1) & ~1;
 }
 
@@ -254,8 +255,8 @@ __device__ __forceinline__ Scr make_scr(int base, int M, int G, int K) {
   s.xb = base + 24 * G;
   s.fx = base + 32 * G;
   s.cs = base + 40 * G;
-  s.ionx = base + 40 * G + 32;
-  int p = base + 40 * G + 32 + 8 * PFX_MAX_IONX_RXN;
+  s.ionx = base + 40 * G + 48;
+  int p = base + 40 * G + 48 + 8 * PFX_MAX_IONX_RXN;
   s.C = p; p += (M + 1) * 8;
   s.gsec = p; p += M * 8;
   s.lnK = p; p += M * 8;
@@ -298,22 +299,110 @@ __device__ __forceinline__ double pow_ratio(double x, double z) {
 // used instead of all being held in SGPRs across the whole loop.
 typedef const __attribute__((address_space(4))) pfx_kernel_params *KP;
 
+// Arrhenius factor exp(Ea/R (1/298.15 - 1/T)) of an activation energy
+// (reaction_mineral.F90:775-780, 786-792); 1 where Ea <= 0
+__device__ __forceinline__ double arrhenius(double Ea, double tempc) {
+  return Ea > 0.0 ? exp(Ea / PFX_IDEAL_GAS * (1.0 / (25.0 + 273.15) - 1.0 / (tempc + 273.15))) : 1.0;
+}
+
+// ln numerator alpha la and ln denominator ln(1 + exp(ln atten + beta la))
+// of prefactor species slot o at ln activity la (reaction_mineral.F90:758-767)
+__device__ __forceinline__ void pref_species_ln(KP Pp, int o, double la, double &num, double &den) {
+  num = LD(Pp->L.o_mpalpha, o) * la;
+  den = log(1.0 + exp(log(LD(Pp->L.o_mpatten, o)) + LD(Pp->L.o_mpbeta, o) * la));
+}
+
+// The rate constant of kinetic mineral k: sum over its prefactors of
+// prod_s (monod factor) * k_p * Arrhenius_p (reaction_mineral.F90:743-786),
+// or the mineral's own constant (with its Arrhenius factor) without
+// prefactors.  jpert >= 0 takes the ln activity of that primary from
+// SD(S.xb, jpert) (a forward difference).  Group-uniform.
+__device__ __forceinline__ double mineral_rate_constant(KP Pp, const Scr &S, int k, int jpert) {
+  if (!(Pp->L.mnrl_flags & 16)) return SD(S.mkeff, k);
+  const int np = LI(Pp->L.o_mpn, k);
+  if (np == 0) return SD(S.mkeff, k);
+  const int PP = Pp->L.npref, PQ = Pp->L.npref_spec;
+  double sum = 0.0;
+  for (int p = 0; p < np; p++) {
+    double lnp = 0.0;
+    for (int q = 0; q < PQ; q++) {
+      const int o = (k * PP + p) * PQ + q;
+      const int sp = LI(Pp->L.o_mpid, o);
+      if (sp < 0) continue;
+      double num, den;
+      pref_species_ln(Pp, o, sp == jpert ? SD(S.xb, sp) : SD(S.la, sp), num, den);
+      lnp = lnp + num;
+      lnp = lnp - den;
+    }
+    sum = sum + exp(lnp) * LD(Pp->L.o_mprate, k * PP + p) * arrhenius(LD(Pp->L.o_mpEa, k * PP + p), SD(S.cs, 4));
+  }
+  return sum;
+}
+
 // Rate (mol/s/m^3 bulk) of kinetic mineral k at ln(Q/K), 0 where the
 // reference skips the mineral (reaction_mineral.F90:690-816).  Group-uniform.
-__device__ __forceinline__ double mineral_rate(KP Pp, const Scr &S, int k, double lnQK) {
+__device__ __forceinline__ double mineral_rate(KP Pp, const Scr &S, int k, double lnQK, int jpert) {
   const double QK = lnQK <= 6.90776 ? exp(lnQK) : 1.0e3;
-  const bool temkin = Pp->L.mnrl_flags & 8;
-  double aff = temkin ? 1.0 - pow(QK, 1.0 / LD(Pp->L.o_mtemkin, k)) : 1.0 - QK;
+  const bool temkin = Pp->L.mnrl_flags & 8, msf_on = Pp->L.mnrl_flags & 32;
+  const double msf = msf_on ? LD(Pp->L.o_mmsf, k) : 1.0;
+  const double tk_ = temkin ? (msf_on ? msf * LD(Pp->L.o_mtemkin, k) : LD(Pp->L.o_mtemkin, k)) : msf;
+  double aff = (temkin || msf_on) ? 1.0 - pow(QK, 1.0 / tk_) : 1.0 - QK;
   const double sgn = copysign(1.0, aff);
   if (!(SD(S.mvf, k) > 0.0 || sgn < 0.0)) return 0.0;
   const double thr = (Pp->L.mnrl_flags & 1) ? LD(Pp->L.o_mthr, k) : 0.0;
   if (thr > 0.0 && sgn < 0.0 && QK < thr) return 0.0;
   const double lim = (Pp->L.mnrl_flags & 2) ? LD(Pp->L.o_mlim, k) : 0.0;
   if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
-  const double Im_const = -SD(S.marea, k);
-  const double rate = SD(S.mkeff, k);
+  double Im_const = -SD(S.marea, k);
+  if (msf_on) Im_const = Im_const / msf;
+  const double rate = mineral_rate_constant(Pp, S, k, jpert);
   return (Pp->L.mnrl_flags & 4) ? Im_const * sgn * pow(fabs(aff), LD(Pp->L.o_mpow, k)) * rate
                              : Im_const * sgn * fabs(aff) * rate;
+}
+
+// Derivative of a prefactor mineral's rate through its prefactors
+// (reaction_mineral.F90:887-944; primary prefactor species): for each
+// species s of prefactor p, dIm/dm_s = (Im / sum) (prefactor_p / monod_s)
+// d monod_s / d a_s * gamma_s * k_p * Arrhenius_p, added to column s of the
+// lane's row with the lane's stoichiometric coefficient nur.  dIm_dsum is
+// Im (mol/s) over the summed rate constant.
+template <int G>
+__device__ __forceinline__ void prefactor_jacobian(KP Pp, const Scr &S, int k, double dIm_dsum, double nur,
+                                                   double *row) {
+  const int np = LI(Pp->L.o_mpn, k);
+  const int PP = Pp->L.npref, PQ = Pp->L.npref_spec;
+  for (int p = 0; p < np; p++) {
+    const double arr = arrhenius(LD(Pp->L.o_mpEa, k * PP + p), SD(S.cs, 4));
+    double lnp = 0.0;
+    for (int q = 0; q < PQ; q++) {
+      const int o = (k * PP + p) * PQ + q;
+      if (LI(Pp->L.o_mpid, o) < 0) continue;
+      double num, den;
+      pref_species_ln(Pp, o, SD(S.la, LI(Pp->L.o_mpid, o)), num, den);
+      lnp = lnp + num;
+      lnp = lnp - den;
+    }
+    for (int q = 0; q < PQ; q++) {
+      const int o = (k * PP + p) * PQ + q;
+      const int sp = LI(Pp->L.o_mpid, o);
+      if (sp < 0) continue;
+      const double la = SD(S.la, sp);
+      double num, den;
+      pref_species_ln(Pp, o, la, num, den);
+      const double lps = num - den;
+      const double dpref = exp(lnp - lps);
+      const double dnum = LD(Pp->L.o_mpalpha, o) * exp(lps - la);
+      const double atten = LD(Pp->L.o_mpatten, o), beta = LD(Pp->L.o_mpbeta, o);
+      const double lgb = beta * la;
+      const double denom = 1.0 + exp(log(atten) + lgb);
+      const double dden = -1.0 * exp(lps) / denom * atten * beta * exp(lgb - la);
+      const double gam_s = exp(la) * SD(S.im, sp);  // activity over molality
+      const double dspec = (dnum + dden) * gam_s;
+      const double dIm = dIm_dsum * dpref * dspec * LD(Pp->L.o_mprate, k * PP + p) * arr;
+#pragma unroll
+      for (int kk = 0; kk < G; kk++) row[kk] += (kk == sp) ? nur * dIm : 0.0;
+    }
+  }
 }
 
 // Reaction part of the Jacobian by forward differences (NUMERICAL_JACOBIAN,
@@ -338,7 +427,7 @@ __device__ __forceinline__ void mineral_fd_jacobian(KP Pp, const Scr &S, int r, 
       if (sp >= 0) lnQK += LD(Pp->L.o_mnu, k * nS + s) * SD(S.la, sp);
     }
     const double nur = live ? LD(Pp->L.o_mnuG, k * G + r) : 0.0;
-    ro += nur * (mineral_rate(Pp, S, k, lnQK) * vol);
+    ro += nur * (mineral_rate(Pp, S, k, lnQK, -1) * vol);
   }
 #pragma unroll
   for (int j = 0; j < G; j++) {
@@ -351,7 +440,7 @@ __device__ __forceinline__ void mineral_fd_jacobian(KP Pp, const Scr &S, int r, 
           if (sp >= 0) lnQK += LD(Pp->L.o_mnu, k * nS + s) * (sp == j ? SD(S.xb, j) : SD(S.la, sp));
         }
         const double nur = live ? LD(Pp->L.o_mnuG, k * G + r) : 0.0;
-        rp += nur * (mineral_rate(Pp, S, k, lnQK) * vol);
+        rp += nur * (mineral_rate(Pp, S, k, lnQK, j) * vol);
       }
       double pert;
       {
@@ -530,8 +619,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           const double v_den = ld_or(P.den_kg, c, dmy);
           const double v_tmp = ld_or(P.temperature, c, dmy);
           double v_m = 1.0, v_gam = 1.0, v_tot = 0.0, v_sorb = 0.0, v_conc = 0.0;
+          double v_fguess = 1.0e-9;
           if (MODE == PFX_MODE_EQUIL) {
             v_conc = ld_or(P.constraint_conc, ci, dmy);
+            v_fguess = ld_or(P.guess_mode == PFX_GUESS_ARRAY ? P.guess : nullptr, ci, dmy);
           } else {
             v_m = ld_or(P.pri_molal, ci, dmy);
             v_gam = ld_or(P.pri_act_coef, ci, dmy);
@@ -566,6 +657,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             SD(S.cs, 3) = den;
           }
           const double tempc = P.temperature ? v_tmp : 25.0;
+          if (r == 0) SD(S.cs, 4) = tempc;
           const double tk = tempc + 273.15;
           const double ln_tk = log(tk);
           if (MODE == PFX_MODE_EQUIL) {
@@ -575,7 +667,9 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             const double molar_to_molal = P.molal ? 1.0 : 1000.0 / den;
             ctype = live ? P.ctype[r] : PFX_CONSTRAINT_FREE;
             const double conc = live ? v_conc : 0.0;
-            free_conc = 1.0e-9;
+            // the starting free ions: a FREE_ION_GUESS (cells->guess with
+            // PFX_GUESS_ARRAY) or 1e-9 (reaction.F90:1476-1482)
+            free_conc = (P.guess_mode == PFX_GUESS_ARRAY && P.guess) ? v_fguess : 1.0e-9;
             tcon = 0.0;
             if (ctype == PFX_CONSTRAINT_TOTAL) tcon = conc * molal_to_molar;
             else if (ctype == PFX_CONSTRAINT_TOTAL_SORB) tcon = conc;  // mol/m^3 bulk (reaction.F90:1490-1492)
@@ -1026,9 +1120,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           if (sp >= 0) lnQK += LD(L.o_mnu, k * L.S + s) * SD(S.la, sp);
         }
         const double QK = lnQK <= 6.90776 ? exp(lnQK) : 1.0e3;
-        const bool temkin = L.mnrl_flags & 8;
-        const double tk_ = temkin ? LD(L.o_mtemkin, k) : 1.0;
-        double aff = temkin ? 1.0 - pow(QK, 1.0 / tk_) : 1.0 - QK;
+        const bool temkin = L.mnrl_flags & 8, msf_on = L.mnrl_flags & 32;
+        const double msf = msf_on ? LD(L.o_mmsf, k) : 1.0;
+        const double tk_ = temkin ? (msf_on ? msf * LD(L.o_mtemkin, k) : LD(L.o_mtemkin, k)) : msf;
+        double aff = (temkin || msf_on) ? 1.0 - pow(QK, 1.0 / tk_) : 1.0 - QK;
         const double sgn = copysign(1.0, aff);
         double Im = 0.0;
         if (SD(S.mvf, k) > 0.0 || sgn < 0.0) {
@@ -1036,8 +1131,9 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           if (!(thr > 0.0 && sgn < 0.0 && QK < thr)) {
             const double lim = (L.mnrl_flags & 2) ? LD(L.o_mlim, k) : 0.0;
             if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
-            const double rate = SD(S.mkeff, k);
+            const double rate = mineral_rate_constant(Pp, S, k, -1);
             double Im_const = -SD(S.marea, k);
+            if (msf_on) Im_const = Im_const / msf;
             const bool pw = L.mnrl_flags & 4;
             Im = pw ? Im_const * sgn * pow(fabs(aff), LD(L.o_mpow, k)) * rate : Im_const * sgn * fabs(aff) * rate;
             if (r == 0) SD(S.mrate, k) = Im;
@@ -1046,7 +1142,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             const double nur = live ? LD(L.o_mnuG, k * G + r) : 0.0;
             Rr += nur * ImV;
             double dIm_dQK = pw ? -ImV * LD(L.o_mpow, k) / fabs(aff) : -Im_const * rate;
-            if (temkin) dIm_dQK = dIm_dQK * (1.0 / tk_) / QK * (1.0 - aff);
+            if (temkin || msf_on) dIm_dQK = dIm_dQK * (1.0 / tk_) / QK * (1.0 - aff);
             double f = nur * dIm_dQK * QK * SD(S.cs, 3) * 1.0e-3;
             if (lim > 0.0) {
               const double dd = 1.0 + (1.0 - aff) / lim;
@@ -1056,6 +1152,8 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 #pragma unroll
               for (int kk = 0; kk < G; kk++) row[kk] += f * LD(L.o_mnuG, k * G + kk) * SD(S.im, kk);
             }
+            if (!fd_jac && (L.mnrl_flags & 16) && LI(L.o_mpn, k) > 0)
+              prefactor_jacobian<G>(Pp, S, k, ImV / rate, nur, row);
           } else if (r == 0) {
             SD(S.mrate, k) = 0.0;
           }

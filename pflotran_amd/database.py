@@ -264,6 +264,27 @@ def water_density_ifc67(t: float = 25.0, p: float = 101325.0) -> float:
 
 
 @dataclass
+class PrefactorSpecies:
+    """One factor a^alpha / (1 + atten a^beta) of a rate prefactor
+    (ts_prefactor_species_type, reaction_mineral_aux.F90; defaults 0 as
+    TSPrefactorSpeciesCreate sets them)."""
+    name: str
+    alpha: float = 0.0
+    beta: float = 0.0
+    atten: float = 0.0
+
+
+@dataclass
+class Prefactor:
+    """A parallel rate term of a kinetic mineral (PREFACTOR block): its rate
+    constant and activation energy default to the mineral's own
+    (reaction_mineral.F90:343-360)."""
+    species: List[PrefactorSpecies]
+    rate: Optional[float] = None               # mol/m^2/s
+    activation_energy: Optional[float] = None  # J/mol
+
+
+@dataclass
 class MineralKinetics:
     rate_constant: float           # mol/m^2/s
     activation_energy: float = 0.0  # J/mol
@@ -271,6 +292,8 @@ class MineralKinetics:
     rate_limiter: float = 0.0
     affinity_power: Optional[float] = None   # beta (AFFINITY_POWER)
     temkin: Optional[float] = None           # sigma (TEMKIN_CONSTANT)
+    prefactors: List[Prefactor] = field(default_factory=list)
+    min_scale_factor: Optional[float] = None  # f (MIN_SCALE_FACTOR)
 
 
 def _optional_per_mineral(vals):
@@ -354,6 +377,18 @@ class ReactionTables:
     kd_freundlich_n: np.ndarray = field(default_factory=lambda: np.zeros(0))
     kd_mineral: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
     soil_particle_density: float = 0.0            # kg/m^3 (ROCK_DENSITY)
+    # rate prefactors (mineral%kinmnrl_num_prefactors, kinmnrl_prefactor_id,
+    # kinmnrl_pref_*): [K], [K][P][Q+1], [K][P][Q], [K][P]; None without any
+    mnrl_num_prefactors: Optional[np.ndarray] = None
+    mnrl_prefactor_id: Optional[np.ndarray] = None
+    mnrl_pref_alpha: Optional[np.ndarray] = None
+    mnrl_pref_beta: Optional[np.ndarray] = None
+    mnrl_pref_atten: Optional[np.ndarray] = None
+    mnrl_pref_rate: Optional[np.ndarray] = None
+    mnrl_pref_activation_energy: Optional[np.ndarray] = None
+    # MIN_SCALE_FACTOR: allocated for every kinetic mineral (1 for those
+    # that do not set it) when any does (reaction_database.F90:1973-1983)
+    mnrl_min_scale_factor: Optional[np.ndarray] = None
     _desc: Optional[abi.ReactionDesc] = None
 
     @property
@@ -429,6 +464,15 @@ class ReactionTables:
             d.eqkddistcoef, d.eqkdlangmuirb = dp(self.kd_coef), dp(self.kd_langmuir_b)
             d.eqkdfreundlichn, d.eqkdmineral = dp(self.kd_freundlich_n), ip(self.kd_mineral)
         d.soil_particle_density = self.soil_particle_density
+        if self.mnrl_num_prefactors is not None:
+            d.max_prefactors = self.mnrl_pref_rate.shape[1]
+            d.max_prefactor_species = self.mnrl_pref_alpha.shape[2]
+            d.kinmnrl_num_prefactors, d.kinmnrl_prefactor_id = (ip(self.mnrl_num_prefactors),
+                                                                ip(self.mnrl_prefactor_id))
+            d.kinmnrl_pref_alpha, d.kinmnrl_pref_beta = dp(self.mnrl_pref_alpha), dp(self.mnrl_pref_beta)
+            d.kinmnrl_pref_atten_coef, d.kinmnrl_pref_rate = dp(self.mnrl_pref_atten), dp(self.mnrl_pref_rate)
+            d.kinmnrl_pref_activation_energy = dp(self.mnrl_pref_activation_energy)
+        d.kinmnrl_min_scale_factor = dp(self.mnrl_min_scale_factor)
         self._desc = d
         return d
 
@@ -574,6 +618,7 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
         kd_freundlich_n=np.array([kr.freundlich_n for kr in isotherms], dtype=np.float64),
         kd_mineral=np.array([mnames_.index(kr.mineral) if kr.mineral else -1 for kr in isotherms], dtype=np.int32),
         soil_particle_density=float(soil_particle_density))
+    pref_kw = prefactor_arrays([minerals[n] for n in mnames], pidx)
     return ReactionTables(
         primary_names=list(primaries), complex_names=list(secondaries), mineral_names=mnames,
         Z=Z, a0=a0, cplx_specid=cs, cplx_stoich=cn, cplx_h2ostoich=ch,
@@ -594,4 +639,37 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
         ionx_to_surf=np.array([mnames.index(rx.mineral) if rx.mineral else -1 for rx in ion_exchange],
                               dtype=np.int32),
         debye=debye_params(temperature, use_bdot), use_log_formulation=int(log_formulation),
-        act_freq=act_freq, act_alg=act_alg, **kd_kw)
+        act_freq=act_freq, act_alg=act_alg,
+        mnrl_min_scale_factor=_optional_per_mineral([minerals[n].min_scale_factor for n in mnames]),
+        **kd_kw, **pref_kw)
+
+
+def prefactor_arrays(kinetics: Sequence[MineralKinetics], pidx: Dict[str, int]) -> Dict[str, np.ndarray]:
+    """The prefactor tables of the kinetic minerals (reaction_database.F90:
+    1798-1829, 1944-1965, 2200-2240): counts, 0-based primary ids with the
+    count in slot 0, alpha/beta/attenuation per species, rate constant and
+    activation energy per prefactor, the mineral's own where the prefactor
+    leaves them unset.  Empty when no mineral has a prefactor."""
+    P = max([len(k.prefactors) for k in kinetics] + [0])
+    if P == 0:
+        return {}
+    Q = max(len(p.species) for k in kinetics for p in k.prefactors)
+    Q = max(Q, 1)
+    K = len(kinetics)
+    num = np.zeros(K, dtype=np.int32)
+    pid = np.zeros((K, P, Q + 1), dtype=np.int32)
+    alpha, beta, atten = (np.zeros((K, P, Q)) for _ in range(3))
+    rate, ea = np.zeros((K, P)), np.zeros((K, P))
+    for k, kin in enumerate(kinetics):
+        num[k] = len(kin.prefactors)
+        for p, pf in enumerate(kin.prefactors):
+            rate[k, p] = kin.rate_constant if pf.rate is None else pf.rate
+            ea[k, p] = kin.activation_energy if pf.activation_energy is None else pf.activation_energy
+            pid[k, p, 0] = len(pf.species)
+            for q, sp in enumerate(pf.species):
+                if sp.name not in pidx:
+                    raise ValueError(f"prefactor species {sp.name}: only primary species are supported")
+                pid[k, p, 1 + q] = pidx[sp.name]
+                alpha[k, p, q], beta[k, p, q], atten[k, p, q] = sp.alpha, sp.beta, sp.atten
+    return dict(mnrl_num_prefactors=num, mnrl_prefactor_id=pid, mnrl_pref_alpha=alpha, mnrl_pref_beta=beta,
+                mnrl_pref_atten=atten, mnrl_pref_rate=rate, mnrl_pref_activation_energy=ea)

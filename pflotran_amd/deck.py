@@ -43,7 +43,8 @@ from typing import Callable, Dict, Iterator, List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import abi
-from .database import (Database, DbSpecies, IonExchangeRxn, KDRxn, MineralKinetics, ReactionTables, build_tables,
+from .database import (Database, DbSpecies, IonExchangeRxn, KDRxn, MineralKinetics, Prefactor, PrefactorSpecies,
+                       ReactionTables, build_tables,
                        constraint_rxn, constraint_rxns, read_database, water_density_ifc67)
 
 
@@ -205,6 +206,7 @@ class Constraint:
     name: str
     concentrations: List[SpeciesConstraint] = field(default_factory=list)
     minerals: List[MineralConstraint] = field(default_factory=list)
+    free_ion_guess: Dict[str, float] = field(default_factory=dict)
 
 
 @dataclass
@@ -409,8 +411,16 @@ class Deck:
                 raise DeckError(f"CONSTRAINT {name}: MINERALS gives no volume fraction/area for {lack}")
             for k, n in enumerate(kin):
                 vf[k], area[k] = got[n].volume_fraction, got[n].area
+        guess = None
+        if con.free_ion_guess:
+            # every primary must be guessed (constraint.F90 guess check,
+            # reaction.F90:1265-1288)
+            lack = [n for n in c.primaries if n not in con.free_ion_guess]
+            if lack:
+                raise DeckError(f"CONSTRAINT {name}: FREE_ION_GUESS gives no value for {lack}")
+            guess = np.array([con.free_ion_guess[n] for n in c.primaries], dtype=np.float64)
         return dict(ctype=ctype, conc=conc, rxn=constraint_rxns(tables, rxns) if rxns else None,
-                    vf=vf, area=area, molal=int(c.molal))
+                    vf=vf, area=area, molal=int(c.molal), guess=guess)
 
     def next_dt(self, dt: float, newton_iterations: int) -> float:
         """The step size after a step of size `dt` that converged in
@@ -528,13 +538,17 @@ _CONSTRAINT_TYPES = {
 _CHEM_UNSUPPORTED = {
     "GENERAL_REACTION", "MICROBIAL_REACTION", "RADIOACTIVE_DECAY_REACTION", "IMMOBILE_SPECIES",
     "COLLOIDS", "REACTION_SANDBOX", "CLM_REACTION", "SOLID_SOLUTIONS", "UPDATE_POROSITY",
-    "UPDATE_TORTUOSITY", "UPDATE_PERMEABILITY", "UPDATE_MINERAL_SURFACE_AREA", "ACTIVITY_H2O",
+    "UPDATE_TORTUOSITY", "UPDATE_PERMEABILITY", "ACTIVITY_H2O",
     "ACTIVITY_WATER", "TRUNCATE_CONCENTRATION", "OPERATOR_SPLIT", "OPERATOR_SPLITTING", "GEOTHERMAL_HPT",
     "UPDATE_MNRL_SURF_AREA_WITH_POR", "UPDATE_ARMOR_MINERAL_SURFACE", "UPDATE_ARMOR_MINERAL_SURFACE_FLAG",
     "EXPLICIT_ADVECTION", "DECOUPLED_EQUILIBRIUM_MINERALS", "USE_FULL_GEOCHEMISTRY", "MINIMUM_POROSITY",
 }
 # keywords with no effect on the per-cell system
-_CHEM_IGNORED = {"NO_CHECKPOINT_ACT_COEFS", "NO_RESTART_MINERAL_VOL_FRAC", "NO_CHECK_UPDATE"}
+# (UPDATE_MINERAL_SURFACE_AREA: a transport-only run updates the surface
+# area only along with the porosity, pm_rt.F90:238-247, 515-523, and
+# UPDATE_POROSITY is refused above)
+_CHEM_IGNORED = {"NO_CHECKPOINT_ACT_COEFS", "NO_RESTART_MINERAL_VOL_FRAC", "NO_CHECK_UPDATE",
+                 "UPDATE_MINERAL_SURFACE_AREA"}
 
 
 def _names(cur: _Cursor, where: str) -> List[str]:
@@ -576,12 +590,69 @@ def _mineral_kinetics(cur: _Cursor) -> Dict[str, MineralKinetics]:
                 k.affinity_power = _num(s[1])
             elif w == "TEMKIN_CONSTANT":
                 k.temkin = _num(s[1])
+            elif w == "MINERAL_SCALE_FACTOR":
+                k.min_scale_factor = _num(s[1])
+            elif w in ("SURFACE_AREA_VOL_FRAC_POWER", "SURFACE_AREA_POROSITY_POWER"):
+                # used only where the surface area is updated, which a
+                # transport-only run does only with UPDATE_POROSITY
+                # (pm_rt.F90:238-247, 515-523), refused by this reader
+                _num(s[1])
+            elif w == "PREFACTOR":
+                k.prefactors.append(_prefactor(cur, name))
             else:
                 raise DeckError(f"MINERAL_KINETICS {name}: {w} is outside the per-cell path")
+        # a prefactor without its own rate constant or activation energy
+        # takes the mineral's (reaction_mineral.F90:343-360)
+        for pf in k.prefactors:
+            if pf.rate is None:
+                if not np.isfinite(k.rate_constant):
+                    raise DeckError(f"Both outer and inner prefactor rate constants uninitialized for kinetic "
+                                    f"mineral {name}.")
+                pf.rate = k.rate_constant
+            if pf.activation_energy is None:
+                pf.activation_energy = k.activation_energy
         if not np.isfinite(k.rate_constant):
-            raise DeckError(f"MINERAL_KINETICS {name}: no RATE_CONSTANT")
+            if not k.prefactors:
+                raise DeckError(f"MINERAL_KINETICS {name}: no RATE_CONSTANT")
+            k.rate_constant = 0.0   # unused: the prefactors carry the rate
         out[name] = k
     return out
+
+
+def _prefactor(cur: _Cursor, mineral: str) -> Prefactor:
+    """A PREFACTOR block (reaction_mineral.F90:227-335): RATE_CONSTANT
+    (10^x when negative, units to mol/m^2-sec), ACTIVATION_ENERGY, and
+    PREFACTOR_SPECIES blocks of ALPHA, BETA, ATTENUATION_COEF (default 0)."""
+    pf = Prefactor(species=[])
+    for t in cur.block(f"MINERAL_KINETICS {mineral} PREFACTOR"):
+        w = t[0].upper()
+        if w == "RATE_CONSTANT":
+            rate = _num(t[1])
+            if rate < 0.0:
+                rate = 10.0 ** rate
+            if len(t) > 2:
+                rate *= units_factor(t[2], "mol/m^2-sec")
+            pf.rate = rate
+        elif w == "ACTIVATION_ENERGY":
+            pf.activation_energy = _value_units(t, 1, "J/mol", "PREFACTOR ACTIVATION_ENERGY")
+        elif w == "PREFACTOR_SPECIES":
+            if len(t) < 2:
+                raise DeckError("PREFACTOR_SPECIES: name missing")
+            sp = PrefactorSpecies(t[1])
+            for u in cur.block(f"PREFACTOR_SPECIES {t[1]}"):
+                v = u[0].upper()
+                if v == "ALPHA":
+                    sp.alpha = _num(u[1])
+                elif v == "BETA":
+                    sp.beta = _num(u[1])
+                elif v == "ATTENUATION_COEF":
+                    sp.atten = _num(u[1])
+                else:
+                    raise DeckError(f"CHEMISTRY,MINERAL_KINETICS,PREFACTOR,SPECIES: {v} unknown")
+            pf.species.append(sp)
+        else:
+            raise DeckError(f"CHEMISTRY,MINERAL_KINETICS,PREFACTOR: {w} unknown")
+    return pf
 
 
 def _ion_exchange(cur: _Cursor) -> IonExchangeRxn:
@@ -765,7 +836,11 @@ def _constraint(cur: _Cursor, name: str) -> Constraint:
                     area *= units_factor(s[3], "m^2/m^3")
                 con.minerals.append(MineralConstraint(s[0], _num(s[1]), area))
         elif w == "FREE_ION_GUESS":
-            raise DeckError(f"CONSTRAINT {name}: FREE_ION_GUESS is not taken by pfx_equilibrate_batch")
+            # the constraint Newton's starting free ions (reaction.F90:1476-1482)
+            for s in cur.block("FREE_ION_GUESS"):
+                if len(s) < 2:
+                    raise DeckError(f"CONSTRAINT {name}: guess '{' '.join(s)}' lacks a value")
+                con.free_ion_guess[s[0]] = _num(s[1])
         else:
             raise DeckError(f"CONSTRAINT {name}: {w} is outside the per-cell path")
     return con

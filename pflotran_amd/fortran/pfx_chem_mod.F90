@@ -52,6 +52,11 @@ module pfx_chem_module
     type(c_ptr) :: eqkdspecid, eqkdtype, eqkddistcoef, eqkdlangmuirb
     type(c_ptr) :: eqkdfreundlichn, eqkdmineral
     real(c_double) :: soil_particle_density
+    integer(c_int32_t) :: max_prefactors, max_prefactor_species
+    type(c_ptr) :: kinmnrl_num_prefactors, kinmnrl_prefactor_id
+    type(c_ptr) :: kinmnrl_pref_alpha, kinmnrl_pref_beta, kinmnrl_pref_atten_coef
+    type(c_ptr) :: kinmnrl_pref_rate, kinmnrl_pref_activation_energy
+    type(c_ptr) :: kinmnrl_min_scale_factor
   end type pfx_reaction_desc
 
   type, bind(c), public :: pfx_cells

@@ -155,8 +155,11 @@ class ColumnRun:
         if self.t.K and setup.get("vf") is not None:
             arr["mnrl_volfrac"][0, :] = setup["vf"]
             arr["mnrl_area"][0, :] = setup["area"]
-        dc = runtime.DeviceCells.from_host(arr, 1)
         o = self.opts(1.0)
+        if setup.get("guess") is not None:
+            arr["guess"][0, :] = setup["guess"]
+            o.guess_mode = abi.PFX_GUESS_ARRAY
+        dc = runtime.DeviceCells.from_host(arr, 1)
         rc, st = runtime.equilibrate(self.tab, dc, setup["ctype"], np.asarray([setup["conc"]], dtype=np.float64), o,
                                      rxn=setup.get("rxn"), molal=setup.get("molal", 0))
         if rc != 0:

@@ -496,6 +496,41 @@ class GiSolver:
             pass
 
 
+def gi_initial_iterate(tables: Tables, cells: DeviceCells, opts: abi.ReactOpts, warm=None):
+    """The global-implicit step's starting iterate from each cell's own
+    reaction step.  timestepper_BE starts SNES from the old solution
+    (timestepper_BE.F90:326); here every cell first solves its reaction step
+    alone (pfx_react_batch: the cell's accumulation and rates over opts.dt,
+    no transport) on a copy of the cells, and where that converged its free
+    ions and exchange state become the Newton's starting iterate in `cells`.
+    cells.total / total_sorb (the old accumulation) and the activity
+    coefficients are left as they are, so the global residual, Jacobian and
+    solution are those of the step from the old state; only the path of the
+    global Newton changes.  warm(tables, copy) may fill copy['guess'] (a
+    learned warm start, solved with PFX_GUESS_ARRAY_FALLBACK from the old free
+    ions as fallback); without it the reaction step starts from the old free
+    ions.  Returns the reaction step's statistics."""
+    torch = _torch()
+    work = DeviceCells({k: (None if v is None else v.clone()) for k, v in cells.t.items()}, cells.ncells)
+    o = abi.ReactOpts()
+    C.memmove(C.addressof(o), C.addressof(opts), C.sizeof(o))
+    if warm is not None:
+        if work.t.get("guess") is None:
+            work.t["guess"] = torch.empty_like(work.t["pri_molal"])
+        warm(tables, work)
+        o.guess_mode = abi.PFX_GUESS_ARRAY_FALLBACK
+        o.fallback_previous = 1
+    else:
+        o.guess_mode = abi.PFX_GUESS_PREVIOUS
+    _, st = react_batch(tables, work, o)
+    ok = (work.t["status"] == 0).unsqueeze(1)
+    cells.t["pri_molal"].copy_(torch.where(ok, work.t["pri_molal"], cells.t["pri_molal"]))
+    if cells.t.get("ionx_ref_sorbed") is not None and work.t.get("ionx_ref_sorbed") is not None:
+        cells.t["ionx_ref_sorbed"].copy_(torch.where(ok, work.t["ionx_ref_sorbed"], cells.t["ionx_ref_sorbed"]))
+    del work
+    return st
+
+
 def gi_dispersion(grid: DeviceGrid, porosity, saturation=None, tortuosity=None, dispersivity=0.0, diffusion=1e-9,
                   dx=(1.0, 1.0, 1.0), bc_type=(0, 0)):
     """pfx_gi_dispersion: fills grid.dd and grid.bc_dd (device) from cell

@@ -156,6 +156,52 @@ def test_gi_newton_step_matches_direct_solves():
     gs.close()
 
 
+def test_predicted_iterate_reaches_the_same_solution():
+    # runtime.gi_initial_iterate: each cell's reaction step first, then the
+    # global Newton from its free ions.  The residual, Jacobian and tests are
+    # those of the step from the old state, so both runs land on the oracle's
+    # Newton solution; the predicted start needs no more Newton iterations.
+    # A learned warm start is stood in for by the old free ions scaled by
+    # 1.02 (it only seeds the reaction step, which falls back to the old
+    # free ions where it fails).
+    t = synth.random_network(15, 50, 3, ionx=True)
+    nx, ny, nz = 5, 4, 3
+    n = nx * ny * nz
+    rng = np.random.default_rng(41)
+    q, dd = faces(rng, nx, ny, nz)
+    bc_q = (np.full((nz, ny), 2e-5), np.full((nz, ny), -2e-5))
+    bc_dd = (np.full((nz, ny), 2e-7), np.zeros((nz, ny)))
+    cells, _, _, _ = speciated_state(t, n, seed=43)
+    bc_total = (cells["total"][0] * 3.0, None)
+    hg = orc.HostGrid(nx, ny, nz, q, dd, bc_q=bc_q, bc_dd=bc_dd, bc_total=bc_total, log_formulation=True)
+    ref = cells.copy()
+    its_c, reason_c, _ = cpu_newton(t, ref, hg, synth.react_opts(), t.N, max_it=20, rtol=1e-10)
+    assert reason_c > 0
+    tab = runtime.Tables(t)
+    g = runtime.DeviceGrid(nx, ny, nz, q, dd, bc_q=bc_q, bc_dd=bc_dd, bc_total=bc_total, log_formulation=True)
+    no = abi.NewtonOpts(rtol=1e-10, max_iterations=20)
+    no.krylov.rtol = 1e-13
+
+    def warm(tb, c):
+        c.t["guess"].copy_(c.t["pri_molal"] * 1.02)
+
+    for how in ("old", "previous", "warm"):
+        dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+        old_total = dc.t["total"].clone()
+        if how != "old":
+            pst = runtime.gi_initial_iterate(tab, dc, synth.react_opts(), warm=warm if how == "warm" else None)
+            assert pst.converged == n
+            # the old accumulation is untouched; the iterate moved
+            assert bool((dc.t["total"] == old_total).all())
+        gs = runtime.GiSolver(tab, g)
+        st = gs.step(dc, g, synth.react_opts(), no)
+        gs.close()
+        out = dc.to_host()
+        assert st.reason > 0, (how, st.as_dict())
+        assert st.iterations <= its_c + 1, (how, st.iterations, its_c)
+        assert np.max(np.abs(out["pri_molal"] - ref["pri_molal"]) / ref["pri_molal"]) <= 1e-9, how
+
+
 def test_one_closed_cell_is_the_reaction_step():
     # no faces, no boundaries: the global-implicit system of one cell is the
     # dt-consistent per-cell system pfx_react_batch solves

@@ -398,3 +398,84 @@ def test_mineral_option_parity():
     assert np.array_equal(out["status"], ref["status"]) and (ref["status"] == 0).mean() > 0.5
     ok = ref["status"] == 0
     assert rel_err(out["pri_molal"][ok], ref["pri_molal"][ok]) <= 1e-10
+
+
+def _with_prefactors(t):
+    """Rate prefactors and the min scale factor (reaction_mineral.F90:
+    704-802): mineral 0 with two parallel prefactors (H+ with an
+    attenuation, two other primaries), mineral 1 with one prefactor that
+    has an activation energy and a scale factor 2, mineral 2 plain."""
+    from pflotran_amd.database import MineralKinetics, Prefactor, PrefactorSpecies, prefactor_arrays
+    names = t.primary_names
+    kin = [MineralKinetics(float(t.mnrl_rate_constant[k]), activation_energy=float(t.mnrl_activation_energy[k]))
+           for k in range(t.K)]
+    kin[0].prefactors = [Prefactor([PrefactorSpecies(names[0], 0.37, 1.0, 1e5), PrefactorSpecies(names[3], 0.3)],
+                                   rate=2.0 * t.mnrl_rate_constant[0]),
+                         Prefactor([PrefactorSpecies(names[5], 0.5, 2.0, 40.0)], rate=t.mnrl_rate_constant[0])]
+    kin[1].prefactors = [Prefactor([PrefactorSpecies(names[2], 0.25, 1.0, 10.0)], rate=t.mnrl_rate_constant[1],
+                                   activation_energy=30000.0)]
+    arr = prefactor_arrays(kin, {n: i for i, n in enumerate(names)})
+    for k, v in arr.items():
+        setattr(t, k, v)
+    t.mnrl_min_scale_factor = np.array([1.0, 2.0, 1.0])
+    return t
+
+
+def test_prefactor_and_scale_factor_parity():
+    t = _with_prefactors(synth.random_network(15, 50, 3, ionx=True))
+    n = 400
+    cells, _, _, _ = speciated_state(t, n)
+    cells["pri_molal"][:] *= np.exp(0.3 * np.sin(np.arange(n * t.N)).reshape(n, t.N))
+    o = synth.react_opts()
+    res_o, jac_o = orc.residual_jacobian(t, cells, o)
+    rate_o = cells["mnrl_rate"].copy()
+    tab = runtime.Tables(t)
+    dc = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    res_g, jac_g = runtime.residual_jacobian(tab, dc, synth.react_opts())
+    rate_g = dc.to_host()["mnrl_rate"]
+    assert np.count_nonzero(rate_o[:, :2]) > n and np.all(np.abs(rate_g - rate_o) <= 1e-12 * np.abs(rate_o))
+    assert np.all(np.abs(jac_g - jac_o) <= jacobian_bound(t, cells, jac_o))
+    psv = 0.25 * 1.0 * 1000.0 * 1.0
+    acc = psv * np.abs(cells["total"]) / synth.STEP_DT
+    assert np.max(np.abs(res_g - res_o) / (acc + np.abs(res_o) + 1e-300)) <= 1e-12
+    start, _, _, _ = speciated_state(t, n)
+    ref = start.copy()
+    orc.react_batch(t, ref, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
+    dr = runtime.DeviceCells.from_host(host_arrays(start), n)
+    runtime.react_batch(tab, dr, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
+    out = dr.to_host()
+    assert np.array_equal(out["status"], ref["status"]) and (ref["status"] == 0).mean() > 0.9
+    ok = ref["status"] == 0
+    assert rel_err(out["pri_molal"][ok], ref["pri_molal"][ok]) <= 1e-10
+
+
+def test_equilibrate_from_a_free_ion_guess(net15):
+    # FREE_ION_GUESS (reaction.F90:1476-1482): the constraint Newton starts
+    # from cells->guess; the oracle from the same guess gives the same free
+    # ions, which are the constraint's solution found from 1e-9 as well
+    t = net15
+    n = 256
+    cells, ctype, conc, ok = speciated_state(t, n, seed=11)
+    tab = runtime.Tables(t)
+    arr = host_arrays(cells)
+    arr["guess"][:] = cells["pri_molal"] * np.exp(0.5 * np.cos(np.arange(n * t.N)).reshape(n, t.N))
+    arr["pri_molal"][:] = 1e-9
+    o = synth.react_opts()
+    o.guess_mode = abi.PFX_GUESS_ARRAY
+    dc = runtime.DeviceCells.from_host(arr, n)
+    rc, _ = runtime.equilibrate(tab, dc, ctype, conc, o)
+    assert rc == 0
+    out = dc.to_host()
+    ref = orc.HostCells({k: (None if v is None else v.copy()) for k, v in arr.items()}, n)
+    rc_o, its_o = orc.equilibrate(t, ctype, conc, ref, o)
+    assert rc_o == 0
+    assert rel_err(out["pri_molal"], ref["pri_molal"]) <= 1e-10
+    assert np.mean(np.abs(out["num_iterations"] - its_o)) < 0.5
+    d0 = runtime.DeviceCells.from_host(host_arrays(cells), n)
+    d0.t["pri_molal"].fill_(1e-9)
+    runtime.equilibrate(tab, d0, ctype, conc, synth.react_opts())
+    plain = d0.to_host()
+    # the constraint Newton stops on max|dm/m| < 1e-6 (or max|R| < 1e-12):
+    # two starts end within that of each other
+    assert rel_err(out["pri_molal"], plain["pri_molal"]) <= 1e-6
+    assert out["num_iterations"].mean() < plain["num_iterations"].mean()

@@ -183,3 +183,118 @@ def test_rate_limiter_jacobian_is_the_reference_expression():
     ratio = (ref / exact)[:, None, None]
     assert not close(Jm, Jfd)
     assert close(Jm, Jfd * ratio)
+
+
+# ---------------------------------------------------------------------------
+# rate prefactors (PREFACTOR / PREFACTOR_SPECIES, reaction_mineral.F90:227-360,
+# 743-786, 887-944) and MIN_SCALE_FACTOR (:704-716, 800-802, 848-858)
+PREF = [dict(rate=3.5e-8, species=[("HCO3-", 0.31, 0.0, 0.0), ("H+", 0.37, 1.0, 1e9)]),
+        dict(rate=None, Ea=30000.0, species=[("Ca++", 0.5, 2.0, 40.0)])]
+
+
+def pref_tables(prefactors=PREF, msf=None, rate_constant=1e-9):
+    from pflotran_amd.database import Prefactor, PrefactorSpecies
+    pf = [Prefactor([PrefactorSpecies(n, a, b, k) for n, a, b, k in p["species"]], rate=p.get("rate"),
+                    activation_energy=p.get("Ea")) for p in prefactors]
+    db = read_database(os.path.join(DB, "calcite.dat"))
+    return build_tables(db, PRIM, CPLX, {"Calcite": MineralKinetics(rate_constant, prefactors=pf,
+                                                                    min_scale_factor=msf)},
+                        act_freq=abi.PFX_ACT_OFF)
+
+
+def pref_law(c, prefactors=PREF, k_outer=1e-9):
+    """sum_p k_p A_p(T) prod_s a_s^alpha / (1 + atten a_s^beta), the
+    prefactor rate constant written out (activities = molalities here)."""
+    total = np.zeros(c.ncells)
+    one = np.ones(c.ncells)
+    for p in prefactors:
+        f = one.copy()
+        for name, alpha, beta, atten in p["species"]:
+            a = c["pri_molal"][:, PRIM.index(name)] * c["pri_act_coef"][:, PRIM.index(name)]
+            f = f * a ** alpha / (1.0 + atten * a ** beta)
+        kp = k_outer if p.get("rate") is None else p["rate"]
+        Ea = p.get("Ea", 0.0) or 0.0
+        arr = np.exp(Ea / R_GAS * (1.0 / 298.15 - 1.0 / (c["temperature"] + 273.15))) if Ea > 0 else 1.0
+        total = total + kp * arr * f
+    return total
+
+
+def test_prefactor_rate_matches_the_law():
+    t = pref_tables()
+    assert t.mnrl_num_prefactors.tolist() == [2] and t.mnrl_pref_rate[0, 1] == 1e-9   # outer rate inherited
+    log_omega = np.array([-3.0, -1.0, -0.2, 0.4, 1.5, 3.2])
+    c = cells_at(t, log_omega)
+    o = abi.default_opts(dt=100.0, den_kg=DEN)
+    rate, _, _ = oracle_rate_and_jac(t, c, o)
+    _, qk, a = tst_rate(t, c, 1.0)
+    want = -c["mnrl_area"][:, 0] * np.sign(a) * np.abs(a) * pref_law(c)
+    assert np.all(np.abs(rate - want) <= 1e-13 * np.abs(want)), (rate, want)
+    # without the attenuation term the H+ factor is a^alpha alone
+    t2 = pref_tables([dict(rate=2e-8, species=[("H+", 0.37, 1.0, 0.0)])])
+    rate2, _, _ = oracle_rate_and_jac(t2, c, o)
+    want2 = -c["mnrl_area"][:, 0] * np.sign(a) * np.abs(a) * 2e-8 * (c["pri_molal"][:, 0] ** 0.37)
+    assert np.all(np.abs(rate2 - want2) <= 1e-13 * np.abs(want2))
+
+
+def test_min_scale_factor_rate_matches_the_law():
+    f = 2.5
+    t = pref_tables([], msf=f)
+    log_omega = np.array([-2.0, -0.4, 0.3, 1.1])
+    c = cells_at(t, log_omega)
+    rate, _, _ = oracle_rate_and_jac(t, c, abi.default_opts(dt=100.0, den_kg=DEN))
+    _, qk, _ = tst_rate(t, c, 1.0)
+    a = 1.0 - qk ** (1.0 / f)
+    want = -c["mnrl_area"][:, 0] / f * np.sign(a) * np.abs(a) * 1e-9
+    assert np.all(np.abs(rate - want) <= 1e-13 * np.abs(want))
+    # with a Temkin constant the exponent is 1 / (f sigma)
+    from pflotran_amd.database import build_tables as bt
+    db = read_database(os.path.join(DB, "calcite.dat"))
+    t3 = bt(db, PRIM, CPLX, {"Calcite": MineralKinetics(1e-9, temkin=1.5, min_scale_factor=f)},
+            act_freq=abi.PFX_ACT_OFF)
+    rate3, _, _ = oracle_rate_and_jac(t3, c, abi.default_opts(dt=100.0, den_kg=DEN))
+    a3 = 1.0 - qk ** (1.0 / (f * 1.5))
+    want3 = -c["mnrl_area"][:, 0] / f * np.sign(a3) * np.abs(a3) * 1e-9
+    assert np.all(np.abs(rate3 - want3) <= 1e-13 * np.abs(want3))
+
+
+@pytest.mark.parametrize("which", ["prefactor", "min_scale_factor"])
+def test_prefactor_and_scale_factor_jacobian_is_the_derivative(which):
+    # at a water density of 1000 kg/m^3 the reference's rho/1000 factor on
+    # the Q/K part is 1, and the prefactor part has none: the analytic
+    # mineral Jacobian is the derivative of the residual
+    log_omega = np.array([-2.0, -0.5, -0.1, 0.7, 1.8])
+    # the mineral's part: J less J with no mineral surface (a zero rate
+    # constant would make the reference's Im / sum 0/0)
+    t = pref_tables() if which == "prefactor" else pref_tables([], msf=2.5)
+    c = cells_at(t, log_omega)
+    # a long step: the accumulation's rounding over the H+ column's tiny
+    # difference step stays below the mineral terms
+    o = abi.default_opts(dt=1e7, den_kg=1000.0)
+
+    def bare(x):
+        x = x.copy()
+        x["mnrl_area"][:] = 0.0
+        return x
+    _, _, J = oracle_rate_and_jac(t, c, o)
+    _, _, J0 = oracle_rate_and_jac(t, bare(c), o)
+    Jm = J - J0
+    Jfd = np.zeros_like(J)
+    for j in range(t.N):
+        h = 1e-4 * c["pri_molal"][:, j]
+        cp, cm = c.copy(), c.copy()
+        cp["pri_molal"][:, j] += h
+        cm["pri_molal"][:, j] -= h
+        _, rp, _ = oracle_rate_and_jac(t, cp, o)
+        _, rm, _ = oracle_rate_and_jac(t, cm, o)
+        _, rp0, _ = oracle_rate_and_jac(t, bare(cp), o)
+        _, rm0, _ = oracle_rate_and_jac(t, bare(cm), o)
+        Jfd[:, :, j] = ((rp - rp0) - (rm - rm0)) / (2 * h[:, None])
+    scale = np.max(np.abs(Jfd), axis=(1, 2), keepdims=True)
+    assert np.max(np.abs(Jm)) > 0.0
+    assert np.all(np.abs(Jm - Jfd) <= 1e-4 * scale)
+    if which == "prefactor":
+        # the H+ column carries the prefactor's derivative: dropping it breaks
+        # the match (a plausible omission)
+        Jq = Jm.copy()
+        Jq[:, :, 0] = Jfd[:, :, 0] * 0.0
+        assert not np.all(np.abs(Jq - Jfd) <= 1e-4 * scale)
