@@ -531,7 +531,8 @@ int pfx_graph_destroy(pfx_graph *graph);
    (reactive_transport.F90:642).  vf_k += mnrl_rate_k * molar_vol_k * dt,
    clamped at 0, with the rates that pfx_react_batch returned at the
    solution (the RKineticMineral evaluation the reference repeats there).
-   Writes cells->mnrl_volfrac (device). */
+   Writes cells->mnrl_volfrac (device).  With opts->stream_ordered it
+   returns without waiting for the stream. */
 int pfx_update_kinetic_state(pfx_tables *tables, pfx_cells *cells, const pfx_react_opts *opts);
 
 #ifdef __cplusplus

@@ -78,6 +78,26 @@ int pfx_bsr_spmv(pfx_bsr *h, const double *values, const double *x, double *y, v
 int pfx_bsr_solve(pfx_bsr *h, const double *values, const double *b, double *x, const pfx_krylov_opts *opts,
                   void *stream, pfx_krylov_stats *stats);
 
+/* Several right-hand sides of one scalar matrix (a handle of block size
+   1): B, X [nrows][ncomp] (device, cell-major: component c of row i at
+   i*ncomp + c).  Each column is solved by its own BiCGStab with Jacobi
+   preconditioning, as pfx_bsr_solve would (the same scalars and tests per
+   column), but every scalar stays on the device: the columns share each
+   launch, and a decided column skips its work.  stream_ordered != 0
+   enqueues exactly opts->max_iterations iterations (at most
+   PFX_KSP_MAX_ENQUEUED) and returns without waiting, so the solve can be
+   captured in a graph; pfx_bsr_multi_stats then reads the outcome.
+   Otherwise the host reads one flag every 4 iterations and stats are
+   filled on return (iterations summed over the columns, the worst reason,
+   the largest norms).  No counterpart in the reference: its transport
+   components are separate KSP solves. */
+#define PFX_KSP_MAX_ENQUEUED 1000
+int pfx_bsr_solve_multi(pfx_bsr *h, const double *values, const double *B, double *X, int32_t ncomp,
+                        const pfx_krylov_opts *opts, void *stream, int32_t stream_ordered, pfx_krylov_stats *stats);
+/* The outcome of the handle's last several-right-hand-side solve once
+   `stream` has run it (waits for the stream). */
+int pfx_bsr_multi_stats(pfx_bsr *h, void *stream, pfx_krylov_stats *stats);
+
 /* SNESSetTolerances meanings (solver.F90:256); zeros select PETSc's
    defaults atol 1e-50, rtol 1e-8, stol 1e-8, max_iterations 50. */
 typedef struct pfx_newton_opts {
@@ -173,14 +193,41 @@ int pfx_gi_solver_timestep(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *g
    assembled once and each component solved by pfx_bsr_solve.
    h: a handle of block size 1 on pfx_gi_structure's pattern for the grid;
    cells->total [ncells][ncomp] (device): in the old totals, out the
-   transported ones; work: device doubles [nnzb + 2 ncells].  stats sum
-   the components' iterations and keep the worst reason.  Returns 0, 1 if
-   a component's solve did not converge, or a negative PFX_E_* code.
+   transported ones; work: device doubles [nnzb + 2 ncells] (the matrix
+   values in the first nnzb).  The components are solved together by
+   pfx_bsr_solve_multi.  stats sum the components' iterations and keep the
+   worst reason.  Returns 0, 1 if a component's solve did not converge, or a
+   negative PFX_E_* code.  With opts->stream_ordered the step is enqueued
+   without waiting (kopts->max_iterations iterations, graph-capturable) and
+   pfx_bsr_multi_stats gives the outcome.
    No counterpart in the current reference build (its operator split is
    disabled). */
 int pfx_os_transport_step(pfx_bsr *h, const pfx_grid *grid, pfx_cells *cells, int32_t ncomp,
                           const pfx_react_opts *opts, double *work, const pfx_krylov_opts *kopts,
                           pfx_krylov_stats *stats);
+
+/* One whole operator-split step captured as a HIP graph: the transport
+   half (pfx_os_transport_step with transport_opts, every component in one
+   several-right-hand-side solve of kopts->max_iterations enqueued
+   iterations), the reaction half (pfx_react_batch with react_opts) and the
+   volume-fraction update (pfx_update_kinetic_state with react_opts), all
+   stream-ordered, in the order of the reference's operator-split step
+   (deprecated/timestepper.F90:2156-2245 transport, RTReact, then
+   RTUpdateKineticState).  The arguments are those of the three calls; the
+   graph holds their pointers and options (dt included), so a replay
+   advances the same cells by one more step of the same size.  Replay with
+   pfx_os_graph_launch; after a replay has run, pfx_os_graph_stats reads the
+   reaction statistics (pfx_react_batch's return value in rc) and the
+   transport solve's outcome.  No counterpart in the reference (its steps are
+   host loops); the captured step is the device-resident form of one
+   RTReact step. */
+typedef struct pfx_os_graph pfx_os_graph;
+int pfx_os_graph_create(pfx_bsr *transport, const pfx_grid *grid, pfx_tables *tables, pfx_cells *cells,
+                        int32_t ncomp, const pfx_react_opts *transport_opts, const pfx_react_opts *react_opts,
+                        double *work, const pfx_krylov_opts *kopts, pfx_os_graph **out);
+int pfx_os_graph_launch(pfx_os_graph *graph, void *stream);
+int pfx_os_graph_stats(pfx_os_graph *graph, int32_t *rc, pfx_stats *react_stats, pfx_krylov_stats *transport_stats);
+int pfx_os_graph_destroy(pfx_os_graph *graph);
 
 #ifdef __cplusplus
 }

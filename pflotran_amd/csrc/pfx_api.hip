@@ -42,6 +42,10 @@ struct pfx_tables {
   uint64_t *d_finish = nullptr;
   int finish_cap = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // side stream of a zoned launch's pivoting follow-up for these tables'
+  // zone (default priority), with its fork and join events
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // the last stream-ordered pfx_react_batch, whose statistics
   // pfx_tables_stats reads (-1: none pending)
   int64_t pending_n = -1;
@@ -427,6 +431,13 @@ struct zone_setup {
   int total_blocks = 0;
 };
 
+int ensure_side(pfx_tables *t) {
+  if (!t->side) HIPCHK(hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking));
+  if (!t->fork_ev) HIPCHK(hipEventCreateWithFlags(&t->fork_ev, hipEventDisableTiming));
+  if (!t->join_ev) HIPCHK(hipEventCreateWithFlags(&t->join_ev, hipEventDisableTiming));
+  return 0;
+}
+
 int zone_lds(const pfx_tables *t) {
   const int G = t->L.G;
   return (int)(t->L.lds_table_bytes + (int64_t)(BLOCK / G) * pfx_scr_doubles(t->L.M, G, t->L.K) * 8);
@@ -445,6 +456,10 @@ int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, c
     if (rc) return rc;
   }
   HIPCHK(hipSetDevice(dev));
+  for (int z = 0; z < nzones; z++) {
+    int rc = ensure_side(tables[z]);
+    if (rc) return rc;
+  }
   zs.nzones = nzones;
   memset(&zs.Z, 0, sizeof(zs.Z));
   zs.Z.nzones = nzones;
@@ -508,6 +523,10 @@ int zones_enqueue(pfx_tables *const *tables, zone_setup &zs, const pfx_kernel_pa
   if (ev0) HIPCHK(hipEventRecord(ev0, s));
   void *args[] = {(void *)&d_params, (void *)&zs.Z, (void *)&d_finish};
   HIPCHK(hipLaunchKernel(zs.fn, dim3((unsigned)zs.total_blocks), dim3(BLOCK), args, (size_t)zs.lds, s));
+  // the zones' pivoting launches run concurrently, each on its tables' side
+  // stream forked from s and joined back into it (their tails overlap
+  // instead of adding up)
+  HIPCHK(hipEventRecord(tables[0]->fork_ev, s));
   for (int z = 0; z < zs.nzones; z++) {
     pfx_tables *t = tables[z];
     pfx_kernel_params P2 = zs.P[z];
@@ -519,8 +538,11 @@ int zones_enqueue(pfx_tables *const *tables, zone_setup &zs, const pfx_kernel_pa
     const int64_t need = (zs.n[z] + (BLOCK / t->L.G) - 1) / (BLOCK / t->L.G);
     const int64_t blocks2 = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)t->num_cu));
     void *args2[] = {(void *)&P2};
-    HIPCHK(hipLaunchKernel(fn_piv, dim3((unsigned)blocks2), dim3(BLOCK), args2, (size_t)lds, s));
+    HIPCHK(hipStreamWaitEvent(t->side, tables[0]->fork_ev, 0));
+    HIPCHK(hipLaunchKernel(fn_piv, dim3((unsigned)blocks2), dim3(BLOCK), args2, (size_t)lds, t->side));
+    HIPCHK(hipEventRecord(t->join_ev, t->side));
   }
+  for (int z = 0; z < zs.nzones; z++) HIPCHK(hipStreamWaitEvent(s, tables[z]->join_ev, 0));
   if (ev1) HIPCHK(hipEventRecord(ev1, s));
   for (int z = 0; z < zs.nzones; z++) {
     int rc = stats_launch(tables[z], zs.P[z], zs.n[z], s);
@@ -979,6 +1001,9 @@ extern "C" int pfx_tables_destroy(pfx_tables *t) {
   if (t->d_defer) (void)hipFree(t->d_defer);
   if (t->ev0) (void)hipEventDestroy(t->ev0);
   if (t->ev1) (void)hipEventDestroy(t->ev1);
+  if (t->fork_ev) (void)hipEventDestroy(t->fork_ev);
+  if (t->join_ev) (void)hipEventDestroy(t->join_ev);
+  if (t->side) (void)hipStreamDestroy(t->side);
   delete t;
   return 0;
 }
@@ -1174,9 +1199,28 @@ extern "C" int pfx_update_kinetic_state(pfx_tables *t, pfx_cells *cells, const p
   hipLaunchKernelGGL(k_kinetic_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, t->d_blob,
                      t->L.o_mmolvol, t->L.K, cells->ncells, cells->mnrl_rate, (double *)cells->mnrl_volfrac, o->dt);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(stream));
+  if (!o->stream_ordered) HIPCHK(hipStreamSynchronize(stream));
   return 0;
 }
+
+namespace pfx_internal {
+// the per-call buffers a reaction step of these cells needs, allocated
+// ahead of a graph capture (no allocation happens inside it)
+int tables_reserve(pfx_tables *t, const pfx_cells *cells) {
+  HIPCHK(hipSetDevice(t->device));
+  int rc = ensure_defer(t, cells->ncells);
+  if (rc) return rc;
+  if (!cells->num_iterations || !cells->status || !cells->rel_res) rc = ensure_scratch(t, cells->ncells);
+  return rc;
+}
+// a captured stream-ordered pfx_react_batch of n cells: its statistics are
+// readable by pfx_tables_stats after each replay
+void tables_rearm_stats(pfx_tables *t, int64_t n, hipStream_t stream, bool fallback) {
+  t->pending_n = n;
+  t->pending_stream = stream;
+  t->pending_fallback = fallback;
+}
+}  // namespace pfx_internal
 
 extern "C" int pfx_tables_timing(pfx_tables *t, double *ms) {
   if (!t || !ms) return PFX_E_ARG;

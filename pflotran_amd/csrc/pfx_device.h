@@ -147,6 +147,13 @@ const int32_t *bsr_row_ptr(const ::pfx_bsr *h);
 int64_t bsr_rows(const ::pfx_bsr *h);
 int32_t bsr_block(const ::pfx_bsr *h);
 int64_t bsr_nnzb(const ::pfx_bsr *h);
+// right-hand-side and solution vectors [nrows][C] of a handle's
+// several-right-hand-side solve (allocated on first use)
+int bsr_multi_io(::pfx_bsr *h, int C, double **B, double **X);
+// allocate what a reaction step of these cells needs (before a graph capture)
+int tables_reserve(pfx_tables *t, const pfx_cells *cells);
+// arm pfx_tables_stats for a replayed stream-ordered reaction step of n cells
+void tables_rearm_stats(pfx_tables *t, int64_t n, hipStream_t stream, bool fallback);
 // event-timed launch bracket; the time goes to the tables' last_ms[slot]
 int timed_launch_begin(pfx_tables *t, hipStream_t s);
 int timed_launch_end(pfx_tables *t, hipStream_t s, int slot);

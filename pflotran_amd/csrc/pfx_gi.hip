@@ -368,6 +368,34 @@ __global__ void k_os_scatter(const os_params P) {
   P.total[c * P.N + P.comp] = P.x[c];
 }
 
+// every component's right-hand side at once, cell-major [n][N] (the
+// layout of the totals and of the several-right-hand-side solve)
+__global__ void k_os_rhs_all(const os_params P) {
+  const int64_t nx = P.nx, ny = P.ny, n = nx * ny * (int64_t)P.nz;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * P.N) return;
+  const int64_t c = e / P.N;
+  const int comp = (int)(e % P.N);
+  const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+  double r = os_accum(P, c) * P.total[e];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    if (P.bc_q[side] && P.bc_total[side] && i == (side == 0 ? 0 : nx - 1)) {
+      const int64_t f = j + ny * k;
+      double Tup, Tdn;
+      flux_coef(P.area[0], P.bc_q[side][f], P.bc_dd[side][f], Tup, Tdn);
+      r += Tup * P.bc_total[side][comp];
+    }
+  }
+  P.rhs[e] = r;
+}
+
+__global__ void k_os_scatter_all(const os_params P) {
+  const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n * P.N) P.total[e] = P.x[e];
+}
+
 extern "C" int pfx_gi_structure(const pfx_grid *g, int32_t *row_ptr, int32_t *col_idx, int64_t *nnzb) {
   if (!g || !nnzb || g->nx < 1 || g->ny < 1 || g->nz < 1) return PFX_E_ARG;
   const int64_t nx = g->nx, ny = g->ny, nz = g->nz, n = nx * ny * nz;
@@ -522,27 +550,121 @@ extern "C" int pfx_os_transport_step(pfx_bsr *h, const pfx_grid *g, pfx_cells *c
     return PFX_E_ARG;
   hipStream_t s = (hipStream_t)o->stream;
   const unsigned gb = (unsigned)((n + 255) / 256);
+  // every component in one several-right-hand-side solve, cell-major as
+  // the totals are (one pfx_bsr_solve per component has the same scalars
+  // and tests per component, in ncomp times the launches)
+  double *B = nullptr, *X = nullptr;
+  if ((rc = pfx_internal::bsr_multi_io(h, ncomp, &B, &X))) return rc;
+  P.rhs = B;
+  P.x = X;
+  const unsigned ge = (unsigned)((n * ncomp + 255) / 256);
   hipLaunchKernelGGL(k_os_matrix, dim3(gb), dim3(256), 0, s, P);
+  hipLaunchKernelGGL(k_os_rhs_all, dim3(ge), dim3(256), 0, s, P);
   GCHK(hipGetLastError());
   pfx_krylov_stats acc;
   memset(&acc, 0, sizeof(acc));
-  acc.reason = PFX_KSP_CONVERGED_RTOL;
-  for (int j = 0; j < ncomp; j++) {
-    P.comp = j;
-    hipLaunchKernelGGL(k_os_rhs, dim3(gb), dim3(256), 0, s, P);
-    GCHK(hipGetLastError());
-    pfx_krylov_stats st;
-    rc = pfx_bsr_solve(h, P.vals, P.rhs, P.x, kopts, o->stream, &st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_os_scatter, dim3(gb), dim3(256), 0, s, P);
-    GCHK(hipGetLastError());
-    // the components' worst outcome: a divergence reason wins
-    acc.iterations += st.iterations;
-    if (st.reason < 0 || (acc.reason > 0 && st.reason == PFX_KSP_CONVERGED_ATOL)) acc.reason = st.reason;
-    acc.rnorm0 = fmax(acc.rnorm0, st.rnorm0);
-    acc.rnorm = fmax(acc.rnorm, st.rnorm);
-    acc.ms += st.ms;
-  }
+  rc = pfx_bsr_solve_multi(h, P.vals, B, X, ncomp, kopts, o->stream, o->stream_ordered, &acc);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_os_scatter_all, dim3(ge), dim3(256), 0, s, P);
+  GCHK(hipGetLastError());
+  if (o->stream_ordered) return 0;   // the outcome: pfx_bsr_multi_stats
+  GCHK(hipStreamSynchronize(s));
   if (stats) *stats = acc;
   return acc.reason < 0 ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// one operator-split step as a graph
+struct pfx_os_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  pfx_tables *tables = nullptr;
+  pfx_bsr *h = nullptr;
+  int64_t ncells = 0;
+  bool fallback = false;
+  int device = 0;
+  hipStream_t last = nullptr;
+};
+
+extern "C" int pfx_os_graph_destroy(pfx_os_graph *g) {
+  if (!g) return 0;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return 0;
+}
+
+extern "C" int pfx_os_graph_create(pfx_bsr *h, const pfx_grid *grid, pfx_tables *tables, pfx_cells *cells,
+                                   int32_t ncomp, const pfx_react_opts *topts, const pfx_react_opts *ropts,
+                                   double *work, const pfx_krylov_opts *kopts, pfx_os_graph **out) {
+  if (!out || !h || !grid || !tables || !cells || !topts || !ropts || !work || !kopts) return PFX_E_ARG;
+  *out = nullptr;
+  if (kopts->max_iterations < 1 || kopts->max_iterations > PFX_KSP_MAX_ENQUEUED) return PFX_E_ARG;
+  pfx_os_graph *g = new pfx_os_graph();
+  GCHK(hipGetDevice(&g->device));
+  g->tables = tables;
+  g->h = h;
+  g->ncells = cells->ncells;
+  g->fallback = ropts->guess_mode == PFX_GUESS_ARRAY_FALLBACK;
+  pfx_react_opts to = *topts, ro = *ropts;
+  to.stream_ordered = 1;
+  ro.stream_ordered = 1;
+  // every buffer the three calls use is allocated before the capture
+  double *B = nullptr, *X = nullptr;
+  int rc0 = pfx_internal::bsr_multi_io(h, ncomp, &B, &X);
+  if (rc0 == 0) rc0 = pfx_internal::tables_reserve(tables, cells);
+  if (rc0) {
+    delete g;
+    return rc0;
+  }
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+    delete g;
+    return PFX_E_DEVICE;
+  }
+  to.stream = (void *)cs;
+  ro.stream = (void *)cs;
+  int rc = 0;
+  bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) == hipSuccess;
+  if (ok) {
+    rc = pfx_os_transport_step(h, grid, cells, ncomp, &to, work, kopts, nullptr);
+    if (rc == 0) rc = pfx_react_batch(tables, cells, &ro, nullptr);
+    if (rc == 0) rc = pfx_update_kinetic_state(tables, cells, &ro);
+    hipGraph_t graph = nullptr;
+    const bool ended = hipStreamEndCapture(cs, &graph) == hipSuccess;
+    g->graph = graph;
+    ok = rc == 0 && ended && graph && hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  }
+  (void)hipStreamDestroy(cs);
+  if (!ok) {
+    (void)hipGetLastError();
+    pfx_os_graph_destroy(g);
+    return rc ? rc : PFX_E_DEVICE;
+  }
+  *out = g;
+  return 0;
+}
+
+extern "C" int pfx_os_graph_launch(pfx_os_graph *g, void *stream) {
+  if (!g || !g->exec) return PFX_E_ARG;
+  GCHK(hipSetDevice(g->device));
+  GCHK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  g->last = (hipStream_t)stream;
+  return 0;
+}
+
+extern "C" int pfx_os_graph_stats(pfx_os_graph *g, int32_t *rc, pfx_stats *rs, pfx_krylov_stats *ks) {
+  if (!g || !g->exec) return PFX_E_ARG;
+  GCHK(hipSetDevice(g->device));
+  GCHK(hipStreamSynchronize(g->last));
+  if (ks) {
+    int r = pfx_bsr_multi_stats(g->h, (void *)g->last, ks);
+    if (r) return r;
+  }
+  pfx_internal::tables_rearm_stats(g->tables, g->ncells, g->last, g->fallback);
+  pfx_stats tmp;
+  const int r = pfx_tables_stats(g->tables, rs ? rs : &tmp);
+  if (r < 0) return r;
+  if (rc) *rc = r;
+  return 0;
 }

@@ -13,6 +13,7 @@
 // (BiCGStab's scalars are needed on the host to steer the iteration anyway).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -454,6 +455,306 @@ __global__ __launch_bounds__(BLOCK) void k_newton_update(int64_t n, double *__re
   block_reduce2(yy, xx, partial);
 }
 
+// ---------------------------------------------------------------------------
+// Several right-hand sides of one scalar (block size 1) matrix, BiCGStab for
+// each with Jacobi preconditioning, every scalar of the iteration on the
+// device.  Vectors are cell-major [n][C] (the transport vector's layout, C
+// components per cell); a workgroup holds RB = BLOCK / C whole rows, thread t
+// owns component t % C of row t / C, so each thread always sees the same
+// component and the per-component partial sums of a workgroup are fixed-order
+// sums.  One-workgroup kernels turn the partial sums into each component's
+// scalars and convergence status; the element kernels skip components whose
+// status is set.  No value goes to the host between iterations: a solve is
+// stream-ordered and can be captured in a graph.
+struct mrhs_sc {
+  double rho, rho_new, alpha, omega, beta;
+  double tol, rnorm0, rnorm, dtol;
+  int status;    // 0 running, PFX_KSP_* once decided
+  int conv;      // the convergence reason of this component (rtol or atol)
+  int it;        // iterations taken
+  int pending;   // converged on |s| this iteration: x += alpha phat owed
+};
+
+__device__ __forceinline__ int mrhs_comps(int C) { return (BLOCK / C) * C; }
+
+// per-component sums over this workgroup's threads: thread c < C of the
+// workgroup writes partial[(blockIdx.x * C + c) * 2 + {0,1}]
+__device__ __forceinline__ void mrhs_block_sum(double a, double b, int C, double *partial) {
+  __shared__ double sh[2][BLOCK];
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  __syncthreads();
+  const int used = mrhs_comps(C);
+  if ((int)threadIdx.x < C) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int t = threadIdx.x; t < used; t += C) {
+      s0 += sh[0][t];
+      s1 += sh[1][t];
+    }
+    partial[((int64_t)blockIdx.x * C + threadIdx.x) * 2] = s0;
+    partial[((int64_t)blockIdx.x * C + threadIdx.x) * 2 + 1] = s1;
+  }
+}
+
+// true in the workgroup that finishes last (its partial sums, and every
+// other workgroup's, are then visible to it); that workgroup turns the
+// partial sums into the scalars and re-arms the counter
+__device__ __forceinline__ bool mrhs_last_block(int *counter) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(counter, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    if (threadIdx.x == 0) *counter = 0;
+  }
+  return last;
+}
+
+// the two sums of component c over the nb workgroups, in workgroup order
+__device__ __forceinline__ void mrhs_sums(int nb, int C, int c, const double *partial, double &a, double &b) {
+  a = 0.0;
+  b = 0.0;
+  for (int k = 0; k < nb; k++) {
+    a += partial[((int64_t)k * C + c) * 2];
+    b += partial[((int64_t)k * C + c) * 2 + 1];
+  }
+}
+
+__device__ __forceinline__ void mrhs_set_done(int C, const mrhs_sc *sc, int *done) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int all = 1;
+    for (int k = 0; k < C; k++) all = all && sc[k].status != 0;
+    *done = all;
+  }
+}
+
+// grid-stride loop over the rows of a workgroup's threads
+#define MRHS_ROWS(n, C)                                                       \
+  const int c_ = threadIdx.x % (C);                                           \
+  const int rl_ = threadIdx.x / (C);                                          \
+  const bool on_ = (int)threadIdx.x < mrhs_comps(C);                          \
+  const int64_t rstep_ = (int64_t)gridDim.x * (BLOCK / (C));                  \
+  for (int64_t row = (int64_t)blockIdx.x * (BLOCK / (C)) + rl_; on_ && row < (n); row += rstep_)
+
+struct mrhs_args {
+  int64_t n;
+  int C, nb, maxit, reserved;
+  const int32_t *rp, *ci;
+  const double *vals, *dinv, *b;
+  double *x, *r, *rh, *p, *v, *s, *t, *ph, *sh;
+  double *partial;
+  mrhs_sc *sc;
+  int *counter, *done, *fail;
+  double rtol, atol, dtol;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_m_diag(int64_t n, const int32_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ ci, const double *__restrict__ vals,
+                                                  double *__restrict__ dinv, int *__restrict__ fail) {
+  const int64_t row = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (row >= n) return;
+  double d = 0.0;
+  for (int32_t k = rp[row]; k < rp[row + 1]; k++)
+    if (ci[k] == row) d = vals[k];
+  if (d == 0.0 || !(d == d)) atomicAdd(fail, 1);
+  dinv[row] = d != 0.0 ? 1.0 / d : 0.0;
+}
+
+// x = 0, r = rhat = b, p = v = 0; then the first scalars and the start
+// tests of every component
+__global__ __launch_bounds__(BLOCK) void k_m_init(const mrhs_args A) {
+  const int C = A.C;
+  double bb = 0.0;
+  MRHS_ROWS(A.n, C) {
+    const int64_t i = row * C + c_;
+    const double bi = A.b[i];
+    A.x[i] = 0.0;
+    A.r[i] = bi;
+    A.rh[i] = bi;
+    A.p[i] = 0.0;
+    A.v[i] = 0.0;
+    bb += bi * bi;
+  }
+  mrhs_block_sum(bb, 0.0, C, A.partial);
+  if (!mrhs_last_block(A.counter)) return;
+  const int c = threadIdx.x;
+  if (c < C) {
+    double sum, unused;
+    mrhs_sums(A.nb, C, c, A.partial, sum, unused);
+    mrhs_sc S;
+    S.rnorm0 = sqrt(sum);
+    S.rnorm = S.rnorm0;
+    S.tol = fmax(A.rtol * S.rnorm0, A.atol);
+    S.dtol = A.dtol * S.rnorm0;
+    S.conv = A.rtol * S.rnorm0 >= A.atol ? PFX_KSP_CONVERGED_RTOL : PFX_KSP_CONVERGED_ATOL;
+    S.rho = 1.0;
+    S.alpha = 1.0;
+    S.omega = 1.0;
+    S.rho_new = sum;
+    S.beta = 0.0;
+    S.it = 0;
+    S.pending = 0;
+    S.status = 0;
+    if (*A.fail > 0) S.status = PFX_KSP_DIVERGED_PC_FAILED;
+    else if (!isfinite(S.rnorm0)) S.status = PFX_KSP_DIVERGED_NANORINF;
+    else if (S.rnorm0 <= S.tol) S.status = S.conv;
+    A.sc[c] = S;
+  }
+  mrhs_set_done(C, A.sc, A.done);
+}
+
+// p = r + beta (p - omega v);  phat = D^-1 p
+__global__ __launch_bounds__(BLOCK) void k_m_p(const mrhs_args A) {
+  if (*A.done) return;
+  const int C = A.C;
+  MRHS_ROWS(A.n, C) {
+    if (A.sc[c_].status != 0) continue;
+    const int64_t i = row * C + c_;
+    const double pv = A.r[i] + A.sc[c_].beta * (A.p[i] - A.sc[c_].omega * A.v[i]);
+    A.p[i] = pv;
+    A.ph[i] = A.dinv[row] * pv;
+  }
+}
+
+// v = A phat, partial (rhat, v); then alpha = rho_new / (rhat, v)
+__global__ __launch_bounds__(BLOCK) void k_m_v(const mrhs_args A) {
+  if (*A.done) return;
+  const int C = A.C;
+  double a = 0.0;
+  MRHS_ROWS(A.n, C) {
+    if (A.sc[c_].status != 0) continue;
+    double acc = 0.0;
+    for (int32_t k = A.rp[row]; k < A.rp[row + 1]; k++) acc += A.vals[k] * A.ph[(int64_t)A.ci[k] * C + c_];
+    const int64_t i = row * C + c_;
+    A.v[i] = acc;
+    a += A.rh[i] * acc;
+  }
+  mrhs_block_sum(a, 0.0, C, A.partial);
+  if (!mrhs_last_block(A.counter)) return;
+  const int c = threadIdx.x;
+  if (c < C && A.sc[c].status == 0) {
+    double rv, unused;
+    mrhs_sums(A.nb, C, c, A.partial, rv, unused);
+    if (rv == 0.0 || !isfinite(rv)) {
+      A.sc[c].status = isfinite(rv) ? PFX_KSP_DIVERGED_BREAKDOWN : PFX_KSP_DIVERGED_NANORINF;
+      A.sc[c].it += 1;
+    } else {
+      A.sc[c].alpha = A.sc[c].rho_new / rv;
+    }
+  }
+}
+
+// s = r - alpha v;  shat = D^-1 s;  partial (s, s); then BiCGStab's early
+// exit: |s| <= tol converges with x += alpha phat owed
+__global__ __launch_bounds__(BLOCK) void k_m_s(const mrhs_args A) {
+  if (*A.done) return;
+  const int C = A.C;
+  double ss = 0.0;
+  MRHS_ROWS(A.n, C) {
+    if (A.sc[c_].status != 0) continue;
+    const int64_t i = row * C + c_;
+    const double si = A.r[i] - A.sc[c_].alpha * A.v[i];
+    A.s[i] = si;
+    A.sh[i] = A.dinv[row] * si;
+    ss += si * si;
+  }
+  mrhs_block_sum(ss, 0.0, C, A.partial);
+  if (!mrhs_last_block(A.counter)) return;
+  const int c = threadIdx.x;
+  if (c < C && A.sc[c].status == 0) {
+    double sum, unused;
+    mrhs_sums(A.nb, C, c, A.partial, sum, unused);
+    const double sn = sqrt(sum);
+    if (!isfinite(sn)) {
+      A.sc[c].status = PFX_KSP_DIVERGED_NANORINF;
+      A.sc[c].it += 1;
+    } else if (sn <= A.sc[c].tol) {
+      A.sc[c].rnorm = sn;
+      A.sc[c].status = A.sc[c].conv;
+      A.sc[c].pending = 1;
+      A.sc[c].it += 1;
+    }
+  }
+}
+
+// t = A shat, partial (t, s), (t, t); then omega = (t, s) / (t, t)
+__global__ __launch_bounds__(BLOCK) void k_m_t(const mrhs_args A) {
+  if (*A.done) return;
+  const int C = A.C;
+  double ts = 0.0, tt = 0.0;
+  MRHS_ROWS(A.n, C) {
+    if (A.sc[c_].status != 0) continue;
+    double acc = 0.0;
+    for (int32_t k = A.rp[row]; k < A.rp[row + 1]; k++) acc += A.vals[k] * A.sh[(int64_t)A.ci[k] * C + c_];
+    const int64_t i = row * C + c_;
+    A.t[i] = acc;
+    ts += A.s[i] * acc;
+    tt += acc * acc;
+  }
+  mrhs_block_sum(ts, tt, C, A.partial);
+  if (!mrhs_last_block(A.counter)) return;
+  const int c = threadIdx.x;
+  if (c < C && A.sc[c].status == 0) {
+    double sts, stt;
+    mrhs_sums(A.nb, C, c, A.partial, sts, stt);
+    if (stt == 0.0 || !isfinite(stt)) {
+      A.sc[c].status = isfinite(stt) ? PFX_KSP_DIVERGED_BREAKDOWN : PFX_KSP_DIVERGED_NANORINF;
+      A.sc[c].it += 1;
+    } else {
+      A.sc[c].omega = sts / stt;
+    }
+  }
+}
+
+// x += alpha phat + omega shat;  r = s - omega t;  partial (r, r),
+// (rhat, r) (a component converged on |s| takes x += alpha phat only); then
+// the residual tests, and the next iteration's start tests (iteration
+// limit, breakdown) and beta
+__global__ __launch_bounds__(BLOCK) void k_m_xr(const mrhs_args A) {
+  if (*A.done) return;
+  const int C = A.C;
+  double rr = 0.0, hr = 0.0;
+  MRHS_ROWS(A.n, C) {
+    const int64_t i = row * C + c_;
+    if (A.sc[c_].pending) {
+      A.x[i] += A.sc[c_].alpha * A.ph[i];
+    } else if (A.sc[c_].status == 0) {
+      A.x[i] += A.sc[c_].alpha * A.ph[i] + A.sc[c_].omega * A.sh[i];
+      const double rv = A.s[i] - A.sc[c_].omega * A.t[i];
+      A.r[i] = rv;
+      rr += rv * rv;
+      hr += A.rh[i] * rv;
+    }
+  }
+  mrhs_block_sum(rr, hr, C, A.partial);
+  if (!mrhs_last_block(A.counter)) return;
+  const int c = threadIdx.x;
+  if (c < C) {
+    mrhs_sc S = A.sc[c];
+    if (S.pending) {
+      S.pending = 0;
+    } else if (S.status == 0) {
+      double srr, shr;
+      mrhs_sums(A.nb, C, c, A.partial, srr, shr);
+      S.it += 1;
+      S.rho = S.rho_new;
+      S.rho_new = shr;
+      S.rnorm = sqrt(srr);
+      if (!isfinite(S.rnorm)) S.status = PFX_KSP_DIVERGED_NANORINF;
+      else if (S.rnorm <= S.tol) S.status = S.conv;
+      else if (S.rnorm >= S.dtol) S.status = PFX_KSP_DIVERGED_DTOL;
+      else if (S.it >= A.maxit) S.status = PFX_KSP_DIVERGED_ITS;
+      else if (S.rho_new == 0.0 || S.omega == 0.0) S.status = PFX_KSP_DIVERGED_BREAKDOWN;
+      else S.beta = (S.rho_new / S.rho) * (S.alpha / S.omega);
+    }
+    A.sc[c] = S;
+  }
+  mrhs_set_done(C, A.sc, A.done);
+}
+
 int grid_for(int64_t work_items) {
   int64_t g = (work_items + BLOCK - 1) / BLOCK;
   if (g > MAX_GRID) g = MAX_GRID;
@@ -476,6 +777,16 @@ struct pfx_bsr {
   double *partial = nullptr, *red = nullptr, *host2 = nullptr;
   int *fail = nullptr, *host_fail = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // several right-hand sides at block size 1 (pfx_bsr_solve_multi): C
+  // components, vectors [nrows][C], the per-component scalars, the
+  // all-decided flag and its pinned host copy
+  int mC = 0;
+  double *mvec = nullptr, *mdinv = nullptr, *mpartial = nullptr;
+  mrhs_sc *msc = nullptr, *mhost_sc = nullptr;
+  int *mdone = nullptr, *mhost_done = nullptr;
+  int *mcounter = nullptr;   // workgroups done in the current reduction (0 between launches)
+  bool mpending = false;   // a stream-ordered solve whose statistics are unread
+  hipStream_t mstream = nullptr;
 };
 
 // shape of a handle, for the other translation units of the library
@@ -607,6 +918,14 @@ extern "C" int pfx_bsr_destroy(pfx_bsr *h) {
   if (h->host_fail) (void)hipHostFree(h->host_fail);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
+  if (h->mvec) (void)hipFree(h->mvec);
+  if (h->mdinv) (void)hipFree(h->mdinv);
+  if (h->mpartial) (void)hipFree(h->mpartial);
+  if (h->msc) (void)hipFree(h->msc);
+  if (h->mdone) (void)hipFree(h->mdone);
+  if (h->mcounter) (void)hipFree(h->mcounter);
+  if (h->mhost_sc) (void)hipHostFree(h->mhost_sc);
+  if (h->mhost_done) (void)hipHostFree(h->mhost_done);
   delete h;
   return 0;
 }
@@ -723,6 +1042,179 @@ extern "C" int pfx_bsr_solve(pfx_bsr *h, const double *values, const double *b, 
   if (st) *st = S;
   return 0;
 }
+
+namespace {
+
+int ensure_multi(pfx_bsr *h, int C) {
+  if (h->mC == C && h->mvec) return 0;
+  if (h->mvec) KCHK(hipFree(h->mvec));
+  if (h->mdinv) KCHK(hipFree(h->mdinv));
+  if (h->mpartial) KCHK(hipFree(h->mpartial));
+  h->mvec = h->mdinv = h->mpartial = nullptr;
+  h->mC = 0;
+  const size_t nv = (size_t)h->nrows * C;
+  // r, rhat, p, v, s, t, phat, shat, and a right-hand side and solution
+  // (pfx_os_transport_step's)
+  if (hipMalloc((void **)&h->mvec, 10 * nv * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&h->mdinv, (size_t)h->nrows * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&h->mpartial, (size_t)MAX_GRID * C * 2 * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    return PFX_E_NOMEM;
+  }
+  if (!h->msc) {
+    if (hipMalloc((void **)&h->msc, PFX_MAX_PRIMARY * sizeof(mrhs_sc)) != hipSuccess ||
+        hipMalloc((void **)&h->mdone, sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&h->mcounter, sizeof(int)) != hipSuccess ||
+        hipMemset(h->mcounter, 0, sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&h->mhost_sc, PFX_MAX_PRIMARY * sizeof(mrhs_sc), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&h->mhost_done, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return PFX_E_NOMEM;
+    }
+  }
+  h->mC = C;
+  return 0;
+}
+
+// the components' outcome as one pfx_krylov_stats: iterations summed, the
+// worst reason (a divergence before an atol convergence before rtol), the
+// largest norms
+void multi_stats(const mrhs_sc *sc, int C, pfx_krylov_stats *st) {
+  pfx_krylov_stats S;
+  memset(&S, 0, sizeof(S));
+  S.reason = PFX_KSP_CONVERGED_RTOL;
+  for (int c = 0; c < C; c++) {
+    S.iterations += sc[c].it;
+    const int r = sc[c].status == 0 ? PFX_KSP_DIVERGED_ITS : sc[c].status;
+    if (r < 0 || (S.reason > 0 && r == PFX_KSP_CONVERGED_ATOL)) S.reason = r;
+    S.rnorm0 = fmax(S.rnorm0, sc[c].rnorm0);
+    S.rnorm = fmax(S.rnorm, sc[c].rnorm);
+  }
+  *st = S;
+}
+
+}  // namespace
+
+extern "C" int pfx_bsr_solve_multi(pfx_bsr *h, const double *values, const double *B, double *X, int32_t ncomp,
+                                   const pfx_krylov_opts *o, void *stream, int32_t stream_ordered,
+                                   pfx_krylov_stats *st) {
+  if (!h || !values || !B || !X || h->N != 1 || ncomp < 1 || ncomp > PFX_MAX_PRIMARY) return PFX_E_ARG;
+  const double rtol = (o && o->rtol > 0.0) ? o->rtol : 1.0e-5;
+  const double atol = (o && o->atol > 0.0) ? o->atol : 1.0e-50;
+  const double dtol = (o && o->dtol > 0.0) ? o->dtol : 1.0e5;
+  const int maxit = (o && o->max_iterations > 0) ? o->max_iterations : 10000;
+  if (stream_ordered && maxit > PFX_KSP_MAX_ENQUEUED) return PFX_E_ARG;
+  KCHK(hipSetDevice(h->device));
+  int rc = ensure_multi(h, ncomp);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int C = ncomp;
+  const int64_t n = h->nrows;
+  const size_t nv = (size_t)n * C;
+  double *r = h->mvec, *rh = r + nv, *p = rh + nv, *v = p + nv, *sv = v + nv, *t = sv + nv, *ph = t + nv,
+         *sh = ph + nv;
+  const int64_t rb = BLOCK / C;
+  int nb = (int)std::min<int64_t>((n + rb - 1) / rb, MAX_GRID);
+  if (nb < 1) nb = 1;
+  const int gd = (int)std::min<int64_t>((n + BLOCK - 1) / BLOCK, 1 << 20);
+  if (!stream_ordered) KCHK(hipEventRecord(h->e0, s));
+  mrhs_args A;
+  memset(&A, 0, sizeof(A));
+  A.n = n;
+  A.C = C;
+  A.nb = nb;
+  A.maxit = maxit;
+  A.rp = h->rp;
+  A.ci = h->ci;
+  A.vals = values;
+  A.dinv = h->mdinv;
+  A.b = B;
+  A.x = X;
+  A.r = r;
+  A.rh = rh;
+  A.p = p;
+  A.v = v;
+  A.s = sv;
+  A.t = t;
+  A.ph = ph;
+  A.sh = sh;
+  A.partial = h->mpartial;
+  A.sc = h->msc;
+  A.counter = h->mcounter;
+  A.done = h->mdone;
+  A.fail = h->fail;
+  A.rtol = rtol;
+  A.atol = atol;
+  A.dtol = dtol;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->fail);
+  hipLaunchKernelGGL(k_m_diag, dim3(gd), dim3(BLOCK), 0, s, n, h->rp, h->ci, values, h->mdinv, h->fail);
+  hipLaunchKernelGGL(k_m_init, dim3(nb), dim3(BLOCK), 0, s, A);
+  KCHK(hipGetLastError());
+  // five launches per iteration; each reduction's last workgroup turns the
+  // partial sums into the scalars
+  auto iteration = [&]() {
+    hipLaunchKernelGGL(k_m_p, dim3(nb), dim3(BLOCK), 0, s, A);
+    hipLaunchKernelGGL(k_m_v, dim3(nb), dim3(BLOCK), 0, s, A);
+    hipLaunchKernelGGL(k_m_s, dim3(nb), dim3(BLOCK), 0, s, A);
+    hipLaunchKernelGGL(k_m_t, dim3(nb), dim3(BLOCK), 0, s, A);
+    hipLaunchKernelGGL(k_m_xr, dim3(nb), dim3(BLOCK), 0, s, A);
+  };
+  if (stream_ordered) {
+    // exactly maxit iterations enqueued; decided components skip their work
+    for (int it = 0; it < maxit; it++) iteration();
+    KCHK(hipGetLastError());
+    h->mpending = true;
+    h->mstream = s;
+    return 0;
+  }
+  // steered from the host in rounds of a few iterations: one 4-byte read
+  // per round, not four per iteration
+  KCHK(hipMemcpyAsync(h->mhost_done, h->mdone, sizeof(int), hipMemcpyDeviceToHost, s));
+  KCHK(hipStreamSynchronize(s));
+  int it = 0;
+  while (!*h->mhost_done && it < maxit) {
+    const int round = std::min(4, maxit - it);
+    for (int k = 0; k < round; k++) iteration();
+    it += round;
+    KCHK(hipGetLastError());
+    KCHK(hipMemcpyAsync(h->mhost_done, h->mdone, sizeof(int), hipMemcpyDeviceToHost, s));
+    KCHK(hipStreamSynchronize(s));
+  }
+  KCHK(hipMemcpyAsync(h->mhost_sc, h->msc, C * sizeof(mrhs_sc), hipMemcpyDeviceToHost, s));
+  KCHK(hipEventRecord(h->e1, s));
+  KCHK(hipEventSynchronize(h->e1));
+  h->mpending = false;
+  if (st) {
+    multi_stats(h->mhost_sc, C, st);
+    float ms = 0.f;
+    KCHK(hipEventElapsedTime(&ms, h->e0, h->e1));
+    st->ms = ms;
+  }
+  return 0;
+}
+
+extern "C" int pfx_bsr_multi_stats(pfx_bsr *h, void *stream, pfx_krylov_stats *st) {
+  if (!h || !st || !h->msc || h->mC < 1) return PFX_E_ARG;
+  KCHK(hipSetDevice(h->device));
+  hipStream_t s = (hipStream_t)stream;
+  KCHK(hipMemcpyAsync(h->mhost_sc, h->msc, h->mC * sizeof(mrhs_sc), hipMemcpyDeviceToHost, s));
+  KCHK(hipStreamSynchronize(s));
+  h->mpending = false;
+  multi_stats(h->mhost_sc, h->mC, st);
+  return 0;
+}
+
+namespace pfx_internal {
+int bsr_multi_io(pfx_bsr *h, int C, double **B, double **X) {
+  int rc = ensure_multi(h, C);
+  if (rc) return rc;
+  const size_t nv = (size_t)h->nrows * C;
+  *B = h->mvec + 8 * nv;
+  *X = h->mvec + 9 * nv;
+  return 0;
+}
+}  // namespace pfx_internal
 
 // ---------------------------------------------------------------------------
 // the global-implicit Newton step

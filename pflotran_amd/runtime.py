@@ -31,7 +31,9 @@ EXPORTED = ["pfx_abi_version", "pfx_tables_create", "pfx_tables_destroy", "pfx_r
             "pfx_surrogate_guess_from",
             # global-implicit step on the device (include/pfx_gi_solver.h)
             "pfx_bsr_create", "pfx_bsr_destroy", "pfx_bsr_spmv", "pfx_bsr_solve", "pfx_gi_solver_create",
-            "pfx_gi_solver_destroy", "pfx_gi_solver_step", "pfx_gi_solver_timestep", "pfx_os_transport_step"]
+            "pfx_gi_solver_destroy", "pfx_gi_solver_step", "pfx_gi_solver_timestep", "pfx_os_transport_step",
+            "pfx_bsr_solve_multi", "pfx_bsr_multi_stats", "pfx_os_graph_create", "pfx_os_graph_launch",
+            "pfx_os_graph_stats", "pfx_os_graph_destroy"]
 
 _lib = None
 
@@ -102,6 +104,13 @@ def load_library(path: str = LIB_PATH):
                                            C.c_double, P(abi.GiStepStats)]
     lib.pfx_os_transport_step.argtypes = [vp, P(abi.Grid), P(abi.Cells), i32, P(abi.ReactOpts), vp,
                                           P(abi.KrylovOpts), P(abi.KrylovStats)]
+    lib.pfx_bsr_solve_multi.argtypes = [vp, vp, vp, vp, i32, P(abi.KrylovOpts), vp, i32, P(abi.KrylovStats)]
+    lib.pfx_bsr_multi_stats.argtypes = [vp, vp, P(abi.KrylovStats)]
+    lib.pfx_os_graph_create.argtypes = [vp, P(abi.Grid), vp, P(abi.Cells), i32, P(abi.ReactOpts), P(abi.ReactOpts),
+                                        vp, P(abi.KrylovOpts), P(vp)]
+    lib.pfx_os_graph_launch.argtypes = [vp, vp]
+    lib.pfx_os_graph_stats.argtypes = [vp, P(i32), P(abi.Stats), P(abi.KrylovStats)]
+    lib.pfx_os_graph_destroy.argtypes = [vp]
     if lib.pfx_abi_version() != abi.PFX_ABI_VERSION:
         raise PfxError("ABI version mismatch between libpfx_chem.so and pflotran_amd.abi")
     _lib = lib
@@ -437,6 +446,53 @@ class OsTransport:
 
     def close(self):
         self.bsr.close()
+
+
+class OsStepGraph:
+    """pfx_os_graph_create / launch / stats: one operator-split step
+    (transport of every component, reaction, volume-fraction update)
+    captured as a HIP graph on `cells`, replayed with launch()."""
+
+    def __init__(self, transport: OsTransport, grid: DeviceGrid, tables: Tables, cells: DeviceCells, ncomp: int,
+                 transport_opts: abi.ReactOpts, react_opts: abi.ReactOpts, rtol: float = 1e-12,
+                 max_iterations: int = 60):
+        self.lib = load_library()
+        self._keep = (transport, grid, tables, cells, grid.struct(), cells.struct())
+        ko = abi.KrylovOpts(rtol=rtol, max_iterations=max_iterations)
+        h = C.c_void_p()
+        rc = self.lib.pfx_os_graph_create(transport.bsr.handle, C.byref(self._keep[4]), tables.handle,
+                                          C.byref(self._keep[5]), int(ncomp), C.byref(transport_opts),
+                                          C.byref(react_opts), transport.work.data_ptr(), C.byref(ko), C.byref(h))
+        if rc != 0:
+            raise PfxError(f"pfx_os_graph_create failed: {rc}")
+        self.handle = h
+
+    def launch(self):
+        torch = _torch()
+        rc = self.lib.pfx_os_graph_launch(self.handle, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if rc != 0:
+            raise PfxError(f"pfx_os_graph_launch failed: {rc}")
+
+    def stats(self):
+        """(reaction return code, reaction Stats, transport KrylovStats) of the
+        last replay."""
+        rc = C.c_int32()
+        st, ks = abi.Stats(), abi.KrylovStats()
+        r = self.lib.pfx_os_graph_stats(self.handle, C.byref(rc), C.byref(st), C.byref(ks))
+        if r != 0:
+            raise PfxError(f"pfx_os_graph_stats failed: {r}")
+        return rc.value, st, ks
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.pfx_os_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class GiSolver:

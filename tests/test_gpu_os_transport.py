@@ -105,3 +105,100 @@ def test_device_resident_operator_split_column():
     assert np.max(np.abs(out["mnrl_volfrac"] - ref["mnrl_volfrac"])) <= 1e-12
     # the inflow water has moved into the column
     assert rel_err(out["total"][0], inflow) < rel_err(cells["total"][0], inflow)
+
+
+def test_multi_rhs_solve_matches_single_solves():
+    # pfx_bsr_solve_multi: every column solved by the same BiCGStab as
+    # pfx_bsr_solve (same scalars and tests per column; the dot products are
+    # summed in another order, so iterates agree to rounding), against the
+    # direct solve; stream-ordered, it gives the same result
+    import ctypes as C
+
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+    rng = np.random.default_rng(3)
+    nx, ny, nz, N = 9, 7, 5, 6
+    n = nx * ny * nz
+    f = random_grid(rng, nx, ny, nz, N)
+    g = runtime.DeviceGrid(nx, ny, nz, **f)
+    rp, ci = runtime.gi_structure(g)
+    rp_h, ci_h = rp.cpu().numpy(), ci.cpu().numpy()
+    vals = rng.normal(size=len(ci_h))
+    for row in range(n):
+        for k in range(rp_h[row], rp_h[row + 1]):
+            if ci_h[k] == row:
+                vals[k] = 8.0 + abs(vals[k])
+    A = sp.csr_matrix((vals, ci_h, rp_h), shape=(n, n))
+    Bm = rng.normal(size=(n, N))
+    Bm[:, 2] = 0.0                        # a zero right-hand side: decided at the start
+    want = np.stack([spla.spsolve(A.tocsc(), Bm[:, c]) for c in range(N)], axis=1)
+    bsr = runtime.Bsr(rp, ci, 1)
+    lib = runtime.load_library()
+    dv = torch.as_tensor(vals, device="cuda")
+    dB = torch.as_tensor(Bm, device="cuda")
+    for ordered in (0, 1):
+        X = torch.zeros_like(dB)
+        ko = abi.KrylovOpts(rtol=1e-13, max_iterations=200)
+        st = abi.KrylovStats()
+        s = torch.cuda.current_stream().cuda_stream
+        rc = lib.pfx_bsr_solve_multi(bsr.handle, dv.data_ptr(), dB.data_ptr(), X.data_ptr(), N, C.byref(ko),
+                                     C.c_void_p(s), ordered, C.byref(st))
+        assert rc == 0
+        if ordered:
+            assert lib.pfx_bsr_multi_stats(bsr.handle, C.c_void_p(s), C.byref(st)) == 0
+        # the zero column decides at the start on atol (rtol |b| = 0): the
+        # worst reason of the columns is an atol convergence
+        assert st.reason == abi.PFX_KSP_CONVERGED_ATOL, (ordered, st.reason)
+        x = X.cpu().numpy()
+        assert np.all(x[:, 2] == 0.0)
+        assert np.max(np.abs(x - want)) <= 1e-10 * np.max(np.abs(want))
+        # each column against its own single-right-hand-side solve
+        for c in (0, 4):
+            xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+            rs, ss = bsr.solve(dv, dB[:, c].contiguous(), xs, rtol=1e-13, max_iterations=200)
+            assert np.max(np.abs(xs.cpu().numpy() - x[:, c])) <= 1e-12 * np.max(np.abs(want[:, c]))
+    bsr.close()
+
+
+def test_operator_split_step_graph_replays_match_eager():
+    """One whole operator-split step (transport of every component,
+    reaction, volume fractions) captured as a graph: each replay, with the
+    eager steps and synchronisations in between, leaves the same bits as the
+    eager step on a twin of the cells."""
+    import torch
+    t = carbonate_column()
+    col = column.Column()
+    cells, inflow = initial_state(t, col.ncells)
+    dt = 86400.0
+    N = t.N
+    tab = runtime.Tables(t)
+    f = column.grid_faces(col, inflow)
+    grid = runtime.DeviceGrid(f["nx"], f["ny"], f["nz"], f["q"], f["dd"], area=f["area"], bc_q=f["bc_q"],
+                              bc_dd=f["bc_dd"], bc_total=f["bc_total"])
+    o_t = abi.default_opts(dt=dt, porosity=col.porosity, saturation=col.saturation, volume=col.volume)
+    de = runtime.DeviceCells.from_host({k: v for k, v in cells.arrays.items()}, col.ncells)
+    dg = runtime.DeviceCells.from_host({k: v for k, v in cells.arrays.items()}, col.ncells)
+    tr_e, tr_g = runtime.OsTransport(grid), runtime.OsTransport(grid)
+    # a warm-up step on a scratch twin sizes the library's buffers before capture
+    dw = runtime.DeviceCells.from_host({k: v for k, v in cells.arrays.items()}, col.ncells)
+    tr_g.step(dw, grid, N, o_t, rtol=1e-12, max_iterations=60)
+    runtime.react_batch(tab, dw, react_opts(dt))
+    print("operator-split step graph: capture", flush=True)
+    graph = runtime.OsStepGraph(tr_g, grid, tab, dg, N, o_t, react_opts(dt), rtol=1e-12, max_iterations=60)
+    for step in range(5):
+        rc, ts = tr_e.step(de, grid, N, o_t, rtol=1e-12, max_iterations=60)
+        assert rc == 0
+        rc, st = runtime.react_batch(tab, de, react_opts(dt))
+        runtime.update_kinetic_state(tab, de, react_opts(dt))
+        torch.cuda.synchronize()
+        graph.launch()
+        rc_g, st_g, ks_g = graph.stats()
+        print(f"replay {step}: transport its {ks_g.iterations} (eager {ts.iterations})", flush=True)
+        assert rc_g == rc == 0 and st_g.converged == st.converged == col.ncells
+        assert ks_g.reason == ts.reason == abi.PFX_KSP_CONVERGED_RTOL and ks_g.iterations == ts.iterations
+        for k in ("total", "pri_molal", "mnrl_volfrac", "pri_act_coef"):
+            assert torch.equal(dg.t[k], de.t[k]), (step, k)
+    graph.close()
+    tr_e.close()
+    tr_g.close()
