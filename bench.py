@@ -758,7 +758,8 @@ def zones_leg(args, log):
     # the trials start from the step's inputs, not from the reacted state the
     # separate launches left
     restore()
-    plan, history = autotune.tune_zones(tabs, dcs, opts)
+    tabs, plan, lanes, lane_history = autotune.tune_zone_lanes(tabs, dcs, opts)
+    history = lane_history
     best = {"variant": plan.variant, "blocks": list(plan.blocks[:4])}
     # the tuned plan eagerly and as a graph replay, alternated, so the two
     # see the same device state
@@ -795,6 +796,7 @@ def zones_leg(args, log):
            "cells_per_s": conv / (float(np.mean(g_ms)) * 1e-3),
            "separate_launches_ms": float(np.mean(sep)), "eager_tuned_plan_ms": float(np.mean(e_ms)),
            "eager_tuned_plan_kernel_ms": float(np.mean(ek_ms)), "autotune_history": history, "blocks": best["blocks"],
+           "group_lanes": lanes,
            "variant": best["variant"], "graph_plan": {"variant": gp.variant, "blocks": list(gp.blocks[:4]),
                                                       "zone_ms": list(gp.zone_ms[:4])},
            "mean_iterations_per_zone": its, "iteration_histogram": hist.tolist(),

@@ -223,6 +223,13 @@ typedef struct pfx_reaction_desc {
      848-858): affinity 1 - (Q/K)^(1/(f sigma)) and the rate divided by f;
      NULL = none */
   const double *kinmnrl_min_scale_factor;       /* [K] or NULL                 */
+
+  /* Lanes per cell group in the per-cell kernels (no counterpart in the
+     reference; a tuning knob): 0 picks the smallest of 8, 16, 32 that holds
+     N; 8, 16 or 32 (>= N) forces it: a group is the LU's row tile, so a wider
+     group runs fewer cells per wavefront with idle padding rows. */
+  int32_t group_lanes;
+  int32_t reserved_lanes;
 } pfx_reaction_desc;
 
 /* Per-cell state, device pointers, cell-major.  In = read, Out = written. */

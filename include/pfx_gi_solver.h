@@ -107,6 +107,13 @@ typedef struct pfx_newton_opts {
   int32_t max_iterations;
   int32_t reserved;
   pfx_krylov_opts krylov;
+  /* PFLOTRAN's own tests, each off at 0 (NEWTON_SOLVER ITOL, ITOL_UPDATE,
+     solver.F90:861-867; the transport process model's ITOL_RELATIVE_UPDATE,
+     pm_rt.F90:171-174): ||F||_inf < itol_res, ||dx||_inf < itol_update and
+     max |dx / x_old| < itol_rel_update (x = ln m in log form) converge */
+  double itol_res;
+  double itol_update;
+  double itol_rel_update;
 } pfx_newton_opts;
 
 /* SNESConvergedReason values */
@@ -119,6 +126,13 @@ typedef struct pfx_newton_opts {
 /* PFLOTRAN's ConvergenceTest: residual infinity norm above max_norm 1e20
    (convergence.F90:230-232, solver.F90:137) */
 #define PFX_SNES_DIVERGED_MAX_NORM       (-10)
+/* PFLOTRAN's own convergence reasons: ||F||_inf below ITOL (convergence.F90:
+   215-217), ||dx||_inf below ITOL_UPDATE (:221-223), and the transport
+   process model's ITOL_RELATIVE_UPDATE, max |dx / x| below it
+   (pm_rt.F90:901-925 with convergence.F90:194-198) */
+#define PFX_SNES_CONVERGED_ITOL_RES        10
+#define PFX_SNES_CONVERGED_ITOL_UPDATE     11
+#define PFX_SNES_CONVERGED_RELATIVE_UPDATE 12
 
 typedef struct pfx_newton_stats {
   int32_t iterations;         /* Newton updates taken */
@@ -147,9 +161,11 @@ int pfx_gi_solver_destroy(pfx_gi_solver *s);
    log form, else scaled by 0.99 min(m / y) where that is below 1); x -= y
    (m <- m exp(-y) in log form); tests in SNESConvergedDefault's order:
    NaN, ||F|| < atol, ||F|| <= rtol ||F0||, ||y|| < stol ||x||, the
-   iteration limit, then PFLOTRAN's ConvergenceTest (convergence.F90:
-   170-245): ||F||_inf > 1e20 diverges, and no test but NaN applies to the
-   initial residual (newton_min_iterations 1: one Newton iteration at
+   iteration limit, then PFLOTRAN's PMRTCheckUpdatePost and ConvergenceTest
+   (pm_rt.F90:885-925, convergence.F90:170-245): the itol tests of
+   pfx_newton_opts converge (reasons 12, 10, 11 in that order of override),
+   ||F||_inf > 1e20 diverges, and no test but NaN applies to the initial
+   residual (newton_min_iterations 1: one Newton iteration at
    least, also from a zero residual).  Totals at the solution come from
    pfx_auxvar_batch. */
 int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_grid *grid, const pfx_react_opts *opts,

@@ -406,18 +406,22 @@ def os_transport_step(grid: HostGrid, acc: np.ndarray, total: np.ndarray):
 
 
 def gi_newton(t: ReactionTables, cells: HostCells, grid: HostGrid, opts: abi.ReactOpts, rtol=1e-8, stol=1e-8,
-              atol=1e-50, max_it=50, max_dlnC=5.0):
+              atol=1e-50, max_it=50, max_dlnC=5.0, itol_res=None, itol_upd=None, rel_update=None,
+              max_norm=1e20):
     """PETSc SNES NEWTONLS with the basic line search on the oracle's
     residual and Jacobian, each linear system solved directly (the
     reference's ILU(0) on a 1-D block-tridiagonal matrix, or its DIRECT
     solver, is exact): PMRTCheckUpdatePre's clip of the log-form update
-    (pm_rt.F90:734-840), then SNESConvergedDefault's tests in its order
-    (||F|| < atol, ||F|| <= rtol ||F0||, ||y|| < stol ||x||, the limit),
-    after which PFLOTRAN's ConvergenceTest (convergence.F90:170-245) marks a
-    residual of infinity norm above 1e20 diverged and takes no decision at
-    iteration 0 (newton_min_iterations 1: a zero initial residual still
-    takes one Newton iteration).  Updates cells["pri_molal"]; returns
-    (iterations, reason)."""
+    (pm_rt.F90:734-840), then the tests of one Newton iteration:
+    SNESConvergedDefault in its order (NaN, ||F|| < atol, and from the first
+    update on ||F|| <= rtol ||F0||, ||y|| < stol ||x||), PFLOTRAN's
+    PMRTCheckUpdatePost (pm_rt.F90:885-925: with ITOL_RELATIVE_UPDATE,
+    max |y / x_old| below it converges, reason 12) and ConvergenceTest
+    (convergence.F90:170-245: ||F||_inf below ITOL converges, reason 10;
+    ||y||_inf below ITOL_UPDATE, 11; ||F||_inf above max_norm diverges, -10;
+    no decision at iteration 0, newton_min_iterations 1), then the
+    iteration limit.  Updates cells["pri_molal"]; returns (iterations,
+    reason)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     N = t.N
@@ -432,6 +436,7 @@ def gi_newton(t: ReactionTables, cells: HostCells, grid: HostGrid, opts: abi.Rea
             return it, abi.PFX_SNES_DIVERGED_MAX_IT
         A = sp.bsr_matrix((v, ci, rp), shape=(n * N, n * N)).tocsc()
         y = spla.spsolve(A, r.ravel())
+        x_old = np.log(cells["pri_molal"].ravel()) if grid.log_formulation else cells["pri_molal"].ravel().copy()
         if grid.log_formulation:
             y = np.sign(y) * np.minimum(np.abs(y), max_dlnC)
             m = cells["pri_molal"].ravel() * np.exp(-y)
@@ -448,16 +453,28 @@ def gi_newton(t: ReactionTables, cells: HostCells, grid: HostGrid, opts: abi.Rea
         r, v, rp, ci = gi_residual_jacobian(t, cells, grid, opts)
         f = np.linalg.norm(r)
         x = np.log(m) if grid.log_formulation else m
+        reason = 0
         if not np.isfinite(f):
-            return it, abi.PFX_SNES_DIVERGED_FNORM_NAN
-        if np.max(np.abs(r)) > 1e20:
-            return it, abi.PFX_SNES_DIVERGED_MAX_NORM
-        if f < atol:
-            return it, abi.PFX_SNES_CONVERGED_FNORM_ABS
-        if f <= rtol * f0:
-            return it, abi.PFX_SNES_CONVERGED_FNORM_RELATIVE
-        if np.linalg.norm(y) < stol * np.linalg.norm(x):
-            return it, abi.PFX_SNES_CONVERGED_SNORM_RELATIVE
+            reason = abi.PFX_SNES_DIVERGED_FNORM_NAN
+        elif f < atol:
+            reason = abi.PFX_SNES_CONVERGED_FNORM_ABS
+        elif f <= rtol * f0:
+            reason = abi.PFX_SNES_CONVERGED_FNORM_RELATIVE
+        elif np.linalg.norm(y) < stol * np.linalg.norm(x):
+            reason = abi.PFX_SNES_CONVERGED_SNORM_RELATIVE
+        if rel_update is not None:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                if np.max(np.abs(y / x_old)) < rel_update:
+                    reason = abi.PFX_SNES_CONVERGED_RELATIVE_UPDATE
+        fi = np.max(np.abs(r)) if r.size else 0.0
+        if itol_res is not None and fi < itol_res:
+            reason = abi.PFX_SNES_CONVERGED_ITOL_RES
+        if itol_upd is not None and np.max(np.abs(y)) < itol_upd:
+            reason = abi.PFX_SNES_CONVERGED_ITOL_UPDATE
+        if fi > max_norm:
+            reason = abi.PFX_SNES_DIVERGED_MAX_NORM
+        if reason != 0:
+            return it, reason
 
 
 def gi_column_run(deck, db=None, max_cuts=16):
@@ -524,8 +541,9 @@ def gi_column_run(deck, db=None, max_cuts=16):
                     grid.bc_total[i][:] = c["total"][0]
         old = {k: cells[k].copy() for k in ("pri_molal", "ionx_ref_sorbed") if cells.arrays.get(k) is not None}
         cuts, wasted, dt_try = 0, 0, dt
+        nw = deck.newton_options()
         while True:
-            its, reason = gi_newton(t, cells, grid, opts(dt_try))
+            its, reason = gi_newton(t, cells, grid, opts(dt_try), **nw)
             if reason > 0:
                 break
             wasted += its

@@ -61,6 +61,7 @@ class ReactionDesc(C.Structure):
         ("kinmnrl_num_prefactors", _ip), ("kinmnrl_prefactor_id", _ip), ("kinmnrl_pref_alpha", _dp),
         ("kinmnrl_pref_beta", _dp), ("kinmnrl_pref_atten_coef", _dp), ("kinmnrl_pref_rate", _dp),
         ("kinmnrl_pref_activation_energy", _dp), ("kinmnrl_min_scale_factor", _dp),
+        ("group_lanes", C.c_int32), ("reserved_lanes", C.c_int32),
     ]
 
 
@@ -141,7 +142,8 @@ class KrylovStats(C.Structure):
 class NewtonOpts(C.Structure):
     """pfx_newton_opts; zeros = PETSc SNES defaults."""
     _fields_ = [("atol", C.c_double), ("rtol", C.c_double), ("stol", C.c_double),
-                ("max_iterations", C.c_int32), ("reserved", C.c_int32), ("krylov", KrylovOpts)]
+                ("max_iterations", C.c_int32), ("reserved", C.c_int32), ("krylov", KrylovOpts),
+                ("itol_res", C.c_double), ("itol_update", C.c_double), ("itol_rel_update", C.c_double)]
 
 
 class NewtonStats(C.Structure):
@@ -171,6 +173,7 @@ PFX_KSP_DIVERGED_NANORINF, PFX_KSP_DIVERGED_PC_FAILED = -9, -11
 PFX_SNES_CONVERGED_FNORM_ABS, PFX_SNES_CONVERGED_FNORM_RELATIVE, PFX_SNES_CONVERGED_SNORM_RELATIVE = 2, 3, 4
 PFX_SNES_DIVERGED_LINEAR_SOLVE, PFX_SNES_DIVERGED_FNORM_NAN, PFX_SNES_DIVERGED_MAX_IT = -3, -4, -5
 PFX_SNES_DIVERGED_MAX_NORM = -10
+PFX_SNES_CONVERGED_ITOL_RES, PFX_SNES_CONVERGED_ITOL_UPDATE, PFX_SNES_CONVERGED_RELATIVE_UPDATE = 10, 11, 12
 
 
 CELL_FIELDS = [f for f, _ in Cells._fields_ if f != "ncells"]

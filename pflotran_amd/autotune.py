@@ -2,12 +2,15 @@
 
 The knobs are the kernel variant of pfx_tables_tune (the register cap: how
 many wavefronts per SIMD the allocation must allow, trading spills for
-latency hiding) and, for zoned launches, the variant and the split of
-workgroups between zones (pfx_zone_plan).  Each candidate runs the real
-step on the real cells, with the step's inputs restored between trials, and
-the fastest device time wins.
+latency hiding) and, for zoned launches, the variant, the split of
+workgroups between zones (pfx_zone_plan) and each zone's group width (the
+lanes per cell, the LU's row tile: pfx_reaction_desc.group_lanes).  Each
+candidate runs the real step on the real cells, with the step's inputs
+restored between trials, and the fastest device time wins.
 """
 from __future__ import annotations
+
+import dataclasses
 
 from . import abi, runtime, workload
 
@@ -61,3 +64,35 @@ def tune_zones(tabs, dcs, opts: abi.ReactOpts, variants=(0, 1, 2), rounds: int =
     for z, b in enumerate(best["blocks"]):
         out.blocks[z] = b
     return out, history
+
+
+def default_lanes(N: int) -> int:
+    return 8 if N <= 8 else (16 if N <= 16 else 32)
+
+
+def tune_zone_lanes(tabs, dcs, opts: abi.ReactOpts, rounds: int = 3):
+    """The zoned launch with each zone's group width chosen by measurement:
+    the default width (the smallest of 8/16/32 holding N) against twice it
+    where that fits (<= 32), one zone at a time, each trial with the split
+    and register variant tuned by tune_zones.  Returns (the tables to use,
+    one per zone; the plan; the lanes per zone; the trial history)."""
+    plan, hist = tune_zones(tabs, dcs, opts, rounds=rounds)
+    best_ms = min(h["kernel_ms"] for h in hist)
+    lanes = [default_lanes(t.host.N) for t in tabs]
+    chosen = list(tabs)
+    history = [{"lanes": list(lanes), "kernel_ms": best_ms}]
+    for z, t in enumerate(tabs):
+        wide = 2 * lanes[z]
+        if wide > 32:
+            continue
+        alt = runtime.Tables(dataclasses.replace(t.host, group_lanes=wide))
+        trial = list(chosen)
+        trial[z] = alt
+        p2, h2 = tune_zones(trial, dcs, opts, variants=(plan.variant,), rounds=rounds)
+        ms = min(h["kernel_ms"] for h in h2)
+        trial_lanes = list(lanes)
+        trial_lanes[z] = wide
+        history.append({"lanes": trial_lanes, "kernel_ms": ms})
+        if ms < best_ms:
+            best_ms, plan, chosen, lanes = ms, p2, trial, trial_lanes
+    return chosen, plan, lanes, history

@@ -691,7 +691,12 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
     if (d->eqcplxh2ostoich && d->eqcplxh2ostoich[i] != 0.0) {
       // water activity is 1 (ln a_w = 0, use_activity_h2o = F): the term vanishes
     }
-  const int G = group_size(N);
+  int G = group_size(N);
+  if (d->group_lanes != 0) {
+    if ((d->group_lanes != 8 && d->group_lanes != 16 && d->group_lanes != 32) || d->group_lanes < N)
+      return PFX_E_ARG;
+    G = d->group_lanes;
+  }
   pfx_dev_layout L;
   memset(&L, 0, sizeof(L));
   L.N = N; L.M = M; L.K = K; L.R = R; L.G = G; L.S = S0;

@@ -433,11 +433,16 @@ __global__ __launch_bounds__(BLOCK) void k_axpy(int64_t n, double alpha, const d
 // form y <- y * scale, m <- m - y.  partial (y, y) and (x, x), x = ln m or m.
 __global__ __launch_bounds__(BLOCK) void k_newton_update(int64_t n, double *__restrict__ m, double *__restrict__ y,
                                                          int use_log, double max_dlnC, double scale,
-                                                         double *__restrict__ partial) {
+                                                         double *__restrict__ partial, double *__restrict__ rel) {
   double yy = 0.0, xx = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
     double d = y[i];
     double mi = m[i];
+    // |update / old solution entry| (PMRTCheckUpdatePost, pm_rt.F90:903-906)
+    if (rel) {
+      const double dd = use_log ? copysign(1.0, d) * fmin(fabs(d), max_dlnC) : d * scale;
+      rel[i] = fabs(dd / (use_log ? log(mi) : mi));
+    }
     if (use_log) {
       d = copysign(1.0, d) * fmin(fabs(d), max_dlnC);
       mi = mi * exp(-d);
@@ -1226,6 +1231,7 @@ struct pfx_gi_solver {
   int64_t n = 0, nnzb = 0;
   int32_t *rp = nullptr, *ci = nullptr;
   double *vals = nullptr, *res = nullptr, *y = nullptr;
+  double *rel = nullptr;   // |dx / x_old| per unknown (ITOL_RELATIVE_UPDATE)
   // the old state a cut time step restores: free ions and exchange state
   double *old_m = nullptr, *old_ionx = nullptr;
   int32_t R = 0;
@@ -1241,6 +1247,7 @@ extern "C" int pfx_gi_solver_destroy(pfx_gi_solver *s) {
   if (s->res) (void)hipFree(s->res);
   if (s->y) (void)hipFree(s->y);
   if (s->old_m) (void)hipFree(s->old_m);
+  if (s->rel) (void)hipFree(s->rel);
   if (s->old_ionx) (void)hipFree(s->old_ionx);
   delete s;
   return 0;
@@ -1270,6 +1277,7 @@ extern "C" int pfx_gi_solver_create(pfx_tables *tables, const pfx_grid *grid, pf
   ok = ok && hipMalloc((void **)&s->res, (size_t)s->n * N * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void **)&s->y, (size_t)s->n * N * sizeof(double)) == hipSuccess;
   ok = ok && hipMalloc((void **)&s->old_m, (size_t)s->n * N * sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&s->rel, (size_t)s->n * N * sizeof(double)) == hipSuccess;
   if (R > 0) ok = ok && hipMalloc((void **)&s->old_ionx, (size_t)s->n * R * sizeof(double)) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
@@ -1293,6 +1301,9 @@ extern "C" int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_
   const double rtol = (no && no->rtol > 0.0) ? no->rtol : 1.0e-8;
   const double stol = (no && no->stol > 0.0) ? no->stol : 1.0e-8;
   const int maxit = (no && no->max_iterations > 0) ? no->max_iterations : 50;
+  const double itol_res = no ? no->itol_res : 0.0;
+  const double itol_upd = no ? no->itol_update : 0.0;
+  const double itol_rel = no ? no->itol_rel_update : 0.0;
   pfx_krylov_opts ko;
   memset(&ko, 0, sizeof(ko));
   if (no) ko = no->krylov;
@@ -1357,7 +1368,7 @@ extern "C" int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_
       if (min_ratio < 1.0) scale = min_ratio * 0.99;
     }
     hipLaunchKernelGGL(k_newton_update, dim3(ge), dim3(BLOCK), 0, hs, n, cells->pri_molal, s->y, use_log, max_dlnC,
-                       scale, h->partial);
+                       scale, h->partial, itol_rel > 0.0 ? s->rel : nullptr);
     KCHK(hipGetLastError());
     double yy = 0.0, xx = 0.0;
     if ((rc = read2(h, ge, hs, &yy, &xx))) return rc;
@@ -1368,12 +1379,28 @@ extern "C" int pfx_gi_solver_step(pfx_gi_solver *s, pfx_cells *cells, const pfx_
     else if (S.fnorm < atol) S.reason = PFX_SNES_CONVERGED_FNORM_ABS;
     else if (S.fnorm <= rtol * S.fnorm0) S.reason = PFX_SNES_CONVERGED_FNORM_RELATIVE;
     else if (sqrt(yy) < stol * sqrt(xx)) S.reason = PFX_SNES_CONVERGED_SNORM_RELATIVE;
-    // then ConvergenceTest's max_norm on ||F||_inf (<= ||F||_2, so only a
+    // PMRTCheckUpdatePost's relative update, then ConvergenceTest's
+    // infinity norms and max_norm (||F||_inf <= ||F||_2: without ITOL only a
     // 2-norm above the bound needs the infinity norm)
-    if (std::isfinite(S.fnorm) && S.fnorm > kMaxNorm) {
+    if (itol_rel > 0.0) {
+      double rmax = 0.0;
+      if ((rc = max_abs(h, ge, hs, n, s->rel, &rmax))) return rc;
+      if (rmax < itol_rel) S.reason = PFX_SNES_CONVERGED_RELATIVE_UPDATE;
+    }
+    if (itol_res > 0.0 || (std::isfinite(S.fnorm) && S.fnorm > kMaxNorm)) {
       double finf = 0.0;
       if ((rc = max_abs(h, ge, hs, n, s->res, &finf))) return rc;
+      if (itol_res > 0.0 && finf < itol_res) S.reason = PFX_SNES_CONVERGED_ITOL_RES;
+      if (itol_upd > 0.0) {
+        double uinf = 0.0;
+        if ((rc = max_abs(h, ge, hs, n, s->y, &uinf))) return rc;
+        if (uinf < itol_upd) S.reason = PFX_SNES_CONVERGED_ITOL_UPDATE;
+      }
       if (finf > kMaxNorm) S.reason = PFX_SNES_DIVERGED_MAX_NORM;
+    } else if (itol_upd > 0.0) {
+      double uinf = 0.0;
+      if ((rc = max_abs(h, ge, hs, n, s->y, &uinf))) return rc;
+      if (uinf < itol_upd) S.reason = PFX_SNES_CONVERGED_ITOL_UPDATE;
     }
   }
   S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();

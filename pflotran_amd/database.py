@@ -2,9 +2,9 @@
 
 Host-side setup that a PFLOTRAN Fortran host would do in its own
 ``DatabaseRead``/``BasisInit`` (src/pflotran/reaction_database.F90:25, 772).
-This module reads PFLOTRAN's ASCII thermodynamic database format for decks
-whose primary species are the database basis (no basis swap), and builds the
-numpy arrays of ``pfx_reaction_desc`` (include/pfx_chem.h).
+This module reads PFLOTRAN's ASCII thermodynamic database format, writes
+every reaction a deck uses in the deck's primary species (Basis), and builds
+the numpy arrays of ``pfx_reaction_desc`` (include/pfx_chem.h).
 """
 from __future__ import annotations
 
@@ -102,6 +102,112 @@ def read_database(path: str) -> Database:
             # entries this build never selects (e.g. malformed test species)
             continue
     return db
+
+
+# --------------------------------------------------------------------------
+# basis: every reaction written in the deck's primary species
+
+
+def _flatten(db: Database, name: str, memo: Dict, stack=(), keep=None):
+    """(composition over the database's primary species, H2O coefficient,
+    log K per database temperature, NaN where missing) of a species: a
+    database primary is itself; a complex, gas or mineral is its reaction
+    with every non-primary species in it replaced by its own reaction
+    (recursively), so log a_X = sum_k c_k log a_k + h log a_w - log K.  A
+    database primary outside `keep` (the chosen primaries) that is also
+    defined by a gas or complex reaction of the same name (O2(g) in
+    hanford.dat, a primary and a gas O2(g) = O2(aq)) is replaced by that
+    reaction."""
+    if name in memo:
+        return memo[name]
+    nt = len(db.temperatures)
+    has_rxn = name in db.gases or name in db.complexes
+    if name in db.primary and (keep is None or name in keep or not has_rxn or name in stack):
+        out = ({name: 1.0}, 0.0, np.zeros(nt))
+    else:
+        r = db.gases.get(name) or db.complexes.get(name) or db.minerals.get(name)
+        if r is None:
+            raise ValueError(f"species {name} is not in the database")
+        if name in stack:
+            raise ValueError(f"species {name}: circular reaction")
+        comp: Dict[str, float] = {}
+        h2o = 0.0
+        lk = np.array([np.nan if abs(v - 500.0) < 1e-10 else v for v in r.logK], dtype=np.float64)
+        for st, sp in r.species:
+            if sp in H2O_NAMES:
+                h2o += st
+                continue
+            c, h, l = _flatten(db, sp, memo, stack + (name,), keep)
+            for k, v in c.items():
+                comp[k] = comp.get(k, 0.0) + st * v
+            h2o += st * h
+            lk = lk + st * l
+        out = (comp, h2o, lk)
+    memo[name] = out
+    return out
+
+
+class Basis:
+    """The deck's primary species as the basis of every reaction, the role
+    of BasisInit's basis handling (reaction_database.F90:1246-1380).  A
+    reaction whose species are all primaries (or H2O) is taken as written.
+    Otherwise each species is reduced to the database's own primaries
+    (_flatten), then written in the chosen primaries by solving
+    sum_p x_p comp(p) = comp(X); its log K becomes logK(X) - sum_p x_p
+    logK(p) (0 for a database primary), per temperature, missing (500) where
+    a term is.  A primary that is a database complex takes its charge, ion
+    size and molar mass from there."""
+
+    def __init__(self, db: Database, primaries: Sequence[str]):
+        self.db = db
+        self.prim = list(primaries)
+        self.pset = set(self.prim)
+        self._memo: Dict = {}
+        self._pc = None
+        for p in self.prim:
+            if p not in db.primary and p not in db.complexes:
+                raise ValueError(f"primary species {p} is not in the database")
+
+    def primary(self, p: str) -> DbSpecies:
+        if p in self.db.primary:
+            return self.db.primary[p]
+        r = self.db.complexes[p]
+        return DbSpecies(p, r.a0, r.Z, r.mw)
+
+    def reaction(self, name: str, section: Dict[str, DbReaction]) -> DbReaction:
+        r = section.get(name)
+        if r is None and name in self.db.primary:
+            # a database primary made secondary by the deck's choice of basis
+            sp = self.db.primary[name]
+            r = DbReaction(name, [(1.0, name)], [0.0] * len(self.db.temperatures), sp.a0, sp.Z, sp.mw)
+        elif r is None:
+            raise ValueError(f"species {name} is not in the database")
+        if all(sp in self.pset or sp in H2O_NAMES for _, sp in r.species):
+            return r
+        if self._pc is None:
+            self._pc = [_flatten(self.db, p, self._memo, keep=self.pset) for p in self.prim]
+            self._basis = sorted({k for c, _, _ in self._pc for k in c})
+            self._A = np.array([[c.get(k, 0.0) for k in self._basis] for c, _, _ in self._pc], dtype=np.float64)
+        c, h, lk = _flatten(self.db, name, self._memo, keep=self.pset)
+        missing = [k for k, v in c.items() if abs(v) > 1e-12 and k not in self._basis]
+        if missing:
+            raise ValueError(f"{name}: species {missing} cannot be written in the primary species")
+        cv = np.array([c.get(k, 0.0) for k in self._basis], dtype=np.float64)
+        x, *_ = np.linalg.lstsq(self._A.T, cv, rcond=None)
+        if np.max(np.abs(self._A.T @ x - cv), initial=0.0) > 1e-9:
+            raise ValueError(f"{name}: reaction cannot be written in the primary species")
+        x = np.where(np.abs(x - np.round(x)) < 1e-12, np.round(x), x)
+        lk2, h2 = lk.copy(), h
+        for xp, (_, hp, lkp) in zip(x, self._pc):
+            if xp != 0.0:
+                lk2 = lk2 - xp * lkp
+                h2 = h2 - xp * hp
+        species = [(float(xp), p) for xp, p in zip(x, self.prim) if xp != 0.0]
+        if abs(h2) > 1e-12:
+            species.append((float(h2), "H2O"))
+        import dataclasses
+        return dataclasses.replace(r, species=species,
+                                   logK=[500.0 if not np.isfinite(v) else float(v) for v in lk2])
 
 
 # --------------------------------------------------------------------------
@@ -389,6 +495,9 @@ class ReactionTables:
     # MIN_SCALE_FACTOR: allocated for every kinetic mineral (1 for those
     # that do not set it) when any does (reaction_database.F90:1973-1983)
     mnrl_min_scale_factor: Optional[np.ndarray] = None
+    # lanes per cell group in the per-cell kernels (0: the smallest that
+    # holds N); a tuning knob of this library
+    group_lanes: int = 0
     _desc: Optional[abi.ReactionDesc] = None
 
     @property
@@ -473,6 +582,7 @@ class ReactionTables:
             d.kinmnrl_pref_atten_coef, d.kinmnrl_pref_rate = dp(self.mnrl_pref_atten), dp(self.mnrl_pref_rate)
             d.kinmnrl_pref_activation_energy = dp(self.mnrl_pref_activation_energy)
         d.kinmnrl_min_scale_factor = dp(self.mnrl_min_scale_factor)
+        d.group_lanes = int(self.group_lanes)
         self._desc = d
         return d
 
@@ -501,7 +611,7 @@ def constraint_rxn(db: Database, name: str, primaries: Sequence[str], kind: str 
     """abi.ConstraintRxn of a mineral or gas constraint (the species of the
     database reaction, water dropped; log K at the reference temperature as
     build_tables takes it, plus the Maier-Kelly coefficients)."""
-    r = (db.minerals if kind == "mineral" else db.gases)[name]
+    r = Basis(db, primaries).reaction(name, db.minerals if kind == "mineral" else db.gases)
     pidx = {n: i for i, n in enumerate(primaries)}
     x = abi.ConstraintRxn()
     k = 0
@@ -509,7 +619,7 @@ def constraint_rxn(db: Database, name: str, primaries: Sequence[str], kind: str 
         if sp in H2O_NAMES:
             continue
         if sp not in pidx:
-            raise ValueError(f"{name}: species {sp} is not a primary (basis swap unsupported)")
+            raise ValueError(f"{name}: species {sp} is not a primary")
         x.specid[k] = pidx[sp]
         x.stoich[k] = st
         k += 1
@@ -543,8 +653,8 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
                  log_formulation: bool = True, act_freq: int = abi.PFX_ACT_OFF,
                  act_alg: int = abi.PFX_ACT_ALG_LAG, use_bdot: bool = True,
                  fit_logk: bool = False) -> ReactionTables:
-    """Flatten a deck's CHEMISTRY block against a database (no basis swap:
-    every complex/mineral species must be a primary or H2O).  log K at the
+    """Flatten a deck's CHEMISTRY block against a database, every reaction
+    written in the deck's primaries (Basis).  log K at the
     reference temperature is the tabulated value when the temperature is a
     database temperature (ReactionInitializeLogK, reaction_aux.F90:1438-1450),
     otherwise the Maier-Kelly fit.  `isotherms` are the ISOTHERM_REACTIONS
@@ -553,12 +663,13 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
     `soil_particle_density` is the ROCK_DENSITY they scale by."""
     minerals = dict(minerals or {})
     pidx = {n: i for i, n in enumerate(primaries)}
+    basis = Basis(db, primaries)
     N, M, K, R = len(primaries), len(secondaries), len(minerals), len(ion_exchange)
     S = 1
     for n in secondaries:
-        S = max(S, sum(1 for _, sp in db.complexes[n].species if sp not in H2O_NAMES))
+        S = max(S, sum(1 for _, sp in basis.reaction(n, db.complexes).species if sp not in H2O_NAMES))
     for n in minerals:
-        S = max(S, sum(1 for _, sp in db.minerals[n].species if sp not in H2O_NAMES))
+        S = max(S, sum(1 for _, sp in basis.reaction(n, db.minerals).species if sp not in H2O_NAMES))
     temps = db.temperatures
     itemp = next((i for i, t in enumerate(temps) if abs(t - temperature) < 1e-10), None)
 
@@ -567,8 +678,8 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
             return lk[itemp]
         return float(logk_eval(logk_fit(temps, lk), temperature))
 
-    Z = np.array([db.primary[n].Z for n in primaries], dtype=np.float64)
-    a0 = np.array([db.primary[n].a0 for n in primaries], dtype=np.float64)
+    Z = np.array([basis.primary(n).Z for n in primaries], dtype=np.float64)
+    a0 = np.array([basis.primary(n).a0 for n in primaries], dtype=np.float64)
 
     def rxn_arrays(names, src):
         spec = np.zeros((len(names), S + 1), dtype=np.int32)
@@ -577,14 +688,14 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
         logK = np.zeros(len(names), dtype=np.float64)
         coef = np.zeros((len(names), 5), dtype=np.float64)
         for i, n in enumerate(names):
-            r = src[n]
+            r = basis.reaction(n, src)
             k = 0
             for st, sp in r.species:
                 if sp in H2O_NAMES:
                     h2o[i] = st
                     continue
                 if sp not in pidx:
-                    raise ValueError(f"{n}: species {sp} is not a primary (basis swap unsupported)")
+                    raise ValueError(f"{n}: species {sp} is not a primary")
                 spec[i, 1 + k] = pidx[sp]
                 nu[i, k] = st
                 k += 1
@@ -622,8 +733,8 @@ def build_tables(db: Database, primaries: Sequence[str], secondaries: Sequence[s
     return ReactionTables(
         primary_names=list(primaries), complex_names=list(secondaries), mineral_names=mnames,
         Z=Z, a0=a0, cplx_specid=cs, cplx_stoich=cn, cplx_h2ostoich=ch,
-        cplx_Z=np.array([db.complexes[n].Z for n in secondaries], dtype=np.float64),
-        cplx_a0=np.array([db.complexes[n].a0 for n in secondaries], dtype=np.float64),
+        cplx_Z=np.array([basis.reaction(n, db.complexes).Z for n in secondaries], dtype=np.float64),
+        cplx_a0=np.array([basis.reaction(n, db.complexes).a0 for n in secondaries], dtype=np.float64),
         cplx_logK=cl, cplx_logKcoef=cc if fit_logk else None,
         mnrl_specid=ms, mnrl_stoich=mn, mnrl_h2ostoich=mh, mnrl_logK=ml,
         mnrl_logKcoef=mc if fit_logk else None,

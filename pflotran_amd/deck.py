@@ -308,8 +308,16 @@ class Deck:
     transport: TransportCard = field(default_factory=TransportCard)
     reference_density: Optional[float] = None     # kg/m^3, REFERENCE_DENSITY
     rock_density: Dict[str, float] = field(default_factory=dict)   # material -> kg/m^3
+    # transport NEWTON_SOLVER tolerances and ITOL_RELATIVE_UPDATE
+    newton: Dict[str, float] = field(default_factory=dict)
 
     # ---------------------------------------------------------------- builds
+
+    def newton_options(self) -> Dict[str, float]:
+        """Keyword arguments of the global-implicit Newton (PETSc's defaults
+        rtol 1e-8, stol 1e-8, atol 1e-50, 50 iterations where the deck sets
+        none; ITOL, ITOL_UPDATE and ITOL_RELATIVE_UPDATE only where set)."""
+        return dict(self.newton)
 
     def tracer_only(self) -> bool:
         """No complexes, minerals, sorption or gases: the reference does not
@@ -987,7 +995,7 @@ def _output(cur: _Cursor, tr: TransportCard, depth: int = 0) -> None:
 def parse_deck(text: str, path: Optional[str] = None) -> Deck:
     cur = _Cursor(_lines(text, os.path.dirname(os.path.abspath(path)) if path else None))
     chem, constraints, porosity = None, {}, {}
-    tc, temperature, numjac, stepper = TimeCard(), 25.0, False, {}
+    tc, temperature, numjac, stepper, newton = TimeCard(), 25.0, False, {}, {}
     tr = TransportCard()
     ref_den, rock = None, {}
     while cur.i < len(cur.lines):
@@ -1027,10 +1035,29 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
                 v = s[0].upper()
                 if v == "NUMERICAL_JACOBIAN":
                     numjac = True
+                elif v == "ITOL_RELATIVE_UPDATE":
+                    newton["rel_update"] = _num(s[1])
+                elif v == "ITOL_SCALED_RESIDUAL":
+                    raise DeckError("SUBSURFACE_TRANSPORT ITOL_SCALED_RESIDUAL is not taken by the time loop")
                 elif v in ("OPERATOR_SPLIT", "OPERATOR_SPLITTING"):
                     raise DeckError("operator-split transport is not reimplemented in the reference either")
         elif w == "TIME":
             tc = _time(cur)
+        elif w == "NEWTON_SOLVER" and (len(t) < 2 or t[1].upper() == "TRANSPORT"):
+            # solver.F90:841-900 (the SNES tolerances and PFLOTRAN's
+            # infinity-norm tests of the transport Newton solve)
+            keys = {"ATOL": "atol", "RTOL": "rtol", "STOL": "stol", "MAXIT": "max_it", "MAX_NORM": "max_norm",
+                    "ITOL": "itol_res", "INF_TOL": "itol_res", "ITOL_RES": "itol_res", "INF_TOL_RES": "itol_res",
+                    "ITOL_UPDATE": "itol_upd", "INF_TOL_UPDATE": "itol_upd"}
+            for s in cur.block("NEWTON_SOLVER"):
+                v = s[0].upper()
+                if v in keys:
+                    newton[keys[v]] = int(_num(s[1])) if v == "MAXIT" else _num(s[1])
+                elif v == "DTOL":
+                    pass   # SNES divergence of ||F|| beyond dtol ||F0||, not met on these decks
+                elif v not in ("PRINT_DETAILED_CONVERGENCE", "PRINT_LINEAR_ITERATIONS", "NO_INFINITY_NORM",
+                               "NO_PRINT_CONVERGENCE"):
+                    raise DeckError(f"NEWTON_SOLVER: {v} is not taken by the time loop")
         elif w == "TIMESTEPPER" and (len(t) < 2 or t[1].upper() == "TRANSPORT"):
             for s in cur.block("TIMESTEPPER"):
                 v = s[0].upper()
@@ -1078,7 +1105,7 @@ def parse_deck(text: str, path: Optional[str] = None) -> Deck:
         raise DeckError("deck has no CHEMISTRY card")
     for k, v in stepper.items():
         setattr(tc, k, v)
-    return Deck(path, chem, constraints, porosity, tc, temperature, numjac, tr, ref_den, rock)
+    return Deck(path, chem, constraints, porosity, tc, temperature, numjac, tr, ref_den, rock, newton)
 
 
 def read_deck(path: str) -> Deck:

@@ -57,6 +57,7 @@ module pfx_chem_module
     type(c_ptr) :: kinmnrl_pref_alpha, kinmnrl_pref_beta, kinmnrl_pref_atten_coef
     type(c_ptr) :: kinmnrl_pref_rate, kinmnrl_pref_activation_energy
     type(c_ptr) :: kinmnrl_min_scale_factor
+    integer(c_int32_t) :: group_lanes, reserved_lanes
   end type pfx_reaction_desc
 
   type, bind(c), public :: pfx_cells
@@ -126,6 +127,7 @@ module pfx_chem_module
     real(c_double) :: atol, rtol, stol
     integer(c_int32_t) :: max_iterations, reserved
     type(pfx_krylov_opts) :: krylov
+    real(c_double) :: itol_res, itol_update, itol_rel_update
   end type pfx_newton_opts
 
   type, bind(c), public :: pfx_newton_stats

@@ -132,7 +132,7 @@ class ColumnRun:
         self.grid = runtime.DeviceGrid(n, 1, 1, qf, ddf, area=(box.area, 1.0, 1.0), bc_q=bc_q, bc_dd=bc_dd,
                                        bc_total=bc_total, log_formulation=bool(t.use_log_formulation))
         self.solver = runtime.GiSolver(self.tab, self.grid)
-        self.newton = newton if newton is not None else abi.NewtonOpts()
+        self.newton = newton if newton is not None else self.deck_newton_opts(deck)
         if self.newton.krylov.rtol <= 0.0:
             self.newton.krylov.rtol = linear_rtol
         if self.newton.krylov.max_iterations <= 0:
@@ -141,6 +141,21 @@ class ColumnRun:
         # fixed accumulation keeps the old state's totals)
         self._scratch = {k: torch.empty_like(self.dc.t[k]) for k in ("total", "total_sorb", "ionx_ref_sorbed")
                          if self.dc.t.get(k) is not None}
+
+    @staticmethod
+    def deck_newton_opts(deck: Deck) -> abi.NewtonOpts:
+        """pfx_newton_opts of the deck's transport NEWTON_SOLVER card and
+        ITOL_RELATIVE_UPDATE (zeros: PETSc's defaults, PFLOTRAN's tests off)."""
+        nw = deck.newton_options()
+        o = abi.NewtonOpts()
+        o.atol, o.rtol, o.stol = nw.get("atol", 0.0), nw.get("rtol", 0.0), nw.get("stol", 0.0)
+        o.max_iterations = int(nw.get("max_it", 0))
+        o.itol_res = nw.get("itol_res", 0.0)
+        o.itol_update = nw.get("itol_upd", 0.0)
+        o.itol_rel_update = nw.get("rel_update", 0.0)
+        if "max_norm" in nw:
+            raise ValueError("NEWTON_SOLVER MAX_NORM: the device step keeps the reference's default 1e20")
+        return o
 
     def opts(self, dt: float) -> abi.ReactOpts:
         b = self.box
@@ -156,6 +171,7 @@ class ColumnRun:
             arr["mnrl_volfrac"][0, :] = setup["vf"]
             arr["mnrl_area"][0, :] = setup["area"]
         o = self.opts(1.0)
+        o.max_iterations = 10000   # the constraint Newton's own cap (reaction.F90:1941)
         if setup.get("guess") is not None:
             arr["guess"][0, :] = setup["guess"]
             o.guess_mode = abi.PFX_GUESS_ARRAY

@@ -33,17 +33,41 @@ COLUMNS = {"1d-calcite": os.path.join("ascem", "1d", "1d-calcite"),
            # isotherm sorption: the linear K_D column and its 1-cell decks
            "solute_KD": os.path.join("default", "column"),
            "solute_KD_w_mineral": os.path.join("default", "batch"),
-           "solute_KD_wo_mineral": os.path.join("default", "batch")}
+           "solute_KD_wo_mineral": os.path.join("default", "batch"),
+           # mineral rate prefactors, a uranium basis with U++++ and O2(g)
+           # reached through other reactions
+           "mineral_prefactor": os.path.join("default", "column")}
 for name, sub in COLUMNS.items():
     shutil.copy(os.path.join(REF, "regression_tests", sub, f"{name}.in"), os.path.join(HERE, "decks", f"{name}.in"))
 
 # excerpt of database/hanford.dat: the species of the 1-D calcite columns
 # (5 primaries, 8 complexes, calcite; CaOH+ and the CO2(g) constraint of the
-# calcite transport decks), of the Valocchi ion-exchange test and of the
-# K_D decks (A(aq), A(s))
+# calcite transport decks), of the Valocchi ion-exchange test, of the K_D
+# decks (A(aq), A(s)) and of the mineral-prefactor column (its CHEMISTRY
+# species lists, read from the deck)
 KEEP = {"H2O", "H+", "HCO3-", "Ca++", "Mg++", "Cl-", "Na+", "CO2(g)", "CaOH+",
         "OH-", "CO2(aq)", "CO3--", "CaHCO3+", "CaCO3(aq)", "MgHCO3+", "MgCO3(aq)", "CaCl+",
         "Calcite", "Halite", "A(aq)", "A(s)"}
+
+
+def chemistry_names(deck):
+    """Species named in a deck's PRIMARY_SPECIES, SECONDARY_SPECIES,
+    GAS_SPECIES and MINERALS lists."""
+    names, block = set(), None
+    for ln in open(deck):
+        w = ln.split("#")[0].split()
+        if not w:
+            continue
+        if w[0] in ("PRIMARY_SPECIES", "SECONDARY_SPECIES", "GAS_SPECIES", "MINERALS") and block is None:
+            block = w[0]
+        elif block and w[0] in ("/", "END"):
+            block = None
+        elif block:
+            names.add(w[0])
+    return names
+
+
+KEEP |= chemistry_names(os.path.join(REF, "regression_tests", "default", "column", "mineral_prefactor.in"))
 out = []
 with open(os.path.join(REF, "database", "hanford.dat")) as f:
     for i, ln in enumerate(f):

@@ -130,3 +130,29 @@ def test_kernel_variants_give_identical_bits(zones):
                     assert np.array_equal(o[k], outs[0][k]), k
     with pytest.raises(runtime.PfxError):
         tab.tune(abi.PFX_KERNEL_EVAL, 1)
+
+
+def test_wider_cell_groups_give_the_same_step(zones):
+    # the group width (lanes per cell, the LU's row tile) is a tuning knob:
+    # padding lanes carry identity rows and add zeros to the group sums, so
+    # a network run 16 or 32 lanes wide takes the same Newton path
+    import dataclasses
+    nets, cells = zones
+    for z in (0, 1):
+        t = nets[z]
+        base = runtime.Tables(t)
+        dc = device(cells[z], cells[z].ncells)
+        runtime.react_batch(base, dc, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
+        ref = outputs(dc)
+        wide = 2 * (8 if t.N <= 8 else 16)
+        tw = runtime.Tables(dataclasses.replace(t, group_lanes=wide))
+        dw = device(cells[z], cells[z].ncells)
+        runtime.react_batch(tw, dw, synth.react_opts(guess_mode=abi.PFX_GUESS_PREVIOUS))
+        out = outputs(dw)
+        assert np.array_equal(out["status"], ref["status"])
+        ok = ref["status"] == 0
+        rel = np.abs(out["pri_molal"][ok] - ref["pri_molal"][ok]) / ref["pri_molal"][ok]
+        assert rel.max() <= 1e-12, (z, wide, rel.max())
+    # a width below N, or not 8/16/32, is refused
+    with pytest.raises(runtime.PfxError):
+        runtime.Tables(dataclasses.replace(nets[1], group_lanes=8))
