@@ -120,6 +120,9 @@ def main():
         def step():
             restore()
             box["rc"], box["st"] = runtime.react_batch(tab, dc, opts)
+        # the trials start from the step's inputs (not from the state a
+        # previous leg left)
+        restore()
         tune = autotune.tune_kernel(tab, dc, lambda: runtime.react_batch(tab, dc, opts))
         log(f"{key} kernel variants (ms): {tune}")
         for _ in range(args.warmup):
@@ -188,6 +191,7 @@ def main():
             box["rc"], box["st"] = runtime.react_batch(tab, dc, warm_opts)
             mlp_ms.append(ev[0].elapsed_time(ev[1]))
 
+        restore()
         surrogate.guess(tab, mlp, dc)
         warm_tune = autotune.tune_kernel(tab, dc, lambda: runtime.react_batch(tab, dc, warm_opts))
         log(f"warm kernel variants (ms): {warm_tune}")

@@ -9,8 +9,6 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpfx_chem.so")
-# profiling build: the same library with the Newton kernel's section timers
-LIB_PROF = os.path.join(HERE, "libpfx_chem_prof.so")
 SOURCES = ["pfx_react.hip", "pfx_api.hip", "pfx_mlp.hip", "pfx_gi.hip", "pfx_krylov.hip"]
 ARCH = os.environ.get("PFX_OFFLOAD_ARCH", "gfx950")
 
@@ -33,19 +31,6 @@ def build(verbose=True):
     return LIB
 
 
-def build_profiling(verbose=True):
-    """libpfx_chem_prof.so: per-section cycle counters in the Newton kernel
-    (pflotran_amd/section_profile.py reads them)."""
-    cmd = hipcc_cmd(LIB_PROF + ".tmp", defines=("PFX_SECTION_TIMERS",))
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(LIB_PROF + ".tmp", LIB_PROF)
-    return LIB_PROF
-
-
 if __name__ == "__main__":
     build()
-    if "--profiling" in sys.argv:
-        build_profiling()
     sys.exit(0)

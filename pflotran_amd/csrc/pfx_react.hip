@@ -30,7 +30,7 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-// Profiling build (-DPFX_SECTION_TIMERS, pflotran_amd/build.py build_profiling):
+// Profiling build (this file compiled with -DPFX_SECTION_TIMERS):
 // shader-clock cycles per section of the Newton pass, summed per wavefront
 // and added into a device array when the wavefront exits; the product build
 // compiles every marker to nothing.  Sections: 0 refill/load, 1 activity,
