@@ -14,12 +14,12 @@
 
 #include "pfx_device.h"
 
-template <int G, int MODE, int W, int LUP>
+template <int G, int MODE, int W, int LUP, bool XM>
 __global__ void pfx_react_kernel(const pfx_kernel_params P);
-template <int W>
+template <int W, bool XM>
 __global__ void pfx_react_zones_kernel(const pfx_kernel_params *zone_params, const pfx_zone_launch Z,
                                        uint64_t *finish);
-extern "C" int pfx_scr_doubles(int M, int G, int K);
+extern "C" int pfx_scr_doubles(int M, int G, int K, int xm);
 
 struct pfx_tables {
   pfx_dev_layout L;
@@ -209,18 +209,28 @@ int ensure_scratch(pfx_tables *t, int64_t n) {
   return 0;
 }
 
-template <int MODE, int W, int LUP>
+template <int MODE, int W, int LUP, bool XM = false>
 const void *kernel_for(int G) {
-  return G == 8 ? (const void *)pfx_react_kernel<8, MODE, W, LUP>
-       : G == 16 ? (const void *)pfx_react_kernel<16, MODE, W, LUP>
-                 : (const void *)pfx_react_kernel<32, MODE, W, LUP>;
+  return G == 8 ? (const void *)pfx_react_kernel<8, MODE, W, LUP, XM>
+       : G == 16 ? (const void *)pfx_react_kernel<16, MODE, W, LUP, XM>
+                 : (const void *)pfx_react_kernel<32, MODE, W, LUP, XM>;
 }
+
+// tables whose minerals have rate prefactors or scale factors run the
+// kernels compiled with them (uncapped)
+bool tables_xm(const pfx_tables *t) { return (t->L.mnrl_flags & 48) != 0; }
 
 // variant v caps the registers at v + 1 wavefronts per SIMD (v = 0: none);
 // only the Newton modes have the capped variants.  lup: the reaction
 // kernel's factorisation policy (PFX_LU_*; the pivoting one has no capped
-// variants, the other modes always pivot)
-const void *kernel_ptr(int G, int mode, int v, int lup = PFX_LU_DEFER) {
+// variants, the other modes always pivot).  xm: the prefactor kernels
+// (reaction and evaluation modes only; no capped variants).
+const void *kernel_ptr(int G, int mode, int v, int lup = PFX_LU_DEFER, bool xm = false) {
+  if (xm && mode == PFX_MODE_REACT)
+    return lup == PFX_LU_PIVOT      ? kernel_for<PFX_MODE_REACT, 1, PFX_LU_PIVOT, true>(G)
+         : lup == PFX_LU_ONDEMAND ? kernel_for<PFX_MODE_REACT, 1, PFX_LU_ONDEMAND, true>(G)
+                                  : kernel_for<PFX_MODE_REACT, 1, PFX_LU_DEFER, true>(G);
+  if (xm && mode == PFX_MODE_EVAL) return kernel_for<PFX_MODE_EVAL, 1, PFX_LU_PIVOT, true>(G);
   switch (mode) {
     case PFX_MODE_REACT:
       if (lup == PFX_LU_PIVOT) return kernel_for<PFX_MODE_REACT, 1, PFX_LU_PIVOT>(G);
@@ -240,9 +250,10 @@ const void *kernel_ptr(int G, int mode, int v, int lup = PFX_LU_DEFER) {
   }
 }
 
-const void *zones_kernel_ptr(int v) {
-  return v == 2 ? (const void *)pfx_react_zones_kernel<3>
-       : v == 1 ? (const void *)pfx_react_zones_kernel<2> : (const void *)pfx_react_zones_kernel<1>;
+const void *zones_kernel_ptr(int v, bool xm) {
+  if (xm) return (const void *)pfx_react_zones_kernel<1, true>;
+  return v == 2 ? (const void *)pfx_react_zones_kernel<3, false>
+       : v == 1 ? (const void *)pfx_react_zones_kernel<2, false> : (const void *)pfx_react_zones_kernel<1, false>;
 }
 
 // queue: the work-queue cursors and fallback counter of this launch (the
@@ -256,13 +267,13 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   if (!queue) queue = t->d_queue;
   const int G = t->L.G;
   const int groups = BLOCK / G;
-  const int64_t lds = t->L.lds_table_bytes + (int64_t)groups * pfx_scr_doubles(t->L.M, G, t->L.K) * 8;
+  const int64_t lds = t->L.lds_table_bytes + (int64_t)groups * pfx_scr_doubles(t->L.M, G, t->L.K, tables_xm(t)) * 8;
   if (lds > kMaxLds) return PFX_E_UNSUP;
   int occ = 0;
   const bool deferring = mode == PFX_MODE_REACT && P.defer != nullptr;
   const int lup = mode != PFX_MODE_REACT ? PFX_LU_PIVOT : (deferring ? PFX_LU_DEFER : PFX_LU_ONDEMAND);
-  const void *fn = kernel_ptr(G, mode, t->variant[mode & 3], lup);
-  const void *fn_piv = deferring ? kernel_ptr(G, mode, 0, PFX_LU_PIVOT) : nullptr;
+  const void *fn = kernel_ptr(G, mode, t->variant[mode & 3], lup, tables_xm(t));
+  const void *fn_piv = deferring ? kernel_ptr(G, mode, 0, PFX_LU_PIVOT, tables_xm(t)) : nullptr;
   if (lds > 64 * 1024) {
     HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (fn_piv) HIPCHK(hipFuncSetAttribute(fn_piv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -440,7 +451,7 @@ int ensure_side(pfx_tables *t) {
 
 int zone_lds(const pfx_tables *t) {
   const int G = t->L.G;
-  return (int)(t->L.lds_table_bytes + (int64_t)(BLOCK / G) * pfx_scr_doubles(t->L.M, G, t->L.K) * 8);
+  return (int)(t->L.lds_table_bytes + (int64_t)(BLOCK / G) * pfx_scr_doubles(t->L.M, G, t->L.K, tables_xm(t)) * 8);
 }
 
 int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, const pfx_react_opts *o,
@@ -482,7 +493,9 @@ int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, c
   }
   if (zs.lds > kMaxLds) return PFX_E_UNSUP;
   zs.variant = plan && plan->variant >= 0 && plan->variant <= 2 ? plan->variant : 0;
-  zs.fn = zones_kernel_ptr(zs.variant);
+  bool xm = false;
+  for (int z = 0; z < nzones; z++) xm = xm || tables_xm(tables[z]);
+  zs.fn = zones_kernel_ptr(zs.variant, xm);
   if (zs.lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(zs.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)zs.lds));
   int occ = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, zs.fn, BLOCK, (size_t)zs.lds));
@@ -533,7 +546,7 @@ int zones_enqueue(pfx_tables *const *tables, zone_setup &zs, const pfx_kernel_pa
     P2.queue = t->d_queue + 2;
     P2.chunk = 1;
     const int64_t lds = zone_lds(t);
-    const void *fn_piv = kernel_ptr(t->L.G, PFX_MODE_REACT, 0, PFX_LU_PIVOT);
+    const void *fn_piv = kernel_ptr(t->L.G, PFX_MODE_REACT, 0, PFX_LU_PIVOT, tables_xm(t));
     if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn_piv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int64_t need = (zs.n[z] + (BLOCK / t->L.G) - 1) / (BLOCK / t->L.G);
     const int64_t blocks2 = std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)t->num_cu));
@@ -867,14 +880,17 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   L.o_mlim = B.add(mlim);
   L.o_mpow = B.add(mpow);
   L.o_mtemkin = B.add(mtk);
-  L.o_mmsf = B.add(mmsf);
-  L.o_mpn = B.add(mpn);
-  L.o_mpid = B.add(mpid);
-  L.o_mpalpha = B.add(mpalpha);
-  L.o_mpbeta = B.add(mpbeta);
-  L.o_mpatten = B.add(mpatten);
-  L.o_mprate = B.add(mprate);
-  L.o_mpEa = B.add(mpEa);
+  // staged only where used (the LDS copy of the blob bounds the occupancy)
+  if (d->kinmnrl_min_scale_factor) L.o_mmsf = B.add(mmsf);
+  if (any_pref) {
+    L.o_mpn = B.add(mpn);
+    L.o_mpid = B.add(mpid);
+    L.o_mpalpha = B.add(mpalpha);
+    L.o_mpbeta = B.add(mpbeta);
+    L.o_mpatten = B.add(mpatten);
+    L.o_mprate = B.add(mprate);
+    L.o_mpEa = B.add(mpEa);
+  }
   // ion exchange
   std::vector<double> ionx_k((size_t)std::max(R, 1) * G, 0.0);
   std::vector<int32_t> ionx_role((size_t)std::max(R, 1) * G, -1);
@@ -958,7 +974,7 @@ extern "C" int pfx_tables_create(const pfx_reaction_desc *d, pfx_tables **out) {
   L.max_dlnC = d->max_dlnC;
   L.max_rel_change_tol = d->max_relative_change_tolerance;
   L.max_residual_tol = d->max_residual_tolerance;
-  const int64_t lds = L.lds_table_bytes + (int64_t)(BLOCK / G) * pfx_scr_doubles(M, G, K) * 8;
+  const int64_t lds = L.lds_table_bytes + (int64_t)(BLOCK / G) * pfx_scr_doubles(M, G, K, (L.mnrl_flags & 48) != 0) * 8;
   if (lds > kMaxLds) return PFX_E_UNSUP;
 
   pfx_tables *t = new pfx_tables();

@@ -229,8 +229,8 @@ __device__ __forceinline__ double logk_5term_ptr(const double *cf, double tk, do
 //   activity coefficients, lnK[M] = -log10 K * LOG_TO_LN of this cell,
 //   la[G] ln activities, mv[G] free ions, im[G] inverse free ions,
 //   xb[G] ion-exchange fractions, mlnK/mkeff/marea/mvf/mrate[K] per mineral,
-//   cs[6] the cell's porosity, saturation, volume, water density and
-//   temperature (C) and a spare slot,
+//   cs[4] the cell's porosity, saturation, volume and water density (cs[6]
+//   in the prefactor kernels: then also its temperature (C) and a spare),
 //   ionx[PFX_MAX_IONX_RXN] the exchange state, fx[G] the fixed accumulation
 //   of each row.  The last three hold cell state that lives across Newton
 //   passes: in LDS it does not occupy registers for the whole loop.
@@ -238,8 +238,8 @@ struct Scr {
   int C, gsec, lnK, la, mv, im, xb, mlnK, mkeff, marea, mvf, mrate, cs, ionx, fx;
 };
 
-__host__ __device__ inline int scr_doubles(int M, int G, int K) {
-  int n = (M + 1) + 2 * M + 5 * G + 5 * K + 6 + PFX_MAX_IONX_RXN;
+__host__ __device__ inline int scr_doubles(int M, int G, int K, bool xm) {
+  int n = (M + 1) + 2 * M + 5 * G + 5 * K + (xm ? 6 : 4) + PFX_MAX_IONX_RXN;
   return (n + 1) & ~1;
 }
 
@@ -247,6 +247,7 @@ __host__ __device__ inline int scr_doubles(int M, int G, int K) {
 // known at compile time: their LDS accesses take the offset as an
 // immediate of the slice base, and no per-field address stays live in a
 // register.  The complex and mineral arrays (sizes M, K) follow.
+template <bool XM>
 __device__ __forceinline__ Scr make_scr(int base, int M, int G, int K) {
   Scr s;
   s.la = base;
@@ -255,8 +256,9 @@ __device__ __forceinline__ Scr make_scr(int base, int M, int G, int K) {
   s.xb = base + 24 * G;
   s.fx = base + 32 * G;
   s.cs = base + 40 * G;
-  s.ionx = base + 40 * G + 48;
-  int p = base + 40 * G + 48 + 8 * PFX_MAX_IONX_RXN;
+  constexpr int cs_bytes = XM ? 48 : 32;
+  s.ionx = base + 40 * G + cs_bytes;
+  int p = base + 40 * G + cs_bytes + 8 * PFX_MAX_IONX_RXN;
   s.C = p; p += (M + 1) * 8;
   s.gsec = p; p += M * 8;
   s.lnK = p; p += M * 8;
@@ -316,9 +318,12 @@ __device__ __forceinline__ void pref_species_ln(KP Pp, int o, double la, double 
 // prod_s (monod factor) * k_p * Arrhenius_p (reaction_mineral.F90:743-786),
 // or the mineral's own constant (with its Arrhenius factor) without
 // prefactors.  jpert >= 0 takes the ln activity of that primary from
-// SD(S.xb, jpert) (a forward difference).  Group-uniform.
+// SD(S.xb, jpert) (a forward difference).  Group-uniform.  XM: the kernel
+// instantiation that carries prefactors and scale factors (without it the
+// rate constant is the mineral's own).
+template <bool XM>
 __device__ __forceinline__ double mineral_rate_constant(KP Pp, const Scr &S, int k, int jpert) {
-  if (!(Pp->L.mnrl_flags & 16)) return SD(S.mkeff, k);
+  if (!XM || !(Pp->L.mnrl_flags & 16)) return SD(S.mkeff, k);
   const int np = LI(Pp->L.o_mpn, k);
   if (np == 0) return SD(S.mkeff, k);
   const int PP = Pp->L.npref, PQ = Pp->L.npref_spec;
@@ -341,9 +346,10 @@ __device__ __forceinline__ double mineral_rate_constant(KP Pp, const Scr &S, int
 
 // Rate (mol/s/m^3 bulk) of kinetic mineral k at ln(Q/K), 0 where the
 // reference skips the mineral (reaction_mineral.F90:690-816).  Group-uniform.
+template <bool XM>
 __device__ __forceinline__ double mineral_rate(KP Pp, const Scr &S, int k, double lnQK, int jpert) {
   const double QK = lnQK <= 6.90776 ? exp(lnQK) : 1.0e3;
-  const bool temkin = Pp->L.mnrl_flags & 8, msf_on = Pp->L.mnrl_flags & 32;
+  const bool temkin = Pp->L.mnrl_flags & 8, msf_on = XM && (Pp->L.mnrl_flags & 32);
   const double msf = msf_on ? LD(Pp->L.o_mmsf, k) : 1.0;
   const double tk_ = temkin ? (msf_on ? msf * LD(Pp->L.o_mtemkin, k) : LD(Pp->L.o_mtemkin, k)) : msf;
   double aff = (temkin || msf_on) ? 1.0 - pow(QK, 1.0 / tk_) : 1.0 - QK;
@@ -355,7 +361,7 @@ __device__ __forceinline__ double mineral_rate(KP Pp, const Scr &S, int k, doubl
   if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
   double Im_const = -SD(S.marea, k);
   if (msf_on) Im_const = Im_const / msf;
-  const double rate = mineral_rate_constant(Pp, S, k, jpert);
+  const double rate = mineral_rate_constant<XM>(Pp, S, k, jpert);
   return (Pp->L.mnrl_flags & 4) ? Im_const * sgn * pow(fabs(aff), LD(Pp->L.o_mpow, k)) * rate
                              : Im_const * sgn * fabs(aff) * rate;
 }
@@ -410,7 +416,7 @@ __device__ __forceinline__ void prefactor_jacobian(KP Pp, const Scr &S, int k, d
 // (Res(m + e_j m_j 1e-5) - Res(m)) / (m_j 1e-5), Res = sum_k nu_ik Im_k V in
 // mineral order from zero; entries below 1e-40 in magnitude become 0.  The
 // perturbed ln activities use the unperturbed coefficients.  Adds to row[].
-template <int G>
+template <int G, bool XM>
 __device__ __forceinline__ void mineral_fd_jacobian(KP Pp, const Scr &S, int r, bool live, double m, double lng,
                                                     double vol, double *row) {
   const int K = Pp->L.K, nS = Pp->L.S;
@@ -427,7 +433,7 @@ __device__ __forceinline__ void mineral_fd_jacobian(KP Pp, const Scr &S, int r, 
       if (sp >= 0) lnQK += LD(Pp->L.o_mnu, k * nS + s) * SD(S.la, sp);
     }
     const double nur = live ? LD(Pp->L.o_mnuG, k * G + r) : 0.0;
-    ro += nur * (mineral_rate(Pp, S, k, lnQK, -1) * vol);
+    ro += nur * (mineral_rate<XM>(Pp, S, k, lnQK, -1) * vol);
   }
 #pragma unroll
   for (int j = 0; j < G; j++) {
@@ -440,7 +446,7 @@ __device__ __forceinline__ void mineral_fd_jacobian(KP Pp, const Scr &S, int r, 
           if (sp >= 0) lnQK += LD(Pp->L.o_mnu, k * nS + s) * (sp == j ? SD(S.xb, j) : SD(S.la, sp));
         }
         const double nur = live ? LD(Pp->L.o_mnuG, k * G + r) : 0.0;
-        rp += nur * (mineral_rate(Pp, S, k, lnQK, j) * vol);
+        rp += nur * (mineral_rate<XM>(Pp, S, k, lnQK, j) * vol);
       }
       double pert;
       {
@@ -461,7 +467,7 @@ __device__ __forceinline__ double ld_or(const double *a, long long i, const unsi
   return *(a ? a + i : (const double *)dummy);
 }
 
-template <int G, int MODE, int LUP>
+template <int G, int MODE, int LUP, bool XM>
 __device__ __forceinline__ void react_body(uint64_t praw) {
   const KP P0 = (KP)praw;
   const int N = P0->L.N, M = P0->L.M, K = P0->L.K, R = P0->L.R;
@@ -484,14 +490,14 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
   const int r = threadIdx.x & (G - 1);
   const int gbase = lane - r;
   const bool live = r < N;  // padding lanes carry an identity row
-  const int sdoubles = scr_doubles(M, G, K);
+  const int sdoubles = scr_doubles(M, G, K, XM);
   // The group's scratch offsets, and the lane's charge and ion size, are
   // rebuilt every pass from opaque copies of their base values: held across
   // the persistent loop they cost ~15 registers that the register-capped
   // variants spilled to scratch and reloaded on every pass.
   int sbase = lds_tab + (int)(threadIdx.x / G) * sdoubles * 8;
   int r_opq = r;
-  if (r == 0) SD(make_scr(sbase, M, G, K).C, M) = 0.0;
+  if (r == 0) SD(make_scr<XM>(sbase, M, G, K).C, M) = 0.0;
 
   // wave-level work queue state (identical in all lanes of the wave)
   long long q_next = 0, q_end = 0;
@@ -531,7 +537,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 #define P (*Pp)
 #define L (Pp->L)
     asm volatile("" : "+v"(sbase), "+v"(r_opq));
-    const Scr S = make_scr(sbase, L.M, G, L.K);
+    const Scr S = make_scr<XM>(sbase, L.M, G, L.K);
     const double Zr = live ? LD(L.o_Z, r_opq) : 0.0;
     const double a0r = live ? LD(L.o_a0, r_opq) : 0.0;
     double tot_out = 0.0, sorb_out = 0.0;  // totals and sorbed totals of this pass
@@ -657,7 +663,9 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
             SD(S.cs, 3) = den;
           }
           const double tempc = P.temperature ? v_tmp : 25.0;
-          if (r == 0) SD(S.cs, 4) = tempc;
+          if constexpr (XM) {
+            if (r == 0) SD(S.cs, 4) = tempc;
+          }
           const double tk = tempc + 273.15;
           const double ln_tk = log(tk);
           if (MODE == PFX_MODE_EQUIL) {
@@ -1120,7 +1128,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           if (sp >= 0) lnQK += LD(L.o_mnu, k * L.S + s) * SD(S.la, sp);
         }
         const double QK = lnQK <= 6.90776 ? exp(lnQK) : 1.0e3;
-        const bool temkin = L.mnrl_flags & 8, msf_on = L.mnrl_flags & 32;
+        const bool temkin = L.mnrl_flags & 8, msf_on = XM && (L.mnrl_flags & 32);
         const double msf = msf_on ? LD(L.o_mmsf, k) : 1.0;
         const double tk_ = temkin ? (msf_on ? msf * LD(L.o_mtemkin, k) : LD(L.o_mtemkin, k)) : msf;
         double aff = (temkin || msf_on) ? 1.0 - pow(QK, 1.0 / tk_) : 1.0 - QK;
@@ -1131,7 +1139,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           if (!(thr > 0.0 && sgn < 0.0 && QK < thr)) {
             const double lim = (L.mnrl_flags & 2) ? LD(L.o_mlim, k) : 0.0;
             if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
-            const double rate = mineral_rate_constant(Pp, S, k, -1);
+            const double rate = mineral_rate_constant<XM>(Pp, S, k, -1);
             double Im_const = -SD(S.marea, k);
             if (msf_on) Im_const = Im_const / msf;
             const bool pw = L.mnrl_flags & 4;
@@ -1152,8 +1160,10 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 #pragma unroll
               for (int kk = 0; kk < G; kk++) row[kk] += f * LD(L.o_mnuG, k * G + kk) * SD(S.im, kk);
             }
-            if (!fd_jac && (L.mnrl_flags & 16) && LI(L.o_mpn, k) > 0)
-              prefactor_jacobian<G>(Pp, S, k, ImV / rate, nur, row);
+            if constexpr (XM) {
+              if (!fd_jac && (L.mnrl_flags & 16) && LI(L.o_mpn, k) > 0)
+                prefactor_jacobian<G>(Pp, S, k, ImV / rate, nur, row);
+            }
           } else if (r == 0) {
             SD(S.mrate, k) = 0.0;
           }
@@ -1161,7 +1171,7 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
           SD(S.mrate, k) = 0.0;
         }
       }
-      if (MODE == PFX_MODE_EVAL && fd_jac && K > 0) mineral_fd_jacobian<G>(Pp, S, r, live, m, lng, vol, row);
+      if (MODE == PFX_MODE_EVAL && fd_jac && K > 0) mineral_fd_jacobian<G, XM>(Pp, S, r, live, m, lng, vol, row);
       rr = gmax<G>(live ? fabs(Rr) / fmax(scale, 1e-300) : 0.0, lane);
       if (MODE == PFX_MODE_EVAL) {
         finish = true;
@@ -1553,63 +1563,76 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
 // W: minimum wavefronts per SIMD the register allocation must allow (1 =
 // as many registers as the code wants; 2, 3 spill what does not fit).  The
 // fastest choice depends on the group size and mode and is picked by
-// measurement (pfx_tables_tune).
-template <int G, int MODE, int W, int LUP>
+// measurement (pfx_tables_tune).  XM: mineral rate prefactors and scale
+// factors compiled in (tables that use them launch these instantiations,
+// uncapped; the others keep the smaller register footprint).
+template <int G, int MODE, int W, int LUP, bool XM>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W))) void pfx_react_kernel(
     const pfx_kernel_params Pk) {
   (void)Pk;
-  react_body<G, MODE, LUP>((uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr());
+  react_body<G, MODE, LUP, XM>((uint64_t)(uintptr_t)__builtin_amdgcn_kernarg_segment_ptr());
 }
 
 // Several chemistry zones in one persistent launch: workgroups
 // [block_start[z], block_start[z+1]) serve zone z, whose parameters (its own
 // tables, cells and queue) are zone_params[z]; each workgroup stamps the
 // wall clock when it retires, so the host can rebalance the split.
+template <bool XM>
 __device__ __forceinline__ void zones_body(const pfx_kernel_params *zone_params, const pfx_zone_launch &Z,
                                            uint64_t *finish) {
   int z = 0;
   while (z + 1 < Z.nzones && (int)blockIdx.x >= Z.block_start[z + 1]) z++;
   const uint64_t praw = (uint64_t)(uintptr_t)(zone_params + z);
   switch (Z.G[z]) {
-    case 8: react_body<8, PFX_MODE_REACT, PFX_LU_DEFER>(praw); break;
-    case 16: react_body<16, PFX_MODE_REACT, PFX_LU_DEFER>(praw); break;
-    default: react_body<32, PFX_MODE_REACT, PFX_LU_DEFER>(praw); break;
+    case 8: react_body<8, PFX_MODE_REACT, PFX_LU_DEFER, XM>(praw); break;
+    case 16: react_body<16, PFX_MODE_REACT, PFX_LU_DEFER, XM>(praw); break;
+    default: react_body<32, PFX_MODE_REACT, PFX_LU_DEFER, XM>(praw); break;
   }
   __syncthreads();
   if (threadIdx.x == 0 && finish) finish[blockIdx.x] = wall_clock64();
 }
 
 // W as for pfx_react_kernel: 1 leaves the registers to the 32-lane path (one
-// wavefront per SIMD when such a zone is present), 2 and 3 cap them
-template <int W>
+// wavefront per SIMD when such a zone is present), 2 and 3 cap them; XM as
+// for pfx_react_kernel (uncapped only)
+template <int W, bool XM>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W))) void pfx_react_zones_kernel(
     const pfx_kernel_params *zone_params, const pfx_zone_launch Z, uint64_t *finish) {
-  zones_body(zone_params, Z, finish);
+  zones_body<XM>(zone_params, Z, finish);
 }
-template __global__ void pfx_react_zones_kernel<1>(const pfx_kernel_params *, const pfx_zone_launch, uint64_t *);
-template __global__ void pfx_react_zones_kernel<2>(const pfx_kernel_params *, const pfx_zone_launch, uint64_t *);
-template __global__ void pfx_react_zones_kernel<3>(const pfx_kernel_params *, const pfx_zone_launch, uint64_t *);
+template __global__ void pfx_react_zones_kernel<1, false>(const pfx_kernel_params *, const pfx_zone_launch,
+                                                          uint64_t *);
+template __global__ void pfx_react_zones_kernel<2, false>(const pfx_kernel_params *, const pfx_zone_launch,
+                                                          uint64_t *);
+template __global__ void pfx_react_zones_kernel<3, false>(const pfx_kernel_params *, const pfx_zone_launch,
+                                                          uint64_t *);
+template __global__ void pfx_react_zones_kernel<1, true>(const pfx_kernel_params *, const pfx_zone_launch,
+                                                         uint64_t *);
 
-#define PFX_INST_LU(G, MODE, W, LUP) \
-  template __global__ void pfx_react_kernel<G, MODE, W, LUP>(const pfx_kernel_params);
-#define PFX_INST(G)                                     \
-  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_DEFER)       \
-  PFX_INST_LU(G, PFX_MODE_REACT, 2, PFX_LU_DEFER)       \
-  PFX_INST_LU(G, PFX_MODE_REACT, 3, PFX_LU_DEFER)       \
-  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_ONDEMAND)    \
-  PFX_INST_LU(G, PFX_MODE_REACT, 2, PFX_LU_ONDEMAND)    \
-  PFX_INST_LU(G, PFX_MODE_REACT, 3, PFX_LU_ONDEMAND)    \
-  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_PIVOT)       \
-  PFX_INST_LU(G, PFX_MODE_EVAL, 1, PFX_LU_PIVOT)        \
-  PFX_INST_LU(G, PFX_MODE_EQUIL, 1, PFX_LU_PIVOT)       \
-  PFX_INST_LU(G, PFX_MODE_EQUIL, 2, PFX_LU_PIVOT)       \
-  PFX_INST_LU(G, PFX_MODE_EQUIL, 3, PFX_LU_PIVOT)       \
-  PFX_INST_LU(G, PFX_MODE_AUX, 1, PFX_LU_PIVOT)
+#define PFX_INST_LU(G, MODE, W, LUP, XM) \
+  template __global__ void pfx_react_kernel<G, MODE, W, LUP, XM>(const pfx_kernel_params);
+#define PFX_INST(G)                                        \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_DEFER, false)    \
+  PFX_INST_LU(G, PFX_MODE_REACT, 2, PFX_LU_DEFER, false)    \
+  PFX_INST_LU(G, PFX_MODE_REACT, 3, PFX_LU_DEFER, false)    \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_ONDEMAND, false) \
+  PFX_INST_LU(G, PFX_MODE_REACT, 2, PFX_LU_ONDEMAND, false) \
+  PFX_INST_LU(G, PFX_MODE_REACT, 3, PFX_LU_ONDEMAND, false) \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_PIVOT, false)    \
+  PFX_INST_LU(G, PFX_MODE_EVAL, 1, PFX_LU_PIVOT, false)     \
+  PFX_INST_LU(G, PFX_MODE_EQUIL, 1, PFX_LU_PIVOT, false)    \
+  PFX_INST_LU(G, PFX_MODE_EQUIL, 2, PFX_LU_PIVOT, false)    \
+  PFX_INST_LU(G, PFX_MODE_EQUIL, 3, PFX_LU_PIVOT, false)    \
+  PFX_INST_LU(G, PFX_MODE_AUX, 1, PFX_LU_PIVOT, false)      \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_DEFER, true)     \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_ONDEMAND, true)  \
+  PFX_INST_LU(G, PFX_MODE_REACT, 1, PFX_LU_PIVOT, true)     \
+  PFX_INST_LU(G, PFX_MODE_EVAL, 1, PFX_LU_PIVOT, true)
 PFX_INST(8)
 PFX_INST(16)
 PFX_INST(32)
 
-extern "C" int pfx_scr_doubles(int M, int G, int K) { return scr_doubles(M, G, K); }
+extern "C" int pfx_scr_doubles(int M, int G, int K, int xm) { return scr_doubles(M, G, K, xm != 0); }
 
 #ifdef PFX_SECTION_TIMERS
 // Profiling build only (no counterpart in the reference): read and clear the
