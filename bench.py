@@ -286,7 +286,9 @@ def main():
     try:
         with open(os.path.join(ROOT, "profiles", "pmc_traffic_pfx_react_kernel_1M.json")) as f:
             prof = json.load(f)["pfx_react_kernel<16,0>"]
-        if n == 1_000_000 and warm is not None:
+        # the counters were read on the uncapped variant: they stand for
+        # the timed kernel only when the autotuner picked that one too
+        if n == 1_000_000 and warm is not None and tab.tune(abi.PFX_KERNEL_REACT) == 0:
             traffic = prof["traffic_bytes"]
     except (OSError, KeyError, ValueError):
         traffic = None
@@ -306,7 +308,8 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": "profiles/pmc_traffic_pfx_react_kernel_1M.json (FETCH_SIZE x2.0, "
                                        "WRITE_SIZE x1.0 from a 256 MiB copy)" if traffic else None,
-                     "kernel": f"pfx_react_kernel<16, 0, {tab.tune(abi.PFX_KERNEL_REACT) + 1}>",
+                     "kernel": f"pfx_react_kernel<16, 0, {tab.tune(abi.PFX_KERNEL_REACT) + 1}, 0, false> "
+                               "(unpivoted) + pfx_react_kernel<16, 0, 1, 1, false> (pivoting follow-up)",
                      "kernel_ms": kern_ms,
                      "flops_per_iteration": fl["flops"], "transcendentals_per_iteration": fl["transcendentals"],
                      "note": "FP64 VALU-bound Newton; MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s",
