@@ -22,6 +22,7 @@
 // (the k order inside a fragment only has to agree between A and B).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -229,16 +230,51 @@ __global__ __launch_bounds__(256) void k_colsum(const float *dZ, int B, int N, f
     atomicAdd(&db[o], red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
-__global__ void k_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1, float b2, float c1,
-                       float c2, float eps) {
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+// Several tensors in one launch (training is launch-bound at these sizes):
+// blockIdx.y picks the tensor, blockIdx.x strides over its elements.
+constexpr int MT_MAX = 2 * MLP_MAX_LAYERS + 1;
+struct mt_tensors {
+  float *p[MT_MAX];
+  long n[MT_MAX];
+  int count;
+};
+struct mt_adam {
+  float *w[MT_MAX], *g[MT_MAX], *m[MT_MAX], *v[MT_MAX];
+  long n[MT_MAX];
+  int count;
+};
+
+__global__ void k_fill_multi(const mt_tensors T, float v) {
+  const int t = blockIdx.y;
+  if (t >= T.count) return;
+  float *p = T.p[t];
+  const long n = T.n[t];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Adam with bias correction over every weight and bias tensor
+__global__ void k_adam_multi(const mt_adam T, float lr, float b1, float b2, float c1, float c2, float eps) {
+  const int t = blockIdx.y;
+  if (t >= T.count) return;
+  float *w = T.w[t], *m = T.m[t], *v = T.v[t];
+  const float *g = T.g[t];
+  const long n = T.n[t];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// grid x for a multi-tensor launch: the largest tensor in 256-element
+// blocks, at most 1024 blocks (the rest by striding)
+inline unsigned mt_blocks(const long *n, int count) {
+  long mx = 1;
+  for (int t = 0; t < count; t++) mx = n[t] > mx ? n[t] : mx;
+  return (unsigned)std::min<long>(1024, (mx + 255) / 256);
 }
 
 // W[N][K] FP32 -> BF16 fragments: uint4 index ((nt * Kp/32 + kt) * 64 + lane)
@@ -636,7 +672,19 @@ extern "C" int pfx_mlp_gradients(pfx_mlp *m, const float *X, const float *T, int
   int rc = forward(m, X, B, m->A[L], true, s);
   if (rc) return rc;
   const int no = m->dims[L];
-  if ((rc = fill(m->loss, 1, 0.f, s))) return rc;
+  // every gradient buffer and the loss cleared in one launch (the weight
+  // gradients and the loss accumulate with atomics)
+  mt_tensors Zt{};
+  for (int l = 0; l < L; l++) {
+    Zt.p[Zt.count] = m->gW[l];
+    Zt.n[Zt.count++] = (long)m->dims[l + 1] * m->dims[l];
+    Zt.p[Zt.count] = m->gb[l];
+    Zt.n[Zt.count++] = m->dims[l + 1];
+  }
+  Zt.p[Zt.count] = m->loss;
+  Zt.n[Zt.count++] = 1;
+  hipLaunchKernelGGL(k_fill_multi, dim3(mt_blocks(Zt.n, Zt.count), (unsigned)Zt.count), dim3(256), 0, s, Zt, 0.f);
+  MCHK(hipGetLastError());
   const long nout = (long)B * no;
   hipLaunchKernelGGL(k_mse_grad, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s, m->A[L], T, m->dZ, m->loss,
                      nout, 1.0f / (float)nout);
@@ -644,12 +692,10 @@ extern "C" int pfx_mlp_gradients(pfx_mlp *m, const float *X, const float *T, int
   for (int l = L - 1; l >= 0; l--) {
     const int K = m->dims[l], N = m->dims[l + 1];
     // dW[n][k] = sum_b dZ[b][n] A[b][k], split over the batch
-    if ((rc = fill(m->gW[l], (long)N * K, 0.f, s))) return rc;
     const int ksplit = B >= 4096 ? 16 : (B >= 512 ? 4 : 1);
     if ((rc = launch_gemm(EPI_ATOMIC, N, K, B, m->dZ, 1, N, m->A[l], 1, K, m->gW[l], K, nullptr, nullptr, ksplit,
                           s)))
       return rc;
-    if ((rc = fill(m->gb[l], N, 0.f, s))) return rc;
     hipLaunchKernelGGL(k_colsum, dim3((unsigned)((N + 63) / 64), (unsigned)((B + COLSUM_ROWS - 1) / COLSUM_ROWS)),
                        dim3(256), 0, s, m->dZ, B, N, m->gb[l]);
     MCHK(hipGetLastError());
@@ -675,13 +721,15 @@ extern "C" int pfx_mlp_adam(pfx_mlp *m, float lr, float beta1, float beta2, floa
   hipStream_t s = (hipStream_t)stream;
   m->adam_step++;
   const float c1 = 1.0f - powf(beta1, (float)m->adam_step), c2 = 1.0f - powf(beta2, (float)m->adam_step);
+  mt_adam T{};
   for (int l = 0; l < m->nlayers; l++) {
-    const long nw = (long)m->dims[l + 1] * m->dims[l], nb = m->dims[l + 1];
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, m->W[l], m->gW[l], m->mW[l],
-                       m->vW[l], nw, lr, beta1, beta2, c1, c2, eps);
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, m->b[l], m->gb[l], m->mb[l],
-                       m->vb[l], nb, lr, beta1, beta2, c1, c2, eps);
+    T.w[T.count] = m->W[l], T.g[T.count] = m->gW[l], T.m[T.count] = m->mW[l], T.v[T.count] = m->vW[l];
+    T.n[T.count++] = (long)m->dims[l + 1] * m->dims[l];
+    T.w[T.count] = m->b[l], T.g[T.count] = m->gb[l], T.m[T.count] = m->mb[l], T.v[T.count] = m->vb[l];
+    T.n[T.count++] = m->dims[l + 1];
   }
+  hipLaunchKernelGGL(k_adam_multi, dim3(mt_blocks(T.n, T.count), (unsigned)T.count), dim3(256), 0, s, T, lr, beta1,
+                     beta2, c1, c2, eps);
   MCHK(hipGetLastError());
   m->sw_dirty = true;
   return 0;
