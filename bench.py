@@ -74,7 +74,8 @@ def main():
     ap.add_argument("--epochs", type=float, default=10.0)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--fallback-after", type=int, default=50)
+    ap.add_argument("--fallback-after", type=int, default=100,
+                    help="warm attempt budget before the restart from the previous free ions")
     ap.add_argument("--no-surrogate", action="store_true", help="plain method only")
     ap.add_argument("--no-gi", action="store_true", help="skip the global-implicit assembly leg")
     ap.add_argument("--no-zones", action="store_true", help="skip the chemistry-zones leg")
