@@ -287,6 +287,7 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   // to balance the tail
   int64_t per_wave = std::max<int64_t>(1, P.ncells / (blocks * (BLOCK / 64)));
   P.chunk = (int32_t)std::max<int64_t>(64 / G, std::min<int64_t>(64, per_wave / 8 + 1));
+  P.waves = (int32_t)(blocks * (BLOCK / 64));
   P.queue = queue;
   P.fallback_count = queue + 1;
   // the pivoting launch: its own cursor, one cell per fetch, one workgroup
@@ -294,6 +295,7 @@ int launch(pfx_tables *t, pfx_kernel_params &P, int mode, hipStream_t stream, do
   pfx_kernel_params P2 = P;
   P2.queue = queue + 2;
   P2.chunk = 1;
+  P2.waves = 0;
   const int64_t blocks2 = std::max<int64_t>(1, std::min<int64_t>(need_blocks, (int64_t)t->num_cu));
   hipLaunchKernelGGL(k_zero_launch, dim3(1), dim3(64), 0, stream, queue, P.defer);
   HIPCHK(hipGetLastError());
@@ -519,6 +521,7 @@ int zones_prepare(int32_t nzones, pfx_tables *const *tables, pfx_cells *cells, c
     // chunk: as in the single-batch launch, from the zone's own share
     const int64_t per_wave = std::max<int64_t>(1, zs.n[z] / ((int64_t)blocks[z] * (BLOCK / 64)));
     zs.P[z].chunk = (int32_t)std::max<int64_t>(64 / zs.Z.G[z], std::min<int64_t>(64, per_wave / 8 + 1));
+    zs.P[z].waves = blocks[z] * (BLOCK / 64);
   }
   zs.Z.block_start[nzones] = acc;
   zs.total_blocks = total;
@@ -545,6 +548,7 @@ int zones_enqueue(pfx_tables *const *tables, zone_setup &zs, const pfx_kernel_pa
     pfx_kernel_params P2 = zs.P[z];
     P2.queue = t->d_queue + 2;
     P2.chunk = 1;
+    P2.waves = 0;
     const int64_t lds = zone_lds(t);
     const void *fn_piv = kernel_ptr(t->L.G, PFX_MODE_REACT, 0, PFX_LU_PIVOT, tables_xm(t));
     if (lds > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn_piv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -114,7 +114,10 @@ struct pfx_kernel_params {
   int32_t aux_passes;              // activity passes of the aux-var refresh
   // work queue
   unsigned int *queue;          // global cell cursor
-  int32_t chunk;                // cells taken per fetch
+  int32_t chunk;                // cells taken per fetch (at most)
+  int32_t waves;                // wavefronts draining the queue: a fetch takes
+                                // a quarter of their fair share of what is
+                                // left, at most chunk (0 = always chunk)
   // cells whose unpivoted factorisation broke down: defer[0] their count,
   // defer[1..] their indices (written by the unpivoted reaction kernel,
   // worked through by the pivoting one); null where the kernel pivots

@@ -554,11 +554,23 @@ __device__ __forceinline__ void react_body(uint64_t praw) {
       while (nneed > 0) {
         long long avail = q_end - q_next;
         if (avail <= 0) {
-          unsigned int base = 0;
-          if (lane == 0) base = atomicAdd(P.queue, (unsigned int)P.chunk);
+          // guided sizes: near the end of the queue a fetch takes less, so
+          // the last wavefronts do not hold long private runs of cells
+          unsigned int base = 0, take_n = (unsigned int)P.chunk;
+          if (lane == 0) {
+            if (P.waves > 0) {
+              const long long left =
+                  ncl - (long long)__hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              const long long share = left / (4LL * P.waves);
+              const long long lo = 64 / G;
+              take_n = (unsigned int)(share < lo ? lo : (share < (long long)take_n ? share : (long long)take_n));
+            }
+            base = atomicAdd(P.queue, take_n);
+          }
           base = __shfl(base, 0);
+          take_n = __shfl(take_n, 0);
           q_next = base;
-          q_end = (long long)base + P.chunk;
+          q_end = (long long)base + take_n;
           if (q_end > ncl) q_end = ncl;
           if (q_next >= ncl) {
             exhausted = true;
